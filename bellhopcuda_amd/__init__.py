@@ -1,0 +1,251 @@
+"""bellhopcuda_amd -- MI355X-native BELLHOP ray/beam tracing hot path.
+
+Python host binding of libbhc_hip.so (include/bhc_hip.h). The flow mirrors the
+reference's library API for one field run (bhc::setup -> bhc::run ->
+bhc::writeout, include/bhc/bhc.hpp:58-374): an `Env2D` (what the .env file
+holds) is preprocessed on the host (C++), handed to the HIP launch layer that
+replaces RunFieldModesImpl (src/mode/fieldimpl.hpp:27-28), and the results
+come back as numpy arrays in the reference's layouts.
+
+There is no CPU execution path: if the HIP library is missing or no GPU is
+visible, the run calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from .env import Env2D, munk_env, c2_env, sweep_envs, subtab  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbhc_hip.so")
+_LIB = None
+
+
+class BhcError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libbhc_hip.so (built by bellhopcuda_amd.build / __graft_entry__.build)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise BhcError("libbhc_hip.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "bhc_preprocess2d": (C.c_int, [P(_abi.Env2D_C), P(C.c_void_p)]),
+        "bhc_problem_view": (P(_abi.Problem2D), [C.c_void_p]),
+        "bhc_problem_free": (None, [C.c_void_p]),
+        "bhc_host_last_error": (C.c_char_p, []),
+        "bhc_last_error": (C.c_char_p, []),
+        "bhc_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_int, P(C.c_int), P(C.c_int)]),
+        "bhc_ctx_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
+        "bhc_ctx_destroy": (None, [C.c_void_p]),
+        "bhc_ctx_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "bhc_ctx_synchronize": (C.c_int, [C.c_void_p]),
+        "bhc_last_kernel_ms": (C.c_int, [C.c_void_p, P(C.c_double)]),
+        "bhc_field_run2d": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.FieldOut)]),
+        "bhc_ray_run2d": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.RayOut)]),
+        "bhc_ray_run2d_list": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.RayOut), _abi.c_int32_p,
+                                         _abi.c_int32_p, _abi.c_int32_p, C.c_int32]),
+        "bhc_problem_upload2d": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.FieldOut), P(C.c_void_p)]),
+        "bhc_field_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "bhc_field_download": (C.c_int, [C.c_void_p, C.c_void_p, P(_abi.FieldOut)]),
+        "bhc_problem_release": (None, [C.c_void_p]),
+        "bhc_scale_pressure2d": (C.c_int, [P(_abi.Problem2D), _abi.c_float_p]),
+        "bhc_write_shd2d": (C.c_int, [C.c_char_p, P(_abi.Problem2D), C.c_char_p, _abi.c_float_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _check(rc, host=False):
+    if rc != 0:
+        msg = (lib().bhc_host_last_error() if host else lib().bhc_last_error()) or b"?"
+        raise BhcError(msg.decode())
+
+
+class Problem:
+    """A preprocessed 2D problem (bhc_problem2d view owned by the C++ side)."""
+
+    def __init__(self, env):
+        self.env = env
+        ec, self._keep = env.to_c()
+        h = C.c_void_p()
+        _check(lib().bhc_preprocess2d(C.byref(ec), C.byref(h)), host=True)
+        self._h = h
+        self.view = lib().bhc_problem_view(h).contents
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.bhc_problem_free(self._h)
+            self._h = None
+
+    @property
+    def nrays(self):
+        v = self.view
+        return (1 if v.iSingle_alpha >= 1 else v.Nalpha) * v.NSz
+
+    @property
+    def field_shape(self):
+        v = self.view
+        return (v.NSz, v.NRz_per_range, v.NRr)
+
+    def scale_pressure(self, field):
+        f = np.ascontiguousarray(field, np.complex64).copy()
+        _check(lib().bhc_scale_pressure2d(C.byref(self.view), f.ctypes.data_as(_abi.c_float_p)), host=True)
+        return f
+
+    def write_shd(self, path, field, title=None):
+        f = np.ascontiguousarray(field, np.complex64)
+        t = (title or self.env.title).encode()
+        rc = lib().bhc_write_shd2d(path.encode(), C.byref(self.view), t, f.ctypes.data_as(_abi.c_float_p))
+        if rc != 0:
+            raise BhcError("writing %s failed" % path)
+
+
+class Context:
+    """One HIP device context (stream + events)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().bhc_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().bhc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.bhc_ctx_destroy(self._h)
+
+    def set_launch(self, blocks_per_cu=0):
+        _check(lib().bhc_ctx_set_launch(self._h, 0, blocks_per_cu))
+
+    def last_kernel_ms(self):
+        v = C.c_double()
+        _check(lib().bhc_last_kernel_ms(self._h, C.byref(v)))
+        return v.value
+
+
+class FieldResult:
+    pass
+
+
+def _make_out(prob, res, stats, max_narr, allow_merging, eigen_memsize):
+    nsz, nrz, nrr = prob.field_shape
+    out = _abi.FieldOut()
+    rt = prob.env.run_type[0]
+    if rt in "CSI":
+        res.field = np.zeros((nsz, nrz, nrr), np.complex64)
+        out.field = res.field.ctypes.data_as(_abi.c_float_p)
+    if rt in "AaVv":
+        res.max_narr = max_narr
+        res.arr = (_abi.Arrival * (nsz * nrz * nrr * max_narr))()
+        res.narr = np.zeros(nsz * nrz * nrr, np.int32)
+        out.arr = C.cast(res.arr, C.POINTER(_abi.Arrival))
+        out.narr = res.narr.ctypes.data_as(_abi.c_int32_p)
+        out.max_narr = max_narr
+        out.allow_merging = allow_merging
+    if rt in "EVv":
+        res.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
+        res.neigen = np.zeros(1, np.int32)
+        out.hits = C.cast(res.hits, C.POINTER(_abi.EigenHit))
+        out.neigen = res.neigen.ctypes.data_as(_abi.c_int32_p)
+        out.eigen_memsize = eigen_memsize
+    if stats:
+        n = prob.nrays
+        res.nsteps = np.zeros(n, np.int32)
+        res.ntop = np.zeros(n, np.int32)
+        res.nbot = np.zeros(n, np.int32)
+        res.xend = np.zeros((n, 2), np.float64)
+        out.ray_nsteps = res.nsteps.ctypes.data_as(_abi.c_int32_p)
+        out.ray_ntop = res.ntop.ctypes.data_as(_abi.c_int32_p)
+        out.ray_nbot = res.nbot.ctypes.data_as(_abi.c_int32_p)
+        out.ray_xend = res.xend.ctypes.data_as(_abi.c_double_p)
+    return out
+
+
+def field_run(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1, eigen_memsize=0):
+    """One field run (TL / arrivals / eigen hits) on the GPU."""
+    prob = env_or_prob if isinstance(env_or_prob, Problem) else Problem(env_or_prob)
+    own = ctx is None
+    ctx = ctx or Context(0)
+    res = FieldResult()
+    out = _make_out(prob, res, stats, max_narr, allow_merging, eigen_memsize)
+    try:
+        _check(lib().bhc_field_run2d(ctx._h, C.byref(prob.view), C.byref(out)))
+        res.kernel_ms = ctx.last_kernel_ms()
+    finally:
+        if own:
+            ctx.close()
+    res.err, res.warn = out.err, out.warn
+    res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
+    res.problem = prob
+    return res
+
+
+class Resident:
+    """A problem whose inputs and outputs stay in HBM across launches."""
+
+    def __init__(self, prob, ctx, stats=False, max_narr=0, allow_merging=1, eigen_memsize=0):
+        self.prob, self.ctx = prob, ctx
+        self.res = FieldResult()
+        self.out = _make_out(prob, self.res, stats, max_narr, allow_merging, eigen_memsize)
+        h = C.c_void_p()
+        _check(lib().bhc_problem_upload2d(ctx._h, C.byref(prob.view), C.byref(self.out), C.byref(h)))
+        self._h = h
+
+    def launch(self):
+        _check(lib().bhc_field_launch(self.ctx._h, self._h))
+
+    def download(self):
+        _check(lib().bhc_field_download(self.ctx._h, self._h, C.byref(self.out)))
+        r = self.res
+        r.err, r.warn = self.out.err, self.out.warn
+        r.total_steps, r.total_contrib = self.out.total_steps, self.out.total_contrib
+        return r
+
+    def release(self):
+        if self._h:
+            lib().bhc_problem_release(self._h)
+            self._h = None
+
+
+def ray_run(env_or_prob, ctx=None, max_points=20000, ialpha=None, isz=None, nsteps_in=None):
+    """Ray run type (or eigenray re-trace with per-ray step limits)."""
+    prob = env_or_prob if isinstance(env_or_prob, Problem) else Problem(env_or_prob)
+    own = ctx is None
+    ctx = ctx or Context(0)
+    n = prob.nrays if ialpha is None else len(ialpha)
+    pts = (_abi.RayPt2D * (n * max_points))()
+    nst = np.zeros(n, np.int32)
+    out = _abi.RayOut()
+    out.points = C.cast(pts, C.POINTER(_abi.RayPt2D))
+    out.nsteps = nst.ctypes.data_as(_abi.c_int32_p)
+    out.max_points_per_ray = max_points
+    try:
+        if ialpha is None:
+            _check(lib().bhc_ray_run2d(ctx._h, C.byref(prob.view), C.byref(out)))
+        else:
+            ia = np.ascontiguousarray(ialpha, np.int32)
+            iz = np.ascontiguousarray(isz if isz is not None else np.zeros(n), np.int32)
+            ns = None if nsteps_in is None else np.ascontiguousarray(nsteps_in, np.int32)
+            _check(lib().bhc_ray_run2d_list(ctx._h, C.byref(prob.view), C.byref(out),
+                                            ia.ctypes.data_as(_abi.c_int32_p), iz.ctypes.data_as(_abi.c_int32_p),
+                                            ns.ctypes.data_as(_abi.c_int32_p) if ns is not None else None, n))
+    finally:
+        if own:
+            ctx.close()
+    arr = np.ctypeslib.as_array(pts).reshape(n, max_points)
+    return arr, nst, (out.err, out.warn)

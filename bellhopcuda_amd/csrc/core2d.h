@@ -1,0 +1,1156 @@
+// core2d.h -- per-ray physics of the 2D tracer, written once for the HIP
+// kernels (device) and for the CPU self-check harness (host).
+//
+// The functions follow the reference's 2D hot path operation-for-operation
+// (order of floating-point operations, float-vs-real literal types, tie
+// breaks), because trajectories are compared bit-for-bit against the oracle.
+// The build uses -ffp-contract=off so that no a*b+c is fused.
+//
+// Data layout: a problem is one contiguous "blob" of reals (SSP knots and
+// coefficients, boundary nodes, receiver ranges, reflection tables) addressed
+// through the offsets in Prob2D. The kernel stages the SSP and boundary part
+// of the blob into LDS and points the same accessors at the LDS copy.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+namespace bhc2d {
+
+// ---------------------------------------------------------------------------
+// Literals and constants (reference src/common.hpp:144-179)
+// ---------------------------------------------------------------------------
+template<typename T> struct RC;
+template<> struct RC<double> {
+    static constexpr double EPS = DBL_EPSILON, MAXV = DBL_MAX, MINPOS = DBL_MIN;
+    static constexpr double PI = M_PI, INF_STEP = 1e-6;
+};
+template<> struct RC<float> {
+    static constexpr float EPS = FLT_EPSILON, MAXV = FLT_MAX, MINPOS = FLT_MIN;
+    static constexpr float PI = (float)M_PI, INF_STEP = 1e-3f;
+};
+template<typename T> __host__ __device__ constexpr T rlit(double d, float f);
+template<> __host__ __device__ constexpr double rlit<double>(double d, float) { return d; }
+template<> __host__ __device__ constexpr float rlit<float>(double, float f) { return f; }
+#define RL(a) (::bhc2d::rlit<real>(a, a##f))
+#define FL(a) (a##f)
+
+template<typename real> __host__ __device__ inline real rmin(real a, real b) { return (b < a) ? b : a; }
+template<typename real> __host__ __device__ inline real rmax(real a, real b) { return (a < b) ? b : a; }
+__host__ __device__ inline int imin(int a, int b) { return (b < a) ? b : a; }
+__host__ __device__ inline int imax(int a, int b) { return (a < b) ? b : a; }
+
+// (int)x with x86-64 cvttsd2si semantics: out-of-range and NaN give INT_MIN.
+template<typename real> __host__ __device__ inline int trunc_int_x86(real x)
+{
+    if(!(x > (real)-2147483649.0 && x < (real)2147483648.0)) return (int)0x80000000;
+    return (int)x;
+}
+
+template<typename real> __host__ __device__ inline real spacing(real v)
+{
+    return std::fabs(std::nextafter(v, (real)0) - v);
+}
+
+// ---------------------------------------------------------------------------
+// Complex arithmetic as gcc lowers std::complex / C99 complex (no contraction)
+// ---------------------------------------------------------------------------
+template<typename real> struct cpx {
+    real re, im;
+};
+template<typename real> __host__ __device__ inline cpx<real> C(real a, real b) { return cpx<real>{a, b}; }
+template<typename real> __host__ __device__ inline cpx<real> operator+(cpx<real> a, cpx<real> b)
+{
+    return C(a.re + b.re, a.im + b.im);
+}
+template<typename real> __host__ __device__ inline cpx<real> operator-(cpx<real> a, cpx<real> b)
+{
+    return C(a.re - b.re, a.im - b.im);
+}
+template<typename real> __host__ __device__ inline cpx<real> operator-(cpx<real> a) { return C(-a.re, -a.im); }
+template<typename real> __host__ __device__ inline cpx<real> cmul(cpx<real> a, cpx<real> b)
+{
+    return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template<typename real> __host__ __device__ inline cpx<real> rmul(real r, cpx<real> b) { return C(r * b.re, r * b.im); }
+template<typename real> __host__ __device__ inline cpx<real> rdivc(cpx<real> a, real r) { return C(a.re / r, a.im / r); }
+// libgcc __divdc3 (Smith's method), finite inputs.
+template<typename real> __host__ __device__ inline cpx<real> cdiv(cpx<real> n, cpx<real> dd)
+{
+    real a = n.re, b = n.im, c = dd.re, d = dd.im, ratio, denom, x, y;
+    if(std::fabs(c) < std::fabs(d)) {
+        ratio = c / d;
+        denom = (c * ratio) + d;
+        x = ((a * ratio) + b) / denom;
+        y = ((b * ratio) - a) / denom;
+    } else {
+        ratio = d / c;
+        denom = (d * ratio) + c;
+        x = ((b * ratio) + a) / denom;
+        y = (b - (a * ratio)) / denom;
+    }
+    return C(x, y);
+}
+template<typename real> __host__ __device__ inline cpx<real> realdiv(real r, cpx<real> dd)
+{
+    return cdiv(C(r, (real)0), dd);
+}
+template<typename real> __host__ __device__ inline real cabs_(cpx<real> a) { return std::hypot(a.re, a.im); }
+// glibc csqrt, finite-input branches.
+template<typename real> __host__ __device__ inline cpx<real> csqrt_(cpx<real> x)
+{
+    real re = x.re, im = x.im;
+    if(im == (real)0) {
+        if(re < (real)0) return C((real)0, std::copysign(std::sqrt(-re), im));
+        return C(std::fabs(std::sqrt(re)), std::copysign((real)0, im));
+    }
+    if(re == (real)0) {
+        real r;
+        if(std::fabs(im) >= (real)2 * RC<real>::MINPOS) r = std::sqrt((real)M_SQRT1_2 * std::fabs(im));
+        else r = (real)0.5 * std::sqrt((real)2 * std::fabs(im));
+        return C(r, std::copysign(r, im));
+    }
+    real d = std::hypot(re, im), r, s;
+    if(re > (real)0) {
+        r = std::sqrt((real)0.5 * (d + re));
+        s = (real)0.5 * (im / r);
+    } else {
+        s = std::sqrt((real)0.5 * (d - re));
+        r = std::fabs((real)0.5 * (im / s));
+    }
+    return C(r, std::copysign(s, im));
+}
+// glibc cexp, finite-input branch.
+template<typename real> __host__ __device__ inline cpx<real> cexp_(cpx<real> x)
+{
+    real s, c;
+    if(std::fabs(x.im) > RC<real>::MINPOS) {
+        s = std::sin(x.im);
+        c = std::cos(x.im);
+    } else {
+        s = x.im;
+        c = (real)1;
+    }
+    real e = std::exp(x.re);
+    return C(e * c, e * s);
+}
+
+template<typename real> struct v2 {
+    real x, y;
+};
+template<typename real> __host__ __device__ inline v2<real> V(real x, real y) { return v2<real>{x, y}; }
+template<typename real> __host__ __device__ inline v2<real> operator+(v2<real> a, v2<real> b) { return V(a.x + b.x, a.y + b.y); }
+template<typename real> __host__ __device__ inline v2<real> operator-(v2<real> a, v2<real> b) { return V(a.x - b.x, a.y - b.y); }
+template<typename real> __host__ __device__ inline v2<real> sc(real s, v2<real> a) { return V(s * a.x, s * a.y); }
+template<typename real> __host__ __device__ inline v2<real> dv(v2<real> a, real s) { return V(a.x / s, a.y / s); }
+template<typename real> __host__ __device__ inline real dot(v2<real> a, v2<real> b) { return a.x * b.x + a.y * b.y; }
+
+// ---------------------------------------------------------------------------
+// Problem descriptor (device view). All offsets index `base` (reals).
+// ---------------------------------------------------------------------------
+enum : int { RT_TL = 0, RT_ARR = 1, RT_EIG = 2, RT_RAY = 3 };
+
+// Boundary node record in the blob (BdryPtFull<false> subset), NODE reals each.
+enum : int {
+    BN_XX = 0, BN_XY, BN_TX, BN_TY, BN_NX, BN_NY, BN_NDNX, BN_NDNY, BN_NDTX, BN_NDTY,
+    BN_LEN, BN_KAPPA, BN_CPR, BN_CPI, BN_CSR, BN_CSI, BN_RHO, BN_PAD, NODE
+};
+
+template<typename real> struct HSd {
+    cpx<real> cP, cS;
+    real rho;
+    char bc;
+};
+
+template<typename real> struct Prob2D {
+    const real *base;      // blob: staged part first (SSP, boundaries), then the rest
+    int32_t staged_reals;  // length of the prefix that may be copied into LDS
+    // SSP
+    char ssp_type;
+    char run0;             // RunType[0]
+    char run1;             // RunType[1]
+    char run3, run4;       // RunType[3] ('X' line source), RunType[4] ('I' irregular)
+    char type2;            // Beam->Type[2] curvature option
+    char ati_curv, bty_curv; // 'C' curvilinear
+    char ati_long, bty_long; // 'L' long format (per-node geoacoustics)
+    int32_t npts, Nr;
+    int32_t o_z, o_rho, o_c, o_cz, o_n2, o_n2z, o_coef, o_segr, o_cmat, o_czmat;
+    int32_t ati_n, bty_n, o_ati, o_bty;
+    // rest of blob (global)
+    int32_t o_Rr, o_trc, o_brc, ntrc, nbrc, o_sbp, nsbp;
+    int32_t o_alpha, o_acos, o_asin;
+    const float *Rz;
+    const float *Sz;
+    int32_t NSz, NRz, NRr, NRz_per_range, Nalpha, iSingle;
+    real Delta_r, Dalpha, freq0, deltas, box_r, box_z;
+    HSd<real> top, bot;
+    int32_t isGaussian, isRayCen;
+};
+
+// Accessor over a blob pointer that may be LDS or global.
+template<typename real> struct View {
+    const real *b;
+    __host__ __device__ real operator[](int i) const { return b[i]; }
+};
+
+// ---------------------------------------------------------------------------
+// Ray state
+// ---------------------------------------------------------------------------
+template<typename real> struct RayPt {
+    v2<real> p, q, x, t;
+    real c, Amp, Phase;
+    cpx<real> tau;
+    int32_t NumTopBnc, NumBotBnc;
+};
+
+template<typename real> struct BdryState {
+    int32_t Iseg;
+    real lmin, lmax;
+    v2<real> x, n;
+};
+
+template<typename real> struct SSPOut {
+    v2<real> gradc;
+    cpx<real> c;
+    real rho, czz, crr, crz;
+};
+
+struct Err {
+    uint32_t err, warn, nerr, nwarn;
+    __host__ __device__ void error(int code)
+    {
+        err |= 1u << code;
+        ++nerr;
+    }
+    __host__ __device__ void warning(int code)
+    {
+        warn |= 1u << code;
+        ++nwarn;
+    }
+};
+
+template<typename real> struct Trace {
+    int32_t iz, ir;             // SSP segment state
+    BdryState<real> top, bot;
+    HSd<real> htop, hbot;       // per-segment geoacoustics (type[1]=='L')
+    real xs_y;
+    real DistBegTop, DistBegBot, DistEndTop, DistEndBot;
+};
+
+template<typename real> struct Infl {
+    real alpha, SrcDeclAngle, omega, BeamWindow, Ratio1, rcp_q0;
+    real phase, qOld;
+    v2<real> rayn1, x;
+    int32_t lastValid, ir;
+};
+
+// ---------------------------------------------------------------------------
+// SSP evaluation (reference src/ssp.hpp)
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline void UpdateSSPSegment(real x, real t, const real *arr, int n, int &iSeg)
+{ // ssp.hpp:36-51
+    if(t >= RL(0.0)) {
+        while(iSeg > 0 && x < arr[iSeg]) --iSeg;
+        while(iSeg < n - 2 && x >= arr[iSeg + 1]) ++iSeg;
+    } else {
+        while(iSeg < n - 2 && x > arr[iSeg + 1]) ++iSeg;
+        while(iSeg > 0 && x <= arr[iSeg]) --iSeg;
+    }
+}
+
+template<typename real> __host__ __device__ inline cpx<real> ldc(const real *p, int i) { return C(p[2 * i], p[2 * i + 1]); }
+
+template<typename real>
+__host__ __device__ inline void EvaluateSSP(v2<real> x, v2<real> t, SSPOut<real> &o, const Prob2D<real> &P,
+                                            const real *B, int &iz, int &ir, Err &e)
+{
+    const real *z = B + P.o_z;
+    o.crr = o.crz = o.czz = RL(0.0);
+    if(P.ssp_type != 'A' && P.ssp_type != 'Q') {
+        UpdateSSPSegment(x.y, t.y, z, P.npts, iz);
+        real w = (x.y - z[iz]) / (z[iz + 1] - z[iz]); // LinInterpDensity ssp.hpp:53-59
+        const real *rho = B + P.o_rho;
+        o.rho = (RL(1.0) - w) * rho[iz] + w * rho[iz + 1];
+        if(P.ssp_type == 'C') { // cLinear ssp.hpp:81-89
+            cpx<real> c = ldc(B + P.o_c, iz), cz = ldc(B + P.o_cz, iz);
+            o.c = c + rmul(x.y - z[iz], cz);
+            o.gradc = V(RL(0.0), cz.re);
+        } else if(P.ssp_type == 'S') { // cCubic ssp.hpp:133-152, SplineALL curves.hpp:190-200
+            const real *cf = B + P.o_coef;
+            int n2 = 2 * P.npts;
+            cpx<real> c1 = ldc(cf, iz), c2 = ldc(cf + n2, iz), c3 = ldc(cf + 2 * n2, iz), c4 = ldc(cf + 3 * n2, iz);
+            real h = x.y - z[iz];
+            const float half = FL(0.5), sixth = FL(1.0) / FL(6.0);
+            cpx<real> f = c1 + rmul(h, c2 + rmul(h, rmul((real)half, c3) + rmul((real)sixth * h, c4)));
+            cpx<real> fx = c2 + rmul(h, c3 + rmul(h * (real)half, c4));
+            cpx<real> fxx = c3 + rmul(h, c4);
+            o.c = f;
+            o.gradc = V(RL(0.0), fx.re);
+            o.czz = fxx.re;
+        } else if(P.ssp_type == 'P') { // cPCHIP ssp.hpp:95-128
+            const real *cf = B + P.o_coef;
+            int n2 = 2 * P.npts;
+            cpx<real> c0 = ldc(cf, iz), c1 = ldc(cf + n2, iz), c2 = ldc(cf + 2 * n2, iz), c3 = ldc(cf + 3 * n2, iz);
+            real xt = x.y - z[iz];
+            if(std::fabs(xt) > RL(1.0e10)) e.warning(16);
+            if(cabs_(c0) > RL(1.0e10) || cabs_(c1) > RL(1.0e10) || cabs_(c2) > RL(1.0e10) || cabs_(c3) > RL(1.0e10))
+                e.warning(17);
+            o.c = c0 + rmul(xt, c1 + rmul(xt, c2 + rmul(xt, c3)));
+            o.gradc = V(RL(0.0), (c1 + rmul(xt, rmul(RL(2.0), c2) + rmul(xt, rmul(RL(3.0), c3)))).re);
+            o.czz = (rmul(RL(2.0), c2) + rmul(xt, rmul(RL(6.0), c3))).re;
+        } else if(P.ssp_type == 'N') { // n2Linear ssp.hpp:64-76
+            cpx<real> a = ldc(B + P.o_n2, iz), b = ldc(B + P.o_n2, iz + 1);
+            o.c = realdiv(RL(1.0), csqrt_(rmul(RL(1.0) - w, a) + rmul(w, b)));
+            real c = o.c.re;
+            o.gradc = V(RL(0.0), RL(-0.5) * (c * c * c) * ldc(B + P.o_n2z, iz).re);
+            o.czz = RL(3.0) * o.gradc.y * o.gradc.y / c;
+        }
+    } else if(P.ssp_type == 'Q') { // Quad ssp.hpp:157-214
+        const real *segr = B + P.o_segr;
+        int Nr = P.Nr;
+        if(x.x < segr[0] || x.x > segr[Nr - 1]) e.error(8);
+        UpdateSSPSegment(x.y, t.y, z, P.npts, iz);
+        UpdateSSPSegment(x.x, t.x, segr, Nr, ir);
+        real w = (x.y - z[iz]) / (z[iz + 1] - z[iz]);
+        const real *rho = B + P.o_rho;
+        o.rho = (RL(1.0) - w) * rho[iz] + w * rho[iz + 1];
+        if(iz >= P.npts - 1 || ir >= Nr - 1) {
+            e.error(9);
+            iz = ir = 0;
+        }
+        const real *cm = B + P.o_cmat, *czm = B + P.o_czmat;
+        real cz1 = czm[iz * Nr + ir], cz2 = czm[iz * Nr + ir + 1];
+        real s2 = x.y - z[iz];
+        real delta_z = z[iz + 1] - z[iz];
+        real c1 = cm[iz * Nr + ir] + s2 * cz1;
+        real c2 = cm[iz * Nr + ir + 1] + s2 * cz2;
+        real delta_r = segr[ir + 1] - segr[ir];
+        real s1 = (x.x - segr[ir]) / delta_r;
+        s1 = rmin(s1, RL(1.0));
+        s1 = rmax(s1, RL(0.0));
+        real c = (RL(1.0) - s1) * c1 + s1 * c2;
+        s2 = s2 / delta_z;
+        real cim = (rmul(RL(1.0) - s2, ldc(B + P.o_c, iz)) + rmul(s2, ldc(B + P.o_c, iz + 1))).im;
+        o.c = C(c, cim);
+        real cz = (RL(1.0) - s1) * cz1 + s1 * cz2;
+        real cr = (c2 - c1) / delta_r;
+        o.crz = (cz2 - cz1) / delta_r;
+        o.gradc = V(cr, cz);
+    } else { // Analytic 2D ssp.hpp:338-371
+        iz = 0;
+        real c0 = FL(1500.0);
+        o.rho = FL(1.0);
+        const float unk1 = FL(1300.0), unk2 = FL(0.00737);
+        real xt = FL(2.0) * (x.y - unk1) / unk1;
+        real emxt = std::exp(-xt);
+        real DxtDz = FL(2.0) / unk1;
+        o.c = C(c0 * (FL(1.0) + unk2 * (xt - FL(1.0) + emxt)), (real)0);
+        real cz = c0 * unk2 * (FL(1.0) - emxt) * DxtDz;
+        o.czz = c0 * unk2 * emxt * (DxtDz * DxtDz);
+        o.gradc = V((real)FL(0.0), cz);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary segment lookup (reference src/boundary.hpp:66-227, 2D)
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline void GetBdrySeg(v2<real> x, v2<real> t, BdryState<real> &bds, const real *bd, int n,
+                                           bool longfmt, HSd<real> &hs, bool isTop, bool isInit, Err &e)
+{
+    if(!isInit
+       && !(x.x < bds.lmin || (x.x == bds.lmin && t.x < FL(0.0)) || x.x > bds.lmax
+            || (x.x == bds.lmax && t.x >= FL(0.0))))
+        return;
+    int s = imin(imax(bds.Iseg, 0), n - 2);
+    if(t.x >= FL(0.0)) {
+        while(s >= 0 && bd[s * NODE + BN_XX] > x.x) --s;
+        while(s >= 0 && s < n - 1 && bd[(s + 1) * NODE + BN_XX] <= x.x) ++s;
+    } else {
+        while(s < n - 1 && bd[(s + 1) * NODE + BN_XX] < x.x) ++s;
+        while(s >= 0 && s < n - 1 && bd[s * NODE + BN_XX] >= x.x) --s;
+    }
+    if(s < 0 || s >= n - 1) {
+        e.error(isTop ? 6 : 7);
+        s = 0;
+    }
+    bds.Iseg = s;
+    const real *b0 = bd + s * NODE;
+    bds.lmin = b0[BN_XX];
+    bds.lmax = b0[NODE + BN_XX];
+    bds.x = V(b0[BN_XX], b0[BN_XY]);
+    bds.n = V(b0[BN_NX], b0[BN_NY]);
+    if(longfmt) {
+        hs.cP = C(b0[BN_CPR], b0[BN_CPI]);
+        hs.cS = C(b0[BN_CSR], b0[BN_CSI]);
+        hs.rho = b0[BN_RHO];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Step (reference src/step.hpp, 2D)
+// ---------------------------------------------------------------------------
+template<typename real> __host__ __device__ inline bool outside_box(v2<real> x, int dim, const Prob2D<real> &P)
+{
+    return dim == 0 ? std::fabs(x.x - RL(0.0)) >= P.box_r : std::fabs(x.y - RL(0.0)) >= P.box_z;
+}
+
+template<typename real>
+__host__ __device__ inline void DepthInterfaceCrossing(bool stepTo, real &hInt, v2<real> &x, v2<real> x0,
+                                                       v2<real> u, int iz0, const real *z)
+{ // step.hpp:41-79
+    if(!stepTo) hInt = RC<real>::MAXV;
+    if(std::fabs(u.y) > RC<real>::EPS) {
+        if(z[iz0] > x.y) {
+            hInt = (z[iz0] - x0.y) / u.y;
+            if(stepTo) {
+                x = x0 + sc(hInt, u);
+                x.y = z[iz0];
+            }
+        } else if(z[iz0 + 1] < x.y) {
+            hInt = (z[iz0 + 1] - x0.y) / u.y;
+            if(stepTo) {
+                x = x0 + sc(hInt, u);
+                x.y = z[iz0 + 1];
+            }
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void BeamBoxCrossing(bool stepTo, int dim, real &hBox, v2<real> &x, v2<real> x0, v2<real> u,
+                                                const Prob2D<real> &P)
+{ // step.hpp:84-108
+    if(!stepTo) hBox = RC<real>::MAXV;
+    if(outside_box(x, dim, P)) {
+        real box = dim == 0 ? P.box_r : P.box_z;
+        real d0 = (dim == 0 ? x0.x : x0.y) - RL(0.0);
+        real uu = dim == 0 ? u.x : u.y;
+        hBox = (box - std::fabs(d0)) / std::fabs(uu);
+        if(stepTo) {
+            x = x0 + sc(hBox, u);
+            real s = RL(0.0) + std::copysign(box, d0);
+            if(dim == 0) x.x = s;
+            else x.y = s;
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void TopBotCrossing(bool stepTo, real &h, const BdryState<real> &bd, v2<real> &x, v2<real> x0,
+                                               v2<real> u, bool &refl)
+{ // step.hpp:113-147
+    if(!stepTo) h = RC<real>::MAXV;
+    v2<real> d = x - bd.x;
+    v2<real> d0 = x0 - bd.x;
+    real w = dot(bd.n, d);
+    if(stepTo ? (w > -RC<real>::INF_STEP) : (w >= RL(0.0))) {
+        h = -dot(d0, bd.n) / dot(u, bd.n);
+        if(stepTo) {
+            x = x0 + sc(h, u);
+            if(std::fabs(bd.n.x) < RC<real>::EPS) x.y = bd.x.y;
+        }
+        refl = true;
+    } else {
+        refl = false;
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void TopBotSegCrossing(bool stepTo, real &hSeg, const BdryState<real> &top,
+                                                  const BdryState<real> &bot, int ir0, v2<real> &x, v2<real> x0,
+                                                  v2<real> u, const Prob2D<real> &P, const real *B, bool &topRefl,
+                                                  bool &botRefl)
+{ // step.hpp:153-205
+    real smin = rmax(top.lmin, bot.lmin);
+    real smax = rmin(top.lmax, bot.lmax);
+    if(P.ssp_type == 'Q') {
+        const real *segr = B + P.o_segr;
+        smin = rmax(smin, segr[ir0]);
+        smax = rmin(smax, segr[ir0 + 1]);
+    }
+    if(!stepTo) hSeg = RC<real>::MAXV;
+    if(std::fabs(u.x) > RC<real>::EPS) {
+        if(x.x < smin) {
+            hSeg = -(x0.x - smin) / u.x;
+            if(stepTo) {
+                x = x0 + sc(hSeg, u);
+                x.x = smin;
+                topRefl = botRefl = false;
+            }
+        } else if(x.x > smax) {
+            hSeg = -(x0.x - smax) / u.x;
+            if(stepTo) {
+                x = x0 + sc(hSeg, u);
+                x.x = smax;
+                topRefl = botRefl = false;
+            }
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void ReduceStep(v2<real> x0, v2<real> u, int iz0, int ir0, const Trace<real> &S,
+                                           const Prob2D<real> &P, const real *B, Err &e, real &h)
+{ // step.hpp:271-342
+    real hInt, hBoxxr, hBoxyz, hTop, hBot, hxSeg;
+    bool dummy;
+    v2<real> x = x0 + sc(h, u);
+    DepthInterfaceCrossing(false, hInt, x, x0, u, iz0, B + P.o_z);
+    BeamBoxCrossing(false, 0, hBoxxr, x, x0, u, P);
+    BeamBoxCrossing(false, 1, hBoxyz, x, x0, u, P);
+    TopBotCrossing(false, hTop, S.top, x, x0, u, dummy);
+    TopBotCrossing(false, hBot, S.bot, x, x0, u, dummy);
+    TopBotSegCrossing(false, hxSeg, S.top, S.bot, ir0, x, x0, u, P, B, dummy, dummy);
+    const real BIG = RC<real>::MAXV;
+    h = rmin(rmin(rmin(h, hInt), rmin(rmin(rmin(hBoxxr, hBoxyz), BIG), rmin(hTop, hBot))),
+             rmin(rmin(hxSeg, BIG), rmin(BIG, BIG)));
+    if(h < RL(-1e-4)) e.warning(8);
+    if(h < RC<real>::INF_STEP * P.deltas) h = RC<real>::INF_STEP * P.deltas;
+}
+
+template<typename real>
+__host__ __device__ inline void StepToBdry(v2<real> x0, v2<real> &x2, v2<real> u, real &h, bool &topRefl, bool &botRefl,
+                                           int iz0, int ir0, const Trace<real> &S, const Prob2D<real> &P,
+                                           const real *B)
+{ // step.hpp:347-424
+    x2 = x0 + sc(h, u);
+    DepthInterfaceCrossing(true, h, x2, x0, u, iz0, B + P.o_z);
+    BeamBoxCrossing(true, 0, h, x2, x0, u, P);
+    BeamBoxCrossing(true, 1, h, x2, x0, u, P);
+    TopBotCrossing(true, h, S.top, x2, x0, u, topRefl);
+    TopBotCrossing(true, h, S.bot, x2, x0, u, botRefl);
+    if(botRefl) topRefl = false;
+    TopBotSegCrossing(true, h, S.top, S.bot, ir0, x2, x0, u, P, B, topRefl, botRefl);
+    if(h < RC<real>::INF_STEP * P.deltas) {
+        h = RC<real>::INF_STEP * P.deltas;
+        x2 = x0 + sc(h, u);
+        v2<real> d = x2 - S.top.x;
+        topRefl = dot(S.top.n, d) > RC<real>::EPS;
+        d = x2 - S.bot.x;
+        if(dot(S.bot.n, d) > RC<real>::EPS) {
+            botRefl = true;
+            topRefl = false;
+        } else {
+            botRefl = false;
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void Step(const RayPt<real> &ray0, RayPt<real> &ray2, Trace<real> &S, const Prob2D<real> &P,
+                                     const real *B, Err &e, bool &topRefl, bool &botRefl)
+{ // step.hpp:654-811 (2D)
+    SSPOut<real> o0, o1, o2;
+    EvaluateSSP(ray0.x, ray0.t, o0, P, B, S.iz, S.ir, e);
+    real c0 = o0.c.re;
+    real cnn0 = o0.crr * (ray0.t.y * ray0.t.y) - FL(2.0) * o0.crz * ray0.t.x * ray0.t.y + o0.czz * (ray0.t.x * ray0.t.x);
+    v2<real> pq0p = sc(-cnn0, ray0.q), pq0q = sc(c0, ray0.p);
+    int iz0 = S.iz, ir0 = S.ir;
+    real csq0 = c0 * c0;
+    v2<real> u0 = sc(c0, ray0.t);
+    real h = P.deltas;
+    ReduceStep(ray0.x, u0, iz0, ir0, S, P, B, e, h);
+    real halfh = FL(0.5) * h;
+    RayPt<real> ray1;
+    ray1.x = ray0.x + sc(halfh, u0);
+    ray1.t = ray0.t - dv(sc(halfh, o0.gradc), csq0);
+    ray1.p = ray0.p + sc(halfh, pq0p);
+    ray1.q = ray0.q + sc(halfh, pq0q);
+
+    EvaluateSSP(ray1.x, ray1.t, o1, P, B, S.iz, S.ir, e);
+    real c1 = o1.c.re;
+    real cnn1 = o1.crr * (ray1.t.y * ray1.t.y) - FL(2.0) * o1.crz * ray1.t.x * ray1.t.y + o1.czz * (ray1.t.x * ray1.t.x);
+    v2<real> pq1p = sc(-cnn1, ray1.q), pq1q = sc(c1, ray1.p);
+    real csq1 = c1 * c1;
+    v2<real> u1 = sc(c1, ray1.t);
+    ReduceStep(ray0.x, u1, iz0, ir0, S, P, B, e, h);
+
+    real w1 = h / (RL(2.0) * halfh);
+    real w0 = RL(1.0) - w1;
+    v2<real> u2 = sc(w0, u0) + sc(w1, u1);
+    v2<real> x2;
+    StepToBdry(ray0.x, x2, u2, h, topRefl, botRefl, iz0, ir0, S, P, B);
+    ray2.x = x2;
+    real hw0 = h * w0, hw1 = h * w1;
+    ray2.t = (ray0.t - dv(sc(hw0, o0.gradc), csq0)) - dv(sc(hw1, o1.gradc), csq1);
+    ray2.tau = (ray0.tau + realdiv(hw0, o0.c)) + realdiv(hw1, o1.c);
+    ray2.p = ray0.p + sc(hw0, pq0p);
+    ray2.q = ray0.q + sc(hw0, pq0q);
+    ray2.p = ray2.p + sc(hw1, pq1p);
+    ray2.q = ray2.q + sc(hw1, pq1q);
+    ray2.Amp = ray0.Amp;
+    ray2.Phase = ray0.Phase;
+    ray2.NumTopBnc = ray0.NumTopBnc;
+    ray2.NumBotBnc = ray0.NumBotBnc;
+
+    EvaluateSSP(ray2.x, ray2.t, o2, P, B, S.iz, S.ir, e);
+    ray2.c = o2.c.re;
+    if(S.iz != iz0 || S.ir != ir0) { // CurvatureCorrection 2D, step.hpp:626-648
+        v2<real> gj = o2.gradc - o0.gradc;
+        v2<real> ray2n = V(-ray2.t.y, ray2.t.x);
+        real cnjump = dot(gj, ray2n);
+        real csjump = dot(gj, ray2.t);
+        real rm = (S.iz != iz0) ? ray2.t.x / ray2.t.y : -ray2.t.y / ray2.t.x;
+        real rn = rm * (FL(2.0) * cnjump - rm * csjump) / ray2.c;
+        ray2.p = ray2.p - sc(rn, ray2.q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reflection (reference src/reflect.hpp, 2D)
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline void Reflect(const RayPt<real> &oldp, RayPt<real> &newp, const HSd<real> &hs, bool isTop,
+                                        v2<real> tB, v2<real> nB, real kappa, const Prob2D<real> &P, const real *B,
+                                        const real *G, int &iz, int &ir, Err &e)
+{
+    real Th = dot(oldp.t, nB);
+    real Tg = dot(oldp.t, tB);
+    newp.NumTopBnc = oldp.NumTopBnc + (isTop ? 1 : 0);
+    newp.NumBotBnc = oldp.NumBotBnc + (isTop ? 0 : 1);
+    newp.x = oldp.x;
+    newp.t = oldp.t - sc(FL(2.0) * Th, nB);
+    SSPOut<real> o;
+    EvaluateSSP(oldp.x, oldp.t, o, P, B, iz, ir, e);
+    real c = o.c.re;
+    newp.c = c;
+    newp.tau = oldp.tau;
+    v2<real> rayt = sc(c, oldp.t);
+    v2<real> rayn = V(-rayt.y, rayt.x);
+    v2<real> rayt_t = sc(c, newp.t);
+    v2<real> rayn_t = V(-(-rayt_t.y), -(rayt_t.x));
+    real rn = FL(2.0) * kappa / (c * c) / Th;
+    real cnjump = -dot(o.gradc, rayn_t - rayn);
+    real csjump = -dot(o.gradc, rayt_t - rayt);
+    if(isTop) {
+        cnjump = -cnjump;
+        rn = -rn;
+    }
+    real rm = Tg / Th;
+    rn += rm * (FL(2.0) * cnjump - rm * csjump) / (c * c);
+    if(P.type2 == 'D') rn *= FL(2.0);
+    else if(P.type2 == 'Z') rn = FL(0.0);
+    newp.p = oldp.p + sc(rn, oldp.q);
+    newp.q = oldp.q;
+
+    if(hs.bc == 'R') {
+        newp.Amp = oldp.Amp;
+        newp.Phase = oldp.Phase;
+    } else if(hs.bc == 'V') {
+        newp.Amp = oldp.Amp;
+        newp.Phase = oldp.Phase + RC<real>::PI;
+    } else if(hs.bc == 'F') { // InterpolateReflectionCoefficient reflect.hpp:39-92
+        const real *tab = G + (isTop ? P.o_trc : P.o_brc);
+        int n = isTop ? P.ntrc : P.nbrc;
+        real theta = (RL(180.0) / RC<real>::PI) * std::fabs(std::atan2(Th, Tg));
+        if(theta > FL(90.0)) theta = FL(180.0) - theta;
+        real r, phi;
+        int iL = 0, iR = n - 1;
+        if(theta < tab[0]) {
+            r = phi = FL(0.0);
+            e.warning(5);
+        } else if(theta > tab[3 * iR]) {
+            r = phi = FL(0.0);
+        } else {
+            while(iL != iR - 1) {
+                int iM = (iL + iR) / 2;
+                if(tab[3 * iM] > theta) iR = iM;
+                else iL = iM;
+            }
+            real a = (theta - tab[3 * iL]) / (tab[3 * iR] - tab[3 * iL]);
+            r = (FL(1.0) - a) * tab[3 * iL + 1] + a * tab[3 * iR + 1];
+            phi = (FL(1.0) - a) * tab[3 * iL + 2] + a * tab[3 * iR + 2];
+        }
+        newp.Amp = oldp.Amp * r;
+        newp.Phase = oldp.Phase + phi;
+    } else if(hs.bc == 'A' || hs.bc == 'G') {
+        real omega = FL(2.0) * RC<real>::PI * P.freq0;
+        cpx<real> kx = C(omega * Tg, (real)0), kz = C(omega * Th, (real)0);
+        cpx<real> kx2 = cmul(kx, kx);
+        cpx<real> f, g;
+        if(hs.cS.re > FL(0.0)) {
+            cpx<real> wS = realdiv(omega, hs.cS), wP = realdiv(omega, hs.cP);
+            cpx<real> kzS2 = kx2 - cmul(wS, wS);
+            cpx<real> kzP2 = kx2 - cmul(wP, wP);
+            cpx<real> kzS = csqrt_(kzS2), kzP = csqrt_(kzP2);
+            cpx<real> mu = rmul(hs.rho, cmul(hs.cS, hs.cS));
+            cpx<real> s = kzS2 + kx2;
+            cpx<real> y2 = cmul(cmul(s, s) - cmul(cmul(rmul(RL(4.0), kzS), kzP), kx2), mu);
+            cpx<real> y4 = cmul(kzP, kx2 - kzS2);
+            f = rmul(omega * omega, y4);
+            g = y2;
+        } else {
+            cpx<real> wP = realdiv(omega, hs.cP);
+            cpx<real> kzP = csqrt_(kx2 - cmul(wP, wP));
+            if(std::fabs(kzP.re) < RC<real>::EPS && kzP.im < RL(0.0)) kzP = -kzP;
+            f = kzP;
+            g = C(hs.rho, (real)0);
+        }
+        cpx<real> J = C((real)0, (real)1);
+        cpx<real> a = rmul(o.rho, f), b = cmul(cmul(J, kz), g);
+        cpx<real> Refl = cdiv(-(a - b), a + b);
+        real aR = cabs_(Refl);
+        if(aR < FL(1.0e-5)) {
+            newp.Amp = FL(0.0);
+            newp.Phase = oldp.Phase;
+        } else {
+            newp.Amp = aR * oldp.Amp;
+            newp.Phase = oldp.Phase + std::atan2(Refl.im, Refl.re);
+        }
+    } else {
+        e.error(4);
+        newp.Amp = NAN;
+        newp.Phase = NAN;
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void Distances(v2<real> x, const Trace<real> &S, real &dt, real &db)
+{ // trace.hpp:37-45
+    dt = -dot(S.top.n, x - S.top.x);
+    db = -dot(S.bot.n, x - S.bot.x);
+}
+
+// ---------------------------------------------------------------------------
+// Ray init / update / terminate (reference src/trace.hpp, 2D)
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline int BinarySearchLEQ(const real *arr, int n, int stride, int offset, real target)
+{ // common_run.hpp:61-80
+    int low = 0, hi = n;
+    while(low < hi) {
+        int t = (low + hi) / 2;
+        if(arr[t * stride + offset] > target) hi = t;
+        else if(t >= n - 1) return n - 1;
+        else if(arr[(t + 1) * stride + offset] > target) return t;
+        else low = t + 1;
+    }
+    return low;
+}
+template<typename real> __host__ __device__ inline int BinarySearchGT(const real *arr, int n, real target)
+{ // common_run.hpp:119-138
+    int low = -1, hi = n - 1;
+    while(low < hi) {
+        int t = (low + hi + 1) / 2;
+        if(arr[t] <= target) low = t;
+        else if(t <= 0) return 0;
+        else if(arr[t - 1] <= target) return t;
+        else hi = t - 1;
+    }
+    return hi;
+}
+
+// Returns false when the ray is not traced (source outside the boundaries).
+template<typename real>
+__host__ __device__ inline bool RayInit(int isz, int ialpha, RayPt<real> &p0, Trace<real> &S, Infl<real> &I,
+                                        const Prob2D<real> &P, const real *B, const real *G, Err &e)
+{ // trace.hpp:76-266 (2D) + Init_Influence influence.hpp:525-644
+    real alpha = G[P.o_alpha + ialpha];
+    real RadDeg = RL(180.0) / RC<real>::PI;
+    I.alpha = alpha;
+    I.SrcDeclAngle = RadDeg * alpha;
+    S.iz = S.ir = 0;
+    v2<real> xs = V((real)FL(0.0), (real)P.Sz[isz]);
+    S.xs_y = xs.y;
+    v2<real> tinit = V(G[P.o_acos + ialpha], G[P.o_asin + ialpha]);
+    SSPOut<real> o;
+    EvaluateSSP(xs, tinit, o, P, B, S.iz, S.ir, e);
+    real c = o.c.re;
+    {
+        real DalphaOpt = std::sqrt(c / (FL(6.0) * P.freq0 * G[P.o_Rr + P.NRr - 1]));
+        int NalphaOpt = 2 + (int)((G[P.o_alpha + P.Nalpha - 1] - G[P.o_alpha]) / DalphaOpt);
+        if(P.run0 == 'C' && P.Nalpha < NalphaOpt && ialpha == 0) e.warning(3);
+    }
+    const real *sbp = G + P.o_sbp;
+    int ibp = BinarySearchLEQ(sbp, P.nsbp, 2, 0, I.SrcDeclAngle);
+    ibp = imin(ibp, P.nsbp - 2);
+    real s = (I.SrcDeclAngle - sbp[2 * ibp]) / (sbp[2 * (ibp + 1)] - sbp[2 * ibp]);
+    float Amp0 = (float)((FL(1.0) - s) * sbp[2 * ibp + 1] + s * sbp[2 * (ibp + 1) + 1]);
+    if(P.run0 == 'S') {
+        float omega = (float)(FL(2.0) * RC<real>::PI * P.freq0);
+        Amp0 = (float)(Amp0 * (sqrtf(FL(2.0)) * std::fabs(std::sin(omega / c * xs.y * G[P.o_asin + ialpha]))));
+    }
+    p0.x = xs;
+    p0.c = c;
+    p0.t = dv(tinit, c);
+    p0.tau = C((real)0, (real)0);
+    p0.Amp = Amp0;
+    p0.Phase = FL(0.0);
+    p0.NumTopBnc = p0.NumBotBnc = 0;
+    p0.p = V((real)FL(1.0), (real)FL(0.0));
+    p0.q = V((real)FL(0.0), (real)FL(1.0));
+    if(P.run1 == 'G') p0.q = V((real)FL(0.0), (real)FL(0.0));
+    S.htop = P.top;
+    S.hbot = P.bot;
+    S.top.Iseg = S.bot.Iseg = 0;
+    GetBdrySeg(xs, p0.t, S.top, B + P.o_ati, P.ati_n, P.ati_long == 'L', S.htop, true, true, e);
+    GetBdrySeg(xs, p0.t, S.bot, B + P.o_bty, P.bty_n, P.bty_long == 'L', S.hbot, false, true, e);
+    Distances(xs, S, S.DistBegTop, S.DistBegBot);
+    if(S.DistBegTop <= FL(0.0) || S.DistBegBot <= FL(0.0)) {
+        e.warning(4);
+        return false;
+    }
+    // Init_Influence
+    I.omega = FL(2.0) * RC<real>::PI * P.freq0;
+    I.BeamWindow = P.isGaussian ? RL(4.0) : RL(1.0);
+    I.Ratio1 = std::sqrt(std::fabs(G[P.o_acos + ialpha]));
+    if(P.run3 == 'X') I.Ratio1 = RL(1.0);
+    if(P.isGaussian) I.Ratio1 /= std::sqrt(FL(2.0) * RC<real>::PI);
+    I.rcp_q0 = P.Dalpha / p0.c;
+    I.phase = FL(0.0);
+    I.qOld = p0.q.x;
+    I.x = p0.x;
+    if(P.isRayCen) {
+        I.rayn1 = sc(p0.c, V(p0.t.y, -p0.t.x));
+        I.lastValid = std::fabs(I.rayn1.y) >= RL(1e-6);
+        I.ir = 0;
+    } else {
+        I.rayn1 = V((real)0, (real)0);
+        I.lastValid = 0;
+        I.ir = BinarySearchGT(G + P.o_Rr, P.NRr, p0.x.x);
+        if(p0.t.x < RL(0.0) && I.ir > 0) --I.ir;
+    }
+    return true;
+}
+
+template<typename real>
+__host__ __device__ inline bool RayUpdate(const RayPt<real> &p0, RayPt<real> &p1, RayPt<real> &p2, Trace<real> &S,
+                                          const Prob2D<real> &P, const real *B, const real *G, Err &e)
+{ // trace.hpp:274-381 (2D)
+    bool topRefl = false, botRefl = false;
+    Step(p0, p1, S, P, B, e, topRefl, botRefl);
+    GetBdrySeg(p1.x, p1.t, S.top, B + P.o_ati, P.ati_n, P.ati_long == 'L', S.htop, true, false, e);
+    GetBdrySeg(p1.x, p1.t, S.bot, B + P.o_bty, P.bty_n, P.bty_long == 'L', S.hbot, false, false, e);
+    Distances(p1.x, S, S.DistEndTop, S.DistEndBot);
+    if(topRefl || botRefl) {
+        const real *bd = B + (topRefl ? P.o_ati : P.o_bty);
+        const BdryState<real> &bs = topRefl ? S.top : S.bot;
+        const real *b0 = bd + bs.Iseg * NODE;
+        const real *b1 = b0 + NODE;
+        v2<real> nI, tI;
+        if((topRefl ? P.ati_curv : P.bty_curv) == 'C') {
+            real sss = dot(p1.x - bs.x, V(b0[BN_TX], b0[BN_TY])) / b0[BN_LEN];
+            nI = sc(FL(1.0) - sss, V(b0[BN_NDNX], b0[BN_NDNY])) + sc(sss, V(b1[BN_NDNX], b1[BN_NDNY]));
+            tI = sc(FL(1.0) - sss, V(b0[BN_NDTX], b0[BN_NDTY])) + sc(sss, V(b1[BN_NDTX], b1[BN_NDTY]));
+        } else {
+            nI = V(b0[BN_NX], b0[BN_NY]);
+            tI = V(b0[BN_TX], b0[BN_TY]);
+        }
+        Reflect(p1, p2, topRefl ? S.htop : S.hbot, topRefl, tI, nI, b0[BN_KAPPA], P, B, G, S.iz, S.ir, e);
+        Distances(p2.x, S, S.DistEndTop, S.DistEndBot);
+        return true;
+    }
+    return false;
+}
+
+// Returns 0 to continue, 1 for a normal stop (Nsteps = is + 1), 2 when the
+// per-ray storage is exhausted (Nsteps = is).
+template<typename real>
+__host__ __device__ inline int RayTerminate(const RayPt<real> &pt, int is, Trace<real> &S, int MaxPoints,
+                                            const Prob2D<real> &P, Err &e)
+{ // trace.hpp:391-477 (2D)
+    bool leftbox = outside_box(pt.x, 0, P) || outside_box(pt.x, 1, P);
+    bool escaped = (S.DistBegTop < FL(0.0) && S.DistEndTop < FL(0.0))
+        || (S.DistBegBot < FL(0.0) && S.DistEndBot < FL(0.0));
+    bool lost = pt.Amp < FL(0.005);
+    if(leftbox || lost || escaped) return 1;
+    if(is >= MaxPoints - 3) {
+        e.warning(1);
+        return 2;
+    }
+    S.DistBegTop = S.DistEndTop;
+    S.DistBegBot = S.DistEndBot;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Influence (reference src/influence.hpp, geometric 2D)
+//
+// The reference walks the receivers of each step segment inside the ray's own
+// thread. Here a segment is first reduced to a descriptor (SegDesc) by the lane
+// that owns the ray -- this applies every update of the ray's influence state
+// in the reference's order -- and the receiver work items of the descriptor are
+// then evaluated by any lane (the whole wavefront, in the kernel). Every
+// contribution is computed from the same operands with the same operations as
+// in the reference, so only the order of the field additions changes.
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline bool IsAtCaustic(real qOld, real q)
+{
+    return (q <= RL(0.0) && qOld > RL(0.0)) || (q >= RL(0.0) && qOld < RL(0.0));
+}
+
+template<typename real> struct SegDesc {
+    // geometry (Cartesian: x_ray, rayt, rayn1, rlen, zmin, zmax, lo, hi, ir range;
+    //           ray-centred: A = previous normal point / normal, B = this one)
+    v2<real> xa, ta, xb, tb;
+    real rlen, zmin, zmax, lo, hi;
+    // beam quantities of InfluenceGeoCore
+    real q0x, dqx, p0c, p1c, p1Amp, phasePt, phase, qOld, p1taure, Ratio1, rcp_q0, RcvrDecl, SrcDecl;
+    cpx<real> tau0, dtau;
+    int32_t ir0, ir1, dir;      // Cartesian: visited receiver ranges ir0..ir1 (step dir)
+    int32_t nitems;             // work items (0 = nothing to scatter)
+    int32_t lastValid;          // ray-centred
+    int32_t ntop, nbot, isz, ialpha, is;
+};
+
+template<typename real>
+__host__ __device__ inline void desc_common(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
+                                            const Infl<real> &I, const Prob2D<real> &P, int is, int isz, int ialpha)
+{
+    d.q0x = p0.q.x;
+    d.dqx = p1.q.x - p0.q.x;
+    d.tau0 = p0.tau;
+    d.dtau = p1.tau - p0.tau;
+    d.p0c = p0.c;
+    d.p1c = p1.c;
+    d.p1Amp = p1.Amp;
+    d.phasePt = P.isGaussian ? p1.Phase : p0.Phase;
+    d.phase = I.phase;
+    d.qOld = I.qOld;
+    d.p1taure = p1.tau.re;
+    d.Ratio1 = I.Ratio1;
+    d.rcp_q0 = I.rcp_q0;
+    d.SrcDecl = I.SrcDeclAngle;
+    d.ntop = p1.NumTopBnc;
+    d.nbot = p1.NumBotBnc;
+    d.isz = isz;
+    d.ialpha = ialpha;
+    d.is = is;
+}
+
+// Cartesian prelude: influence.hpp:1201-1297 and the receiver walk :1388-1402.
+template<typename real>
+__host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
+                                             Infl<real> &I, int is, const Prob2D<real> &P, const real *G, int isz,
+                                             int ialpha)
+{
+    const real *Rr = G + P.o_Rr;
+    d.nitems = 0;
+    real rA = I.x.x, rB = p1.x.x;
+    v2<real> rayt = p1.x - p0.x;
+    real rlen = std::sqrt(dot(rayt, rayt));
+    if(rlen < RL(1.0e3) * spacing(p1.x.x)) return;
+    rayt = dv(rayt, rlen);
+    real phaseq = p0.q.x;
+    if(IsAtCaustic(I.qOld, phaseq)) I.phase += RC<real>::PI / FL(2.0);
+    I.qOld = phaseq;
+    real sigma = rmax(std::fabs(p0.q.x), std::fabs(p1.q.x)) * I.rcp_q0 / std::fabs(rayt.x);
+    if(P.isGaussian) {
+        real lambda = p0.c / P.freq0;
+        sigma = rmax(sigma, rmin(FL(0.2) * P.freq0 * p1.tau.re, RC<real>::PI * lambda));
+    }
+    real RadiusMax = I.BeamWindow * sigma;
+    if(std::fabs(rayt.x) > FL(0.50001)) {
+        d.zmin = rmin(p0.x.y, p1.x.y) - RadiusMax;
+        d.zmax = rmax(p0.x.y, p1.x.y) + RadiusMax;
+    } else {
+        d.zmin = -RC<real>::MAXV;
+        d.zmax = RC<real>::MAXV;
+    }
+    d.lo = rmin(rA, rB);
+    d.hi = rmax(rA, rB);
+    // walk: the set of visited receiver ranges is contiguous
+    int ir = I.ir, first = I.ir;
+    for(;;) {
+        int irTT;
+        if(rB > Rr[ir]) {
+            if(ir >= P.NRr - 1) break;
+            irTT = ir + 1;
+            if(Rr[irTT] >= rB) break;
+        } else {
+            if(ir <= 0) break;
+            irTT = ir - 1;
+            if(Rr[irTT] <= rB) break;
+        }
+        ir = irTT;
+    }
+    I.ir = ir;
+    I.x = p1.x;
+    d.ir0 = first;
+    d.ir1 = ir;
+    d.dir = (ir >= first) ? 1 : -1;
+    int nr = (ir >= first ? ir - first : first - ir) + 1;
+    d.nitems = nr * P.NRz_per_range;
+    d.xa = p0.x;
+    d.ta = rayt;
+    d.tb = V(-rayt.y, rayt.x);
+    d.rlen = rlen;
+    d.RcvrDecl = (RL(180.0) / RC<real>::PI) * std::atan2(rayt.y, rayt.x);
+    desc_common(d, p0, p1, I, P, is, isz, ialpha);
+}
+
+// Ray-centred prelude: influence.hpp:1030-1071.
+template<typename real>
+__host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
+                                               Infl<real> &I, int is, const Prob2D<real> &P, int isz, int ialpha)
+{
+    d.nitems = 0;
+    real phaseq = p0.q.x;
+    if(IsAtCaustic(I.qOld, phaseq)) I.phase += RC<real>::PI / FL(2.0);
+    I.qOld = phaseq;
+    v2<real> rayn1 = sc(p1.c, V(p1.t.y, -p1.t.x));
+    if(std::fabs(rayn1.y) < RL(1e-10)) return;
+    if(std::fabs(p1.x.x - p0.x.x) < RL(1.0e3) * spacing(p1.x.x)) {
+        I.lastValid = 1;
+        I.x = p1.x;
+        I.rayn1 = rayn1;
+        return;
+    }
+    d.lastValid = I.lastValid;
+    d.xa = I.x;
+    d.ta = I.rayn1;
+    d.xb = p1.x;
+    d.tb = rayn1;
+    d.RcvrDecl = (RL(180.0) / RC<real>::PI) * std::atan2(p1.t.y, p1.t.x);
+    desc_common(d, p0, p1, I, P, is, isz, ialpha);
+    d.nitems = P.NRz_per_range;
+    I.lastValid = 1;
+    I.x = p1.x;
+    I.rayn1 = rayn1;
+}
+
+// InfluenceGeoCore (influence.hpp:897-1024, 2D) + ApplyContribution (:432-473).
+template<typename real, int RT, typename Sink>
+__host__ __device__ inline void contrib(const SegDesc<real> &d, real s, real n1, int ir, int iz, const Prob2D<real> &P,
+                                        Sink &sink)
+{
+    real qFinal = d.q0x + s * d.dqx;
+    real sigma, sigma_orig;
+    sigma = sigma_orig = std::fabs(qFinal * d.rcp_q0);
+    if(P.isGaussian) {
+        real lambda = d.p0c / P.freq0;
+        sigma = rmax(sigma, rmin(FL(0.2) * P.freq0 * d.p1taure, RC<real>::PI * lambda));
+    }
+    real BW = P.isGaussian ? RL(4.0) : RL(1.0);
+    if(n1 > BW * sigma || n1 == BW * sigma) return;
+    cpx<real> delay = d.tau0 + rmul(s, d.dtau);
+    real cfactor = std::sqrt(d.p1c);
+    real cnst = d.Ratio1 * cfactor * d.p1Amp / std::sqrt(std::fabs(qFinal));
+    real w;
+    if(P.isGaussian) {
+        real t = n1 / sigma;
+        w = std::exp(FL(-0.5) * (t * t)) * (sigma_orig / sigma);
+    } else {
+        w = (sigma - n1) / sigma;
+    }
+    real phaseInt = d.phasePt + d.phase;
+    if(IsAtCaustic(d.qOld, qFinal)) phaseInt = d.phase + RC<real>::PI / FL(2.0);
+    size_t idx = ((size_t)d.isz * P.NRz_per_range + iz) * P.NRr + ir;
+    real omega = FL(2.0) * RC<real>::PI * P.freq0;
+    if constexpr(RT == RT_TL) {
+        float fr, fi;
+        if(P.run0 == 'C') {
+            cpx<real> arg = rmul(omega, delay);
+            arg.re = arg.re - phaseInt;
+            cpx<real> ex = cexp_(C(arg.im, -arg.re)); // std::exp(-J * arg)
+            cpx<real> dd = rmul(cnst * w, ex);
+            fr = (float)dd.re;
+            fi = (float)dd.im;
+        } else {
+            real v = cnst * std::exp(omega * delay.im);
+            v = (v * v) * w;
+            if(P.isGaussian) v *= std::sqrt(FL(2.0) * RC<real>::PI);
+            fr = (float)v;
+            fi = 0.0f;
+        }
+        sink.add_field(idx, fr, fi);
+    } else if constexpr(RT == RT_ARR) {
+        sink.add_arrival(idx, cnst * w, phaseInt, delay, d.SrcDecl, d.RcvrDecl, d.ntop, d.nbot, d.ialpha, d.isz,
+                         d.is);
+        if(P.run0 == 'V' || P.run0 == 'v') sink.add_eigen(d.is, ir, iz, d.isz, d.ialpha);
+    } else if constexpr(RT == RT_EIG) {
+        sink.add_eigen(d.is, ir, iz, d.isz, d.ialpha);
+    }
+    sink.count();
+}
+
+template<typename real> __host__ __device__ inline int RToIR(real r, const Prob2D<real> &P, const real *Rr)
+{ // influence.hpp:82-89
+    real temp = (r - Rr[0]) / P.Delta_r;
+    if(std::fabs(temp - std::round(temp)) < RL(1e-6)) temp = std::round(temp);
+    return imax(imin(trunc_int_x86(temp), P.NRr - 1), 0);
+}
+
+// Cartesian work item k -> (ir, iz): influence.hpp:1298-1386 (2D).
+template<typename real, int RT, typename Sink>
+__host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
+                                          const float *Rz, Sink &sink)
+{
+    int nz = P.NRz_per_range;
+    int j = k / nz, iz = k - j * nz;
+    int ir = d.ir0 + j * d.dir;
+    real rr = G[P.o_Rr + ir];
+    if(!(rr >= d.lo && rr < d.hi)) return;
+    int tiz = (P.run4 == 'I') ? ir : iz;
+    real zr = (real)Rz[tiz];
+    if(zr < d.zmin || zr > d.zmax) return;
+    v2<real> xrr = V(rr, zr) - d.xa;
+    real s = dot(xrr, d.ta) / d.rlen;
+    real n1 = std::fabs(dot(xrr, d.tb));
+    contrib<real, RT>(d, s, n1, ir, iz, P, sink);
+}
+
+// Ray-centred work item = one receiver depth: influence.hpp:1075-1187 (2D).
+template<typename real, int RT, typename Sink>
+__host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, const Prob2D<real> &P, const real *G,
+                                            const float *Rz, Sink &sink)
+{
+    const real *Rr = G + P.o_Rr;
+    real zR = (real)Rz[iz];
+    real nA, rA, nB, rB;
+    int irA, irB;
+    if(!d.lastValid) {
+        nA = RL(1e10);
+        rA = RL(1e10);
+        irA = 0;
+    } else {
+        nA = (zR - d.xa.y) / d.ta.y;
+        rA = d.xa.x + nA * d.ta.x;
+        irA = RToIR(rA, P, Rr);
+    }
+    nB = (zR - d.xb.y) / d.tb.y;
+    rB = d.xb.x + nB * d.tb.x;
+    irB = RToIR(rB, P, Rr);
+    if(irA == irB) return;
+    int notii = (irB <= irA) ? 0 : 1;
+    int step = (notii << 1) - 1;
+    for(int ir = irA + notii; ir != irB + notii; ir += step) {
+        real s = (Rr[ir] - rA) / (rB - rA);
+        real n1 = std::fabs(nA + s * (nB - nA));
+        contrib<real, RT>(d, s, n1, ir, iz, P, sink);
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void prelude(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1, Infl<real> &I,
+                                        int is, const Prob2D<real> &P, const real *G, int isz, int ialpha)
+{
+    if(P.isRayCen) prelude_raycen(d, p0, p1, I, is, P, isz, ialpha);
+    else prelude_cart(d, p0, p1, I, is, P, G, isz, ialpha);
+}
+
+template<typename real, int RT, typename Sink>
+__host__ __device__ inline void item(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
+                                     const float *Rz, Sink &sink)
+{
+    if(P.isRayCen) item_raycen<real, RT>(d, k, P, G, Rz, sink);
+    else item_cart<real, RT>(d, k, P, G, Rz, sink);
+}
+
+// Job index -> (isz, ialpha), reference src/common.hpp:365-391 (2D).
+template<typename real> __host__ __device__ inline void JobIndices(int job, const Prob2D<real> &P, int &isz, int &ia)
+{
+    if(P.iSingle >= 1) {
+        ia = P.iSingle - 1;
+    } else {
+        ia = job % P.Nalpha;
+        job /= P.Nalpha;
+    }
+    isz = job;
+}
+
+} // namespace bhc2d

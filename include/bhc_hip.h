@@ -1,0 +1,324 @@
+/*
+ * bhc_hip.h -- C ABI of the MI355X-native BELLHOP ray/beam tracing hot path.
+ *
+ * Two layers are exported:
+ *
+ *  1. Preprocessing (host C++): a raw 2D environment description (the values an
+ *     .env/.bty/.ati/.brc file holds, in user units) is turned into the
+ *     preprocessed problem that the reference hands to its launch layer.
+ *     Replaces the Preprocess() step of the reference's params modules,
+ *     src/api.cpp:391-393 -> src/module/{ssp.hpp:453-545, boundary.hpp:499-544,
+ *     boundarycond.hpp:166-224, atten.cpp:99-194, beaminfo.hpp:221-238,
+ *     rayangles.hpp:157-190, rcvrranges.hpp:74-85, szrz.hpp:138-142, sbp.hpp:143-153}.
+ *
+ *  2. The device boundary (HIP): a preprocessed problem plus caller-owned output
+ *     buffers in, accumulated field / arrivals / eigen hits / ray paths out.
+ *     Replaces bhc::mode::RunFieldModesImpl<CFG,O3D,R3D>(bhcParams&, bhcOutputs&)
+ *     (src/mode/fieldimpl.hpp:27-28, CPU src/mode/fieldimpl.cpp.in:55-77,
+ *     CUDA src/mode/fieldimpl.cu.in:51-64) and the ray-mode worker pool
+ *     RunRayMode (src/mode/ray.cpp:167-188).
+ *
+ * All real-valued scalars and arrays cross the ABI as double. In 32-bit mode
+ * (precision == 32) every such value holds a float exactly (preprocessing is
+ * done in float, as the reference's BHC_USE_FLOATS build does), so the
+ * conversion is lossless both ways. Receiver and source depths are float in
+ * both modes, as in the reference (Position::Rz, Position::Sz).
+ *
+ * Every entry point returns an int status: 0 = success, < 0 = failure (message
+ * via bhc_last_error()). Nothing throws across the ABI and nothing resets the
+ * device. Physics errors and warnings are reported through err/warn bitmasks
+ * that use the reference numbering (src/util/errors.hpp:61-93).
+ */
+#ifndef BHC_HIP_H
+#define BHC_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BHC_ABI_VERSION 1
+
+/* ---- error / warning bit numbers (src/util/errors.hpp:61-93) ---------------- */
+#define BHC_ERR_TEMPLATE 0
+#define BHC_ERR_JOBNUM 1
+#define BHC_ERR_RAYINIT 2
+#define BHC_ERR_BOUNDARY_SEG_CONTAINS_NAN 3
+#define BHC_ERR_BOUNDARY_CONDITION_TYPE 4
+#define BHC_ERR_INVALID_SSP_TYPE 5
+#define BHC_ERR_OUTSIDE_ALTIMETRY 6
+#define BHC_ERR_OUTSIDE_BATHYMETRY 7
+#define BHC_ERR_OUTSIDE_SSP 8
+#define BHC_ERR_QUAD_ISEG 9
+#define BHC_ERR_INVALID_IMAGE_INDEX 10
+
+#define BHC_WARN_RAYS_OUTOFMEMORY 0
+#define BHC_WARN_ONERAY_OUTOFMEMORY 1
+#define BHC_WARN_UNBOUNDED_BEAM 2
+#define BHC_WARN_TOO_FEW_BEAMS 3
+#define BHC_WARN_SOURCE_OUTSIDE_BOUNDARIES 4
+#define BHC_WARN_OUTSIDE_REFLCOEF 5
+#define BHC_WARN_OUTSIDE_ALTIMETRY 6
+#define BHC_WARN_OUTSIDE_BATHYMETRY 7
+#define BHC_WARN_STEP_NEGATIVE_H 8
+#define BHC_WARN_CPCHIP_INVALIDXT 16
+#define BHC_WARN_CPCHIP_INVALIDCCOEF 17
+
+/* ---- output record layouts (byte-identical to the reference) ----------------- */
+
+/* Arrival, include/bhc/structs.hpp:376-380 (40 bytes). */
+typedef struct bhc_arrival {
+    int32_t NTopBnc, NBotBnc;
+    float SrcDeclAngle, SrcAzimAngle, RcvrDeclAngle, RcvrAzimAngle, a, Phase;
+    float delay[2]; /* complex delay (re, im) */
+} bhc_arrival;
+
+/* EigenHit, include/bhc/structs.hpp:357-365 (36 bytes). */
+typedef struct bhc_eigen_hit {
+    int32_t is, itheta, ir, iz, isx, isy, isz, ialpha, ibeta;
+} bhc_eigen_hit;
+
+/* rayPt<false> in the 64-bit layout, include/bhc/structs.hpp:397-424 (112 bytes). */
+typedef struct bhc_ray_pt2d {
+    double p[2], q[2];
+    int32_t NumTopBnc, NumBotBnc;
+    double x[2], t[2];
+    double c, Amp, Phase;
+    double tau[2];
+} bhc_ray_pt2d;
+
+/* ---- raw (user-unit) 2D environment: input to preprocessing ------------------ */
+
+/* Halfspace / boundary condition, HSInfo + HSExtra (include/bhc/structs.hpp:112-125).
+ * bc: 'V' vacuum, 'R' rigid, 'A' acousto-elastic halfspace, 'F' reflection table,
+ * 'G' grain size. Speeds in m/s, attenuations in the SSP's AttenUnit (halfspace)
+ * or dB/wavelength ('L' provinces). */
+typedef struct bhc_halfspace_in {
+    char bc;
+    char pad[7];
+    double depth;           /* boundary depth (m): Top = first SSP depth, Bot = last */
+    double alphaR, betaR, rho, alphaI, betaI;
+    double zTemp;           /* depth used for the crci call ('A') */
+    double Mz;              /* grain size ('G') */
+} bhc_halfspace_in;
+
+/* Altimetry / bathymetry (BdryInfoTopBot<false>), user units: range in km, depth in m.
+ * npts == 0 selects the flat default boundary at halfspace depth. The reference
+ * extends a file boundary by one point at each end (src/module/boundary.hpp:203-226);
+ * the caller passes only the file's points. */
+typedef struct bhc_bdry_in {
+    char type[2];           /* [0]: 'L' linear / 'C' curvilinear; [1]: 'S' short / 'L' long */
+    char pad[6];
+    int32_t npts;
+    int32_t pad2;
+    const double *r_km;     /* [npts] */
+    const double *z;        /* [npts] */
+    /* per-point geoacoustics when type[1] == 'L' (each [npts], may be NULL otherwise) */
+    const double *alphaR, *betaR, *rho, *alphaI, *betaI;
+} bhc_bdry_in;
+
+/* Reflection coefficient table (ReflectionInfoTopBot): theta (deg), |R|, phase (deg). */
+typedef struct bhc_refl_in {
+    int32_t npts;
+    int32_t pad;
+    const double *theta, *r, *phi;
+} bhc_refl_in;
+
+typedef struct bhc_env2d {
+    int32_t precision;      /* 64 or 32 */
+    int32_t pad0;
+    char title[80];
+    double freq0;           /* Hz */
+    /* sound speed profile (SSPStructure) */
+    char ssp_type;          /* 'N','C','S','P','Q','A' */
+    char atten_unit[2];     /* e.g. "W ", "WT", "F " */
+    char pad1[5];
+    int32_t ssp_npts;
+    int32_t ssp_nr;         /* 'Q' only: number of profile ranges */
+    const double *ssp_z, *ssp_alphaR, *ssp_alphaI, *ssp_rho, *ssp_betaR, *ssp_betaI;
+    const double *ssp_r_km;  /* 'Q' only: [ssp_nr] profile ranges */
+    const double *ssp_cmat;  /* 'Q' only: [ssp_npts][ssp_nr], row = depth */
+    /* Thorp / Francois-Garrison volume attenuation parameters (AttenInfo) */
+    double fg_T, fg_S, fg_pH, fg_z_bar;
+    /* boundaries */
+    bhc_halfspace_in top, bot;
+    bhc_bdry_in ati, bty;
+    bhc_refl_in trc, brc;   /* top / bottom reflection tables ('F') */
+    /* source / receivers (Position) */
+    int32_t NSz, NRz, NRr, pad2;
+    const float *Sz;        /* [NSz] m */
+    const float *Rz;        /* [NRz] m */
+    const double *Rr_km;    /* [NRr] km */
+    /* launch angles (AngleInfo alpha), degrees, sorted increasing */
+    int32_t Nalpha, iSingle_alpha;
+    const double *alpha_deg;
+    /* beam (BeamStructure<false>) */
+    char RunType[7];        /* [0] C/S/I/A/a/E/R/V/v, [1] beam type G/g/B/b/^, [3] R/X, [4] R/I */
+    char BeamType[3];       /* Beam->Type[1..3] (width, curvature, shift); "   " if absent */
+    char pad3[6];
+    double deltas;          /* m, 0 = automatic */
+    double box_r_km, box_z; /* beam box */
+    double epsMultiplier, rLoop;
+    int32_t iBeamWindow, Nimage;
+    /* source beam pattern (SBPInfo), angle (deg) / level (dB) pairs; 0 = omni */
+    int32_t NSBPPts, pad4;
+    const double *sbp;      /* [2*NSBPPts] */
+} bhc_env2d;
+
+/* ---- preprocessed 2D problem: what the launch layer consumes ----------------- */
+
+/* One boundary node (BdryPtFull<false> minus the parse-only fields). */
+typedef struct bhc_bdry_node2d {
+    double x[2], t[2], n[2], Noden[2], Nodet[2];
+    double Len, kappa;
+    double cP[2], cS[2], rho;   /* node geoacoustics ('L' boundaries) */
+} bhc_bdry_node2d;
+
+typedef struct bhc_hs2d {
+    char bc;
+    char pad[7];
+    double cP[2], cS[2];
+    double rho, depth;
+} bhc_hs2d;
+
+typedef struct bhc_problem2d {
+    int32_t precision;      /* 64 or 32 */
+    char RunType[7];
+    char BeamType[4];       /* Beam->Type[0..3] */
+    char ssp_type;
+    int32_t pad0;
+    double freq0;
+    /* SSP */
+    int32_t ssp_npts, ssp_nr;
+    const double *ssp_z;     /* [npts] */
+    const double *ssp_c;     /* [npts][2] complex */
+    const double *ssp_cz;    /* [npts][2] */
+    const double *ssp_n2;    /* [npts][2]  ('N') */
+    const double *ssp_n2z;   /* [npts][2]  ('N') */
+    const double *ssp_coef;  /* [4][npts][2]: cSpline ('S') or cCoef ('P') */
+    const double *ssp_rho;   /* [npts] */
+    const double *ssp_seg_r; /* [nr] m ('Q') */
+    const double *ssp_cmat;  /* [npts][nr] ('Q') */
+    const double *ssp_czmat; /* [npts-1][nr] ('Q') */
+    /* boundaries */
+    bhc_hs2d top_hs, bot_hs;
+    char ati_type[2], bty_type[2];
+    int32_t ati_npts, bty_npts;
+    const bhc_bdry_node2d *ati, *bty;
+    int32_t trc_npts, brc_npts;
+    const double *trc, *brc;    /* [npts][3] theta (rad->deg as reference: degrees), r, phi (rad) */
+    /* positions */
+    int32_t NSz, NRz, NRr, NRz_per_range;
+    const float *Sz, *Rz;
+    const double *Rr;           /* m */
+    double Delta_r;
+    /* angles */
+    int32_t Nalpha, iSingle_alpha;
+    const double *alpha;        /* rad */
+    const double *alpha_cos;    /* cos(alpha) from host libm */
+    const double *alpha_sin;    /* sin(alpha) from host libm */
+    double Dalpha;
+    /* beam */
+    double deltas, box_r, box_z, epsMultiplier, rLoop;
+    int32_t iBeamWindow, Nimage;
+    int32_t NSBPPts, pad1;
+    const double *sbp;          /* [NSBPPts][2] degrees, linear level */
+} bhc_problem2d;
+
+/* ---- outputs ----------------------------------------------------------------- */
+
+typedef struct bhc_field_out {
+    /* TL (RunType C/S/I): complex field, caller-zeroed, layout
+     * [NSz][Ntheta=1][NRz_per_range][NRr] of float pairs (uAllSources, pre-ScalePressure). */
+    float *field;
+    /* Arrivals (A/a/V/v): [NSz*NRz_per_range*NRr][MaxNArr] and counts. */
+    bhc_arrival *arr;
+    int32_t *narr;
+    int32_t max_narr;
+    int32_t allow_merging;  /* 1 reproduces the single-threaded merge order deterministically */
+    /* Eigen hits (E/V/v) */
+    bhc_eigen_hit *hits;
+    int32_t *neigen;
+    int32_t eigen_memsize;
+    int32_t pad;
+    /* Optional per-ray statistics, [number of rays] each, may be NULL:
+     * steps taken, top/bottom bounces, final (r, z). */
+    int32_t *ray_nsteps, *ray_ntop, *ray_nbot;
+    double *ray_xend;       /* [rays][2] */
+    /* Error reporting (ErrState) */
+    uint32_t err, warn, err_count, warn_count;
+    /* Work counters (for the roofline): total steps, contributions */
+    int64_t total_steps, total_contrib;
+} bhc_field_out;
+
+typedef struct bhc_ray_out {
+    bhc_ray_pt2d *points;   /* [NRays][MaxPointsPerRay] */
+    int32_t *nsteps;        /* [NRays] */
+    int32_t max_points_per_ray;
+    int32_t pad;
+    uint32_t err, warn, err_count, warn_count;
+} bhc_ray_out;
+
+/* ---- preprocessing (host C++) ----------------------------------------------- */
+
+typedef struct bhc_problem_owner bhc_problem_owner;
+
+/* Preprocess a raw environment. On success *out owns all arrays; the view is
+ * valid until bhc_problem_free(). */
+int bhc_preprocess2d(const bhc_env2d *env, bhc_problem_owner **out);
+const bhc_problem2d *bhc_problem_view(const bhc_problem_owner *p);
+void bhc_problem_free(bhc_problem_owner *p);
+
+/* ---- device context ---------------------------------------------------------- */
+
+typedef struct bhc_ctx bhc_ctx;
+
+int bhc_ctx_create(int device, bhc_ctx **out);
+void bhc_ctx_destroy(bhc_ctx *ctx);
+const char *bhc_last_error(void);
+int bhc_device_info(int device, char *name, int name_len, int *num_cus, int *lds_bytes);
+
+/* One-shot field run (TL / arrivals / eigen): upload, launch, wait, copy back.
+ * The drop-in for RunFieldModesImpl. */
+int bhc_field_run2d(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_field_out *out);
+
+/* Ray run type: per-ray trajectories, replaces RunRayMode. */
+int bhc_ray_run2d(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_ray_out *out);
+/* Ray mode over an explicit ray list, for the eigenray re-trace
+ * (EigenModePostWorker, src/mode/eigen.cpp:24-48): launch angle and source
+ * index per ray and an optional step limit per ray (EigenHit::is). */
+int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_ray_out *out, const int32_t *ialpha,
+                       const int32_t *isz, const int32_t *nsteps_in, int32_t nrays);
+
+/* Resident-problem interface (inputs stay in HBM across launches). `spec`
+ * gives the output shapes: max_narr / allow_merging (arrivals), eigen_memsize
+ * (eigen hits), and whether per-ray statistics are kept (ray_nsteps != NULL);
+ * its pointers are not retained. */
+typedef struct bhc_dev_problem bhc_dev_problem;
+int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, const bhc_field_out *spec,
+                         bhc_dev_problem **out);
+int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp);   /* zero field + trace, async */
+int bhc_field_download(bhc_ctx *ctx, bhc_dev_problem *dp, bhc_field_out *out);
+void bhc_problem_release(bhc_dev_problem *dp);
+int bhc_ctx_synchronize(bhc_ctx *ctx);
+/* Duration (ms) of the tracer kernel of the last launch on this context,
+ * from HIP events recorded on the context's stream around it. */
+int bhc_last_kernel_ms(bhc_ctx *ctx, double *ms);
+
+/* Launch tuning (rays per workgroup, workgroups per CU); 0 = autotuned default. */
+int bhc_ctx_set_launch(bhc_ctx *ctx, int threads_per_block, int blocks_per_cu);
+
+/* ---- post-processing and file layouts (host) ----------------------------------- */
+
+/* ScalePressure + PostProcessTL for 2D (src/influence.hpp:1556-1627, src/mode/tl.cpp:106-180). */
+int bhc_scale_pressure2d(const bhc_problem2d *prob, float *field);
+/* .shd file (WriteHeader + WriteOutTL, src/mode/tl.cpp:38-101, 211-248). */
+int bhc_write_shd2d(const char *path, const bhc_problem2d *prob, const char *title, const float *field);
+const char *bhc_host_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BHC_HIP_H */

@@ -1,0 +1,139 @@
+"""ORACLE loader -- test infrastructure only (tests/, smoke(), bench.py cpu_baseline).
+
+ctypes binding of liboracle.so, the plain-C restatement of the reference's 2D
+hot path in oracle/bellhop2d.c. Never imported by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from bellhopcuda_amd import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        for sfx in ("d", "f"):
+            f = getattr(L, "oracle_field_run2d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(_abi.Env2D_C), C.POINTER(_abi.FieldOut), C.c_int, C.c_int, C.c_int]
+            f = getattr(L, "oracle_ray_run2d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(_abi.Env2D_C), C.POINTER(_abi.RayOut), _abi.c_int32_p,
+                          _abi.c_int32_p, _abi.c_int32_p, C.c_int]
+            f = getattr(L, "oracle_scale_pressure2d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(_abi.Env2D_C), _abi.c_float_p]
+            f = getattr(L, "oracle_preprocess_dump2d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(_abi.Env2D_C), _abi.c_double_p, C.c_int64, C.POINTER(C.c_int64)]
+        _LIB = L
+    return _LIB
+
+
+def _sfx(env):
+    return "d" if env.precision == 64 else "f"
+
+
+class FieldResult:
+    pass
+
+
+def field_run(env, nthreads=0, job_begin=0, job_end=-1, stats=True, max_narr=0, allow_merging=1,
+              eigen_memsize=0):
+    """Run the oracle on an Env2D. Returns a FieldResult with .field (complex64,
+    shape [NSz, NRz_per_range, NRr], pre-ScalePressure) and per-ray stats."""
+    ec, keep = env.to_c()
+    nsz, nrz, nrr = env.field_shape
+    nj = env.nrays if job_end < 0 else job_end - job_begin
+    res = FieldResult()
+    res.field = np.zeros((nsz, nrz, nrr), np.complex64)
+    out = _abi.FieldOut()
+    out.field = res.field.ctypes.data_as(_abi.c_float_p)
+    rt = env.run_type[0]
+    if rt in "AaVv":
+        res.max_narr = max_narr
+        res.arr = (_abi.Arrival * (nsz * nrz * nrr * max_narr))()
+        res.narr = np.zeros(nsz * nrz * nrr, np.int32)
+        out.arr = C.cast(res.arr, C.POINTER(_abi.Arrival))
+        out.narr = res.narr.ctypes.data_as(_abi.c_int32_p)
+        out.max_narr = max_narr
+        out.allow_merging = allow_merging
+    if rt in "EVv":
+        res.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
+        res.neigen = np.zeros(1, np.int32)
+        out.hits = C.cast(res.hits, C.POINTER(_abi.EigenHit))
+        out.neigen = res.neigen.ctypes.data_as(_abi.c_int32_p)
+        out.eigen_memsize = eigen_memsize
+    if stats:
+        res.nsteps = np.zeros(nj, np.int32)
+        res.ntop = np.zeros(nj, np.int32)
+        res.nbot = np.zeros(nj, np.int32)
+        res.xend = np.zeros((nj, 2), np.float64)
+        out.ray_nsteps = res.nsteps.ctypes.data_as(_abi.c_int32_p)
+        out.ray_ntop = res.ntop.ctypes.data_as(_abi.c_int32_p)
+        out.ray_nbot = res.nbot.ctypes.data_as(_abi.c_int32_p)
+        out.ray_xend = res.xend.ctypes.data_as(_abi.c_double_p)
+    f = getattr(lib(), "oracle_field_run2d_" + _sfx(env))
+    rc = f(C.byref(ec), C.byref(out), nthreads, job_begin, job_end)
+    if rc != 0:
+        raise RuntimeError("oracle_field_run2d failed")
+    res.err, res.warn = out.err, out.warn
+    res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
+    del keep
+    return res
+
+
+def scale_pressure(env, field):
+    ec, keep = env.to_c()
+    f = np.ascontiguousarray(field, np.complex64).copy()
+    getattr(lib(), "oracle_scale_pressure2d_" + _sfx(env))(C.byref(ec), f.ctypes.data_as(_abi.c_float_p))
+    return f
+
+
+def ray_run(env, max_points=20000, ialpha=None, isz=None, nsteps_in=None):
+    ec, keep = env.to_c()
+    n = env.nrays if ialpha is None else len(ialpha)
+    pts = (_abi.RayPt2D * (n * max_points))()
+    nst = np.zeros(n, np.int32)
+    out = _abi.RayOut()
+    out.points = C.cast(pts, C.POINTER(_abi.RayPt2D))
+    out.nsteps = nst.ctypes.data_as(_abi.c_int32_p)
+    out.max_points_per_ray = max_points
+    ia = isz_a = ns = None
+    if ialpha is not None:
+        ia = np.ascontiguousarray(ialpha, np.int32)
+        isz_a = np.ascontiguousarray(isz if isz is not None else np.zeros(n), np.int32)
+    if nsteps_in is not None:
+        ns = np.ascontiguousarray(nsteps_in, np.int32)
+    f = getattr(lib(), "oracle_ray_run2d_" + _sfx(env))
+    rc = f(C.byref(ec), C.byref(out),
+           ia.ctypes.data_as(_abi.c_int32_p) if ia is not None else None,
+           isz_a.ctypes.data_as(_abi.c_int32_p) if isz_a is not None else None,
+           ns.ctypes.data_as(_abi.c_int32_p) if ns is not None else None, n)
+    if rc != 0:
+        raise RuntimeError("oracle_ray_run2d failed")
+    arr = np.ctypeslib.as_array(pts).reshape(n, max_points)
+    return arr, nst, (out.err, out.warn)
+
+
+def preprocess_dump(env, cap=1 << 22):
+    ec, keep = env.to_c()
+    buf = np.zeros(cap, np.float64)
+    used = C.c_int64(0)
+    getattr(lib(), "oracle_preprocess_dump2d_" + _sfx(env))(C.byref(ec), buf.ctypes.data_as(_abi.c_double_p),
+                                                           cap, C.byref(used))
+    return buf[:used.value].copy()

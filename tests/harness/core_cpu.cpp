@@ -1,0 +1,103 @@
+// Test harness only: runs the product's per-ray core (core2d.h + pack2d.h) on
+// the CPU, one ray after another, with the wavefront scatter replaced by a
+// plain loop over each segment's work items. It exercises exactly the
+// functions the HIP kernel calls, so a disagreement with the oracle found here
+// is a bug in the shared core, not in the kernel's launch plumbing.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bhc_hip.h"
+#include "../../bellhopcuda_amd/csrc/core2d.h"
+#include "../../bellhopcuda_amd/csrc/pack2d.h"
+
+using namespace bhc2d;
+
+namespace {
+struct HostSink {
+    float *field;
+    long long nc = 0;
+    void add_field(size_t idx, float re, float im)
+    {
+        field[2 * idx] += re;
+        field[2 * idx + 1] += im;
+    }
+    template<typename real>
+    void add_arrival(size_t, real, real, cpx<real>, real, real, int, int, int, int, int)
+    {
+    }
+    void add_eigen(int, int, int, int, int) {}
+    void count() { ++nc; }
+};
+
+template<typename real> int run(const bhc_problem2d *prob, bhc_field_out *out)
+{
+    std::vector<real> blob;
+    Prob2D<real> P;
+    std::string err;
+    if(pack<real>(prob, blob, P, err)) {
+        fprintf(stderr, "%s\n", err.c_str());
+        return -1;
+    }
+    P.base = blob.data();
+    P.Rz = prob->Rz;
+    P.Sz = prob->Sz;
+    const real *B = blob.data(), *G = blob.data();
+    HostSink sink{out->field};
+    Err e = {0, 0, 0, 0};
+    int njobs = (prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * prob->NSz;
+    long long steps = 0;
+    for(int job = 0; job < njobs; ++job) {
+        int isz, ia;
+        JobIndices(job, P, isz, ia);
+        RayPt<real> p0, p1, p2;
+        Trace<real> S;
+        Infl<real> I;
+        if(!RayInit(isz, ia, p0, S, I, P, B, G, e)) {
+            if(out->ray_nsteps) {
+                out->ray_nsteps[job] = 0;
+                out->ray_ntop[job] = out->ray_nbot[job] = 0;
+                out->ray_xend[2 * job] = out->ray_xend[2 * job + 1] = 0;
+            }
+            continue;
+        }
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            bool two = RayUpdate(p0, p1, p2, S, P, B, G, e);
+            SegDesc<real> d;
+            prelude(d, p0, p1, I, is, P, G, isz, ia);
+            for(int k = 0; k < d.nitems; ++k) item<real, RT_TL>(d, k, P, G, prob->Rz, sink);
+            ++is;
+            if(two) {
+                prelude(d, p1, p2, I, is, P, G, isz, ia);
+                for(int k = 0; k < d.nitems; ++k) item<real, RT_TL>(d, k, P, G, prob->Rz, sink);
+                ++is;
+            }
+            p0 = two ? p2 : p1;
+            if(RayTerminate(p0, is, S, 100000, P, e)) break;
+        }
+        steps += is;
+        if(out->ray_nsteps) {
+            out->ray_nsteps[job] = is;
+            out->ray_ntop[job] = p0.NumTopBnc;
+            out->ray_nbot[job] = p0.NumBotBnc;
+            out->ray_xend[2 * job] = (double)p0.x.x;
+            out->ray_xend[2 * job + 1] = (double)p0.x.y;
+        }
+    }
+    out->total_steps = steps;
+    out->total_contrib = sink.nc;
+    out->err = e.err;
+    out->warn = e.warn;
+    out->err_count = e.nerr;
+    out->warn_count = e.nwarn;
+    return 0;
+}
+} // namespace
+
+extern "C" int core_field_run2d(const bhc_problem2d *prob, bhc_field_out *out)
+{
+    return prob->precision == 64 ? run<double>(prob, out) : run<float>(prob, out);
+}
