@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libbhc_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BHC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["trace2d.hip"]
+HIP_SOURCES = ["trace2d.hip", "opcount2d.cpp"]
 CXX_SOURCES = ["host2d.cpp", "writers2d.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
@@ -37,7 +37,7 @@ def build(verbose=False):
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        _run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(objdir, src + ".o")

@@ -151,10 +151,12 @@ template<typename real> __host__ __device__ inline real dot(v2<real> a, v2<real>
 // ---------------------------------------------------------------------------
 enum : int { RT_TL = 0, RT_ARR = 1, RT_EIG = 2, RT_RAY = 3 };
 
-// Boundary node record in the blob (BdryPtFull<false> subset), NODE reals each.
+// Boundary node record in the blob (BdryPtFull<false> subset). Straight-segment
+// boundaries with one set of geoacoustics use the first NODE_SHORT reals of a
+// node; curvilinear ('C') or per-node geoacoustics ('L') use all NODE_FULL.
 enum : int {
-    BN_XX = 0, BN_XY, BN_TX, BN_TY, BN_NX, BN_NY, BN_NDNX, BN_NDNY, BN_NDTX, BN_NDTY,
-    BN_LEN, BN_KAPPA, BN_CPR, BN_CPI, BN_CSR, BN_CSI, BN_RHO, BN_PAD, NODE
+    BN_XX = 0, BN_XY, BN_TX, BN_TY, BN_NX, BN_NY, BN_LEN, BN_KAPPA, NODE_SHORT,
+    BN_NDNX = NODE_SHORT, BN_NDNY, BN_NDTX, BN_NDTY, BN_CPR, BN_CPI, BN_CSR, BN_CSI, BN_RHO, BN_PAD, NODE_FULL
 };
 
 template<typename real> struct HSd {
@@ -176,7 +178,7 @@ template<typename real> struct Prob2D {
     char ati_long, bty_long; // 'L' long format (per-node geoacoustics)
     int32_t npts, Nr;
     int32_t o_z, o_rho, o_c, o_cz, o_n2, o_n2z, o_coef, o_segr, o_cmat, o_czmat;
-    int32_t ati_n, bty_n, o_ati, o_bty;
+    int32_t ati_n, bty_n, o_ati, o_bty, node;  // node: reals per boundary node
     // rest of blob (global)
     int32_t o_Rr, o_trc, o_brc, ntrc, nbrc, o_sbp, nsbp;
     int32_t o_alpha, o_acos, o_asin;
@@ -186,12 +188,10 @@ template<typename real> struct Prob2D {
     real Delta_r, Dalpha, freq0, deltas, box_r, box_z;
     HSd<real> top, bot;
     int32_t isGaussian, isRayCen;
-};
-
-// Accessor over a blob pointer that may be LDS or global.
-template<typename real> struct View {
-    const real *b;
-    __host__ __device__ real operator[](int i) const { return b[i]; }
+    int32_t rz_sorted;     // receiver depths strictly increasing (enables depth windows)
+    float *field;          // this problem's complex field (TL runs)
+    int32_t njobs;         // rays of this problem
+    int32_t pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -262,22 +262,25 @@ __host__ __device__ inline void UpdateSSPSegment(real x, real t, const real *arr
 
 template<typename real> __host__ __device__ inline cpx<real> ldc(const real *p, int i) { return C(p[2 * i], p[2 * i + 1]); }
 
-template<typename real>
+// ST: SSP type fixed at compile time ('C','S','P','N','Q','A'), or 0 to read it
+// from the problem at run time.
+template<typename real, char ST = 0>
 __host__ __device__ inline void EvaluateSSP(v2<real> x, v2<real> t, SSPOut<real> &o, const Prob2D<real> &P,
                                             const real *B, int &iz, int &ir, Err &e)
 {
+    const char sst = ST ? ST : P.ssp_type;
     const real *z = B + P.o_z;
     o.crr = o.crz = o.czz = RL(0.0);
-    if(P.ssp_type != 'A' && P.ssp_type != 'Q') {
+    if(sst != 'A' && sst != 'Q') {
         UpdateSSPSegment(x.y, t.y, z, P.npts, iz);
         real w = (x.y - z[iz]) / (z[iz + 1] - z[iz]); // LinInterpDensity ssp.hpp:53-59
         const real *rho = B + P.o_rho;
         o.rho = (RL(1.0) - w) * rho[iz] + w * rho[iz + 1];
-        if(P.ssp_type == 'C') { // cLinear ssp.hpp:81-89
+        if(sst == 'C') { // cLinear ssp.hpp:81-89
             cpx<real> c = ldc(B + P.o_c, iz), cz = ldc(B + P.o_cz, iz);
             o.c = c + rmul(x.y - z[iz], cz);
             o.gradc = V(RL(0.0), cz.re);
-        } else if(P.ssp_type == 'S') { // cCubic ssp.hpp:133-152, SplineALL curves.hpp:190-200
+        } else if(sst == 'S') { // cCubic ssp.hpp:133-152, SplineALL curves.hpp:190-200
             const real *cf = B + P.o_coef;
             int n2 = 2 * P.npts;
             cpx<real> c1 = ldc(cf, iz), c2 = ldc(cf + n2, iz), c3 = ldc(cf + 2 * n2, iz), c4 = ldc(cf + 3 * n2, iz);
@@ -289,7 +292,7 @@ __host__ __device__ inline void EvaluateSSP(v2<real> x, v2<real> t, SSPOut<real>
             o.c = f;
             o.gradc = V(RL(0.0), fx.re);
             o.czz = fxx.re;
-        } else if(P.ssp_type == 'P') { // cPCHIP ssp.hpp:95-128
+        } else if(sst == 'P') { // cPCHIP ssp.hpp:95-128
             const real *cf = B + P.o_coef;
             int n2 = 2 * P.npts;
             cpx<real> c0 = ldc(cf, iz), c1 = ldc(cf + n2, iz), c2 = ldc(cf + 2 * n2, iz), c3 = ldc(cf + 3 * n2, iz);
@@ -300,14 +303,14 @@ __host__ __device__ inline void EvaluateSSP(v2<real> x, v2<real> t, SSPOut<real>
             o.c = c0 + rmul(xt, c1 + rmul(xt, c2 + rmul(xt, c3)));
             o.gradc = V(RL(0.0), (c1 + rmul(xt, rmul(RL(2.0), c2) + rmul(xt, rmul(RL(3.0), c3)))).re);
             o.czz = (rmul(RL(2.0), c2) + rmul(xt, rmul(RL(6.0), c3))).re;
-        } else if(P.ssp_type == 'N') { // n2Linear ssp.hpp:64-76
+        } else if(sst == 'N') { // n2Linear ssp.hpp:64-76
             cpx<real> a = ldc(B + P.o_n2, iz), b = ldc(B + P.o_n2, iz + 1);
             o.c = realdiv(RL(1.0), csqrt_(rmul(RL(1.0) - w, a) + rmul(w, b)));
             real c = o.c.re;
             o.gradc = V(RL(0.0), RL(-0.5) * (c * c * c) * ldc(B + P.o_n2z, iz).re);
             o.czz = RL(3.0) * o.gradc.y * o.gradc.y / c;
         }
-    } else if(P.ssp_type == 'Q') { // Quad ssp.hpp:157-214
+    } else if(sst == 'Q') { // Quad ssp.hpp:157-214
         const real *segr = B + P.o_segr;
         int Nr = P.Nr;
         if(x.x < segr[0] || x.x > segr[Nr - 1]) e.error(8);
@@ -358,7 +361,7 @@ __host__ __device__ inline void EvaluateSSP(v2<real> x, v2<real> t, SSPOut<real>
 // ---------------------------------------------------------------------------
 template<typename real>
 __host__ __device__ inline void GetBdrySeg(v2<real> x, v2<real> t, BdryState<real> &bds, const real *bd, int n,
-                                           bool longfmt, HSd<real> &hs, bool isTop, bool isInit, Err &e)
+                                           int NODE, bool longfmt, HSd<real> &hs, bool isTop, bool isInit, Err &e)
 {
     if(!isInit
        && !(x.x < bds.lmin || (x.x == bds.lmin && t.x < FL(0.0)) || x.x > bds.lmax
@@ -539,12 +542,12 @@ __host__ __device__ inline void StepToBdry(v2<real> x0, v2<real> &x2, v2<real> u
     }
 }
 
-template<typename real>
+template<typename real, char ST = 0>
 __host__ __device__ inline void Step(const RayPt<real> &ray0, RayPt<real> &ray2, Trace<real> &S, const Prob2D<real> &P,
                                      const real *B, Err &e, bool &topRefl, bool &botRefl)
 { // step.hpp:654-811 (2D)
     SSPOut<real> o0, o1, o2;
-    EvaluateSSP(ray0.x, ray0.t, o0, P, B, S.iz, S.ir, e);
+    EvaluateSSP<real, ST>(ray0.x, ray0.t, o0, P, B, S.iz, S.ir, e);
     real c0 = o0.c.re;
     real cnn0 = o0.crr * (ray0.t.y * ray0.t.y) - FL(2.0) * o0.crz * ray0.t.x * ray0.t.y + o0.czz * (ray0.t.x * ray0.t.x);
     v2<real> pq0p = sc(-cnn0, ray0.q), pq0q = sc(c0, ray0.p);
@@ -560,7 +563,7 @@ __host__ __device__ inline void Step(const RayPt<real> &ray0, RayPt<real> &ray2,
     ray1.p = ray0.p + sc(halfh, pq0p);
     ray1.q = ray0.q + sc(halfh, pq0q);
 
-    EvaluateSSP(ray1.x, ray1.t, o1, P, B, S.iz, S.ir, e);
+    EvaluateSSP<real, ST>(ray1.x, ray1.t, o1, P, B, S.iz, S.ir, e);
     real c1 = o1.c.re;
     real cnn1 = o1.crr * (ray1.t.y * ray1.t.y) - FL(2.0) * o1.crz * ray1.t.x * ray1.t.y + o1.czz * (ray1.t.x * ray1.t.x);
     v2<real> pq1p = sc(-cnn1, ray1.q), pq1q = sc(c1, ray1.p);
@@ -586,7 +589,7 @@ __host__ __device__ inline void Step(const RayPt<real> &ray0, RayPt<real> &ray2,
     ray2.NumTopBnc = ray0.NumTopBnc;
     ray2.NumBotBnc = ray0.NumBotBnc;
 
-    EvaluateSSP(ray2.x, ray2.t, o2, P, B, S.iz, S.ir, e);
+    EvaluateSSP<real, ST>(ray2.x, ray2.t, o2, P, B, S.iz, S.ir, e);
     ray2.c = o2.c.re;
     if(S.iz != iz0 || S.ir != ir0) { // CurvatureCorrection 2D, step.hpp:626-648
         v2<real> gj = o2.gradc - o0.gradc;
@@ -602,7 +605,7 @@ __host__ __device__ inline void Step(const RayPt<real> &ray0, RayPt<real> &ray2,
 // ---------------------------------------------------------------------------
 // Reflection (reference src/reflect.hpp, 2D)
 // ---------------------------------------------------------------------------
-template<typename real>
+template<typename real, char ST = 0>
 __host__ __device__ inline void Reflect(const RayPt<real> &oldp, RayPt<real> &newp, const HSd<real> &hs, bool isTop,
                                         v2<real> tB, v2<real> nB, real kappa, const Prob2D<real> &P, const real *B,
                                         const real *G, int &iz, int &ir, Err &e)
@@ -614,7 +617,7 @@ __host__ __device__ inline void Reflect(const RayPt<real> &oldp, RayPt<real> &ne
     newp.x = oldp.x;
     newp.t = oldp.t - sc(FL(2.0) * Th, nB);
     SSPOut<real> o;
-    EvaluateSSP(oldp.x, oldp.t, o, P, B, iz, ir, e);
+    EvaluateSSP<real, ST>(oldp.x, oldp.t, o, P, B, iz, ir, e);
     real c = o.c.re;
     newp.c = c;
     newp.tau = oldp.tau;
@@ -744,7 +747,7 @@ template<typename real> __host__ __device__ inline int BinarySearchGT(const real
 }
 
 // Returns false when the ray is not traced (source outside the boundaries).
-template<typename real>
+template<typename real, char ST = 0>
 __host__ __device__ inline bool RayInit(int isz, int ialpha, RayPt<real> &p0, Trace<real> &S, Infl<real> &I,
                                         const Prob2D<real> &P, const real *B, const real *G, Err &e)
 { // trace.hpp:76-266 (2D) + Init_Influence influence.hpp:525-644
@@ -757,7 +760,7 @@ __host__ __device__ inline bool RayInit(int isz, int ialpha, RayPt<real> &p0, Tr
     S.xs_y = xs.y;
     v2<real> tinit = V(G[P.o_acos + ialpha], G[P.o_asin + ialpha]);
     SSPOut<real> o;
-    EvaluateSSP(xs, tinit, o, P, B, S.iz, S.ir, e);
+    EvaluateSSP<real, ST>(xs, tinit, o, P, B, S.iz, S.ir, e);
     real c = o.c.re;
     {
         real DalphaOpt = std::sqrt(c / (FL(6.0) * P.freq0 * G[P.o_Rr + P.NRr - 1]));
@@ -786,8 +789,8 @@ __host__ __device__ inline bool RayInit(int isz, int ialpha, RayPt<real> &p0, Tr
     S.htop = P.top;
     S.hbot = P.bot;
     S.top.Iseg = S.bot.Iseg = 0;
-    GetBdrySeg(xs, p0.t, S.top, B + P.o_ati, P.ati_n, P.ati_long == 'L', S.htop, true, true, e);
-    GetBdrySeg(xs, p0.t, S.bot, B + P.o_bty, P.bty_n, P.bty_long == 'L', S.hbot, false, true, e);
+    GetBdrySeg(xs, p0.t, S.top, B + P.o_ati, P.ati_n, P.node, P.ati_long == 'L', S.htop, true, true, e);
+    GetBdrySeg(xs, p0.t, S.bot, B + P.o_bty, P.bty_n, P.node, P.bty_long == 'L', S.hbot, false, true, e);
     Distances(xs, S, S.DistBegTop, S.DistBegBot);
     if(S.DistBegTop <= FL(0.0) || S.DistBegBot <= FL(0.0)) {
         e.warning(4);
@@ -816,20 +819,20 @@ __host__ __device__ inline bool RayInit(int isz, int ialpha, RayPt<real> &p0, Tr
     return true;
 }
 
-template<typename real>
+template<typename real, char ST = 0>
 __host__ __device__ inline bool RayUpdate(const RayPt<real> &p0, RayPt<real> &p1, RayPt<real> &p2, Trace<real> &S,
                                           const Prob2D<real> &P, const real *B, const real *G, Err &e)
 { // trace.hpp:274-381 (2D)
     bool topRefl = false, botRefl = false;
-    Step(p0, p1, S, P, B, e, topRefl, botRefl);
-    GetBdrySeg(p1.x, p1.t, S.top, B + P.o_ati, P.ati_n, P.ati_long == 'L', S.htop, true, false, e);
-    GetBdrySeg(p1.x, p1.t, S.bot, B + P.o_bty, P.bty_n, P.bty_long == 'L', S.hbot, false, false, e);
+    Step<real, ST>(p0, p1, S, P, B, e, topRefl, botRefl);
+    GetBdrySeg(p1.x, p1.t, S.top, B + P.o_ati, P.ati_n, P.node, P.ati_long == 'L', S.htop, true, false, e);
+    GetBdrySeg(p1.x, p1.t, S.bot, B + P.o_bty, P.bty_n, P.node, P.bty_long == 'L', S.hbot, false, false, e);
     Distances(p1.x, S, S.DistEndTop, S.DistEndBot);
     if(topRefl || botRefl) {
         const real *bd = B + (topRefl ? P.o_ati : P.o_bty);
         const BdryState<real> &bs = topRefl ? S.top : S.bot;
-        const real *b0 = bd + bs.Iseg * NODE;
-        const real *b1 = b0 + NODE;
+        const real *b0 = bd + bs.Iseg * P.node;
+        const real *b1 = b0 + P.node;
         v2<real> nI, tI;
         if((topRefl ? P.ati_curv : P.bty_curv) == 'C') {
             real sss = dot(p1.x - bs.x, V(b0[BN_TX], b0[BN_TY])) / b0[BN_LEN];
@@ -839,7 +842,7 @@ __host__ __device__ inline bool RayUpdate(const RayPt<real> &p0, RayPt<real> &p1
             nI = V(b0[BN_NX], b0[BN_NY]);
             tI = V(b0[BN_TX], b0[BN_TY]);
         }
-        Reflect(p1, p2, topRefl ? S.htop : S.hbot, topRefl, tI, nI, b0[BN_KAPPA], P, B, G, S.iz, S.ir, e);
+        Reflect<real, ST>(p1, p2, topRefl ? S.htop : S.hbot, topRefl, tI, nI, b0[BN_KAPPA], P, B, G, S.iz, S.ir, e);
         Distances(p2.x, S, S.DistEndTop, S.DistEndBot);
         return true;
     }
@@ -883,6 +886,30 @@ __host__ __device__ inline bool IsAtCaustic(real qOld, real q)
     return (q <= RL(0.0) && qOld > RL(0.0)) || (q >= RL(0.0) && qOld < RL(0.0));
 }
 
+
+// Receiver-depth index window [i0, i1] (inclusive, clamped, possibly empty with
+// i1 < i0) that contains every depth Rz[i] with zlo <= Rz[i] <= zhi, widened by
+// one index on each side; requires strictly increasing Rz.
+template<typename real>
+__host__ __device__ inline void depth_window(const float *Rz, int n, real zlo, real zhi, int &i0, int &i1)
+{
+    int lo = 0, hi = n; // first index with Rz >= zlo
+    while(lo < hi) {
+        int m = (lo + hi) >> 1;
+        if((real)Rz[m] < zlo) lo = m + 1;
+        else hi = m;
+    }
+    i0 = imax(lo - 1, 0);
+    lo = 0;
+    hi = n; // first index with Rz > zhi
+    while(lo < hi) {
+        int m = (lo + hi) >> 1;
+        if((real)Rz[m] <= zhi) lo = m + 1;
+        else hi = m;
+    }
+    i1 = imin(lo, n - 1);
+}
+
 template<typename real> struct SegDesc {
     // geometry (Cartesian: x_ray, rayt, rayn1, rlen, zmin, zmax, lo, hi, ir range;
     //           ray-centred: A = previous normal point / normal, B = this one)
@@ -890,11 +917,14 @@ template<typename real> struct SegDesc {
     real rlen, zmin, zmax, lo, hi;
     // beam quantities of InfluenceGeoCore
     real q0x, dqx, p0c, p1c, p1Amp, phasePt, phase, qOld, p1taure, Ratio1, rcp_q0, RcvrDecl, SrcDecl;
+    real wmax;                  // upper bound of BeamWindow * sigma over the segment
     cpx<real> tau0, dtau;
     int32_t ir0, ir1, dir;      // Cartesian: visited receiver ranges ir0..ir1 (step dir)
+    int32_t iz0, nz;            // depth window: receiver depths iz0 .. iz0+nz-1
     int32_t nitems;             // work items (0 = nothing to scatter)
     int32_t lastValid;          // ray-centred
     int32_t ntop, nbot, isz, ialpha, is;
+    int32_t prob;               // problem index (batched runs)
 };
 
 template<typename real>
@@ -974,7 +1004,15 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     d.ir1 = ir;
     d.dir = (ir >= first) ? 1 : -1;
     int nr = (ir >= first ? ir - first : first - ir) + 1;
-    d.nitems = nr * P.NRz_per_range;
+    d.iz0 = 0;
+    d.nz = P.NRz_per_range;
+    if(P.rz_sorted && P.run4 != 'I' && d.zmin > -RC<real>::MAXV) {
+        int i0, i1;
+        depth_window(P.Rz, P.NRz_per_range, d.zmin, d.zmax, i0, i1);
+        d.iz0 = i0;
+        d.nz = imax(0, i1 - i0 + 1);
+    }
+    d.nitems = nr * d.nz;
     d.xa = p0.x;
     d.ta = rayt;
     d.tb = V(-rayt.y, rayt.x);
@@ -1008,6 +1046,33 @@ __host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<rea
     d.RcvrDecl = (RL(180.0) / RC<real>::PI) * std::atan2(p1.t.y, p1.t.x);
     desc_common(d, p0, p1, I, P, is, isz, ialpha);
     d.nitems = P.NRz_per_range;
+    {
+        // sigma(s) <= max(|q0|, |q1|) * rcp_q0 (q is linear in s), raised to the
+        // AdjustSigma floor; 1e-4 relative slack covers the rounding of s and n1.
+        double q0 = std::fabs((double)p0.q.x), q1 = std::fabs((double)p1.q.x);
+        double qmax = (q0 > q1 ? q0 : q1) + 1e-3 * std::fabs((double)p1.q.x - (double)p0.q.x);
+        double sig = qmax * std::fabs((double)I.rcp_q0);
+        if(P.isGaussian) {
+            double adj = (double)rmin(FL(0.2) * P.freq0 * p1.tau.re, RC<real>::PI * (p0.c / P.freq0));
+            sig = sig > adj ? sig : adj;
+        }
+        d.wmax = (real)((P.isGaussian ? 4.0 : 1.0) * sig * (1.0 + 1e-4));
+    }
+    d.iz0 = 0;
+    d.nz = P.NRz_per_range;
+    if(P.rz_sorted && d.lastValid) {
+        // n at depth z is linear between the normals at A and B: a depth can only
+        // contribute when that range of n meets [-wmax, wmax] (|normal| = 1), i.e.
+        // within wmax (plus slack) of the depths of A and B.
+        double ya = (double)d.xa.y, yb = (double)d.xb.y, W = (double)d.wmax;
+        double slack = 1e-4 * W + 1e-6 * (std::fabs(ya) + std::fabs(yb)) + 1e-3;
+        double zlo = (ya < yb ? ya : yb) - W - slack, zhi = (ya < yb ? yb : ya) + W + slack;
+        int i0, i1;
+        depth_window(P.Rz, P.NRz_per_range, (real)zlo, (real)zhi, i0, i1);
+        d.iz0 = i0;
+        d.nz = imax(0, i1 - i0 + 1);
+    }
+    d.nitems = d.nz;
     I.lastValid = 1;
     I.x = p1.x;
     I.rayn1 = rayn1;
@@ -1018,6 +1083,7 @@ template<typename real, int RT, typename Sink>
 __host__ __device__ inline void contrib(const SegDesc<real> &d, real s, real n1, int ir, int iz, const Prob2D<real> &P,
                                         Sink &sink)
 {
+    sink.mark(); // op-count hook (no-op in the kernels)
     real qFinal = d.q0x + s * d.dqx;
     real sigma, sigma_orig;
     sigma = sigma_orig = std::fabs(qFinal * d.rcp_q0);
@@ -1057,7 +1123,7 @@ __host__ __device__ inline void contrib(const SegDesc<real> &d, real s, real n1,
             fr = (float)v;
             fi = 0.0f;
         }
-        sink.add_field(idx, fr, fi);
+        sink.add_field(P, idx, fr, fi);
     } else if constexpr(RT == RT_ARR) {
         sink.add_arrival(idx, cnst * w, phaseInt, delay, d.SrcDecl, d.RcvrDecl, d.ntop, d.nbot, d.ialpha, d.isz,
                          d.is);
@@ -1080,8 +1146,7 @@ template<typename real, int RT, typename Sink>
 __host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
                                           const float *Rz, Sink &sink)
 {
-    int nz = P.NRz_per_range;
-    int j = k / nz, iz = k - j * nz;
+    int j = k / d.nz, iz = d.iz0 + (k - j * d.nz);
     int ir = d.ir0 + j * d.dir;
     real rr = G[P.o_Rr + ir];
     if(!(rr >= d.lo && rr < d.hi)) return;
@@ -1118,7 +1183,41 @@ __host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, cons
     if(irA == irB) return;
     int notii = (irB <= irA) ? 0 : 1;
     int step = (notii << 1) - 1;
-    for(int ir = irA + notii; ir != irB + notii; ir += step) {
+    // The reference visits ir = irA+notii ... irB+notii-step and rejects most of
+    // them in the beam-window test. Receivers outside a conservative window
+    // (|n| <= wmax, with slack) cannot pass that test and are not visited.
+    int blo = irA + notii, bhi = irB + notii - step;
+    if(blo > bhi) {
+        int t = blo;
+        blo = bhi;
+        bhi = t;
+    }
+    if(bhi - blo >= 4) {
+        double dn = (double)nB - (double)nA;
+        double W = (double)d.wmax + 1e-4 * (std::fabs((double)nA) + std::fabs((double)nB)) + 1e-30;
+        if(dn == 0.0) {
+            if(std::fabs((double)nA) > W) return;
+        } else {
+            double s1 = (-W - (double)nA) / dn, s2 = (W - (double)nA) / dn;
+            double slo = s1 < s2 ? s1 : s2, shi = s1 < s2 ? s2 : s1;
+            double dr = (double)rB - (double)rA;
+            double ra = (double)rA + slo * dr, rb = (double)rA + shi * dr;
+            double rlo = ra < rb ? ra : rb, rhi = ra < rb ? rb : ra;
+            double t0 = (rlo - (double)Rr[0]) / (double)P.Delta_r, t1 = (rhi - (double)Rr[0]) / (double)P.Delta_r;
+            double lo_lim = (double)blo - 2.0, hi_lim = (double)bhi + 2.0;
+            t0 = t0 < lo_lim ? lo_lim : (t0 > hi_lim ? hi_lim : t0);
+            t1 = t1 < lo_lim ? lo_lim : (t1 > hi_lim ? hi_lim : t1);
+            if(!(t0 == t0) || !(t1 == t1)) { // NaN: no pruning
+                t0 = lo_lim;
+                t1 = hi_lim;
+            }
+            int c0 = (int)std::floor(t0) - 1, c1 = (int)std::ceil(t1) + 1;
+            blo = imax(blo, c0);
+            bhi = imin(bhi, c1);
+            if(blo > bhi) return;
+        }
+    }
+    for(int ir = blo; ir <= bhi; ++ir) {
         real s = (Rr[ir] - rA) / (rB - rA);
         real n1 = std::fabs(nA + s * (nB - nA));
         contrib<real, RT>(d, s, n1, ir, iz, P, sink);
@@ -1137,7 +1236,7 @@ template<typename real, int RT, typename Sink>
 __host__ __device__ inline void item(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
                                      const float *Rz, Sink &sink)
 {
-    if(P.isRayCen) item_raycen<real, RT>(d, k, P, G, Rz, sink);
+    if(P.isRayCen) item_raycen<real, RT>(d, d.iz0 + k, P, G, Rz, sink);
     else item_cart<real, RT>(d, k, P, G, Rz, sink);
 }
 

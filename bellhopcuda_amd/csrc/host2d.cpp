@@ -1,3 +1,6 @@
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE
+#endif
 // host2d.cpp -- host front end of the 2D hot path: preprocessing of a raw
 // environment into the problem the launch layer consumes, ScalePressure, and
 // the .shd / .arr / .ray writers.
@@ -49,6 +52,9 @@ template<> struct Lit<float> {
 #define FLH(a) (a##f)
 
 template<typename real> using cx = std::complex<real>;
+
+inline void host_sincos(double a, double *s, double *c) { ::sincos(a, s, c); }
+inline void host_sincos(float a, float *s, float *c) { ::sincosf(a, s, c); }
 
 template<typename real> struct Pre {
     const bhc_env2d *env;
@@ -538,9 +544,14 @@ template<typename real> int preprocess(const bhc_env2d *env, bhc_problem_owner *
     v.Dalpha = (double)Dalpha;
     for(int i = 0; i < env->Nalpha; ++i) {
         o->alpha.push_back((double)al[i]);
-        // launch tangent (cos, sin), src/trace.hpp:61, with the host libm
-        o->acos_.push_back((double)std::cos(al[i]));
-        o->asin_.push_back((double)std::sin(al[i]));
+        // launch tangent (cos, sin), src/trace.hpp:61, with the host libm. The
+        // reference evaluates cos(alpha) and sin(alpha) in one expression, which
+        // g++ -O3 turns into one sincos call; glibc's sincos can differ from sin
+        // in the last bit, so it is called here the same way.
+        real sn, cs;
+        host_sincos(al[i], &sn, &cs);
+        o->acos_.push_back((double)cs);
+        o->asin_.push_back((double)sn);
     }
     // beam, src/module/beaminfo.hpp:221-238
     v.box_r = (double)((real)env->box_r_km * FLH(1000.0));

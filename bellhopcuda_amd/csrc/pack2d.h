@@ -46,16 +46,19 @@ template<typename real> int pack(const bhc_problem2d *p, std::vector<real> &blob
         P.o_cmat = put(p->ssp_cmat, (size_t)N * p->ssp_nr);
         P.o_czmat = put(p->ssp_czmat, (size_t)N * p->ssp_nr);
     }
+    // node record: compact unless a boundary is curvilinear or long format
+    bool full = p->ati_type[0] == 'C' || p->bty_type[0] == 'C' || p->ati_type[1] == 'L' || p->bty_type[1] == 'L';
+    P.node = full ? NODE_FULL : NODE_SHORT;
     auto put_bdry = [&](const bhc_bdry_node2d *nodes, int n) {
         int32_t off = (int32_t)blob.size();
         for(int i = 0; i < n; ++i) {
             const bhc_bdry_node2d &b = nodes[i];
-            real rec[NODE] = {(real)b.x[0],     (real)b.x[1],     (real)b.t[0],     (real)b.t[1],
-                              (real)b.n[0],     (real)b.n[1],     (real)b.Noden[0], (real)b.Noden[1],
-                              (real)b.Nodet[0], (real)b.Nodet[1], (real)b.Len,      (real)b.kappa,
-                              (real)b.cP[0],    (real)b.cP[1],    (real)b.cS[0],    (real)b.cS[1],
-                              (real)b.rho,      (real)0};
-            for(int k = 0; k < NODE; ++k) blob.push_back(rec[k]);
+            real rec[NODE_FULL] = {(real)b.x[0],     (real)b.x[1],     (real)b.t[0],     (real)b.t[1],
+                                   (real)b.n[0],     (real)b.n[1],     (real)b.Len,      (real)b.kappa,
+                                   (real)b.Noden[0], (real)b.Noden[1], (real)b.Nodet[0], (real)b.Nodet[1],
+                                   (real)b.cP[0],    (real)b.cP[1],    (real)b.cS[0],    (real)b.cS[1],
+                                   (real)b.rho,      (real)0};
+            for(int k = 0; k < P.node; ++k) blob.push_back(rec[k]);
         }
         return off;
     };
@@ -106,6 +109,9 @@ template<typename real> int pack(const bhc_problem2d *p, std::vector<real> &blob
     };
     P.top = hs(p->top_hs);
     P.bot = hs(p->bot_hs);
+    P.rz_sorted = 1;
+    for(int i = 1; i < p->NRz; ++i)
+        if(!(p->Rz[i] > p->Rz[i - 1])) P.rz_sorted = 0;
     char t0 = p->BeamType[0];
     P.isGaussian = (t0 == 'B' || t0 == 'b');
     P.isRayCen = (t0 == 'g' || t0 == 'b');

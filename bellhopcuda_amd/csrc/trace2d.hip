@@ -61,8 +61,12 @@ struct ArrCand {
     int32_t ntop, nbot;
 };
 
+// Stored step segment: the descriptor plus padding to a 16-byte multiple.
+template<typename real> struct alignas(16) Seg {
+    SegDesc<real> d;
+};
+
 struct DevOut {
-    float *field;
     ArrCand *cand;
     int32_t *ncand;
     int32_t cand_cap;
@@ -79,18 +83,22 @@ struct DevOut {
     int32_t *nsteps_out;
     unsigned long long *counters; // [0] steps, [1] contributions
     uint32_t *errw;             // err, warn, nerr, nwarn
-    int32_t *work;
-    int32_t njobs;
+    int32_t *work;              // ray work counter (relative to job_begin)
+    int32_t job_begin, job_end; // ray range of this trace pass
+    void *segs;                 // segment buffer (Seg<real>[seg_cap])
+    uint32_t *seg_count;
+    uint32_t seg_cap;
+    uint32_t *overflow;
 };
 
 template<typename real> struct DevSink {
     const DevOut *O;
     const Prob2D<real> *P;
     unsigned long long nc;
-    __device__ void add_field(size_t idx, float re, float im)
+    __device__ void add_field(const Prob2D<real> &Pp, size_t idx, float re, float im)
     {
-        unsafeAtomicAdd(O->field + 2 * idx, re);
-        unsafeAtomicAdd(O->field + 2 * idx + 1, im);
+        unsafeAtomicAdd(Pp.field + 2 * idx, re);
+        unsafeAtomicAdd(Pp.field + 2 * idx + 1, im);
     }
     __device__ void add_arrival(size_t idx, real amp, real phase, cpx<real> delay, real src, real rcv, int ntop,
                                 int nbot, int ialpha, int isz, int is)
@@ -127,34 +135,8 @@ template<typename real> struct DevSink {
         O->hits[mi] = h;
     }
     __device__ void count() { ++nc; }
+    __device__ void mark() {}
 };
-
-// Broadcast lane j's value to the wavefront (scalar registers).
-template<typename T> __device__ inline T bcast(T v, int j)
-{
-    static_assert(sizeof(T) % 4 == 0, "bcast needs 4-byte multiples");
-    constexpr int N = sizeof(T) / 4;
-    uint32_t w[N];
-    __builtin_memcpy(w, &v, sizeof(T));
-#pragma unroll
-    for(int i = 0; i < N; ++i) w[i] = (uint32_t)__builtin_amdgcn_readlane((int)w[i], j);
-    T r;
-    __builtin_memcpy(&r, w, sizeof(T));
-    return r;
-}
-
-template<typename real, int RT>
-__device__ inline void scatter_wave(const SegDesc<real> &d, int lane, const Prob2D<real> &P, const real *G,
-                                    DevSink<real> &sink)
-{
-    unsigned long long m = __ballot(d.nitems > 0);
-    while(m) {
-        int j = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        SegDesc<real> u = bcast(d, j);
-        for(int k = lane; k < u.nitems; k += WAVE) item<real, RT>(u, k, P, G, P.Rz, sink);
-    }
-}
 
 template<typename real> __device__ inline const real *stage(const Prob2D<real> &P, unsigned char *lds)
 {
@@ -174,16 +156,42 @@ __device__ inline void flush_err(const Err &e, const DevOut &O)
     if(e.nwarn) atomicAdd(O.errw + 3, e.nwarn);
 }
 
-// Field modes (TL / arrivals / eigen): MainFieldModes, src/trace.hpp:525-589.
-template<typename real, int RT>
-__global__ void __launch_bounds__(256) k_field2d(Prob2D<real> P, DevOut O)
+// Append this lane's descriptor (if it has work items) to the segment buffer
+// with one atomic per wavefront. Returns false when the buffer is full (the
+// whole pass is then discarded by the host and the ray range split).
+template<typename real>
+__device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut &O)
+{
+    bool has = d.nitems > 0;
+    unsigned long long m = __ballot(has);
+    if(!m) return;
+    int cnt = __popcll(m);
+    int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if(lane == leader) base = atomicAdd(O.seg_count, (uint32_t)cnt);
+    base = __shfl(base, leader);
+    if(has) {
+        uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if(slot < O.seg_cap) {
+            reinterpret_cast<Seg<real> *>(O.segs)[slot].d = d;
+        } else {
+            atomicOr(O.overflow, 1u);
+        }
+    }
+}
+
+// Trace kernel of the field modes (TL / arrivals / eigen): the ray loop of
+// MainFieldModes (src/trace.hpp:525-589). Each step's influence state update
+// is done here (prelude); the receiver work goes to the segment buffer.
+template<typename real, char ST>
+__global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const real *B = stage(P, lds_raw);
     const real *G = P.base;
     const int lane = threadIdx.x & (WAVE - 1);
+    const int njobs = O.job_end - O.job_begin;
     Err e = {0, 0, 0, 0};
-    DevSink<real> sink{&O, &P, 0};
     RayPt<real> p0;
     Trace<real> S;
     Infl<real> I;
@@ -199,14 +207,14 @@ __global__ void __launch_bounds__(256) k_field2d(Prob2D<real> P, DevOut O)
                 int base = 0;
                 if(lane == leader) base = atomicAdd(O.work, cnt);
                 base = __shfl(base, leader);
-                if(base + cnt >= O.njobs) exhausted = true;
+                if(base + cnt >= njobs) exhausted = true;
                 if(!active) {
                     int my = base + __popcll(need & ((1ull << lane) - 1ull));
-                    if(my < O.njobs) {
-                        job = my;
+                    if(my < njobs) {
+                        job = O.job_begin + my;
                         JobIndices(job, P, isz, ia);
                         is = 0;
-                        active = RayInit(isz, ia, p0, S, I, P, B, G, e);
+                        active = RayInit<real, ST>(isz, ia, p0, S, I, P, B, G, e);
                         if(!active && O.stats) {
                             O.stats[3 * job] = 0;
                             O.stats[3 * job + 1] = 0;
@@ -223,29 +231,36 @@ __global__ void __launch_bounds__(256) k_field2d(Prob2D<real> P, DevOut O)
             continue;
         }
         if((iter & 31) == 31) {
+            // HasErrored (src/util/errors.hpp:48-51), polled every 32 steps and
+            // made wavefront-uniform so that all lanes leave together; a full
+            // segment buffer ends the pass the same way.
+            if(e.err) atomicOr(O.errw + 0, e.err);
             uint32_t ge = __hip_atomic_load(O.errw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if(ge != 0u || e.err != 0u) {
+            uint32_t of = __hip_atomic_load(O.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if(__any(ge != 0u || e.err != 0u || of != 0u)) {
                 exhausted = true;
                 active = false;
                 continue;
             }
         }
         RayPt<real> p1, p2;
-        SegDesc<real> dA, dB;
-        dA.nitems = 0;
-        dB.nitems = 0;
+        SegDesc<real> d;
+        d.nitems = 0;
         bool two = false;
         if(active) {
-            two = RayUpdate(p0, p1, p2, S, P, B, G, e);
-            prelude(dA, p0, p1, I, is, P, G, isz, ia);
+            two = RayUpdate<real, ST>(p0, p1, p2, S, P, B, G, e);
+            prelude(d, p0, p1, I, is, P, G, isz, ia);
             ++is;
-            if(two) {
-                prelude(dB, p1, p2, I, is, P, G, isz, ia);
+        }
+        append_seg(d, lane, O);
+        if(__any(two)) {
+            d.nitems = 0;
+            if(active && two) {
+                prelude(d, p1, p2, I, is, P, G, isz, ia);
                 ++is;
             }
+            append_seg(d, lane, O);
         }
-        scatter_wave<real, RT>(dA, lane, P, G, sink);
-        scatter_wave<real, RT>(dB, lane, P, G, sink);
         if(active) {
             p0 = two ? p2 : p1;
             int t = RayTerminate(p0, is, S, MAXN, P, e);
@@ -263,15 +278,227 @@ __global__ void __launch_bounds__(256) k_field2d(Prob2D<real> P, DevOut O)
         }
     }
     flush_err(e, O);
-    // wave-reduce the work counters before the global atomics
-    for(int off = WAVE / 2; off > 0; off >>= 1) {
-        steps += __shfl_xor(steps, off);
-        sink.nc += __shfl_xor(sink.nc, off);
+    for(int off = WAVE / 2; off > 0; off >>= 1) steps += __shfl_xor(steps, off);
+    if(lane == 0) atomicAdd(O.counters + 0, steps);
+}
+
+// Scatter kernel: one wavefront per segment; the descriptor is wavefront-
+// uniform (scalar loads) and the 64 lanes split its receiver work items
+// (influence.hpp:1075-1187 ray-centred depths, :1298-1386 Cartesian ranges x
+// depths).
+template<typename real, int RT>
+__global__ void __launch_bounds__(256) k_scatter2d(Prob2D<real> P, DevOut O)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
+    const int nwaves = gridDim.x * (blockDim.x / WAVE);
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    DevSink<real> sink{&O, &P, 0};
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
+        const SegDesc<real> d = segs[s].d;
+        for(int k = lane; k < d.nitems; k += WAVE) item<real, RT>(d, k, P, P.base, P.Rz, sink);
     }
-    if(lane == 0) {
-        atomicAdd(O.counters + 0, steps);
-        atomicAdd(O.counters + 1, sink.nc);
+    for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
+    if(lane == 0 && sink.nc) atomicAdd(O.counters + 1, sink.nc);
+}
+
+// ---------------------------------------------------------------------------
+// Range-binned TL scatter with an LDS field tile.
+//
+// Segments are counting-sorted by the receiver-range column at their centre
+// (bins of BIN_W columns). A workgroup takes a chunk of one bin and keeps the
+// complex field of TILE_W columns around the bin (all NRz_per_range depths) in
+// LDS; contributions inside the tile are added with LDS atomics, the rest go
+// straight to the global field, and the tile is flushed once at the end. With
+// ~10^3 contributions per receiver per field, this removes almost all global
+// atomics.
+// ---------------------------------------------------------------------------
+struct BinCfg {
+    int32_t nbins, bin_w, tile_w, tile_lo_off; // tile columns = [b*bin_w - tile_lo_off, + tile_w)
+    int32_t chunk;                              // segments per workgroup
+    int32_t *counts, *offsets, *cursor, *chunk_start;
+    uint32_t *idx;
+    int32_t *nchunks;
+};
+
+template<typename real> __device__ inline int seg_bin(const SegDesc<real> &d, const Prob2D<real> &P, const BinCfg &C)
+{
+    int col;
+    if(P.isRayCen) {
+        double c = 0.5 * ((double)d.xb.x + (d.lastValid ? (double)d.xa.x : (double)d.xb.x));
+        double t = (c - (double)P.base[P.o_Rr]) / (double)P.Delta_r;
+        t = t < -1.0 ? -1.0 : (t > (double)P.NRr ? (double)P.NRr : t);
+        col = (int)t;
+    } else {
+        col = (d.ir0 + d.ir1) / 2;
     }
+    col = imax(0, imin(col, P.NRr - 1));
+    return imin(col / C.bin_w, C.nbins - 1);
+}
+
+// Bin histogram: each workgroup owns a contiguous slice of the segments and
+// counts it in LDS first (one global atomic per bin per workgroup).
+template<typename real> __global__ void __launch_bounds__(256) k_bin_count(Prob2D<real> P, DevOut O, BinCfg C)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    int32_t *h = reinterpret_cast<int32_t *>(lds_raw);
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    uint32_t s0 = blockIdx.x * per, s1 = min(n, s0 + per);
+    for(int b = threadIdx.x; b < C.nbins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+    __syncthreads();
+    for(int b = threadIdx.x; b < C.nbins; b += blockDim.x)
+        if(h[b]) atomicAdd(C.counts + b, h[b]);
+}
+
+// Exclusive scan of the bin counts and of the per-bin chunk counts (one workgroup).
+__global__ void __launch_bounds__(1024) k_bin_scan(BinCfg C)
+{
+    __shared__ int32_t part_s[1024], part_c[1024];
+    int t = threadIdx.x, nb = C.nbins;
+    int per = (nb + 1023) / 1024;
+    int b0 = t * per, b1 = min(nb, b0 + per);
+    int ss = 0, sc = 0;
+    for(int b = b0; b < b1; ++b) {
+        ss += C.counts[b];
+        sc += (C.counts[b] + C.chunk - 1) / C.chunk;
+    }
+    part_s[t] = ss;
+    part_c[t] = sc;
+    __syncthreads();
+    if(t == 0) {
+        int a = 0, c = 0;
+        for(int i = 0; i < 1024; ++i) {
+            int x = part_s[i], y = part_c[i];
+            part_s[i] = a;
+            part_c[i] = c;
+            a += x;
+            c += y;
+        }
+        *C.nchunks = c;
+    }
+    __syncthreads();
+    ss = part_s[t];
+    sc = part_c[t];
+    for(int b = b0; b < b1; ++b) {
+        C.offsets[b] = ss;
+        C.cursor[b] = ss;
+        int nc = (C.counts[b] + C.chunk - 1) / C.chunk;
+        for(int k = 0; k < nc; ++k) C.chunk_start[sc + k] = b; // chunk -> bin
+        ss += C.counts[b];
+        sc += nc;
+    }
+}
+
+// Bin fill with the same slicing as k_bin_count: the workgroup reserves its
+// range in every bin with one global atomic, then ranks its segments in LDS.
+template<typename real> __global__ void __launch_bounds__(256) k_bin_fill(Prob2D<real> P, DevOut O, BinCfg C)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    int32_t *h = reinterpret_cast<int32_t *>(lds_raw);
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    uint32_t s0 = blockIdx.x * per, s1 = min(n, s0 + per);
+    for(int b = threadIdx.x; b < C.nbins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+    __syncthreads();
+    for(int b = threadIdx.x; b < C.nbins; b += blockDim.x)
+        if(h[b]) h[b] = atomicAdd(C.cursor + b, h[b]);
+    __syncthreads();
+    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        int pos = atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+        C.idx[pos] = s;
+    }
+}
+
+template<typename real> struct TileSink {
+    float *tile;          // LDS [tile_w][nz] complex
+    int col0, tile_w, nz, nrr;
+    unsigned long long nc;
+    __device__ void add_field(const Prob2D<real> &Pp, size_t idx, float re, float im)
+    {
+        int ir = (int)(idx % (size_t)nrr);
+        int iz = (int)((idx / (size_t)nrr) % (size_t)nz);
+        int c = ir - col0;
+        if(c >= 0 && c < tile_w && (idx / ((size_t)nrr * nz)) == 0) {
+            int k = c * nz + iz; // real parts first, then imaginary parts (bank-conflict free)
+            atomicAdd(tile + k, re);
+            atomicAdd(tile + tile_w * nz + k, im);
+        } else {
+            unsafeAtomicAdd(Pp.field + 2 * idx, re);
+            unsafeAtomicAdd(Pp.field + 2 * idx + 1, im);
+        }
+    }
+    template<typename R2> __device__ void add_arrival(size_t, R2, R2, cpx<R2>, R2, R2, int, int, int, int, int) {}
+    __device__ void add_eigen(int, int, int, int, int) {}
+    __device__ void count() { ++nc; }
+    __device__ void mark() {}
+};
+
+// One workgroup per (bin, chunk); the field of source isz lives at
+// P.field + isz * plane, so the tile is per source: segments of other
+// sources fall back to global atomics (single-source runs are the common case).
+template<typename real>
+__global__ void __launch_bounds__(256) k_scatter_tile(Prob2D<real> P, DevOut O, BinCfg C)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float *tile = reinterpret_cast<float *>(lds_raw);
+    int nchunks = *C.nchunks;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int nw = blockDim.x / WAVE;
+    const int nz = P.NRz_per_range;
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    unsigned long long total = 0;
+    for(int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        int b = C.chunk_start[ch];
+        int first_chunk = ch;
+        while(first_chunk > 0 && C.chunk_start[first_chunk - 1] == b) --first_chunk; // chunk index within bin
+        int k0 = C.offsets[b] + (ch - first_chunk) * C.chunk;
+        int k1 = min(C.offsets[b] + C.counts[b], k0 + C.chunk);
+        int col0 = b * C.bin_w - C.tile_lo_off;
+        int ncol = C.tile_w;
+        for(int i = threadIdx.x; i < 2 * ncol * nz; i += blockDim.x) tile[i] = 0.f;
+        __syncthreads();
+        TileSink<real> sink{tile, col0, ncol, nz, P.NRr, 0};
+        for(int k = k0 + wave; k < k1; k += nw) {
+            uint32_t si = __builtin_amdgcn_readfirstlane(C.idx[k]);
+            const SegDesc<real> d = segs[si].d;
+            if(d.isz != 0) {
+                DevSink<real> gs{&O, &P, 0};
+                for(int it = lane; it < d.nitems; it += WAVE) item<real, RT_TL>(d, it, P, P.base, P.Rz, gs);
+                sink.nc += gs.nc;
+                continue;
+            }
+            for(int it = lane; it < d.nitems; it += WAVE) item<real, RT_TL>(d, it, P, P.base, P.Rz, sink);
+        }
+        __syncthreads();
+        // flush: tile column c, depth iz -> field (isz 0)
+        for(int i = threadIdx.x; i < ncol * nz; i += blockDim.x) {
+            int c = i / nz, iz = i - c * nz;
+            int ir = col0 + c;
+            if(ir < 0 || ir >= P.NRr) continue;
+            float re = tile[i], im = tile[ncol * nz + i];
+            if(re != 0.f || im != 0.f) {
+                size_t idx = (size_t)iz * P.NRr + ir;
+                unsafeAtomicAdd(P.field + 2 * idx, re);
+                unsafeAtomicAdd(P.field + 2 * idx + 1, im);
+            }
+        }
+        __syncthreads();
+        total += sink.nc;
+    }
+    for(int off = WAVE / 2; off > 0; off >>= 1) total += __shfl_xor(total, off);
+    if(lane == 0 && total) atomicAdd(O.counters + 1, total);
 }
 
 template<typename real> __device__ inline void store_pt(bhc_ray_pt2d *d, const RayPt<real> &s)
@@ -304,7 +531,7 @@ __global__ void __launch_bounds__(256) k_ray2d(Prob2D<real> P, DevOut O)
     const real *G = P.base;
     Err e = {0, 0, 0, 0};
     const int M = O.max_points;
-    for(int job = blockIdx.x * blockDim.x + threadIdx.x; job < O.njobs; job += gridDim.x * blockDim.x) {
+    for(int job = blockIdx.x * blockDim.x + threadIdx.x; job < O.job_end; job += gridDim.x * blockDim.x) {
         int isz, ia;
         if(O.job_alpha) {
             ia = O.job_alpha[job];
@@ -460,11 +687,18 @@ __global__ void __launch_bounds__(256) k_arr_merge(const ArrCand *cand, const in
 struct bhc_ctx {
     int device;
     hipStream_t stream;
-    hipEvent_t ev0, ev1;
+    hipEvent_t ev0, ev1, evm;   // start, end, trace/scatter boundary of the last pass
     int num_cus;
     int lds_max;
     int tpb, bpc;
     char name[256];
+    // segment buffer shared by all problems of the context (grown on demand)
+    void *segs;
+    size_t seg_bytes;
+    uint32_t *d_seg_count, *d_overflow;
+    uint32_t *d_bin_idx;        // binned segment order (seg capacity entries)
+    int passes;                 // trace/scatter passes of the last launch
+    int use_tiles;              // 0 disables the LDS-tiled TL scatter
 };
 
 struct bhc_dev_problem {
@@ -494,6 +728,9 @@ struct bhc_dev_problem {
     uint32_t *d_errw;
     int32_t *d_work;
     double freq0;
+    // range bins of the tiled TL scatter
+    int nbins, bin_w, tile_w, tile_lo_off, chunk, max_chunks;
+    int32_t *d_bins;            // counts | offsets | cursor | chunk_start | nchunks
 };
 
 extern "C" const char *bhc_last_error(void) { return g_last_error.c_str(); }
@@ -524,9 +761,14 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     snprintf(c->name, sizeof(c->name), "%s", prop.gcnArchName);
     c->tpb = 256;
     c->bpc = 2;
+    c->use_tiles = 1;
+    if(const char *e = getenv("BHC_TL_TILES")) c->use_tiles = atoi(e) != 0;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));
     HIPCHECK(hipEventCreate(&c->ev1));
+    HIPCHECK(hipEventCreate(&c->evm));
+    HIPCHECK(hipMalloc((void **)&c->d_seg_count, sizeof(uint32_t)));
+    HIPCHECK(hipMalloc((void **)&c->d_overflow, sizeof(uint32_t)));
     *out = c;
     return 0;
 }
@@ -538,6 +780,11 @@ extern "C" void bhc_ctx_destroy(bhc_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->evm);
+    if(ctx->segs) (void)hipFree(ctx->segs);
+    if(ctx->d_bin_idx) (void)hipFree(ctx->d_bin_idx);
+    (void)hipFree(ctx->d_seg_count);
+    (void)hipFree(ctx->d_overflow);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -554,6 +801,18 @@ extern "C" int bhc_ctx_set_launch(bhc_ctx *ctx, int threads_per_block, int block
 extern "C" int bhc_ctx_synchronize(bhc_ctx *ctx)
 {
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int bhc_last_phase_ms(bhc_ctx *ctx, double *trace_ms, double *scatter_ms, int *passes)
+{
+    float a = 0.f, b = 0.f;
+    HIPCHECK(hipEventSynchronize(ctx->ev1));
+    HIPCHECK(hipEventElapsedTime(&a, ctx->ev0, ctx->evm));
+    HIPCHECK(hipEventElapsedTime(&b, ctx->evm, ctx->ev1));
+    *trace_ms = a;
+    *scatter_ms = b;
+    *passes = ctx->passes;
     return 0;
 }
 
@@ -597,12 +856,16 @@ extern "C" void bhc_problem_release(bhc_dev_problem *dp)
     if(!dp) return;
     (void)hipSetDevice(dp->ctx->device);
     (void)hipStreamSynchronize(dp->ctx->stream);
-    void *ptrs[] = {dp->d_blob, dp->d_Rz,   dp->d_Sz,     dp->d_field,    dp->d_cand, dp->d_ncand,
+    void *ptrs[] = {dp->d_bins, dp->d_blob, dp->d_Rz,   dp->d_Sz,     dp->d_field,    dp->d_cand, dp->d_ncand,
                     dp->d_arr,  dp->d_narr, dp->d_hits,   dp->d_neigen,   dp->d_stats, dp->d_xend,
                     dp->d_counters, dp->d_errw, dp->d_work};
     for(void *p : ptrs)
         if(p) (void)hipFree(p);
     delete dp;
+}
+
+namespace {
+int ensure_segs(bhc_ctx *c);
 }
 
 extern "C" int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, const bhc_field_out *spec,
@@ -653,8 +916,8 @@ extern "C" int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, con
         bhc_problem_release(dp);
         return -1;
     }
-    // Stage in LDS when the SSP + boundary tables fit in 64 KiB.
-    dp->lds_bytes = (staged <= 65536) ? staged : 0;
+    // Stage in LDS when the SSP + boundary tables fit the workgroup's LDS.
+    dp->lds_bytes = (staged <= (size_t)std::min(ctx->lds_max, 65536)) ? staged : 0;
     if(!dp->lds_bytes) {
         dp->P64.staged_reals = 0;
         dp->P32.staged_reals = 0;
@@ -681,6 +944,33 @@ extern "C" int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, con
     if(tl && dalloc(&dp->d_field, 2 * dp->nrcv)) {
         bhc_problem_release(dp);
         return -1;
+    }
+    dp->P64.field = dp->P32.field = dp->d_field;
+    dp->P64.njobs = dp->P32.njobs = dp->njobs;
+    if(tl && prob->NRr > 1 && prob->Delta_r > 0) {
+        // LDS tile of 64 KiB: tile_w range columns x all depths, complex float
+        int nz = prob->NRz_per_range;
+        int tile_w = (int)(65536 / ((size_t)nz * 2 * sizeof(float)));
+        int margin = (int)std::ceil(prob->deltas / (2.0 * prob->Delta_r)) + 2;
+        if(tile_w >= 6 && (prob->NRr + std::max(1, tile_w - 2 * margin) - 1) / std::max(1, tile_w - 2 * margin) <= 16384) {
+            int bin_w = std::max(1, tile_w - 2 * margin);
+            if(bin_w == 1) margin = (tile_w - 1) / 2;
+            dp->tile_w = tile_w;
+            dp->bin_w = bin_w;
+            dp->tile_lo_off = margin;
+            dp->nbins = (prob->NRr + bin_w - 1) / bin_w;
+            dp->chunk = 2048;
+            if(ensure_segs(ctx)) {
+                bhc_problem_release(dp);
+                return -1;
+            }
+            size_t cap = ctx->seg_bytes / sizeof(Seg<float>);
+            dp->max_chunks = (int)std::min<size_t>(cap / dp->chunk + dp->nbins + 1, 0x7FFFFFFF);
+            if(dalloc(&dp->d_bins, 3 * (size_t)dp->nbins + 1 + dp->max_chunks)) {
+                bhc_problem_release(dp);
+                return -1;
+            }
+        }
     }
     if(arr) {
         dp->max_narr = spec ? spec->max_narr : 0;
@@ -727,7 +1017,6 @@ DevOut make_out(bhc_dev_problem *dp)
 {
     DevOut O;
     memset(&O, 0, sizeof(O));
-    O.field = dp->d_field;
     O.cand = dp->d_cand;
     O.ncand = dp->d_ncand;
     O.cand_cap = dp->cand_cap;
@@ -739,18 +1028,105 @@ DevOut make_out(bhc_dev_problem *dp)
     O.counters = dp->d_counters;
     O.errw = dp->d_errw;
     O.work = dp->d_work;
-    O.njobs = dp->njobs;
+    O.job_begin = 0;
+    O.job_end = dp->njobs;
     return O;
 }
 
-template<typename real, int RT> int launch_field(bhc_dev_problem *dp, const Prob2D<real> &P, const DevOut &O)
+size_t seg_size(int precision) { return precision == 64 ? sizeof(Seg<double>) : sizeof(Seg<float>); }
+
+// Segment buffer: BHC_SEG_BUFFER_MB (default 16384 MiB) of HBM, allocated once per context.
+int ensure_segs(bhc_ctx *c)
+{
+    if(c->segs) return 0;
+    size_t mb = 16384;
+    if(const char *e = getenv("BHC_SEG_BUFFER_MB")) mb = (size_t)std::max(16L, atol(e));
+    c->seg_bytes = mb << 20;
+    HIPCHECK(hipMalloc(&c->segs, c->seg_bytes));
+    size_t maxseg = c->seg_bytes / sizeof(Seg<float>);
+    HIPCHECK(hipMalloc((void **)&c->d_bin_idx, maxseg * sizeof(uint32_t)));
+    return 0;
+}
+
+template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D<real> &P, DevOut O)
 {
     bhc_ctx *c = dp->ctx;
-    int waves = (dp->njobs + WAVE - 1) / WAVE;
-    int blocks = std::min((waves + 3) / 4, c->num_cus * c->bpc);
-    blocks = std::max(blocks, 1);
-    hipLaunchKernelGGL((k_field2d<real, RT>), dim3(blocks), dim3(256), dp->lds_bytes, c->stream, P, O);
-    HIPCHECK(hipGetLastError());
+    hipStream_t s = c->stream;
+    if(ensure_segs(c)) return -1;
+    O.segs = c->segs;
+    O.seg_count = c->d_seg_count;
+    O.overflow = c->d_overflow;
+    O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg<real>), 0xFFFFFFF0u);
+    std::vector<std::pair<int, int>> todo{{0, dp->njobs}};
+    c->passes = 0;
+    bool first = true;
+    while(!todo.empty()) {
+        auto r = todo.back();
+        todo.pop_back();
+        O.job_begin = r.first;
+        O.job_end = r.second;
+        HIPCHECK(hipMemsetAsync(c->d_seg_count, 0, sizeof(uint32_t), s));
+        HIPCHECK(hipMemsetAsync(c->d_overflow, 0, sizeof(uint32_t), s));
+        HIPCHECK(hipMemsetAsync(dp->d_work, 0, sizeof(int32_t), s));
+        int nj = r.second - r.first;
+        int waves = (nj + WAVE - 1) / WAVE;
+        // one wavefront per workgroup spreads the rays over all CUs (the trace
+        // kernel is register-bound at one wavefront per SIMD)
+        int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
+        if(first) HIPCHECK(hipEventRecord(c->ev0, s));
+        switch(P.ssp_type) {
+        case 'C': hipLaunchKernelGGL((k_trace2d<real, 'C'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
+        case 'S': hipLaunchKernelGGL((k_trace2d<real, 'S'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
+        case 'P': hipLaunchKernelGGL((k_trace2d<real, 'P'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
+        case 'N': hipLaunchKernelGGL((k_trace2d<real, 'N'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
+        case 'Q': hipLaunchKernelGGL((k_trace2d<real, 'Q'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
+        default: hipLaunchKernelGGL((k_trace2d<real, 'A'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
+        }
+        HIPCHECK(hipGetLastError());
+        uint32_t of = 0;
+        HIPCHECK(hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if(of) {
+            if(nj <= 1) return fail("segment buffer too small for a single ray (raise BHC_SEG_BUFFER_MB)");
+            int mid = r.first + nj / 2;
+            todo.push_back({mid, r.second});
+            todo.push_back({r.first, mid});
+            // the trace pass wrote only per-ray statistics and the step counter:
+            // the retried halves rewrite the statistics; undo the counter
+            HIPCHECK(hipMemsetAsync(dp->d_counters, 0, sizeof(unsigned long long), s));
+            continue;
+        }
+        if(first) HIPCHECK(hipEventRecord(c->evm, s));
+        if(RT == RT_TL && c->use_tiles && dp->nbins > 0) {
+            BinCfg B;
+            B.nbins = dp->nbins;
+            B.bin_w = dp->bin_w;
+            B.tile_w = dp->tile_w;
+            B.tile_lo_off = dp->tile_lo_off;
+            B.chunk = dp->chunk;
+            B.counts = dp->d_bins;
+            B.offsets = B.counts + dp->nbins;
+            B.cursor = B.offsets + dp->nbins;
+            B.nchunks = B.cursor + dp->nbins;
+            B.chunk_start = B.nchunks + 1;
+            B.idx = c->d_bin_idx;
+            HIPCHECK(hipMemsetAsync(B.counts, 0, dp->nbins * sizeof(int32_t), s));
+            int gb = c->num_cus * 4;
+            size_t hl = (size_t)dp->nbins * sizeof(int32_t);
+            hipLaunchKernelGGL((k_bin_count<real>), dim3(gb), dim3(256), hl, s, P, O, B);
+            hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, B);
+            hipLaunchKernelGGL((k_bin_fill<real>), dim3(gb), dim3(256), hl, s, P, O, B);
+            size_t lds = (size_t)dp->tile_w * dp->NRz_per_range * 2 * sizeof(float);
+            hipLaunchKernelGGL((k_scatter_tile<real>), dim3(c->num_cus * 4), dim3(256), lds, s, P, O, B);
+        } else {
+            int sblocks = c->num_cus * 8;
+            hipLaunchKernelGGL((k_scatter2d<real, RT>), dim3(sblocks), dim3(256), 0, s, P, O);
+        }
+        HIPCHECK(hipGetLastError());
+        ++c->passes;
+        first = false;
+    }
+    HIPCHECK(hipEventRecord(c->ev1, s));
     return 0;
 }
 } // namespace
@@ -766,22 +1142,19 @@ extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
     if(dp->d_neigen) HIPCHECK(hipMemsetAsync(dp->d_neigen, 0, sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(dp->d_counters, 0, 2 * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(dp->d_errw, 0, 4 * sizeof(uint32_t), s));
-    HIPCHECK(hipMemsetAsync(dp->d_work, 0, sizeof(int32_t), s));
     DevOut O = make_out(dp);
     char r0 = dp->run0;
     int rt = (r0 == 'C' || r0 == 'S' || r0 == 'I') ? RT_TL : (r0 == 'E') ? RT_EIG : RT_ARR;
-    HIPCHECK(hipEventRecord(ctx->ev0, s));
     int rc;
     if(dp->precision == 64) {
-        rc = rt == RT_TL ? launch_field<double, RT_TL>(dp, dp->P64, O)
-            : rt == RT_ARR ? launch_field<double, RT_ARR>(dp, dp->P64, O)
-                           : launch_field<double, RT_EIG>(dp, dp->P64, O);
+        rc = rt == RT_TL ? run_passes<double, RT_TL>(dp, dp->P64, O)
+            : rt == RT_ARR ? run_passes<double, RT_ARR>(dp, dp->P64, O)
+                           : run_passes<double, RT_EIG>(dp, dp->P64, O);
     } else {
-        rc = rt == RT_TL ? launch_field<float, RT_TL>(dp, dp->P32, O)
-            : rt == RT_ARR ? launch_field<float, RT_ARR>(dp, dp->P32, O)
-                           : launch_field<float, RT_EIG>(dp, dp->P32, O);
+        rc = rt == RT_TL ? run_passes<float, RT_TL>(dp, dp->P32, O)
+            : rt == RT_ARR ? run_passes<float, RT_ARR>(dp, dp->P32, O)
+                           : run_passes<float, RT_EIG>(dp, dp->P32, O);
     }
-    HIPCHECK(hipEventRecord(ctx->ev1, s));
     if(rc) return rc;
     if(rt == RT_ARR) {
         int nrcv = (int)dp->nrcv;
@@ -866,7 +1239,8 @@ extern "C" int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_r
     if(bhc_problem_upload2d(ctx, &p2, &spec, &dp)) return -1;
     int njobs = nrays >= 0 ? nrays : dp->njobs;
     DevOut O = make_out(dp);
-    O.njobs = njobs;
+    O.job_begin = 0;
+    O.job_end = njobs;
     O.max_points = out->max_points_per_ray;
     int32_t *d_ia = nullptr, *d_isz = nullptr, *d_nin = nullptr, *d_nout = nullptr;
     bhc_ray_pt2d *d_pts = nullptr;

@@ -306,6 +306,9 @@ int bhc_ctx_synchronize(bhc_ctx *ctx);
 /* Duration (ms) of the tracer kernel of the last launch on this context,
  * from HIP events recorded on the context's stream around it. */
 int bhc_last_kernel_ms(bhc_ctx *ctx, double *ms);
+/* Split of the last launch: trace kernel of the first pass, scatter kernel of
+ * the first pass (ms), and the number of trace/scatter passes it took. */
+int bhc_last_phase_ms(bhc_ctx *ctx, double *trace_ms, double *scatter_ms, int *passes);
 
 /* Launch tuning (rays per workgroup, workgroups per CU); 0 = autotuned default. */
 int bhc_ctx_set_launch(bhc_ctx *ctx, int threads_per_block, int blocks_per_cu);

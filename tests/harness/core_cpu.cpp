@@ -18,7 +18,7 @@ namespace {
 struct HostSink {
     float *field;
     long long nc = 0;
-    void add_field(size_t idx, float re, float im)
+    template<typename P> void add_field(const P &, size_t idx, float re, float im)
     {
         field[2 * idx] += re;
         field[2 * idx + 1] += im;
@@ -29,6 +29,7 @@ struct HostSink {
     }
     void add_eigen(int, int, int, int, int) {}
     void count() { ++nc; }
+    void mark() {}
 };
 
 template<typename real> int run(const bhc_problem2d *prob, bhc_field_out *out)
@@ -100,4 +101,50 @@ template<typename real> int run(const bhc_problem2d *prob, bhc_field_out *out)
 extern "C" int core_field_run2d(const bhc_problem2d *prob, bhc_field_out *out)
 {
     return prob->precision == 64 ? run<double>(prob, out) : run<float>(prob, out);
+}
+
+namespace {
+template<typename real> int run_ray(const bhc_problem2d *prob, int isz, int ia, int maxp, bhc_ray_pt2d *pts, int *nsteps)
+{
+    std::vector<real> blob;
+    Prob2D<real> P;
+    std::string err;
+    if(pack<real>(prob, blob, P, err)) return -1;
+    P.base = blob.data();
+    P.Rz = prob->Rz;
+    P.Sz = prob->Sz;
+    const real *B = blob.data(), *G = blob.data();
+    Err e = {0, 0, 0, 0};
+    RayPt<real> p0, p1, p2;
+    Trace<real> S;
+    Infl<real> I;
+    auto st = [&](int k, const RayPt<real> &r) {
+        bhc_ray_pt2d &d = pts[k];
+        d.x[0] = r.x.x; d.x[1] = r.x.y; d.t[0] = r.t.x; d.t[1] = r.t.y; d.p[0] = r.p.x; d.p[1] = r.p.y;
+        d.q[0] = r.q.x; d.q[1] = r.q.y; d.c = r.c; d.Amp = r.Amp; d.Phase = r.Phase; d.tau[0] = r.tau.re;
+        d.tau[1] = r.tau.im; d.NumTopBnc = r.NumTopBnc; d.NumBotBnc = r.NumBotBnc;
+    };
+    bool ok = RayInit(isz, ia, p0, S, I, P, B, G, e);
+    st(0, p0);
+    if(!ok) { *nsteps = 1; return 0; }
+    int is = 0, N = -1;
+    for(;;) {
+        bool two = RayUpdate(p0, p1, p2, S, P, B, G, e);
+        st(is + 1, p1);
+        if(two) st(is + 2, p2);
+        is += two ? 2 : 1;
+        p0 = two ? p2 : p1;
+        int t = RayTerminate(p0, is, S, maxp, P, e);
+        if(t == 1) { N = is + 1; break; }
+        if(t == 2) { N = is; break; }
+    }
+    *nsteps = N;
+    return 0;
+}
+} // namespace
+
+extern "C" int core_ray_run2d(const bhc_problem2d *prob, int isz, int ia, int maxp, bhc_ray_pt2d *pts, int *nsteps)
+{
+    return prob->precision == 64 ? run_ray<double>(prob, isz, ia, maxp, pts, nsteps)
+                                  : run_ray<float>(prob, isz, ia, maxp, pts, nsteps);
 }

@@ -1,0 +1,32 @@
+"""Shared parity checks: GPU (or product core) results against the oracle.
+
+Bars (from the task's parity definition):
+  * per-ray step and bounce counts equal on >= 99.9 % of rays (FP64);
+  * ray end points within 1e-9 relative (FP64);
+  * TL within 0.1 dB where TL < 120 dB.
+"""
+import numpy as np
+
+
+def tl_db(field):
+    a = np.abs(field).astype(np.float64)
+    with np.errstate(divide="ignore"):
+        return -20.0 * np.log10(np.maximum(a, 1e-300))
+
+
+def compare_field(o, g, scaled_o, scaled_g, fp64=True, min_exact=0.999, path_rtol=1e-9, tl_tol=0.1):
+    rep = {}
+    rep["steps_exact"] = float(np.mean(o.nsteps == g.nsteps))
+    rep["bounces_exact"] = float(np.mean((o.ntop == g.ntop) & (o.nbot == g.nbot)))
+    same = (o.nsteps == g.nsteps)
+    scale = np.maximum(np.abs(o.xend), 1.0)
+    rel = np.max(np.abs(o.xend - g.xend) / scale, axis=1)
+    rep["xend_max_rel"] = float(np.max(rel[same])) if same.any() else 0.0
+    tlo, tlg = tl_db(scaled_o), tl_db(scaled_g)
+    m = tlo < 120.0
+    rep["tl_max_db"] = float(np.max(np.abs(tlo - tlg)[m])) if m.any() else 0.0
+    rep["tl_points"] = int(m.sum())
+    ok = rep["steps_exact"] >= min_exact and rep["bounces_exact"] >= min_exact and rep["tl_max_db"] <= tl_tol
+    if fp64:
+        ok = ok and rep["xend_max_rel"] <= path_rtol
+    return ok, rep

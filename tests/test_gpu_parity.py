@@ -1,0 +1,75 @@
+"""GPU parity: the HIP tracer against the oracle on the same inputs.
+
+Each case prints its name before touching the GPU so that a failing run names
+its case. Sizes are small enough for the oracle to finish in seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bellhopcuda_amd as bh
+from bellhopcuda_amd.env import munk_env, c2_env, Boundary, ReflTable, Halfspace, subtab
+import oracle
+from parity import compare_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(name):
+    print("\n[gpu-case]", name, flush=True)
+    sys.stdout.flush()
+
+
+def _run_both(env, ctx):
+    o = oracle.field_run(env, nthreads=0)
+    g = bh.field_run(env, ctx=ctx, stats=True)
+    so = oracle.scale_pressure(env, o.field)
+    sg = g.problem.scale_pressure(g.field)
+    return o, g, so, sg
+
+
+CASES = {
+    "munk_hat_cart_f64": lambda: munk_env(nbeams=401, nrz=101, nrr=201),
+    "munk_hat_cart_f32": lambda: munk_env(nbeams=401, nrz=101, nrr=201, precision=32),
+    "munk_gauss_cart": lambda: munk_env(nbeams=301, nrz=101, nrr=201, run_type="CB"),
+    "munk_hat_raycen": lambda: munk_env(nbeams=301, nrz=101, nrr=201, run_type="Cg"),
+    "munk_gauss_raycen": lambda: munk_env(nbeams=301, nrz=101, nrr=201, run_type="Cb"),
+    "munk_incoherent": lambda: munk_env(nbeams=301, nrz=101, nrr=201, run_type="IG"),
+    "munk_semicoherent": lambda: munk_env(nbeams=301, nrz=101, nrr=201, run_type="SG"),
+    "munk_spline": lambda: munk_env(nbeams=301, nrz=101, nrr=201, ssp_type="S"),
+    "munk_pchip": lambda: munk_env(nbeams=301, nrz=101, nrr=201, ssp_type="P"),
+    "munk_n2linear": lambda: munk_env(nbeams=301, nrz=101, nrr=201, ssp_type="N"),
+    "seamount_spline_raycen": lambda: c2_env(nbeams=401, nrz=61, nrr=401),
+    "seamount_spline_raycen_f32": lambda: c2_env(nbeams=401, nrz=61, nrr=401, precision=32),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_field_parity(name, gpu_ctx):
+    _announce(name)
+    env = CASES[name]()
+    o, g, so, sg = _run_both(env, gpu_ctx)
+    assert g.err == o.err, (g.err, o.err)
+    assert g.warn == o.warn, (g.warn, o.warn)
+    ok, rep = compare_field(o, g, so, sg, fp64=(env.precision == 64))
+    print(name, rep, flush=True)
+    assert ok, rep
+    assert g.total_steps == o.total_steps
+    if env.precision == 64:
+        assert g.total_contrib == o.total_contrib
+
+
+def test_ray_mode_parity(gpu_ctx):
+    _announce("ray_mode_munk")
+    env = munk_env(nbeams=101, run_type="RG")
+    pts_o, ns_o, _ = oracle.ray_run(env, max_points=4000)
+    pts_g, ns_g, _ = bh.ray_run(env, ctx=gpu_ctx, max_points=4000)
+    assert np.array_equal(ns_o, ns_g)
+    for k in range(len(ns_o)):
+        n = ns_o[k]
+        xo = np.stack([pts_o[k, :n]["x"]], 0)
+        xg = np.stack([pts_g[k, :n]["x"]], 0)
+        assert np.array_equal(xo, xg), k
+        assert np.array_equal(pts_o[k, :n]["NumTopBnc"], pts_g[k, :n]["NumTopBnc"])
+        assert np.array_equal(pts_o[k, :n]["NumBotBnc"], pts_g[k, :n]["NumBotBnc"])
