@@ -615,6 +615,16 @@ extern "C" int bhc_preprocess2d(const bhc_env2d *env, bhc_problem_owner **out)
     if(env->precision != 64 && env->precision != 32) return hfail("precision must be 32 or 64");
     if(env->NSz < 1 || env->NRz < 1 || env->NRr < 1 || env->Nalpha < 1)
         return hfail("sources, receivers and launch angles must be non-empty");
+    // Run types of the reference (src/module/runtype.hpp): 'C'oherent,
+    // 'S'emicoherent, 'I'ncoherent TL, 'R'ay, 'E'igenray, 'A'/'a' arrivals.
+    if(!std::strchr("CSIREAa", env->RunType[0]) || env->RunType[0] == '\0')
+        return hfail("run type '%c' is not supported", env->RunType[0]);
+    // Influence types on this path are the geometric ones (hat / Gaussian,
+    // Cartesian / ray-centred); Cerveny ('R', 'C') and simple Gaussian ('S')
+    // beams are not.
+    if(!std::strchr("GgBb^ ", env->RunType[1]) || env->RunType[1] == '\0')
+        return hfail("beam type '%c' is not a geometric type (Cerveny / simple Gaussian beams are not on this path)",
+                     env->RunType[1]);
     bhc_problem_owner *o = new bhc_problem_owner();
     int rc = env->precision == 64 ? preprocess<double>(env, o) : preprocess<float>(env, o);
     if(rc) {

@@ -321,6 +321,15 @@ int bhc_scale_pressure2d(const bhc_problem2d *prob, float *field);
 int bhc_write_shd2d(const char *path, const bhc_problem2d *prob, const char *title, const float *field);
 const char *bhc_host_last_error(void);
 
+/* ---- measurement (no counterpart in the reference) ------------------------------ */
+
+/* Algorithmic floating-point operation counts of the TL path for `prob`: the
+ * same tracer core instantiated on the host with a counting scalar type, run
+ * over `nsample` rays spread over the fan. out[0..5] = operations per ray
+ * (initialisation), per step (step + influence prelude), per accepted receiver
+ * contribution, and the sample's rays, steps and contributions. */
+int bhc_opcount2d(const bhc_problem2d *prob, int nsample, double *out);
+
 #ifdef __cplusplus
 }
 #endif

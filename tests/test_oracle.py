@@ -1,0 +1,437 @@
+"""Pins the oracle (oracle/bellhop2d.c) against what the mathematics fixes.
+
+The reference ships no 2D golden vectors (its only fixture,
+test/individual/analysis_3d/example_analysis_3d.arr.txt, is a 3D isovelocity
+arrivals file) and cannot be built here, so each part of the oracle is checked
+against closed forms, invariants and textbook results, chosen so that a
+plausible mistake (a dropped term, a wrong sign or index) fails one of them.
+
+What pins what (oracle function -> test):
+  RayInit launch direction, Step in isovelocity (x' = c t, q' = c p, tau' = 1/c),
+  boundary/box step reduction, flat Reflect (vacuum +pi, rigid 0)
+      -> test_isovelocity_rays_are_folded_straight_lines,
+         test_isovelocity_step_counts_closed_form
+  Step gradient terms (t' = -grad c / c^2) and the Heun blend
+      -> test_linear_gradient_ray_is_circular_arc
+  Range-independent invariance (t.x bitwise constant, flat reflections)
+      -> test_tx_invariant_range_independent
+  EvaluateSSP c-linear / n2-linear / spline (not-a-knot) / Munk analytic
+      -> test_ssp_values_along_rays
+  PCHIP: knot interpolation and monotonicity only (its derivative seeding
+  is the reference's own and no published routine reproduces it, so beyond
+  these properties it is parity unpinned)
+      -> test_pchip_properties
+  Reflect, acousto-elastic halfspace (fluid, cs = 0) -> Rayleigh coefficient
+      -> test_halfspace_reflection_rayleigh
+  Reflect, tabulated coefficient ('F') -> linear interpolation in grazing angle
+      -> test_reflection_table_interpolation
+  Influence geometric hat / Gaussian, Cartesian and ray-centred, coherent sum
+  and ScalePressure -> point source in free space, Lloyd mirror
+      -> test_free_space_point_source, test_lloyd_mirror
+  ScalePressure factors -> test_scale_pressure_closed_form
+  Incoherent / semicoherent sums -> test_incoherent_and_semicoherent_closed_form
+  Arrivals (AddArr) delay and amplitude in free space
+      -> test_free_space_arrivals
+  Eigen hits (RecordEigenHit) -> test_eigen_hits_bracket_receivers
+Parity unpinned (no closed form; held only by faithfulness to the reference
+source text): the PCHIP derivative seeding, curvilinear boundary interpolation
+('C' boundaries), the quad SSP, the attenuation-unit conversions other than
+dB/wavelength and the AddArr merge tie-breaks.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+import oracle
+from bellhopcuda_amd.env import Env2D, Halfspace, ReflTable, subtab, munk_c, munk_env
+
+
+def iso_env(D=1000.0, c=1500.0, zs=300.0, alpha=(-30.0, 30.0), nb=61, top="V", bot="R", rmax_km=5.0,
+            run_type="R", deltas=137.0, precision=64, rz=None, rr_km=None, freq=100.0, bot_hs=None):
+    return Env2D(
+        title="iso", precision=precision, freq0=freq, ssp_type="C",
+        ssp_z=np.array([0.0, D]), ssp_c=np.array([c, c]),
+        top=Halfspace(top, depth=0.0), bot=bot_hs or Halfspace(bot, depth=D),
+        Sz=np.array([zs], np.float32),
+        Rz=np.asarray(rz if rz is not None else [0.0], np.float32),
+        Rr_km=np.asarray(rr_km if rr_km is not None else [0.0, rmax_km], np.float64),
+        alpha_deg=subtab(alpha[0], alpha[1], nb), run_type=run_type, deltas=deltas,
+        box_r_km=rmax_km, box_z=D + 100.0)
+
+
+def _rays(env, max_points=6000):
+    pts, nst, ew = oracle.ray_run(env, max_points=max_points)
+    assert ew == (0, 0)
+    return [pts[i][:nst[i]] for i in range(len(nst))]
+
+
+def _fold(u, D):
+    u = np.mod(u, 2 * D)
+    return np.where(u > D, 2 * D - u, u)
+
+
+# --------------------------------------------------------------------------
+# straight rays
+# --------------------------------------------------------------------------
+
+def test_isovelocity_rays_are_folded_straight_lines():
+    D, zs, c = 1000.0, 300.0, 1500.0
+    env = iso_env(D=D, zs=zs, c=c, top="V", bot="R")
+    for a, r in zip(env.alpha_deg, _rays(env)):
+        al = math.radians(a)
+        x, z = r["x"][:, 0], r["x"][:, 1]
+        # unfolded depth along the launch direction, folded by the mirrors at 0 and D
+        assert np.allclose(z, _fold(zs + x * math.tan(al), D), rtol=0, atol=1e-9 * D)
+        s = x / math.cos(al)                     # path length
+        assert np.allclose(r["tau"][:, 0], s / c, rtol=1e-12, atol=1e-15)
+        assert np.all(r["tau"][:, 1] == 0.0)
+        assert np.allclose(r["q"][:, 0], c * s, rtol=1e-12, atol=1e-9)   # q' = c p, p = 1
+        assert np.all(r["q"][:, 1] == 1.0) and np.all(r["p"][:, 0] == 1.0)
+        assert np.allclose(np.hypot(r["t"][:, 0], r["t"][:, 1]) * c, 1.0, rtol=1e-14)
+        assert np.all(r["Amp"] == 1.0)
+        # vacuum top adds pi per bounce, rigid bottom nothing
+        assert np.allclose(r["Phase"], math.pi * r["NumTopBnc"], rtol=0, atol=1e-12)
+        # bounce counts: boundary crossings of the unfolded line before the end
+        u_end = zs + x[-1] * math.tan(al)
+        k = np.arange(-50, 51)
+        crossings_top = np.sum(((2 * k * D) > min(zs, u_end)) & ((2 * k * D) < max(zs, u_end)))
+        crossings_bot = np.sum((((2 * k + 1) * D) > min(zs, u_end)) & (((2 * k + 1) * D) < max(zs, u_end)))
+        assert r["NumTopBnc"][-1] == crossings_top
+        assert r["NumBotBnc"][-1] == crossings_bot
+
+
+def _expected_points(alpha_deg, D, zs, ds, box_r):
+    """Points a straight ray stores: one per step of min(ds, distance to the
+    next boundary / the box edge), plus a duplicated point at each reflection."""
+    a = math.radians(alpha_deg)
+    ca, sa = math.cos(a), math.sin(a)
+    s_box = box_r / ca
+    events = []
+    if sa != 0.0:
+        k = math.floor(zs / D) + 1 if sa > 0 else math.ceil(zs / D) - 1
+        while True:
+            s = (k * D - zs) / sa
+            if s >= s_box:
+                break
+            events.append(s)
+            k += 1 if sa > 0 else -1
+    legs = np.diff([0.0] + events + [s_box])
+    assert np.all(np.abs(legs / ds - np.round(legs / ds)) > 1e-6), "ambiguous case"
+    steps = int(sum(math.ceil(leg / ds) for leg in legs))
+    return steps + 1 + len(events)
+
+
+@pytest.mark.parametrize("deltas,zs", [(137.0, 300.0), (61.0, 777.0)])
+def test_isovelocity_step_counts_closed_form(deltas, zs):
+    D = 1000.0
+    env = iso_env(D=D, zs=zs, deltas=deltas, nb=61, alpha=(-31.0, 29.0))
+    pts, nst, _ = oracle.ray_run(env, max_points=6000)
+    exp = [_expected_points(a, D, zs, deltas, 5000.0) for a in env.alpha_deg]
+    assert list(nst) == exp
+
+
+# --------------------------------------------------------------------------
+# curved rays
+# --------------------------------------------------------------------------
+
+def test_linear_gradient_ray_is_circular_arc():
+    """c(z) = c0 + g z: rays are circles centred at the depth where c would
+    vanish, radius 1 / (g xi) with xi = cos(theta)/c the ray parameter."""
+    c0, g, D = 1500.0, 0.05, 4000.0
+    env = Env2D(title="grad", precision=64, freq0=50.0, ssp_type="C",
+                ssp_z=np.array([0.0, D]), ssp_c=np.array([c0, c0 + g * D]),
+                top=Halfspace("V", depth=0.0), bot=Halfspace("R", depth=D),
+                Sz=np.array([1000.0], np.float32), Rz=np.array([0.0], np.float32),
+                Rr_km=np.array([0.0, 20.0]), alpha_deg=subtab(-20.0, 20.0, 9), run_type="R",
+                deltas=20.0, box_r_km=20.0, box_z=D + 100.0)
+    zc = -c0 / g
+    for r in _rays(env):
+        nb = r["NumTopBnc"] + r["NumBotBnc"]
+        seg = r[nb == 0]                          # the arc before the first bounce
+        xi = seg["t"][0, 0]
+        R = 1.0 / (g * xi)
+        # centre range from the launch point: (x0 - rc)^2 + (z0 - zc)^2 = R^2,
+        # on the side the launch direction fixes
+        t0, x0 = seg["t"][0], seg["x"][0]
+        dz = x0[1] - zc
+        rc = x0[0] + np.sign(t0[1]) * math.sqrt(max(R * R - dz * dz, 0.0))
+        dist = np.hypot(seg["x"][:, 0] - rc, seg["x"][:, 1] - zc)
+        assert np.max(np.abs(dist - R)) / R < 1e-7
+        # xi = t.x is conserved exactly (grad c has no range component)
+        assert np.all(seg["t"][:, 0] == xi)
+
+
+def test_tx_invariant_range_independent():
+    env = munk_env(nbeams=41, nrz=3, nrr=3, run_type="R")
+    for r in _rays(env, max_points=20000):
+        assert np.all(r["t"][:, 0] == r["t"][0, 0])
+
+
+# --------------------------------------------------------------------------
+# sound speed
+# --------------------------------------------------------------------------
+
+def _ssp_env(ssp_type, z, c, D):
+    return Env2D(title="ssp", precision=64, freq0=50.0, ssp_type=ssp_type,
+                 ssp_z=z, ssp_c=c, top=Halfspace("V", depth=0.0), bot=Halfspace("R", depth=D),
+                 Sz=np.array([900.0], np.float32), Rz=np.array([0.0], np.float32),
+                 Rr_km=np.array([0.0, 30.0]), alpha_deg=subtab(-14.0, 14.0, 15), run_type="R",
+                 deltas=97.0, box_r_km=30.0, box_z=D + 100.0)
+
+
+@pytest.mark.parametrize("ssp_type", ["C", "N", "S", "A"])
+def test_ssp_values_along_rays(ssp_type):
+    D = 5000.0
+    # the analytic profile still walks the tabulated layers for its step
+    # reduction (a single layer keeps every interface on a boundary)
+    z = np.linspace(0.0, D, 26 if ssp_type != "A" else 2)
+    c = munk_c(z)
+    env = _ssp_env(ssp_type, z, c, D)
+    for r in _rays(env, max_points=20000):
+        zz, cc = r["x"][:, 1], r["c"]
+        if ssp_type == "C":
+            ref = np.interp(zz, z, c)
+            tol = 1e-12
+        elif ssp_type == "N":      # 1/c^2 linear between knots
+            ref = 1.0 / np.sqrt(np.interp(zz, z, 1.0 / c ** 2))
+            tol = 1e-12
+        elif ssp_type == "S":      # cubic spline, not-a-knot ends
+            ref = CubicSpline(z, c, bc_type="not-a-knot")(zz)
+            tol = 1e-10
+        else:                      # Munk's canonical profile, closed form; the
+            # reference holds its constants 1300 and 0.00737 as floats
+            eps = float(np.float32(0.00737))
+            eta = 2.0 * (zz - 1300.0) / 1300.0
+            ref = 1500.0 * (1.0 + eps * (eta - 1.0 + np.exp(-eta)))
+            tol = 1e-12
+        assert np.max(np.abs(cc - ref) / ref) < tol
+
+
+def test_pchip_properties():
+    D = 3000.0
+    z = np.array([0.0, 200.0, 500.0, 800.0, 1500.0, 2200.0, 3000.0])
+    c = np.array([1540.0, 1520.0, 1495.0, 1490.0, 1492.0, 1510.0, 1530.0])
+    env = _ssp_env("P", z, c, D)
+    env.Sz = np.array([700.0], np.float32)
+    seen = 0
+    for r in _rays(env, max_points=20000):
+        zz, cc = r["x"][:, 1], r["c"]
+        for i in range(len(z) - 1):
+            m = (zz >= z[i]) & (zz <= z[i + 1])
+            if not m.any():
+                continue
+            seen += m.sum()
+            lo, hi = min(c[i], c[i + 1]), max(c[i], c[i + 1])
+            # shape preservation: between monotone neighbours the interpolant
+            # stays within the bracket of the two knot values
+            monotone = (i == 0 or (c[i] - c[i - 1]) * (c[i + 1] - c[i]) > 0) and \
+                       (i + 2 >= len(c) or (c[i + 2] - c[i + 1]) * (c[i + 1] - c[i]) > 0)
+            if monotone:
+                assert np.all(cc[m] >= lo - 1e-9) and np.all(cc[m] <= hi + 1e-9)
+            at = m & np.isin(zz, z)
+            for zk, ck in zip(zz[at], cc[at]):
+                assert abs(ck - c[np.where(z == zk)[0][0]]) < 1e-9
+    assert seen > 100
+
+
+# --------------------------------------------------------------------------
+# reflection
+# --------------------------------------------------------------------------
+
+def _first_bottom_bounce(r):
+    k = int(np.argmax(r["NumBotBnc"] == 1))
+    assert k > 0
+    return r[k - 1], r[k]
+
+
+def test_halfspace_reflection_rayleigh():
+    """Fluid halfspace: R = (rho2 kz1 - rho1 kz2) / (rho2 kz1 + rho1 kz2), with
+    the e^{+i omega t} convention of the field (contributions carry
+    e^{-i omega tau}), so the evanescent kz2 has a negative imaginary part."""
+    D, c1, c2, rho2, f = 1000.0, 1500.0, 1700.0, 1.5, 100.0
+    env = iso_env(D=D, c=c1, zs=500.0, alpha=(5.0, 85.0), nb=33, rmax_km=10.0, deltas=50.0, freq=f,
+                  bot_hs=Halfspace("A", depth=D, alphaR=c2, rho=rho2, zTemp=D))
+    w = 2 * math.pi * f
+    n_total = 0
+    for a, r in zip(env.alpha_deg, _rays(env)):
+        before, after = _first_bottom_bounce(r)
+        g = math.radians(a)
+        kx, kz1 = w * math.cos(g) / c1, w * math.sin(g) / c1
+        kz2 = np.sqrt(complex((w / c2) ** 2 - kx ** 2))
+        if kz2.imag > 0:
+            kz2 = -kz2
+        R = (rho2 * kz1 - kz2) / (rho2 * kz1 + kz2)
+        n_total += abs(R) > 0.999999
+        assert after["Amp"] / before["Amp"] == pytest.approx(abs(R), rel=1e-10)
+        assert after["Phase"] - before["Phase"] == pytest.approx(np.angle(R), abs=1e-10)
+    assert 0 < n_total < len(env.alpha_deg)   # both total and partial reflection covered
+
+
+def test_reflection_table_interpolation():
+    th = np.array([0.0, 20.0, 45.0, 90.0])
+    rr = np.array([1.0, 0.8, 0.5, 0.2])
+    ph = np.array([0.0, 10.0, 25.0, 40.0])         # degrees in the table
+    D = 1000.0
+    env = iso_env(D=D, zs=500.0, alpha=(3.0, 80.0), nb=17, rmax_km=12.0, deltas=50.0,
+                  bot_hs=Halfspace("F", depth=D))
+    env.brc = ReflTable(th, rr, ph)
+    for a, r in zip(env.alpha_deg, _rays(env)):
+        before, after = _first_bottom_bounce(r)
+        assert after["Amp"] / before["Amp"] == pytest.approx(np.interp(a, th, rr), rel=1e-10)
+        assert after["Phase"] - before["Phase"] == pytest.approx(math.radians(np.interp(a, th, ph)), abs=1e-10)
+
+
+# --------------------------------------------------------------------------
+# influence and field
+# --------------------------------------------------------------------------
+
+def _field_env(D, zs, rz, rr_km, nb, alpha, run_type, top="V", bot="V", freq=200.0):
+    return Env2D(title="fs", precision=64, freq0=freq, ssp_type="C", ssp_z=np.array([0.0, D]),
+                 ssp_c=np.array([1500.0, 1500.0]), top=Halfspace(top, depth=0.0), bot=Halfspace(bot, depth=D),
+                 Sz=np.array([zs], np.float32), Rz=np.asarray(rz, np.float32),
+                 Rr_km=np.asarray(rr_km, np.float64), alpha_deg=subtab(alpha[0], alpha[1], nb),
+                 run_type=run_type, deltas=0.0, box_r_km=rr_km[-1] * 1.01, box_z=D + 100.0)
+
+
+@pytest.mark.parametrize("run_type,tol_db", [("CG", 0.01), ("Cg", 0.01), ("CB", 0.5), ("Cb", 0.5)])
+def test_free_space_point_source(run_type, tol_db):
+    """Deep isovelocity water, receivers far from both boundaries: the scaled
+    field is -exp(-ikR)/R (BELLHOP's sign and time convention)."""
+    D, zs, f = 20000.0, 10000.0, 200.0
+    rz = subtab(9000.0, 11000.0, 41, np.float32)
+    rr = subtab(0.2, 3.0, 57)
+    env = _field_env(D, zs, rz, rr, 2001, (-80.0, 80.0), run_type, freq=f)
+    o = oracle.field_run(env, nthreads=0, stats=False)
+    p = oracle.scale_pressure(env, o.field)[0]
+    R = np.hypot(rr[None, :] * 1000.0, rz[:, None].astype(np.float64) - zs)
+    m = np.ones_like(R, bool)
+    if run_type[1] in "gb":
+        m[:, 0] = False      # ray-centred influence never reaches the first receiver range
+    if run_type[1] in "Bb":
+        m &= rr[None, :] >= 1.0   # Gaussian beams: far field only
+    tl = -20 * np.log10(np.abs(p[m]))
+    assert np.max(np.abs(tl - 20 * np.log10(R[m]))) < tol_db
+    if run_type[1] in "Gg":
+        k = 2 * math.pi * f / 1500.0
+        ph = np.angle(-p * np.exp(1j * k * R))[m]
+        assert np.max(np.abs(ph)) < 0.01
+
+
+def test_lloyd_mirror():
+    """Vacuum surface: p = -(e^{-ikR1}/R1 - e^{-ikR2}/R2) with the image source at -zs."""
+    zs, f = 50.0, 100.0
+    rz = subtab(10.0, 200.0, 39, np.float32)
+    rr = subtab(0.5, 5.0, 91)
+    env = _field_env(20000.0, zs, rz, rr, 4001, (-80.0, 80.0), "CG", freq=f)
+    o = oracle.field_run(env, nthreads=0, stats=False)
+    p = oracle.scale_pressure(env, o.field)[0]
+    k = 2 * math.pi * f / 1500.0
+    z = rz[:, None].astype(np.float64)
+    r = rr[None, :] * 1000.0
+    R1, R2 = np.hypot(r, z - zs), np.hypot(r, z + zs)
+    ref = -(np.exp(-1j * k * R1) / R1 - np.exp(-1j * k * R2) / R2)
+    m = np.abs(ref) * R1 > 0.3           # away from the interference nulls
+    assert m.sum() > 1000
+    d = np.abs(20 * np.log10(np.abs(p[m]) / np.abs(ref[m])))
+    assert d.max() < 0.01
+    assert np.max(np.abs(np.angle(p[m] * np.conj(ref[m])))) < 0.01
+
+
+def test_scale_pressure_closed_form():
+    """Geometric beams: cnst = -1, point source cylindrical spreading 1/sqrt(r),
+    zero at r = 0; incoherent runs take sqrt of the intensity first."""
+    rr = np.array([0.0, 0.5, 2.0, 7.5])
+    for rt in ("CG", "IG"):
+        env = iso_env(rz=[10.0, 20.0, 30.0], rr_km=rr, run_type=rt)
+        u = (np.arange(12).reshape(1, 3, 4) + 1.0 + 0.5j).astype(np.complex64)
+        if rt[0] == "I":
+            u = u.real.astype(np.complex64)
+        got = oracle.scale_pressure(env, u)
+        base = u if rt[0] == "C" else np.sqrt(u.real).astype(np.complex64)
+        fac = -1.0 / np.sqrt(np.where(rr == 0.0, np.inf, rr * 1000.0))
+        assert np.allclose(got, base * fac[None, None, :].astype(np.float32), rtol=1e-6, atol=0)
+
+
+def test_incoherent_and_semicoherent_closed_form():
+    """Incoherent runs add beam intensities: with hat beams (a partition of
+    unity) a vacuum-surface Lloyd mirror gives |p|^2 = 1/R1^2 + 1/R2^2.
+    Semicoherent runs weight each ray by the Lloyd factor 2 sin^2(k zs sin a)
+    and add intensities; under an impedance-matched surface (no reflection)
+    |p|^2 = 2 sin^2(k zs sin a) / R^2 with a the direct-path launch angle."""
+    f, k = 200.0, 2 * math.pi * 200.0 / 1500.0
+    rz = subtab(100.0, 400.0, 31, np.float32)
+    rr = subtab(0.5, 3.0, 51)
+    z = rz[:, None].astype(np.float64)
+    r = rr[None, :] * 1000.0
+    zs = 60.0
+    R1, R2 = np.hypot(r, z - zs), np.hypot(r, z + zs)
+    env = _field_env(20000.0, zs, rz, rr, 4001, (-80.0, 80.0), "IG", freq=f)
+    p = oracle.scale_pressure(env, oracle.field_run(env, stats=False).field)[0]
+    assert np.all(p.imag == 0)
+    ref = np.sqrt(1.0 / R1 ** 2 + 1.0 / R2 ** 2)
+    assert np.max(np.abs(20 * np.log10(np.abs(p) / ref))) < 0.01
+
+    zs = 20.0
+    R = np.hypot(r, z - zs)
+    env = _field_env(20000.0, zs, rz, rr, 4001, (-80.0, 80.0), "SG", freq=f)
+    env.top = Halfspace("A", depth=0.0, alphaR=1500.0, rho=1.0, zTemp=0.0)
+    p = oracle.scale_pressure(env, oracle.field_run(env, stats=False).field)[0]
+    sin_a = (z - zs) / R
+    ref = np.sqrt(2.0 * np.sin(k * zs * sin_a) ** 2) / R
+    m = np.abs(np.sin(k * zs * sin_a)) > 0.3
+    assert m.sum() > 500
+    assert np.max(np.abs(20 * np.log10(np.abs(p[m]) / ref[m]))) < 0.05
+
+
+# --------------------------------------------------------------------------
+# arrivals and eigen hits
+# --------------------------------------------------------------------------
+
+def test_free_space_arrivals():
+    """One direct arrival per receiver with delay R/c (straight-ray travel
+    time to the receiver's projection on the beam, within the beam width)."""
+    D, zs = 20000.0, 10000.0
+    rz = subtab(9500.0, 10500.0, 11, np.float32)
+    rr = subtab(1.0, 3.0, 11)
+    env = _field_env(D, zs, rz, rr, 801, (-40.0, 40.0), "AG", freq=200.0)
+    o = oracle.field_run(env, stats=False, max_narr=20, allow_merging=1)
+    arr = np.ctypeslib.as_array(o.arr).reshape(1, len(rz), len(rr), 20)
+    narr = o.narr.reshape(1, len(rz), len(rr))
+    R = np.hypot(rr[None, :] * 1000.0, rz[:, None].astype(np.float64) - zs)
+    assert np.all(narr >= 1)
+    for iz in range(len(rz)):
+        for ir in range(len(rr)):
+            a = arr[0, iz, ir, :narr[0, iz, ir]]
+            assert np.all(a["NTopBnc"] == 0) and np.all(a["NBotBnc"] == 0)
+            assert np.allclose(a["delay"][:, 0], R[iz, ir] / 1500.0, rtol=1e-5)
+            decl = math.degrees(math.atan2(rz[iz] - zs, rr[ir] * 1000.0))
+            assert np.all(np.abs(a["SrcDeclAngle"] - decl) < 0.5)
+
+
+def test_eigen_hits_bracket_receivers():
+    """Each eigen hit names a ray step whose segment brackets the receiver
+    range and passes within the beam width of the receiver."""
+    D, zs = 20000.0, 10000.0
+    rz = subtab(9800.0, 10200.0, 5, np.float32)
+    rr = subtab(1.0, 2.0, 5)
+    env = _field_env(D, zs, rz, rr, 201, (-20.0, 20.0), "EG", freq=200.0)
+    o = oracle.field_run(env, stats=False, eigen_memsize=100000)
+    n = int(o.neigen[0])
+    assert n > 0
+    hits = np.ctypeslib.as_array(o.hits)[:n]
+    env_r = _field_env(D, zs, rz, rr, 201, (-20.0, 20.0), "R", freq=200.0)
+    rays = _rays(env_r)
+    dalpha = math.radians(40.0 / 200)
+    for h in hits:
+        r = rays[h["ialpha"]]
+        xa, xb = r["x"][h["is_"]], r["x"][h["is_"] + 1]
+        rec = np.array([rr[h["ir"]] * 1000.0, float(rz[h["iz"]])])
+        assert min(xa[0], xb[0]) <= rec[0] <= max(xa[0], xb[0])
+        d = xb - xa
+        u = d / np.linalg.norm(d)
+        normal_dist = abs(u[0] * (rec - xa)[1] - u[1] * (rec - xa)[0])
+        # hat beam half-width in isovelocity: q / q0 = (distance travelled) * dalpha
+        width = np.linalg.norm(rec - np.array([0.0, zs])) * dalpha
+        assert normal_dist <= width * 1.01
