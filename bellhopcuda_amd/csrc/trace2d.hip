@@ -10,10 +10,19 @@
 //  * the SSP knots / coefficients and the boundary nodes of the problem are
 //    staged in LDS once per workgroup;
 //  * after each step, every lane reduces its step segment to a descriptor
-//    (SegDesc) and the wavefront scatters the segments' receiver work items
-//    cooperatively: the descriptor of lane j is broadcast with readlane into
-//    scalar registers and the 64 lanes split its receivers. The receiver
-//    loops, not the integrator, dominate dense TL grids.
+//    (SegDesc: the receiver range / depth window it can reach and the
+//    interpolation constants of the influence) and appends it to a segment
+//    buffer in HBM (one atomic per wavefront).
+// A second phase scatters the segments' receiver work items:
+//  * k_scatter2d: one wavefront per segment, descriptor in scalar registers,
+//    the 64 lanes split its receivers, FP32 atomics into the field;
+//  * tiled TL path: segments are counting-sorted into receiver-range bins
+//    (k_bin_count / k_bin_scan / k_bin_fill); k_scatter_tile accumulates one
+//    range tile of the field in LDS and flushes it with one atomic per
+//    receiver, which keeps the dense grids' atomics out of L2.
+// A full segment buffer ends the pass; the host splits the ray range and
+// traces the rest in further passes (the field is only written by the
+// scatter, so the split leaves the result unchanged).
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstring>
