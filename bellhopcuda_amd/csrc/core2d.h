@@ -916,7 +916,9 @@ template<typename real> struct SegDesc {
     v2<real> xa, ta, xb, tb;
     real rlen, zmin, zmax, lo, hi;
     // beam quantities of InfluenceGeoCore
-    real q0x, dqx, p0c, p1c, p1Amp, phasePt, phase, qOld, p1taure, Ratio1, rcp_q0, RcvrDecl, SrcDecl;
+    real q0x, dqx, phasePt, phase, qOld, rcp_q0, RcvrDecl, SrcDecl;
+    real A0;                    // Ratio1 * sqrt(c1) * Amp1: numerator of the contribution amplitude
+    real sigfloor;              // Gaussian beams: lower bound of the beam width (AdjustSigma)
     real wmax;                  // upper bound of BeamWindow * sigma over the segment
     cpx<real> tau0, dtau;
     int32_t ir0, ir1, dir;      // Cartesian: visited receiver ranges ir0..ir1 (step dir)
@@ -935,14 +937,13 @@ __host__ __device__ inline void desc_common(SegDesc<real> &d, const RayPt<real> 
     d.dqx = p1.q.x - p0.q.x;
     d.tau0 = p0.tau;
     d.dtau = p1.tau - p0.tau;
-    d.p0c = p0.c;
-    d.p1c = p1.c;
-    d.p1Amp = p1.Amp;
+    // per-segment factors of InfluenceGeoCore (influence.hpp:970-992), in the
+    // reference's operation order so that every contribution is bit-identical
+    d.A0 = I.Ratio1 * std::sqrt(p1.c) * p1.Amp;
+    d.sigfloor = P.isGaussian ? rmin(FL(0.2) * P.freq0 * p1.tau.re, RC<real>::PI * (p0.c / P.freq0)) : RL(0.0);
     d.phasePt = P.isGaussian ? p1.Phase : p0.Phase;
     d.phase = I.phase;
     d.qOld = I.qOld;
-    d.p1taure = p1.tau.re;
-    d.Ratio1 = I.Ratio1;
     d.rcp_q0 = I.rcp_q0;
     d.SrcDecl = I.SrcDeclAngle;
     d.ntop = p1.NumTopBnc;
@@ -1078,8 +1079,25 @@ __host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<rea
     I.rayn1 = rayn1;
 }
 
+// cos and sin of an FP64 phase at FP32 accuracy: Cody-Waite reduction by 2*pi
+// in FP64 (two-part constant, exact to ~1e-11 rad for |a| < 1e6), then the FP32
+// sincos of the remainder in [-pi, pi].
+__host__ __device__ inline void sincos_phase(double a, float &s, float &c)
+{
+    const double INV_2PI = 0.15915494309189535;
+    const double TWO_PI_HI = 6.283185307179586;       // 2*pi rounded to double
+    const double TWO_PI_LO = 2.4492935982947064e-16;  // 2*pi - TWO_PI_HI
+    double k = std::rint(a * INV_2PI);
+    double r = (a - k * TWO_PI_HI) - k * TWO_PI_LO;
+    sincosf((float)r, &s, &c);
+}
+
 // InfluenceGeoCore (influence.hpp:897-1024, 2D) + ApplyContribution (:432-473).
-template<typename real, int RT, typename Sink>
+// FAST = false follows the reference's arithmetic to the last bit. FAST = true
+// evaluates the final TL term -- a unit phasor and Gaussian weight that are
+// then rounded to the FP32 field -- at FP32 accuracy: the phase is reduced in
+// FP64 first, the geometry, delay and amplitude stay in `real`.
+template<typename real, int RT, typename Sink, bool FAST = false>
 __host__ __device__ inline void contrib(const SegDesc<real> &d, real s, real n1, int ir, int iz, const Prob2D<real> &P,
                                         Sink &sink)
 {
@@ -1087,19 +1105,16 @@ __host__ __device__ inline void contrib(const SegDesc<real> &d, real s, real n1,
     real qFinal = d.q0x + s * d.dqx;
     real sigma, sigma_orig;
     sigma = sigma_orig = std::fabs(qFinal * d.rcp_q0);
-    if(P.isGaussian) {
-        real lambda = d.p0c / P.freq0;
-        sigma = rmax(sigma, rmin(FL(0.2) * P.freq0 * d.p1taure, RC<real>::PI * lambda));
-    }
+    if(P.isGaussian) sigma = rmax(sigma, d.sigfloor);
     real BW = P.isGaussian ? RL(4.0) : RL(1.0);
     if(n1 > BW * sigma || n1 == BW * sigma) return;
     cpx<real> delay = d.tau0 + rmul(s, d.dtau);
-    real cfactor = std::sqrt(d.p1c);
-    real cnst = d.Ratio1 * cfactor * d.p1Amp / std::sqrt(std::fabs(qFinal));
+    real cnst = d.A0 / std::sqrt(std::fabs(qFinal));
     real w;
     if(P.isGaussian) {
         real t = n1 / sigma;
-        w = std::exp(FL(-0.5) * (t * t)) * (sigma_orig / sigma);
+        if constexpr(FAST) w = (real)expf((float)(FL(-0.5) * (t * t))) * (sigma_orig / sigma);
+        else w = std::exp(FL(-0.5) * (t * t)) * (sigma_orig / sigma);
     } else {
         w = (sigma - n1) / sigma;
     }
@@ -1112,12 +1127,23 @@ __host__ __device__ inline void contrib(const SegDesc<real> &d, real s, real n1,
         if(P.run0 == 'C') {
             cpx<real> arg = rmul(omega, delay);
             arg.re = arg.re - phaseInt;
-            cpx<real> ex = cexp_(C(arg.im, -arg.re)); // std::exp(-J * arg)
-            cpx<real> dd = rmul(cnst * w, ex);
-            fr = (float)dd.re;
-            fi = (float)dd.im;
+            if constexpr(FAST) {
+                // e^{-i arg} = e^{arg.im} (cos arg.re - i sin arg.re)
+                float sn, cs;
+                sincos_phase((double)arg.re, sn, cs);
+                float m = (float)(cnst * w) * expf((float)arg.im);
+                fr = m * cs;
+                fi = -(m * sn);
+            } else {
+                cpx<real> ex = cexp_(C(arg.im, -arg.re)); // std::exp(-J * arg)
+                cpx<real> dd = rmul(cnst * w, ex);
+                fr = (float)dd.re;
+                fi = (float)dd.im;
+            }
         } else {
-            real v = cnst * std::exp(omega * delay.im);
+            real v;
+            if constexpr(FAST) v = cnst * (real)expf((float)(omega * delay.im));
+            else v = cnst * std::exp(omega * delay.im);
             v = (v * v) * w;
             if(P.isGaussian) v *= std::sqrt(FL(2.0) * RC<real>::PI);
             fr = (float)v;
@@ -1142,7 +1168,7 @@ template<typename real> __host__ __device__ inline int RToIR(real r, const Prob2
 }
 
 // Cartesian work item k -> (ir, iz): influence.hpp:1298-1386 (2D).
-template<typename real, int RT, typename Sink>
+template<typename real, int RT, typename Sink, bool FAST = false>
 __host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
                                           const float *Rz, Sink &sink)
 {
@@ -1156,17 +1182,18 @@ __host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const P
     v2<real> xrr = V(rr, zr) - d.xa;
     real s = dot(xrr, d.ta) / d.rlen;
     real n1 = std::fabs(dot(xrr, d.tb));
-    contrib<real, RT>(d, s, n1, ir, iz, P, sink);
+    contrib<real, RT, Sink, FAST>(d, s, n1, ir, iz, P, sink);
 }
 
-// Ray-centred work item = one receiver depth: influence.hpp:1075-1187 (2D).
-template<typename real, int RT, typename Sink>
-__host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, const Prob2D<real> &P, const real *G,
-                                            const float *Rz, Sink &sink)
+// Ray-centred receiver depth iz (influence.hpp:1075-1187, 2D): the normals at
+// A and B cross the depth at ranges rA, rB (normal distances nA, nB); the
+// candidate receivers are ir = blo..bhi. Returns false when there are none.
+template<typename real>
+__host__ __device__ inline bool raycen_range(const SegDesc<real> &d, int iz, const Prob2D<real> &P, const real *Rr,
+                                             const float *Rz, real &nA, real &rA, real &nB, real &rB, int &blo,
+                                             int &bhi)
 {
-    const real *Rr = G + P.o_Rr;
     real zR = (real)Rz[iz];
-    real nA, rA, nB, rB;
     int irA, irB;
     if(!d.lastValid) {
         nA = RL(1e10);
@@ -1180,13 +1207,14 @@ __host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, cons
     nB = (zR - d.xb.y) / d.tb.y;
     rB = d.xb.x + nB * d.tb.x;
     irB = RToIR(rB, P, Rr);
-    if(irA == irB) return;
+    if(irA == irB) return false;
     int notii = (irB <= irA) ? 0 : 1;
     int step = (notii << 1) - 1;
     // The reference visits ir = irA+notii ... irB+notii-step and rejects most of
     // them in the beam-window test. Receivers outside a conservative window
     // (|n| <= wmax, with slack) cannot pass that test and are not visited.
-    int blo = irA + notii, bhi = irB + notii - step;
+    blo = irA + notii;
+    bhi = irB + notii - step;
     if(blo > bhi) {
         int t = blo;
         blo = bhi;
@@ -1196,7 +1224,7 @@ __host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, cons
         double dn = (double)nB - (double)nA;
         double W = (double)d.wmax + 1e-4 * (std::fabs((double)nA) + std::fabs((double)nB)) + 1e-30;
         if(dn == 0.0) {
-            if(std::fabs((double)nA) > W) return;
+            if(std::fabs((double)nA) > W) return false;
         } else {
             double s1 = (-W - (double)nA) / dn, s2 = (W - (double)nA) / dn;
             double slo = s1 < s2 ? s1 : s2, shi = s1 < s2 ? s2 : s1;
@@ -1214,14 +1242,32 @@ __host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, cons
             int c0 = (int)std::floor(t0) - 1, c1 = (int)std::ceil(t1) + 1;
             blo = imax(blo, c0);
             bhi = imin(bhi, c1);
-            if(blo > bhi) return;
+            if(blo > bhi) return false;
         }
     }
-    for(int ir = blo; ir <= bhi; ++ir) {
-        real s = (Rr[ir] - rA) / (rB - rA);
-        real n1 = std::fabs(nA + s * (nB - nA));
-        contrib<real, RT>(d, s, n1, ir, iz, P, sink);
-    }
+    return true;
+}
+
+// One ray-centred receiver (ir on depth iz with the crossing data above).
+template<typename real, int RT, typename Sink, bool FAST = false>
+__host__ __device__ inline void raycen_one(const SegDesc<real> &d, int ir, int iz, real nA, real rA, real nB, real rB,
+                                           const Prob2D<real> &P, const real *Rr, Sink &sink)
+{
+    real s = (Rr[ir] - rA) / (rB - rA);
+    real n1 = std::fabs(nA + s * (nB - nA));
+    contrib<real, RT, Sink, FAST>(d, s, n1, ir, iz, P, sink);
+}
+
+// Ray-centred work item = one receiver depth.
+template<typename real, int RT, typename Sink, bool FAST = false>
+__host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, const Prob2D<real> &P, const real *G,
+                                            const float *Rz, Sink &sink)
+{
+    const real *Rr = G + P.o_Rr;
+    real nA, rA, nB, rB;
+    int blo, bhi;
+    if(!raycen_range(d, iz, P, Rr, Rz, nA, rA, nB, rB, blo, bhi)) return;
+    for(int ir = blo; ir <= bhi; ++ir) raycen_one<real, RT, Sink, FAST>(d, ir, iz, nA, rA, nB, rB, P, Rr, sink);
 }
 
 template<typename real>
@@ -1232,12 +1278,12 @@ __host__ __device__ inline void prelude(SegDesc<real> &d, const RayPt<real> &p0,
     else prelude_cart(d, p0, p1, I, is, P, G, isz, ialpha);
 }
 
-template<typename real, int RT, typename Sink>
+template<typename real, int RT, typename Sink, bool FAST = false>
 __host__ __device__ inline void item(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
                                      const float *Rz, Sink &sink)
 {
-    if(P.isRayCen) item_raycen<real, RT>(d, d.iz0 + k, P, G, Rz, sink);
-    else item_cart<real, RT>(d, k, P, G, Rz, sink);
+    if(P.isRayCen) item_raycen<real, RT, Sink, FAST>(d, d.iz0 + k, P, G, Rz, sink);
+    else item_cart<real, RT, Sink, FAST>(d, k, P, G, Rz, sink);
 }
 
 // Job index -> (isz, ialpha), reference src/common.hpp:365-391 (2D).

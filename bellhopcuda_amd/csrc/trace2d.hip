@@ -165,26 +165,91 @@ __device__ inline void flush_err(const Err &e, const DevOut &O)
     if(e.nwarn) atomicAdd(O.errw + 3, e.nwarn);
 }
 
-// Append this lane's descriptor (if it has work items) to the segment buffer
-// with one atomic per wavefront. Returns false when the buffer is full (the
-// whole pass is then discarded by the host and the ray range split).
+// Segment slots are reserved SEG_BLOCK at a time per wavefront (one atomic per
+// block instead of one per step). Slots of a block that stay unused are marked
+// empty (nitems = 0) and skipped by the scatter phase.
+constexpr uint32_t SEG_BLOCK = 256;
+
+struct SegBlock {
+    uint32_t base, used, cap;
+};
+
+template<typename real> __device__ inline void seal_block(const SegBlock &B, int lane, const DevOut &O)
+{
+    Seg<real> *segs = reinterpret_cast<Seg<real> *>(O.segs);
+    for(uint32_t k = B.used + (uint32_t)lane; k < B.cap; k += WAVE) {
+        uint32_t slot = B.base + k;
+        if(slot < O.seg_cap) segs[slot].d.nitems = 0;
+    }
+}
+
+// Append this lane's descriptor (if it has work items) to the segment buffer.
+// Called by all lanes of the wavefront. A slot beyond the buffer sets the
+// overflow flag: the host then discards the pass and splits the ray range.
 template<typename real>
-__device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut &O)
+__device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut &O, SegBlock &B)
 {
     bool has = d.nitems > 0;
     unsigned long long m = __ballot(has);
     if(!m) return;
-    int cnt = __popcll(m);
-    int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if(lane == leader) base = atomicAdd(O.seg_count, (uint32_t)cnt);
-    base = __shfl(base, leader);
+    uint32_t cnt = (uint32_t)__popcll(m);
+    if(B.used + cnt > B.cap) {
+        seal_block<real>(B, lane, O);
+        uint32_t base = 0;
+        if(lane == 0) base = atomicAdd(O.seg_count, SEG_BLOCK);
+        B.base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+        B.used = 0;
+        B.cap = SEG_BLOCK;
+    }
     if(has) {
-        uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        uint32_t slot = B.base + B.used + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if(slot < O.seg_cap) {
             reinterpret_cast<Seg<real> *>(O.segs)[slot].d = d;
         } else {
             atomicOr(O.overflow, 1u);
+        }
+    }
+    B.used += cnt;
+}
+
+// Scatter one segment's receivers with the 64 lanes of a wavefront (d is
+// wavefront-uniform). Cartesian items are (range, depth) pairs already.
+// Ray-centred items are depths with different numbers of candidate ranges, so
+// the (depth, range) pairs of up to 64 depths are flattened over the lanes:
+// a prefix sum of the per-depth counts, then each lane finds the depth of its
+// pair by a binary search over the lanes' prefix values.
+template<typename real, int RT, bool FAST, typename Sink>
+__device__ inline void scatter_seg(const SegDesc<real> &d, const Prob2D<real> &P, Sink &sink, int lane)
+{
+    if(!P.isRayCen) {
+        for(int it = lane; it < d.nitems; it += WAVE) item_cart<real, RT, Sink, FAST>(d, it, P, P.base, P.Rz, sink);
+        return;
+    }
+    const real *Rr = P.base + P.o_Rr;
+    for(int g = 0; g < d.nz; g += WAVE) {
+        const int iz = d.iz0 + g + lane;
+        real nA = 0, rA = 0, nB = 0, rB = 1;
+        int blo = 0, bhi = -1;
+        if(g + lane < d.nz && !raycen_range(d, iz, P, Rr, P.Rz, nA, rA, nB, rB, blo, bhi)) bhi = blo - 1;
+        const int cnt = bhi - blo + 1;
+        int incl = cnt;
+        for(int off = 1; off < WAVE; off <<= 1) {
+            int v = __shfl_up(incl, off);
+            if(lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, WAVE - 1);
+        const int excl = incl - cnt;
+        for(int t0 = 0; t0 < total; t0 += WAVE) {
+            const int t = t0 + lane;
+            int own = 0;
+            for(int b = WAVE / 2; b >= 1; b >>= 1) {
+                int probe = own + b < WAVE ? own + b : WAVE - 1;
+                int e = __shfl(excl, probe);
+                if(own + b < WAVE && e <= t) own += b;
+            }
+            const real onA = __shfl(nA, own), orA = __shfl(rA, own), onB = __shfl(nB, own), orB = __shfl(rB, own);
+            const int oblo = __shfl(blo, own), oexcl = __shfl(excl, own), oiz = __shfl(iz, own);
+            if(t < total) raycen_one<real, RT, Sink, FAST>(d, oblo + (t - oexcl), oiz, onA, orA, onB, orB, P, Rr, sink);
         }
     }
 }
@@ -207,6 +272,7 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
     int job = 0, isz = 0, ia = 0, is = 0;
     bool active = false, exhausted = false;
     unsigned long long steps = 0;
+    SegBlock blk{0u, 0u, 0u};
     for(int iter = 0;; ++iter) {
         if(!exhausted) {
             unsigned long long need = __ballot(!active);
@@ -261,14 +327,14 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
             prelude(d, p0, p1, I, is, P, G, isz, ia);
             ++is;
         }
-        append_seg(d, lane, O);
+        append_seg(d, lane, O, blk);
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
                 prelude(d, p1, p2, I, is, P, G, isz, ia);
                 ++is;
             }
-            append_seg(d, lane, O);
+            append_seg(d, lane, O, blk);
         }
         if(active) {
             p0 = two ? p2 : p1;
@@ -286,6 +352,7 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
             }
         }
     }
+    seal_block<real>(blk, lane, O);
     flush_err(e, O);
     for(int off = WAVE / 2; off > 0; off >>= 1) steps += __shfl_xor(steps, off);
     if(lane == 0) atomicAdd(O.counters + 0, steps);
@@ -295,7 +362,7 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
 // uniform (scalar loads) and the 64 lanes split its receiver work items
 // (influence.hpp:1075-1187 ray-centred depths, :1298-1386 Cartesian ranges x
 // depths).
-template<typename real, int RT>
+template<typename real, int RT, bool FAST>
 __global__ void __launch_bounds__(256) k_scatter2d(Prob2D<real> P, DevOut O)
 {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -307,7 +374,7 @@ __global__ void __launch_bounds__(256) k_scatter2d(Prob2D<real> P, DevOut O)
     const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
         const SegDesc<real> d = segs[s].d;
-        for(int k = lane; k < d.nitems; k += WAVE) item<real, RT>(d, k, P, P.base, P.Rz, sink);
+        if(d.nitems > 0) scatter_seg<real, RT, FAST>(d, P, sink, lane);
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
     if(lane == 0 && sink.nc) atomicAdd(O.counters + 1, sink.nc);
@@ -360,7 +427,8 @@ template<typename real> __global__ void __launch_bounds__(256) k_bin_count(Prob2
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x) h[b] = 0;
     __syncthreads();
     const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
-    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x)
+        if(segs[s].d.nitems > 0) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
     __syncthreads();
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x)
         if(h[b]) atomicAdd(C.counts + b, h[b]);
@@ -418,12 +486,14 @@ template<typename real> __global__ void __launch_bounds__(256) k_bin_fill(Prob2D
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x) h[b] = 0;
     __syncthreads();
     const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
-    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+    for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x)
+        if(segs[s].d.nitems > 0) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
     __syncthreads();
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x)
         if(h[b]) h[b] = atomicAdd(C.cursor + b, h[b]);
     __syncthreads();
     for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        if(segs[s].d.nitems <= 0) continue;
         int pos = atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
         C.idx[pos] = s;
     }
@@ -456,8 +526,10 @@ template<typename real> struct TileSink {
 // One workgroup per (bin, chunk); the field of source isz lives at
 // P.field + isz * plane, so the tile is per source: segments of other
 // sources fall back to global atomics (single-source runs are the common case).
-template<typename real>
-__global__ void __launch_bounds__(256) k_scatter_tile(Prob2D<real> P, DevOut O, BinCfg C)
+constexpr int TILE_THREADS = 512;
+
+template<typename real, bool FAST>
+__global__ void __launch_bounds__(TILE_THREADS, 4) k_scatter_tile(Prob2D<real> P, DevOut O, BinCfg C)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float *tile = reinterpret_cast<float *>(lds_raw);
@@ -484,11 +556,11 @@ __global__ void __launch_bounds__(256) k_scatter_tile(Prob2D<real> P, DevOut O, 
             const SegDesc<real> d = segs[si].d;
             if(d.isz != 0) {
                 DevSink<real> gs{&O, &P, 0};
-                for(int it = lane; it < d.nitems; it += WAVE) item<real, RT_TL>(d, it, P, P.base, P.Rz, gs);
+                scatter_seg<real, RT_TL, FAST>(d, P, gs, lane);
                 sink.nc += gs.nc;
                 continue;
             }
-            for(int it = lane; it < d.nitems; it += WAVE) item<real, RT_TL>(d, it, P, P.base, P.Rz, sink);
+            scatter_seg<real, RT_TL, FAST>(d, P, sink, lane);
         }
         __syncthreads();
         // flush: tile column c, depth iz -> field (isz 0)
@@ -708,6 +780,7 @@ struct bhc_ctx {
     uint32_t *d_bin_idx;        // binned segment order (seg capacity entries)
     int passes;                 // trace/scatter passes of the last launch
     int use_tiles;              // 0 disables the LDS-tiled TL scatter
+    int exact_phasor;           // 1: TL phasor in the reference's full arithmetic
 };
 
 struct bhc_dev_problem {
@@ -772,6 +845,8 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     c->bpc = 2;
     c->use_tiles = 1;
     if(const char *e = getenv("BHC_TL_TILES")) c->use_tiles = atoi(e) != 0;
+    c->exact_phasor = 0;
+    if(const char *e = getenv("BHC_EXACT_PHASOR")) c->exact_phasor = atoi(e) != 0;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));
     HIPCHECK(hipEventCreate(&c->ev1));
@@ -1126,10 +1201,18 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
             hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, B);
             hipLaunchKernelGGL((k_bin_fill<real>), dim3(gb), dim3(256), hl, s, P, O, B);
             size_t lds = (size_t)dp->tile_w * dp->NRz_per_range * 2 * sizeof(float);
-            hipLaunchKernelGGL((k_scatter_tile<real>), dim3(c->num_cus * 4), dim3(256), lds, s, P, O, B);
+            if(c->exact_phasor)
+                hipLaunchKernelGGL((k_scatter_tile<real, false>), dim3(c->num_cus * 4), dim3(TILE_THREADS), lds, s,
+                                   P, O, B);
+            else
+                hipLaunchKernelGGL((k_scatter_tile<real, true>), dim3(c->num_cus * 4), dim3(TILE_THREADS), lds, s,
+                                   P, O, B);
         } else {
             int sblocks = c->num_cus * 8;
-            hipLaunchKernelGGL((k_scatter2d<real, RT>), dim3(sblocks), dim3(256), 0, s, P, O);
+            if(RT == RT_TL && !c->exact_phasor)
+                hipLaunchKernelGGL((k_scatter2d<real, RT, true>), dim3(sblocks), dim3(256), 0, s, P, O);
+            else
+                hipLaunchKernelGGL((k_scatter2d<real, RT, false>), dim3(sblocks), dim3(256), 0, s, P, O);
         }
         HIPCHECK(hipGetLastError());
         ++c->passes;

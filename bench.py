@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (the reference kernel for "
+                         "scaling profiler byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the tracer has no multi-GPU path (problems and ray ranges would be "
@@ -80,6 +83,14 @@ def main():
     L = bh.lib()
     L.bhc_last_phase_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.bhc_opcount2d.argtypes = [C.POINTER(_abi.Problem2D), C.c_int, C.POINTER(C.c_double)]
+
+    if a.copy_calibration_mb > 0:
+        import torch
+        src = torch.ones(a.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     env, desc = workload(a.workload, a.precision)
     prob = bh.Problem(env)

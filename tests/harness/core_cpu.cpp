@@ -32,7 +32,7 @@ struct HostSink {
     void mark() {}
 };
 
-template<typename real> int run(const bhc_problem2d *prob, bhc_field_out *out)
+template<typename real, bool FAST> int run(const bhc_problem2d *prob, bhc_field_out *out)
 {
     std::vector<real> blob;
     Prob2D<real> P;
@@ -69,11 +69,11 @@ template<typename real> int run(const bhc_problem2d *prob, bhc_field_out *out)
             bool two = RayUpdate(p0, p1, p2, S, P, B, G, e);
             SegDesc<real> d;
             prelude(d, p0, p1, I, is, P, G, isz, ia);
-            for(int k = 0; k < d.nitems; ++k) item<real, RT_TL>(d, k, P, G, prob->Rz, sink);
+            for(int k = 0; k < d.nitems; ++k) item<real, RT_TL, HostSink, FAST>(d, k, P, G, prob->Rz, sink);
             ++is;
             if(two) {
                 prelude(d, p1, p2, I, is, P, G, isz, ia);
-                for(int k = 0; k < d.nitems; ++k) item<real, RT_TL>(d, k, P, G, prob->Rz, sink);
+                for(int k = 0; k < d.nitems; ++k) item<real, RT_TL, HostSink, FAST>(d, k, P, G, prob->Rz, sink);
                 ++is;
             }
             p0 = two ? p2 : p1;
@@ -100,7 +100,13 @@ template<typename real> int run(const bhc_problem2d *prob, bhc_field_out *out)
 
 extern "C" int core_field_run2d(const bhc_problem2d *prob, bhc_field_out *out)
 {
-    return prob->precision == 64 ? run<double>(prob, out) : run<float>(prob, out);
+    return prob->precision == 64 ? run<double, false>(prob, out) : run<float, false>(prob, out);
+}
+
+// The same with the FP32-accuracy TL phasor the GPU scatter uses by default.
+extern "C" int core_field_run2d_fast(const bhc_problem2d *prob, bhc_field_out *out)
+{
+    return prob->precision == 64 ? run<double, true>(prob, out) : run<float, true>(prob, out);
 }
 
 namespace {

@@ -73,3 +73,56 @@ def test_ray_mode_parity(gpu_ctx):
         assert np.array_equal(xo, xg), k
         assert np.array_equal(pts_o[k, :n]["NumTopBnc"], pts_g[k, :n]["NumTopBnc"])
         assert np.array_equal(pts_o[k, :n]["NumBotBnc"], pts_g[k, :n]["NumBotBnc"])
+
+
+def _arrivals(res, shape, max_narr):
+    a = np.ctypeslib.as_array(res.arr).reshape(shape + (max_narr,))
+    return a, res.narr.reshape(shape)
+
+
+ARR_CASES = {
+    "munk_arrivals_hat": lambda: munk_env(nbeams=301, nrz=11, nrr=21, run_type="AG"),
+    "munk_arrivals_gauss_raycen": lambda: munk_env(nbeams=301, nrz=11, nrr=21, run_type="Ab"),
+    "seamount_arrivals": lambda: c2_env(nbeams=401, nrz=21, nrr=101, run_type="Ab"),
+    "seamount_arrivals_f32": lambda: c2_env(nbeams=401, nrz=21, nrr=101, run_type="Ab", precision=32),
+}
+
+
+@pytest.mark.parametrize("name", list(ARR_CASES))
+def test_arrivals_parity(name, gpu_ctx):
+    """Arrivals with merging: the GPU merge replays the single-threaded AddArr
+    order (rays in job order), so it is compared with the oracle run on one
+    thread: equal counts per receiver, equal bounce counts, delays and
+    amplitudes to FP32 rounding."""
+    _announce(name)
+    env = ARR_CASES[name]()
+    max_narr = 100
+    o = oracle.field_run(env, nthreads=1, max_narr=max_narr, allow_merging=1)
+    g = bh.field_run(env, ctx=gpu_ctx, stats=True, max_narr=max_narr, allow_merging=1)
+    assert (g.err, g.warn) == (o.err, o.warn)
+    assert np.array_equal(g.nsteps, o.nsteps)
+    shape = env.field_shape
+    ao, no = _arrivals(o, shape, max_narr)
+    ag, ng = _arrivals(g, shape, max_narr)
+    assert np.array_equal(no, ng)
+    assert no.sum() > 0
+    for idx in zip(*np.nonzero(no)):
+        n = no[idx]
+        x, y = ao[idx][:n], ag[idx][:n]
+        assert np.array_equal(x["NTopBnc"], y["NTopBnc"]) and np.array_equal(x["NBotBnc"], y["NBotBnc"])
+        assert np.allclose(x["delay"], y["delay"], rtol=1e-6, atol=1e-9)
+        assert np.allclose(x["a"], y["a"], rtol=1e-5, atol=1e-12)
+        assert np.allclose(x["SrcDeclAngle"], y["SrcDeclAngle"], rtol=1e-6, atol=1e-6)
+
+
+def test_eigen_hits_parity(gpu_ctx):
+    """Eigen hits as an order-free set (the reference appends them with an
+    atomic counter, so their order is not defined)."""
+    _announce("munk_eigen")
+    env = munk_env(nbeams=201, nrz=5, nrr=11, run_type="EG")
+    o = oracle.field_run(env, nthreads=0, eigen_memsize=200000)
+    g = bh.field_run(env, ctx=gpu_ctx, eigen_memsize=200000)
+    no, ng = int(o.neigen[0]), int(g.neigen[0])
+    assert no == ng and no > 0
+    key = lambda h: sorted(zip(h["ialpha"], h["is_"], h["ir"], h["iz"], h["isz"]))  # noqa: E731
+    assert key(np.ctypeslib.as_array(o.hits)[:no]) == key(np.ctypeslib.as_array(g.hits)[:ng])
