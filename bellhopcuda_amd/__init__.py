@@ -58,6 +58,15 @@ def lib():
         "bhc_problem_release": (None, [C.c_void_p]),
         "bhc_scale_pressure2d": (C.c_int, [P(_abi.Problem2D), _abi.c_float_p]),
         "bhc_write_shd2d": (C.c_int, [C.c_char_p, P(_abi.Problem2D), C.c_char_p, _abi.c_float_p]),
+        "bhc_sweep_upload2d": (C.c_int, [C.c_void_p, P(P(_abi.Problem2D)), C.c_int, C.c_int, P(C.c_void_p)]),
+        "bhc_sweep_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "bhc_sweep_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, P(_abi.FieldOut)]),
+        "bhc_sweep_stats": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), P(C.c_longlong),
+                                      P(C.c_longlong)]),
+        "bhc_sweep_release": (None, [C.c_void_p]),
+        "bhc_last_phase_ms": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int)]),
+        "bhc_opcount2d": (C.c_int, [P(_abi.Problem2D), C.c_int, P(C.c_double)]),
+        "bhc_subtab": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -249,3 +258,47 @@ def ray_run(env_or_prob, ctx=None, max_points=20000, ialpha=None, isz=None, nste
             ctx.close()
     arr = np.ctypeslib.as_array(pts).reshape(n, max_points)
     return arr, nst, (out.err, out.warn)
+
+
+class Sweep:
+    """Many TL problems traced in one persistent launch (bhc_sweep_*)."""
+
+    def __init__(self, problems, ctx, stats=False):
+        self.problems = list(problems)
+        self.ctx = ctx
+        n = len(self.problems)
+        arr = (C.POINTER(_abi.Problem2D) * n)(*[C.pointer(p.view) for p in self.problems])
+        h = C.c_void_p()
+        _check(lib().bhc_sweep_upload2d(ctx._h, arr, n, 1 if stats else 0, C.byref(h)))
+        self._h = h
+        self.with_stats = stats
+
+    def launch(self):
+        _check(lib().bhc_sweep_launch(self.ctx._h, self._h))
+
+    def stats(self):
+        t, s, p = C.c_double(), C.c_double(), C.c_int()
+        st, co = C.c_longlong(), C.c_longlong()
+        _check(lib().bhc_sweep_stats(self._h, C.byref(t), C.byref(s), C.byref(p), C.byref(st), C.byref(co)))
+        return {"trace_ms": t.value, "scatter_ms": s.value, "passes": p.value, "steps": st.value,
+                "contrib": co.value}
+
+    def download(self, i):
+        prob = self.problems[i]
+        res = FieldResult()
+        out = _make_out(prob, res, self.with_stats, 0, 1, 0)
+        _check(lib().bhc_sweep_download(self.ctx._h, self._h, i, C.byref(out)))
+        res.err, res.warn = out.err, out.warn
+        res.total_steps = out.total_steps
+        res.problem = prob
+        return res
+
+    def release(self):
+        if self._h:
+            lib().bhc_sweep_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.bhc_sweep_release(self._h)
+            self._h = None

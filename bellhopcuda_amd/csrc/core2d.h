@@ -192,6 +192,8 @@ template<typename real> struct Prob2D {
     float *field;          // this problem's complex field (TL runs)
     int32_t njobs;         // rays of this problem
     int32_t pad;
+    int32_t *ray_stats;    // batched runs: per-ray steps, top / bottom bounces (or null)
+    double *ray_xend;      // batched runs: per-ray final (r, z) (or null)
 };
 
 // ---------------------------------------------------------------------------

@@ -608,6 +608,28 @@ void bind(bhc_problem_owner *o)
 
 extern "C" const char *bhc_host_last_error(void) { return g_host_error.c_str(); }
 
+// SubTab (src/common_setup.hpp:282-316): x[i] = fma(i, (x1 - x0) / (n - 1), x0)
+// in the vector's own precision.
+extern "C" int bhc_subtab(double x0, double x1, int32_t n, int32_t precision, void *out)
+{
+    if(!out || n < 0 || (precision != 64 && precision != 32)) return hfail("bhc_subtab: bad arguments");
+    if(precision == 64) {
+        double *x = static_cast<double *>(out);
+        if(n == 1) x[0] = x0;
+        if(n < 2) return 0;
+        double d = (x1 - x0) / (double)(n - 1);
+        for(int32_t i = 0; i < n; ++i) x[i] = std::fma((double)i, d, x0);
+    } else {
+        float *x = static_cast<float *>(out);
+        float a = (float)x0, b = (float)x1;
+        if(n == 1) x[0] = a;
+        if(n < 2) return 0;
+        float d = (b - a) / (float)(n - 1);
+        for(int32_t i = 0; i < n; ++i) x[i] = std::fmaf((float)i, d, a);
+    }
+    return 0;
+}
+
 extern "C" int bhc_preprocess2d(const bhc_env2d *env, bhc_problem_owner **out)
 {
     *out = nullptr;

@@ -1382,3 +1382,583 @@ extern "C" int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_r
     bhc_problem_release(dp);
     return rc;
 }
+
+// ===========================================================================
+// Batched sweeps: many independent 2D TL problems in one persistent launch.
+//
+// Every wavefront works on one problem at a time (its descriptor is loaded
+// with scalar loads and its SSP / boundary tables are staged in the
+// workgroup's LDS), its lanes take that problem's rays from a per-problem
+// work counter as they free up, and when the problem has no unclaimed rays
+// left and the wavefront's rays have all ended, the wavefront moves to the
+// first problem that still has rays. Segments carry their problem index and
+// one scatter pass serves every problem. When the segment buffer nears its
+// end, every wavefront saves its in-flight rays and exits; the host scatters
+// what was traced and relaunches, and the saved rays resume where they
+// stopped. The SSP type is dispatched at run time.
+// ===========================================================================
+namespace {
+
+template<typename real> struct RaySave {
+    RayPt<real> p0;
+    Trace<real> S;
+    Infl<real> I;
+    int32_t job, is, isz, ia, active, pad;
+};
+
+template<typename real> struct BatchDev {
+    const Prob2D<real> *probs;
+    int32_t nprob;
+    int32_t lds_cap;               // reals of LDS per workgroup for staging
+    int32_t *ray_next;             // [nprob] next unclaimed ray
+    int32_t *pcursor;              // lowest problem that may have unclaimed rays
+    uint32_t *perr;                // [nprob][4] err, warn, nerr, nwarn
+    unsigned long long *psteps;    // [nprob] steps
+    const RaySave<real> *save_in;  // [nsave_in][64]
+    const int32_t *save_in_pid;
+    int32_t nsave_in;
+    int32_t *resume_next;
+    RaySave<real> *save_out;
+    int32_t *save_out_pid;
+    int32_t *nsave_out;
+    int32_t *more;                 // set when a wavefront stopped taking new problems
+    uint32_t suspend_at;           // segment count at which wavefronts suspend
+};
+
+template<typename real> __device__ inline const real *stage_problem(const Prob2D<real> &P, int lds_cap,
+                                                                    unsigned char *lds)
+{
+    if(P.staged_reals <= 0 || P.staged_reals > lds_cap) return P.base;
+    real *L = reinterpret_cast<real *>(lds);
+    __syncthreads();
+    for(int i = threadIdx.x; i < P.staged_reals; i += blockDim.x) L[i] = P.base[i];
+    __syncthreads();
+    return L;
+}
+
+__device__ inline void flush_problem_err(Err &e, uint32_t *perr)
+{
+    if(e.err) atomicOr(perr + 0, e.err);
+    if(e.warn) atomicOr(perr + 1, e.warn);
+    if(e.nerr) atomicAdd(perr + 2, e.nerr);
+    if(e.nwarn) atomicAdd(perr + 3, e.nwarn);
+    e = Err{0, 0, 0, 0};
+}
+
+template<typename real>
+__global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x;
+    int pid = -1;
+    const Prob2D<real> *PP = Bd.probs;
+    const real *B = nullptr;
+    bool pexh = true, active = false, suspended = false;
+    Err e = {0, 0, 0, 0};
+    RayPt<real> p0;
+    Trace<real> S;
+    Infl<real> I;
+    int job = 0, isz = 0, ia = 0, is = 0;
+    unsigned long long steps = 0, all_steps = 0;
+    SegBlock blk{0u, 0u, 0u};
+    for(int iter = 0;; ++iter) {
+        if(!pexh) {
+            unsigned long long need = __ballot(!active);
+            if(need) {
+                int cnt = __popcll(need);
+                int leader = __ffsll((unsigned long long)need) - 1;
+                int base = 0;
+                if(lane == leader) base = atomicAdd(Bd.ray_next + pid, cnt);
+                base = __shfl(base, leader);
+                const int nr = PP->njobs;
+                if(base + cnt >= nr) pexh = true;
+                if(!active) {
+                    int my = base + __popcll(need & ((1ull << lane) - 1ull));
+                    if(my < nr) {
+                        job = my;
+                        JobIndices(job, *PP, isz, ia);
+                        is = 0;
+                        active = RayInit<real, 0>(isz, ia, p0, S, I, *PP, B, PP->base, e);
+                        if(!active && PP->ray_stats) {
+                            PP->ray_stats[3 * job] = 0;
+                            PP->ray_stats[3 * job + 1] = 0;
+                            PP->ray_stats[3 * job + 2] = 0;
+                            PP->ray_xend[2 * job] = 0.0;
+                            PP->ray_xend[2 * job + 1] = 0.0;
+                        }
+                    }
+                }
+            }
+        }
+        if(!__any(active)) {
+            // the wavefront is idle: close the current problem's books, then
+            // take a saved wavefront or the next problem with unclaimed rays
+            if(pid >= 0) {
+                flush_problem_err(e, Bd.perr + 4 * pid);
+                unsigned long long t = steps;
+                for(int off = WAVE / 2; off > 0; off >>= 1) t += __shfl_xor(t, off);
+                if(lane == 0 && t) atomicAdd(Bd.psteps + pid, t);
+                all_steps += steps;
+                steps = 0;
+            }
+            if(suspended) break;
+            int u = -1;
+            if(Bd.nsave_in > 0) {
+                if(lane == 0) u = atomicAdd(Bd.resume_next, 1);
+                u = __shfl(u, 0);
+            }
+            if(u >= 0 && u < Bd.nsave_in) {
+                pid = __builtin_amdgcn_readfirstlane(Bd.save_in_pid[u]);
+                PP = Bd.probs + pid;
+                B = stage_problem(*PP, Bd.lds_cap, lds_raw);
+                const RaySave<real> &sv = Bd.save_in[(size_t)u * WAVE + lane];
+                p0 = sv.p0;
+                S = sv.S;
+                I = sv.I;
+                job = sv.job;
+                is = sv.is;
+                isz = sv.isz;
+                ia = sv.ia;
+                active = sv.active != 0;
+                pexh = false;
+                continue;
+            }
+            // no new problems once the segment buffer nears its end: the next
+            // pass continues from the work counters
+            uint32_t sc = __hip_atomic_load(O.seg_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if(__builtin_amdgcn_readfirstlane(sc) >= Bd.suspend_at) {
+                if(lane == 0) atomicOr(Bd.more, 1);
+                break;
+            }
+            int p = 0;
+            if(lane == 0) {
+                p = __hip_atomic_load(Bd.pcursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while(p < Bd.nprob
+                      && __hip_atomic_load(Bd.ray_next + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          >= Bd.probs[p].njobs) {
+                    atomicMax(Bd.pcursor, p + 1);
+                    ++p;
+                }
+            }
+            p = __builtin_amdgcn_readfirstlane(__shfl(p, 0));
+            if(p >= Bd.nprob) break;
+            pid = p;
+            PP = Bd.probs + pid;
+            B = stage_problem(*PP, Bd.lds_cap, lds_raw);
+            pexh = false;
+            continue;
+        }
+        if((iter & 31) == 31) {
+            // HasErrored of this problem (its rays stop, other problems go on),
+            // and suspension when the segment buffer nears its end
+            if(e.err) atomicOr(Bd.perr + 4 * pid, e.err);
+            uint32_t ge = __hip_atomic_load(Bd.perr + 4 * pid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t sc = __hip_atomic_load(O.seg_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if(__any(ge != 0u || e.err != 0u)) {
+                if(lane == 0) atomicMax(Bd.ray_next + pid, PP->njobs);
+                active = false;
+                pexh = true;
+                continue;
+            }
+            if(sc >= Bd.suspend_at) {
+                int u = 0;
+                if(lane == 0) u = atomicAdd(Bd.nsave_out, 1);
+                u = __shfl(u, 0);
+                if(lane == 0) Bd.save_out_pid[u] = pid;
+                RaySave<real> &sv = Bd.save_out[(size_t)u * WAVE + lane];
+                sv.p0 = p0;
+                sv.S = S;
+                sv.I = I;
+                sv.job = job;
+                sv.is = is;
+                sv.isz = isz;
+                sv.ia = ia;
+                sv.active = active ? 1 : 0;
+                active = false;
+                pexh = true;
+                suspended = true;
+                continue;
+            }
+        }
+        const Prob2D<real> &P = *PP;
+        RayPt<real> p1, p2;
+        SegDesc<real> d;
+        d.nitems = 0;
+        bool two = false;
+        if(active) {
+            two = RayUpdate<real, 0>(p0, p1, p2, S, P, B, P.base, e);
+            prelude(d, p0, p1, I, is, P, P.base, isz, ia);
+            d.prob = pid;
+            ++is;
+        }
+        append_seg(d, lane, O, blk);
+        if(__any(two)) {
+            d.nitems = 0;
+            if(active && two) {
+                prelude(d, p1, p2, I, is, P, P.base, isz, ia);
+                d.prob = pid;
+                ++is;
+            }
+            append_seg(d, lane, O, blk);
+        }
+        if(active) {
+            p0 = two ? p2 : p1;
+            if(RayTerminate(p0, is, S, MAXN, P, e)) {
+                active = false;
+                steps += (unsigned long long)is;
+                if(P.ray_stats) {
+                    P.ray_stats[3 * job] = is;
+                    P.ray_stats[3 * job + 1] = p0.NumTopBnc;
+                    P.ray_stats[3 * job + 2] = p0.NumBotBnc;
+                    P.ray_xend[2 * job] = (double)p0.x.x;
+                    P.ray_xend[2 * job + 1] = (double)p0.x.y;
+                }
+            }
+        }
+    }
+    seal_block<real>(blk, lane, O);
+    for(int off = WAVE / 2; off > 0; off >>= 1) all_steps += __shfl_xor(all_steps, off);
+    if(lane == 0) atomicAdd(O.counters + 0, all_steps);
+}
+
+// Scatter of a batched pass: one wavefront per segment, the segment's problem
+// descriptor read through a wavefront-uniform pointer.
+template<typename real, bool FAST>
+__global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut O)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
+    const int nwaves = gridDim.x * (blockDim.x / WAVE);
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    unsigned long long nc = 0;
+    for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
+        const SegDesc<real> d = segs[s].d;
+        if(d.nitems <= 0) continue;
+        const Prob2D<real> &P = Bd.probs[__builtin_amdgcn_readfirstlane(d.prob)];
+        DevSink<real> sink{&O, &P, 0};
+        scatter_seg<real, RT_TL, FAST>(d, P, sink, lane);
+        nc += sink.nc;
+    }
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
+} // namespace
+
+struct bhc_sweep {
+    bhc_ctx *ctx;
+    int precision, nprob;
+    int with_stats;
+    std::vector<int> njobs, nrz, nrr, nsz;
+    std::vector<size_t> field_off, stats_off;   // in floats / rays
+    size_t field_floats, total_rays;
+    void *d_blobs;
+    void *d_probs;
+    float *d_rzsz;
+    float *d_fields;
+    int32_t *d_stats;
+    double *d_xend;
+    int32_t *d_ray_next, *d_pcursor, *d_resume, *d_nsave, *d_more;
+    uint32_t *d_perr;
+    unsigned long long *d_psteps, *d_counters;
+    void *d_save[2];
+    int32_t *d_save_pid[2];
+    int max_waves;
+    int lds_bytes;
+    int passes;
+    int scatter_blocks;
+    double trace_ms, scatter_ms;
+};
+
+extern "C" void bhc_sweep_release(bhc_sweep *sw)
+{
+    if(!sw) return;
+    (void)hipSetDevice(sw->ctx->device);
+    (void)hipStreamSynchronize(sw->ctx->stream);
+    void *ptrs[] = {sw->d_blobs, sw->d_probs, sw->d_rzsz,  sw->d_fields,   sw->d_stats,  sw->d_xend,
+                    sw->d_ray_next, sw->d_pcursor, sw->d_resume, sw->d_nsave, sw->d_more, sw->d_perr, sw->d_psteps,
+                    sw->d_counters, sw->d_save[0], sw->d_save[1], sw->d_save_pid[0], sw->d_save_pid[1]};
+    for(void *p : ptrs)
+        if(p) (void)hipFree(p);
+    delete sw;
+}
+
+namespace {
+
+template<typename real>
+int sweep_upload(bhc_sweep *sw, const bhc_problem2d *const *probs)
+{
+    bhc_ctx *ctx = sw->ctx;
+    int n = sw->nprob;
+    std::vector<Prob2D<real>> P(n);
+    std::vector<real> blobs;
+    std::vector<size_t> boff(n);
+    std::vector<float> rzsz;
+    std::vector<size_t> rzoff(n), szoff(n);
+    size_t max_staged = 0;
+    sw->field_off.resize(n);
+    sw->stats_off.resize(n);
+    sw->field_floats = 0;
+    sw->total_rays = 0;
+    for(int i = 0; i < n; ++i) {
+        const bhc_problem2d *p = probs[i];
+        std::vector<real> blob;
+        std::string err;
+        if(pack<real>(p, blob, P[i], err)) return fail("problem %d: %s", i, err.c_str());
+        boff[i] = blobs.size();
+        blobs.insert(blobs.end(), blob.begin(), blob.end());
+        while(blobs.size() % 32) blobs.push_back((real)0);
+        max_staged = std::max(max_staged, (size_t)P[i].staged_reals);
+        rzoff[i] = rzsz.size();
+        rzsz.insert(rzsz.end(), p->Rz, p->Rz + p->NRz);
+        szoff[i] = rzsz.size();
+        rzsz.insert(rzsz.end(), p->Sz, p->Sz + p->NSz);
+        sw->njobs.push_back((p->iSingle_alpha >= 1 ? 1 : p->Nalpha) * p->NSz);
+        sw->nsz.push_back(p->NSz);
+        sw->nrz.push_back(p->NRz_per_range);
+        sw->nrr.push_back(p->NRr);
+        sw->field_off[i] = sw->field_floats;
+        sw->field_floats += 2 * (size_t)p->NSz * p->NRz_per_range * p->NRr;
+        sw->field_floats = (sw->field_floats + 63) / 64 * 64;
+        sw->stats_off[i] = sw->total_rays;
+        sw->total_rays += sw->njobs.back();
+    }
+    // LDS per workgroup: up to 40 KiB, so that four single-wavefront
+    // workgroups fit a CU's 160 KiB; larger tables stay in global memory
+    size_t cap_reals = std::min<size_t>((size_t)40960, (size_t)ctx->lds_max) / sizeof(real);
+    size_t lds_reals = std::min(max_staged, cap_reals);
+    sw->lds_bytes = (int)(lds_reals * sizeof(real));
+    if(dalloc((real **)&sw->d_blobs, blobs.size()) || dalloc(&sw->d_rzsz, rzsz.size())
+       || dalloc(&sw->d_fields, sw->field_floats) || dalloc((Prob2D<real> **)&sw->d_probs, (size_t)n)
+       || dalloc(&sw->d_ray_next, (size_t)n) || dalloc(&sw->d_pcursor, 1) || dalloc(&sw->d_resume, 1)
+       || dalloc(&sw->d_nsave, 1) || dalloc(&sw->d_more, 1) || dalloc(&sw->d_perr, 4 * (size_t)n)
+       || dalloc(&sw->d_psteps, (size_t)n)
+       || dalloc(&sw->d_counters, 2))
+        return -1;
+    if(sw->with_stats
+       && (dalloc(&sw->d_stats, 3 * sw->total_rays) || dalloc(&sw->d_xend, 2 * sw->total_rays)))
+        return -1;
+    HIPCHECK(hipMemcpy(sw->d_blobs, blobs.data(), blobs.size() * sizeof(real), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(sw->d_rzsz, rzsz.data(), rzsz.size() * sizeof(float), hipMemcpyHostToDevice));
+    for(int i = 0; i < n; ++i) {
+        P[i].base = (const real *)sw->d_blobs + boff[i];
+        if((size_t)P[i].staged_reals > lds_reals) P[i].staged_reals = 0;
+        P[i].Rz = sw->d_rzsz + rzoff[i];
+        P[i].Sz = sw->d_rzsz + szoff[i];
+        P[i].field = sw->d_fields + sw->field_off[i];
+        P[i].njobs = sw->njobs[i];
+        if(sw->with_stats) {
+            P[i].ray_stats = sw->d_stats + 3 * sw->stats_off[i];
+            P[i].ray_xend = sw->d_xend + 2 * sw->stats_off[i];
+        }
+    }
+    HIPCHECK(hipMemcpy(sw->d_probs, P.data(), n * sizeof(Prob2D<real>), hipMemcpyHostToDevice));
+    // one single-wavefront workgroup per SIMD: the trace kernel runs at one
+    // wavefront per SIMD (register bound), and 4 x 40 KiB of LDS fill a CU
+    sw->max_waves = ctx->num_cus * 4;
+    size_t save_units = (size_t)sw->max_waves;
+    for(int k = 0; k < 2; ++k)
+        if(dalloc((RaySave<real> **)&sw->d_save[k], save_units * WAVE) || dalloc(&sw->d_save_pid[k], save_units))
+            return -1;
+    sw->scatter_blocks = ctx->num_cus * 8;
+    return 0;
+}
+
+template<typename real> int sweep_run(bhc_sweep *sw)
+{
+    bhc_ctx *c = sw->ctx;
+    hipStream_t s = c->stream;
+    if(ensure_segs(c)) return -1;
+    BatchDev<real> Bd;
+    memset(&Bd, 0, sizeof(Bd));
+    Bd.probs = (const Prob2D<real> *)sw->d_probs;
+    Bd.nprob = sw->nprob;
+    Bd.lds_cap = sw->lds_bytes / (int)sizeof(real);
+    Bd.ray_next = sw->d_ray_next;
+    Bd.pcursor = sw->d_pcursor;
+    Bd.perr = sw->d_perr;
+    Bd.psteps = sw->d_psteps;
+    Bd.resume_next = sw->d_resume;
+    Bd.nsave_out = sw->d_nsave;
+    Bd.more = sw->d_more;
+    DevOut O;
+    memset(&O, 0, sizeof(O));
+    O.counters = sw->d_counters;
+    O.segs = c->segs;
+    O.seg_count = c->d_seg_count;
+    O.overflow = c->d_overflow;
+    O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg<real>), 0xFFFFFFF0u);
+    int waves = sw->max_waves;
+    // every wavefront may add 2 segments per lane per step between two polls
+    // (32 steps apart) plus one partly used slot block
+    size_t margin = (size_t)waves * (2 * 32 * WAVE + 2 * SEG_BLOCK);
+    if(margin * 2 >= O.seg_cap) return fail("segment buffer too small for a batched sweep (raise BHC_SEG_BUFFER_MB)");
+    Bd.suspend_at = (uint32_t)(O.seg_cap - margin);
+    HIPCHECK(hipMemsetAsync(sw->d_fields, 0, sw->field_floats * sizeof(float), s));
+    HIPCHECK(hipMemsetAsync(sw->d_ray_next, 0, sw->nprob * sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(sw->d_pcursor, 0, sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(sw->d_perr, 0, 4 * sw->nprob * sizeof(uint32_t), s));
+    HIPCHECK(hipMemsetAsync(sw->d_psteps, 0, sw->nprob * sizeof(unsigned long long), s));
+    HIPCHECK(hipMemsetAsync(sw->d_counters, 0, 2 * sizeof(unsigned long long), s));
+    if(sw->with_stats) {
+        HIPCHECK(hipMemsetAsync(sw->d_stats, 0, 3 * sw->total_rays * sizeof(int32_t), s));
+        HIPCHECK(hipMemsetAsync(sw->d_xend, 0, 2 * sw->total_rays * sizeof(double), s));
+    }
+    HIPCHECK(hipEventRecord(c->ev0, s));
+    int nsave = 0, cur = 0;
+    sw->passes = 0;
+    sw->trace_ms = sw->scatter_ms = 0.0;
+    hipEvent_t e0, e1, e2;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventCreate(&e2));
+    int rc = 0;
+    for(;;) {
+        Bd.save_in = (const RaySave<real> *)sw->d_save[cur];
+        Bd.save_in_pid = sw->d_save_pid[cur];
+        Bd.nsave_in = nsave;
+        Bd.save_out = (RaySave<real> *)sw->d_save[1 - cur];
+        Bd.save_out_pid = sw->d_save_pid[1 - cur];
+        if(hipMemsetAsync(c->d_seg_count, 0, sizeof(uint32_t), s) != hipSuccess
+           || hipMemsetAsync(c->d_overflow, 0, sizeof(uint32_t), s) != hipSuccess
+           || hipMemsetAsync(sw->d_resume, 0, sizeof(int32_t), s) != hipSuccess
+           || hipMemsetAsync(sw->d_nsave, 0, sizeof(int32_t), s) != hipSuccess
+           || hipMemsetAsync(sw->d_more, 0, sizeof(int32_t), s) != hipSuccess) {
+            rc = fail("sweep: memset failed");
+            break;
+        }
+        (void)hipEventRecord(e0, s);
+        hipLaunchKernelGGL(k_trace_batch<real>, dim3(waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
+        (void)hipEventRecord(e1, s);
+        if(c->exact_phasor)
+            hipLaunchKernelGGL((k_scatter_batch<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+        else
+            hipLaunchKernelGGL((k_scatter_batch<real, true>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+        (void)hipEventRecord(e2, s);
+        if(hipGetLastError() != hipSuccess) {
+            rc = fail("sweep: kernel launch failed");
+            break;
+        }
+        int32_t ns = 0, more = 0;
+        uint32_t of = 0;
+        if(hipMemcpyAsync(&ns, sw->d_nsave, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess
+           || hipMemcpyAsync(&more, sw->d_more, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess
+           || hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess
+           || hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail("sweep: pass failed");
+            break;
+        }
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, e0, e1);
+        (void)hipEventElapsedTime(&b, e1, e2);
+        sw->trace_ms += a;
+        sw->scatter_ms += b;
+        ++sw->passes;
+        if(of) {
+            rc = fail("sweep: segment buffer overflow (suspension margin too small)");
+            break;
+        }
+        if(ns == 0 && !more) break;
+        if(sw->passes > 100000) {
+            rc = fail("sweep: no progress");
+            break;
+        }
+        nsave = ns;
+        cur = 1 - cur;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    HIPCHECK(hipEventRecord(c->ev1, s));
+    c->passes = sw->passes;
+    return rc;
+}
+
+} // namespace
+
+extern "C" int bhc_sweep_upload2d(bhc_ctx *ctx, const bhc_problem2d *const *probs, int nprob, int with_stats,
+                                  bhc_sweep **out)
+{
+    *out = nullptr;
+    if(!ctx || !probs || nprob < 1) return fail("bhc_sweep_upload2d: no problems");
+    int prec = probs[0]->precision;
+    for(int i = 0; i < nprob; ++i) {
+        if(check_problem(probs[i])) return -1;
+        char r0 = probs[i]->RunType[0];
+        if(r0 != 'C' && r0 != 'S' && r0 != 'I') return fail("sweep problem %d: only TL run types are batched", i);
+        if(probs[i]->precision != prec) return fail("sweep problem %d: all problems must share the precision", i);
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    bhc_sweep *sw = new bhc_sweep();
+    sw->ctx = ctx;
+    sw->precision = prec;
+    sw->nprob = nprob;
+    sw->with_stats = with_stats;
+    int rc = prec == 64 ? sweep_upload<double>(sw, probs) : sweep_upload<float>(sw, probs);
+    if(rc) {
+        bhc_sweep_release(sw);
+        return -1;
+    }
+    *out = sw;
+    return 0;
+}
+
+extern "C" int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw)
+{
+    if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_launch: bad arguments");
+    HIPCHECK(hipSetDevice(ctx->device));
+    return sw->precision == 64 ? sweep_run<double>(sw) : sweep_run<float>(sw);
+}
+
+extern "C" int bhc_sweep_stats(bhc_sweep *sw, double *trace_ms, double *scatter_ms, int *passes, long long *steps,
+                               long long *contrib)
+{
+    if(!sw) return fail("bhc_sweep_stats: null sweep");
+    unsigned long long cnt[2] = {0, 0};
+    HIPCHECK(hipStreamSynchronize(sw->ctx->stream));
+    HIPCHECK(hipMemcpy(cnt, sw->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if(trace_ms) *trace_ms = sw->trace_ms;
+    if(scatter_ms) *scatter_ms = sw->scatter_ms;
+    if(passes) *passes = sw->passes;
+    if(steps) *steps = (long long)cnt[0];
+    if(contrib) *contrib = (long long)cnt[1];
+    return 0;
+}
+
+extern "C" int bhc_sweep_download(bhc_ctx *ctx, bhc_sweep *sw, int iprob, bhc_field_out *out)
+{
+    if(!ctx || !sw || iprob < 0 || iprob >= sw->nprob || !out) return fail("bhc_sweep_download: bad arguments");
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    size_t nf = 2 * (size_t)sw->nsz[iprob] * sw->nrz[iprob] * sw->nrr[iprob];
+    if(out->field)
+        HIPCHECK(hipMemcpy(out->field, sw->d_fields + sw->field_off[iprob], nf * sizeof(float),
+                           hipMemcpyDeviceToHost));
+    uint32_t ew[4];
+    HIPCHECK(hipMemcpy(ew, sw->d_perr + 4 * iprob, sizeof(ew), hipMemcpyDeviceToHost));
+    out->err = ew[0];
+    out->warn = ew[1];
+    out->err_count = ew[2];
+    out->warn_count = ew[3];
+    unsigned long long st = 0;
+    HIPCHECK(hipMemcpy(&st, sw->d_psteps + iprob, sizeof(st), hipMemcpyDeviceToHost));
+    out->total_steps = (int64_t)st;
+    out->total_contrib = 0;
+    if(out->ray_nsteps && sw->with_stats) {
+        int nj = sw->njobs[iprob];
+        std::vector<int32_t> stv(3 * (size_t)nj);
+        HIPCHECK(hipMemcpy(stv.data(), sw->d_stats + 3 * sw->stats_off[iprob], stv.size() * sizeof(int32_t),
+                           hipMemcpyDeviceToHost));
+        for(int j = 0; j < nj; ++j) {
+            out->ray_nsteps[j] = stv[3 * j];
+            if(out->ray_ntop) out->ray_ntop[j] = stv[3 * j + 1];
+            if(out->ray_nbot) out->ray_nbot[j] = stv[3 * j + 2];
+        }
+        if(out->ray_xend)
+            HIPCHECK(hipMemcpy(out->ray_xend, sw->d_xend + 2 * sw->stats_off[iprob], 2 * (size_t)nj * sizeof(double),
+                               hipMemcpyDeviceToHost));
+    }
+    return 0;
+}

@@ -30,18 +30,19 @@ _libm.fmaf.argtypes = [C.c_float, C.c_float, C.c_float]
 
 
 def subtab(x0, x1, n, dtype=np.float64):
-    """SubTab (src/common_setup.hpp:282-316) in float64 or float32."""
+    """SubTab (src/common_setup.hpp:282-316) in float64 or float32, through
+    the library's bhc_subtab (one fused multiply-add per element)."""
     if n <= 0:
         return np.zeros(0, dtype)
     if n == 1:
         return np.array([x0], dtype)
-    if dtype == np.float32:
-        a, b = np.float32(x0), np.float32(x1)
-        d = np.float32((b - a) / np.float32(n - 1))
-        return np.array([_libm.fmaf(float(np.float32(i)), float(d), float(a)) for i in range(n)],
-                        np.float32)
-    d = (float(x1) - float(x0)) / float(n - 1)
-    return np.array([_libm.fma(float(i), d, float(x0)) for i in range(n)], np.float64)
+    from . import lib
+    out = np.zeros(n, np.float32 if dtype == np.float32 else np.float64)
+    rc = lib().bhc_subtab(float(x0), float(x1), n, 32 if dtype == np.float32 else 64,
+                          out.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise ValueError("subtab failed")
+    return out
 
 
 @dataclass

@@ -1,16 +1,24 @@
 """Throughput benchmark of the 2D ray/beam-tracing hot path on one MI355X.
 
-Default workload (one "step" = one complete TL field): the range-dependent
-seamount problem -- 40-knot spline SSP, 500-segment bathymetry, acousto-elastic
-halfspace, source 50 m, 100 Hz, 20001 ray-centred Gaussian beams over +-60 deg,
-501 x 5001 receivers to 200 km, FP64. Inputs are resident in HBM before the
-timed region; each step zeroes the field and traces + scatters every ray.
+Workloads (one "step" = one pass of the hot path over one batch of input):
 
-Prints one JSON line: fields/s (value), rays/s, the FP64 roofline of the field
-kernels (algorithmic operations F = f_I R + f_S S + f_C C, with f_* measured by
-the op-counting instantiation of the same core and R, S, C counted by the
-kernels, divided by the kernels' time from HIP events), and the CPU baseline
-(the oracle's std-thread pool on a bounded ray sample of the same workload).
+* sweep (default): the heterogeneous batched sweep -- 4096 independent 2D TL
+  problems drawn from one seed (10-200 SSP knots, c-linear or PCHIP; flat or
+  1-500-segment bathymetry, depth 100-5000 m; 25-1000 Hz; 500-20000 hat
+  beams; 51-501 x 201-5001 receivers), FP64. A step traces every problem in
+  one persistent launch (bhc_sweep_launch). The plain per-environment GPU
+  launch (one resident problem per launch) is timed on a sample of the same
+  problems and reported beside it.
+* seamount: one range-dependent TL field -- 40-knot spline SSP, 500-segment
+  bathymetry, 20001 ray-centred Gaussian beams, 501 x 5001 receivers, FP64.
+* munk: one Munk-profile TL field, 2001 hat beams, 201 x 1001 receivers.
+
+Inputs are resident in HBM before the timed region. Prints one JSON line:
+fields/s (value), rays/s, the FP64 roofline of the field kernels (algorithmic
+operations F = f_I R + f_S S + f_C C, f_* from the op-counting instantiation of
+the same core, R, S, C counted by the kernels, over the kernels' time from HIP
+events) and the CPU baseline (the oracle's thread pool on a bounded sample of
+rays of the same workload).
 """
 import argparse
 import ctypes as C
@@ -26,7 +34,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 vector
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 
 
-def workload(name, precision):
+def single_workload(name, precision):
     import bellhopcuda_amd as bh
     if name == "seamount":
         return bh.c2_env(precision=precision), "seamount 2D range-dependent TL, 20001 'b' beams, 501x5001 receivers"
@@ -35,71 +43,49 @@ def workload(name, precision):
     raise SystemExit("unknown workload " + name)
 
 
-def cpu_baseline(env, nrays_total, target_s):
-    """Oracle (plain-C restatement, std::thread-style pool over all host cores)
-    on chunks of rays spread over the fan; returns rays/s and TL fields/s."""
+def cpu_rays_per_s(envs, target_s, cores):
+    """Oracle (plain-C restatement, thread pool over `cores` host threads) on
+    chunks of rays spread over each environment's fan, cycling over `envs`,
+    doubling the chunk until about `target_s` seconds were spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    cores = os.cpu_count() or 1
-    nchunks = 8
-    per = 8
-    done = 0
-    t_total = 0.0
-    while t_total < target_s and per <= nrays_total // nchunks:
-        t0 = time.time()
-        for k in range(nchunks):
-            b = max(0, int((k + 0.5) * nrays_total / nchunks) - per // 2)
-            oracle.field_run(env, nthreads=cores, job_begin=b, job_end=b + per, stats=False)
-            done += per
-        t_total += time.time() - t0
+    nchunks, per, done, t_total, rounds = 8, 8, 0, 0.0, 0
+    while t_total < target_s and rounds < 12:
+        for env in envs:
+            n = env.nrays
+            p = min(per, max(1, n // nchunks))
+            t0 = time.time()
+            for k in range(nchunks):
+                b = max(0, min(n - p, int((k + 0.5) * n / nchunks) - p // 2))
+                oracle.field_run(env, nthreads=cores, job_begin=b, job_end=b + p, stats=False)
+                done += p
+            t_total += time.time() - t0
+            if t_total >= target_s:
+                break
         per *= 2
-    rays_s = done / t_total
-    return {"value": rays_s / nrays_total, "unit": "TL fields/s", "rays_per_s": rays_s, "cores": cores,
-            "kind": "port",
-            "sample": "%d rays in %d chunks spread over the %d-ray fan, full receiver grid, %.1f s" % (
-                done, nchunks, nrays_total, t_total)}
+        rounds += 1
+    return done / t_total, done, t_total
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="seamount")
-    ap.add_argument("--precision", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="first run one device-to-device copy of this many MiB (the reference kernel for "
-                         "scaling profiler byte counters)")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print("bench.py measures one GPU: the tracer has no multi-GPU path (problems and ray ranges would be "
-              "sharded per GPU with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def opcounts(L, _abi, probs, nsample):
+    """Mean per-unit operation counts over `probs` (op-counting core)."""
+    acc = [0.0, 0.0, 0.0]
+    for p in probs:
+        oc = (C.c_double * 6)()
+        if L.bhc_opcount2d(C.byref(p.view), nsample, oc) != 0:
+            raise SystemExit("op count failed")
+        for k in range(3):
+            acc[k] += oc[k]
+    return [v / len(probs) for v in acc]
 
+
+def run_single(a):
     import bellhopcuda_amd as bh
     from bellhopcuda_amd import _abi
     L = bh.lib()
-    L.bhc_last_phase_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
-    L.bhc_opcount2d.argtypes = [C.POINTER(_abi.Problem2D), C.c_int, C.POINTER(C.c_double)]
-
-    if a.copy_calibration_mb > 0:
-        import torch
-        src = torch.ones(a.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
-    env, desc = workload(a.workload, a.precision)
+    env, desc = single_workload(a.workload, a.precision)
     prob = bh.Problem(env)
-    # per-unit algorithmic operation counts (same core, op-counting scalar type)
-    oc = (C.c_double * 6)()
-    if L.bhc_opcount2d(C.byref(prob.view), 40, oc) != 0:
-        raise SystemExit("op count failed")
-    f_I, f_S, f_C = oc[0], oc[1], oc[2]
-
+    f_I, f_S, f_C = opcounts(L, _abi, [prob], 40)
     ctx = bh.Context(0)
     res = bh.Resident(prob, ctx)
     for _ in range(a.warmup):
@@ -119,7 +105,6 @@ def main():
     L.bhc_ctx_synchronize(ctx._h)
     wall = time.perf_counter() - t0
     out = res.download()
-    ms_per_step = 1e3 * wall / a.steps
     fields_s = a.steps / wall
     R, S, Cn = prob.nrays, out.total_steps, out.total_contrib
     F = f_I * R + f_S * S + f_C * Cn
@@ -128,26 +113,147 @@ def main():
     achieved = F / (k_ms * 1e-3) / 1e12
     line = {
         "metric": "TL fields/s + rays/s on 1 MI355X",
-        "value": fields_s,
-        "unit": "TL fields/s",
-        "rays_per_s": fields_s * R,
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "value": fields_s, "unit": "TL fields/s", "rays_per_s": fields_s * R,
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision == 64 else "f32",
-        "data": "synthetic (seeded environment)",
+        "dtype": "f64" if a.precision == 64 else "f32", "data": "synthetic (seeded environment)",
         "config": {"workload": desc, "rays": R, "receivers": int(env.field_shape[1] * env.field_shape[2]),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel_ms": k_ms, "trace_ms": sum(tms) / len(tms), "scatter_ms": sum(sms) / len(sms),
-                     "flop_per_field": F, "f_I": f_I, "f_S": f_S, "f_C": f_C,
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel_ms": k_ms, "trace_ms": sum(tms) / len(tms),
+                     "scatter_ms": sum(sms) / len(sms), "flop_per_field": F, "f_I": f_I, "f_S": f_S, "f_C": f_C,
                      "rays": R, "steps": S, "contributions": Cn},
         "errors": {"err": out.err, "warn": out.warn},
     }
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(env, R, a.cpu_seconds)
+        cores = os.cpu_count() or 1
+        rs, done, t = cpu_rays_per_s([env], a.cpu_seconds, cores)
+        line["cpu_baseline"] = {"value": rs / R, "unit": "TL fields/s", "rays_per_s": rs, "cores": cores,
+                                "kind": "port",
+                                "sample": "%d rays in chunks spread over the %d-ray fan, full receiver grid, %.1f s"
+                                          % (done, R, t)}
+    return line
+
+
+def run_sweep(a):
+    import bellhopcuda_amd as bh
+    from bellhopcuda_amd import _abi
+    L = bh.lib()
+    envs = bh.sweep_envs(n=a.sweep_n, precision=a.precision, scale=a.sweep_scale)
+    t0 = time.perf_counter()
+    probs = [bh.Problem(e) for e in envs]
+    t_pre = time.perf_counter() - t0
+    ctx = bh.Context(0)
+    nwarm = min(len(probs), 64)
+    warm = bh.Sweep(probs[:nwarm], ctx)
+    warm.launch()
+    warm.release()
+    sw = bh.Sweep(probs, ctx)
+    R = sum(p.nrays for p in probs)
+    for _ in range(max(0, a.warmup - 1)):
+        sw.launch()
+    L.bhc_ctx_synchronize(ctx._h)
+    kern, tr_ms, sc_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sw.launch()
+        st = sw.stats()
+        tr_ms.append(st["trace_ms"])
+        sc_ms.append(st["scatter_ms"])
+        kern.append(ctx.last_kernel_ms())
+    L.bhc_ctx_synchronize(ctx._h)
+    wall = time.perf_counter() - t0
+    st = sw.stats()
+    errs = [sw.download(i) for i in range(0, len(probs), max(1, len(probs) // 64))]
+    nerr = sum(1 for r in errs if r.err)
+    sw.release()
+    fields_s = a.steps * len(probs) / wall
+
+    # plain per-environment GPU launches on a sample of the same problems
+    step = max(1, len(probs) // a.plain_sample)
+    sample = probs[::step][:a.plain_sample]
+    t_plain = 0.0
+    for p in sample:
+        r = bh.Resident(p, ctx)
+        r.launch()
+        L.bhc_ctx_synchronize(ctx._h)
+        t1 = time.perf_counter()
+        r.launch()
+        L.bhc_ctx_synchronize(ctx._h)
+        t_plain += time.perf_counter() - t1
+        r.release()
+    plain_fields_s = len(sample) / t_plain
+
+    f_I, f_S, f_C = opcounts(L, _abi, probs[::max(1, len(probs) // 16)][:16], 16)
+    S, Cn = st["steps"], st["contrib"]
+    F = f_I * R + f_S * S + f_C * Cn
+    k_ms = (sum(tr_ms) + sum(sc_ms)) / len(tr_ms)
+    peak = FP64_PEAK_TFLOPS if a.precision == 64 else FP32_PEAK_TFLOPS
+    achieved = F / (k_ms * 1e-3) / 1e12
+    line = {
+        "metric": "TL fields/s + rays/s on 1 MI355X, batched/surrogate vs plain GPU",
+        "value": fields_s, "unit": "TL fields/s", "rays_per_s": fields_s * R / len(probs),
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == 64 else "f32", "data": "synthetic (seeded sweep, seed 20260004)",
+        "config": {"workload": "batched sweep of %d independent 2D TL problems (10-200 knots c-linear/PCHIP, "
+                               "flat or 1-500-segment bathymetry, 25-1000 Hz, 500-20000 hat beams, "
+                               "51-501 x 201-5001 receivers)%s" % (
+                                   len(probs), "" if a.sweep_scale == 1.0 else ", beams/grids scaled %.3g" % a.sweep_scale),
+                   "problems": len(probs), "rays": R, "parallelism": "single GPU"},
+        "plain_per_env": {"value": plain_fields_s, "unit": "TL fields/s",
+                          "sample": "%d of the %d problems, one resident launch each" % (len(sample), len(probs))},
+        "batched_over_plain": fields_s / plain_fields_s,
+        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel_ms": k_ms, "trace_ms": sum(tr_ms) / len(tr_ms),
+                     "scatter_ms": sum(sc_ms) / len(sc_ms), "passes": st["passes"], "flop_per_step": F,
+                     "f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S, "contributions": Cn},
+        "errors": {"problems_with_errors_in_sample": nerr},
+        "host_preprocess_s": t_pre,
+    }
+    if not a.no_cpu:
+        cores = os.cpu_count() or 1
+        cs = envs[::max(1, len(envs) // 8)][:8]
+        rs, done, t = cpu_rays_per_s(cs, a.cpu_seconds, cores)
+        mean_rays = R / len(probs)
+        line["cpu_baseline"] = {"value": rs / mean_rays, "unit": "TL fields/s", "rays_per_s": rs, "cores": cores,
+                                "kind": "port",
+                                "sample": "%d rays from 8 of the sweep's problems (chunks spread over each fan, "
+                                          "full receiver grids), %.1f s; fields/s = rays/s / mean rays per problem"
+                                          % (done, t)}
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "seamount", "munk"])
+    ap.add_argument("--precision", type=int, default=64)
+    ap.add_argument("--sweep-n", type=int, default=4096)
+    ap.add_argument("--sweep-scale", type=float, default=1.0)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (the reference kernel for "
+                         "scaling profiler byte counters)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the tracer has no multi-GPU path (problems and ray ranges would be "
+              "sharded per GPU with no exchange). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if a.copy_calibration_mb > 0:
+        import torch
+        src = torch.ones(a.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    line = run_sweep(a) if a.workload == "sweep" else run_single(a)
     print(json.dumps(line), flush=True)
 
 

@@ -268,6 +268,9 @@ typedef struct bhc_problem_owner bhc_problem_owner;
 /* Preprocess a raw environment. On success *out owns all arrays; the view is
  * valid until bhc_problem_free(). */
 int bhc_preprocess2d(const bhc_env2d *env, bhc_problem_owner **out);
+/* SubTab (src/common_setup.hpp:282-316): fills out[0..n) with
+ * fma(i, (x1 - x0) / (n - 1), x0) in double (precision 64) or float (32). */
+int bhc_subtab(double x0, double x1, int32_t n, int32_t precision, void *out);
 const bhc_problem2d *bhc_problem_view(const bhc_problem_owner *p);
 void bhc_problem_free(bhc_problem_owner *p);
 
@@ -312,6 +315,28 @@ int bhc_last_phase_ms(bhc_ctx *ctx, double *trace_ms, double *scatter_ms, int *p
 
 /* Launch tuning (rays per workgroup, workgroups per CU); 0 = autotuned default. */
 int bhc_ctx_set_launch(bhc_ctx *ctx, int threads_per_block, int blocks_per_cu);
+
+/* ---- batched sweeps ---------------------------------------------------------------
+ * Many independent TL problems (run types C / S / I, one precision) traced in
+ * one persistent launch: rays of all problems are taken from device-side work
+ * counters, each problem's field stays in HBM until downloaded. Replaces a
+ * caller's loop of RunFieldModesImpl calls over environments
+ * (src/mode/fieldimpl.hpp:27-28), one per problem. */
+typedef struct bhc_sweep bhc_sweep;
+/* Upload `nprob` preprocessed problems; with_stats keeps per-ray step / bounce
+ * counts and end points. */
+int bhc_sweep_upload2d(bhc_ctx *ctx, const bhc_problem2d *const *probs, int nprob, int with_stats,
+                       bhc_sweep **out);
+/* Zero every field and trace + scatter all problems (blocks until done). */
+int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw);
+/* Field, errors / warnings, step count and (with_stats) per-ray statistics of
+ * problem iprob; out->field holds NSz * NRz_per_range * NRr complex floats. */
+int bhc_sweep_download(bhc_ctx *ctx, bhc_sweep *sw, int iprob, bhc_field_out *out);
+/* Kernel times (HIP events, summed over passes), passes, total steps and
+ * receiver contributions of the last launch. */
+int bhc_sweep_stats(bhc_sweep *sw, double *trace_ms, double *scatter_ms, int *passes, long long *steps,
+                    long long *contrib);
+void bhc_sweep_release(bhc_sweep *sw);
 
 /* ---- post-processing and file layouts (host) ----------------------------------- */
 
