@@ -67,6 +67,19 @@ def lib():
         "bhc_last_phase_ms": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int)]),
         "bhc_opcount2d": (C.c_int, [P(_abi.Problem2D), C.c_int, P(C.c_double)]),
         "bhc_subtab": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
+        "bhc_postprocess_arrivals2d": (C.c_int, [P(_abi.Problem2D), P(_abi.Arrival), _abi.c_int32_p, C.c_int32,
+                                                 _abi.c_int32_p]),
+        "bhc_write_arr2d": (C.c_int, [C.c_char_p, P(_abi.Problem2D), P(_abi.Arrival), _abi.c_int32_p, C.c_int32,
+                                      _abi.c_int32_p, C.c_int]),
+        "bhc_write_ray2d": (C.c_int, [C.c_char_p, P(_abi.Problem2D), C.c_char_p, P(_abi.RayPt2D), _abi.c_int32_p,
+                                      C.c_int32, _abi.c_double_p, C.c_int32]),
+        "bhc_ld_open": (C.c_void_p, [C.c_char_p]),
+        "bhc_ld_i32": (None, [C.c_void_p, C.c_int32]),
+        "bhc_ld_f32": (None, [C.c_void_p, C.c_float]),
+        "bhc_ld_f64": (None, [C.c_void_p, C.c_double]),
+        "bhc_ld_str": (None, [C.c_void_p, C.c_char_p]),
+        "bhc_ld_char": (None, [C.c_void_p, C.c_char]),
+        "bhc_ld_close": (C.c_int, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

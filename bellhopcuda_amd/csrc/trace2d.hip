@@ -781,6 +781,7 @@ struct bhc_ctx {
     int passes;                 // trace/scatter passes of the last launch
     int use_tiles;              // 0 disables the LDS-tiled TL scatter
     int exact_phasor;           // 1: TL phasor in the reference's full arithmetic
+    size_t seg_mb;              // segment buffer size (MiB), allocated at first use
 };
 
 struct bhc_dev_problem {
@@ -847,6 +848,8 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     if(const char *e = getenv("BHC_TL_TILES")) c->use_tiles = atoi(e) != 0;
     c->exact_phasor = 0;
     if(const char *e = getenv("BHC_EXACT_PHASOR")) c->exact_phasor = atoi(e) != 0;
+    c->seg_mb = 16384;
+    if(const char *e = getenv("BHC_SEG_BUFFER_MB")) c->seg_mb = (size_t)std::max(16L, atol(e));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));
     HIPCHECK(hipEventCreate(&c->ev1));
@@ -1123,9 +1126,7 @@ size_t seg_size(int precision) { return precision == 64 ? sizeof(Seg<double>) : 
 int ensure_segs(bhc_ctx *c)
 {
     if(c->segs) return 0;
-    size_t mb = 16384;
-    if(const char *e = getenv("BHC_SEG_BUFFER_MB")) mb = (size_t)std::max(16L, atol(e));
-    c->seg_bytes = mb << 20;
+    c->seg_bytes = c->seg_mb << 20;
     HIPCHECK(hipMalloc(&c->segs, c->seg_bytes));
     size_t maxseg = c->seg_bytes / sizeof(Seg<float>);
     HIPCHECK(hipMalloc((void **)&c->d_bin_idx, maxseg * sizeof(uint32_t)));

@@ -345,6 +345,30 @@ int bhc_scale_pressure2d(const bhc_problem2d *prob, float *field);
 /* .shd file (WriteHeader + WriteOutTL, src/mode/tl.cpp:38-101, 211-248). */
 int bhc_write_shd2d(const char *path, const bhc_problem2d *prob, const char *title, const float *field);
 const char *bhc_host_last_error(void);
+/* PostProcessArrivals for 2D (src/mode/arr.cpp:24-73), in place: counts
+ * clamped to max_narr, amplitudes scaled by 1/sqrt(r) (1e5 at r = 0, 4 sqrt(pi)
+ * for a line source), maximum count per source written to maxn[NSz]. */
+int bhc_postprocess_arrivals2d(const bhc_problem2d *prob, bhc_arrival *arr, int32_t *narr, int32_t max_narr,
+                               int32_t *maxn);
+/* .arr file (WriteOutArrivals, src/mode/arr.cpp:88-238): list-directed ASCII
+ * (run types 'A' / 'V', binary = 0) or Fortran unformatted ('a' / 'v'). */
+int bhc_write_arr2d(const char *path, const bhc_problem2d *prob, const bhc_arrival *arr, const int32_t *narr,
+                    int32_t max_narr, const int32_t *maxn, int binary);
+/* .ray file (OpenRAYFile + WriteRay, src/mode/ray.hpp:168-244): nrays rays of
+ * max_points points each (the first nsteps[r] valid) and their launch angles
+ * in degrees. */
+int bhc_write_ray2d(const char *path, const bhc_problem2d *prob, const char *title, const bhc_ray_pt2d *pts,
+                    const int32_t *nsteps, int32_t max_points, const double *decl_deg, int32_t nrays);
+/* The list-directed writer of the .arr / .ray files as a stream (LDOFile,
+ * src/util/ldio.hpp:409-584, FORTRAN_OUTPUT style). */
+typedef struct bhc_ldfile bhc_ldfile;
+bhc_ldfile *bhc_ld_open(const char *path);
+void bhc_ld_i32(bhc_ldfile *f, int32_t v);
+void bhc_ld_f32(bhc_ldfile *f, float v);
+void bhc_ld_f64(bhc_ldfile *f, double v);
+void bhc_ld_str(bhc_ldfile *f, const char *s);
+void bhc_ld_char(bhc_ldfile *f, char c);
+int bhc_ld_close(bhc_ldfile *f);
 
 /* ---- measurement (no counterpart in the reference) ------------------------------ */
 
