@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libbhc_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BHC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["trace2d.hip", "opcount2d.cpp"]
+HIP_SOURCES = ["trace2d.hip", "mlp.hip", "opcount2d.cpp"]
 CXX_SOURCES = ["host2d.cpp", "writers2d.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]

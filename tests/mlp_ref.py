@@ -1,0 +1,96 @@
+"""CPU FP64 restatement of the surrogate MLP (test infrastructure).
+
+Same network and the same BF16 roundings as the GPU kernels (features,
+weights, hidden activations and the dZ operands of the backward products are
+rounded to BF16; everything else, and every accumulation, is FP64 here), so
+the GPU gradients differ from these only by FP32 summation order.
+"""
+import numpy as np
+
+D_IN, N_FREQ, D_FF, WIDTH, N_HID = 16, 32, 64, 256, 6
+
+
+def bf16(a):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    a = np.asarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def layout():
+    off, o = {}, 0
+    for l in range(N_HID):
+        k = D_FF if l == 0 else WIDTH
+        off["W%d" % l] = (o, (k, WIDTH))
+        o += k * WIDTH
+        off["b%d" % l] = (o, (WIDTH,))
+        o += WIDTH
+    off["Wout"] = (o, (WIDTH,))
+    o += WIDTH
+    off["bout"] = (o, (1,))
+    o += 1
+    return off, o
+
+
+def unpack(p):
+    off, n = layout()
+    assert p.size == n
+    return {k: p[o:o + int(np.prod(s))].reshape(s) for k, (o, s) in off.items()}
+
+
+def _erf(x):
+    from scipy.special import erf
+    return erf(x)
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + _erf(z / np.sqrt(2.0)))
+
+
+def gelu_d(z):
+    return 0.5 * (1.0 + _erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+
+
+def features(x, Bf):
+    # the projection is summed in FP32 in input order, as the feature kernel does
+    x32, B32 = np.asarray(x, np.float32), np.asarray(Bf, np.float32)
+    s = np.zeros((x32.shape[0], N_FREQ), np.float32)
+    for d in range(D_IN):
+        s = (s + B32[None, :, d] * x32[:, d:d + 1]).astype(np.float32)
+    s = s.astype(np.float64)
+    F = np.empty((x.shape[0], D_FF))
+    F[:, 0::2] = np.sin(2 * np.pi * s)
+    F[:, 1::2] = np.cos(2 * np.pi * s)
+    return bf16(F)
+
+
+def forward_backward(p, Bf, x, y):
+    """One member: returns (out, loss, grads as a flat array)."""
+    P = unpack(p.astype(np.float64))
+    Wb = {k: bf16(v) if k.startswith("W") else v for k, v in P.items()}
+    n = x.shape[0]
+    H = [features(x, Bf)]
+    Z = []
+    for l in range(N_HID):
+        z = H[-1] @ Wb["W%d" % l] + P["b%d" % l]
+        Z.append(z)
+        H.append(bf16(gelu(z)))
+    out = H[-1] @ Wb["Wout"] + P["bout"][0]
+    d = out - y
+    loss = np.mean(d * d)
+    g = 2.0 * d / n
+    G = {"Wout": H[-1].T @ g, "bout": np.array([g.sum()])}
+    dH = np.outer(g, Wb["Wout"])
+    for l in range(N_HID - 1, -1, -1):
+        dZ = dH * gelu_d(Z[l])
+        G["b%d" % l] = dZ.sum(0)
+        dZb = bf16(dZ)
+        G["W%d" % l] = H[l].T @ dZb
+        if l > 0:
+            dH = dZb @ Wb["W%d" % l].T
+    off, ntot = layout()
+    flat = np.zeros(ntot)
+    for k, (o, s) in off.items():
+        flat[o:o + int(np.prod(s))] = np.asarray(G[k]).reshape(-1)
+    return out, loss, flat

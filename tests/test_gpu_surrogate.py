@@ -1,0 +1,54 @@
+"""GPU: the surrogate MLP's matrix-core forward/backward against the CPU FP64
+restatement (tests/mlp_ref.py), and training that learns."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import bellhopcuda_amd as bh
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(name):
+    print("\n[gpu-case]", name, flush=True)
+    sys.stdout.flush()
+
+
+def test_mlp_gradients_match_fp64_restatement():
+    _announce("mlp_gradients")
+    from bellhopcuda_amd import surrogate
+    m = surrogate.Mlp(members=2, max_batch=256, seed=7)
+    rng = np.random.default_rng(1)
+    x = rng.random((96, 16), dtype=np.float32)
+    y = rng.normal(size=96).astype(np.float32)
+    P, Bf = m.params()
+    g, loss = m.gradients(x, y)
+    for e in range(2):
+        out_r, loss_r, g_r = mlp_ref.forward_backward(P[e], Bf, x, y.astype(np.float64))
+        assert loss[e] == pytest.approx(loss_r, rel=2e-4)
+        scale = np.max(np.abs(g_r))
+        err = np.max(np.abs(g[e] - g_r)) / scale
+        print("member", e, "loss", loss[e], loss_r, "max rel grad err", err, flush=True)
+        assert err < 2e-3
+        pred = m.predict(x)
+        assert np.allclose(pred[e], out_r, rtol=1e-3, atol=1e-3 * np.max(np.abs(out_r)))
+
+
+def test_mlp_training_reduces_loss():
+    _announce("mlp_training")
+    from bellhopcuda_amd import surrogate
+    rng = np.random.default_rng(2)
+    X = rng.random((20000, 16), dtype=np.float32)
+    Y = (np.sin(3 * X[:, 0]) + X[:, 1] ** 2 - 0.5 * X[:, 2]).astype(np.float32)
+    m = surrogate.Mlp(members=4, max_batch=1024, seed=3)
+    m.set_dataset(X, Y)
+    l0 = m.train(steps=5, batch=1024, lr=1e-3, seed=1)
+    l1 = m.train(steps=400, batch=1024, lr=1e-3, seed=2)
+    print("loss", l0, l1, flush=True)
+    assert np.all(l1 < 0.1 * l0)
+    pred = m.predict(X[:2000])
+    rms = np.sqrt(np.mean((pred.mean(0) - Y[:2000]) ** 2))
+    assert rms < 0.1 * np.std(Y)
