@@ -1415,9 +1415,9 @@ template<typename real> struct BatchDev {
     int32_t *pcursor;              // lowest problem that may have unclaimed rays
     uint32_t *perr;                // [nprob][4] err, warn, nerr, nwarn
     unsigned long long *psteps;    // [nprob] steps
-    const RaySave<real> *save_in;  // [nsave_in][64]
+    const RaySave<real> *save_in;  // [*nsave_in][64]
     const int32_t *save_in_pid;
-    int32_t nsave_in;
+    const int32_t *nsave_in;       // saved wavefronts of the previous pass
     int32_t *resume_next;
     RaySave<real> *save_out;
     int32_t *save_out_pid;
@@ -1462,6 +1462,7 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
     int job = 0, isz = 0, ia = 0, is = 0;
     unsigned long long steps = 0, all_steps = 0;
     SegBlock blk{0u, 0u, 0u};
+    const int nsave_in = __builtin_amdgcn_readfirstlane(*Bd.nsave_in);
     for(int iter = 0;; ++iter) {
         if(!pexh) {
             unsigned long long need = __ballot(!active);
@@ -1504,11 +1505,11 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
             }
             if(suspended) break;
             int u = -1;
-            if(Bd.nsave_in > 0) {
+            if(nsave_in > 0) {
                 if(lane == 0) u = atomicAdd(Bd.resume_next, 1);
                 u = __shfl(u, 0);
             }
-            if(u >= 0 && u < Bd.nsave_in) {
+            if(u >= 0 && u < nsave_in) {
                 pid = __builtin_amdgcn_readfirstlane(Bd.save_in_pid[u]);
                 PP = Bd.probs + pid;
                 B = stage_problem(*PP, Bd.lds_cap, lds_raw);
@@ -1646,6 +1647,20 @@ __global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut
     if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
 }
 
+// Per-pass counters reset by a kernel (the passes are replayed as HIP graphs,
+// which take no memset nodes here).
+__global__ void k_pass_reset(uint32_t *seg_count, uint32_t *overflow, int32_t *resume, int32_t *nsave_out,
+                             int32_t *more)
+{
+    if(threadIdx.x == 0) {
+        *seg_count = 0u;
+        *overflow = 0u;
+        *resume = 0;
+        *nsave_out = 0;
+        *more = 0;
+    }
+}
+
 } // namespace
 
 struct bhc_sweep {
@@ -1661,16 +1676,22 @@ struct bhc_sweep {
     float *d_fields;
     int32_t *d_stats;
     double *d_xend;
-    int32_t *d_ray_next, *d_pcursor, *d_resume, *d_nsave, *d_more;
+    int32_t *d_ray_next, *d_pcursor, *d_resume, *d_nsave, *d_more;   // d_nsave: [2], one per parity
     uint32_t *d_perr;
     unsigned long long *d_psteps, *d_counters;
     void *d_save[2];
     int32_t *d_save_pid[2];
     int max_waves;
-    int lds_bytes;
+    int lds_bytes, lds_bytes_max;
     int passes;
     int scatter_blocks;
     double trace_ms, scatter_ms;
+    // HIP graphs of one pass (reset, trace, scatter), one per parity of the
+    // saved-ray buffers; rebuilt when the launch configuration changes
+    bool use_graph;
+    hipGraphExec_t graph[2];
+    int graph_key[3];
+    bool tuned;
 };
 
 extern "C" void bhc_sweep_release(bhc_sweep *sw)
@@ -1683,6 +1704,8 @@ extern "C" void bhc_sweep_release(bhc_sweep *sw)
                     sw->d_counters, sw->d_save[0], sw->d_save[1], sw->d_save_pid[0], sw->d_save_pid[1]};
     for(void *p : ptrs)
         if(p) (void)hipFree(p);
+    for(int k = 0; k < 2; ++k)
+        if(sw->graph[k]) (void)hipGraphExecDestroy(sw->graph[k]);
     delete sw;
 }
 
@@ -1731,10 +1754,11 @@ int sweep_upload(bhc_sweep *sw, const bhc_problem2d *const *probs)
     size_t cap_reals = std::min<size_t>((size_t)40960, (size_t)ctx->lds_max) / sizeof(real);
     size_t lds_reals = std::min(max_staged, cap_reals);
     sw->lds_bytes = (int)(lds_reals * sizeof(real));
+    sw->lds_bytes_max = sw->lds_bytes;
     if(dalloc((real **)&sw->d_blobs, blobs.size()) || dalloc(&sw->d_rzsz, rzsz.size())
        || dalloc(&sw->d_fields, sw->field_floats) || dalloc((Prob2D<real> **)&sw->d_probs, (size_t)n)
        || dalloc(&sw->d_ray_next, (size_t)n) || dalloc(&sw->d_pcursor, 1) || dalloc(&sw->d_resume, 1)
-       || dalloc(&sw->d_nsave, 1) || dalloc(&sw->d_more, 1) || dalloc(&sw->d_perr, 4 * (size_t)n)
+       || dalloc(&sw->d_nsave, 2) || dalloc(&sw->d_more, 1) || dalloc(&sw->d_perr, 4 * (size_t)n)
        || dalloc(&sw->d_psteps, (size_t)n)
        || dalloc(&sw->d_counters, 2))
         return -1;
@@ -1767,23 +1791,25 @@ int sweep_upload(bhc_sweep *sw, const bhc_problem2d *const *probs)
     return 0;
 }
 
-template<typename real> int sweep_run(bhc_sweep *sw)
+template<typename real> BatchDev<real> batch_dev(bhc_sweep *sw, int nprob)
 {
-    bhc_ctx *c = sw->ctx;
-    hipStream_t s = c->stream;
-    if(ensure_segs(c)) return -1;
     BatchDev<real> Bd;
     memset(&Bd, 0, sizeof(Bd));
     Bd.probs = (const Prob2D<real> *)sw->d_probs;
-    Bd.nprob = sw->nprob;
+    Bd.nprob = nprob;
     Bd.lds_cap = sw->lds_bytes / (int)sizeof(real);
     Bd.ray_next = sw->d_ray_next;
     Bd.pcursor = sw->d_pcursor;
     Bd.perr = sw->d_perr;
     Bd.psteps = sw->d_psteps;
     Bd.resume_next = sw->d_resume;
-    Bd.nsave_out = sw->d_nsave;
     Bd.more = sw->d_more;
+    return Bd;
+}
+
+template<typename real> DevOut batch_out(bhc_sweep *sw)
+{
+    bhc_ctx *c = sw->ctx;
     DevOut O;
     memset(&O, 0, sizeof(O));
     O.counters = sw->d_counters;
@@ -1791,15 +1817,51 @@ template<typename real> int sweep_run(bhc_sweep *sw)
     O.seg_count = c->d_seg_count;
     O.overflow = c->d_overflow;
     O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg<real>), 0xFFFFFFF0u);
-    int waves = sw->max_waves;
+    return O;
+}
+
+// Enqueue one pass of parity `cur` (reads saved rays from buffer cur, saves
+// into 1 - cur) on the context's stream.
+template<typename real> void enqueue_pass(bhc_sweep *sw, BatchDev<real> Bd, const DevOut &O, int cur, hipEvent_t e_mid)
+{
+    bhc_ctx *c = sw->ctx;
+    hipStream_t s = c->stream;
+    Bd.save_in = (const RaySave<real> *)sw->d_save[cur];
+    Bd.save_in_pid = sw->d_save_pid[cur];
+    Bd.nsave_in = sw->d_nsave + cur;
+    Bd.save_out = (RaySave<real> *)sw->d_save[1 - cur];
+    Bd.save_out_pid = sw->d_save_pid[1 - cur];
+    Bd.nsave_out = sw->d_nsave + (1 - cur);
+    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, c->d_seg_count, c->d_overflow, sw->d_resume,
+                       sw->d_nsave + (1 - cur), sw->d_more);
+    hipLaunchKernelGGL(k_trace_batch<real>, dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
+    if(e_mid) (void)hipEventRecord(e_mid, s);
+    if(c->exact_phasor)
+        hipLaunchKernelGGL((k_scatter_batch<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+    else
+        hipLaunchKernelGGL((k_scatter_batch<real, true>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+}
+
+// Trace + scatter the first `nprob` problems: zero their fields and counters,
+// then passes until no ray is left. Graph mode replays one captured graph per
+// pass; eager mode times the trace and scatter kernels of every pass.
+template<typename real> int sweep_core(bhc_sweep *sw, int nprob, bool graph)
+{
+    bhc_ctx *c = sw->ctx;
+    hipStream_t s = c->stream;
+    if(ensure_segs(c)) return -1;
+    BatchDev<real> Bd = batch_dev<real>(sw, nprob);
+    DevOut O = batch_out<real>(sw);
     // every wavefront may add 2 segments per lane per step between two polls
     // (32 steps apart) plus one partly used slot block
-    size_t margin = (size_t)waves * (2 * 32 * WAVE + 2 * SEG_BLOCK);
+    size_t margin = (size_t)sw->max_waves * (2 * 32 * WAVE + 2 * SEG_BLOCK);
     if(margin * 2 >= O.seg_cap) return fail("segment buffer too small for a batched sweep (raise BHC_SEG_BUFFER_MB)");
     Bd.suspend_at = (uint32_t)(O.seg_cap - margin);
-    HIPCHECK(hipMemsetAsync(sw->d_fields, 0, sw->field_floats * sizeof(float), s));
+    size_t nf = nprob == sw->nprob ? sw->field_floats : sw->field_off[nprob];
+    HIPCHECK(hipMemsetAsync(sw->d_fields, 0, nf * sizeof(float), s));
     HIPCHECK(hipMemsetAsync(sw->d_ray_next, 0, sw->nprob * sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(sw->d_pcursor, 0, sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(sw->d_nsave, 0, 2 * sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(sw->d_perr, 0, 4 * sw->nprob * sizeof(uint32_t), s));
     HIPCHECK(hipMemsetAsync(sw->d_psteps, 0, sw->nprob * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(sw->d_counters, 0, 2 * sizeof(unsigned long long), s));
@@ -1807,8 +1869,25 @@ template<typename real> int sweep_run(bhc_sweep *sw)
         HIPCHECK(hipMemsetAsync(sw->d_stats, 0, 3 * sw->total_rays * sizeof(int32_t), s));
         HIPCHECK(hipMemsetAsync(sw->d_xend, 0, 2 * sw->total_rays * sizeof(double), s));
     }
+    if(graph) {
+        int key[3] = {sw->scatter_blocks, sw->lds_bytes, nprob};
+        if(!sw->graph[0] || memcmp(key, sw->graph_key, sizeof(key)) != 0) {
+            for(int k = 0; k < 2; ++k) {
+                if(sw->graph[k]) (void)hipGraphExecDestroy(sw->graph[k]);
+                sw->graph[k] = nullptr;
+                hipGraph_t g = nullptr;
+                HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                enqueue_pass<real>(sw, Bd, O, k, nullptr);
+                HIPCHECK(hipStreamEndCapture(s, &g));
+                hipError_t e = hipGraphInstantiate(&sw->graph[k], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCHECK(e);
+            }
+            memcpy(sw->graph_key, key, sizeof(key));
+        }
+    }
     HIPCHECK(hipEventRecord(c->ev0, s));
-    int nsave = 0, cur = 0;
+    int cur = 0;
     sw->passes = 0;
     sw->trace_ms = sw->scatter_ms = 0.0;
     hipEvent_t e0, e1, e2;
@@ -1817,26 +1896,15 @@ template<typename real> int sweep_run(bhc_sweep *sw)
     HIPCHECK(hipEventCreate(&e2));
     int rc = 0;
     for(;;) {
-        Bd.save_in = (const RaySave<real> *)sw->d_save[cur];
-        Bd.save_in_pid = sw->d_save_pid[cur];
-        Bd.nsave_in = nsave;
-        Bd.save_out = (RaySave<real> *)sw->d_save[1 - cur];
-        Bd.save_out_pid = sw->d_save_pid[1 - cur];
-        if(hipMemsetAsync(c->d_seg_count, 0, sizeof(uint32_t), s) != hipSuccess
-           || hipMemsetAsync(c->d_overflow, 0, sizeof(uint32_t), s) != hipSuccess
-           || hipMemsetAsync(sw->d_resume, 0, sizeof(int32_t), s) != hipSuccess
-           || hipMemsetAsync(sw->d_nsave, 0, sizeof(int32_t), s) != hipSuccess
-           || hipMemsetAsync(sw->d_more, 0, sizeof(int32_t), s) != hipSuccess) {
-            rc = fail("sweep: memset failed");
-            break;
-        }
         (void)hipEventRecord(e0, s);
-        hipLaunchKernelGGL(k_trace_batch<real>, dim3(waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
-        (void)hipEventRecord(e1, s);
-        if(c->exact_phasor)
-            hipLaunchKernelGGL((k_scatter_batch<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
-        else
-            hipLaunchKernelGGL((k_scatter_batch<real, true>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+        if(graph) {
+            if(hipGraphLaunch(sw->graph[cur], s) != hipSuccess) {
+                rc = fail("sweep: graph launch failed");
+                break;
+            }
+        } else {
+            enqueue_pass<real>(sw, Bd, O, cur, e1);
+        }
         (void)hipEventRecord(e2, s);
         if(hipGetLastError() != hipSuccess) {
             rc = fail("sweep: kernel launch failed");
@@ -1844,7 +1912,7 @@ template<typename real> int sweep_run(bhc_sweep *sw)
         }
         int32_t ns = 0, more = 0;
         uint32_t of = 0;
-        if(hipMemcpyAsync(&ns, sw->d_nsave, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess
+        if(hipMemcpyAsync(&ns, sw->d_nsave + (1 - cur), sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess
            || hipMemcpyAsync(&more, sw->d_more, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess
            || hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess
            || hipStreamSynchronize(s) != hipSuccess) {
@@ -1852,8 +1920,12 @@ template<typename real> int sweep_run(bhc_sweep *sw)
             break;
         }
         float a = 0.f, b = 0.f;
-        (void)hipEventElapsedTime(&a, e0, e1);
-        (void)hipEventElapsedTime(&b, e1, e2);
+        if(graph) {
+            (void)hipEventElapsedTime(&a, e0, e2);
+        } else {
+            (void)hipEventElapsedTime(&a, e0, e1);
+            (void)hipEventElapsedTime(&b, e1, e2);
+        }
         sw->trace_ms += a;
         sw->scatter_ms += b;
         ++sw->passes;
@@ -1866,7 +1938,6 @@ template<typename real> int sweep_run(bhc_sweep *sw)
             rc = fail("sweep: no progress");
             break;
         }
-        nsave = ns;
         cur = 1 - cur;
     }
     (void)hipEventDestroy(e0);
@@ -1875,6 +1946,38 @@ template<typename real> int sweep_run(bhc_sweep *sw)
     HIPCHECK(hipEventRecord(c->ev1, s));
     c->passes = sw->passes;
     return rc;
+}
+
+// Launch-configuration search on the sweep's first problems (eager passes,
+// kernel times from HIP events): the scatter grid and whether the problems'
+// tables are staged in LDS. Keeps the fastest.
+template<typename real> int sweep_tune(bhc_sweep *sw)
+{
+    int nsub = std::min(sw->nprob, 64);
+    int cus = sw->ctx->num_cus;
+    int best_blocks = sw->scatter_blocks, best_lds = sw->lds_bytes;
+    double best = 1e300;
+    const int grids[3] = {cus * 4, cus * 8, cus * 16};
+    const int ldss[2] = {sw->lds_bytes_max, 0};
+    for(int li = 0; li < (sw->lds_bytes_max > 0 ? 2 : 1); ++li)
+        for(int gi = 0; gi < 3; ++gi) {
+            sw->scatter_blocks = grids[gi];
+            sw->lds_bytes = ldss[li];
+            double t = 0.0;
+            for(int rep = 0; rep < 2; ++rep) {
+                if(sweep_core<real>(sw, nsub, false)) return -1;
+                if(rep == 1) t = sw->trace_ms + sw->scatter_ms;
+            }
+            if(t < best) {
+                best = t;
+                best_blocks = grids[gi];
+                best_lds = ldss[li];
+            }
+        }
+    sw->scatter_blocks = best_blocks;
+    sw->lds_bytes = best_lds;
+    sw->tuned = true;
+    return 0;
 }
 
 } // namespace
@@ -1897,6 +2000,8 @@ extern "C" int bhc_sweep_upload2d(bhc_ctx *ctx, const bhc_problem2d *const *prob
     sw->precision = prec;
     sw->nprob = nprob;
     sw->with_stats = with_stats;
+    sw->use_graph = true;
+    if(const char *e = getenv("BHC_SWEEP_GRAPH")) sw->use_graph = atoi(e) != 0;
     int rc = prec == 64 ? sweep_upload<double>(sw, probs) : sweep_upload<float>(sw, probs);
     if(rc) {
         bhc_sweep_release(sw);
@@ -1910,7 +2015,25 @@ extern "C" int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw)
 {
     if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_launch: bad arguments");
     HIPCHECK(hipSetDevice(ctx->device));
-    return sw->precision == 64 ? sweep_run<double>(sw) : sweep_run<float>(sw);
+    return sw->precision == 64 ? sweep_core<double>(sw, sw->nprob, sw->use_graph)
+                               : sweep_core<float>(sw, sw->nprob, sw->use_graph);
+}
+
+extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *lds_bytes)
+{
+    if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_autotune: bad arguments");
+    HIPCHECK(hipSetDevice(ctx->device));
+    int rc = sw->precision == 64 ? sweep_tune<double>(sw) : sweep_tune<float>(sw);
+    if(scatter_blocks) *scatter_blocks = sw->scatter_blocks;
+    if(lds_bytes) *lds_bytes = sw->lds_bytes;
+    return rc;
+}
+
+extern "C" int bhc_sweep_set_graph(bhc_sweep *sw, int use_graph)
+{
+    if(!sw) return fail("bhc_sweep_set_graph: null sweep");
+    sw->use_graph = use_graph != 0;
+    return 0;
 }
 
 extern "C" int bhc_sweep_stats(bhc_sweep *sw, double *trace_ms, double *scatter_ms, int *passes, long long *steps,

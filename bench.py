@@ -150,6 +150,7 @@ def run_sweep(a):
     warm.launch()
     warm.release()
     sw = bh.Sweep(probs, ctx)
+    tuned = sw.autotune()
     R = sum(p.nrays for p in probs)
     for _ in range(max(0, a.warmup - 1)):
         sw.launch()
@@ -205,6 +206,7 @@ def run_sweep(a):
         "plain_per_env": {"value": plain_fields_s, "unit": "TL fields/s",
                           "sample": "%d of the %d problems, one resident launch each" % (len(sample), len(probs))},
         "batched_over_plain": fields_s / plain_fields_s,
+        "autotuned": tuned,
         "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": None, "kernel_ms": k_ms, "trace_ms": sum(tr_ms) / len(tr_ms),
