@@ -64,7 +64,7 @@ def lib():
         "bhc_sweep_stats": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), P(C.c_longlong),
                                       P(C.c_longlong)]),
         "bhc_sweep_release": (None, [C.c_void_p]),
-        "bhc_sweep_autotune": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_int)]),
+        "bhc_sweep_autotune": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "bhc_sweep_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
         "bhc_last_phase_ms": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int)]),
         "bhc_opcount2d": (C.c_int, [P(_abi.Problem2D), C.c_int, P(C.c_double)]),
@@ -292,9 +292,9 @@ class Sweep:
         _check(lib().bhc_sweep_launch(self.ctx._h, self._h))
 
     def autotune(self):
-        b, l = C.c_int(), C.c_int()
-        _check(lib().bhc_sweep_autotune(self.ctx._h, self._h, C.byref(b), C.byref(l)))
-        return {"scatter_blocks": b.value, "lds_bytes": l.value}
+        b, l, m = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().bhc_sweep_autotune(self.ctx._h, self._h, C.byref(b), C.byref(l), C.byref(m)))
+        return {"scatter_blocks": b.value, "lds_bytes": l.value, "scatter_lanes": m.value}
 
     def set_graph(self, on):
         _check(lib().bhc_sweep_set_graph(self._h, 1 if on else 0))

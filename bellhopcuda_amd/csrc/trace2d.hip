@@ -1446,23 +1446,40 @@ __device__ inline void flush_problem_err(Err &e, uint32_t *perr)
     e = Err{0, 0, 0, 0};
 }
 
-template<typename real>
-__global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
+// One wavefront on one problem until the problem has no unclaimed rays and
+// the wavefront's rays have ended (returns 0), or until the segment buffer
+// nears its end and the rays are saved (returns 1). `resume` restores a saved
+// wavefront. The SSP type is a template parameter, as in the single-problem
+// kernel, so that each type runs specialised code.
+template<typename real, char ST>
+__device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bd, const DevOut &O, int pid,
+                                                     const RaySave<real> *resume, SegBlock &blk, unsigned char *lds,
+                                                     unsigned long long &all_steps)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x;
-    int pid = -1;
-    const Prob2D<real> *PP = Bd.probs;
-    const real *B = nullptr;
-    bool pexh = true, active = false, suspended = false;
+    const Prob2D<real> &P = Bd.probs[pid];
+    const real *B = stage_problem(P, Bd.lds_cap, lds);
+    const real *G = P.base;
+    const int nr = P.njobs;
+    bool pexh = false, active = false;
+    int result = 0;
     Err e = {0, 0, 0, 0};
     RayPt<real> p0;
     Trace<real> S;
     Infl<real> I;
     int job = 0, isz = 0, ia = 0, is = 0;
-    unsigned long long steps = 0, all_steps = 0;
-    SegBlock blk{0u, 0u, 0u};
-    const int nsave_in = __builtin_amdgcn_readfirstlane(*Bd.nsave_in);
+    unsigned long long steps = 0;
+    if(resume) {
+        const RaySave<real> &sv = resume[lane];
+        p0 = sv.p0;
+        S = sv.S;
+        I = sv.I;
+        job = sv.job;
+        is = sv.is;
+        isz = sv.isz;
+        ia = sv.ia;
+        active = sv.active != 0;
+    }
     for(int iter = 0;; ++iter) {
         if(!pexh) {
             unsigned long long need = __ballot(!active);
@@ -1472,84 +1489,26 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
                 int base = 0;
                 if(lane == leader) base = atomicAdd(Bd.ray_next + pid, cnt);
                 base = __shfl(base, leader);
-                const int nr = PP->njobs;
                 if(base + cnt >= nr) pexh = true;
                 if(!active) {
                     int my = base + __popcll(need & ((1ull << lane) - 1ull));
                     if(my < nr) {
                         job = my;
-                        JobIndices(job, *PP, isz, ia);
+                        JobIndices(job, P, isz, ia);
                         is = 0;
-                        active = RayInit<real, 0>(isz, ia, p0, S, I, *PP, B, PP->base, e);
-                        if(!active && PP->ray_stats) {
-                            PP->ray_stats[3 * job] = 0;
-                            PP->ray_stats[3 * job + 1] = 0;
-                            PP->ray_stats[3 * job + 2] = 0;
-                            PP->ray_xend[2 * job] = 0.0;
-                            PP->ray_xend[2 * job + 1] = 0.0;
+                        active = RayInit<real, ST>(isz, ia, p0, S, I, P, B, G, e);
+                        if(!active && P.ray_stats) {
+                            P.ray_stats[3 * job] = 0;
+                            P.ray_stats[3 * job + 1] = 0;
+                            P.ray_stats[3 * job + 2] = 0;
+                            P.ray_xend[2 * job] = 0.0;
+                            P.ray_xend[2 * job + 1] = 0.0;
                         }
                     }
                 }
             }
         }
-        if(!__any(active)) {
-            // the wavefront is idle: close the current problem's books, then
-            // take a saved wavefront or the next problem with unclaimed rays
-            if(pid >= 0) {
-                flush_problem_err(e, Bd.perr + 4 * pid);
-                unsigned long long t = steps;
-                for(int off = WAVE / 2; off > 0; off >>= 1) t += __shfl_xor(t, off);
-                if(lane == 0 && t) atomicAdd(Bd.psteps + pid, t);
-                all_steps += steps;
-                steps = 0;
-            }
-            if(suspended) break;
-            int u = -1;
-            if(nsave_in > 0) {
-                if(lane == 0) u = atomicAdd(Bd.resume_next, 1);
-                u = __shfl(u, 0);
-            }
-            if(u >= 0 && u < nsave_in) {
-                pid = __builtin_amdgcn_readfirstlane(Bd.save_in_pid[u]);
-                PP = Bd.probs + pid;
-                B = stage_problem(*PP, Bd.lds_cap, lds_raw);
-                const RaySave<real> &sv = Bd.save_in[(size_t)u * WAVE + lane];
-                p0 = sv.p0;
-                S = sv.S;
-                I = sv.I;
-                job = sv.job;
-                is = sv.is;
-                isz = sv.isz;
-                ia = sv.ia;
-                active = sv.active != 0;
-                pexh = false;
-                continue;
-            }
-            // no new problems once the segment buffer nears its end: the next
-            // pass continues from the work counters
-            uint32_t sc = __hip_atomic_load(O.seg_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if(__builtin_amdgcn_readfirstlane(sc) >= Bd.suspend_at) {
-                if(lane == 0) atomicOr(Bd.more, 1);
-                break;
-            }
-            int p = 0;
-            if(lane == 0) {
-                p = __hip_atomic_load(Bd.pcursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                while(p < Bd.nprob
-                      && __hip_atomic_load(Bd.ray_next + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                          >= Bd.probs[p].njobs) {
-                    atomicMax(Bd.pcursor, p + 1);
-                    ++p;
-                }
-            }
-            p = __builtin_amdgcn_readfirstlane(__shfl(p, 0));
-            if(p >= Bd.nprob) break;
-            pid = p;
-            PP = Bd.probs + pid;
-            B = stage_problem(*PP, Bd.lds_cap, lds_raw);
-            pexh = false;
-            continue;
-        }
+        if(!__any(active)) break;
         if((iter & 31) == 31) {
             // HasErrored of this problem (its rays stop, other problems go on),
             // and suspension when the segment buffer nears its end
@@ -1557,12 +1516,10 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
             uint32_t ge = __hip_atomic_load(Bd.perr + 4 * pid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t sc = __hip_atomic_load(O.seg_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if(__any(ge != 0u || e.err != 0u)) {
-                if(lane == 0) atomicMax(Bd.ray_next + pid, PP->njobs);
-                active = false;
-                pexh = true;
-                continue;
+                if(lane == 0) atomicMax(Bd.ray_next + pid, nr);
+                break;
             }
-            if(sc >= Bd.suspend_at) {
+            if(__builtin_amdgcn_readfirstlane(sc) >= Bd.suspend_at) {
                 int u = 0;
                 if(lane == 0) u = atomicAdd(Bd.nsave_out, 1);
                 u = __shfl(u, 0);
@@ -1576,20 +1533,17 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
                 sv.isz = isz;
                 sv.ia = ia;
                 sv.active = active ? 1 : 0;
-                active = false;
-                pexh = true;
-                suspended = true;
-                continue;
+                result = 1;
+                break;
             }
         }
-        const Prob2D<real> &P = *PP;
         RayPt<real> p1, p2;
         SegDesc<real> d;
         d.nitems = 0;
         bool two = false;
         if(active) {
-            two = RayUpdate<real, 0>(p0, p1, p2, S, P, B, P.base, e);
-            prelude(d, p0, p1, I, is, P, P.base, isz, ia);
+            two = RayUpdate<real, ST>(p0, p1, p2, S, P, B, G, e);
+            prelude(d, p0, p1, I, is, P, G, isz, ia);
             d.prob = pid;
             ++is;
         }
@@ -1597,7 +1551,7 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
-                prelude(d, p1, p2, I, is, P, P.base, isz, ia);
+                prelude(d, p1, p2, I, is, P, G, isz, ia);
                 d.prob = pid;
                 ++is;
             }
@@ -1617,6 +1571,69 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
                 }
             }
         }
+    }
+    flush_problem_err(e, Bd.perr + 4 * pid);
+    unsigned long long t = steps;
+    for(int off = WAVE / 2; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if(lane == 0 && t) atomicAdd(Bd.psteps + pid, t);
+    all_steps += steps;
+    return result;
+}
+
+template<typename real>
+__device__ inline int run_problem_any(const BatchDev<real> &Bd, const DevOut &O, int pid, const RaySave<real> *resume,
+                                      SegBlock &blk, unsigned char *lds, unsigned long long &all_steps)
+{
+    switch(Bd.probs[pid].ssp_type) {
+    case 'C': return run_problem<real, 'C'>(Bd, O, pid, resume, blk, lds, all_steps);
+    case 'P': return run_problem<real, 'P'>(Bd, O, pid, resume, blk, lds, all_steps);
+    case 'S': return run_problem<real, 'S'>(Bd, O, pid, resume, blk, lds, all_steps);
+    case 'N': return run_problem<real, 'N'>(Bd, O, pid, resume, blk, lds, all_steps);
+    case 'Q': return run_problem<real, 'Q'>(Bd, O, pid, resume, blk, lds, all_steps);
+    default: return run_problem<real, 'A'>(Bd, O, pid, resume, blk, lds, all_steps);
+    }
+}
+
+template<typename real>
+__global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x;
+    SegBlock blk{0u, 0u, 0u};
+    unsigned long long all_steps = 0;
+    const int nsave_in = __builtin_amdgcn_readfirstlane(*Bd.nsave_in);
+    for(;;) {
+        // a saved wavefront of the previous pass first
+        int u = -1;
+        if(nsave_in > 0) {
+            if(lane == 0) u = atomicAdd(Bd.resume_next, 1);
+            u = __builtin_amdgcn_readfirstlane(__shfl(u, 0));
+        }
+        if(u >= 0 && u < nsave_in) {
+            int pid = __builtin_amdgcn_readfirstlane(Bd.save_in_pid[u]);
+            if(run_problem_any(Bd, O, pid, Bd.save_in + (size_t)u * WAVE, blk, lds_raw, all_steps)) break;
+            continue;
+        }
+        // no new problems once the segment buffer nears its end: the next
+        // pass continues from the work counters
+        uint32_t sc = __hip_atomic_load(O.seg_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if(__builtin_amdgcn_readfirstlane(sc) >= Bd.suspend_at) {
+            if(lane == 0) atomicOr(Bd.more, 1);
+            break;
+        }
+        int p = 0;
+        if(lane == 0) {
+            p = __hip_atomic_load(Bd.pcursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while(p < Bd.nprob
+                  && __hip_atomic_load(Bd.ray_next + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      >= Bd.probs[p].njobs) {
+                atomicMax(Bd.pcursor, p + 1);
+                ++p;
+            }
+        }
+        p = __builtin_amdgcn_readfirstlane(__shfl(p, 0));
+        if(p >= Bd.nprob) break;
+        if(run_problem_any<real>(Bd, O, p, nullptr, blk, lds_raw, all_steps)) break;
     }
     seal_block<real>(blk, lane, O);
     for(int off = WAVE / 2; off > 0; off >>= 1) all_steps += __shfl_xor(all_steps, off);
@@ -1645,6 +1662,31 @@ __global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
     if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
+// Scatter of a batched pass with one lane per segment: a wavefront loads 64
+// descriptors at once (one memory round trip for all of them) and every lane
+// walks its own segment's receivers. Batched sweeps of hat beams produce a
+// segment per step with only a few receivers each, where a wavefront per
+// segment would spend its time waiting for the descriptor.
+template<typename real, bool FAST>
+__global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut O)
+{
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    unsigned long long nc = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+        const SegDesc<real> d = segs[s].d;
+        if(d.nitems <= 0) continue;
+        const Prob2D<real> &P = Bd.probs[d.prob];
+        DevSink<real> sink{&O, &P, 0};
+        for(int it = 0; it < d.nitems; ++it) item<real, RT_TL, DevSink<real>, FAST>(d, it, P, P.base, P.Rz, sink);
+        nc += sink.nc;
+    }
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if((threadIdx.x & (WAVE - 1)) == 0 && nc) atomicAdd(O.counters + 1, nc);
 }
 
 // Per-pass counters reset by a kernel (the passes are replayed as HIP graphs,
@@ -1685,12 +1727,13 @@ struct bhc_sweep {
     int lds_bytes, lds_bytes_max;
     int passes;
     int scatter_blocks;
+    int scatter_lanes;          // 1: one lane per segment, 0: one wavefront per segment
     double trace_ms, scatter_ms;
     // HIP graphs of one pass (reset, trace, scatter), one per parity of the
     // saved-ray buffers; rebuilt when the launch configuration changes
     bool use_graph;
     hipGraphExec_t graph[2];
-    int graph_key[3];
+    int graph_key[4];
     bool tuned;
 };
 
@@ -1788,6 +1831,7 @@ int sweep_upload(bhc_sweep *sw, const bhc_problem2d *const *probs)
         if(dalloc((RaySave<real> **)&sw->d_save[k], save_units * WAVE) || dalloc(&sw->d_save_pid[k], save_units))
             return -1;
     sw->scatter_blocks = ctx->num_cus * 8;
+    sw->scatter_lanes = 1;
     return 0;
 }
 
@@ -1836,10 +1880,17 @@ template<typename real> void enqueue_pass(bhc_sweep *sw, BatchDev<real> Bd, cons
                        sw->d_nsave + (1 - cur), sw->d_more);
     hipLaunchKernelGGL(k_trace_batch<real>, dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
     if(e_mid) (void)hipEventRecord(e_mid, s);
-    if(c->exact_phasor)
-        hipLaunchKernelGGL((k_scatter_batch<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
-    else
-        hipLaunchKernelGGL((k_scatter_batch<real, true>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+    if(sw->scatter_lanes) {
+        if(c->exact_phasor)
+            hipLaunchKernelGGL((k_scatter_lanes<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+        else
+            hipLaunchKernelGGL((k_scatter_lanes<real, true>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+    } else {
+        if(c->exact_phasor)
+            hipLaunchKernelGGL((k_scatter_batch<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+        else
+            hipLaunchKernelGGL((k_scatter_batch<real, true>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
+    }
 }
 
 // Trace + scatter the first `nprob` problems: zero their fields and counters,
@@ -1870,7 +1921,7 @@ template<typename real> int sweep_core(bhc_sweep *sw, int nprob, bool graph)
         HIPCHECK(hipMemsetAsync(sw->d_xend, 0, 2 * sw->total_rays * sizeof(double), s));
     }
     if(graph) {
-        int key[3] = {sw->scatter_blocks, sw->lds_bytes, nprob};
+        int key[4] = {sw->scatter_blocks, sw->lds_bytes, nprob, sw->scatter_lanes};
         if(!sw->graph[0] || memcmp(key, sw->graph_key, sizeof(key)) != 0) {
             for(int k = 0; k < 2; ++k) {
                 if(sw->graph[k]) (void)hipGraphExecDestroy(sw->graph[k]);
@@ -1955,27 +2006,31 @@ template<typename real> int sweep_tune(bhc_sweep *sw)
 {
     int nsub = std::min(sw->nprob, 64);
     int cus = sw->ctx->num_cus;
-    int best_blocks = sw->scatter_blocks, best_lds = sw->lds_bytes;
+    int best_blocks = sw->scatter_blocks, best_lds = sw->lds_bytes, best_lanes = sw->scatter_lanes;
     double best = 1e300;
     const int grids[3] = {cus * 4, cus * 8, cus * 16};
     const int ldss[2] = {sw->lds_bytes_max, 0};
-    for(int li = 0; li < (sw->lds_bytes_max > 0 ? 2 : 1); ++li)
-        for(int gi = 0; gi < 3; ++gi) {
-            sw->scatter_blocks = grids[gi];
-            sw->lds_bytes = ldss[li];
-            double t = 0.0;
-            for(int rep = 0; rep < 2; ++rep) {
-                if(sweep_core<real>(sw, nsub, false)) return -1;
-                if(rep == 1) t = sw->trace_ms + sw->scatter_ms;
+    for(int mode = 0; mode < 2; ++mode)
+        for(int li = 0; li < (sw->lds_bytes_max > 0 ? 2 : 1); ++li)
+            for(int gi = 0; gi < 3; ++gi) {
+                sw->scatter_lanes = mode;
+                sw->scatter_blocks = grids[gi];
+                sw->lds_bytes = ldss[li];
+                double t = 0.0;
+                for(int rep = 0; rep < 2; ++rep) {
+                    if(sweep_core<real>(sw, nsub, false)) return -1;
+                    if(rep == 1) t = sw->trace_ms + sw->scatter_ms;
+                }
+                if(t < best) {
+                    best = t;
+                    best_blocks = grids[gi];
+                    best_lds = ldss[li];
+                    best_lanes = mode;
+                }
             }
-            if(t < best) {
-                best = t;
-                best_blocks = grids[gi];
-                best_lds = ldss[li];
-            }
-        }
     sw->scatter_blocks = best_blocks;
     sw->lds_bytes = best_lds;
+    sw->scatter_lanes = best_lanes;
     sw->tuned = true;
     return 0;
 }
@@ -2019,13 +2074,15 @@ extern "C" int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw)
                                : sweep_core<float>(sw, sw->nprob, sw->use_graph);
 }
 
-extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *lds_bytes)
+extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *lds_bytes,
+                                  int *scatter_lanes)
 {
     if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_autotune: bad arguments");
     HIPCHECK(hipSetDevice(ctx->device));
     int rc = sw->precision == 64 ? sweep_tune<double>(sw) : sweep_tune<float>(sw);
     if(scatter_blocks) *scatter_blocks = sw->scatter_blocks;
     if(lds_bytes) *lds_bytes = sw->lds_bytes;
+    if(scatter_lanes) *scatter_lanes = sw->scatter_lanes;
     return rc;
 }
 
