@@ -56,6 +56,8 @@ def lib():
         "bhc_field_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
         "bhc_field_download": (C.c_int, [C.c_void_p, C.c_void_p, P(_abi.FieldOut)]),
         "bhc_problem_release": (None, [C.c_void_p]),
+        "bhc_problem_autotune": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int)]),
+        "bhc_problem_set_scatter": (C.c_int, [C.c_void_p, C.c_int]),
         "bhc_scale_pressure2d": (C.c_int, [P(_abi.Problem2D), _abi.c_float_p]),
         "bhc_write_shd2d": (C.c_int, [C.c_char_p, P(_abi.Problem2D), C.c_char_p, _abi.c_float_p]),
         "bhc_sweep_upload2d": (C.c_int, [C.c_void_p, P(P(_abi.Problem2D)), C.c_int, C.c_int, P(C.c_void_p)]),
@@ -232,6 +234,15 @@ class Resident:
 
     def launch(self):
         _check(lib().bhc_field_launch(self.ctx._h, self._h))
+
+    def autotune(self):
+        """Pick the fastest scatter mapping; returns 1 (tiles), 2 (wavefront) or 3 (lane)."""
+        m = C.c_int()
+        _check(lib().bhc_problem_autotune(self.ctx._h, self._h, C.byref(m)))
+        return m.value
+
+    def set_scatter(self, mode):
+        _check(lib().bhc_problem_set_scatter(self._h, mode))
 
     def download(self):
         _check(lib().bhc_field_download(self.ctx._h, self._h, C.byref(self.out)))

@@ -391,6 +391,26 @@ __global__ void __launch_bounds__(256) k_scatter2d(Prob2D<real> P, DevOut O)
 // ~10^3 contributions per receiver per field, this removes almost all global
 // atomics.
 // ---------------------------------------------------------------------------
+// Single-problem scatter with one lane per segment (segments with few
+// receivers each: hat beams on coarse depth windows).
+template<typename real, int RT, bool FAST>
+__global__ void __launch_bounds__(256) k_scatter2d_lanes(Prob2D<real> P, DevOut O)
+{
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    DevSink<real> sink{&O, &P, 0};
+    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+        const SegDesc<real> d = segs[s].d;
+        for(int it = 0; it < d.nitems; ++it) item<real, RT, DevSink<real>, FAST>(d, it, P, P.base, P.Rz, sink);
+    }
+    for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
+    if((threadIdx.x & (WAVE - 1)) == 0 && sink.nc) atomicAdd(O.counters + 1, sink.nc);
+}
+
+enum : int { SCATTER_AUTO = 0, SCATTER_TILED = 1, SCATTER_WAVE = 2, SCATTER_LANES = 3 };
+
 struct BinCfg {
     int32_t nbins, bin_w, tile_w, tile_lo_off; // tile columns = [b*bin_w - tile_lo_off, + tile_w)
     int32_t chunk;                              // segments per workgroup
@@ -813,6 +833,7 @@ struct bhc_dev_problem {
     double freq0;
     // range bins of the tiled TL scatter
     int nbins, bin_w, tile_w, tile_lo_off, chunk, max_chunks;
+    int scatter_mode;           // SCATTER_AUTO / _TILED / _WAVE / _LANES
     int32_t *d_bins;            // counts | offsets | cursor | chunk_start | nchunks
 };
 
@@ -1182,7 +1203,16 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
             continue;
         }
         if(first) HIPCHECK(hipEventRecord(c->evm, s));
-        if(RT == RT_TL && c->use_tiles && dp->nbins > 0) {
+        int mode = dp->scatter_mode;
+        if(mode == SCATTER_AUTO) mode = (RT == RT_TL && c->use_tiles && dp->nbins > 0) ? SCATTER_TILED : SCATTER_WAVE;
+        if(mode == SCATTER_TILED && !(RT == RT_TL && dp->nbins > 0)) mode = SCATTER_WAVE;
+        if(mode == SCATTER_LANES) {
+            int sblocks = c->num_cus * 8;
+            if(RT == RT_TL && !c->exact_phasor)
+                hipLaunchKernelGGL((k_scatter2d_lanes<real, RT, true>), dim3(sblocks), dim3(256), 0, s, P, O);
+            else
+                hipLaunchKernelGGL((k_scatter2d_lanes<real, RT, false>), dim3(sblocks), dim3(256), 0, s, P, O);
+        } else if(mode == SCATTER_TILED) {
             BinCfg B;
             B.nbins = dp->nbins;
             B.bin_w = dp->bin_w;
@@ -1224,6 +1254,8 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
 }
 } // namespace
 
+extern "C" int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode_out);
+
 extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
 {
     if(!dp || dp->ctx != ctx) return fail("bhc_field_launch: problem not uploaded on this context");
@@ -1258,6 +1290,43 @@ extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
                            dp->d_errw);
         HIPCHECK(hipGetLastError());
     }
+    return 0;
+}
+
+extern "C" int bhc_problem_set_scatter(bhc_dev_problem *dp, int mode)
+{
+    if(!dp || mode < SCATTER_AUTO || mode > SCATTER_LANES) return fail("bhc_problem_set_scatter: bad arguments");
+    dp->scatter_mode = mode;
+    return 0;
+}
+
+// Scatter-mapping search for a resident problem: every available mapping
+// (range-binned LDS tiles for TL runs, one wavefront per segment, one lane per
+// segment) is launched twice and the fastest kept for later launches.
+extern "C" int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode_out)
+{
+    if(!dp || dp->ctx != ctx) return fail("bhc_problem_autotune: problem not uploaded on this context");
+    int modes[3] = {SCATTER_TILED, SCATTER_WAVE, SCATTER_LANES};
+    int best_mode = SCATTER_AUTO;
+    double best = 1e300;
+    bool tl = dp->run0 == 'C' || dp->run0 == 'S' || dp->run0 == 'I';
+    for(int m : modes) {
+        if(m == SCATTER_TILED && !(tl && dp->nbins > 0 && ctx->use_tiles)) continue;
+        dp->scatter_mode = m;
+        double t = 0.0;
+        for(int rep = 0; rep < 2; ++rep) {
+            if(bhc_field_launch(ctx, dp)) return -1;
+            double ms = 0.0;
+            if(bhc_last_kernel_ms(ctx, &ms)) return -1;
+            if(rep == 1) t = ms;
+        }
+        if(t < best) {
+            best = t;
+            best_mode = m;
+        }
+    }
+    dp->scatter_mode = best_mode;
+    if(mode_out) *mode_out = best_mode;
     return 0;
 }
 

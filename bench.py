@@ -175,9 +175,10 @@ def run_sweep(a):
     step = max(1, len(probs) // a.plain_sample)
     sample = probs[::step][:a.plain_sample]
     t_plain = 0.0
+    modes = []
     for p in sample:
         r = bh.Resident(p, ctx)
-        r.launch()
+        modes.append(r.autotune())
         L.bhc_ctx_synchronize(ctx._h)
         t1 = time.perf_counter()
         r.launch()
@@ -204,7 +205,9 @@ def run_sweep(a):
                                    len(probs), "" if a.sweep_scale == 1.0 else ", beams/grids scaled %.3g" % a.sweep_scale),
                    "problems": len(probs), "rays": R, "parallelism": "single GPU"},
         "plain_per_env": {"value": plain_fields_s, "unit": "TL fields/s",
-                          "sample": "%d of the %d problems, one resident launch each" % (len(sample), len(probs))},
+                          "sample": "%d of the %d problems, one resident launch each with its autotuned scatter "
+                                    "mapping (tiles / wavefront / lane: %d / %d / %d)" % (
+                                        len(sample), len(probs), modes.count(1), modes.count(2), modes.count(3))},
         "batched_over_plain": fields_s / plain_fields_s,
         "autotuned": tuned,
         "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",

@@ -305,6 +305,12 @@ int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, const bhc_fiel
 int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp);   /* zero field + trace, async */
 int bhc_field_download(bhc_ctx *ctx, bhc_dev_problem *dp, bhc_field_out *out);
 void bhc_problem_release(bhc_dev_problem *dp);
+/* Scatter-mapping search for a resident problem (range-binned LDS tiles, one
+ * wavefront or one lane per step segment): each is launched twice, the fastest
+ * is kept for later launches; *mode_out = 1 tiles, 2 wavefront, 3 lane. */
+int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode_out);
+/* Force a scatter mapping (0 = automatic choice, 1 tiles, 2 wavefront, 3 lane). */
+int bhc_problem_set_scatter(bhc_dev_problem *dp, int mode);
 int bhc_ctx_synchronize(bhc_ctx *ctx);
 /* Duration (ms) of the tracer kernel of the last launch on this context,
  * from HIP events recorded on the context's stream around it. */
