@@ -16,6 +16,7 @@
 // (k, n) of B is B[k * sbk + n * sbn], so the backward products read the
 // forward tensors transposed without copies. Members are the grid's z axis.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -84,7 +85,7 @@ __device__ inline float gelu_d(float z)
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-enum : int { EPI_STORE = 0, EPI_BIAS_GELU = 1 };
+enum : int { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_ATOMIC = 2 };
 
 struct GemmArgs {
     const void *A;
@@ -96,6 +97,7 @@ struct GemmArgs {
     const float *bias;        // EPI_BIAS_GELU: bias[n] (member stride zbias)
     long long zbias;
     bf16_t *H;                // EPI_BIAS_GELU: GELU(C) as BF16, same layout as C
+    int ksplit;               // EPI_ATOMIC: K is split over ksplit slices (grid z = members x ksplit)
 };
 
 // C = A * B (FP32), or with EPI_BIAS_GELU: C = A * B + bias and H = GELU(C).
@@ -105,15 +107,22 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
 {
     __shared__ __align__(16) bf16_t As[TILE][KSTEP + 8];
     __shared__ __align__(16) bf16_t Bt[TILE][KSTEP + 8];
-    const int z = blockIdx.z;
+    const int ks = EPI == EPI_ATOMIC ? (int)(blockIdx.z % g.ksplit) : 0;
+    const int z = EPI == EPI_ATOMIC ? (int)(blockIdx.z / g.ksplit) : (int)blockIdx.z;
     const TA *A = static_cast<const TA *>(g.A) + z * g.zA;
     const TB *B = static_cast<const TB *>(g.B) + z * g.zB;
     float *C = g.C + z * g.zC;
     const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    int kbeg = 0, kend = g.K;
+    if constexpr(EPI == EPI_ATOMIC) {
+        int per = (g.K / KSTEP + g.ksplit - 1) / g.ksplit * KSTEP;
+        kbeg = ks * per;
+        kend = min(g.K, kbeg + per);
+    }
     f32x16 acc = {};
-    for(int k0 = 0; k0 < g.K; k0 += KSTEP) {
+    for(int k0 = kbeg; k0 < kend; k0 += KSTEP) {
         // stage A[m0 .. m0+63][k0 .. k0+31] and B[k0 .. +31][n0 .. +63] (transposed)
         for(int i = t; i < TILE * KSTEP; i += 256) {
             int r = i / KSTEP, c = i % KSTEP;
@@ -154,7 +163,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
             v += g.bias[z * g.zbias + n];
             g.H[z * g.zC + m * g.ldc + n] = f2bf(gelu(v));
         }
-        C[m * g.ldc + n] = v;
+        if constexpr(EPI == EPI_ATOMIC) unsafeAtomicAdd(C + m * g.ldc + n, v);
+        else C[m * g.ldc + n] = v;
     }
 }
 
@@ -202,19 +212,30 @@ __global__ void k_out_bwd(const float *out, const float *y, int n, const bf16_t 
     for(int k = threadIdx.x; k < WIDTH; k += blockDim.x) dH[e * zH + (long long)b * WIDTH + k] = g * bf2f(w[k]);
 }
 
-// dWout[e][k] = sum_b H5[e][b][k] dOut[e][b]; dbout[e] = sum_b dOut[e][b].
+constexpr int RCHUNK = 128; // batch rows per block of the column reductions
+
+// dWout[e][k] += sum over a chunk of b of H5[e][b][k] dOut[e][b]; dbout likewise
+// (G's output-layer slots are zeroed first).
 __global__ void k_out_wgrad(const bf16_t *H, const float *dout, int n, long long zH, float *G)
 {
-    int e = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
-    if(k > WIDTH) return;
+    int e = blockIdx.y, k = threadIdx.x;
+    int b0 = blockIdx.x * RCHUNK, b1 = min(n, b0 + RCHUNK);
     float s = 0.f;
     if(k < WIDTH) {
-        for(int b = 0; b < n; ++b) s += bf2f(H[e * zH + (long long)b * WIDTH + k]) * dout[(long long)e * n + b];
-        G[e * N_PARAM + OFF_WOUT + k] = s;
-    } else {
-        for(int b = 0; b < n; ++b) s += dout[(long long)e * n + b];
-        G[e * N_PARAM + OFF_BOUT] = s;
+        for(int b = b0; b < b1; ++b) s += bf2f(H[e * zH + (long long)b * WIDTH + k]) * dout[(long long)e * n + b];
+        unsafeAtomicAdd(G + e * N_PARAM + OFF_WOUT + k, s);
+    } else if(k == WIDTH) {
+        for(int b = b0; b < b1; ++b) s += dout[(long long)e * n + b];
+        unsafeAtomicAdd(G + e * N_PARAM + OFF_BOUT, s);
     }
+}
+
+// zero the gradient slots that are accumulated by atomics: biases, output
+// layer, and the weights of the split-K products
+__global__ void k_zero(float *p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) p[i] = 0.0f;
 }
 
 // dZ = dH * GELU'(Z) in place of dH (FP32).
@@ -224,14 +245,15 @@ __global__ void k_gelu_bwd(float *dH, const float *Z, long long total)
     if(i < total) dH[i] *= gelu_d(Z[i]);
 }
 
-// db[e][n] = sum_b dZ[e][b][n] (one thread per column).
+// db[e][c] += sum over a chunk of b of dZ[e][b][c] (one thread per column).
 __global__ void k_bias_grad(const float *dZ, int n, long long zZ, float *G, size_t off)
 {
-    int e = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+    int e = blockIdx.y, c = threadIdx.x;
+    int b0 = blockIdx.x * RCHUNK, b1 = min(n, b0 + RCHUNK);
     if(c >= WIDTH) return;
     float s = 0.f;
-    for(int b = 0; b < n; ++b) s += dZ[e * zZ + (long long)b * WIDTH + c];
-    G[e * N_PARAM + off + c] = s;
+    for(int b = b0; b < b1; ++b) s += dZ[e * zZ + (long long)b * WIDTH + c];
+    unsafeAtomicAdd(G + e * N_PARAM + off + c, s);
 }
 
 // Adam (FP32 master weights) and the BF16 copy the products read.
@@ -294,7 +316,7 @@ namespace {
 template<typename TA, typename TB, int EPI>
 void gemm(hipStream_t s, int members, const void *A, long long sam, long long sak, long long zA, const void *B,
           long long sbk, long long sbn, long long zB, float *C, long long ldc, long long zC, int M, int N, int K,
-          const float *bias = nullptr, long long zbias = 0, bf16_t *H = nullptr)
+          const float *bias = nullptr, long long zbias = 0, bf16_t *H = nullptr, int ksplit = 1)
 {
     GemmArgs g;
     g.A = A;
@@ -314,7 +336,8 @@ void gemm(hipStream_t s, int members, const void *A, long long sam, long long sa
     g.bias = bias;
     g.zbias = zbias;
     g.H = H;
-    dim3 grid((N + TILE - 1) / TILE, (M + TILE - 1) / TILE, members);
+    g.ksplit = ksplit;
+    dim3 grid((N + TILE - 1) / TILE, (M + TILE - 1) / TILE, members * (EPI == EPI_ATOMIC ? ksplit : 1));
     hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, g);
 }
 
@@ -342,22 +365,27 @@ void backward(bhc_mlp *m, int n)
     hipStream_t s = m->stream;
     const int E = m->members;
     const long long zH = (long long)m->max_batch * WIDTH;
-    hipMemsetAsync(m->loss, 0, E * sizeof(float), s);
+    const long long np = (long long)E * N_PARAM;
+    const int nchunk = (n + RCHUNK - 1) / RCHUNK;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, m->G, np);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, m->loss, (long long)E);
     hipLaunchKernelGGL(k_out_bwd, dim3(n, E), dim3(256), 0, s, m->out, m->y, n, m->Pbf, m->dH, zH, m->dout,
                        m->loss);
-    hipLaunchKernelGGL(k_out_wgrad, dim3((WIDTH + 1 + 255) / 256, E), dim3(256), 0, s, m->H[N_HID - 1], m->dout, n,
-                       zH, m->G);
+    hipLaunchKernelGGL(k_out_wgrad, dim3(nchunk, E), dim3(WIDTH + 64), 0, s, m->H[N_HID - 1], m->dout, n, zH,
+                       m->G);
     float *dH = m->dH, *dHn = m->dH2;
     for(int l = N_HID - 1; l >= 0; --l) {
         const long long tot = (long long)E * zH;
         hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dH, m->Z[l], tot);
-        hipLaunchKernelGGL(k_bias_grad, dim3(1, E), dim3(WIDTH), 0, s, dH, n, zH, m->G, off_b(l));
+        hipLaunchKernelGGL(k_bias_grad, dim3(nchunk, E), dim3(WIDTH), 0, s, dH, n, zH, m->G, off_b(l));
         const int K_in = l == 0 ? D_FF : WIDTH;
-        // dW = Hprev^T dZ : M = K_in, N = WIDTH, K = n
+        // dW = Hprev^T dZ : M = K_in, N = WIDTH, K = n, split over K so that
+        // the 16 (or 4) output tiles per member still fill the GPU
         const void *Hp = l == 0 ? (const void *)m->F : (const void *)m->H[l - 1];
         const long long zHp = l == 0 ? 0 : zH;
-        gemm<bf16_t, float, EPI_STORE>(s, E, Hp, 1, K_in, zHp, dH, WIDTH, 1, zH, m->G + off_W(l), WIDTH, N_PARAM,
-                                       K_in, WIDTH, n);
+        const int ksplit = std::max(1, std::min(64, n / 512));
+        gemm<bf16_t, float, EPI_ATOMIC>(s, E, Hp, 1, K_in, zHp, dH, WIDTH, 1, zH, m->G + off_W(l), WIDTH, N_PARAM,
+                                        K_in, WIDTH, n, nullptr, 0, nullptr, ksplit);
         if(l > 0) {
             // dHprev = dZ W^T : M = n, N = WIDTH (K_in), K = WIDTH
             gemm<float, bf16_t, EPI_STORE>(s, E, dH, WIDTH, 1, zH, m->Pbf + off_W(l), 1, WIDTH, N_PARAM, dHn, WIDTH,
@@ -431,8 +459,10 @@ extern "C" int bhc_mlp_create(int device, int members, int max_batch, unsigned l
     std::vector<float> P(np, 0.0f), Bf(N_FREQ * D_IN);
     std::normal_distribution<double> nd(0.0, 1.0);
     for(auto &b : Bf) b = (float)(sigma * nd(rng));
+    // hidden layers: He-uniform; the output layer starts at zero, so that a
+    // network trained on residuals starts from the reference it corrects
     for(int e = 0; e < members; ++e) {
-        for(int l = 0; l <= N_HID; ++l) {
+        for(int l = 0; l < N_HID; ++l) {
             int fan_in = l == 0 ? D_FF : WIDTH, fan_out = l == N_HID ? 1 : WIDTH;
             double a = std::sqrt(6.0 / fan_in);
             std::uniform_real_distribution<double> ud(-a, a);
@@ -457,6 +487,15 @@ extern "C" int bhc_mlp_get_params(bhc_mlp *m, float *params, float *fourier)
     MCHECK(hipStreamSynchronize(m->stream));
     if(params) MCHECK(hipMemcpy(params, m->P, (size_t)m->members * N_PARAM * 4, hipMemcpyDeviceToHost));
     if(fourier) MCHECK(hipMemcpy(fourier, m->Bf, N_FREQ * D_IN * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Replace the Fourier projection matrix B [32][16] (features sin / cos of
+// 2 pi B x), e.g. with per-input frequency scales.
+extern "C" int bhc_mlp_set_fourier(bhc_mlp *m, const float *B)
+{
+    MCHECK(hipSetDevice(m->device));
+    MCHECK(hipMemcpy(m->Bf, B, N_FREQ * D_IN * 4, hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -13,7 +13,8 @@ Environment family (14 parameters, each mapped from [0, 1] or N(0, 1)):
   profile over D = 4000 m; a Gaussian seamount (height 0-2500 m, centre
   10-40 km, width 2-10 km) and the bottom's compressional speed 1550-1800 m/s.
   Receivers: 51 depths x 201 ranges to 50 km; 1001 Gaussian beams over
-  +-30 degrees.
+  +-30 degrees, incoherent TL (the smooth field a coordinate network can
+  represent; coherent interference is left to the tracer).
 """
 import ctypes as C
 import math
@@ -25,7 +26,7 @@ from . import _abi, lib, BhcError, Problem, Sweep, Resident
 from .env import Env2D, Halfspace, Boundary, subtab, munk_c
 
 D_IN = 16
-TL_MIN, TL_MAX, TL_MID, TL_SCALE = 40.0, 140.0, 90.0, 25.0
+TL_MIN, TL_MAX, TL_SCALE = 40.0, 140.0, 25.0
 DEPTH = 4000.0
 R_MAX_KM = 50.0
 
@@ -42,6 +43,7 @@ def _mlp_lib():
             ("bhc_mlp_num_params", C.c_int, []),
             ("bhc_mlp_get_params", C.c_int, [C.c_void_p, fp, fp]),
             ("bhc_mlp_set_params", C.c_int, [C.c_void_p, fp]),
+            ("bhc_mlp_set_fourier", C.c_int, [C.c_void_p, fp]),
             ("bhc_mlp_predict", C.c_int, [C.c_void_p, fp, C.c_longlong, fp]),
             ("bhc_mlp_gradients", C.c_int, [C.c_void_p, fp, fp, C.c_int, fp, fp]),
             ("bhc_mlp_set_dataset", C.c_int, [C.c_void_p, fp, fp, C.c_longlong]),
@@ -63,7 +65,7 @@ def _fp(a):
 class Mlp:
     """Ensemble of `members` networks (bhc_mlp_*)."""
 
-    def __init__(self, members=4, max_batch=16384, seed=0, sigma=2.0, device=0):
+    def __init__(self, members=4, max_batch=16384, seed=0, sigma=1.0, device=0):
         L = _mlp_lib()
         h = C.c_void_p()
         if L.bhc_mlp_create(device, members, max_batch, seed, sigma, C.byref(h)) != 0:
@@ -80,6 +82,11 @@ class Mlp:
         Bf = np.zeros((32, 16), np.float32)
         self._check(_mlp_lib().bhc_mlp_get_params(self._h, _fp(P), _fp(Bf)))
         return P, Bf
+
+    def set_fourier(self, B):
+        B = np.ascontiguousarray(B, np.float32)
+        assert B.shape == (32, 16)
+        self._check(_mlp_lib().bhc_mlp_set_fourier(self._h, _fp(B)))
 
     def set_params(self, P):
         P = np.ascontiguousarray(P, np.float32)
@@ -161,8 +168,20 @@ def environment(theta, precision=64, nbeams=1001, nrz=51, nrr=201):
         bot=Halfspace("A", depth=DEPTH, alphaR=cp, betaR=0.0, rho=1.8, alphaI=0.5, zTemp=DEPTH),
         bty=Boundary("LS", r_km=rb, z=db), Sz=np.array([zs], np.float32),
         Rz=subtab(0.0, DEPTH, nrz, np.float32), Rr_km=subtab(R_MAX_KM / (nrr - 1), R_MAX_KM, nrr),
-        alpha_deg=subtab(-30.0, 30.0, nbeams), run_type="CB", deltas=0.0,
+        alpha_deg=subtab(-30.0, 30.0, nbeams), run_type="IB", deltas=0.0,
         box_r_km=R_MAX_KM + 1.0, box_z=DEPTH + 100.0)
+
+
+# Frequency scale of each input in the Fourier projection: range and depth
+# carry the field's structure, the environment parameters modulate it
+# smoothly (low frequencies there keep the network from keying on individual
+# training environments).
+INPUT_SCALES = np.array([2.0, 2.0, 0.5, 0.5] + [0.25] * 8 + [0.5] * 4)
+
+
+def fourier_matrix(seed):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(size=(32, 16)) * INPUT_SCALES[None, :]).astype(np.float32)
 
 
 def inputs(theta, rz, rr_km):
@@ -205,25 +224,31 @@ def trace_fields(thetas, ctx, chunk=256):
     return out, envs, t_sweep
 
 
-def dataset(thetas, fields, envs, points_per_field, seed):
+def reference_field(fields):
+    """Mean TL over the training fields, receiver by receiver: the network
+    learns each environment's deviation from it."""
+    return np.mean(np.stack(fields), axis=0)
+
+
+def dataset(thetas, fields, envs, points_per_field, seed, ref):
     rng = np.random.default_rng(seed)
     X, Y = [], []
     for th, tl, e in zip(thetas, fields, envs):
         x = inputs(th, e.Rz, e.Rr_km)
-        y = ((tl.reshape(-1) - TL_MID) / TL_SCALE).astype(np.float32)
+        y = ((tl - ref).reshape(-1) / TL_SCALE).astype(np.float32)
         k = rng.choice(x.shape[0], size=min(points_per_field, x.shape[0]), replace=False)
         X.append(x[k])
         Y.append(y[k])
     return np.concatenate(X), np.concatenate(Y)
 
 
-def predict_fields(mlp, thetas, envs):
+def predict_fields(mlp, thetas, envs, ref):
     """Ensemble predictions (dB) [members, n_env, nrz, nrr]."""
     xs = [inputs(th, e.Rz, e.Rr_km) for th, e in zip(thetas, envs)]
     shape = (len(envs[0].Rz), len(envs[0].Rr_km))
-    pred = mlp.predict(np.concatenate(xs))
-    pred = pred * TL_SCALE + TL_MID
-    return np.clip(pred, TL_MIN, TL_MAX).reshape(mlp.members, len(thetas), *shape)
+    pred = mlp.predict(np.concatenate(xs)).reshape(mlp.members, len(thetas), *shape)
+    pred = pred * TL_SCALE + ref[None, None]
+    return np.clip(pred, TL_MIN, TL_MAX)
 
 
 def screen(pred, spread_db=1.0):

@@ -244,14 +244,17 @@ def run_surrogate(a):
     t0 = time.perf_counter()
     f_tr, e_tr, t_tr = sg.trace_fields(th_tr, ctx)
     t_train_fields = time.perf_counter() - t0
-    X, Y = sg.dataset(th_tr, f_tr, e_tr, a.sur_points, 1)
+    ref = sg.reference_field(f_tr)
+    X, Y = sg.dataset(th_tr, f_tr, e_tr, a.sur_points, 1, ref)
     mlp = sg.Mlp(members=4, max_batch=a.sur_batch, seed=11)
+    mlp.set_fourier(sg.fourier_matrix(12))
     mlp.set_dataset(X, Y)
     t0 = time.perf_counter()
     done, train_ms = 0, 0.0
     while done < a.sur_steps:
         k = min(500, a.sur_steps - done)
-        loss = mlp.train(k, a.sur_batch, 1e-3, seed=done)
+        lr = a.sur_lr * (0.1 if done >= 0.9 * a.sur_steps else 1.0)
+        loss = mlp.train(k, a.sur_batch, lr, seed=done)
         train_ms += mlp.last_ms()
         done += k
     t_train = time.perf_counter() - t0
@@ -274,7 +277,7 @@ def run_surrogate(a):
     plain_fields_s = ns / t_plain
     # screened answer: network everywhere, tracer where the ensemble disagrees
     t0 = time.perf_counter()
-    pred = sg.predict_fields(mlp, th_ev, e_ev)
+    pred = sg.predict_fields(mlp, th_ev, e_ev, ref)
     mean, fallback, rms_spread = sg.screen(pred, 1.0)
     t_pred = time.perf_counter() - t0
     fb = [i for i in range(len(th_ev)) if fallback[i]]
@@ -288,8 +291,10 @@ def run_surrogate(a):
             final[i] = f_fb[k]
     err = (final - truth).reshape(-1)
     err_net = (mean - truth).reshape(-1)
-    base = np.stack(f_tr).mean(0)
-    err_base = (base[None] - truth).reshape(-1)
+    k_in = list(range(0, len(th_tr), max(1, len(th_tr) // 100)))[:100]
+    pred_in = sg.predict_fields(mlp, th_tr[k_in], [e_tr[i] for i in k_in], ref).mean(0)
+    err_in = (pred_in - np.stack([f_tr[i] for i in k_in])).reshape(-1)
+    err_base = (ref[None] - truth).reshape(-1)
     screened_fields_s = len(th_ev) / (t_pred + t_fb)
     peak = 2500.0
     return {
@@ -299,13 +304,15 @@ def run_surrogate(a):
         "vs_baseline": None, "dtype": "bf16 (network), f64 (tracer)",
         "data": "synthetic (seeded environment family, seeds 20260005 / 20260006)",
         "config": {"workload": "surrogate-screened sweep: %d training fields, %d held-out environments, "
-                               "51 x 201 receivers, 1001 'B' beams; 4 x MLP(64 Fourier features, 6 x 256 GELU)"
+                               "51 x 201 receivers, 1001 'B' beams, incoherent TL; 4 x MLP(64 Fourier features, 6 x 256 GELU)"
                                % (len(th_tr), len(th_ev)), "parallelism": "single GPU"},
         "plain_per_env": {"value": plain_fields_s, "unit": "TL fields/s"},
         "screened_over_plain": screened_fields_s / plain_fields_s,
         "fallback_fraction": float(np.mean(fallback)),
         "error_db": {"rms": float(np.sqrt(np.mean(err ** 2))), "p95": float(np.percentile(np.abs(err), 95)),
                      "network_only_rms": float(np.sqrt(np.mean(err_net ** 2))),
+                     "network_in_sample_rms": float(np.sqrt(np.mean(err_in ** 2))),
+                     "median_rms_spread": float(np.median(rms_spread)),
                      "mean_field_baseline_rms": float(np.sqrt(np.mean(err_base ** 2)))},
         "training": {"fields": len(th_tr), "points": int(X.shape[0]), "steps": a.sur_steps,
                      "batch": a.sur_batch, "final_loss": [float(v) for v in loss], "seconds": t_train,
@@ -327,8 +334,9 @@ def main():
     ap.add_argument("--sur-train", type=int, default=20000)
     ap.add_argument("--sur-eval", type=int, default=1000)
     ap.add_argument("--sur-points", type=int, default=1000)
-    ap.add_argument("--sur-steps", type=int, default=4000)
+    ap.add_argument("--sur-steps", type=int, default=20000)
     ap.add_argument("--sur-batch", type=int, default=16384)
+    ap.add_argument("--sur-lr", type=float, default=1e-3)
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--sweep-n", type=int, default=4096)
     ap.add_argument("--sweep-scale", type=float, default=1.0)

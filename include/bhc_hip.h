@@ -395,6 +395,7 @@ const char *bhc_mlp_last_error(void);
 int bhc_mlp_num_params(void);                                   /* per member */
 int bhc_mlp_get_params(bhc_mlp *m, float *params, float *fourier); /* [members][P], [32][16] */
 int bhc_mlp_set_params(bhc_mlp *m, const float *params);
+int bhc_mlp_set_fourier(bhc_mlp *m, const float *B);            /* [32][16] */
 int bhc_mlp_predict(bhc_mlp *m, const float *x, long long n, float *out); /* x [n][16] -> out [members][n] */
 int bhc_mlp_gradients(bhc_mlp *m, const float *x, const float *y, int n, float *grads, float *loss);
 int bhc_mlp_set_dataset(bhc_mlp *m, const float *X, const float *Y, long long n);

@@ -17,14 +17,22 @@ def _announce(name):
     sys.stdout.flush()
 
 
-def test_mlp_gradients_match_fp64_restatement():
-    _announce("mlp_gradients")
+@pytest.mark.parametrize("n", [96, 4096])
+def test_mlp_gradients_match_fp64_restatement(n):
+    """n = 4096 takes the split-K path of the weight-gradient products."""
+    _announce("mlp_gradients_%d" % n)
     from bellhopcuda_amd import surrogate
-    m = surrogate.Mlp(members=2, max_batch=256, seed=7)
+    m = surrogate.Mlp(members=2, max_batch=max(256, n), seed=7)
     rng = np.random.default_rng(1)
-    x = rng.random((96, 16), dtype=np.float32)
-    y = rng.normal(size=96).astype(np.float32)
+    x = rng.random((n, 16), dtype=np.float32)
+    y = rng.normal(size=n).astype(np.float32)
     P, Bf = m.params()
+    # the output layer starts at zero: give it values so that every layer's
+    # gradient is exercised
+    off, _ = mlp_ref.layout()
+    o = off["Wout"][0]
+    P[:, o:o + 257] = rng.normal(scale=0.1, size=(2, 257)).astype(np.float32)
+    m.set_params(P)
     g, loss = m.gradients(x, y)
     for e in range(2):
         out_r, loss_r, g_r = mlp_ref.forward_backward(P[e], Bf, x, y.astype(np.float64))
@@ -33,8 +41,12 @@ def test_mlp_gradients_match_fp64_restatement():
         err = np.max(np.abs(g[e] - g_r)) / scale
         print("member", e, "loss", loss[e], loss_r, "max rel grad err", err, flush=True)
         assert err < 2e-3
+        # a hidden activation on a BF16 rounding boundary can round the other
+        # way from FP32 pre-activations: most outputs agree to 1e-3, all to 1e-2
         pred = m.predict(x)
-        assert np.allclose(pred[e], out_r, rtol=1e-3, atol=1e-3 * np.max(np.abs(out_r)))
+        tol = np.max(np.abs(out_r))
+        close = np.isclose(pred[e], out_r, rtol=1e-3, atol=1e-3 * tol)
+        assert np.mean(close) >= 0.9 and np.max(np.abs(pred[e] - out_r)) <= 1e-2 * tol
 
 
 def test_mlp_training_reduces_loss():
