@@ -45,6 +45,14 @@ CASES = {
 }
 
 
+import cases  # noqa: E402
+
+for _n, _f in cases.FEATURES.items():
+    CASES["feature_" + _n] = _f
+for _n in ("quad_ssp", "altimetry", "reflection_table", "several_sources", "long_format_bathymetry"):
+    CASES["feature_%s_f32" % _n] = (lambda f: lambda: f(32))(cases.FEATURES[_n])
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_field_parity(name, gpu_ctx):
     _announce(name)

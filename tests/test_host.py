@@ -167,6 +167,13 @@ CORE_CASES = {
 }
 
 
+import cases  # noqa: E402
+
+for _n, _f in cases.FEATURES.items():
+    CORE_CASES[_n] = _f
+    CORE_CASES[_n + "_f32"] = (lambda f: lambda: f(32))(_f)
+
+
 @pytest.mark.parametrize("name", list(CORE_CASES))
 def test_core_matches_oracle_bitwise(name, harness):
     env = CORE_CASES[name]()

@@ -435,3 +435,36 @@ def test_eigen_hits_bracket_receivers():
         # hat beam half-width in isovelocity: q / q0 = (distance travelled) * dalpha
         width = np.linalg.norm(rec - np.array([0.0, zs])) * dalpha
         assert normal_dist <= width * 1.01
+
+
+@pytest.mark.parametrize("unit,level", [("N ", 2e-5), ("M ", 1e-4), ("F ", 0.5), ("W ", 0.2), ("Q ", 300.0),
+                                        ("L ", 1e-5), ("WT", 0.0)])
+def test_volume_attenuation_closed_form(unit, level):
+    """crci (src/module/atten.cpp:99-194): in free space the attenuation adds
+    8.686 alpha_T s dB along the path s, alpha_T in nepers/m from the unit's
+    definition: N nepers/m, M dB/m, F dB/(m kHz), W dB/wavelength, Q quality
+    factor (omega / (2 c Q)), L loss parameter (alpha omega / c), and with a
+    'T' second letter Thorp's formula in dB/km,
+    3.3e-3 + 0.11 f^2/(1+f^2) + 44 f^2/(4100+f^2) + 3e-4 f^2 (f in kHz)."""
+    D, zs, f, c = 20000.0, 10000.0, 200.0, 1500.0
+    rz = subtab(9500.0, 10500.0, 11, np.float32)
+    rr = subtab(0.5, 3.0, 11)       # before the steepest surface-reflected beams arrive (3.3 km)
+
+    def tl(u, lev):
+        env = _field_env(D, zs, rz, rr, 2001, (-80.0, 80.0), "CG", freq=f)
+        env.atten_unit = u
+        env.ssp_alphaI = np.full(2, lev)
+        o = oracle.field_run(env, nthreads=0, stats=False)
+        return -20 * np.log10(np.abs(oracle.scale_pressure(env, o.field)[0]))
+
+    w = 2 * math.pi * f
+    alpha_np = {"N": level, "M": level / 8.6858896, "F": level * f / 8685.8896, "W": level * f / (8.6858896 * c),
+                "Q": w / (2.0 * c * level) if level else 0.0, "L": level * w / c}[unit[0]]
+    if unit[1] == "T":
+        fk2 = (f / 1000.0) ** 2
+        alpha_np += (3.3e-3 + 0.11 * fk2 / (1 + fk2) + 44.0 * fk2 / (4100.0 + fk2) + 3e-4 * fk2) / 8685.8896
+    R = np.hypot(rr[None, :] * 1000.0, rz[:, None].astype(np.float64) - zs)
+    excess = tl(unit, level) - tl("N ", 0.0)
+    expect = 20.0 * np.log10(math.e) * alpha_np * R
+    assert np.max(np.abs(excess - expect)) < 1e-3 + 1e-3 * np.max(expect)
+    assert np.max(expect) > 0.02
