@@ -1169,13 +1169,11 @@ template<typename real> __host__ __device__ inline int RToIR(real r, const Prob2
     return imax(imin(trunc_int_x86(temp), P.NRr - 1), 0);
 }
 
-// Cartesian work item k -> (ir, iz): influence.hpp:1298-1386 (2D).
+// Cartesian receiver (ir, iz) of a segment: influence.hpp:1298-1386 (2D).
 template<typename real, int RT, typename Sink, bool FAST = false>
-__host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
-                                          const float *Rz, Sink &sink)
+__host__ __device__ inline void item_cart_at(const SegDesc<real> &d, int ir, int iz, const Prob2D<real> &P,
+                                             const real *G, const float *Rz, Sink &sink)
 {
-    int j = k / d.nz, iz = d.iz0 + (k - j * d.nz);
-    int ir = d.ir0 + j * d.dir;
     real rr = G[P.o_Rr + ir];
     if(!(rr >= d.lo && rr < d.hi)) return;
     int tiz = (P.run4 == 'I') ? ir : iz;
@@ -1185,6 +1183,52 @@ __host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const P
     real s = dot(xrr, d.ta) / d.rlen;
     real n1 = std::fabs(dot(xrr, d.tb));
     contrib<real, RT, Sink, FAST>(d, s, n1, ir, iz, P, sink);
+}
+
+// Cartesian work item k -> (ir, iz) of the segment's bounding box.
+template<typename real, int RT, typename Sink, bool FAST = false>
+__host__ __device__ inline void item_cart(const SegDesc<real> &d, int k, const Prob2D<real> &P, const real *G,
+                                          const float *Rz, Sink &sink)
+{
+    int j = k / d.nz, iz = d.iz0 + (k - j * d.nz);
+    item_cart_at<real, RT, Sink, FAST>(d, d.ir0 + j * d.dir, iz, P, G, Rz, sink);
+}
+
+// Cartesian column j (receiver range ir = ir0 + j dir) of a segment: the depth
+// indices [i0, i1] that can pass the beam-window test. At range r the segment
+// line is at depth z_l, the normal distance of depth z is n = |t_r| |z - z_l|
+// (unit tangent t) and the receiver's projection is s = s_l + (z - z_l) t_z / L
+// (L the segment length). The beam width obeys
+// sigma <= |1/q0| (|q0| + |dq| |s|) (+ the Gaussian floor), so n < BW sigma
+// requires |z - z_l| (|t_r| - BW |1/q0| |dq| |t_z| / L) < BW (|1/q0| (|q0| +
+// |dq| |s_l|) + floor). Depths outside that band, widened for rounding, are
+// skipped; the others go through every test of item_cart_at. Irregular grids,
+// unsorted depths and steep segments (non-positive coefficient) keep the
+// whole window.
+template<typename real>
+__host__ __device__ inline void cart_column(const SegDesc<real> &d, int j, const Prob2D<real> &P, const real *G,
+                                            int &ir, int &i0, int &i1)
+{
+    ir = d.ir0 + j * d.dir;
+    i0 = d.iz0;
+    i1 = d.iz0 + d.nz - 1;
+    if(!P.rz_sorted || P.run4 == 'I' || d.nz <= 2) return;
+    const double tx = (double)d.ta.x, tz = (double)d.ta.y, atx = std::fabs(tx);
+    const double BW = P.isGaussian ? 4.0 : 1.0;
+    const double rcp = std::fabs((double)d.rcp_q0), dq = std::fabs((double)d.dqx), L = (double)d.rlen;
+    const double coef = atx - BW * rcp * dq * std::fabs(tz) / L;
+    if(!(atx > 1e-9) || !(coef > 1e-3 * atx)) return;
+    const double rr = (double)G[P.o_Rr + ir], ra = (double)d.xa.x, za = (double)d.xa.y;
+    const double zl = za + (rr - ra) * tz / tx;
+    const double sl = ((rr - ra) * tx + (zl - za) * tz) / L;
+    const double rhs =
+        BW * (rcp * (std::fabs((double)d.q0x) + dq * std::fabs(sl)) + (P.isGaussian ? std::fmax((double)d.sigfloor, 0.0) : 0.0));
+    const double dz = rhs / coef * (1.0 + 1e-4) + 1e-4 * (std::fabs(zl) + std::fabs(rr) + std::fabs(za)) + 1e-2;
+    if(!(dz == dz) || !(zl == zl) || !(dz < 1e30)) return;
+    int a, b;
+    depth_window(P.Rz, P.NRz_per_range, zl - dz, zl + dz, a, b);
+    i0 = imax(i0, a);
+    i1 = imin(i1, b);
 }
 
 // Ray-centred receiver depth iz (influence.hpp:1075-1187, 2D): the normals at
@@ -1286,6 +1330,25 @@ __host__ __device__ inline void item(const SegDesc<real> &d, int k, const Prob2D
 {
     if(P.isRayCen) item_raycen<real, RT, Sink, FAST>(d, d.iz0 + k, P, G, Rz, sink);
     else item_cart<real, RT, Sink, FAST>(d, k, P, G, Rz, sink);
+}
+
+// Every receiver of one segment, in one thread: the Cartesian columns with
+// their depth bands, or the ray-centred depths with their range windows.
+template<typename real, int RT, typename Sink, bool FAST = false>
+__host__ __device__ inline void seg_items(const SegDesc<real> &d, const Prob2D<real> &P, const real *G,
+                                          const float *Rz, Sink &sink)
+{
+    if(d.nitems <= 0) return;
+    if(P.isRayCen) {
+        for(int k = 0; k < d.nz; ++k) item_raycen<real, RT, Sink, FAST>(d, d.iz0 + k, P, G, Rz, sink);
+        return;
+    }
+    const int nr = (d.ir1 - d.ir0) * d.dir + 1;
+    for(int j = 0; j < nr; ++j) {
+        int ir, i0, i1;
+        cart_column(d, j, P, G, ir, i0, i1);
+        for(int iz = i0; iz <= i1; ++iz) item_cart_at<real, RT, Sink, FAST>(d, ir, iz, P, G, Rz, sink);
+    }
 }
 
 // Job index -> (isz, ialpha), reference src/common.hpp:365-391 (2D).

@@ -159,7 +159,7 @@ extern "C" int bhc_opcount2d(const bhc_problem2d *prob, int nsample, double *out
                     ops_step += opc::n;
                     ++is;
                 }
-                for(int it = 0; it < d.nitems; ++it) item<CR, RT_TL>(d, it, P, G, prob->Rz, sink);
+                seg_items<CR, RT_TL>(d, P, G, prob->Rz, sink);
             }
             p0 = two ? p2 : p1;
             if(RayTerminate(p0, is, S, 100000, P, e)) break;

@@ -212,17 +212,54 @@ __device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut
     B.used += cnt;
 }
 
+// Exclusive prefix sum of cnt over the wavefront; returns the total.
+__device__ inline int wave_excl_scan(int cnt, int lane, int &excl)
+{
+    int incl = cnt;
+    for(int off = 1; off < WAVE; off <<= 1) {
+        int v = __shfl_up(incl, off);
+        if(lane >= off) incl += v;
+    }
+    excl = incl - cnt;
+    return __shfl(incl, WAVE - 1);
+}
+
+// Lane whose [excl, excl + cnt) holds flattened index t (excl non-decreasing).
+__device__ inline int wave_owner(int excl, int t)
+{
+    int own = 0;
+    for(int b = WAVE / 2; b >= 1; b >>= 1) {
+        int probe = own + b < WAVE ? own + b : WAVE - 1;
+        int e = __shfl(excl, probe);
+        if(own + b < WAVE && e <= t) own += b;
+    }
+    return own;
+}
+
 // Scatter one segment's receivers with the 64 lanes of a wavefront (d is
-// wavefront-uniform). Cartesian items are (range, depth) pairs already.
-// Ray-centred items are depths with different numbers of candidate ranges, so
-// the (depth, range) pairs of up to 64 depths are flattened over the lanes:
-// a prefix sum of the per-depth counts, then each lane finds the depth of its
-// pair by a binary search over the lanes' prefix values.
+// wavefront-uniform). Both geometries have rows of different lengths --
+// Cartesian range columns with their depth bands, ray-centred depths with
+// their range windows -- so the items of up to 64 rows are flattened over the
+// lanes: a prefix sum of the row lengths, then each lane finds the row of its
+// item by a binary search over the lanes' prefix values.
 template<typename real, int RT, bool FAST, typename Sink>
 __device__ inline void scatter_seg(const SegDesc<real> &d, const Prob2D<real> &P, Sink &sink, int lane)
 {
     if(!P.isRayCen) {
-        for(int it = lane; it < d.nitems; it += WAVE) item_cart<real, RT, Sink, FAST>(d, it, P, P.base, P.Rz, sink);
+        const int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        for(int g = 0; g < nr; g += WAVE) {
+            int ir = 0, i0 = 0, i1 = -1;
+            if(g + lane < nr) cart_column(d, g + lane, P, P.base, ir, i0, i1);
+            const int cnt = i1 >= i0 ? i1 - i0 + 1 : 0;
+            int excl;
+            const int total = wave_excl_scan(cnt, lane, excl);
+            for(int t0 = 0; t0 < total; t0 += WAVE) {
+                const int t = t0 + lane;
+                const int own = wave_owner(excl, t);
+                const int oir = __shfl(ir, own), oi0 = __shfl(i0, own), oexcl = __shfl(excl, own);
+                if(t < total) item_cart_at<real, RT, Sink, FAST>(d, oir, oi0 + (t - oexcl), P, P.base, P.Rz, sink);
+            }
+        }
         return;
     }
     const real *Rr = P.base + P.o_Rr;
@@ -232,21 +269,11 @@ __device__ inline void scatter_seg(const SegDesc<real> &d, const Prob2D<real> &P
         int blo = 0, bhi = -1;
         if(g + lane < d.nz && !raycen_range(d, iz, P, Rr, P.Rz, nA, rA, nB, rB, blo, bhi)) bhi = blo - 1;
         const int cnt = bhi - blo + 1;
-        int incl = cnt;
-        for(int off = 1; off < WAVE; off <<= 1) {
-            int v = __shfl_up(incl, off);
-            if(lane >= off) incl += v;
-        }
-        const int total = __shfl(incl, WAVE - 1);
-        const int excl = incl - cnt;
+        int excl;
+        const int total = wave_excl_scan(cnt, lane, excl);
         for(int t0 = 0; t0 < total; t0 += WAVE) {
             const int t = t0 + lane;
-            int own = 0;
-            for(int b = WAVE / 2; b >= 1; b >>= 1) {
-                int probe = own + b < WAVE ? own + b : WAVE - 1;
-                int e = __shfl(excl, probe);
-                if(own + b < WAVE && e <= t) own += b;
-            }
+            const int own = wave_owner(excl, t);
             const real onA = __shfl(nA, own), orA = __shfl(rA, own), onB = __shfl(nB, own), orB = __shfl(rB, own);
             const int oblo = __shfl(blo, own), oexcl = __shfl(excl, own), oiz = __shfl(iz, own);
             if(t < total) raycen_one<real, RT, Sink, FAST>(d, oblo + (t - oexcl), oiz, onA, orA, onB, orB, P, Rr, sink);
@@ -403,7 +430,7 @@ __global__ void __launch_bounds__(256) k_scatter2d_lanes(Prob2D<real> P, DevOut 
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
         const SegDesc<real> d = segs[s].d;
-        for(int it = 0; it < d.nitems; ++it) item<real, RT, DevSink<real>, FAST>(d, it, P, P.base, P.Rz, sink);
+        seg_items<real, RT, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
     if((threadIdx.x & (WAVE - 1)) == 0 && sink.nc) atomicAdd(O.counters + 1, sink.nc);
@@ -1751,7 +1778,7 @@ __global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut
         if(d.nitems <= 0) continue;
         const Prob2D<real> &P = Bd.probs[d.prob];
         DevSink<real> sink{&O, &P, 0};
-        for(int it = 0; it < d.nitems; ++it) item<real, RT_TL, DevSink<real>, FAST>(d, it, P, P.base, P.Rz, sink);
+        seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
         nc += sink.nc;
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);

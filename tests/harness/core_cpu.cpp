@@ -1,6 +1,6 @@
 // Test harness only: runs the product's per-ray core (core2d.h + pack2d.h) on
 // the CPU, one ray after another, with the wavefront scatter replaced by a
-// plain loop over each segment's work items. It exercises exactly the
+// plain loop over each segment's receivers. It exercises exactly the
 // functions the HIP kernel calls, so a disagreement with the oracle found here
 // is a bug in the shared core, not in the kernel's launch plumbing.
 #include <cstdio>
@@ -69,11 +69,11 @@ template<typename real, bool FAST> int run(const bhc_problem2d *prob, bhc_field_
             bool two = RayUpdate(p0, p1, p2, S, P, B, G, e);
             SegDesc<real> d;
             prelude(d, p0, p1, I, is, P, G, isz, ia);
-            for(int k = 0; k < d.nitems; ++k) item<real, RT_TL, HostSink, FAST>(d, k, P, G, prob->Rz, sink);
+            seg_items<real, RT_TL, HostSink, FAST>(d, P, G, prob->Rz, sink);
             ++is;
             if(two) {
                 prelude(d, p1, p2, I, is, P, G, isz, ia);
-                for(int k = 0; k < d.nitems; ++k) item<real, RT_TL, HostSink, FAST>(d, k, P, G, prob->Rz, sink);
+                seg_items<real, RT_TL, HostSink, FAST>(d, P, G, prob->Rz, sink);
                 ++is;
             }
             p0 = two ? p2 : p1;
