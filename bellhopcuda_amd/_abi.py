@@ -134,3 +134,64 @@ class RayOut(C.Structure):
                 ("max_points_per_ray", C.c_int32), ("pad", C.c_int32),
                 ("err", C.c_uint32), ("warn", C.c_uint32), ("err_count", C.c_uint32),
                 ("warn_count", C.c_uint32)]
+
+
+class Bdry3DIn(C.Structure):
+    _fields_ = [("type", C.c_char * 2), ("pad", C.c_char * 6), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("x_km", c_double_p), ("y_km", c_double_p), ("z", c_double_p)]
+
+
+class Env3D_C(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("pad0", C.c_int32), ("freq0", C.c_double),
+        ("ssp_type", C.c_char), ("atten_unit", C.c_char * 2), ("pad1", C.c_char * 5),
+        ("ssp_npts", C.c_int32), ("pad2", C.c_int32),
+        ("ssp_z", c_double_p), ("ssp_alphaR", c_double_p), ("ssp_alphaI", c_double_p),
+        ("ssp_rho", c_double_p), ("ssp_betaR", c_double_p), ("ssp_betaI", c_double_p),
+        ("hex_nx", C.c_int32), ("hex_ny", C.c_int32), ("hex_nz", C.c_int32), ("pad3", C.c_int32),
+        ("hex_x_km", c_double_p), ("hex_y_km", c_double_p), ("hex_z", c_double_p), ("hex_c", c_double_p),
+        ("fg_T", C.c_double), ("fg_S", C.c_double), ("fg_pH", C.c_double), ("fg_z_bar", C.c_double),
+        ("top", HalfspaceIn), ("bot", HalfspaceIn),
+        ("ati", Bdry3DIn), ("bty", Bdry3DIn), ("trc", ReflIn), ("brc", ReflIn),
+        ("NSx", C.c_int32), ("NSy", C.c_int32), ("NSz", C.c_int32), ("NRz", C.c_int32),
+        ("NRr", C.c_int32), ("Ntheta", C.c_int32),
+        ("Sx_km", c_double_p), ("Sy_km", c_double_p), ("Sz", c_float_p), ("Rz", c_float_p),
+        ("Rr_km", c_double_p), ("theta_deg", c_double_p),
+        ("Nalpha", C.c_int32), ("iSingle_alpha", C.c_int32), ("Nbeta", C.c_int32),
+        ("iSingle_beta", C.c_int32), ("alpha_deg", c_double_p), ("beta_deg", c_double_p),
+        ("RunType", C.c_char * 7), ("BeamType", C.c_char * 3), ("pad4", C.c_char * 6),
+        ("deltas", C.c_double), ("box_x_km", C.c_double), ("box_y_km", C.c_double), ("box_z", C.c_double),
+        ("NSBPPts", C.c_int32), ("pad5", C.c_int32), ("sbp", c_double_p),
+    ]
+
+
+class BdryNode3D(C.Structure):
+    _fields_ = [("x", C.c_double * 3), ("n1", C.c_double * 3), ("n2", C.c_double * 3)]
+
+
+class Problem3D(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("RunType", C.c_char * 7), ("BeamType", C.c_char * 4),
+        ("ssp_type", C.c_char), ("pad0", C.c_int32), ("freq0", C.c_double),
+        ("ssp_npts", C.c_int32), ("pad1", C.c_int32),
+        ("ssp_z", c_double_p), ("ssp_c", c_double_p), ("ssp_cz", c_double_p), ("ssp_n2", c_double_p),
+        ("ssp_n2z", c_double_p), ("ssp_coef", c_double_p), ("ssp_rho", c_double_p),
+        ("hex_nx", C.c_int32), ("hex_ny", C.c_int32), ("hex_nz", C.c_int32), ("pad2", C.c_int32),
+        ("hex_x", c_double_p), ("hex_y", c_double_p), ("hex_z", c_double_p), ("hex_c", c_double_p),
+        ("hex_cz", c_double_p),
+        ("top_hs", HS2D), ("bot_hs", HS2D),
+        ("ati_nx", C.c_int32), ("ati_ny", C.c_int32), ("bty_nx", C.c_int32), ("bty_ny", C.c_int32),
+        ("ati", C.POINTER(BdryNode3D)), ("bty", C.POINTER(BdryNode3D)),
+        ("trc_npts", C.c_int32), ("brc_npts", C.c_int32), ("trc", c_double_p), ("brc", c_double_p),
+        ("NSx", C.c_int32), ("NSy", C.c_int32), ("NSz", C.c_int32), ("NRz", C.c_int32),
+        ("NRr", C.c_int32), ("Ntheta", C.c_int32),
+        ("Sx", c_double_p), ("Sy", c_double_p), ("Sz", c_float_p), ("Rz", c_float_p),
+        ("Rr", c_double_p), ("t_rcvr", c_double_p), ("Delta_r", C.c_double),
+        ("Nalpha", C.c_int32), ("iSingle_alpha", C.c_int32), ("Nbeta", C.c_int32),
+        ("iSingle_beta", C.c_int32),
+        ("alpha", c_double_p), ("alpha_cos", c_double_p), ("alpha_sin", c_double_p),
+        ("beta", c_double_p), ("beta_cos", c_double_p), ("beta_sin", c_double_p),
+        ("Dalpha", C.c_double), ("Dbeta", C.c_double),
+        ("deltas", C.c_double), ("box_x", C.c_double), ("box_y", C.c_double), ("box_z", C.c_double),
+        ("NSBPPts", C.c_int32), ("pad3", C.c_int32), ("sbp", c_double_p),
+    ]

@@ -63,6 +63,8 @@ extern "C" {
 #define BHC_WARN_OUTSIDE_ALTIMETRY 6
 #define BHC_WARN_OUTSIDE_BATHYMETRY 7
 #define BHC_WARN_STEP_NEGATIVE_H 8
+#define BHC_WARN_TRIDIAG_H_NEGATIVE 9
+#define BHC_WARN_TRIDIAG_H_GROWING 10
 #define BHC_WARN_CPCHIP_INVALIDXT 16
 #define BHC_WARN_CPCHIP_INVALIDCCOEF 17
 
@@ -351,6 +353,118 @@ int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *ld
  * captured HIP graph; 0: enqueued kernel by kernel with per-kernel timing. */
 int bhc_sweep_set_graph(bhc_sweep *sw, int use_graph);
 void bhc_sweep_release(bhc_sweep *sw);
+
+/* ---- 3D (BELLHOP3D geometry: x, y, depth) ------------------------------------------
+ * Full 3D ray tracing (R3D = O3D = true): 3D rays with 2x2 p / q matrices and
+ * torsion, triangulated bathymetry / altimetry grids, depth-only or hexahedral
+ * sound speed, geometric hat / Gaussian beams in Cartesian coordinates, TL. */
+
+/* Altimetry / bathymetry grid (BdryInfoTopBot<true>, src/module/boundary.hpp:107-150):
+ * nx x ny nodes, each rectangle split into two triangles along its diagonal.
+ * nx == 0 selects the flat default boundary at the halfspace depth
+ * (src/module/boundary.hpp:56-78). Only type "RS" (rectilinear, one set of
+ * geoacoustics) is accepted. */
+typedef struct bhc_bdry3d_in {
+    char type[2];
+    char pad[6];
+    int32_t nx, ny;
+    const double *x_km;     /* [nx] */
+    const double *y_km;     /* [ny] */
+    const double *z;        /* [nx][ny] (m), index ix * ny + iy */
+} bhc_bdry3d_in;
+
+typedef struct bhc_env3d {
+    int32_t precision;      /* 64 or 32 */
+    int32_t pad0;
+    double freq0;
+    /* sound speed: 'C', 'N', 'S', 'P' depth-only profiles, or 'H' hexahedral grid */
+    char ssp_type;
+    char atten_unit[2];
+    char pad1[5];
+    int32_t ssp_npts, pad2;
+    const double *ssp_z, *ssp_alphaR, *ssp_alphaI, *ssp_rho, *ssp_betaR, *ssp_betaI;
+    /* 'H' only: the .ssp grid (src/module/ssp.hpp:144-186), x / y in km, c index (ix * ny + iy) * nz + iz */
+    int32_t hex_nx, hex_ny, hex_nz, pad3;
+    const double *hex_x_km, *hex_y_km, *hex_z, *hex_c;
+    double fg_T, fg_S, fg_pH, fg_z_bar;
+    bhc_halfspace_in top, bot;
+    bhc_bdry3d_in ati, bty;
+    bhc_refl_in trc, brc;
+    /* sources and receivers (Position): Sx, Sy, Rr in km; receiver bearings in
+     * degrees with the last of a full 360-degree sweep already removed */
+    int32_t NSx, NSy, NSz, NRz, NRr, Ntheta;
+    const double *Sx_km, *Sy_km;
+    const float *Sz, *Rz;
+    const double *Rr_km;
+    const double *theta_deg;
+    /* launch declinations alpha and bearings beta (degrees, sorted) */
+    int32_t Nalpha, iSingle_alpha, Nbeta, iSingle_beta;
+    const double *alpha_deg, *beta_deg;
+    char RunType[7];        /* [0] C / S / I; [1] G / ^ / B (Cartesian beams); [5] '3' */
+    char BeamType[3];
+    char pad4[6];
+    double deltas;          /* m, 0 = automatic */
+    double box_x_km, box_y_km, box_z;
+    int32_t NSBPPts, pad5;
+    const double *sbp;
+} bhc_env3d;
+
+/* Boundary grid node: position and the unit normals of the two triangles of
+ * the cell whose lower corner it is (BdryPtFull<true>::x, n1, n2). */
+typedef struct bhc_bdry_node3d {
+    double x[3], n1[3], n2[3];
+} bhc_bdry_node3d;
+
+typedef struct bhc_problem3d {
+    int32_t precision;
+    char RunType[7];
+    char BeamType[4];
+    char ssp_type;
+    int32_t pad0;
+    double freq0;
+    /* depth-only profile (the 2D layout of bhc_problem2d) */
+    int32_t ssp_npts, pad1;
+    const double *ssp_z, *ssp_c, *ssp_cz, *ssp_n2, *ssp_n2z, *ssp_coef, *ssp_rho;
+    /* hexahedral grid, meters; c and cz index (ix * ny + iy) * nz + iz and (ix * ny + iy) * (nz - 1) + iz */
+    int32_t hex_nx, hex_ny, hex_nz, pad2;
+    const double *hex_x, *hex_y, *hex_z, *hex_c, *hex_cz;
+    bhc_hs2d top_hs, bot_hs;
+    int32_t ati_nx, ati_ny, bty_nx, bty_ny;
+    const bhc_bdry_node3d *ati, *bty;
+    int32_t trc_npts, brc_npts;
+    const double *trc, *brc;
+    int32_t NSx, NSy, NSz, NRz, NRr, Ntheta;
+    const double *Sx, *Sy;      /* m */
+    const float *Sz, *Rz;
+    const double *Rr;           /* m */
+    const double *t_rcvr;       /* [Ntheta][2] cos, sin of the bearings (host libm) */
+    double Delta_r;
+    int32_t Nalpha, iSingle_alpha, Nbeta, iSingle_beta;
+    const double *alpha, *alpha_cos, *alpha_sin;   /* rad; cos / sin from host libm */
+    const double *beta, *beta_cos, *beta_sin;
+    double Dalpha, Dbeta;
+    double deltas, box_x, box_y, box_z;
+    int32_t NSBPPts, pad3;
+    const double *sbp;
+} bhc_problem3d;
+
+typedef struct bhc_problem3d_owner bhc_problem3d_owner;
+/* 3D preprocessing: the Preprocess() steps of src/module/{ssp.hpp:453-489,
+ * boundary.hpp:499-527 + 600-705, beaminfo.hpp:221-238, rayangles.hpp:157-190,
+ * rcvrbearings.hpp:98-110, sxsy.hpp:105-112, rcvrranges.hpp:74-85}. */
+int bhc_preprocess3d(const bhc_env3d *env, bhc_problem3d_owner **out);
+const bhc_problem3d *bhc_problem3d_view(const bhc_problem3d_owner *p);
+void bhc_problem3d_free(bhc_problem3d_owner *p);
+
+/* 3D field run, RunFieldModesImpl<CFG, true, true> (src/mode/fieldimpl.hpp:27-28):
+ * out->field is [NSz][NSx][NSy][Ntheta][NRz][NRr] complex floats, caller-zeroed
+ * (GetFieldAddr, src/common.hpp:393-405); per-ray statistics as in 2D, with
+ * ray_xend holding [rays][3] end points. Rays are numbered like GetJobIndices
+ * (src/common.hpp:365-391): alpha fastest, then beta, Sy, Sx, Sz. */
+int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out);
+/* ScalePressure for 3D geometric beams (src/influence.hpp:1556-1627): incoherent
+ * and semicoherent fields become sqrt(intensity); coherent fields are unchanged. */
+int bhc_scale_pressure3d(const bhc_problem3d *prob, float *field);
 
 /* ---- post-processing and file layouts (host) ----------------------------------- */
 

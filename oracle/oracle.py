@@ -137,3 +137,71 @@ def preprocess_dump(env, cap=1 << 22):
     getattr(lib(), "oracle_preprocess_dump2d_" + _sfx(env))(C.byref(ec), buf.ctypes.data_as(_abi.c_double_p),
                                                            cap, C.byref(used))
     return buf[:used.value].copy()
+
+
+def _lib3():
+    L = lib()
+    if not getattr(L, "_bind3", False):
+        from bellhopcuda_amd import _abi as A
+        for sfx in ("d", "f"):
+            f = getattr(L, "oracle_field_run3d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(A.Env3D_C), C.POINTER(A.FieldOut), C.c_int, C.c_int, C.c_int]
+            f = getattr(L, "oracle_preprocess_dump3d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(A.Env3D_C), A.c_double_p, C.c_int64, C.POINTER(C.c_int64)]
+        L._bind3 = True
+    return L
+
+
+def field_run3d(env, nthreads=0, job_begin=0, job_end=-1):
+    """Run the 3D oracle on an Env3D. Returns a FieldResult with .field
+    (complex64, shape env.field_shape, pre-ScalePressure) and per-ray stats
+    (.nsteps, .ntop, .nbot, .xend [rays, 3])."""
+    L = _lib3()
+    ec, keep = env.to_c()
+    nj = env.nrays
+    if job_end < 0 or job_end > nj:
+        job_end = nj
+    n = job_end - job_begin
+    fld = np.zeros(int(np.prod(env.field_shape)) * 2, np.float32)
+    ns = np.zeros(max(n, 1), np.int32)
+    nt = np.zeros(max(n, 1), np.int32)
+    nb = np.zeros(max(n, 1), np.int32)
+    xe = np.zeros((max(n, 1), 3), np.float64)
+    out = _abi.FieldOut()
+    out.field = fld.ctypes.data_as(_abi.c_float_p)
+    out.ray_nsteps = ns.ctypes.data_as(_abi.c_int32_p)
+    out.ray_ntop = nt.ctypes.data_as(_abi.c_int32_p)
+    out.ray_nbot = nb.ctypes.data_as(_abi.c_int32_p)
+    out.ray_xend = xe.ctypes.data_as(_abi.c_double_p)
+    f = getattr(L, "oracle_field_run3d_" + _sfx(env))
+    rc = f(C.byref(ec), C.byref(out), nthreads, job_begin, job_end)
+    if rc != 0:
+        raise RuntimeError("oracle_field_run3d failed (unsupported 3D environment)")
+    r = FieldResult()
+    r.field = fld.view(np.complex64).reshape(env.field_shape)
+    r.nsteps, r.ntop, r.nbot, r.xend = ns[:n], nt[:n], nb[:n], xe[:n]
+    r.total_steps, r.total_contrib = out.total_steps, out.total_contrib
+    r.err, r.warn = out.err, out.warn
+    return r
+
+
+def scale_pressure3d(env, field):
+    """ScalePressure for 3D geometric beams (src/influence.hpp:1556-1627):
+    sqrt of the intensity for incoherent / semicoherent runs."""
+    f = np.array(field, np.complex64, copy=True)
+    if env.run_type[0] != "C":
+        f = np.sqrt(f.real.astype(np.float32)).astype(np.complex64)
+    return f
+
+
+def preprocess_dump3d(env, cap=1 << 22):
+    L = _lib3()
+    ec, keep = env.to_c()
+    dst = np.zeros(cap, np.float64)
+    used = C.c_int64()
+    f = getattr(L, "oracle_preprocess_dump3d_" + _sfx(env))
+    if f(C.byref(ec), dst.ctypes.data_as(_abi.c_double_p), cap, C.byref(used)) != 0:
+        raise RuntimeError("oracle_preprocess_dump3d failed")
+    return dst[:min(used.value, cap)]
