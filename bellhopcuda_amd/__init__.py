@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _abi
 from .env import Env2D, munk_env, c2_env, sweep_envs, subtab  # noqa: F401
+from .env3d import Env3D, c3_env, free_env3d, munk_env3d  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbhc_hip.so")
@@ -84,6 +85,12 @@ def lib():
         "bhc_ld_str": (None, [C.c_void_p, C.c_char_p]),
         "bhc_ld_char": (None, [C.c_void_p, C.c_char]),
         "bhc_ld_close": (C.c_int, [C.c_void_p]),
+        "bhc_preprocess3d": (C.c_int, [P(_abi.Env3D_C), P(C.c_void_p)]),
+        "bhc_problem3d_view": (P(_abi.Problem3D), [C.c_void_p]),
+        "bhc_problem3d_free": (None, [C.c_void_p]),
+        "bhc_host3d_last_error": (C.c_char_p, []),
+        "bhc_field_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut)]),
+        "bhc_scale_pressure3d": (C.c_int, [P(_abi.Problem3D), _abi.c_float_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -336,3 +343,72 @@ class Sweep:
         if getattr(self, "_h", None) and _LIB is not None:
             _LIB.bhc_sweep_release(self._h)
             self._h = None
+
+
+class Problem3D:
+    """A preprocessed 3D problem (bhc_problem3d view owned by the C++ side)."""
+
+    def __init__(self, env):
+        self.env = env
+        ec, self._keep = env.to_c()
+        h = C.c_void_p()
+        if lib().bhc_preprocess3d(C.byref(ec), C.byref(h)) != 0:
+            raise BhcError((lib().bhc_host3d_last_error() or b"?").decode())
+        self._h = h
+        self.view = lib().bhc_problem3d_view(h).contents
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.bhc_problem3d_free(self._h)
+            self._h = None
+
+    @property
+    def nrays(self):
+        v = self.view
+        na = 1 if v.iSingle_alpha >= 1 else v.Nalpha
+        nb = 1 if v.iSingle_beta >= 1 else v.Nbeta
+        return na * nb * v.NSx * v.NSy * v.NSz
+
+    @property
+    def field_shape(self):
+        v = self.view
+        return (v.NSz, v.NSx, v.NSy, v.Ntheta, v.NRz, v.NRr)
+
+    def scale_pressure(self, field):
+        f = np.ascontiguousarray(field, np.complex64).copy()
+        if lib().bhc_scale_pressure3d(C.byref(self.view), f.ctypes.data_as(_abi.c_float_p)) != 0:
+            raise BhcError((lib().bhc_host3d_last_error() or b"?").decode())
+        return f
+
+
+def field_run3d(env_or_prob, ctx=None, stats=False):
+    """3D TL field run on the GPU (RunFieldModesImpl<CFG, true, true>). Returns
+    a FieldResult: .field (complex64, [NSz, NSx, NSy, Ntheta, NRz, NRr],
+    pre-ScalePressure), .total_steps, .total_contrib, .err, .warn and, with
+    stats=True, per-ray .nsteps / .ntop / .nbot / .xend [rays, 3]."""
+    prob = env_or_prob if isinstance(env_or_prob, Problem3D) else Problem3D(env_or_prob)
+    own = ctx is None
+    if own:
+        ctx = Context(0)
+    try:
+        res = FieldResult()
+        nrays = prob.nrays
+        res.field = np.zeros(prob.field_shape, np.complex64)
+        out = _abi.FieldOut()
+        out.field = res.field.ctypes.data_as(_abi.c_float_p)
+        if stats:
+            res.nsteps = np.zeros(nrays, np.int32)
+            res.ntop = np.zeros(nrays, np.int32)
+            res.nbot = np.zeros(nrays, np.int32)
+            res.xend = np.zeros((nrays, 3), np.float64)
+            out.ray_nsteps = res.nsteps.ctypes.data_as(_abi.c_int32_p)
+            out.ray_ntop = res.ntop.ctypes.data_as(_abi.c_int32_p)
+            out.ray_nbot = res.nbot.ctypes.data_as(_abi.c_int32_p)
+            out.ray_xend = res.xend.ctypes.data_as(_abi.c_double_p)
+        _check(lib().bhc_field_run3d(ctx._h, C.byref(prob.view), C.byref(out)))
+        res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
+        res.err, res.warn = out.err, out.warn
+        return res
+    finally:
+        if own:
+            ctx.close()

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BHC_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["trace2d.hip", "mlp.hip", "opcount2d.cpp"]
-CXX_SOURCES = ["host2d.cpp", "writers2d.cpp"]
+CXX_SOURCES = ["host2d.cpp", "host3d.cpp", "writers2d.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1,0 +1,1033 @@
+// core3d.h -- per-ray physics of the full-3D tracer (3D rays, triangulated
+// boundary grids, depth-only or hexahedral sound speed, Cartesian geometric
+// beams with receiver bearings), written once for the device and the host.
+//
+// Trajectories (positions, tangents, step and bounce counts) follow the
+// reference's 3D path operation for operation: they use only + - * / sqrt and
+// the host-computed launch cos / sin, so they are compared bit for bit. The
+// beam quantities (p, q, torsion phi) also pass through cos / sin / acos of
+// the torsion, evaluated with the device's libm; they reach only the field.
+//
+// Vector / matrix conventions are those of glm, which the reference uses:
+// column-major m[col][row], dot(a, b) = (a.x*b.x + a.y*b.y) + a.z*b.z,
+// (A*B)[c][r] = A[0][r]*B[c][0] + A[1][r]*B[c][1].
+#pragma once
+#include "core2d.h"
+
+namespace bhc3d {
+
+using namespace bhc2d;
+
+template<typename real> struct v3 {
+    real x, y, z;
+};
+template<typename real> __host__ __device__ inline v3<real> V3(real x, real y, real z) { return v3<real>{x, y, z}; }
+template<typename real> __host__ __device__ inline v3<real> operator+(v3<real> a, v3<real> b)
+{
+    return V3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template<typename real> __host__ __device__ inline v3<real> operator-(v3<real> a, v3<real> b)
+{
+    return V3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template<typename real> __host__ __device__ inline v3<real> operator-(v3<real> a) { return V3(-a.x, -a.y, -a.z); }
+template<typename real> __host__ __device__ inline v3<real> sc3(real s, v3<real> a)
+{
+    return V3(s * a.x, s * a.y, s * a.z);
+}
+template<typename real> __host__ __device__ inline v3<real> dv3(v3<real> a, real s)
+{
+    return V3(a.x / s, a.y / s, a.z / s);
+}
+template<typename real> __host__ __device__ inline real dot3(v3<real> a, v3<real> b)
+{
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+template<typename real> __host__ __device__ inline real len3(v3<real> a) { return std::sqrt(dot3(a, a)); }
+template<typename real> __host__ __device__ inline v3<real> cross3(v3<real> a, v3<real> b)
+{
+    return V3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+
+template<typename real> struct m2 {
+    real a00, a01, a10, a11; // a[col][row]
+};
+template<typename real> __host__ __device__ inline m2<real> M2(real a00, real a01, real a10, real a11)
+{
+    return m2<real>{a00, a01, a10, a11};
+}
+template<typename real> __host__ __device__ inline m2<real> mmul(m2<real> a, m2<real> b)
+{
+    return M2(a.a00 * b.a00 + a.a10 * b.a01, a.a01 * b.a00 + a.a11 * b.a01, a.a00 * b.a10 + a.a10 * b.a11,
+              a.a01 * b.a10 + a.a11 * b.a11);
+}
+template<typename real> __host__ __device__ inline m2<real> madd(m2<real> a, m2<real> b)
+{
+    return M2(a.a00 + b.a00, a.a01 + b.a01, a.a10 + b.a10, a.a11 + b.a11);
+}
+template<typename real> __host__ __device__ inline m2<real> msub(m2<real> a, m2<real> b)
+{
+    return M2(a.a00 - b.a00, a.a01 - b.a01, a.a10 - b.a10, a.a11 - b.a11);
+}
+template<typename real> __host__ __device__ inline m2<real> mscl(real s, m2<real> a)
+{
+    return M2(s * a.a00, s * a.a01, s * a.a10, s * a.a11);
+}
+template<typename real> __host__ __device__ inline m2<real> mtr(m2<real> a) { return M2(a.a00, a.a10, a.a01, a.a11); }
+template<typename real> __host__ __device__ inline real mdet(m2<real> a) { return a.a00 * a.a11 - a.a10 * a.a01; }
+
+template<typename real> struct TD3;
+template<> struct TD3<double> {
+    static constexpr double THRESH = 3e-6; // TRIDIAG_THRESH, src/boundary.hpp:36-40
+};
+template<> struct TD3<float> {
+    static constexpr float THRESH = 3e-3f;
+};
+
+// Boundary grid node in the blob: position, then the unit normals of the two
+// triangles of the cell whose lower corner it is.
+enum : int { N3_X = 0, N3_N1 = 3, N3_N2 = 6, NODE3 = 9 };
+
+// Device view of a 3D problem. Offsets index `base` (reals).
+template<typename real> struct Prob3D {
+    Prob2D<real> s;        // depth-only profile: base, ssp_type, npts, o_z, o_rho, o_c, o_cz, o_n2, o_n2z, o_coef
+    const real *base;
+    char run0, run1, ssp3; // RunType[0], RunType[1], SSP type ('H' hexahedral)
+    char pad;
+    int32_t nz3, o_z3, o_rho3;          // depth table and densities (after SegZToZ for 'H')
+    int32_t hx, hy, o_hx, o_hy, o_hc, o_hcz;
+    int32_t ati_nx, ati_ny, bty_nx, bty_ny, o_ati, o_bty;
+    int32_t o_Rr, o_trc, o_brc, ntrc, nbrc, o_sbp, nsbp;
+    int32_t o_alpha, o_acos, o_asin, o_beta, o_bcos, o_bsin, o_Sx, o_Sy, o_trcv;
+    const float *Rz, *Sz;
+    int32_t NSx, NSy, NSz, NRz, NRr, Ntheta, Nalpha, iSingleA, Nbeta, iSingleB;
+    real Dalpha, Dbeta, freq0, deltas, box_x, box_y, box_z;
+    HSd<real> top, bot;
+    int32_t isGaussian, rz_sorted;
+    float *field;
+    int32_t njobs, pad2;
+};
+
+template<typename real> struct Ray3 {
+    m2<real> p, q;
+    real phi;
+    v3<real> x, t;
+    real c, Amp, Phase;
+    cpx<real> tau;
+    int32_t NumTopBnc, NumBotBnc;
+};
+
+template<typename real> struct BState3 {
+    int32_t ix, iy;
+    real xmin, xmax, ymin, ymax;
+    v3<real> x, xmid, n;
+    bool onEdge, side, justSteppedTo, outgoingSide;
+};
+
+template<typename real> struct SSPOut3 {
+    v3<real> gradc;
+    cpx<real> c;
+    real rho, cxx, cyy, czz, cxy, cxz, cyz;
+};
+
+template<typename real> struct Trace3 {
+    int32_t sx, sy, sz, sr;     // SSP segment state
+    BState3<real> top, bot;
+    v3<real> xs;
+    int32_t iSmall;
+};
+
+// ---------------------------------------------------------------------------
+// SSP: Hexahedral (src/ssp.hpp:220-307) and the 3D EvaluateSSP (:377-465)
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline void EvaluateSSP3(v3<real> x, v3<real> t, SSPOut3<real> &o, const Prob3D<real> &P,
+                                             Trace3<real> &S, Err &e)
+{
+    const real *B = P.base;
+    if(P.ssp3 == 'H') {
+        const real *sx = B + P.o_hx, *sy = B + P.o_hy, *z = B + P.o_z3;
+        const int nx = P.hx, ny = P.hy, nz = P.nz3;
+        if(x.x < sx[0] || x.x > sx[nx - 1] || x.y < sy[0] || x.y > sy[ny - 1]) e.error(8);
+        UpdateSSPSegment(x.x, t.x, sx, nx, S.sx);
+        UpdateSSPSegment(x.y, t.y, sy, ny, S.sy);
+        UpdateSSPSegment(x.z, t.z, z, nz, S.sz);
+        const int ix = S.sx, iy = S.sy, iz = S.sz;
+        const real *cm = B + P.o_hc, *czm = B + P.o_hcz;
+        size_t b11 = (size_t)ix * ny + iy, b21 = (size_t)(ix + 1) * ny + iy;
+        size_t b12 = (size_t)ix * ny + iy + 1, b22 = (size_t)(ix + 1) * ny + iy + 1;
+        real cz11 = czm[b11 * (nz - 1) + iz], cz12 = czm[b21 * (nz - 1) + iz];
+        real cz21 = czm[b12 * (nz - 1) + iz], cz22 = czm[b22 * (nz - 1) + iz];
+        real s3 = x.z - z[iz];
+        real c11 = cm[b11 * nz + iz] + s3 * cz11;
+        real c21 = cm[b21 * nz + iz] + s3 * cz12;
+        real c12 = cm[b12 * nz + iz] + s3 * cz21;
+        real c22 = cm[b22 * nz + iz] + s3 * cz22;
+        real s1 = (x.x - sx[ix]) / (sx[ix + 1] - sx[ix]);
+        s1 = rmax(rmin(s1, RL(1.0)), RL(0.0));
+        real s2 = (x.y - sy[iy]) / (sy[iy + 1] - sy[iy]);
+        s2 = rmax(rmin(s2, RL(1.0)), RL(0.0));
+        real c1 = c11 + s1 * (c21 - c11);
+        real c2 = c12 + s1 * (c22 - c12);
+        real cy = (c2 - c1) / (sy[iy + 1] - sy[iy]);
+        c1 = c11 + s2 * (c12 - c11);
+        c2 = c21 + s2 * (c22 - c21);
+        real c = c1 + s1 * (c2 - c1);
+        // the attenuation term reads a profile that SegZToZ
+        // (src/module/ssp.hpp:568-573) set to real NaNs: its imaginary part is +0
+        o.c = C(c, (real)0);
+        real cx = (c2 - c1) / (sx[ix + 1] - sx[ix]);
+        real cz1 = cz11 + s2 * (cz21 - cz11);
+        real cz2 = cz12 + s2 * (cz22 - cz12);
+        real cz = cz1 + s1 * (cz2 - cz1);
+        o.gradc = V3(cx, cy, cz);
+        o.cxx = o.cyy = o.czz = o.cxy = o.cxz = o.cyz = RL(0.0);
+        const real *rho = B + P.o_rho3;
+        real w = (x.z - z[iz]) / (z[iz + 1] - z[iz]);
+        o.rho = (RL(1.0) - w) * rho[iz] + w * rho[iz + 1];
+        return;
+    }
+    SSPOut<real> o2;
+    EvaluateSSP<real, 0>(V((real)RL(0.0), x.z), V((real)RL(0.0), t.z), o2, P.s, B, S.sz, S.sr, e);
+    o.gradc = V3((real)RL(0.0), (real)RL(0.0), o2.gradc.y);
+    o.cxx = o.cyy = o.cxy = o.cxz = o.cyz = RL(0.0);
+    o.czz = o2.czz;
+    o.c = o2.c;
+    o.rho = o2.rho;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary grids (src/boundary.hpp:66-168, 3D)
+// ---------------------------------------------------------------------------
+template<typename real> __host__ __device__ inline v3<real> ld3(const real *p) { return V3(p[0], p[1], p[2]); }
+
+template<typename real>
+__host__ __device__ inline void GetBdrySeg3(v3<real> x, v3<real> t, BState3<real> &b, const real *bd, int nx, int ny,
+                                            bool isTop, bool isInit, Err &e)
+{
+#define BX(i) bd[(size_t)(i) * NODE3 + 0]
+#define BY(i) bd[(size_t)(i) * NODE3 + 1]
+    b.ix = imin(imax(b.ix, 0), nx - 2);
+    b.iy = imin(imax(b.iy, 0), ny - 2);
+    if(t.x >= FL(0.0)) {
+        while(b.ix >= 0 && BX(b.ix * ny) > x.x) --b.ix;
+        while(b.ix >= 0 && b.ix < nx - 1 && BX((b.ix + 1) * ny) <= x.x) ++b.ix;
+    } else {
+        while(b.ix < nx - 1 && BX((b.ix + 1) * ny) < x.x) ++b.ix;
+        while(b.ix >= 0 && b.ix < nx - 1 && BX(b.ix * ny) >= x.x) --b.ix;
+    }
+    if(t.y >= FL(0.0)) {
+        while(b.iy >= 0 && BY(b.iy) > x.y) --b.iy;
+        while(b.iy >= 0 && b.iy < ny - 1 && BY(b.iy + 1) <= x.y) ++b.iy;
+    } else {
+        while(b.iy < ny - 1 && BY(b.iy + 1) < x.y) ++b.iy;
+        while(b.iy >= 0 && b.iy < ny - 1 && BY(b.iy) >= x.y) --b.iy;
+    }
+    if(b.ix == -1 && BX(0) == x.x) b.ix = 0;
+    if(b.ix == nx - 1 && BX((nx - 1) * ny) == x.x) b.ix = nx - 2;
+    if(b.iy == -1 && BY(0) == x.y) b.iy = 0;
+    if(b.iy == ny - 1 && BY(ny - 1) == x.y) b.iy = ny - 2;
+    if(b.ix < 0 || b.ix >= nx - 1 || b.iy < 0 || b.iy >= ny - 1) {
+        e.warning(isTop ? 6 : 7);
+        b.ix = imin(imax(b.ix, 0), nx - 2);
+        b.iy = imin(imax(b.iy, 0), ny - 2);
+    }
+    b.xmin = BX(b.ix * ny);
+    b.xmax = BX((b.ix + 1) * ny);
+    b.ymin = BY(b.iy);
+    b.ymax = BY(b.iy + 1);
+#undef BX
+#undef BY
+    const real *n0 = bd + ((size_t)b.ix * ny + b.iy) * NODE3;
+    const real *n11 = bd + ((size_t)(b.ix + 1) * ny + (b.iy + 1)) * NODE3;
+    b.x = ld3(n0);
+    v3<real> s = b.x + ld3(n11);
+    b.xmid = V3(s.x * RL(0.5), s.y * RL(0.5), s.z * RL(0.5));
+    v2<real> tri_n = V(-(b.ymax - b.ymin), b.xmax - b.xmin);
+    tri_n = dv(tri_n, std::sqrt(dot(tri_n, tri_n)));
+    v2<real> temp = V(x.x - b.xmid.x, x.y - b.xmid.y);
+    real over = dot(temp, tri_n);
+    b.onEdge = std::fabs(over) < TD3<real>::THRESH;
+    if(!isInit && b.justSteppedTo) b.side = b.outgoingSide;
+    else if(!isInit && b.onEdge) b.side = dot(V(t.x, t.y), tri_n) >= RL(0.0);
+    else b.side = over >= RL(0.0);
+    b.justSteppedTo = false;
+    b.n = ld3(n0 + (b.side ? N3_N2 : N3_N1));
+    if(!std::isfinite(b.x.z) || !std::isfinite(b.n.x) || !std::isfinite(b.n.y) || !std::isfinite(b.n.z)) e.error(3);
+}
+
+// ---------------------------------------------------------------------------
+// Step (src/step.hpp, 3D instantiation)
+// ---------------------------------------------------------------------------
+template<typename real> __host__ __device__ inline real cmp3(v3<real> a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
+template<typename real> __host__ __device__ inline void set3(v3<real> &a, int d, real v)
+{
+    if(d == 0) a.x = v;
+    else if(d == 1) a.y = v;
+    else a.z = v;
+}
+
+template<typename real>
+__host__ __device__ inline void DepthInterfaceCrossing3(bool stepTo, real &hInt, v3<real> &x, v3<real> x0,
+                                                        v3<real> u, int iz, const Prob3D<real> &P)
+{ // step.hpp:41-80 (3D: the top and bottom layers extrapolate)
+    if(!stepTo) hInt = RC<real>::MAXV;
+    if(std::fabs(u.z) > RC<real>::EPS) {
+        const real *z = P.base + P.o_z3;
+        if(z[iz] > x.z && iz > 0) {
+            hInt = (z[iz] - x0.z) / u.z;
+            if(stepTo) {
+                x = x0 + sc3(hInt, u);
+                x.z = z[iz];
+            }
+        } else if(z[iz + 1] < x.z && iz + 1 < P.nz3 - 1) {
+            hInt = (z[iz + 1] - x0.z) / u.z;
+            if(stepTo) {
+                x = x0 + sc3(hInt, u);
+                x.z = z[iz + 1];
+            }
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void BeamBoxCrossing3(bool stepTo, int dim, real &hBox, v3<real> &x, v3<real> x0,
+                                                 v3<real> u, const Prob3D<real> &P, v3<real> xs)
+{ // step.hpp:85-111, common_run.hpp:33-47
+    if(!stepTo) hBox = RC<real>::MAXV;
+    v3<real> ctr = V3(xs.x, xs.y, (real)RL(0.0));
+    real box = dim == 0 ? P.box_x : (dim == 1 ? P.box_y : P.box_z);
+    if(std::fabs(cmp3(x, dim) - cmp3(ctr, dim)) >= box) {
+        real d0 = cmp3(x0, dim) - cmp3(ctr, dim);
+        hBox = (box - std::fabs(d0)) / std::fabs(cmp3(u, dim));
+        if(stepTo) {
+            x = x0 + sc3(hBox, u);
+            set3(x, dim, cmp3(ctr, dim) + std::copysign(box, d0));
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void TopBotCrossing3(bool stepTo, real &h, const BState3<real> &bd, v3<real> &x,
+                                                v3<real> x0, v3<real> u, bool &refl)
+{ // step.hpp:116-151
+    if(!stepTo) h = RC<real>::MAXV;
+    v3<real> d = x - bd.x, d0 = x0 - bd.x;
+    real w = dot3(bd.n, d);
+    if(stepTo ? (w > -RC<real>::INF_STEP) : (w >= RL(0.0))) {
+        h = -dot3(d0, bd.n) / dot3(u, bd.n);
+        if(stepTo) {
+            x = x0 + sc3(h, u);
+            if(std::fabs(bd.n.x) < RC<real>::EPS && std::fabs(bd.n.y) < RC<real>::EPS) x.z = bd.x.z;
+        }
+        refl = true;
+    } else {
+        refl = false;
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void TopBotSegCrossing3(bool stepTo, bool isY, real &hSeg, const Trace3<real> &S,
+                                                   int iSeg, v3<real> &x, v3<real> x0, v3<real> u,
+                                                   const Prob3D<real> &P, bool &topRefl, bool &botRefl)
+{ // step.hpp:157-207
+    real lmin = rmax(isY ? S.top.ymin : S.top.xmin, isY ? S.bot.ymin : S.bot.xmin);
+    real lmax = rmin(isY ? S.top.ymax : S.top.xmax, isY ? S.bot.ymax : S.bot.xmax);
+    if(P.ssp3 == 'H') {
+        const real *sw = P.base + (isY ? P.o_hy : P.o_hx);
+        lmin = rmax(lmin, sw[iSeg]);
+        lmax = rmin(lmax, sw[iSeg + 1]);
+    }
+    real xw = isY ? x.y : x.x, x0w = isY ? x0.y : x0.x, uw = isY ? u.y : u.x;
+    if(!stepTo) hSeg = RC<real>::MAXV;
+    if(std::fabs(uw) > RC<real>::EPS) {
+        if(xw < lmin) {
+            hSeg = -(x0w - lmin) / uw;
+            if(stepTo) {
+                x = x0 + sc3(hSeg, u);
+                if(isY) x.y = lmin;
+                else x.x = lmin;
+                topRefl = botRefl = false;
+            }
+        } else if(xw > lmax) {
+            hSeg = -(x0w - lmax) / uw;
+            if(stepTo) {
+                x = x0 + sc3(hSeg, u);
+                if(isY) x.y = lmax;
+                else x.x = lmax;
+                topRefl = botRefl = false;
+            }
+        }
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void TriDiagCrossing(bool stepTo, real &h, BState3<real> &bd, v3<real> &x, v3<real> x0,
+                                                v3<real> u, bool &topRefl, bool &botRefl, Err &e)
+{ // step.hpp:212-260
+    if(!stepTo) h = RC<real>::MAXV;
+    if(bd.onEdge) return;
+    v3<real> d = x - bd.xmid, d0 = x0 - bd.xmid;
+    v3<real> tri_n = V3(-(bd.ymax - bd.ymin), bd.xmax - bd.xmin, (real)RL(0.0));
+    tri_n = dv3(tri_n, len3(tri_n));
+    real over = dot3(d, tri_n);
+    bool newSide = over >= RL(0.0);
+    if(newSide == bd.side) return;
+    if(std::fabs(over) < TD3<real>::THRESH) return;
+    real hnew = -dot3(d0, tri_n) / dot3(u, tri_n);
+    if(hnew < RL(0.0)) {
+        e.warning(9);
+        h = RL(0.0);
+    } else {
+        if(stepTo && hnew >= h) e.warning(10);
+        h = hnew;
+    }
+    if(stepTo) {
+        x = x0 + sc3(h, u);
+        bd.justSteppedTo = true;
+        bd.outgoingSide = newSide;
+        topRefl = botRefl = false;
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void ReduceStep3(v3<real> x0, v3<real> u, const int iSeg0[3], Trace3<real> &S,
+                                            const Prob3D<real> &P, Err &e, real &h)
+{ // step.hpp:271-337
+    real hInt, hBx, hBy, hBz, hTop, hBot, hxS, hyS, hTD, hBD;
+    bool dummy = false;
+    v3<real> x = x0 + sc3(h, u);
+    DepthInterfaceCrossing3(false, hInt, x, x0, u, iSeg0[2], P);
+    BeamBoxCrossing3(false, 0, hBx, x, x0, u, P, S.xs);
+    BeamBoxCrossing3(false, 1, hBy, x, x0, u, P, S.xs);
+    BeamBoxCrossing3(false, 2, hBz, x, x0, u, P, S.xs);
+    TopBotCrossing3(false, hTop, S.top, x, x0, u, dummy);
+    TopBotCrossing3(false, hBot, S.bot, x, x0, u, dummy);
+    TopBotSegCrossing3(false, false, hxS, S, iSeg0[0], x, x0, u, P, dummy, dummy);
+    TopBotSegCrossing3(false, true, hyS, S, iSeg0[1], x, x0, u, P, dummy, dummy);
+    TriDiagCrossing(false, hTD, S.top, x, x0, u, dummy, dummy, e);
+    TriDiagCrossing(false, hBD, S.bot, x, x0, u, dummy, dummy, e);
+    h = rmin(rmin(rmin(h, hInt), rmin(rmin(rmin(hBx, hBy), hBz), rmin(hTop, hBot))), rmin(rmin(hxS, hyS), rmin(hTD, hBD)));
+    if(h < RL(-1e-4)) e.warning(8);
+    if(h < RC<real>::INF_STEP * P.deltas) {
+        h = RC<real>::INF_STEP * P.deltas;
+        ++S.iSmall;
+    } else {
+        S.iSmall = 0;
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void StepToBdry3(v3<real> x0, v3<real> &x2, v3<real> u, real &h, bool &topRefl,
+                                            bool &botRefl, const int iSeg0[3], Trace3<real> &S, const Prob3D<real> &P,
+                                            Err &e)
+{ // step.hpp:342-420
+    x2 = x0 + sc3(h, u);
+    DepthInterfaceCrossing3(true, h, x2, x0, u, iSeg0[2], P);
+    BeamBoxCrossing3(true, 0, h, x2, x0, u, P, S.xs);
+    BeamBoxCrossing3(true, 1, h, x2, x0, u, P, S.xs);
+    BeamBoxCrossing3(true, 2, h, x2, x0, u, P, S.xs);
+    TopBotCrossing3(true, h, S.top, x2, x0, u, topRefl);
+    TopBotCrossing3(true, h, S.bot, x2, x0, u, botRefl);
+    if(botRefl) topRefl = false;
+    TopBotSegCrossing3(true, false, h, S, iSeg0[0], x2, x0, u, P, topRefl, botRefl);
+    TopBotSegCrossing3(true, true, h, S, iSeg0[1], x2, x0, u, P, topRefl, botRefl);
+    TriDiagCrossing(true, h, S.top, x2, x0, u, topRefl, botRefl, e);
+    TriDiagCrossing(true, h, S.bot, x2, x0, u, topRefl, botRefl, e);
+    if(h < RC<real>::INF_STEP * P.deltas) {
+        h = RC<real>::INF_STEP * P.deltas;
+        x2 = x0 + sc3(h, u);
+        if(dot3(S.top.n, x2 - S.top.x) > RC<real>::EPS) {
+            topRefl = true;
+            S.top.justSteppedTo = false;
+            S.bot.justSteppedTo = false;
+        } else {
+            topRefl = false;
+        }
+        if(dot3(S.bot.n, x2 - S.bot.x) > RC<real>::EPS) {
+            botRefl = true;
+            topRefl = false;
+            S.top.justSteppedTo = false;
+            S.bot.justSteppedTo = false;
+        } else {
+            botRefl = false;
+        }
+    }
+}
+
+// RayNormal / RayNormal_unit, src/common_run.hpp:144-215
+template<typename real>
+__host__ __device__ inline void RayNormal3(v3<real> t, real phi, bool ignorephi0, real c, v3<real> &e1, v3<real> &e2)
+{
+    real rl = std::sqrt(t.x * t.x + t.y * t.y);
+    if(std::fabs(phi) >= RC<real>::EPS || ignorephi0) {
+        real cosphi = std::cos(phi), sinphi = std::sin(phi);
+        e1.x = (c * t.x * t.z * cosphi + t.y * sinphi) / rl;
+        e1.y = (c * t.y * t.z * cosphi - t.x * sinphi) / rl;
+        e1.z = -c * rl * cosphi;
+        e2.x = (c * t.x * t.z * sinphi - t.y * cosphi) / rl;
+        e2.y = (c * t.y * t.z * sinphi + t.x * cosphi) / rl;
+        e2.z = -c * rl * sinphi;
+    } else {
+        e1 = V3(c * t.x * t.z / rl, c * t.y * t.z / rl, -c * rl);
+        e2 = V3(-t.y / rl, t.x / rl, (real)RL(0.0));
+    }
+}
+
+template<typename real> struct PQ3 {
+    real phi;
+    m2<real> p, q;
+};
+
+template<typename real>
+__host__ __device__ inline PQ3<real> DeltaPQ3(const Ray3<real> &ray, const SSPOut3<real> &o)
+{ // Get_c_partials + ComputeDeltaPQ, step.hpp:425-473 (3D)
+    const real c = o.c.re;
+    v3<real> e1, e2;
+    RayNormal3(ray.t, ray.phi, false, c, e1, e2);
+    real cnn = o.cxx * (e1.x * e1.x) + o.cyy * (e1.y * e1.y) + o.czz * (e1.z * e1.z) + FL(2.0) * o.cxy * e1.x * e1.y
+        + FL(2.0) * o.cxz * e1.x * e1.z + FL(2.0) * o.cyz * e1.y * e1.z;
+    real cmn = o.cxx * e1.x * e2.x + o.cyy * e1.y * e2.y + o.czz * e1.z * e2.z + o.cxy * (e1.x * e2.y + e2.x * e1.y)
+        + o.cxz * (e1.x * e2.z + e2.x * e1.z) + o.cyz * (e1.y * e2.z + e2.y * e1.z);
+    real cmm = o.cxx * (e2.x * e2.x) + o.cyy * (e2.y * e2.y) + o.czz * (e2.z * e2.z) + FL(2.0) * o.cxy * e2.x * e2.y
+        + FL(2.0) * o.cxz * e2.x * e2.z + FL(2.0) * o.cyz * e2.y * e2.z;
+    PQ3<real> pq;
+    pq.phi = (FL(1.0) / c) * ray.t.z * (ray.t.y * o.gradc.x - ray.t.x * o.gradc.y)
+        / (ray.t.x * ray.t.x + ray.t.y * ray.t.y);
+    m2<real> cm = mscl(-RL(1.0) / (c * c), M2(cnn, cmn, cmn, cmm));
+    pq.p = mmul(ray.q, mtr(cm));
+    pq.q = mscl(c, ray.p);
+    return pq;
+}
+
+template<typename real>
+__host__ __device__ inline void UpdatePQ3(Ray3<real> &r1, const Ray3<real> &r0, real h, const PQ3<real> &pq)
+{ // step.hpp:475-481
+    real phi = r0.phi + h * pq.phi;
+    m2<real> p = madd(r0.p, mscl(h, pq.p));
+    m2<real> q = madd(r0.q, mscl(h, pq.q));
+    r1.phi = phi;
+    r1.p = p;
+    r1.q = q;
+}
+
+template<typename real>
+__host__ __device__ inline void CurvCorr3D(Ray3<real> &ray, m2<real> D, real Tg, real Th, real cn1, real cn2, real cs,
+                                           v3<real> rayn1, v3<real> rayn2, v3<real> e1, v3<real> e2, bool reflect)
+{ // step.hpp:486-572
+    real rm = Tg / Th;
+    real csq = ray.c * ray.c;
+    m2<real> r;
+    r.a00 = FL(2.0) / csq * D.a00 / Th + rm * (FL(2.0) * cn1 - rm * cs) / csq;
+    r.a01 = FL(2.0) / ray.c * D.a10 * std::copysign(RL(1.0), -Th) + rm * cn2 / csq;
+    r.a10 = r.a01;
+    r.a11 = FL(2.0) * D.a11 * Th;
+    if(reflect) r.a10 = -r.a10;
+    else r = mscl(RL(-1.0), r);
+    bool noChange = false;
+    if(reflect)
+        noChange = std::fabs(r.a00) < RC<real>::EPS && std::fabs(r.a01) < RC<real>::EPS
+            && std::fabs(r.a11) < RC<real>::EPS;
+    if(!noChange) {
+        m2<real> R;
+        R.a00 = dot3(rayn1, e1);
+        R.a10 = dot3(rayn1, e2);
+        R.a01 = -R.a10;
+        R.a11 = dot3(rayn2, e2);
+        m2<real> p_in = mmul(R, ray.p);
+        m2<real> q_in = mmul(R, ray.q);
+        m2<real> p_out = madd(p_in, mmul(r, q_in));
+        ray.p = mmul(mtr(R), p_out);
+    }
+    if(reflect) {
+        real d = dot3(rayn1, e1);
+        ray.phi += FL(2.0) * std::acos(rmax(rmin(d, RL(1.0)), RL(-1.0)));
+    }
+}
+
+template<typename real>
+__host__ __device__ inline void TangentNormals3(const Ray3<real> &ray, real c, v3<real> nB, v3<real> &rayt,
+                                                v3<real> &rayn1, v3<real> &rayn2, real sgn)
+{ // CalcTangent_Normals, step.hpp:574-584
+    rayt = sc3(c, ray.t);
+    rayn2 = sc3(sgn, cross3(rayt, nB));
+    rayn2 = dv3(rayn2, len3(rayn2));
+    rayn1 = -cross3(rayt, rayn2);
+}
+
+template<typename real>
+__host__ __device__ inline void CurvCorrSSP3(Ray3<real> &ray, v3<real> gj, bool dz, bool dx)
+{ // CurvatureCorrection, step.hpp:589-627 (3D)
+    v3<real> nB = V3((real)RL(0.0), (real)RL(0.0), (real)RL(0.0));
+    if(dz) nB.z = -std::copysign(RL(1.0), ray.t.z);
+    else if(dx) nB.x = -std::copysign(RL(1.0), ray.t.x);
+    else nB.y = -std::copysign(RL(1.0), ray.t.y);
+    real Th = dot3(ray.t, nB);
+    v3<real> tB = ray.t - sc3(Th, nB);
+    tB = dv3(tB, len3(tB));
+    real Tg = dot3(ray.t, tB);
+    v3<real> rayt, rayn1, rayn2, e1, e2;
+    TangentNormals3(ray, ray.c, nB, rayt, rayn1, rayn2, RL(1.0));
+    RayNormal3(rayt, ray.phi, true, (real)RL(1.0), e1, e2);
+    CurvCorr3D(ray, M2((real)RL(0.0), (real)RL(0.0), (real)RL(0.0), (real)RL(0.0)), Tg, Th, dot3(gj, rayn1),
+               dot3(gj, rayn2), dot3(gj, rayt), rayn1, rayn2, e1, e2, false);
+}
+
+template<typename real>
+__host__ __device__ inline void Step3(const Ray3<real> &ray0, Ray3<real> &ray2, Trace3<real> &S,
+                                      const Prob3D<real> &P, Err &e, bool &topRefl, bool &botRefl)
+{ // step.hpp:654-811 (3D)
+    Ray3<real> ray1;
+    SSPOut3<real> o0, o1, o2;
+    EvaluateSSP3(ray0.x, ray0.t, o0, P, S, e);
+    PQ3<real> pq0 = DeltaPQ3(ray0, o0);
+    const int iSeg0[3] = {S.sx, S.sy, S.sz};
+    real c0 = o0.c.re, csq0 = c0 * c0;
+    v3<real> u0 = sc3(c0, ray0.t);
+    real h = P.deltas;
+    ReduceStep3(ray0.x, u0, iSeg0, S, P, e, h);
+    real halfh = FL(0.5) * h;
+    ray1.x = ray0.x + sc3(halfh, u0);
+    ray1.t = ray0.t - dv3(sc3(halfh, o0.gradc), csq0);
+    UpdatePQ3(ray1, ray0, halfh, pq0);
+    EvaluateSSP3(ray1.x, ray1.t, o1, P, S, e);
+    PQ3<real> pq1 = DeltaPQ3(ray1, o1);
+    real c1 = o1.c.re, csq1 = c1 * c1;
+    v3<real> u1 = sc3(c1, ray1.t);
+    ReduceStep3(ray0.x, u1, iSeg0, S, P, e, h);
+    real w1 = h / (RL(2.0) * halfh);
+    real w0 = RL(1.0) - w1;
+    v3<real> u2 = sc3(w0, u0) + sc3(w1, u1);
+    v3<real> x2;
+    StepToBdry3(ray0.x, x2, u2, h, topRefl, botRefl, iSeg0, S, P, e);
+    ray2.x = x2;
+    real hw0 = h * w0, hw1 = h * w1;
+    ray2.t = (ray0.t - dv3(sc3(hw0, o0.gradc), csq0)) - dv3(sc3(hw1, o1.gradc), csq1);
+    ray2.tau = ray0.tau + realdiv(hw0, o0.c) + realdiv(hw1, o1.c);
+    UpdatePQ3(ray2, ray0, hw0, pq0);
+    UpdatePQ3(ray2, ray2, hw1, pq1);
+    ray2.Amp = ray0.Amp;
+    ray2.Phase = ray0.Phase;
+    ray2.NumTopBnc = ray0.NumTopBnc;
+    ray2.NumBotBnc = ray0.NumBotBnc;
+    EvaluateSSP3(ray2.x, ray2.t, o2, P, S, e);
+    ray2.c = o2.c.re;
+    if(S.sz != iSeg0[2] || S.sx != iSeg0[0] || S.sy != iSeg0[1])
+        CurvCorrSSP3(ray2, o2.gradc - o0.gradc, S.sz != iSeg0[2], S.sx != iSeg0[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Reflection (src/reflect.hpp:125-452, 3D, rectilinear boundaries)
+// ---------------------------------------------------------------------------
+template<typename real>
+__host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &newp, const HSd<real> &hs, bool isTop,
+                                         v3<real> nB, const Prob3D<real> &P, Trace3<real> &S, Err &e)
+{
+    real Th = dot3(oldp.t, nB);
+    v3<real> tB = oldp.t - sc3(Th, nB);
+    tB = sc3(RL(1.0) / len3(tB), tB);
+    real Tg = dot3(oldp.t, tB);
+    newp.NumTopBnc = oldp.NumTopBnc + (isTop ? 1 : 0);
+    newp.NumBotBnc = oldp.NumBotBnc + (isTop ? 0 : 1);
+    newp.x = oldp.x;
+    newp.t = oldp.t - sc3(FL(2.0) * Th, nB);
+    SSPOut3<real> o;
+    EvaluateSSP3(oldp.x, newp.t, o, P, S, e);
+    real c = o.c.re;
+    newp.c = c;
+    newp.tau = oldp.tau;
+    v3<real> rayt, rayn1, rayn2, rayt_t, rayn1_t, rayn2_t;
+    TangentNormals3(oldp, c, nB, rayt, rayn1, rayn2, RL(-1.0));
+    TangentNormals3(newp, c, nB, rayt_t, rayn1_t, rayn2_t, RL(-1.0));
+    v2<real> t_rot = V(rayt.x, rayt.y);
+    t_rot = sc(RL(1.0) / std::sqrt(dot(t_rot, t_rot)), t_rot);
+    v2<real> n_rot = V(rayn2.x, rayn2.y);
+    n_rot = sc(RL(1.0) / std::sqrt(dot(n_rot, n_rot)), n_rot);
+    m2<real> R = M2(t_rot.x, t_rot.y, n_rot.x, n_rot.y);
+    // a rectilinear grid carries no curvature: kappaMat = 0
+    const real k0 = RL(0.0) / FL(2.0);
+    m2<real> DMat = mmul(mtr(R), mmul(mscl((real)FL(2.0), M2(k0, k0, k0, k0)), R));
+    real cn1 = dot3(o.gradc, -rayn1_t - rayn1);
+    real cn2 = dot3(o.gradc, -rayn2_t - rayn2);
+    real cs = -dot3(o.gradc, rayt_t - rayt);
+    v3<real> e1, e2;
+    RayNormal3(oldp.t, oldp.phi, false, oldp.c, e1, e2);
+    newp.p = oldp.p;
+    newp.q = oldp.q;
+    newp.phi = oldp.phi;
+    CurvCorr3D(newp, DMat, Tg, Th, cn1, cn2, cs, rayn1, rayn2, e1, e2, true);
+
+    if(hs.bc == 'R') {
+        newp.Amp = oldp.Amp;
+        newp.Phase = oldp.Phase;
+    } else if(hs.bc == 'V') {
+        newp.Amp = oldp.Amp;
+        newp.Phase = oldp.Phase + RC<real>::PI;
+    } else if(hs.bc == 'F') { // InterpolateReflectionCoefficient reflect.hpp:39-92
+        const real *tab = P.base + (isTop ? P.o_trc : P.o_brc);
+        int n = isTop ? P.ntrc : P.nbrc;
+        real theta = (RL(180.0) / RC<real>::PI) * std::fabs(std::atan2(Th, Tg));
+        if(theta > FL(90.0)) theta = FL(180.0) - theta;
+        real r, phi;
+        int iL = 0, iR = n - 1;
+        if(theta < tab[0]) {
+            r = phi = FL(0.0);
+            e.warning(5);
+        } else if(theta > tab[3 * iR]) {
+            r = phi = FL(0.0);
+        } else {
+            while(iL != iR - 1) {
+                int iM = (iL + iR) / 2;
+                if(tab[3 * iM] > theta) iR = iM;
+                else iL = iM;
+            }
+            real a = (theta - tab[3 * iL]) / (tab[3 * iR] - tab[3 * iL]);
+            r = (FL(1.0) - a) * tab[3 * iL + 1] + a * tab[3 * iR + 1];
+            phi = (FL(1.0) - a) * tab[3 * iL + 2] + a * tab[3 * iR + 2];
+        }
+        newp.Amp = oldp.Amp * r;
+        newp.Phase = oldp.Phase + phi;
+    } else if(hs.bc == 'A' || hs.bc == 'G') { // fluid halfspace, O3D branch reflect.hpp:279-299
+        real omega = FL(2.0) * RC<real>::PI * P.freq0;
+        cpx<real> gk = C(omega * Tg, (real)0);
+        cpx<real> gk2 = cmul(gk, gk);
+        real w1 = omega / c;
+        cpx<real> eps = C((real)0, RC<real>::MINPOS); // J * REAL_MINPOS
+        cpx<real> g1sq = (C(w1 * w1, (real)0) - gk2) - eps;
+        cpx<real> wP = realdiv(omega, hs.cP);
+        cpx<real> g2sq = (cmul(wP, wP) - gk2) - eps;
+        cpx<real> g1 = csqrt_(-g1sq), g2 = csqrt_(-g2sq);
+        cpx<real> a = rmul(hs.rho, g1), b = rmul(o.rho, g2);
+        cpx<real> Refl = cdiv(a - b, a + b);
+        real aR = cabs_(Refl);
+        if(aR < FL(1.0e-5)) {
+            newp.Amp = FL(0.0);
+            newp.Phase = oldp.Phase;
+        } else {
+            newp.Amp = aR * oldp.Amp;
+            newp.Phase = oldp.Phase + std::atan2(Refl.im, Refl.re);
+        }
+    } else {
+        e.error(4);
+        newp.Amp = NAN;
+        newp.Phase = NAN;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ray init / update / terminate (src/trace.hpp, 3D)
+// ---------------------------------------------------------------------------
+template<typename real> struct Infl3 {
+    real Ratio1, rcp_q0, rcp_qhat0, omega, BeamWindow;
+    real phase, qOld;
+    v3<real> xs;
+    int32_t ir;
+};
+
+template<typename real>
+__host__ __device__ inline void JobIndices3(int job, const Prob3D<real> &P, int &ia, int &ib, int &isy, int &isx,
+                                            int &isz)
+{ // common.hpp:365-391
+    if(P.iSingleA >= 1) ia = P.iSingleA - 1;
+    else {
+        ia = job % P.Nalpha;
+        job /= P.Nalpha;
+    }
+    if(P.iSingleB >= 1) ib = P.iSingleB - 1;
+    else {
+        ib = job % P.Nbeta;
+        job /= P.Nbeta;
+    }
+    isy = job % P.NSy;
+    job /= P.NSy;
+    isx = job % P.NSx;
+    job /= P.NSx;
+    isz = job;
+}
+
+template<typename real>
+__host__ __device__ inline bool RayInit3(int ia, int ib, int isx, int isy, int isz, Ray3<real> &p0, Trace3<real> &S,
+                                         Infl3<real> &I, const Prob3D<real> &P, Err &e)
+{ // trace.hpp:51-67, 76-266 (3D) + Init_Influence influence.hpp:525-644 (geometric Cartesian)
+    const real *G = P.base;
+    real alpha = G[P.o_alpha + ia];
+    real SrcDecl = (RL(180.0) / RC<real>::PI) * alpha;
+    S.sx = S.sy = S.sz = S.sr = 0;
+    S.xs = V3(G[P.o_Sx + isx], G[P.o_Sy + isy], (real)P.Sz[isz]);
+    SSPOut3<real> o;
+    EvaluateSSP3(S.xs, V3((real)FL(0.0), (real)FL(0.0), (real)FL(1.0)), o, P, S, e);
+    real c = o.c.re;
+    const real *sbp = G + P.o_sbp;
+    int ibp = BinarySearchLEQ(sbp, P.nsbp, 2, 0, SrcDecl);
+    ibp = imin(ibp, P.nsbp - 2);
+    real s = (SrcDecl - sbp[2 * ibp]) / (sbp[2 * (ibp + 1)] - sbp[2 * ibp]);
+    float Amp0 = (float)((FL(1.0) - s) * sbp[2 * ibp + 1] + s * sbp[2 * (ibp + 1) + 1]);
+    const real ca = G[P.o_acos + ia], sa = G[P.o_asin + ia];
+    if(P.run0 == 'S') {
+        float omega = (float)(FL(2.0) * RC<real>::PI * P.freq0);
+        Amp0 = (float)(Amp0 * (sqrtf(FL(2.0)) * std::fabs(std::sin(omega / c * S.xs.z * sa))));
+    }
+    v3<real> tinit = V3(ca * G[P.o_bcos + ib], ca * G[P.o_bsin + ib], sa);
+    p0.x = S.xs;
+    p0.c = c;
+    p0.t = dv3(tinit, c);
+    p0.tau = C((real)0, (real)0);
+    p0.Amp = Amp0;
+    p0.Phase = FL(0.0);
+    p0.NumTopBnc = p0.NumBotBnc = 0;
+    p0.p = M2((real)FL(1.0), (real)FL(0.0), (real)FL(0.0), (real)FL(1.0));
+    p0.q = M2((real)FL(0.0), (real)FL(0.0), (real)FL(0.0), (real)FL(0.0));
+    p0.phi = FL(0.0);
+    S.top = BState3<real>{};
+    S.bot = BState3<real>{};
+    GetBdrySeg3(S.xs, p0.t, S.top, G + P.o_ati, P.ati_nx, P.ati_ny, true, true, e);
+    GetBdrySeg3(S.xs, p0.t, S.bot, G + P.o_bty, P.bty_nx, P.bty_ny, false, true, e);
+    real dt = -dot3(S.top.n, S.xs - S.top.x), db = -dot3(S.bot.n, S.xs - S.bot.x);
+    if(dt <= FL(0.0) || db <= FL(0.0)) {
+        e.warning(4);
+        return false;
+    }
+    S.iSmall = 0;
+    // Init_Influence
+    I.omega = FL(2.0) * RC<real>::PI * P.freq0;
+    I.BeamWindow = P.isGaussian ? RL(4.0) : RL(1.0);
+    I.Ratio1 = std::sqrt(std::fabs(ca));
+    I.Ratio1 *= std::sqrt(P.Dalpha * P.Dbeta) / p0.c;
+    if(P.isGaussian) I.Ratio1 /= FL(2.0) * RC<real>::PI;
+    I.rcp_q0 = P.Dalpha / p0.c;
+    I.rcp_qhat0 = std::fabs(ca) * P.Dbeta / p0.c;
+    I.phase = FL(0.0);
+    I.qOld = mdet(p0.q);
+    I.xs = p0.x;
+    I.ir = BinarySearchGT(G + P.o_Rr, P.NRr, (real)RL(0.0));
+    return true;
+}
+
+template<typename real>
+__host__ __device__ inline bool RayUpdate3(const Ray3<real> &p0, Ray3<real> &p1, Ray3<real> &p2, Trace3<real> &S,
+                                           const Prob3D<real> &P, Err &e)
+{ // trace.hpp:274-381 (3D, rectilinear boundaries)
+    bool topRefl = false, botRefl = false;
+    Step3(p0, p1, S, P, e, topRefl, botRefl);
+    GetBdrySeg3(p1.x, p1.t, S.top, P.base + P.o_ati, P.ati_nx, P.ati_ny, true, false, e);
+    GetBdrySeg3(p1.x, p1.t, S.bot, P.base + P.o_bty, P.bty_nx, P.bty_ny, false, false, e);
+    if(topRefl || botRefl) {
+        v3<real> n = topRefl ? S.top.n : S.bot.n;
+        Reflect3(p1, p2, topRefl ? P.top : P.bot, topRefl, n, P, S, e);
+        return true;
+    }
+    return false;
+}
+
+template<typename real>
+__host__ __device__ inline bool RayTerminate3(const Ray3<real> &pt, int is, const Trace3<real> &S,
+                                              const Prob3D<real> &P, Err &e)
+{ // trace.hpp:391-477 (3D)
+    v3<real> x = pt.x, t = pt.t, xs = S.xs;
+    bool leftbox = std::fabs(x.x - xs.x) >= P.box_x || std::fabs(x.y - xs.y) >= P.box_y
+        || std::fabs(x.z - RL(0.0)) >= P.box_z;
+    const real *bb = P.base + P.o_bty, *ab = P.base + P.o_ati;
+    real minx = rmax(bb[0], ab[0]);
+    real miny = rmax(bb[1], ab[1]);
+    real maxx = rmin(bb[(size_t)(P.bty_nx - 1) * P.bty_ny * NODE3], ab[(size_t)(P.ati_nx - 1) * P.ati_ny * NODE3]);
+    real maxy = rmin(bb[(size_t)(P.bty_ny - 1) * NODE3 + 1], ab[(size_t)(P.ati_ny - 1) * NODE3 + 1]);
+    bool e0 = x.x < minx || x.y < miny || (x.x == minx && t.x < RL(0.0)) || (x.y == miny && t.y < RL(0.0));
+    bool eN = x.x > maxx || x.y > maxy || (x.x == maxx && t.x > RL(0.0)) || (x.y == maxy && t.y > RL(0.0));
+    bool small = S.iSmall > 50;
+    bool lost = pt.Amp < FL(0.005);
+    if(leftbox || lost || e0 || eN || small) return true;
+    if(is >= 100000 - 3) {
+        e.warning(1);
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Influence: geometric Cartesian beams (src/influence.hpp:897-1024, 1201-1406, 3D)
+// ---------------------------------------------------------------------------
+// Step segment descriptor: the receiver work of one step, scattered later.
+template<typename real> struct Seg3D {
+    v3<real> xa, ta, e1, e2;    // segment start, unit tangent, ray normals
+    real rlen, lo, hi;          // length; receiver ranges in [lo, hi)
+    real zlo, zhi, Ldiag;       // depth extent of the segment, beam half-diagonal
+    m2<real> q0, dq;
+    cpx<real> tau0, dtau;
+    real A0;                    // Ratio1 * c1 * Amp1
+    real phase0, qOld;          // Phase0 + accumulated caustic phase; det q at the segment start
+    real rcp_q0, rcp_qhat0;
+    real xs_x, xs_y;
+    int32_t ir0, ir1, dir;      // receiver ranges visited by the walk
+    int32_t nitems;             // > 0 when the walk brackets at least one range
+    int32_t isz, isx, isy, pad;
+};
+
+template<typename real>
+__host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, const Ray3<real> &p1, Infl3<real> &I,
+                                         int is, const Prob3D<real> &P, int isz, int isx, int isy)
+{
+    d.nitems = 0;
+    const real *Rr = P.base + P.o_Rr;
+    real dax = p0.x.x - I.xs.x, day = p0.x.y - I.xs.y, dbx = p1.x.x - I.xs.x, dby = p1.x.y - I.xs.y;
+    real rA = std::sqrt(dax * dax + day * day);
+    real rB = std::sqrt(dbx * dbx + dby * dby);
+    if(std::fabs(rA - rB) <= RL(1.0e3) * spacing(rA)) return; // IsDuplicatePoint
+    if(is == 0) I.ir = (rB > rA) ? 0 : P.NRr - 1;
+    v3<real> rayt = p1.x - p0.x;
+    real rlen = len3(rayt);
+    if(rlen <= RL(1.0e3) * spacing(p1.x.x)) return;
+    rayt = dv3(rayt, rlen);
+    v3<real> e1, e2;
+    RayNormal3(rayt, p1.phi, true, (real)RL(1.0), e1, e2);
+    real phaseq = mdet(p0.q);
+    if((phaseq <= RL(0.0) && I.qOld > RL(0.0)) || (phaseq >= RL(0.0) && I.qOld < RL(0.0)))
+        I.phase += RC<real>::PI / FL(2.0);
+    I.qOld = phaseq;
+    real a0 = p0.q.a00 * I.rcp_q0, b0 = p0.q.a10 * I.rcp_q0, a1 = p1.q.a00 * I.rcp_q0, b1 = p1.q.a10 * I.rcp_q0;
+    real l1 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
+    a0 = p0.q.a01 * I.rcp_qhat0;
+    b0 = p0.q.a11 * I.rcp_qhat0;
+    a1 = p1.q.a01 * I.rcp_qhat0;
+    b1 = p1.q.a11 * I.rcp_qhat0;
+    real l2 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
+    d.Ldiag = std::sqrt(l1 * l1 + l2 * l2);
+    // receiver walk: the visited ranges are contiguous
+    int ir = I.ir, first = I.ir;
+    for(;;) {
+        int irTT;
+        if(rB > Rr[ir]) {
+            if(ir >= P.NRr - 1) break;
+            irTT = ir + 1;
+            if(Rr[irTT] >= rB) break;
+        } else {
+            if(ir <= 0) break;
+            irTT = ir - 1;
+            if(Rr[irTT] <= rB) break;
+        }
+        ir = irTT;
+    }
+    I.ir = ir;
+    d.ir0 = first;
+    d.ir1 = ir;
+    d.dir = (ir >= first) ? 1 : -1;
+    d.lo = rmin(rA, rB);
+    d.hi = rmax(rA, rB);
+    d.xa = p0.x;
+    d.ta = rayt;
+    d.e1 = e1;
+    d.e2 = e2;
+    d.rlen = rlen;
+    d.zlo = rmin(p0.x.z, p1.x.z);
+    d.zhi = rmax(p0.x.z, p1.x.z);
+    d.q0 = p0.q;
+    d.dq = msub(p1.q, p0.q);
+    d.tau0 = p0.tau;
+    d.dtau = p1.tau - p0.tau;
+    d.A0 = I.Ratio1 * p1.c * p1.Amp;
+    d.phase0 = p0.Phase + I.phase;
+    d.qOld = I.qOld;
+    d.rcp_q0 = I.rcp_q0;
+    d.rcp_qhat0 = I.rcp_qhat0;
+    d.xs_x = I.xs.x;
+    d.xs_y = I.xs.y;
+    d.isz = isz;
+    d.isx = isx;
+    d.isy = isy;
+    d.nitems = ((ir >= first ? ir - first : first - ir) + 1) * P.Ntheta;
+}
+
+// One (range, bearing) row of a segment: the row tests of the reference and
+// the depth window [zmin, zmax]. Returns false when the row has no receivers.
+template<typename real>
+__host__ __device__ inline bool row3(const Seg3D<real> &d, int ir, int itheta, const Prob3D<real> &P, real &xr,
+                                     real &yr, real &zmin, real &zmax)
+{
+    const real *G = P.base;
+    real rr = G[P.o_Rr + ir];
+    if(!(rr >= d.lo && rr < d.hi)) return false;
+    const real BW = P.isGaussian ? RL(4.0) : RL(1.0);
+    real tx = G[P.o_trcv + 2 * itheta], ty = G[P.o_trcv + 2 * itheta + 1];
+    xr = d.xs_x + rr * tx;
+    yr = d.xs_y + rr * ty;
+    real mprime = std::fabs((xr - d.xa.x) * (-d.ta.y) + (yr - d.xa.y) * d.ta.x);
+    if(!P.isGaussian && mprime > BW * d.Ldiag) return false;
+    real sdot = (xr - d.xs_x) * (d.xa.x - d.xs_x) + (yr - d.xs_y) * (d.xa.y - d.xs_y);
+    if(sdot < RL(0.0)) return false;
+    real nrz = d.ta.x * tx - d.ta.y * (-ty);
+    if(std::fabs(nrz) < RL(1e-9)) return false;
+    real Lz = BW * d.Ldiag / std::fabs(nrz);
+    zmin = d.zlo - Lz;
+    zmax = d.zhi + Lz;
+    return true;
+}
+
+template<typename real, typename Sink, bool FAST = false>
+__host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, int iz, real xr, real yr, real zmin,
+                                      real zmax, const Prob3D<real> &P, Sink &sink)
+{
+    real zr = (real)P.Rz[iz];
+    if(zr < zmin || zr > zmax) return;
+    v3<real> xrr = V3(xr, yr, zr) - d.xa;
+    real s = dot3(xrr, d.ta) / d.rlen;
+    real n1 = std::fabs(dot3(xrr, d.e1));
+    real n2 = std::fabs(dot3(xrr, d.e2));
+    sink.mark();
+    m2<real> qI = madd(d.q0, mscl(s, d.dq));
+    qI.a00 *= d.rcp_q0;
+    qI.a10 *= d.rcp_q0;
+    qI.a01 *= d.rcp_qhat0;
+    qI.a11 *= d.rcp_qhat0;
+    real qF = mdet(qI);
+    real l1 = std::sqrt(qI.a00 * qI.a00 + qI.a10 * qI.a10);
+    real l2 = std::sqrt(qI.a01 * qI.a01 + qI.a11 * qI.a11);
+    if(std::fabs(l1) < RC<real>::EPS || std::fabs(l2) < RC<real>::EPS) return;
+    if(std::fabs(qF) < RC<real>::EPS) return;
+    real n1p = std::fabs((-qI.a01 * n2 + qI.a11 * n1) / qF);
+    real n2p = std::fabs((qI.a00 * n2 - qI.a10 * n1) / qF);
+    real bcd = P.isGaussian ? n1p + n2p : rmax(n1p, n2p);
+    real BW = P.isGaussian ? RL(4.0) : RL(1.0);
+    if(bcd > BW * RL(1.0)) return;
+    cpx<real> delay = d.tau0 + rmul(s, d.dtau);
+    real cnst = d.A0 / std::sqrt(std::fabs(qF));
+    real w;
+    if(P.isGaussian) {
+        if constexpr(FAST) w = (real)expf((float)(FL(-0.5) * (n1p * n1p + n2p * n2p)));
+        else w = std::exp(FL(-0.5) * (n1p * n1p + n2p * n2p));
+    } else {
+        w = (FL(1.0) - n1p) * (FL(1.0) - n2p);
+    }
+    real phaseInt = d.phase0;
+    if((qF <= RL(0.0) && d.qOld > RL(0.0)) || (qF >= RL(0.0) && d.qOld < RL(0.0)))
+        phaseInt = phaseInt + RC<real>::PI / FL(2.0);
+    size_t idx = (((((size_t)d.isz * P.NSx + d.isx) * P.NSy + d.isy) * P.Ntheta + itheta) * P.NRz + iz) * P.NRr + ir;
+    real omega = FL(2.0) * RC<real>::PI * P.freq0;
+    float fr, fi;
+    if(P.run0 == 'C') {
+        cpx<real> arg = rmul(omega, delay);
+        arg.re = arg.re - phaseInt;
+        if constexpr(FAST) {
+            float sn, cs;
+            sincos_phase((double)arg.re, sn, cs);
+            float m = (float)(cnst * w) * expf((float)arg.im);
+            fr = m * cs;
+            fi = -(m * sn);
+        } else {
+            cpx<real> ex = cexp_(C(arg.im, -arg.re));
+            cpx<real> dd = rmul(cnst * w, ex);
+            fr = (float)dd.re;
+            fi = (float)dd.im;
+        }
+    } else {
+        real v;
+        if constexpr(FAST) v = cnst * (real)expf((float)(omega * delay.im));
+        else v = cnst * std::exp(omega * delay.im);
+        v = (v * v) * w;
+        if(P.isGaussian) v *= FL(2.0) * RC<real>::PI;
+        fr = (float)v;
+        fi = 0.0f;
+    }
+    sink.add3(idx, fr, fi);
+    sink.count();
+}
+
+} // namespace bhc3d

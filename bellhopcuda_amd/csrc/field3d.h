@@ -1,0 +1,398 @@
+// field3d.h -- kernels and launch layer of the full-3D TL path. Included at
+// the end of trace2d.hip: it shares the device context, the segment buffer
+// and the segment-block allocator of the 2D path.
+//
+// Same split as in 2D: k_trace3d runs one ray per lane (persistent lanes
+// refilled from a device work counter) and reduces every step to a Seg3D
+// descriptor; k_scatter3d gives each segment one wavefront and flattens its
+// (range, bearing) rows' depth windows over the 64 lanes. Replaces
+// RunFieldModesImpl<CFG, true, true> (src/mode/fieldimpl.hpp:27-28).
+#include "core3d.h"
+#include "pack3d.h"
+
+namespace {
+
+using namespace bhc3d;
+
+template<typename real> struct alignas(16) Seg3 {
+    Seg3D<real> d;
+};
+
+struct DevOut3 {
+    int32_t *stats;             // [njobs][3] steps, top / bottom bounces (or null)
+    double *xend;               // [njobs][3]
+    unsigned long long *counters; // [0] steps, [1] contributions
+    uint32_t *errw;
+    int32_t *work;
+    int32_t job_begin, job_end;
+    void *segs;
+    uint32_t *seg_count;
+    uint32_t seg_cap;
+    uint32_t *overflow;
+};
+
+struct Sink3 {
+    float *field;
+    unsigned long long nc;
+    __device__ void add3(size_t idx, float re, float im)
+    {
+        unsafeAtomicAdd(field + 2 * idx, re);
+        unsafeAtomicAdd(field + 2 * idx + 1, im);
+    }
+    __device__ void count() { ++nc; }
+    __device__ void mark() {}
+};
+
+template<typename real> __device__ inline void seal3(const SegBlock &B, int lane, const DevOut3 &O)
+{
+    Seg3<real> *segs = reinterpret_cast<Seg3<real> *>(O.segs);
+    for(uint32_t k = B.used + (uint32_t)lane; k < B.cap; k += WAVE) {
+        uint32_t slot = B.base + k;
+        if(slot < O.seg_cap) segs[slot].d.nitems = 0;
+    }
+}
+
+template<typename real>
+__device__ inline void append3(const Seg3D<real> &d, int lane, const DevOut3 &O, SegBlock &B)
+{
+    bool has = d.nitems > 0;
+    unsigned long long m = __ballot(has);
+    if(!m) return;
+    uint32_t cnt = (uint32_t)__popcll(m);
+    if(B.used + cnt > B.cap) {
+        seal3<real>(B, lane, O);
+        uint32_t base = 0;
+        if(lane == 0) base = atomicAdd(O.seg_count, SEG_BLOCK);
+        B.base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+        B.used = 0;
+        B.cap = SEG_BLOCK;
+    }
+    if(has) {
+        uint32_t slot = B.base + B.used + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if(slot < O.seg_cap) reinterpret_cast<Seg3<real> *>(O.segs)[slot].d = d;
+        else atomicOr(O.overflow, 1u);
+    }
+    B.used += cnt;
+}
+
+// Trace kernel: the ray loop of MainFieldModes (src/trace.hpp:525-589), 3D.
+template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<real> P, DevOut3 O)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int njobs = O.job_end - O.job_begin;
+    Err e = {0, 0, 0, 0};
+    Ray3<real> p0;
+    Trace3<real> S;
+    Infl3<real> I;
+    int job = 0, ia = 0, ib = 0, isx = 0, isy = 0, isz = 0, is = 0;
+    bool active = false, exhausted = false;
+    unsigned long long steps = 0;
+    SegBlock blk{0u, 0u, 0u};
+    for(int iter = 0;; ++iter) {
+        if(!exhausted) {
+            unsigned long long need = __ballot(!active);
+            if(need) {
+                int cnt = __popcll(need);
+                int leader = __ffsll((unsigned long long)need) - 1;
+                int base = 0;
+                if(lane == leader) base = atomicAdd(O.work, cnt);
+                base = __shfl(base, leader);
+                if(base + cnt >= njobs) exhausted = true;
+                if(!active) {
+                    int my = base + __popcll(need & ((1ull << lane) - 1ull));
+                    if(my < njobs) {
+                        job = O.job_begin + my;
+                        JobIndices3(job, P, ia, ib, isy, isx, isz);
+                        is = 0;
+                        active = RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e);
+                        if(!active && O.stats) {
+                            O.stats[3 * job] = O.stats[3 * job + 1] = O.stats[3 * job + 2] = 0;
+                            O.xend[3 * job] = O.xend[3 * job + 1] = O.xend[3 * job + 2] = 0.0;
+                        }
+                    }
+                }
+            }
+        }
+        if(!__any(active)) {
+            if(exhausted) break;
+            continue;
+        }
+        if((iter & 31) == 31) {
+            // HasErrored (src/util/errors.hpp:48-51) and a full segment buffer
+            // end the pass for the whole wavefront
+            if(e.err) atomicOr(O.errw + 0, e.err);
+            uint32_t ge = __hip_atomic_load(O.errw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t of = __hip_atomic_load(O.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if(__any(ge != 0u || e.err != 0u || of != 0u)) {
+                exhausted = true;
+                active = false;
+                continue;
+            }
+        }
+        Ray3<real> p1, p2;
+        Seg3D<real> d;
+        d.nitems = 0;
+        bool two = false;
+        if(active) {
+            two = RayUpdate3(p0, p1, p2, S, P, e);
+            prelude3(d, p0, p1, I, is, P, isz, isx, isy);
+            ++is;
+        }
+        append3(d, lane, O, blk);
+        if(__any(two)) {
+            d.nitems = 0;
+            if(active && two) {
+                prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                ++is;
+            }
+            append3(d, lane, O, blk);
+        }
+        if(active) {
+            p0 = two ? p2 : p1;
+            if(RayTerminate3(p0, is, S, P, e)) {
+                active = false;
+                steps += (unsigned long long)is;
+                if(O.stats) {
+                    O.stats[3 * job] = is;
+                    O.stats[3 * job + 1] = p0.NumTopBnc;
+                    O.stats[3 * job + 2] = p0.NumBotBnc;
+                    O.xend[3 * job] = (double)p0.x.x;
+                    O.xend[3 * job + 1] = (double)p0.x.y;
+                    O.xend[3 * job + 2] = (double)p0.x.z;
+                }
+            }
+        }
+    }
+    seal3<real>(blk, lane, O);
+    if(e.err) atomicOr(O.errw + 0, e.err);
+    if(e.warn) atomicOr(O.errw + 1, e.warn);
+    if(e.nerr) atomicAdd(O.errw + 2, e.nerr);
+    if(e.nwarn) atomicAdd(O.errw + 3, e.nwarn);
+    for(int off = WAVE / 2; off > 0; off >>= 1) steps += __shfl_xor(steps, off);
+    if(lane == 0) atomicAdd(O.counters + 0, steps);
+}
+
+// Scatter kernel: one wavefront per segment. Rows are the (range, bearing)
+// pairs the segment's receiver walk visits (influence.hpp:1298-1386, 3D);
+// each row's depth window is flattened over the lanes.
+template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
+    const int nwaves = gridDim.x * (blockDim.x / WAVE);
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    Sink3 sink{P.field, 0};
+    const Seg3<real> *segs = reinterpret_cast<const Seg3<real> *>(O.segs);
+    for(uint32_t si = (uint32_t)wave; si < n; si += (uint32_t)nwaves) {
+        const Seg3D<real> d = segs[si].d;
+        if(d.nitems <= 0) continue;
+        const int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        const int nrow = nr * P.Ntheta;
+        for(int g = 0; g < nrow; g += WAVE) {
+            const int row = g + lane;
+            int ir = 0, it = 0, i0 = 0, i1 = -1;
+            real xr = 0, yr = 0, zmin = 0, zmax = 0;
+            if(row < nrow) {
+                int j = row / P.Ntheta;
+                it = row - j * P.Ntheta;
+                ir = d.ir0 + j * d.dir;
+                if(row3(d, ir, it, P, xr, yr, zmin, zmax)) {
+                    if(P.rz_sorted && zmin == zmin && zmax == zmax) {
+                        depth_window(P.Rz, P.NRz, zmin, zmax, i0, i1);
+                    } else {
+                        i0 = 0;
+                        i1 = P.NRz - 1;
+                    }
+                }
+            }
+            const int cnt = i1 >= i0 ? i1 - i0 + 1 : 0;
+            int excl;
+            const int total = wave_excl_scan(cnt, lane, excl);
+            for(int t0 = 0; t0 < total; t0 += WAVE) {
+                const int t = t0 + lane;
+                const int own = wave_owner(excl, t);
+                const int oir = __shfl(ir, own), oit = __shfl(it, own), oi0 = __shfl(i0, own);
+                const int oexcl = __shfl(excl, own);
+                const real oxr = __shfl(xr, own), oyr = __shfl(yr, own);
+                const real ozmin = __shfl(zmin, own), ozmax = __shfl(zmax, own);
+                if(t < total) item3<real, Sink3, FAST>(d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, sink);
+            }
+        }
+    }
+    unsigned long long nc = sink.nc;
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
+size_t seg3_size(int precision) { return precision == 64 ? sizeof(Seg3<double>) : sizeof(Seg3<float>); }
+
+template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_out *out)
+{
+    std::vector<real> blob;
+    Prob3D<real> P;
+    std::string perr;
+    if(pack3<real>(p, blob, P, perr)) return fail("%s", perr.c_str());
+    long long nj64 = 1;
+    nj64 *= (p->iSingle_alpha >= 1 ? 1 : p->Nalpha);
+    nj64 *= (p->iSingle_beta >= 1 ? 1 : p->Nbeta);
+    nj64 *= (long long)p->NSx * p->NSy * p->NSz;
+    if(nj64 >= (1ll << 30)) return fail("3D problem: too many rays");
+    const int njobs = (int)nj64;
+    const size_t nrcv = (size_t)p->NSz * p->NSx * p->NSy * p->Ntheta * p->NRz * p->NRr;
+    hipStream_t s = c->stream;
+    real *d_blob = nullptr;
+    float *d_Rz = nullptr, *d_Sz = nullptr, *d_field = nullptr;
+    int32_t *d_stats = nullptr, *d_work = nullptr;
+    double *d_xend = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    uint32_t *d_errw = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);
+        void *ptrs[] = {d_blob, d_Rz, d_Sz, d_field, d_stats, d_work, d_xend, d_cnt, d_errw};
+        for(void *q : ptrs)
+            if(q) (void)hipFree(q);
+    };
+    int rc = 0;
+#define TRY(x) \
+    do { \
+        if((x) != 0) { \
+            cleanup(); \
+            return -1; \
+        } \
+    } while(0)
+#define HTRY(x) \
+    do { \
+        hipError_t _e = (x); \
+        if(_e != hipSuccess) { \
+            fail("%s failed: %s", #x, hipGetErrorString(_e)); \
+            cleanup(); \
+            return -1; \
+        } \
+    } while(0)
+    TRY(dalloc(&d_blob, blob.size()));
+    TRY(dalloc(&d_Rz, (size_t)p->NRz));
+    TRY(dalloc(&d_Sz, (size_t)p->NSz));
+    TRY(dalloc(&d_field, 2 * nrcv));
+    TRY(dalloc(&d_work, 1));
+    TRY(dalloc(&d_cnt, 2));
+    TRY(dalloc(&d_errw, 4));
+    if(out->ray_nsteps) {
+        TRY(dalloc(&d_stats, 3 * (size_t)njobs));
+        TRY(dalloc(&d_xend, 3 * (size_t)njobs));
+    }
+    HTRY(hipMemcpyAsync(d_blob, blob.data(), blob.size() * sizeof(real), hipMemcpyHostToDevice, s));
+    HTRY(hipMemcpyAsync(d_Rz, p->Rz, p->NRz * sizeof(float), hipMemcpyHostToDevice, s));
+    HTRY(hipMemcpyAsync(d_Sz, p->Sz, p->NSz * sizeof(float), hipMemcpyHostToDevice, s));
+    HTRY(hipMemsetAsync(d_field, 0, 2 * nrcv * sizeof(float), s));
+    HTRY(hipMemsetAsync(d_cnt, 0, 2 * sizeof(unsigned long long), s));
+    HTRY(hipMemsetAsync(d_errw, 0, 4 * sizeof(uint32_t), s));
+    P.base = d_blob;
+    P.s.base = d_blob;
+    P.Rz = d_Rz;
+    P.Sz = d_Sz;
+    P.field = d_field;
+    P.njobs = njobs;
+    TRY(ensure_segs(c));
+    DevOut3 O;
+    memset(&O, 0, sizeof(O));
+    O.stats = d_stats;
+    O.xend = d_xend;
+    O.counters = d_cnt;
+    O.errw = d_errw;
+    O.work = d_work;
+    O.segs = c->segs;
+    O.seg_count = c->d_seg_count;
+    O.overflow = c->d_overflow;
+    O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg3<real>), 0xFFFFFFF0u);
+    std::vector<std::pair<int, int>> todo{{0, njobs}};
+    c->passes = 0;
+    bool first = true;
+    while(!todo.empty()) {
+        auto r = todo.back();
+        todo.pop_back();
+        O.job_begin = r.first;
+        O.job_end = r.second;
+        HTRY(hipMemsetAsync(c->d_seg_count, 0, sizeof(uint32_t), s));
+        HTRY(hipMemsetAsync(c->d_overflow, 0, sizeof(uint32_t), s));
+        HTRY(hipMemsetAsync(d_work, 0, sizeof(int32_t), s));
+        int nj = r.second - r.first;
+        int waves = (nj + WAVE - 1) / WAVE;
+        int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
+        if(first) HTRY(hipEventRecord(c->ev0, s));
+        hipLaunchKernelGGL((k_trace3d<real>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
+        HTRY(hipGetLastError());
+        uint32_t of = 0;
+        HTRY(hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HTRY(hipStreamSynchronize(s));
+        if(of) {
+            if(nj <= 1) {
+                fail("segment buffer too small for a single 3D ray (raise BHC_SEG_BUFFER_MB)");
+                cleanup();
+                return -1;
+            }
+            int mid = r.first + nj / 2;
+            todo.push_back({mid, r.second});
+            todo.push_back({r.first, mid});
+            HTRY(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), s));
+            continue;
+        }
+        if(first) HTRY(hipEventRecord(c->evm, s));
+        int sblocks = c->num_cus * 8;
+        if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d<real, false>), dim3(sblocks), dim3(256), 0, s, P, O);
+        else hipLaunchKernelGGL((k_scatter3d<real, true>), dim3(sblocks), dim3(256), 0, s, P, O);
+        HTRY(hipGetLastError());
+        ++c->passes;
+        first = false;
+    }
+    HTRY(hipEventRecord(c->ev1, s));
+    if(out->field) HTRY(hipMemcpyAsync(out->field, d_field, 2 * nrcv * sizeof(float), hipMemcpyDeviceToHost, s));
+    uint32_t ew[4];
+    unsigned long long cnt[2];
+    HTRY(hipMemcpyAsync(ew, d_errw, sizeof(ew), hipMemcpyDeviceToHost, s));
+    HTRY(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> st;
+    if(d_stats) {
+        st.resize(3 * (size_t)njobs);
+        HTRY(hipMemcpyAsync(st.data(), d_stats, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if(out->ray_xend)
+            HTRY(hipMemcpyAsync(out->ray_xend, d_xend, 3 * (size_t)njobs * sizeof(double), hipMemcpyDeviceToHost,
+                                s));
+    }
+    HTRY(hipStreamSynchronize(s));
+    out->err = ew[0];
+    out->warn = ew[1];
+    out->err_count = ew[2];
+    out->warn_count = ew[3];
+    out->total_steps = (int64_t)cnt[0];
+    out->total_contrib = (int64_t)cnt[1];
+    if(d_stats)
+        for(int j = 0; j < njobs; ++j) {
+            out->ray_nsteps[j] = st[3 * j];
+            if(out->ray_ntop) out->ray_ntop[j] = st[3 * j + 1];
+            if(out->ray_nbot) out->ray_nbot[j] = st[3 * j + 2];
+        }
+    cleanup();
+#undef TRY
+#undef HTRY
+    return rc;
+}
+
+} // namespace
+
+extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out)
+{
+    if(!ctx || !prob || !out) return fail("bhc_field_run3d: null argument");
+    if(prob->precision != 64 && prob->precision != 32) return fail("bhc_field_run3d: precision must be 32 or 64");
+    char r0 = prob->RunType[0], r1 = prob->RunType[1];
+    if(r0 != 'C' && r0 != 'S' && r0 != 'I') return fail("bhc_field_run3d: run type '%c' is not a TL run type", r0);
+    if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B')
+        return fail("bhc_field_run3d: beam type '%c' is not a Cartesian geometric beam", r1);
+    if(prob->NSx < 1 || prob->NSy < 1 || prob->NSz < 1 || prob->NRz < 1 || prob->NRr < 1 || prob->Ntheta < 1
+       || prob->Nalpha < 1 || prob->Nbeta < 1)
+        return fail("bhc_field_run3d: empty source / receiver / angle set");
+    if(prob->NSBPPts < 2) return fail("bhc_field_run3d: source beam pattern needs 2 points");
+    if(!(prob->deltas > 0)) return fail("bhc_field_run3d: deltas must be positive");
+    if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_field_run3d: hipSetDevice failed");
+    return prob->precision == 64 ? run3d<double>(ctx, prob, out) : run3d<float>(ctx, prob, out);
+}

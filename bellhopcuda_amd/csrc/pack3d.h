@@ -1,0 +1,128 @@
+// pack3d.h -- packs a preprocessed 3D problem (bhc_problem3d, doubles) into
+// the 3D tracer's blob in the run's real type and fills the device view.
+#pragma once
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bhc_hip.h"
+#include "core3d.h"
+
+namespace bhc3d {
+
+// Blob layout: [depth table z, rho][depth-only profile c, cz, n2, n2z, coef, or
+// the hexahedral x, y, c, cz][bty nodes][ati nodes][Rr][trc][brc][sbp]
+// [alpha, cos, sin][beta, cos, sin][Sx][Sy][bearing cos / sin].
+template<typename real>
+int pack3(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &P, std::string &err)
+{
+    auto fail = [&](const char *m) {
+        err = m;
+        return -1;
+    };
+    memset(&P, 0, sizeof(P));
+    auto put = [&](const double *src, size_t n) {
+        int32_t off = (int32_t)blob.size();
+        for(size_t i = 0; i < n; ++i) blob.push_back(src ? (real)src[i] : (real)0);
+        return off;
+    };
+    const bool hex = p->ssp_type == 'H';
+    int N = p->ssp_npts;
+    if(N < 2) return fail("3D problem: SSP needs at least 2 depths");
+    P.ssp3 = p->ssp_type;
+    P.nz3 = N;
+    P.o_z3 = put(p->ssp_z, N);
+    P.o_rho3 = put(p->ssp_rho, N);
+    Prob2D<real> &s = P.s;
+    s.ssp_type = hex ? 'C' : p->ssp_type;
+    s.npts = N;
+    s.o_z = P.o_z3;
+    s.o_rho = P.o_rho3;
+    if(!hex) {
+        s.o_c = put(p->ssp_c, 2 * (size_t)N);
+        s.o_cz = put(p->ssp_cz, 2 * (size_t)N);
+        if(p->ssp_type == 'N') {
+            s.o_n2 = put(p->ssp_n2, 2 * (size_t)N);
+            s.o_n2z = put(p->ssp_n2z, 2 * (size_t)N);
+        }
+        if(p->ssp_type == 'S' || p->ssp_type == 'P') s.o_coef = put(p->ssp_coef, 8 * (size_t)N);
+    } else {
+        P.hx = p->hex_nx;
+        P.hy = p->hex_ny;
+        if(p->hex_nz != N) return fail("3D problem: hexahedral depth count mismatch");
+        P.o_hx = put(p->hex_x, P.hx);
+        P.o_hy = put(p->hex_y, P.hy);
+        P.o_hc = put(p->hex_c, (size_t)P.hx * P.hy * N);
+        P.o_hcz = put(p->hex_cz, (size_t)P.hx * P.hy * (N - 1));
+    }
+    auto put_nodes = [&](const bhc_bdry_node3d *nd, int n) {
+        int32_t off = (int32_t)blob.size();
+        for(int i = 0; i < n; ++i) { // NODE3 reals per node: x, n1, n2
+            for(int k = 0; k < 3; ++k) blob.push_back((real)nd[i].x[k]);
+            for(int k = 0; k < 3; ++k) blob.push_back((real)nd[i].n1[k]);
+            for(int k = 0; k < 3; ++k) blob.push_back((real)nd[i].n2[k]);
+        }
+        return off;
+    };
+    if(p->ati_nx < 2 || p->ati_ny < 2 || p->bty_nx < 2 || p->bty_ny < 2) return fail("3D problem: boundary grids need 2 x 2 nodes");
+    P.bty_nx = p->bty_nx;
+    P.bty_ny = p->bty_ny;
+    P.ati_nx = p->ati_nx;
+    P.ati_ny = p->ati_ny;
+    P.o_bty = put_nodes(p->bty, p->bty_nx * p->bty_ny);
+    P.o_ati = put_nodes(p->ati, p->ati_nx * p->ati_ny);
+    P.o_Rr = put(p->Rr, p->NRr);
+    P.o_trc = put(p->trc, 3 * (size_t)p->trc_npts);
+    P.o_brc = put(p->brc, 3 * (size_t)p->brc_npts);
+    P.ntrc = p->trc_npts;
+    P.nbrc = p->brc_npts;
+    P.o_sbp = put(p->sbp, 2 * (size_t)p->NSBPPts);
+    P.nsbp = p->NSBPPts;
+    P.o_alpha = put(p->alpha, p->Nalpha);
+    P.o_acos = put(p->alpha_cos, p->Nalpha);
+    P.o_asin = put(p->alpha_sin, p->Nalpha);
+    P.o_beta = put(p->beta, p->Nbeta);
+    P.o_bcos = put(p->beta_cos, p->Nbeta);
+    P.o_bsin = put(p->beta_sin, p->Nbeta);
+    P.o_Sx = put(p->Sx, p->NSx);
+    P.o_Sy = put(p->Sy, p->NSy);
+    P.o_trcv = put(p->t_rcvr, 2 * (size_t)p->Ntheta);
+    if(blob.size() >= (size_t)0x7FFFFFFF) return fail("3D problem too large for 32-bit offsets");
+    P.run0 = p->RunType[0];
+    P.run1 = p->RunType[1];
+    P.NSx = p->NSx;
+    P.NSy = p->NSy;
+    P.NSz = p->NSz;
+    P.NRz = p->NRz;
+    P.NRr = p->NRr;
+    P.Ntheta = p->Ntheta;
+    P.Nalpha = p->Nalpha;
+    P.iSingleA = p->iSingle_alpha;
+    P.Nbeta = p->Nbeta;
+    P.iSingleB = p->iSingle_beta;
+    P.Dalpha = (real)p->Dalpha;
+    P.Dbeta = (real)p->Dbeta;
+    P.freq0 = (real)p->freq0;
+    P.deltas = (real)p->deltas;
+    P.box_x = (real)p->box_x;
+    P.box_y = (real)p->box_y;
+    P.box_z = (real)p->box_z;
+    auto hs = [](const bhc_hs2d &h) {
+        HSd<real> r;
+        r.cP = C((real)h.cP[0], (real)h.cP[1]);
+        r.cS = C((real)h.cS[0], (real)h.cS[1]);
+        r.rho = (real)h.rho;
+        r.bc = h.bc;
+        return r;
+    };
+    P.top = hs(p->top_hs);
+    P.bot = hs(p->bot_hs);
+    P.rz_sorted = 1;
+    for(int i = 1; i < p->NRz; ++i)
+        if(!(p->Rz[i] > p->Rz[i - 1])) P.rz_sorted = 0;
+    P.isGaussian = p->RunType[1] == 'B';
+    return 0;
+}
+
+
+} // namespace bhc3d

@@ -2239,3 +2239,5 @@ extern "C" int bhc_sweep_download(bhc_ctx *ctx, bhc_sweep *sw, int iprob, bhc_fi
     }
     return 0;
 }
+
+#include "field3d.h"

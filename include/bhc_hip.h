@@ -455,6 +455,7 @@ typedef struct bhc_problem3d_owner bhc_problem3d_owner;
 int bhc_preprocess3d(const bhc_env3d *env, bhc_problem3d_owner **out);
 const bhc_problem3d *bhc_problem3d_view(const bhc_problem3d_owner *p);
 void bhc_problem3d_free(bhc_problem3d_owner *p);
+const char *bhc_host3d_last_error(void);
 
 /* 3D field run, RunFieldModesImpl<CFG, true, true> (src/mode/fieldimpl.hpp:27-28):
  * out->field is [NSz][NSx][NSy][Ntheta][NRz][NRr] complex floats, caller-zeroed

@@ -154,3 +154,113 @@ extern "C" int core_ray_run2d(const bhc_problem2d *prob, int isz, int ia, int ma
     return prob->precision == 64 ? run_ray<double>(prob, isz, ia, maxp, pts, nsteps)
                                   : run_ray<float>(prob, isz, ia, maxp, pts, nsteps);
 }
+
+// ---------------------------------------------------------------------------
+// 3D: the product's 3D core (core3d.h + pack3d.h) run serially on the CPU,
+// with each segment's rows and depth windows enumerated as the scatter kernel
+// enumerates them.
+// ---------------------------------------------------------------------------
+#include "../../bellhopcuda_amd/csrc/core3d.h"
+#include "../../bellhopcuda_amd/csrc/pack3d.h"
+
+namespace {
+struct HostSink3 {
+    float *field;
+    long long nc = 0;
+    void add3(size_t idx, float re, float im)
+    {
+        field[2 * idx] += re;
+        field[2 * idx + 1] += im;
+    }
+    void count() { ++nc; }
+    void mark() {}
+};
+
+template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
+{
+    using namespace bhc3d;
+    std::vector<real> blob;
+    Prob3D<real> P;
+    std::string err;
+    if(pack3<real>(prob, blob, P, err)) {
+        fprintf(stderr, "%s\n", err.c_str());
+        return -1;
+    }
+    P.base = blob.data();
+    P.s.base = blob.data();
+    P.Rz = prob->Rz;
+    P.Sz = prob->Sz;
+    HostSink3 sink{out->field};
+    Err e = {0, 0, 0, 0};
+    int njobs = (prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta)
+        * prob->NSx * prob->NSy * prob->NSz;
+    long long steps = 0;
+    auto scatter = [&](const Seg3D<real> &d) {
+        if(d.nitems <= 0) return;
+        int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        for(int j = 0; j < nr; ++j)
+            for(int it = 0; it < P.Ntheta; ++it) {
+                int ir = d.ir0 + j * d.dir, i0 = 0, i1 = -1;
+                real xr, yr, zmin, zmax;
+                if(!row3(d, ir, it, P, xr, yr, zmin, zmax)) continue;
+                if(P.rz_sorted && zmin == zmin && zmax == zmax) depth_window(P.Rz, P.NRz, zmin, zmax, i0, i1);
+                else {
+                    i0 = 0;
+                    i1 = P.NRz - 1;
+                }
+                for(int iz = i0; iz <= i1; ++iz) item3<real, HostSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, sink);
+            }
+    };
+    for(int job = 0; job < njobs; ++job) {
+        int ia, ib, isx, isy, isz;
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
+        Ray3<real> p0, p1, p2;
+        Trace3<real> S;
+        Infl3<real> I;
+        if(!RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e)) {
+            if(out->ray_nsteps) {
+                out->ray_nsteps[job] = out->ray_ntop[job] = out->ray_nbot[job] = 0;
+                for(int k = 0; k < 3; ++k) out->ray_xend[3 * job + k] = 0;
+            }
+            continue;
+        }
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            bool two = RayUpdate3(p0, p1, p2, S, P, e);
+            Seg3D<real> d;
+            prelude3(d, p0, p1, I, is, P, isz, isx, isy);
+            scatter(d);
+            ++is;
+            if(two) {
+                prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                scatter(d);
+                ++is;
+            }
+            p0 = two ? p2 : p1;
+            if(RayTerminate3(p0, is, S, P, e)) break;
+        }
+        steps += is;
+        if(out->ray_nsteps) {
+            out->ray_nsteps[job] = is;
+            out->ray_ntop[job] = p0.NumTopBnc;
+            out->ray_nbot[job] = p0.NumBotBnc;
+            out->ray_xend[3 * job] = (double)p0.x.x;
+            out->ray_xend[3 * job + 1] = (double)p0.x.y;
+            out->ray_xend[3 * job + 2] = (double)p0.x.z;
+        }
+    }
+    out->total_steps = steps;
+    out->total_contrib = sink.nc;
+    out->err = e.err;
+    out->warn = e.warn;
+    out->err_count = e.nerr;
+    out->warn_count = e.nwarn;
+    return 0;
+}
+} // namespace
+
+extern "C" int core_field_run3d(const bhc_problem3d *prob, bhc_field_out *out)
+{
+    return prob->precision == 64 ? run3<double>(prob, out) : run3<float>(prob, out);
+}

@@ -1,0 +1,208 @@
+"""The 3D oracle and the product's 3D core on the CPU.
+
+What pins the 3D oracle (oracle/bellhop3d.c):
+  * free space (isovelocity, absorbing halfspaces above and below): coherent
+    hat beams and incoherent Gaussian beams reproduce spherical spreading,
+    TL = 20 log10 R, to a fraction of a dB -- this fixes the 3D amplitude
+    (Ratio1 with sqrt(Dalpha Dbeta) / c, rcp_q0, rcp_qhat0), the q-matrix
+    evolution, the beam coordinates n1', n2' and the Gaussian scale 2 pi;
+  * on a depth-only Munk environment the 3D field on a bearing equals the
+    independently pinned 2D oracle's field (the 3D ray equations reduce to
+    2D ones, reflections included) -- this fixes steps, boundary crossings,
+    the fluid-halfspace reflection and the coherent phase;
+  * a 3D bathymetry grid that is flat reproduces the flat default boundary;
+  * a hexahedral grid with no lateral variation reproduces the same field as
+    the equivalent c-linear depth profile.
+The product's 3D core (core3d.h, run serially by tests/harness/core_cpu.cpp
+with the scatter kernel's row / depth-window enumeration) is compared with
+the oracle bit for bit on trajectories and contribution counts. Its
+preprocessing (host3d.cpp) is compared value for value.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import bellhopcuda_amd as bh
+from bellhopcuda_amd import _abi
+from bellhopcuda_amd.env import munk_env, munk_c
+from bellhopcuda_amd.env3d import free_env3d, munk_env3d, c3_env, Grid3D
+import oracle
+from parity import tl_db
+
+_H = None
+
+
+def harness():
+    global _H
+    if _H is None:
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "harness", "_core_cpu.so")
+        _H = C.CDLL(path)
+        _H.core_field_run3d.restype = C.c_int
+        _H.core_field_run3d.argtypes = [C.POINTER(_abi.Problem3D), C.POINTER(_abi.FieldOut)]
+    return _H
+
+
+class _Res:
+    pass
+
+
+def core_run3d(env):
+    p = bh.Problem3D(env)
+    n = p.nrays
+    r = _Res()
+    r.field = np.zeros(p.field_shape, np.complex64)
+    r.nsteps, r.ntop, r.nbot = (np.zeros(n, np.int32) for _ in range(3))
+    r.xend = np.zeros((n, 3))
+    out = _abi.FieldOut()
+    out.field = r.field.ctypes.data_as(_abi.c_float_p)
+    out.ray_nsteps = r.nsteps.ctypes.data_as(_abi.c_int32_p)
+    out.ray_ntop = r.ntop.ctypes.data_as(_abi.c_int32_p)
+    out.ray_nbot = r.nbot.ctypes.data_as(_abi.c_int32_p)
+    out.ray_xend = r.xend.ctypes.data_as(_abi.c_double_p)
+    assert harness().core_field_run3d(C.byref(p.view), C.byref(out)) == 0
+    r.total_contrib, r.total_steps, r.err, r.warn = out.total_contrib, out.total_steps, out.err, out.warn
+    return r
+
+
+def _free_tl_error(env, f):
+    Rz = env.Rz.astype(np.float64)
+    Rr = env.Rr_km * 1000.0
+    R = np.sqrt((Rz[:, None] - float(env.Sz[0])) ** 2 + Rr[None, :] ** 2)
+    m = R > 500.0
+    return tl_db(f)[:, m] - 20.0 * np.log10(R[m])[None, :]
+
+
+@pytest.mark.parametrize("run_type,tol_med,tol_p95", [("CG   3", 0.05, 0.2), ("IB   3", 0.1, 0.25)])
+def test_free_space_spherical_spreading(run_type, tol_med, tol_p95):
+    env = free_env3d(nalpha=181, nbeta=72, run_type=run_type)
+    o = oracle.field_run3d(env)
+    f = oracle.scale_pressure3d(env, o.field)[0, 0, 0]   # [Ntheta, NRz, NRr]
+    d = _free_tl_error(env, f)
+    assert o.err == 0
+    assert abs(np.median(d)) < tol_med, np.median(d)
+    assert np.percentile(np.abs(d), 95) < tol_p95, np.percentile(np.abs(d), 95)
+    # the medium is axially symmetric: every bearing sees the same field
+    assert np.allclose(tl_db(f[0]), tl_db(f[-1]), atol=0.05)
+
+
+@pytest.mark.parametrize("rt3,rt2,tol_med", [("CG   3", "CG", 0.1), ("IB   3", "IB", 0.15)])
+def test_depth_only_munk_matches_2d(rt3, rt2, tol_med):
+    e3 = munk_env3d(nalpha=801, nbeta=41, beta=(-4.0, 4.0), nrz=51, nrr=201, run_type=rt3)
+    e2 = munk_env(nbeams=801, nrz=51, nrr=201, run_type=rt2, r_max_km=20.0)
+    o3 = oracle.field_run3d(e3)
+    o2 = oracle.field_run(e2)
+    t3 = tl_db(oracle.scale_pressure3d(e3, o3.field)[0, 0, 0, 0])
+    t2 = tl_db(oracle.scale_pressure(e2, o2.field)[0])
+    m = (t2 < 90) & (t3 < 90)
+    m[:, :10] = False
+    d = np.abs(t3 - t2)[m]
+    assert m.sum() > 3000
+    assert np.median(d) < tol_med, np.median(d)
+    # the in-plane ray (beta = 0) takes the 2D ray's steps and bounces
+    r3 = o3.nsteps.reshape(41, 801)[20]
+    r2 = o2.nsteps
+    assert np.mean(r3 == r2) > 0.99
+
+
+def test_flat_grid_equals_default_boundary():
+    base = munk_env3d(nalpha=101, nbeta=5, nrz=21, nrr=41)
+    grid = munk_env3d(nalpha=101, nbeta=5, nrz=21, nrr=41)
+    x = np.linspace(-25.0, 25.0, 11)
+    grid.bty = Grid3D(x_km=x, y_km=x, z=np.full((11, 11), 5000.0))
+    a, b = oracle.field_run3d(base), oracle.field_run3d(grid)
+    # the grid's cell edges and diagonals add step ends (TopBotSegCrossing,
+    # TriDiagCrossing), so step counts grow; bounces and the field do not change
+    assert np.all(b.nsteps >= a.nsteps)
+    assert np.array_equal(a.nbot, b.nbot) and np.array_equal(a.ntop, b.ntop)
+    ta, tb = tl_db(a.field), tl_db(b.field)
+    m = ta < 100
+    assert np.median(np.abs(ta - tb)[m]) < 0.05
+
+
+def test_laterally_uniform_hexahedral_equals_clinear():
+    e1 = munk_env3d(nalpha=101, nbeta=5, nrz=21, nrr=41, bot="V")
+    e2 = munk_env3d(nalpha=101, nbeta=5, nrz=21, nrr=41, bot="V")
+    z = e1.ssp_z
+    e2.ssp_type = "H"
+    e2.hex_x_km = np.array([-30.0, 30.0])
+    e2.hex_y_km = np.array([-30.0, 30.0])
+    e2.hex_z = z
+    e2.hex_c = np.broadcast_to(munk_c(z)[None, None, :], (2, 2, len(z))).copy()
+    a, b = oracle.field_run3d(e1), oracle.field_run3d(e2)
+    assert np.mean(a.nsteps == b.nsteps) > 0.99
+    ta, tb = tl_db(a.field), tl_db(b.field)
+    m = ta < 100
+    assert np.median(np.abs(ta - tb)[m]) < 0.01
+
+
+CORE_CASES = {
+    "free_gauss": lambda: free_env3d(),
+    "free_hat": lambda: free_env3d(run_type="CG   3"),
+    "free_incoherent_f32": lambda: free_env3d(run_type="IB   3", precision=32),
+    "munk_hat": lambda: munk_env3d(nalpha=101, nbeta=5, run_type="CG   3"),
+    "munk_semicoherent_pchip": lambda: munk_env3d(nalpha=101, nbeta=5, run_type="SB   3", ssp_type="P"),
+    "munk_n2linear_bearings": lambda: munk_env3d(nalpha=61, nbeta=9, beta=(-40.0, 40.0), ntheta=8,
+                                                 ssp_type="N"),
+    "shelf_canyon_hexahedral": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=41),
+    "shelf_canyon_hexahedral_f32": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=41,
+                                                  precision=32),
+}
+
+
+@pytest.mark.parametrize("name", list(CORE_CASES))
+def test_core3d_matches_oracle(name):
+    env = CORE_CASES[name]()
+    o = oracle.field_run3d(env)
+    g = core_run3d(env)
+    assert np.array_equal(g.nsteps, o.nsteps)
+    assert np.array_equal(g.ntop, o.ntop) and np.array_equal(g.nbot, o.nbot)
+    assert np.array_equal(g.xend, o.xend)
+    assert g.total_contrib == o.total_contrib
+    assert g.err == o.err and g.warn == o.warn
+    to, tg = tl_db(o.field), tl_db(g.field)
+    m = to < 120
+    assert np.max(np.abs(to - tg)[m]) < 1e-3
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_preprocess3d_matches_oracle(precision):
+    env = c3_env(nalpha=21, nbeta=12, ntheta=6, nrz=11, nrr=21, nxy=31, precision=precision)
+    ref = oracle.preprocess_dump3d(env)
+    p = bh.Problem3D(env)
+    v = p.view
+    got = []
+    for nodes, nx, ny in ((v.bty, v.bty_nx, v.bty_ny), (v.ati, v.ati_nx, v.ati_ny)):
+        for i in range(nx * ny):
+            b = nodes[i]
+            got += list(b.x) + list(b.n1) + list(b.n2)
+    nz = v.hex_nz
+    got += [v.hex_cz[i] for i in range(v.hex_nx * v.hex_ny * (nz - 1))]
+    got += [v.t_rcvr[i] for i in range(2 * v.Ntheta)]
+    got += [v.beta[i] for i in range(v.Nbeta)]
+    got += [v.Sx[i] for i in range(v.NSx)] + [v.Sy[i] for i in range(v.NSy)]
+    got += [v.deltas, v.box_x, v.box_y, v.box_z, v.Dbeta]
+    got = np.array(got)
+    assert got.shape == ref.shape
+    # the bottom grid's normals are only defined for cells (the last row /
+    # column of nodes carries none): compare where the oracle holds a value
+    ok = np.isfinite(ref)
+    assert np.array_equal(got[ok], ref[ok])
+
+
+def test_preprocess3d_rejects_unsupported():
+    for rt in ("Cg   3", "Cb   3", "AB   3", "EB   3", "RB   3"):
+        with pytest.raises(bh.BhcError):
+            bh.Problem3D(free_env3d(run_type=rt))
+    e = free_env3d()
+    e.ssp_type = "Q"
+    with pytest.raises(bh.BhcError):
+        bh.Problem3D(e)
+
+
+def test_abi_exports_3d_symbols():
+    L = bh.lib()
+    for name in ("bhc_preprocess3d", "bhc_problem3d_view", "bhc_problem3d_free", "bhc_host3d_last_error",
+                 "bhc_field_run3d", "bhc_scale_pressure3d"):
+        assert hasattr(L, name)
