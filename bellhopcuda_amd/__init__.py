@@ -91,6 +91,7 @@ def lib():
         "bhc_host3d_last_error": (C.c_char_p, []),
         "bhc_field_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut)]),
         "bhc_scale_pressure3d": (C.c_int, [P(_abi.Problem3D), _abi.c_float_p]),
+        "bhc_opcount3d": (C.c_int, [P(_abi.Problem3D), C.c_int, P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

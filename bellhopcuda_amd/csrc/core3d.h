@@ -692,7 +692,7 @@ __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &new
         cpx<real> gk = C(omega * Tg, (real)0);
         cpx<real> gk2 = cmul(gk, gk);
         real w1 = omega / c;
-        cpx<real> eps = C((real)0, RC<real>::MINPOS); // J * REAL_MINPOS
+        cpx<real> eps = C((real)0, (real)RC<real>::MINPOS); // J * REAL_MINPOS
         cpx<real> g1sq = (C(w1 * w1, (real)0) - gk2) - eps;
         cpx<real> wP = realdiv(omega, hs.cP);
         cpx<real> g2sq = (cmul(wP, wP) - gk2) - eps;
@@ -857,6 +857,7 @@ template<typename real> struct Seg3D {
     real phase0, qOld;          // Phase0 + accumulated caustic phase; det q at the segment start
     real rcp_q0, rcp_qhat0;
     real xs_x, xs_y;
+    real F0, Fd;                // Frobenius norms of the scaled q at the segment start and of its change
     int32_t ir0, ir1, dir;      // receiver ranges visited by the walk
     int32_t nitems;             // > 0 when the walk brackets at least one range
     int32_t isz, isx, isy, pad;
@@ -928,6 +929,14 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     d.qOld = I.qOld;
     d.rcp_q0 = I.rcp_q0;
     d.rcp_qhat0 = I.rcp_qhat0;
+    {
+        real u00 = p0.q.a00 * I.rcp_q0, u10 = p0.q.a10 * I.rcp_q0;
+        real u01 = p0.q.a01 * I.rcp_qhat0, u11 = p0.q.a11 * I.rcp_qhat0;
+        real v00 = d.dq.a00 * I.rcp_q0, v10 = d.dq.a10 * I.rcp_q0;
+        real v01 = d.dq.a01 * I.rcp_qhat0, v11 = d.dq.a11 * I.rcp_qhat0;
+        d.F0 = std::sqrt(u00 * u00 + u10 * u10 + u01 * u01 + u11 * u11);
+        d.Fd = std::sqrt(v00 * v00 + v10 * v10 + v01 * v01 + v11 * v11);
+    }
     d.xs_x = I.xs.x;
     d.xs_y = I.xs.y;
     d.isz = isz;
@@ -959,6 +968,50 @@ __host__ __device__ inline bool row3(const Seg3D<real> &d, int ir, int itheta, c
     zmin = d.zlo - Lz;
     zmax = d.zhi + Lz;
     return true;
+}
+
+// Depth band of a row that can pass the beam-window test. With u = (n1, n2)
+// the receiver's offset from the segment's line (|u| = d, its perpendicular
+// distance) and Q the row-scaled q at the receiver's projection s, the test
+// needs |Q^-1 u| <= K (K = 4 for Gaussian beams, whose test bounds
+// n1' + n2' >= |Q^-1 u|; sqrt 2 for hat beams, max(n1', n2') >= |Q^-1 u| /
+// sqrt 2), hence d <= K |Q|_F <= K (F0 + |s| Fd). Along the receiver column,
+// d^2 = a^2 (z - zc)^2 + dmin^2 (a = horizontal length of the unit tangent,
+// zc the depth of closest approach) and |s| <= |s(zc)| + |z - zc| |tz| / L:
+// the receivers are limited to |z - zc| <= K (F0 + Fd |s(zc)|) / (a - K Fd |tz| / L),
+// and the row is empty when dmin exceeds the bound at the band's edge. The
+// band is widened for rounding; receivers inside it go through every test of
+// the reference. Returns false for an empty row; (wlo, whi) is the band, or
+// the full window [zmin, zmax] when the bound does not apply.
+template<typename real>
+__host__ __device__ inline bool band3(const Seg3D<real> &d, real xr, real yr, real zmin, real zmax,
+                                      const Prob3D<real> &P, real &wlo, real &whi)
+{
+    wlo = zmin;
+    whi = zmax;
+    const double K = P.isGaussian ? 4.0 : 1.4142135623730951;
+    const double tx = (double)d.ta.x, ty = (double)d.ta.y, tz = (double)d.ta.z;
+    const double a2 = tx * tx + ty * ty, a = std::sqrt(a2);
+    const double L = (double)d.rlen, F0 = (double)d.F0, Fd = (double)d.Fd;
+    const double coef = a - K * Fd * std::fabs(tz) / L;
+    if(!(a > 1e-6) || !(coef > 1e-3 * a)) return true;
+    const double hx = (double)xr - (double)d.xa.x, hy = (double)yr - (double)d.xa.y;
+    const double hta = hx * tx + hy * ty;
+    const double zeta = hta * tz / a2;                 // zc - za
+    const double m = hx * ty - hy * tx;                // horizontal distance to the ray's vertical plane x a
+    const double dmin = std::fabs(m) / a;
+    const double sc = (hta + zeta * tz) / L;           // s at the closest approach
+    const double R0 = K * (F0 + Fd * std::fabs(sc));
+    const double mag = std::fabs(hx) + std::fabs(hy) + std::fabs((double)d.xa.z) + std::fabs(zeta) + 1.0;
+    const double slack = 1e-4 * mag + 1e-2;
+    const double dz = R0 / coef * (1.0 + 1e-4) + slack;
+    if(!(dz == dz) || !(dz < 1e30) || !(dmin == dmin)) return true;
+    const double dlim = K * (F0 + Fd * (std::fabs(sc) + dz * std::fabs(tz) / L)) * (1.0 + 1e-4) + slack;
+    if(dmin > dlim) return false;
+    const double zc = (double)d.xa.z + zeta;
+    wlo = rmax(zmin, (real)(zc - dz));
+    whi = rmin(zmax, (real)(zc + dz));
+    return whi >= wlo;
 }
 
 template<typename real, typename Sink, bool FAST = false>

@@ -197,9 +197,10 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
                 int j = row / P.Ntheta;
                 it = row - j * P.Ntheta;
                 ir = d.ir0 + j * d.dir;
-                if(row3(d, ir, it, P, xr, yr, zmin, zmax)) {
-                    if(P.rz_sorted && zmin == zmin && zmax == zmax) {
-                        depth_window(P.Rz, P.NRz, zmin, zmax, i0, i1);
+                real wlo, whi;
+                if(row3(d, ir, it, P, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {
+                    if(P.rz_sorted && wlo == wlo && whi == whi) {
+                        depth_window(P.Rz, P.NRz, wlo, whi, i0, i1);
                     } else {
                         i0 = 0;
                         i1 = P.NRz - 1;

@@ -83,6 +83,8 @@ inline CR round(CR a) { return CR(std::round(a.v)); }
 inline CR floor(CR a) { return CR(std::floor(a.v)); }
 inline CR ceil(CR a) { return CR(std::ceil(a.v)); }
 inline CR copysign(CR a, CR b) { return CR(std::copysign(a.v, b.v)); }
+inline CR acos(CR a) { ++opc::n; return CR(std::acos(a.v)); }
+inline bool isfinite(CR a) { return std::isfinite(a.v); }
 } // namespace std
 
 #include "core2d.h"
@@ -96,6 +98,15 @@ template<> __host__ __device__ constexpr CR rlit<CR>(double d, float) { return C
 } // namespace bhc2d
 
 #include "pack2d.h"
+#include "core3d.h"
+
+namespace bhc3d {
+template<> struct TD3<CR> {
+    static constexpr double THRESH = TD3<double>::THRESH;
+};
+} // namespace bhc3d
+
+#include "pack3d.h"
 
 using namespace bhc2d;
 
@@ -174,5 +185,102 @@ extern "C" int bhc_opcount2d(const bhc_problem2d *prob, int nsample, double *out
     out[3] = (double)rays;
     out[4] = (double)steps;
     out[5] = (double)contrib;
+    return 0;
+}
+
+namespace {
+struct CountSink3 {
+    long long contrib_ops = 0, ncontrib = 0, m = 0;
+    void add3(size_t, float, float) {}
+    void mark() { m = opc::n; }
+    void count()
+    {
+        ++ncontrib;
+        contrib_ops += opc::n - m;
+    }
+};
+} // namespace
+
+// The 3D path: f_I per ray (RayInit3 + Init_Influence), f_S per step
+// (RayUpdate3: Step3, boundary lookup, reflection, plus the segment prelude),
+// f_C per accepted contribution (InfluenceGeoCore + the TL term), over rays
+// [njobs * k / nsample]. Row / depth-band enumeration and rejected receivers
+// are not counted.
+extern "C" int bhc_opcount3d(const bhc_problem3d *prob, int nsample, double *out)
+{
+    using namespace bhc3d;
+    std::vector<CR> blob;
+    Prob3D<CR> P;
+    std::string err;
+    if(pack3<CR>(prob, blob, P, err)) return -1;
+    P.base = blob.data();
+    P.s.base = blob.data();
+    P.Rz = prob->Rz;
+    P.Sz = prob->Sz;
+    long long njobs = (long long)(prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta)
+        * prob->NSx * prob->NSy * prob->NSz;
+    if(nsample < 1) nsample = 1;
+    if(nsample > njobs) nsample = (int)njobs;
+    long long ops_init = 0, ops_step = 0, rays = 0, steps = 0;
+    CountSink3 sink;
+    Err e = {0, 0, 0, 0};
+    auto scatter = [&](const Seg3D<CR> &d) {
+        if(d.nitems <= 0) return;
+        int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        for(int j = 0; j < nr; ++j)
+            for(int it = 0; it < P.Ntheta; ++it) {
+                long long save = opc::n;
+                int ir = d.ir0 + j * d.dir, i0 = 0, i1 = P.NRz - 1;
+                CR xr, yr, zmin, zmax, wlo, whi;
+                bool ok = row3(d, ir, it, P, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi);
+                opc::n = save;
+                if(!ok) continue;
+                if(P.rz_sorted) depth_window(P.Rz, P.NRz, wlo.v, whi.v, i0, i1);
+                for(int iz = i0; iz <= i1; ++iz) {
+                    save = opc::n;
+                    item3<CR, CountSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, sink);
+                    opc::n = save;
+                }
+            }
+    };
+    for(int k = 0; k < nsample; ++k) {
+        int job = (int)((long long)k * njobs / nsample);
+        int ia, ib, isx, isy, isz;
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
+        Ray3<CR> p0, p1, p2;
+        Trace3<CR> S;
+        Infl3<CR> I;
+        opc::n = 0;
+        bool ok = RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e);
+        ops_init += opc::n;
+        ++rays;
+        if(!ok) continue;
+        int is = 0;
+        for(;;) {
+            opc::n = 0;
+            bool two = RayUpdate3(p0, p1, p2, S, P, e);
+            Seg3D<CR> d;
+            prelude3(d, p0, p1, I, is, P, isz, isx, isy);
+            ops_step += opc::n;
+            ++is;
+            scatter(d);
+            if(two) {
+                opc::n = 0;
+                prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                ops_step += opc::n;
+                ++is;
+                scatter(d);
+            }
+            p0 = two ? p2 : p1;
+            if(RayTerminate3(p0, is, S, P, e)) break;
+        }
+        steps += is;
+    }
+    out[0] = rays ? (double)ops_init / rays : 0.0;
+    out[1] = steps ? (double)ops_step / steps : 0.0;
+    out[2] = sink.ncontrib ? (double)sink.contrib_ops / sink.ncontrib : 0.0;
+    out[3] = (double)rays;
+    out[4] = (double)steps;
+    out[5] = (double)sink.ncontrib;
     return 0;
 }

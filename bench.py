@@ -12,6 +12,8 @@ Workloads (one "step" = one pass of the hot path over one batch of input):
 * seamount: one range-dependent TL field -- 40-knot spline SSP, 500-segment
   bathymetry, 20001 ray-centred Gaussian beams, 501 x 5001 receivers, FP64.
 * munk: one Munk-profile TL field, 2001 hat beams, 201 x 1001 receivers.
+* c3: one full-3D Gaussian-beam TL field -- hexahedral SSP, 201 x 201
+  bathymetry grid, 181 x 1001 beams, 36 bearings x 101 depths x 1001 ranges.
 
 Inputs are resident in HBM before the timed region. Prints one JSON line:
 fields/s (value), rays/s, the FP64 roofline of the field kernels (algorithmic
@@ -133,6 +135,81 @@ def run_single(a):
                                 "kind": "port",
                                 "sample": "%d rays in chunks spread over the %d-ray fan, full receiver grid, %.1f s"
                                           % (done, R, t)}
+    return line
+
+
+def run_c3(a):
+    """3D Gaussian-beam TL on the shelf-canyon environment: hexahedral SSP,
+    201 x 201 bathymetry grid, 181 bearings x 1001 declinations, 36 bearings x
+    101 depths x 1001 receiver ranges (bhc_field_run3d)."""
+    import numpy as np
+    import bellhopcuda_amd as bh
+    from bellhopcuda_amd.env3d import c3_env
+    L = bh.lib()
+    env = c3_env(precision=a.precision)
+    prob = bh.Problem3D(env)
+    R = prob.nrays
+    oc = (C.c_double * 6)()
+    if L.bhc_opcount3d(C.byref(prob.view), 64, oc) != 0:
+        raise SystemExit("3D op count failed")
+    f_I, f_S, f_C = oc[0], oc[1], oc[2]
+    ctx = bh.Context(0)
+    for _ in range(a.warmup):
+        bh.field_run3d(prob, ctx=ctx)
+    kms, tms, sms, walls = [], [], [], []
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        out = bh.field_run3d(prob, ctx=ctx)
+        walls.append(time.perf_counter() - t1)
+        kms.append(ctx.last_kernel_ms())
+        tr, sc, npass = C.c_double(), C.c_double(), C.c_int()
+        L.bhc_last_phase_ms(ctx._h, C.byref(tr), C.byref(sc), C.byref(npass))
+        tms.append(tr.value)
+        sms.append(sc.value)
+    k_ms = sum(kms) / len(kms)
+    S, Cn = out.total_steps, out.total_contrib
+    F = f_I * R + f_S * S + f_C * Cn
+    peak = FP64_PEAK_TFLOPS if a.precision == 64 else FP32_PEAK_TFLOPS
+    achieved = F / (k_ms * 1e-3) / 1e12
+    fields_s = 1e3 / k_ms
+    line = {
+        "metric": "TL fields/s + rays/s on 1 MI355X",
+        "value": fields_s, "unit": "TL fields/s", "rays_per_s": fields_s * R,
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": k_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == 64 else "f32", "data": "synthetic (seeded 3D environment, seed 20260003)",
+        "config": {"workload": "3D Gaussian-beam TL: 201x201 shelf-canyon bathymetry over 40x40 km, hexahedral "
+                               "21x21x50 SSP, 200 Hz, 181 bearings x 1001 declinations, receivers 36 bearings x "
+                               "101 depths x 1001 ranges to 20 km",
+                   "rays": R, "receivers": int(np.prod(prob.field_shape)), "parallelism": "single GPU"},
+        "timing": "value and ms_per_step from HIP events around the field kernels (inputs resident in HBM); "
+                  "wall_ms_per_step adds the per-call upload, allocation and download",
+        "wall_ms_per_step": 1e3 * sum(walls) / len(walls),
+        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel_ms": k_ms, "trace_ms_first_pass": sum(tms) / len(tms),
+                     "rest_ms": sum(sms) / len(sms), "flop_per_field": F, "f_I": f_I, "f_S": f_S, "f_C": f_C,
+                     "rays": R, "steps": S, "contributions": Cn},
+        "errors": {"err": out.err, "warn": out.warn},
+    }
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        cores = os.cpu_count() or 1
+        nch, per, done, t_total = 8, 16, 0, 0.0
+        while t_total < a.cpu_seconds and per <= R // nch:
+            t0 = time.time()
+            for k in range(nch):
+                b = max(0, min(R - per, int((k + 0.5) * R / nch) - per // 2))
+                oracle.field_run3d(env, nthreads=cores, job_begin=b, job_end=b + per)
+                done += per
+            t_total += time.time() - t0
+            per *= 2
+        rs = done / t_total
+        line["cpu_baseline"] = {"value": rs / R, "unit": "TL fields/s", "rays_per_s": rs, "cores": cores,
+                                "kind": "port",
+                                "sample": "%d rays in chunks spread over the %d-ray fan, full receiver grid, %.1f s"
+                                          % (done, R, t_total)}
     return line
 
 
@@ -330,7 +407,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "seamount", "munk", "surrogate"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "seamount", "munk", "surrogate", "c3"])
     ap.add_argument("--sur-train", type=int, default=20000)
     ap.add_argument("--sur-eval", type=int, default=1000)
     ap.add_argument("--sur-points", type=int, default=1000)
@@ -362,6 +439,8 @@ def main():
         line = run_sweep(a)
     elif a.workload == "surrogate":
         line = run_surrogate(a)
+    elif a.workload == "c3":
+        line = run_c3(a)
     else:
         line = run_single(a)
     print(json.dumps(line), flush=True)

@@ -526,6 +526,8 @@ double bhc_mlp_step_flops(bhc_mlp *m, int batch);
  * (initialisation), per step (step + influence prelude), per accepted receiver
  * contribution, and the sample's rays, steps and contributions. */
 int bhc_opcount2d(const bhc_problem2d *prob, int nsample, double *out);
+/* The same for the 3D path (per ray, per step, per accepted contribution). */
+int bhc_opcount3d(const bhc_problem3d *prob, int nsample, double *out);
 
 #ifdef __cplusplus
 }

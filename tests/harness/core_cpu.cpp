@@ -202,8 +202,10 @@ template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
             for(int it = 0; it < P.Ntheta; ++it) {
                 int ir = d.ir0 + j * d.dir, i0 = 0, i1 = -1;
                 real xr, yr, zmin, zmax;
+                real wlo, whi;
                 if(!row3(d, ir, it, P, xr, yr, zmin, zmax)) continue;
-                if(P.rz_sorted && zmin == zmin && zmax == zmax) depth_window(P.Rz, P.NRz, zmin, zmax, i0, i1);
+                if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
+                if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(P.Rz, P.NRz, wlo, whi, i0, i1);
                 else {
                     i0 = 0;
                     i1 = P.NRz - 1;
