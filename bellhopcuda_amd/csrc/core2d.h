@@ -179,6 +179,7 @@ template<typename real> struct Prob2D {
     int32_t npts, Nr;
     int32_t o_z, o_rho, o_c, o_cz, o_n2, o_n2z, o_coef, o_segr, o_cmat, o_czmat;
     int32_t ati_n, bty_n, o_ati, o_bty, node;  // node: reals per boundary node
+    int32_t o_rzs;         // receiver depths as reals, at the end of the staged prefix
     // rest of blob (global)
     int32_t o_Rr, o_trc, o_brc, ntrc, nbrc, o_sbp, nsbp;
     int32_t o_alpha, o_acos, o_asin;
@@ -892,8 +893,8 @@ __host__ __device__ inline bool IsAtCaustic(real qOld, real q)
 // Receiver-depth index window [i0, i1] (inclusive, clamped, possibly empty with
 // i1 < i0) that contains every depth Rz[i] with zlo <= Rz[i] <= zhi, widened by
 // one index on each side; requires strictly increasing Rz.
-template<typename real>
-__host__ __device__ inline void depth_window(const float *Rz, int n, real zlo, real zhi, int &i0, int &i1)
+template<typename real, typename ZT>
+__host__ __device__ inline void depth_window(const ZT *Rz, int n, real zlo, real zhi, int &i0, int &i1)
 {
     int lo = 0, hi = n; // first index with Rz >= zlo
     while(lo < hi) {
@@ -958,8 +959,8 @@ __host__ __device__ inline void desc_common(SegDesc<real> &d, const RayPt<real> 
 // Cartesian prelude: influence.hpp:1201-1297 and the receiver walk :1388-1402.
 template<typename real>
 __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
-                                             Infl<real> &I, int is, const Prob2D<real> &P, const real *G, int isz,
-                                             int ialpha)
+                                             Infl<real> &I, int is, const Prob2D<real> &P, const real *G,
+                                             const real *RzT, int isz, int ialpha)
 {
     const real *Rr = G + P.o_Rr;
     d.nitems = 0;
@@ -1011,7 +1012,7 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     d.nz = P.NRz_per_range;
     if(P.rz_sorted && P.run4 != 'I' && d.zmin > -RC<real>::MAXV) {
         int i0, i1;
-        depth_window(P.Rz, P.NRz_per_range, d.zmin, d.zmax, i0, i1);
+        depth_window(RzT, P.NRz_per_range, d.zmin, d.zmax, i0, i1);
         d.iz0 = i0;
         d.nz = imax(0, i1 - i0 + 1);
     }
@@ -1027,7 +1028,8 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
 // Ray-centred prelude: influence.hpp:1030-1071.
 template<typename real>
 __host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
-                                               Infl<real> &I, int is, const Prob2D<real> &P, int isz, int ialpha)
+                                               Infl<real> &I, int is, const Prob2D<real> &P, const real *RzT, int isz,
+                                               int ialpha)
 {
     d.nitems = 0;
     real phaseq = p0.q.x;
@@ -1071,7 +1073,7 @@ __host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<rea
         double slack = 1e-4 * W + 1e-6 * (std::fabs(ya) + std::fabs(yb)) + 1e-3;
         double zlo = (ya < yb ? ya : yb) - W - slack, zhi = (ya < yb ? yb : ya) + W + slack;
         int i0, i1;
-        depth_window(P.Rz, P.NRz_per_range, (real)zlo, (real)zhi, i0, i1);
+        depth_window(RzT, P.NRz_per_range, (real)zlo, (real)zhi, i0, i1);
         d.iz0 = i0;
         d.nz = imax(0, i1 - i0 + 1);
     }
@@ -1316,12 +1318,15 @@ __host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, cons
     for(int ir = blo; ir <= bhi; ++ir) raycen_one<real, RT, Sink, FAST>(d, ir, iz, nA, rA, nB, rB, P, Rr, sink);
 }
 
+// RzT: the receiver depths as reals (the staged copy when the problem's
+// tables sit in LDS).
 template<typename real>
 __host__ __device__ inline void prelude(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1, Infl<real> &I,
-                                        int is, const Prob2D<real> &P, const real *G, int isz, int ialpha)
+                                        int is, const Prob2D<real> &P, const real *G, const real *RzT, int isz,
+                                        int ialpha)
 {
-    if(P.isRayCen) prelude_raycen(d, p0, p1, I, is, P, isz, ialpha);
-    else prelude_cart(d, p0, p1, I, is, P, G, isz, ialpha);
+    if(P.isRayCen) prelude_raycen(d, p0, p1, I, is, P, RzT, isz, ialpha);
+    else prelude_cart(d, p0, p1, I, is, P, G, RzT, isz, ialpha);
 }
 
 template<typename real, int RT, typename Sink, bool FAST = false>

@@ -105,7 +105,7 @@ template<typename real> struct Prob3D {
     HSd<real> top, bot;
     int32_t isGaussian, rz_sorted;
     float *field;
-    int32_t njobs, pad2;
+    int32_t njobs, stage_tables; // stage_tables: the scatter copies Rr and the bearing table to LDS
 };
 
 template<typename real> struct Ray3 {
@@ -947,15 +947,15 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
 
 // One (range, bearing) row of a segment: the row tests of the reference and
 // the depth window [zmin, zmax]. Returns false when the row has no receivers.
+// Rr and trcv: the receiver ranges and bearing (cos, sin) table (in the blob or a staged copy).
 template<typename real>
-__host__ __device__ inline bool row3(const Seg3D<real> &d, int ir, int itheta, const Prob3D<real> &P, real &xr,
-                                     real &yr, real &zmin, real &zmax)
+__host__ __device__ inline bool row3(const Seg3D<real> &d, int ir, int itheta, const Prob3D<real> &P,
+                                     const real *Rr, const real *trcv, real &xr, real &yr, real &zmin, real &zmax)
 {
-    const real *G = P.base;
-    real rr = G[P.o_Rr + ir];
+    real rr = Rr[ir];
     if(!(rr >= d.lo && rr < d.hi)) return false;
     const real BW = P.isGaussian ? RL(4.0) : RL(1.0);
-    real tx = G[P.o_trcv + 2 * itheta], ty = G[P.o_trcv + 2 * itheta + 1];
+    real tx = trcv[2 * itheta], ty = trcv[2 * itheta + 1];
     xr = d.xs_x + rr * tx;
     yr = d.xs_y + rr * ty;
     real mprime = std::fabs((xr - d.xa.x) * (-d.ta.y) + (yr - d.xa.y) * d.ta.x);
@@ -1016,9 +1016,9 @@ __host__ __device__ inline bool band3(const Seg3D<real> &d, real xr, real yr, re
 
 template<typename real, typename Sink, bool FAST = false>
 __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, int iz, real xr, real yr, real zmin,
-                                      real zmax, const Prob3D<real> &P, Sink &sink)
+                                      real zmax, const Prob3D<real> &P, const float *Rz, Sink &sink)
 {
-    real zr = (real)P.Rz[iz];
+    real zr = (real)Rz[iz];
     if(zr < zmin || zr > zmax) return;
     v3<real> xrr = V3(xr, yr, zr) - d.xa;
     real s = dot3(xrr, d.ta) / d.rlen;

@@ -180,6 +180,21 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
     const int nwaves = gridDim.x * (blockDim.x / WAVE);
+    // receiver ranges and bearing table staged in LDS when they fit
+    extern __shared__ __align__(16) unsigned char lds3[];
+    const real *Rr = P.base + P.o_Rr, *trcv = P.base + P.o_trcv;
+    const float *Rz = P.Rz;
+    if(P.stage_tables) {
+        real *L = reinterpret_cast<real *>(lds3);
+        float *Lz = reinterpret_cast<float *>(L + P.NRr + 2 * P.Ntheta);
+        for(int i = threadIdx.x; i < P.NRr; i += blockDim.x) L[i] = Rr[i];
+        for(int i = threadIdx.x; i < 2 * P.Ntheta; i += blockDim.x) L[P.NRr + i] = trcv[i];
+        for(int i = threadIdx.x; i < P.NRz; i += blockDim.x) Lz[i] = Rz[i];
+        __syncthreads();
+        Rr = L;
+        trcv = L + P.NRr;
+        Rz = Lz;
+    }
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
     Sink3 sink{P.field, 0};
@@ -198,9 +213,9 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
                 it = row - j * P.Ntheta;
                 ir = d.ir0 + j * d.dir;
                 real wlo, whi;
-                if(row3(d, ir, it, P, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {
+                if(row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {
                     if(P.rz_sorted && wlo == wlo && whi == whi) {
-                        depth_window(P.Rz, P.NRz, wlo, whi, i0, i1);
+                        depth_window(Rz, P.NRz, wlo, whi, i0, i1);
                     } else {
                         i0 = 0;
                         i1 = P.NRz - 1;
@@ -217,13 +232,58 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
                 const int oexcl = __shfl(excl, own);
                 const real oxr = __shfl(xr, own), oyr = __shfl(yr, own);
                 const real ozmin = __shfl(zmin, own), ozmax = __shfl(zmax, own);
-                if(t < total) item3<real, Sink3, FAST>(d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, sink);
+                if(t < total) item3<real, Sink3, FAST>(d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, Rz, sink);
             }
         }
     }
     unsigned long long nc = sink.nc;
     for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
     if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
+// Scatter with one lane per segment: each lane walks its own segment's rows
+// and depth windows serially (no cross-lane flattening).
+template<typename real, bool FAST>
+__global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3 O)
+{
+    extern __shared__ __align__(16) unsigned char lds3l[];
+    const real *Rr = P.base + P.o_Rr, *trcv = P.base + P.o_trcv;
+    const float *Rz = P.Rz;
+    if(P.stage_tables) {
+        real *L = reinterpret_cast<real *>(lds3l);
+        float *Lz = reinterpret_cast<float *>(L + P.NRr + 2 * P.Ntheta);
+        for(int i = threadIdx.x; i < P.NRr; i += blockDim.x) L[i] = Rr[i];
+        for(int i = threadIdx.x; i < 2 * P.Ntheta; i += blockDim.x) L[P.NRr + i] = trcv[i];
+        for(int i = threadIdx.x; i < P.NRz; i += blockDim.x) Lz[i] = Rz[i];
+        __syncthreads();
+        Rr = L;
+        trcv = L + P.NRr;
+        Rz = Lz;
+    }
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    Sink3 sink{P.field, 0};
+    const Seg3<real> *segs = reinterpret_cast<const Seg3<real> *>(O.segs);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for(uint32_t si = blockIdx.x * blockDim.x + threadIdx.x; si < n; si += stride) {
+        const Seg3D<real> d = segs[si].d;
+        if(d.nitems <= 0) continue;
+        const int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        for(int j = 0; j < nr; ++j) {
+            const int ir = d.ir0 + j * d.dir;
+            for(int it = 0; it < P.Ntheta; ++it) {
+                real xr, yr, zmin, zmax, wlo, whi;
+                if(!row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax)) continue;
+                if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
+                int i0 = 0, i1 = P.NRz - 1;
+                if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(Rz, P.NRz, wlo, whi, i0, i1);
+                for(int iz = i0; iz <= i1; ++iz) item3<real, Sink3, FAST>(d, ir, it, iz, xr, yr, zmin, zmax, P, Rz, sink);
+            }
+        }
+    }
+    unsigned long long nc = sink.nc;
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if((threadIdx.x & (WAVE - 1)) == 0 && nc) atomicAdd(O.counters + 1, nc);
 }
 
 size_t seg3_size(int precision) { return precision == 64 ? sizeof(Seg3<double>) : sizeof(Seg3<float>); }
@@ -294,6 +354,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     P.Sz = d_Sz;
     P.field = d_field;
     P.njobs = njobs;
+    P.stage_tables = ((size_t)p->NRr + 2 * (size_t)p->Ntheta) * sizeof(real) + (size_t)p->NRz * sizeof(float) <= 56 * 1024;
     TRY(ensure_segs(c));
     DevOut3 O;
     memset(&O, 0, sizeof(O));
@@ -306,9 +367,22 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     O.seg_count = c->d_seg_count;
     O.overflow = c->d_overflow;
     O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg3<real>), 0xFFFFFFF0u);
-    std::vector<std::pair<int, int>> todo{{0, njobs}};
+    // split the rays into passes that fit the segment buffer, from the
+    // segments per ray of earlier runs on this context (a pass that still
+    // overflows is halved and retraced)
+    std::vector<std::pair<int, int>> todo;
+    {
+        int chunk = njobs;
+        if(c->seg_per_ray3 > 0) {
+            double fit = (double)O.seg_cap / (c->seg_per_ray3 * 1.1 + (double)SEG_BLOCK / WAVE);
+            chunk = (int)std::max(1.0, std::min((double)njobs, fit));
+        }
+        int np = (njobs + chunk - 1) / chunk;
+        for(int k = np - 1; k >= 0; --k) todo.push_back({(int)((long long)k * njobs / np), (int)((long long)(k + 1) * njobs / np)});
+    }
     c->passes = 0;
     bool first = true;
+    unsigned long long segs_total = 0;
     while(!todo.empty()) {
         auto r = todo.back();
         todo.pop_back();
@@ -338,15 +412,27 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             HTRY(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), s));
             continue;
         }
+        {
+            uint32_t ns = 0;
+            HTRY(hipMemcpy(&ns, c->d_seg_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            segs_total += ns;
+        }
         if(first) HTRY(hipEventRecord(c->evm, s));
         int sblocks = c->num_cus * 8;
-        if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d<real, false>), dim3(sblocks), dim3(256), 0, s, P, O);
-        else hipLaunchKernelGGL((k_scatter3d<real, true>), dim3(sblocks), dim3(256), 0, s, P, O);
+        size_t lds = P.stage_tables ? ((size_t)P.NRr + 2 * (size_t)P.Ntheta) * sizeof(real) + (size_t)P.NRz * sizeof(float) : 0;
+        if(c->scatter3_lanes) {
+            if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d_lanes<real, false>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d_lanes<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+        } else {
+            if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d<real, false>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+        }
         HTRY(hipGetLastError());
         ++c->passes;
         first = false;
     }
     HTRY(hipEventRecord(c->ev1, s));
+    c->seg_per_ray3 = std::max(c->seg_per_ray3, (double)segs_total / (double)njobs);
     if(out->field) HTRY(hipMemcpyAsync(out->field, d_field, 2 * nrcv * sizeof(float), hipMemcpyDeviceToHost, s));
     uint32_t ew[4];
     unsigned long long cnt[2];

@@ -160,13 +160,13 @@ extern "C" int bhc_opcount2d(const bhc_problem2d *prob, int nsample, double *out
             opc::n = 0;
             bool two = RayUpdate(p0, p1, p2, S, P, B, G, e);
             SegDesc<CR> d;
-            prelude(d, p0, p1, I, is, P, G, isz, ia);
+            prelude(d, p0, p1, I, is, P, G, G + P.o_rzs, isz, ia);
             ops_step += opc::n;
             ++is;
             for(int pass = 0; pass < (two ? 2 : 1); ++pass) {
                 if(pass == 1) {
                     opc::n = 0;
-                    prelude(d, p1, p2, I, is, P, G, isz, ia);
+                    prelude(d, p1, p2, I, is, P, G, G + P.o_rzs, isz, ia);
                     ops_step += opc::n;
                     ++is;
                 }
@@ -232,13 +232,13 @@ extern "C" int bhc_opcount3d(const bhc_problem3d *prob, int nsample, double *out
                 long long save = opc::n;
                 int ir = d.ir0 + j * d.dir, i0 = 0, i1 = P.NRz - 1;
                 CR xr, yr, zmin, zmax, wlo, whi;
-                bool ok = row3(d, ir, it, P, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi);
+                bool ok = row3(d, ir, it, P, P.base + P.o_Rr, P.base + P.o_trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi);
                 opc::n = save;
                 if(!ok) continue;
                 if(P.rz_sorted) depth_window(P.Rz, P.NRz, wlo.v, whi.v, i0, i1);
                 for(int iz = i0; iz <= i1; ++iz) {
                     save = opc::n;
-                    item3<CR, CountSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, sink);
+                    item3<CR, CountSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, P.Rz, sink);
                     opc::n = save;
                 }
             }

@@ -351,14 +351,14 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
         bool two = false;
         if(active) {
             two = RayUpdate<real, ST>(p0, p1, p2, S, P, B, G, e);
-            prelude(d, p0, p1, I, is, P, G, isz, ia);
+            prelude(d, p0, p1, I, is, P, G, B + P.o_rzs, isz, ia);
             ++is;
         }
         append_seg(d, lane, O, blk);
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
-                prelude(d, p1, p2, I, is, P, G, isz, ia);
+                prelude(d, p1, p2, I, is, P, G, B + P.o_rzs, isz, ia);
                 ++is;
             }
             append_seg(d, lane, O, blk);
@@ -829,6 +829,8 @@ struct bhc_ctx {
     int use_tiles;              // 0 disables the LDS-tiled TL scatter
     int exact_phasor;           // 1: TL phasor in the reference's full arithmetic
     size_t seg_mb;              // segment buffer size (MiB), allocated at first use
+    double seg_per_ray3;        // 3D runs: largest segments-per-ray seen (sizes the first pass)
+    int scatter3_lanes;         // 3D scatter mapping: 0 one wavefront per segment, 1 one lane per segment
 };
 
 struct bhc_dev_problem {
@@ -897,6 +899,7 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     c->exact_phasor = 0;
     if(const char *e = getenv("BHC_EXACT_PHASOR")) c->exact_phasor = atoi(e) != 0;
     c->seg_mb = 16384;
+    if(const char *e = getenv("BHC_SCATTER3_LANES")) c->scatter3_lanes = atoi(e) != 0;
     if(const char *e = getenv("BHC_SEG_BUFFER_MB")) c->seg_mb = (size_t)std::max(16L, atol(e));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));
@@ -1639,7 +1642,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bd, c
         bool two = false;
         if(active) {
             two = RayUpdate<real, ST>(p0, p1, p2, S, P, B, G, e);
-            prelude(d, p0, p1, I, is, P, G, isz, ia);
+            prelude(d, p0, p1, I, is, P, G, B + P.o_rzs, isz, ia);
             d.prob = pid;
             ++is;
         }
@@ -1647,7 +1650,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bd, c
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
-                prelude(d, p1, p2, I, is, P, G, isz, ia);
+                prelude(d, p1, p2, I, is, P, G, B + P.o_rzs, isz, ia);
                 d.prob = pid;
                 ++is;
             }

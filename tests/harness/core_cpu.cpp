@@ -68,11 +68,11 @@ template<typename real, bool FAST> int run(const bhc_problem2d *prob, bhc_field_
             if(e.err) break;
             bool two = RayUpdate(p0, p1, p2, S, P, B, G, e);
             SegDesc<real> d;
-            prelude(d, p0, p1, I, is, P, G, isz, ia);
+            prelude(d, p0, p1, I, is, P, G, G + P.o_rzs, isz, ia);
             seg_items<real, RT_TL, HostSink, FAST>(d, P, G, prob->Rz, sink);
             ++is;
             if(two) {
-                prelude(d, p1, p2, I, is, P, G, isz, ia);
+                prelude(d, p1, p2, I, is, P, G, G + P.o_rzs, isz, ia);
                 seg_items<real, RT_TL, HostSink, FAST>(d, P, G, prob->Rz, sink);
                 ++is;
             }
@@ -203,14 +203,14 @@ template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
                 int ir = d.ir0 + j * d.dir, i0 = 0, i1 = -1;
                 real xr, yr, zmin, zmax;
                 real wlo, whi;
-                if(!row3(d, ir, it, P, xr, yr, zmin, zmax)) continue;
+                if(!row3(d, ir, it, P, P.base + P.o_Rr, P.base + P.o_trcv, xr, yr, zmin, zmax)) continue;
                 if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
                 if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(P.Rz, P.NRz, wlo, whi, i0, i1);
                 else {
                     i0 = 0;
                     i1 = P.NRz - 1;
                 }
-                for(int iz = i0; iz <= i1; ++iz) item3<real, HostSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, sink);
+                for(int iz = i0; iz <= i1; ++iz) item3<real, HostSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, P.Rz, sink);
             }
     };
     for(int job = 0; job < njobs; ++job) {
