@@ -102,6 +102,7 @@ template<typename real> struct Prob3D {
     const float *Rz, *Sz;
     int32_t NSx, NSy, NSz, NRz, NRr, Ntheta, Nalpha, iSingleA, Nbeta, iSingleB;
     real Dalpha, Dbeta, freq0, deltas, box_x, box_y, box_z;
+    real rz_absmax;        // largest |receiver depth|
     HSd<real> top, bot;
     int32_t isGaussian, rz_sorted;
     float *field;
@@ -1012,6 +1013,35 @@ __host__ __device__ inline bool band3(const Seg3D<real> &d, real xr, real yr, re
     wlo = rmax(zmin, (real)(zc - dz));
     whi = rmin(zmax, (real)(zc + dz));
     return whi >= wlo;
+}
+
+// Bearing prefilter of a segment: can any receiver on bearing (ct, st) with a
+// range in [rlo, rhi] pass the beam-window test? An accepted receiver lies
+// within d <= K (F0 + Fd |s|) of the segment's line, with |s| <= |X - xa| / L
+// <= (rhi + rA + zspan) / L (rA the segment start's horizontal distance from
+// the source, zspan the largest depth offset of a receiver); and d is at least
+// the horizontal distance of the receiver column from the line's vertical
+// plane, |m0 + r (ct ty - st tx)| / a. The bearing is dropped when that
+// distance exceeds the bound for every r in [rlo, rhi] (with rounding slack).
+template<typename real>
+__host__ __device__ inline bool bearing3(const Seg3D<real> &d, real ct, real st, real rlo, real rhi, real zspan,
+                                         const Prob3D<real> &P)
+{
+    const double K = P.isGaussian ? 4.0 : 1.4142135623730951;
+    const double tx = (double)d.ta.x, ty = (double)d.ta.y;
+    const double a = std::sqrt(tx * tx + ty * ty);
+    if(!(a > 1e-6)) return true;
+    const double ex = (double)d.xa.x - (double)d.xs_x, ey = (double)d.xa.y - (double)d.xs_y;
+    const double rA = std::sqrt(ex * ex + ey * ey);
+    const double smax = ((double)rhi + rA + (double)zspan) / (double)d.rlen;
+    const double mag = (double)rhi + rA + (double)zspan + std::fabs((double)d.xa.x) + std::fabs((double)d.xa.y) + 1.0;
+    const double dmax = K * ((double)d.F0 + (double)d.Fd * smax) * (1.0 + 1e-4) + 1e-4 * mag + 1e-2;
+    const double m0 = -ex * ty + ey * tx;              // (xs - xa) x t
+    const double sg = (double)ct * ty - (double)st * tx;
+    const double f0 = m0 + (double)rlo * sg, f1 = m0 + (double)rhi * sg;
+    const double fmin = (f0 * f1 <= 0.0) ? 0.0 : std::fmin(std::fabs(f0), std::fabs(f1));
+    if(!(fmin == fmin) || !(dmax == dmax)) return true;
+    return fmin / a <= dmax;
 }
 
 template<typename real, typename Sink, bool FAST = false>

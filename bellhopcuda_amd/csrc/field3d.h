@@ -195,6 +195,8 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
         trcv = L + P.NRr;
         Rz = Lz;
     }
+    __shared__ int cand_all[4][WAVE];   // candidate bearings of each wavefront's segment
+    int *cand = cand_all[(threadIdx.x / WAVE) & 3];
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
     Sink3 sink{P.field, 0};
@@ -203,14 +205,30 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
         const Seg3D<real> d = segs[si].d;
         if(d.nitems <= 0) continue;
         const int nr = (d.ir1 - d.ir0) * d.dir + 1;
-        const int nrow = nr * P.Ntheta;
+        // bearings that can hold an accepted receiver (all of them beyond 64)
+        int nb = P.Ntheta;
+        const bool pre = P.Ntheta <= WAVE;
+        if(pre) {
+            real r0 = Rr[d.ir0], r1 = Rr[d.ir1];
+            real rlo = rmin(r0, r1), rhi = rmax(r0, r1);
+            real zs = P.rz_absmax + std::fabs(d.xa.z);
+            bool c = lane < P.Ntheta && bearing3(d, trcv[2 * lane], trcv[2 * lane + 1], rlo, rhi, zs, P);
+            unsigned long long m = __ballot(c);
+            if(c) cand[__popcll(m & ((1ull << lane) - 1ull))] = lane;
+            nb = __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const int nrow = nr * nb;
         for(int g = 0; g < nrow; g += WAVE) {
             const int row = g + lane;
             int ir = 0, it = 0, i0 = 0, i1 = -1;
             real xr = 0, yr = 0, zmin = 0, zmax = 0;
             if(row < nrow) {
-                int j = row / P.Ntheta;
-                it = row - j * P.Ntheta;
+                int j = row / nb;
+                int k = row - j * nb;
+                it = pre ? cand[k] : k;
                 ir = d.ir0 + j * d.dir;
                 real wlo, whi;
                 if(row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {

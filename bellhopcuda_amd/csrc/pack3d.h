@@ -117,6 +117,8 @@ int pack3(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &P, std:
     };
     P.top = hs(p->top_hs);
     P.bot = hs(p->bot_hs);
+    P.rz_absmax = 0;
+    for(int i = 0; i < p->NRz; ++i) P.rz_absmax = rmax(P.rz_absmax, (real)std::fabs((double)p->Rz[i]));
     P.rz_sorted = 1;
     for(int i = 1; i < p->NRz; ++i)
         if(!(p->Rz[i] > p->Rz[i - 1])) P.rz_sorted = 0;

@@ -198,8 +198,12 @@ template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
     auto scatter = [&](const Seg3D<real> &d) {
         if(d.nitems <= 0) return;
         int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        const real *Rr = P.base + P.o_Rr, *tr = P.base + P.o_trcv;
+        real rlo = rmin(Rr[d.ir0], Rr[d.ir1]), rhi = rmax(Rr[d.ir0], Rr[d.ir1]);
+        real zs = P.rz_absmax + std::fabs(d.xa.z);
         for(int j = 0; j < nr; ++j)
             for(int it = 0; it < P.Ntheta; ++it) {
+                if(P.Ntheta <= 64 && !bearing3(d, tr[2 * it], tr[2 * it + 1], rlo, rhi, zs, P)) continue;
                 int ir = d.ir0 + j * d.dir, i0 = 0, i1 = -1;
                 real xr, yr, zmin, zmax;
                 real wlo, whi;
