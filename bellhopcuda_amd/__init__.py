@@ -92,6 +92,9 @@ def lib():
         "bhc_field_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut)]),
         "bhc_scale_pressure3d": (C.c_int, [P(_abi.Problem3D), _abi.c_float_p]),
         "bhc_opcount3d": (C.c_int, [P(_abi.Problem3D), C.c_int, P(C.c_double)]),
+        "bhc_ray_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.RayOut3D)]),
+        "bhc_write_ray3d": (C.c_int, [C.c_char_p, P(_abi.Problem3D), C.c_char_p, P(_abi.RayPt3D), _abi.c_int32_p,
+                                      C.c_int32, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -413,3 +416,34 @@ def field_run3d(env_or_prob, ctx=None, stats=False):
     finally:
         if own:
             ctx.close()
+
+
+def ray_run3d(env_or_prob, ctx=None, max_points=4000):
+    """3D ray run type on the GPU (RunRayMode). Returns (problem, points
+    [rays * max_points] of bhc_ray_pt3d, nsteps [rays])."""
+    prob = env_or_prob if isinstance(env_or_prob, Problem3D) else Problem3D(env_or_prob)
+    own = ctx is None
+    if own:
+        ctx = Context(0)
+    try:
+        n = prob.nrays
+        pts = (_abi.RayPt3D * (n * max_points))()
+        ns = np.zeros(n, np.int32)
+        out = _abi.RayOut3D()
+        out.points = C.cast(pts, C.POINTER(_abi.RayPt3D))
+        out.nsteps = ns.ctypes.data_as(_abi.c_int32_p)
+        out.max_points_per_ray = max_points
+        _check(lib().bhc_ray_run3d(ctx._h, C.byref(prob.view), C.byref(out)))
+        return prob, pts, ns
+    finally:
+        if own:
+            ctx.close()
+
+
+def write_ray3d(path, prob, pts, nsteps, max_points, title="bellhop-mi355x"):
+    """.ray file of a 3D ray run (bhc_write_ray3d)."""
+    rc = lib().bhc_write_ray3d(path.encode(), C.byref(prob.view), title.encode(),
+                               C.cast(pts, C.POINTER(_abi.RayPt3D)), np.ascontiguousarray(nsteps, np.int32)
+                               .ctypes.data_as(_abi.c_int32_p), max_points, len(nsteps))
+    if rc != 0:
+        raise BhcError("writing %s failed" % path)

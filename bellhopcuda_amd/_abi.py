@@ -195,3 +195,17 @@ class Problem3D(C.Structure):
         ("deltas", C.c_double), ("box_x", C.c_double), ("box_y", C.c_double), ("box_z", C.c_double),
         ("NSBPPts", C.c_int32), ("pad3", C.c_int32), ("sbp", c_double_p),
     ]
+
+
+class RayPt3D(C.Structure):
+    _fields_ = [("p", C.c_double * 4), ("q", C.c_double * 4), ("phi", C.c_double), ("dummy", C.c_double),
+                ("NumTopBnc", C.c_int32), ("NumBotBnc", C.c_int32),
+                ("x", C.c_double * 3), ("t", C.c_double * 3),
+                ("c", C.c_double), ("Amp", C.c_double), ("Phase", C.c_double), ("tau", C.c_double * 2)]
+
+
+class RayOut3D(C.Structure):
+    _fields_ = [("points", C.POINTER(RayPt3D)), ("nsteps", c_int32_p),
+                ("max_points_per_ray", C.c_int32), ("pad", C.c_int32),
+                ("err", C.c_uint32), ("warn", C.c_uint32), ("err_count", C.c_uint32),
+                ("warn_count", C.c_uint32)]

@@ -13,6 +13,7 @@
 // (A*B)[c][r] = A[0][r]*B[c][0] + A[1][r]*B[c][1].
 #pragma once
 #include "core2d.h"
+#include "../../include/bhc_hip.h"
 
 namespace bhc3d {
 
@@ -820,9 +821,11 @@ __host__ __device__ inline bool RayUpdate3(const Ray3<real> &p0, Ray3<real> &p1,
     return false;
 }
 
+// Returns true when the ray ends; nsteps is set as the reference sets Nsteps
+// (is + 1 when the ray leaves / dies, is at the storage limit).
 template<typename real>
 __host__ __device__ inline bool RayTerminate3(const Ray3<real> &pt, int is, const Trace3<real> &S,
-                                              const Prob3D<real> &P, Err &e)
+                                              const Prob3D<real> &P, Err &e, int maxPoints, int &nsteps)
 { // trace.hpp:391-477 (3D)
     v3<real> x = pt.x, t = pt.t, xs = S.xs;
     bool leftbox = std::fabs(x.x - xs.x) >= P.box_x || std::fabs(x.y - xs.y) >= P.box_y
@@ -836,12 +839,51 @@ __host__ __device__ inline bool RayTerminate3(const Ray3<real> &pt, int is, cons
     bool eN = x.x > maxx || x.y > maxy || (x.x == maxx && t.x > RL(0.0)) || (x.y == maxy && t.y > RL(0.0));
     bool small = S.iSmall > 50;
     bool lost = pt.Amp < FL(0.005);
-    if(leftbox || lost || e0 || eN || small) return true;
-    if(is >= 100000 - 3) {
+    if(leftbox || lost || e0 || eN || small) {
+        nsteps = is + 1;
+        return true;
+    }
+    if(is >= maxPoints - 3) {
         e.warning(1);
+        nsteps = is;
         return true;
     }
     return false;
+}
+
+template<typename real>
+__host__ __device__ inline bool RayTerminate3(const Ray3<real> &pt, int is, const Trace3<real> &S,
+                                              const Prob3D<real> &P, Err &e)
+{
+    int n;
+    return RayTerminate3(pt, is, S, P, e, 100000, n);
+}
+
+template<typename real> __host__ __device__ inline void store_pt3(bhc_ray_pt3d &d, const Ray3<real> &r)
+{
+    d.p[0] = (double)r.p.a00;
+    d.p[1] = (double)r.p.a01;
+    d.p[2] = (double)r.p.a10;
+    d.p[3] = (double)r.p.a11;
+    d.q[0] = (double)r.q.a00;
+    d.q[1] = (double)r.q.a01;
+    d.q[2] = (double)r.q.a10;
+    d.q[3] = (double)r.q.a11;
+    d.phi = (double)r.phi;
+    d.dummy = 0.0;
+    d.NumTopBnc = r.NumTopBnc;
+    d.NumBotBnc = r.NumBotBnc;
+    d.x[0] = (double)r.x.x;
+    d.x[1] = (double)r.x.y;
+    d.x[2] = (double)r.x.z;
+    d.t[0] = (double)r.t.x;
+    d.t[1] = (double)r.t.y;
+    d.t[2] = (double)r.t.z;
+    d.c = (double)r.c;
+    d.Amp = (double)r.Amp;
+    d.Phase = (double)r.Phase;
+    d.tau[0] = (double)r.tau.re;
+    d.tau[1] = (double)r.tau.im;
 }
 
 // ---------------------------------------------------------------------------

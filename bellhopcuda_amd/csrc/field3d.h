@@ -304,6 +304,107 @@ __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3
     if((threadIdx.x & (WAVE - 1)) == 0 && nc) atomicAdd(O.counters + 1, nc);
 }
 
+// Ray run type in 3D: MainRayMode (src/trace.hpp:482-520), one ray per lane,
+// every point stored.
+template<typename real>
+__global__ void __launch_bounds__(64) k_ray3d(Prob3D<real> P, bhc_ray_pt3d *pts, int32_t *nsteps, int M, int njobs,
+                                            uint32_t *errw)
+{
+    const int job = blockIdx.x * blockDim.x + threadIdx.x;
+    if(job >= njobs) return;
+    Err e = {0, 0, 0, 0};
+    int ia, ib, isx, isy, isz;
+    JobIndices3(job, P, ia, ib, isy, isx, isz);
+    bhc_ray_pt3d *out = pts + (size_t)job * M;
+    Ray3<real> p0, p1, p2;
+    Trace3<real> S;
+    Infl3<real> I;
+    int N = -1;
+    if(!RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e)) {
+        store_pt3(out[0], p0);
+        N = 1;
+    } else {
+        store_pt3(out[0], p0);
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            bool two = RayUpdate3(p0, p1, p2, S, P, e);
+            store_pt3(out[is + 1], p1);
+            if(two) store_pt3(out[is + 2], p2);
+            is += two ? 2 : 1;
+            p0 = two ? p2 : p1;
+            if(RayTerminate3(p0, is, S, P, e, M, N)) break;
+        }
+        if(N < 0) N = is + 1;
+    }
+    nsteps[job] = N;
+    if(e.err) atomicOr(errw + 0, e.err);
+    if(e.warn) atomicOr(errw + 1, e.warn);
+    if(e.nerr) atomicAdd(errw + 2, e.nerr);
+    if(e.nwarn) atomicAdd(errw + 3, e.nwarn);
+}
+
+template<typename real> int ray3d(bhc_ctx *c, const bhc_problem3d *p, bhc_ray_out3d *out)
+{
+    std::vector<real> blob;
+    Prob3D<real> P;
+    std::string perr;
+    if(pack3<real>(p, blob, P, perr)) return fail("%s", perr.c_str());
+    long long nj64 = (long long)(p->iSingle_alpha >= 1 ? 1 : p->Nalpha) * (p->iSingle_beta >= 1 ? 1 : p->Nbeta)
+        * p->NSx * p->NSy * p->NSz;
+    const int M = out->max_points_per_ray;
+    if(M < 4) return fail("bhc_ray_run3d: max_points_per_ray must be at least 4");
+    if(nj64 * (long long)M > (1ll << 31)) return fail("bhc_ray_run3d: too many ray points for one call");
+    const int njobs = (int)nj64;
+    hipStream_t s = c->stream;
+    real *d_blob = nullptr;
+    float *d_Rz = nullptr, *d_Sz = nullptr;
+    bhc_ray_pt3d *d_pts = nullptr;
+    int32_t *d_ns = nullptr;
+    uint32_t *d_errw = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);
+        void *ptrs[] = {d_blob, d_Rz, d_Sz, d_pts, d_ns, d_errw};
+        for(void *q : ptrs)
+            if(q) (void)hipFree(q);
+    };
+    if(dalloc(&d_blob, blob.size()) || dalloc(&d_Rz, (size_t)p->NRz) || dalloc(&d_Sz, (size_t)p->NSz)
+       || dalloc(&d_pts, (size_t)njobs * M) || dalloc(&d_ns, (size_t)njobs) || dalloc(&d_errw, 4)) {
+        cleanup();
+        return -1;
+    }
+    if(hipMemcpyAsync(d_blob, blob.data(), blob.size() * sizeof(real), hipMemcpyHostToDevice, s) != hipSuccess
+       || hipMemcpyAsync(d_Rz, p->Rz, p->NRz * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess
+       || hipMemcpyAsync(d_Sz, p->Sz, p->NSz * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess
+       || hipMemsetAsync(d_errw, 0, 4 * sizeof(uint32_t), s) != hipSuccess) {
+        cleanup();
+        return fail("bhc_ray_run3d: upload failed");
+    }
+    P.base = d_blob;
+    P.s.base = d_blob;
+    P.Rz = d_Rz;
+    P.Sz = d_Sz;
+    P.njobs = njobs;
+    hipLaunchKernelGGL((k_ray3d<real>), dim3((njobs + WAVE - 1) / WAVE), dim3(WAVE), 0, s, P, d_pts, d_ns, M, njobs,
+                       d_errw);
+    if(hipGetLastError() != hipSuccess) rc = fail("bhc_ray_run3d: launch failed");
+    uint32_t ew[4] = {0, 0, 0, 0};
+    if(!rc && (hipMemcpyAsync(out->points, d_pts, (size_t)njobs * M * sizeof(bhc_ray_pt3d), hipMemcpyDeviceToHost, s)
+                   != hipSuccess
+               || hipMemcpyAsync(out->nsteps, d_ns, (size_t)njobs * sizeof(int32_t), hipMemcpyDeviceToHost, s)
+                   != hipSuccess
+               || hipMemcpyAsync(ew, d_errw, sizeof(ew), hipMemcpyDeviceToHost, s) != hipSuccess
+               || hipStreamSynchronize(s) != hipSuccess))
+        rc = fail("bhc_ray_run3d: download failed");
+    out->err = ew[0];
+    out->warn = ew[1];
+    out->err_count = ew[2];
+    out->warn_count = ew[3];
+    cleanup();
+    return rc;
+}
+
 size_t seg3_size(int precision) { return precision == 64 ? sizeof(Seg3<double>) : sizeof(Seg3<float>); }
 
 template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_out *out)
@@ -500,4 +601,14 @@ extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_fiel
     if(!(prob->deltas > 0)) return fail("bhc_field_run3d: deltas must be positive");
     if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_field_run3d: hipSetDevice failed");
     return prob->precision == 64 ? run3d<double>(ctx, prob, out) : run3d<float>(ctx, prob, out);
+}
+
+extern "C" int bhc_ray_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_out3d *out)
+{
+    if(!ctx || !prob || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d: null argument");
+    if(prob->NSx < 1 || prob->NSy < 1 || prob->NSz < 1 || prob->Nalpha < 1 || prob->Nbeta < 1)
+        return fail("bhc_ray_run3d: empty source / angle set");
+    if(!(prob->deltas > 0)) return fail("bhc_ray_run3d: deltas must be positive");
+    if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_ray_run3d: hipSetDevice failed");
+    return prob->precision == 64 ? ray3d<double>(ctx, prob, out) : ray3d<float>(ctx, prob, out);
 }

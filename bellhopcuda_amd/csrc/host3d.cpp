@@ -253,11 +253,12 @@ extern "C" int bhc_preprocess3d(const bhc_env3d *e, bhc_problem3d_owner **out)
     if(e->NSx < 1 || e->NSy < 1 || e->NSz < 1 || e->NRz < 1 || e->NRr < 1 || e->Ntheta < 1 || e->Nalpha < 1
        || e->Nbeta < 1)
         return fail3("sources, receivers and launch angles must be non-empty");
-    // TL run types with Cartesian geometric beams: src/module/runtype.hpp maps
-    // '^' and ' ' to 'G'. Ray-centred 3D beams ('g', 'b'), arrivals, eigenrays
-    // and the ray run type are not on this 3D path.
-    if(e->RunType[0] != 'C' && e->RunType[0] != 'S' && e->RunType[0] != 'I')
-        return fail3("3D run type '%c' is not supported (TL run types C / S / I)", e->RunType[0]);
+    // TL run types with Cartesian geometric beams (src/module/runtype.hpp maps
+    // '^' and ' ' to 'G') and the ray run type. Ray-centred 3D beams ('g',
+    // 'b'), arrivals and eigenrays are not on this 3D path.
+    if(e->RunType[0] != 'C' && e->RunType[0] != 'S' && e->RunType[0] != 'I' && e->RunType[0] != 'R')
+        return fail3("3D run type '%c' is not supported (TL run types C / S / I and the ray run type R)",
+                     e->RunType[0]);
     if(e->RunType[1] != 'G' && e->RunType[1] != '^' && e->RunType[1] != ' ' && e->RunType[1] != 'B')
         return fail3("3D beam type '%c' is not supported (Cartesian geometric beams G / B)", e->RunType[1]);
     if(e->ssp_type != 'C' && e->ssp_type != 'N' && e->ssp_type != 'S' && e->ssp_type != 'P' && e->ssp_type != 'H')

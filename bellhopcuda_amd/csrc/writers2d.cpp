@@ -424,6 +424,67 @@ extern "C" int bhc_write_ray2d(const char *path, const bhc_problem2d *prob, cons
                                  : write_ray<float>(path, prob, title, pts, nsteps, max_points, decl_deg, nrays);
 }
 
+// 3D .ray file: header with NSx NSy NSz, Nalpha Nbeta and "xyz"; per ray the
+// launch declination in radians (SrcDeclAngle * DegRad, src/mode/ray.hpp:226-230),
+// the point count, the bounce counts and every point (x, y, z).
+template<typename R> int write_ray3(const char *path, const bhc_problem3d *p, const char *title,
+                                    const bhc_ray_pt3d *pts, const int32_t *nsteps, int32_t max_points, int32_t nrays)
+{
+    LDWriter o;
+    if(!o.open(path)) return -1;
+    auto real_item = [&](double v) {
+        if(sizeof(R) == 8) o.f64(v);
+        else o.f32((float)v);
+    };
+    o.str(title);
+    o.ch('\n');
+    real_item(p->freq0);
+    o.ch('\n');
+    o.i32(p->NSx);
+    o.i32(p->NSy);
+    o.i32(p->NSz);
+    o.ch('\n');
+    o.i32(p->Nalpha);
+    o.i32(p->Nbeta);
+    o.ch('\n');
+    real_item(p->top_hs.depth);
+    o.ch('\n');
+    real_item(p->bot_hs.depth);
+    o.ch('\n');
+    o.str("xyz");
+    o.ch('\n');
+    const R RadDeg = (R)180.0f / (R)M_PI, DegRad = (R)M_PI / (R)180.0f;
+    for(int r = 0; r < nrays; ++r) {
+        int n = nsteps[r];
+        if(n <= 0) continue;
+        if(n > max_points) n = max_points;
+        int ia = p->iSingle_alpha >= 1 ? p->iSingle_alpha - 1 : r % p->Nalpha;
+        R decl = RadDeg * (R)p->alpha[ia];
+        const bhc_ray_pt3d *ray = pts + (size_t)r * max_points;
+        real_item((double)(decl * DegRad));
+        o.ch('\n');
+        o.i32(n);
+        o.i32(ray[n - 1].NumTopBnc);
+        o.i32(ray[n - 1].NumBotBnc);
+        o.ch('\n');
+        for(int k = 0; k < n; ++k) {
+            real_item(ray[k].x[0]);
+            real_item(ray[k].x[1]);
+            real_item(ray[k].x[2]);
+            o.ch('\n');
+        }
+    }
+    return o.close() ? 0 : -1;
+}
+
+extern "C" int bhc_write_ray3d(const char *path, const bhc_problem3d *prob, const char *title,
+                               const bhc_ray_pt3d *pts, const int32_t *nsteps, int32_t max_points, int32_t nrays)
+{
+    if(!path || !prob || !pts || !nsteps || max_points < 1) return -1;
+    return prob->precision == 64 ? write_ray3<double>(path, prob, title, pts, nsteps, max_points, nrays)
+                                 : write_ray3<float>(path, prob, title, pts, nsteps, max_points, nrays);
+}
+
 // The list-directed writer as a stream, for companion files in the same layout.
 struct bhc_ldfile {
     LDWriter w;

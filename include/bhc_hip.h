@@ -91,6 +91,17 @@ typedef struct bhc_ray_pt2d {
     double tau[2];
 } bhc_ray_pt2d;
 
+/* rayPt<true> in the 64-bit layout, include/bhc/structs.hpp:397-424 (176 bytes):
+ * p, q are 2x2 matrices stored column-major (p[0] = p[0][0], p[1] = p[0][1], ...). */
+typedef struct bhc_ray_pt3d {
+    double p[4], q[4];
+    double phi, dummy;
+    int32_t NumTopBnc, NumBotBnc;
+    double x[3], t[3];
+    double c, Amp, Phase;
+    double tau[2];
+} bhc_ray_pt3d;
+
 /* ---- raw (user-unit) 2D environment: input to preprocessing ------------------ */
 
 /* Halfspace / boundary condition, HSInfo + HSExtra (include/bhc/structs.hpp:112-125).
@@ -463,6 +474,21 @@ const char *bhc_host3d_last_error(void);
  * ray_xend holding [rays][3] end points. Rays are numbered like GetJobIndices
  * (src/common.hpp:365-391): alpha fastest, then beta, Sy, Sx, Sz. */
 int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out);
+/* 3D ray run type (RunRayMode, src/mode/ray.cpp:167-188, with MainRayMode
+ * src/trace.hpp:482-520 in 3D): every ray of the fan in job order, up to
+ * max_points_per_ray points each. */
+typedef struct bhc_ray_out3d {
+    bhc_ray_pt3d *points;   /* [NRays][max_points_per_ray] */
+    int32_t *nsteps;        /* [NRays] */
+    int32_t max_points_per_ray;
+    int32_t pad;
+    uint32_t err, warn, err_count, warn_count;
+} bhc_ray_out3d;
+int bhc_ray_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_out3d *out);
+/* .ray file of a 3D ray run (OpenRAYFile + WriteRay, src/mode/ray.hpp:168-244):
+ * nrays rays in job order. */
+int bhc_write_ray3d(const char *path, const bhc_problem3d *prob, const char *title, const bhc_ray_pt3d *pts,
+                    const int32_t *nsteps, int32_t max_points, int32_t nrays);
 /* ScalePressure for 3D geometric beams (src/influence.hpp:1556-1627): incoherent
  * and semicoherent fields become sqrt(intensity); coherent fields are unchanged. */
 int bhc_scale_pressure3d(const bhc_problem3d *prob, float *field);

@@ -1251,3 +1251,68 @@ int SFX(oracle_preprocess_dump3d)(const bhc_env3d *env, double *dst, int64_t cap
     prob3_free(&Q);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Ray run type, src/trace.hpp:482-520 (3D)                                    */
+/* ------------------------------------------------------------------------- */
+static void store_pt3(bhc_ray_pt3d *d, const RayPt3 *s)
+{
+    d->p[0] = s->p.m[0][0]; d->p[1] = s->p.m[0][1]; d->p[2] = s->p.m[1][0]; d->p[3] = s->p.m[1][1];
+    d->q[0] = s->q.m[0][0]; d->q[1] = s->q.m[0][1]; d->q[2] = s->q.m[1][0]; d->q[3] = s->q.m[1][1];
+    d->phi = s->phi;
+    d->dummy = 0.0;
+    d->NumTopBnc = s->NumTopBnc;
+    d->NumBotBnc = s->NumBotBnc;
+    d->x[0] = s->x.x; d->x[1] = s->x.y; d->x[2] = s->x.z;
+    d->t[0] = s->t.x; d->t[1] = s->t.y; d->t[2] = s->t.z;
+    d->c = s->c; d->Amp = s->Amp; d->Phase = s->Phase;
+    d->tau[0] = creal(s->tau); d->tau[1] = cimag(s->tau);
+}
+
+static void MainRayMode3(RayInit3Info *ri, RayPt3 *ray, int *Nsteps, int MaxPointsPerRay, const Prob3 *Q,
+                         ErrState *err)
+{
+    TraceState3 S;
+    if(!RayInit3(ri, &ray[0], &S, Q, err)) {
+        *Nsteps = 1;
+        return;
+    }
+    int is = 0;
+    for(;;) {
+        if(HasErrored(err)) break;
+        int two = RayUpdate3(&ray[is], &ray[is + 1], &ray[is + 2], &S, Q, err);
+        if(*Nsteps >= 0 && is >= *Nsteps) {
+            *Nsteps = is + 2;
+            break;
+        }
+        is += two ? 2 : 1;
+        if(RayTerminate3(&ray[is], Nsteps, is, &S, MaxPointsPerRay, Q, err)) break;
+    }
+}
+
+/* Exported: the 3D ray run, every ray in job order. */
+int SFX(oracle_ray_run3d)(const bhc_env3d *env, bhc_ray_pt3d *points, int32_t *nsteps, int max_points)
+{
+    Prob3 Q;
+    if(prob3_from_env(env, &Q)) return -1;
+    ErrState err;
+    atomic_init(&err.error, 0u);
+    atomic_init(&err.warning, 0u);
+    atomic_init(&err.errCount, 0u);
+    atomic_init(&err.warnCount, 0u);
+    int nj = num_jobs3(&Q);
+    RayPt3 *buf = xcalloc((size_t)max_points + 3, sizeof(RayPt3));
+    for(int k = 0; k < nj; ++k) {
+        RayInit3Info ri;
+        if(!job_indices3(&ri, k, &Q)) break;
+        int N = -1;
+        memset(buf, 0, sizeof(RayPt3) * ((size_t)max_points + 3));
+        MainRayMode3(&ri, buf, &N, max_points, &Q, &err);
+        nsteps[k] = N;
+        int n = N < max_points ? N : max_points;
+        for(int i = 0; i < n; ++i) store_pt3(&points[(size_t)k * max_points + i], &buf[i]);
+    }
+    free(buf);
+    prob3_free(&Q);
+    return 0;
+}

@@ -205,3 +205,19 @@ def preprocess_dump3d(env, cap=1 << 22):
     if f(C.byref(ec), dst.ctypes.data_as(_abi.c_double_p), cap, C.byref(used)) != 0:
         raise RuntimeError("oracle_preprocess_dump3d failed")
     return dst[:min(used.value, cap)]
+
+
+def ray_run3d(env, max_points=4000):
+    """3D ray run of the oracle: (points [rays, max_points] of bhc_ray_pt3d, nsteps [rays])."""
+    L = _lib3()
+    from bellhopcuda_amd import _abi as A
+    f = getattr(L, "oracle_ray_run3d_" + _sfx(env))
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(A.Env3D_C), C.POINTER(A.RayPt3D), A.c_int32_p, C.c_int]
+    ec, keep = env.to_c()
+    n = env.nrays
+    pts = (A.RayPt3D * (n * max_points))()
+    ns = np.zeros(n, np.int32)
+    if f(C.byref(ec), pts, ns.ctypes.data_as(A.c_int32_p), max_points) != 0:
+        raise RuntimeError("oracle_ray_run3d failed")
+    return pts, ns

@@ -270,3 +270,53 @@ extern "C" int core_field_run3d(const bhc_problem3d *prob, bhc_field_out *out)
 {
     return prob->precision == 64 ? run3<double>(prob, out) : run3<float>(prob, out);
 }
+
+namespace {
+template<typename real> int ray3(const bhc_problem3d *prob, bhc_ray_pt3d *pts, int32_t *nsteps, int M)
+{
+    using namespace bhc3d;
+    std::vector<real> blob;
+    Prob3D<real> P;
+    std::string err;
+    if(pack3<real>(prob, blob, P, err)) return -1;
+    P.base = blob.data();
+    P.s.base = blob.data();
+    P.Rz = prob->Rz;
+    P.Sz = prob->Sz;
+    Err e = {0, 0, 0, 0};
+    int njobs = (prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta)
+        * prob->NSx * prob->NSy * prob->NSz;
+    for(int job = 0; job < njobs; ++job) {
+        int ia, ib, isx, isy, isz;
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
+        bhc_ray_pt3d *out = pts + (size_t)job * M;
+        Ray3<real> p0, p1, p2;
+        Trace3<real> S;
+        Infl3<real> I;
+        int N = -1;
+        bool ok = RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e);
+        store_pt3(out[0], p0);
+        if(!ok) {
+            nsteps[job] = 1;
+            continue;
+        }
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            bool two = RayUpdate3(p0, p1, p2, S, P, e);
+            store_pt3(out[is + 1], p1);
+            if(two) store_pt3(out[is + 2], p2);
+            is += two ? 2 : 1;
+            p0 = two ? p2 : p1;
+            if(RayTerminate3(p0, is, S, P, e, M, N)) break;
+        }
+        nsteps[job] = N < 0 ? is + 1 : N;
+    }
+    return 0;
+}
+} // namespace
+
+extern "C" int core_ray_run3d(const bhc_problem3d *prob, bhc_ray_pt3d *pts, int32_t *nsteps, int M)
+{
+    return prob->precision == 64 ? ray3<double>(prob, pts, nsteps, M) : ray3<float>(prob, pts, nsteps, M);
+}

@@ -71,3 +71,37 @@ def test_run3d_rejects_unsupported(gpu_ctx):
     env = free_env3d(run_type="Cb   3")
     with pytest.raises(bh.BhcError):
         bh.Problem3D(env)
+
+
+RAY_CASES = {
+    "munk_rays": lambda: munk_env3d(nalpha=41, nbeta=5, beta=(-30.0, 30.0), run_type="R    3"),
+    "shelf_canyon_rays": lambda: c3_env(nalpha=21, nbeta=12, ntheta=4, nrz=11, nrr=21, nxy=51, run_type="R    3"),
+    "shelf_canyon_rays_f32": lambda: c3_env(nalpha=21, nbeta=12, ntheta=4, nrz=11, nrr=21, nxy=51,
+                                            run_type="R    3", precision=32),
+}
+
+
+@pytest.mark.parametrize("name", list(RAY_CASES))
+def test_ray_mode3d_parity(name, gpu_ctx):
+    """Ray run type in 3D: step counts and every point's position and tangent
+    equal the oracle's (FP64 and FP32: they involve no transcendental); p, q
+    and the torsion to 1e-6 relative in FP64 (device cos / sin / acos of the
+    torsion) and 1e-2 in FP32, where one ulp of the torsion's single-precision
+    trigonometry compounds over the reflections."""
+    print("\n[gpu-case-3d-ray]", name, flush=True)
+    env = RAY_CASES[name]()
+    M = 3000
+    po, no = oracle.ray_run3d(env, max_points=M)
+    prob, pg, ng = bh.ray_run3d(env, ctx=gpu_ctx, max_points=M)
+    assert np.array_equal(no, ng)
+    ao = np.ctypeslib.as_array(po).reshape(len(no), M)
+    ag = np.ctypeslib.as_array(pg).reshape(len(ng), M)
+    for k in range(len(no)):
+        n = min(no[k], M)
+        assert np.array_equal(ao[k, :n]["x"], ag[k, :n]["x"])
+        assert np.array_equal(ao[k, :n]["t"], ag[k, :n]["t"])
+        assert np.array_equal(ao[k, :n]["NumBotBnc"], ag[k, :n]["NumBotBnc"])
+        tol = 1e-6 if env.precision == 64 else 1e-2
+        for f in ("p", "q"):
+            a, b = ao[k, :n][f], ag[k, :n][f]
+            assert np.allclose(a, b, rtol=tol, atol=tol * max(1.0, float(np.max(np.abs(a))))), (k, f)

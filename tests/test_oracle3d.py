@@ -192,7 +192,7 @@ def test_preprocess3d_matches_oracle(precision):
 
 
 def test_preprocess3d_rejects_unsupported():
-    for rt in ("Cg   3", "Cb   3", "AB   3", "EB   3", "RB   3"):
+    for rt in ("Cg   3", "Cb   3", "AB   3", "EB   3"):
         with pytest.raises(bh.BhcError):
             bh.Problem3D(free_env3d(run_type=rt))
     e = free_env3d()
@@ -204,5 +204,76 @@ def test_preprocess3d_rejects_unsupported():
 def test_abi_exports_3d_symbols():
     L = bh.lib()
     for name in ("bhc_preprocess3d", "bhc_problem3d_view", "bhc_problem3d_free", "bhc_host3d_last_error",
-                 "bhc_field_run3d", "bhc_scale_pressure3d"):
+                 "bhc_field_run3d", "bhc_scale_pressure3d", "bhc_opcount3d", "bhc_ray_run3d", "bhc_write_ray3d"):
         assert hasattr(L, name)
+
+
+def test_ray_mode3d_straight_rays_and_2d_plane():
+    # isovelocity: every ray is straight along its launch direction
+    env = free_env3d(nalpha=11, nbeta=8, run_type="R    3")
+    pts, ns = oracle.ray_run3d(env, max_points=400)
+    M = 400
+    for k in range(env.nrays):
+        n = ns[k]
+        assert n >= 2
+        x = np.array([list(pts[k * M + i].x) for i in range(n)])
+        d = x[-1] - x[0]
+        ia, ib = k % 11, k // 11
+        a, b = np.radians(env.alpha_deg[ia]), np.radians(env.beta_deg[ib])
+        u = np.array([np.cos(a) * np.cos(b), np.cos(a) * np.sin(b), np.sin(a)])
+        assert np.allclose(d / np.linalg.norm(d), u, atol=1e-12)
+    # depth-only Munk: the beta = 0 ray stays in its vertical plane and follows the 2D ray
+    e3 = munk_env3d(nalpha=21, nbeta=3, beta=(-1.0, 1.0), run_type="R    3")
+    pts3, ns3 = oracle.ray_run3d(e3, max_points=3000)
+    e2 = munk_env(nbeams=21, run_type="R ", r_max_km=20.0)
+    arr2, ns2, _ = oracle.ray_run(e2, max_points=3000)
+    for ia in range(21):
+        k = 1 * 21 + ia
+        n3 = ns3[k]
+        assert n3 == ns2[ia]
+        x3 = np.array([list(pts3[k * 3000 + i].x) for i in range(n3)])
+        x2 = np.array(arr2[ia, :n3]["x"])
+        assert np.max(np.abs(x3[:, 1])) == 0.0
+        assert np.max(np.abs(x3[:, 0] - x2[:, 0])) < 1e-6 and np.max(np.abs(x3[:, 2] - x2[:, 1])) < 1e-6
+
+
+def test_write_ray3d(tmp_path):
+    env = free_env3d(nalpha=5, nbeta=4, run_type="R    3")
+    pts, ns = oracle.ray_run3d(env, max_points=400)
+    prob = bh.Problem3D(env)
+    path = str(tmp_path / "free3d.ray")
+    bh.write_ray3d(path, prob, pts, ns, 400, title="free space 3D")
+    lines = open(path).read().splitlines()
+    assert lines[0].strip().strip("'") == "free space 3D"
+    assert lines[2].split() == ["1", "1", "1"] and lines[3].split() == ["5", "4"]
+    assert lines[6].strip().strip("'") == "xyz"
+    # first ray: declination in radians, then count and bounces, then n points of 3 values
+    assert abs(float(lines[7]) - np.radians(env.alpha_deg[0])) < 1e-12
+    n, ntop, nbot = map(int, lines[8].split())
+    assert n == ns[0] and len(lines[9].split()) == 3
+    assert len(lines) == 7 + sum(2 + int(v) for v in ns)
+
+
+def test_core_ray_mode3d_matches_oracle():
+    """The product's 3D ray core on the host equals the oracle point for point
+    (positions, tangents, p, q, torsion), FP64 and FP32."""
+    H = harness()
+    H.core_ray_run3d.restype = C.c_int
+    H.core_ray_run3d.argtypes = [C.POINTER(_abi.Problem3D), C.POINTER(_abi.RayPt3D), _abi.c_int32_p, C.c_int]
+    M = 3000
+    for env in (munk_env3d(nalpha=41, nbeta=5, beta=(-30.0, 30.0), run_type="R    3"),
+                c3_env(nalpha=11, nbeta=8, ntheta=4, nrz=11, nrr=21, nxy=41, run_type="R    3", precision=32)):
+        po, no = oracle.ray_run3d(env, max_points=M)
+        p = bh.Problem3D(env)
+        n = p.nrays
+        pg = (_abi.RayPt3D * (n * M))()
+        ng = np.zeros(n, np.int32)
+        assert H.core_ray_run3d(C.byref(p.view), C.cast(pg, C.POINTER(_abi.RayPt3D)),
+                                ng.ctypes.data_as(_abi.c_int32_p), M) == 0
+        assert np.array_equal(no, ng)
+        ao = np.ctypeslib.as_array(po).reshape(n, M)
+        ag = np.ctypeslib.as_array(pg).reshape(n, M)
+        for k in range(n):
+            m = min(no[k], M)
+            for f in ("x", "t", "p", "q", "phi", "tau", "Amp"):
+                assert np.array_equal(ao[k, :m][f], ag[k, :m][f]), (k, f)
