@@ -95,6 +95,10 @@ def lib():
         "bhc_ray_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.RayOut3D)]),
         "bhc_write_ray3d": (C.c_int, [C.c_char_p, P(_abi.Problem3D), C.c_char_p, P(_abi.RayPt3D), _abi.c_int32_p,
                                       C.c_int32, C.c_int32]),
+        "bhc_postprocess_arrivals3d": (C.c_int, [P(_abi.Problem3D), P(_abi.Arrival), _abi.c_int32_p, C.c_int32,
+                                                 _abi.c_int32_p]),
+        "bhc_write_arr3d": (C.c_int, [C.c_char_p, P(_abi.Problem3D), P(_abi.Arrival), _abi.c_int32_p, C.c_int32,
+                                      _abi.c_int32_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -378,6 +382,10 @@ class Problem3D:
         v = self.view
         return (v.NSz, v.NSx, v.NSy, v.Ntheta, v.NRz, v.NRr)
 
+    @property
+    def run_type(self):
+        return bytes(self.view.RunType).decode("latin-1")
+
     def scale_pressure(self, field):
         f = np.ascontiguousarray(field, np.complex64).copy()
         if lib().bhc_scale_pressure3d(C.byref(self.view), f.ctypes.data_as(_abi.c_float_p)) != 0:
@@ -385,10 +393,12 @@ class Problem3D:
         return f
 
 
-def field_run3d(env_or_prob, ctx=None, stats=False):
-    """3D TL field run on the GPU (RunFieldModesImpl<CFG, true, true>). Returns
+def field_run3d(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1, eigen_memsize=0):
+    """3D field run on the GPU (RunFieldModesImpl<CFG, true, true>). Returns
     a FieldResult: .field (complex64, [NSz, NSx, NSy, Ntheta, NRz, NRr],
-    pre-ScalePressure), .total_steps, .total_contrib, .err, .warn and, with
+    pre-ScalePressure) for TL runs, .arr / .narr ([receivers][max_narr] in
+    field order) for arrivals runs ('A', 'a'), .hits / .neigen for eigenray
+    runs ('E'); .total_steps, .total_contrib, .err, .warn and, with
     stats=True, per-ray .nsteps / .ntop / .nbot / .xend [rays, 3]."""
     prob = env_or_prob if isinstance(env_or_prob, Problem3D) else Problem3D(env_or_prob)
     own = ctx is None
@@ -400,6 +410,22 @@ def field_run3d(env_or_prob, ctx=None, stats=False):
         res.field = np.zeros(prob.field_shape, np.complex64)
         out = _abi.FieldOut()
         out.field = res.field.ctypes.data_as(_abi.c_float_p)
+        rt = prob.run_type[0]
+        nrcv = int(np.prod(prob.field_shape))
+        if rt in "Aa":
+            res.max_narr = max_narr
+            res.arr = (_abi.Arrival * (nrcv * max(max_narr, 1)))()
+            res.narr = np.zeros(nrcv, np.int32)
+            out.arr = C.cast(res.arr, C.POINTER(_abi.Arrival))
+            out.narr = res.narr.ctypes.data_as(_abi.c_int32_p)
+            out.max_narr = max_narr
+            out.allow_merging = allow_merging
+        if rt == "E":
+            res.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
+            res.neigen = np.zeros(1, np.int32)
+            out.hits = C.cast(res.hits, C.POINTER(_abi.EigenHit))
+            out.neigen = res.neigen.ctypes.data_as(_abi.c_int32_p)
+            out.eigen_memsize = eigen_memsize
         if stats:
             res.nsteps = np.zeros(nrays, np.int32)
             res.ntop = np.zeros(nrays, np.int32)

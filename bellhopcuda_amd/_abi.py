@@ -194,6 +194,7 @@ class Problem3D(C.Structure):
         ("Dalpha", C.c_double), ("Dbeta", C.c_double),
         ("deltas", C.c_double), ("box_x", C.c_double), ("box_y", C.c_double), ("box_z", C.c_double),
         ("NSBPPts", C.c_int32), ("pad3", C.c_int32), ("sbp", c_double_p),
+        ("theta", c_double_p),
     ]
 
 

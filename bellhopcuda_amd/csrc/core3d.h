@@ -903,7 +903,8 @@ template<typename real> struct Seg3D {
     real F0, Fd;                // Frobenius norms of the scaled q at the segment start and of its change
     int32_t ir0, ir1, dir;      // receiver ranges visited by the walk
     int32_t nitems;             // > 0 when the walk brackets at least one range
-    int32_t isz, isx, isy, pad;
+    int32_t isz, isx, isy, job; // source indices and ray (job) number
+    int32_t is, ntop, nbot, pad; // step index and the end point's bounce counts (arrivals, eigen hits)
 };
 
 template<typename real>
@@ -985,6 +986,9 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     d.isz = isz;
     d.isx = isx;
     d.isy = isy;
+    d.is = is;
+    d.ntop = p1.NumTopBnc;
+    d.nbot = p1.NumBotBnc;
     d.nitems = ((ir >= first ? ir - first : first - ir) + 1) * P.Ntheta;
 }
 
@@ -1086,7 +1090,9 @@ __host__ __device__ inline bool bearing3(const Seg3D<real> &d, real ct, real st,
     return fmin / a <= dmax;
 }
 
-template<typename real, typename Sink, bool FAST = false>
+// RT selects ApplyContribution's branch (influence.hpp:432-473): TL field
+// term, AddArr candidate (arrivals) or RecordEigenHit (eigen hits).
+template<typename real, typename Sink, bool FAST = false, int RT = RT_TL>
 __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, int iz, real xr, real yr, real zmin,
                                       real zmax, const Prob3D<real> &P, const float *Rz, Sink &sink)
 {
@@ -1125,6 +1131,15 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
     if((qF <= RL(0.0) && d.qOld > RL(0.0)) || (qF >= RL(0.0) && d.qOld < RL(0.0)))
         phaseInt = phaseInt + RC<real>::PI / FL(2.0);
     size_t idx = (((((size_t)d.isz * P.NSx + d.isx) * P.NSy + d.isy) * P.Ntheta + itheta) * P.NRz + iz) * P.NRr + ir;
+    if constexpr(RT == RT_ARR) {
+        sink.arr3(idx, cnst * w, phaseInt, delay, d, P);
+        sink.count();
+        return;
+    } else if constexpr(RT == RT_EIG) {
+        sink.eig3(itheta, ir, iz, d, P);
+        sink.count();
+        return;
+    }
     real omega = FL(2.0) * RC<real>::PI * P.freq0;
     float fr, fi;
     if(P.run0 == 'C') {

@@ -29,11 +29,18 @@ struct DevOut3 {
     uint32_t *seg_count;
     uint32_t seg_cap;
     uint32_t *overflow;
+    ArrCand *cand;              // arrivals: [receivers][cand_cap] raw AddArr calls
+    int32_t *ncand;
+    int32_t cand_cap;
+    int32_t eigen_memsize;
+    bhc_eigen_hit *hits;        // eigen hits
+    int32_t *neigen;
 };
 
 struct Sink3 {
     float *field;
     unsigned long long nc;
+    const DevOut3 *O;
     __device__ void add3(size_t idx, float re, float im)
     {
         unsafeAtomicAdd(field + 2 * idx, re);
@@ -41,6 +48,51 @@ struct Sink3 {
     }
     __device__ void count() { ++nc; }
     __device__ void mark() {}
+    // AddArr (arrivals.hpp:45-130) as a candidate keyed by (ray, step); the
+    // merge kernel replays the single-threaded rule per receiver. Angles as
+    // RayInit (trace.hpp:99) and ReceiverAngles (influence.hpp:123-136, 3D).
+    template<typename real>
+    __device__ void arr3(size_t idx, real amp, real phase, cpx<real> delay, const Seg3D<real> &d, const Prob3D<real> &P)
+    {
+        int slot = atomicAdd(O->ncand + idx, 1);
+        if(slot >= O->cand_cap) return;
+        int ia, ib, isy, isx, isz;
+        JobIndices3(d.job, P, ia, ib, isy, isx, isz);
+        const real RadDeg = RL(180.0) / RC<real>::PI;
+        ArrCand c;
+        c.key = ((long long)d.job << 17) | (long long)d.is;
+        c.a = (double)amp;
+        c.phase = (double)phase;
+        c.dre = (double)delay.re;
+        c.dim = (double)delay.im;
+        c.src = (float)(RadDeg * P.base[P.o_alpha + ia]);
+        c.srcaz = (float)(RadDeg * P.base[P.o_beta + ib]);
+        c.rcv = (float)(RadDeg * std::atan2(d.ta.z, std::sqrt(d.ta.x * d.ta.x + d.ta.y * d.ta.y)));
+        c.rcvaz = (float)(RadDeg * std::atan2(d.ta.y, d.ta.x));
+        c.ntop = d.ntop;
+        c.nbot = d.nbot;
+        O->cand[idx * (size_t)O->cand_cap + slot] = c;
+    }
+    // RecordEigenHit, eigenrays.hpp:24-41
+    template<typename real>
+    __device__ void eig3(int itheta, int ir, int iz, const Seg3D<real> &d, const Prob3D<real> &P)
+    {
+        int mi = atomicAdd(O->neigen, 1);
+        if(mi >= O->eigen_memsize) return;
+        int ia, ib, isy, isx, isz;
+        JobIndices3(d.job, P, ia, ib, isy, isx, isz);
+        bhc_eigen_hit h;
+        h.is = d.is;
+        h.itheta = itheta;
+        h.ir = ir;
+        h.iz = iz;
+        h.isx = isx;
+        h.isy = isy;
+        h.isz = isz;
+        h.ialpha = ia;
+        h.ibeta = ib;
+        O->hits[mi] = h;
+    }
 };
 
 template<typename real> __device__ inline void seal3(const SegBlock &B, int lane, const DevOut3 &O)
@@ -136,6 +188,7 @@ template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<r
         if(active) {
             two = RayUpdate3(p0, p1, p2, S, P, e);
             prelude3(d, p0, p1, I, is, P, isz, isx, isy);
+            d.job = job;
             ++is;
         }
         append3(d, lane, O, blk);
@@ -143,6 +196,7 @@ template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<r
             d.nitems = 0;
             if(active && two) {
                 prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                d.job = job;
                 ++is;
             }
             append3(d, lane, O, blk);
@@ -175,7 +229,8 @@ template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<r
 // Scatter kernel: one wavefront per segment. Rows are the (range, bearing)
 // pairs the segment's receiver walk visits (influence.hpp:1298-1386, 3D);
 // each row's depth window is flattened over the lanes.
-template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
+template<typename real, bool FAST, int RT = RT_TL>
+__global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
 {
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
@@ -199,7 +254,7 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
     int *cand = cand_all[(threadIdx.x / WAVE) & 3];
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
-    Sink3 sink{P.field, 0};
+    Sink3 sink{P.field, 0, &O};
     const Seg3<real> *segs = reinterpret_cast<const Seg3<real> *>(O.segs);
     for(uint32_t si = (uint32_t)wave; si < n; si += (uint32_t)nwaves) {
         const Seg3D<real> d = segs[si].d;
@@ -250,7 +305,8 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
                 const int oexcl = __shfl(excl, own);
                 const real oxr = __shfl(xr, own), oyr = __shfl(yr, own);
                 const real ozmin = __shfl(zmin, own), ozmax = __shfl(zmax, own);
-                if(t < total) item3<real, Sink3, FAST>(d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, Rz, sink);
+                if(t < total)
+                    item3<real, Sink3, FAST, RT>(d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, Rz, sink);
             }
         }
     }
@@ -261,7 +317,7 @@ template<typename real, bool FAST> __global__ void __launch_bounds__(256) k_scat
 
 // Scatter with one lane per segment: each lane walks its own segment's rows
 // and depth windows serially (no cross-lane flattening).
-template<typename real, bool FAST>
+template<typename real, bool FAST, int RT = RT_TL>
 __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3 O)
 {
     extern __shared__ __align__(16) unsigned char lds3l[];
@@ -280,7 +336,7 @@ __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3
     }
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
-    Sink3 sink{P.field, 0};
+    Sink3 sink{P.field, 0, &O};
     const Seg3<real> *segs = reinterpret_cast<const Seg3<real> *>(O.segs);
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t si = blockIdx.x * blockDim.x + threadIdx.x; si < n; si += stride) {
@@ -295,7 +351,8 @@ __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3
                 if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
                 int i0 = 0, i1 = P.NRz - 1;
                 if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(Rz, P.NRz, wlo, whi, i0, i1);
-                for(int iz = i0; iz <= i1; ++iz) item3<real, Sink3, FAST>(d, ir, it, iz, xr, yr, zmin, zmax, P, Rz, sink);
+                for(int iz = i0; iz <= i1; ++iz)
+                    item3<real, Sink3, FAST, RT>(d, ir, it, iz, xr, yr, zmin, zmax, P, Rz, sink);
             }
         }
     }
@@ -427,9 +484,16 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     double *d_xend = nullptr;
     unsigned long long *d_cnt = nullptr;
     uint32_t *d_errw = nullptr;
+    ArrCand *d_cand = nullptr;
+    int32_t *d_ncand = nullptr, *d_narr = nullptr, *d_neigen = nullptr;
+    bhc_arrival *d_arr = nullptr;
+    bhc_eigen_hit *d_hits = nullptr;
+    const char r0 = p->RunType[0];
+    const int rt = (r0 == 'A' || r0 == 'a') ? RT_ARR : (r0 == 'E') ? RT_EIG : RT_TL;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(s);
-        void *ptrs[] = {d_blob, d_Rz, d_Sz, d_field, d_stats, d_work, d_xend, d_cnt, d_errw};
+        void *ptrs[] = {d_blob, d_Rz, d_Sz, d_field, d_stats, d_work, d_xend, d_cnt, d_errw,
+                        d_cand, d_ncand, d_narr, d_neigen, d_arr, d_hits};
         for(void *q : ptrs)
             if(q) (void)hipFree(q);
     };
@@ -467,6 +531,26 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     HTRY(hipMemsetAsync(d_field, 0, 2 * nrcv * sizeof(float), s));
     HTRY(hipMemsetAsync(d_cnt, 0, 2 * sizeof(unsigned long long), s));
     HTRY(hipMemsetAsync(d_errw, 0, 4 * sizeof(uint32_t), s));
+    int cand_cap = 0;
+    if(rt == RT_ARR) {
+        // candidate slots per receiver: 16 GiB worth, raised to 8 x MaxNArr when
+        // that still fits 64 GiB; a receiver with more candidates than slots
+        // keeps the first ones and raises the out-of-memory warning
+        size_t per = nrcv * sizeof(ArrCand);
+        size_t cap = std::max<size_t>(1, std::min<size_t>(((size_t)16 << 30) / per, 65536));
+        size_t want = 8 * (size_t)out->max_narr;
+        if(cap < want && want * per <= ((size_t)64 << 30)) cap = want;
+        cand_cap = (int)cap;
+        TRY(dalloc(&d_cand, nrcv * cap));
+        TRY(dalloc(&d_ncand, nrcv));
+        TRY(dalloc(&d_arr, nrcv * (size_t)out->max_narr));
+        TRY(dalloc(&d_narr, nrcv));
+        HTRY(hipMemsetAsync(d_ncand, 0, nrcv * sizeof(int32_t), s));
+    } else if(rt == RT_EIG) {
+        TRY(dalloc(&d_hits, (size_t)out->eigen_memsize));
+        TRY(dalloc(&d_neigen, 1));
+        HTRY(hipMemsetAsync(d_neigen, 0, sizeof(int32_t), s));
+    }
     P.base = d_blob;
     P.s.base = d_blob;
     P.Rz = d_Rz;
@@ -486,6 +570,12 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     O.seg_count = c->d_seg_count;
     O.overflow = c->d_overflow;
     O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg3<real>), 0xFFFFFFF0u);
+    O.cand = d_cand;
+    O.ncand = d_ncand;
+    O.cand_cap = cand_cap;
+    O.hits = d_hits;
+    O.neigen = d_neigen;
+    O.eigen_memsize = rt == RT_EIG ? out->eigen_memsize : 0;
     // split the rays into passes that fit the segment buffer, from the
     // segments per ray of earlier runs on this context (a pass that still
     // overflows is halved and retraced)
@@ -501,7 +591,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     }
     c->passes = 0;
     bool first = true;
-    unsigned long long segs_total = 0;
+    unsigned long long segs_total = 0, steps_done = 0;
     while(!todo.empty()) {
         auto r = todo.back();
         todo.pop_back();
@@ -517,7 +607,9 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         hipLaunchKernelGGL((k_trace3d<real>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
         HTRY(hipGetLastError());
         uint32_t of = 0;
+        unsigned long long steps_now = 0;
         HTRY(hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HTRY(hipMemcpyAsync(&steps_now, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HTRY(hipStreamSynchronize(s));
         if(of) {
             if(nj <= 1) {
@@ -528,9 +620,12 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             int mid = r.first + nj / 2;
             todo.push_back({mid, r.second});
             todo.push_back({r.first, mid});
-            HTRY(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), s));
+            // drop the abandoned pass's steps, keep the completed passes'
+            HTRY(hipMemcpyAsync(d_cnt, &steps_done, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+            HTRY(hipStreamSynchronize(s));
             continue;
         }
+        steps_done = steps_now;
         {
             uint32_t ns = 0;
             HTRY(hipMemcpy(&ns, c->d_seg_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -539,7 +634,13 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         if(first) HTRY(hipEventRecord(c->evm, s));
         int sblocks = c->num_cus * 8;
         size_t lds = P.stage_tables ? ((size_t)P.NRr + 2 * (size_t)P.Ntheta) * sizeof(real) + (size_t)P.NRz * sizeof(float) : 0;
-        if(c->scatter3_lanes) {
+        if(rt == RT_ARR) {
+            if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
+        } else if(rt == RT_EIG) {
+            if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
+        } else if(c->scatter3_lanes) {
             if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d_lanes<real, false>), dim3(sblocks), dim3(256), lds, s, P, O);
             else hipLaunchKernelGGL((k_scatter3d_lanes<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
         } else {
@@ -550,9 +651,33 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         ++c->passes;
         first = false;
     }
+    if(rt == RT_ARR) {
+        // AddArr in the single-threaded order (or the first MaxNArr without merging)
+        const double om = 2.0 * M_PI * p->freq0;
+        const float omf = 2.0f * (float)M_PI * (float)p->freq0;
+        hipLaunchKernelGGL(k_arr_merge, dim3((unsigned)((nrcv + 255) / 256)), dim3(256), 0, s, d_cand, d_ncand, cand_cap,
+                           d_arr, d_narr, out->max_narr, (int)nrcv, out->allow_merging, omf, om,
+                           sizeof(real) == 8 ? 1 : 0, d_errw);
+        HTRY(hipGetLastError());
+    }
     HTRY(hipEventRecord(c->ev1, s));
     c->seg_per_ray3 = std::max(c->seg_per_ray3, (double)segs_total / (double)njobs);
-    if(out->field) HTRY(hipMemcpyAsync(out->field, d_field, 2 * nrcv * sizeof(float), hipMemcpyDeviceToHost, s));
+    if(rt == RT_TL && out->field)
+        HTRY(hipMemcpyAsync(out->field, d_field, 2 * nrcv * sizeof(float), hipMemcpyDeviceToHost, s));
+    if(rt == RT_ARR) {
+        HTRY(hipMemcpyAsync(out->arr, d_arr, nrcv * (size_t)out->max_narr * sizeof(bhc_arrival), hipMemcpyDeviceToHost,
+                            s));
+        HTRY(hipMemcpyAsync(out->narr, d_narr, nrcv * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    }
+    int32_t ne = 0;
+    if(rt == RT_EIG) {
+        HTRY(hipMemcpyAsync(&ne, d_neigen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HTRY(hipStreamSynchronize(s));
+        *out->neigen = ne;
+        int nh = std::min(ne, out->eigen_memsize);
+        if(nh > 0)
+            HTRY(hipMemcpyAsync(out->hits, d_hits, (size_t)nh * sizeof(bhc_eigen_hit), hipMemcpyDeviceToHost, s));
+    }
     uint32_t ew[4];
     unsigned long long cnt[2];
     HTRY(hipMemcpyAsync(ew, d_errw, sizeof(ew), hipMemcpyDeviceToHost, s));
@@ -591,7 +716,12 @@ extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_fiel
     if(!ctx || !prob || !out) return fail("bhc_field_run3d: null argument");
     if(prob->precision != 64 && prob->precision != 32) return fail("bhc_field_run3d: precision must be 32 or 64");
     char r0 = prob->RunType[0], r1 = prob->RunType[1];
-    if(r0 != 'C' && r0 != 'S' && r0 != 'I') return fail("bhc_field_run3d: run type '%c' is not a TL run type", r0);
+    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'A' && r0 != 'a' && r0 != 'E')
+        return fail("bhc_field_run3d: run type '%c' is not a TL, arrivals or eigenray run type", r0);
+    if((r0 == 'A' || r0 == 'a') && (!out->arr || !out->narr || out->max_narr < 1))
+        return fail("bhc_field_run3d: arrivals run needs arr, narr and max_narr >= 1");
+    if(r0 == 'E' && (!out->hits || !out->neigen || out->eigen_memsize < 1))
+        return fail("bhc_field_run3d: eigenray run needs hits, neigen and eigen_memsize >= 1");
     if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B')
         return fail("bhc_field_run3d: beam type '%c' is not a Cartesian geometric beam", r1);
     if(prob->NSx < 1 || prob->NSy < 1 || prob->NSz < 1 || prob->NRz < 1 || prob->NRr < 1 || prob->Ntheta < 1

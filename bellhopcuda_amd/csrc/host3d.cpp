@@ -24,7 +24,7 @@
 struct bhc_problem3d_owner {
     bhc_problem3d v;
     bhc_problem_owner *p2 = nullptr;
-    std::vector<double> z3, rho3, hx, hy, hc, hcz, Sx, Sy, trcv, beta, bcos, bsin;
+    std::vector<double> z3, rho3, hx, hy, hc, hcz, Sx, Sy, trcv, theta, beta, bcos, bsin;
     std::vector<bhc_bdry_node3d> ati, bty;
     ~bhc_problem3d_owner() { bhc_problem_free(p2); }
 };
@@ -209,6 +209,8 @@ template<typename real> int preprocess3d(const bhc_env3d *e, bhc_problem3d_owner
         o->trcv.push_back((double)s);
     }
     v.t_rcvr = o->trcv.data();
+    for(int i = 0; i < e->Ntheta; ++i) o->theta.push_back((double)(real)e->theta_deg[i]);
+    v.theta = o->theta.data();
     v.Nalpha = p.Nalpha;
     v.iSingle_alpha = p.iSingle_alpha;
     v.alpha = p.alpha;
@@ -253,12 +255,13 @@ extern "C" int bhc_preprocess3d(const bhc_env3d *e, bhc_problem3d_owner **out)
     if(e->NSx < 1 || e->NSy < 1 || e->NSz < 1 || e->NRz < 1 || e->NRr < 1 || e->Ntheta < 1 || e->Nalpha < 1
        || e->Nbeta < 1)
         return fail3("sources, receivers and launch angles must be non-empty");
-    // TL run types with Cartesian geometric beams (src/module/runtype.hpp maps
-    // '^' and ' ' to 'G') and the ray run type. Ray-centred 3D beams ('g',
-    // 'b'), arrivals and eigenrays are not on this 3D path.
-    if(e->RunType[0] != 'C' && e->RunType[0] != 'S' && e->RunType[0] != 'I' && e->RunType[0] != 'R')
-        return fail3("3D run type '%c' is not supported (TL run types C / S / I and the ray run type R)",
-                     e->RunType[0]);
+    // TL, arrivals ('A' ASCII / 'a' binary file), eigenray and ray run types with
+    // Cartesian geometric beams (src/module/runtype.hpp maps '^' and ' ' to
+    // 'G'). Ray-centred 3D beams ('g', 'b') and the combined arrivals +
+    // eigenray run types ('V', 'v') are not on this 3D path.
+    const char r0 = e->RunType[0];
+    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'R' && r0 != 'A' && r0 != 'a' && r0 != 'E')
+        return fail3("3D run type '%c' is not supported (C / S / I TL, A / a arrivals, E eigenrays, R rays)", r0);
     if(e->RunType[1] != 'G' && e->RunType[1] != '^' && e->RunType[1] != ' ' && e->RunType[1] != 'B')
         return fail3("3D beam type '%c' is not supported (Cartesian geometric beams G / B)", e->RunType[1]);
     if(e->ssp_type != 'C' && e->ssp_type != 'N' && e->ssp_type != 'S' && e->ssp_type != 'P' && e->ssp_type != 'H')

@@ -66,7 +66,8 @@ constexpr int WAVE = 64;
 struct ArrCand {
     long long key;
     double a, phase, dre, dim; // full precision: AddArr compares these before rounding
-    float src, rcv;
+    float src, rcv;             // source / receiver declination (deg)
+    float srcaz, rcvaz;         // source / receiver azimuth (deg; NaN in 2D)
     int32_t ntop, nbot;
 };
 
@@ -123,6 +124,8 @@ template<typename real> struct DevSink {
         c.dim = (double)delay.im;
         c.src = (float)src;
         c.rcv = (float)rcv;
+        c.srcaz = NAN;
+        c.rcvaz = NAN;
         c.ntop = ntop;
         c.nbot = nbot;
         O->cand[idx * (size_t)O->cand_cap + slot] = c;
@@ -744,9 +747,9 @@ __global__ void __launch_bounds__(256) k_arr_merge(const ArrCand *cand, const in
             a.delay[0] = (float)v.dre;
             a.delay[1] = (float)v.dim;
             a.SrcDeclAngle = v.src;
-            a.SrcAzimAngle = NAN;
+            a.SrcAzimAngle = v.srcaz;
             a.RcvrDeclAngle = v.rcv;
-            a.RcvrAzimAngle = NAN;
+            a.RcvrAzimAngle = v.rcvaz;
             a.NTopBnc = v.ntop;
             a.NBotBnc = v.nbot;
             continue;
@@ -785,9 +788,9 @@ __global__ void __launch_bounds__(256) k_arr_merge(const ArrCand *cand, const in
             a.delay[0] = (float)v.dre;
             a.delay[1] = (float)v.dim;
             a.SrcDeclAngle = v.src;
-            a.SrcAzimAngle = NAN;
+            a.SrcAzimAngle = v.srcaz;
             a.RcvrDeclAngle = v.rcv;
-            a.RcvrAzimAngle = NAN;
+            a.RcvrAzimAngle = v.rcvaz;
             a.NTopBnc = v.ntop;
             a.NBotBnc = v.nbot;
         } else {
@@ -799,9 +802,9 @@ __global__ void __launch_bounds__(256) k_arr_merge(const ArrCand *cand, const in
             a.delay[1] = w1 * a.delay[1] + w2 * (float)v.dim;
             a.a = AmpTot;
             a.SrcDeclAngle = w1 * a.SrcDeclAngle + w2 * v.src;
-            a.SrcAzimAngle = w1 * a.SrcAzimAngle + w2 * (float)NAN;
+            a.SrcAzimAngle = w1 * a.SrcAzimAngle + w2 * v.srcaz;
             a.RcvrDeclAngle = w1 * a.RcvrDeclAngle + w2 * v.rcv;
-            a.RcvrAzimAngle = w1 * a.RcvrAzimAngle + w2 * (float)NAN;
+            a.RcvrAzimAngle = w1 * a.RcvrAzimAngle + w2 * v.rcvaz;
         }
     }
     narr[r] = Nt;

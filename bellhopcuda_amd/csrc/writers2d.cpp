@@ -485,6 +485,160 @@ extern "C" int bhc_write_ray3d(const char *path, const bhc_problem3d *prob, cons
                                  : write_ray3<float>(path, prob, title, pts, nsteps, max_points, nrays);
 }
 
+// Receiver index of the 3D field / arrival arrays (GetFieldAddr, src/common.hpp:393-405).
+static size_t rcv_index3(const bhc_problem3d *p, int isz, int isx, int isy, int itheta, int iz, int ir)
+{
+    return (((((size_t)isz * p->NSx + isx) * p->NSy + isy) * p->Ntheta + itheta) * p->NRz + iz) * p->NRr + ir;
+}
+
+extern "C" int bhc_postprocess_arrivals3d(const bhc_problem3d *p, bhc_arrival *arr, int32_t *narr, int32_t max_narr,
+                                          int32_t *maxn)
+{
+    (void)arr; // 3D arrivals keep their amplitudes (factor 1)
+    if(!p || !narr || !maxn || max_narr < 1) return -1;
+    for(int isz = 0; isz < p->NSz; ++isz)
+        for(int isx = 0; isx < p->NSx; ++isx)
+            for(int isy = 0; isy < p->NSy; ++isy) {
+                int32_t m = 0;
+                for(int it = 0; it < p->Ntheta; ++it)
+                    for(int iz = 0; iz < p->NRz; ++iz)
+                        for(int ir = 0; ir < p->NRr; ++ir) {
+                            size_t b = rcv_index3(p, isz, isx, isy, it, iz, ir);
+                            if(narr[b] > max_narr) narr[b] = max_narr;
+                            m = std::max(m, narr[b]);
+                        }
+                maxn[((size_t)isz * p->NSx + isx) * p->NSy + isy] = m;
+            }
+    return 0;
+}
+
+// 3D .arr: the source and receiver coordinate vectors keep their types
+// (Sx / Sy / Rr / theta real, Sz / Rz float); the ASCII phase is written in the
+// real type (RadDeg * Phase), the binary one as float.
+template<typename R> static int write_arr3(const char *path, const bhc_problem3d *p, const bhc_arrival *arr,
+                                           const int32_t *narr, int32_t max_narr, const int32_t *maxn, bool binary)
+{
+    const R RadDeg = (R)180 / (R)M_PI;
+    if(!binary) {
+        LDWriter o;
+        if(!o.open(path)) return -1;
+        auto real_item = [&](double v) {
+            if(sizeof(R) == 8) o.f64(v);
+            else o.f32((float)v);
+        };
+        o.str("3D");
+        o.ch('\n');
+        real_item(p->freq0);
+        o.ch('\n');
+        o.i32(p->NSx);
+        for(int i = 0; i < p->NSx; ++i) real_item(p->Sx[i]);
+        o.ch('\n');
+        o.i32(p->NSy);
+        for(int i = 0; i < p->NSy; ++i) real_item(p->Sy[i]);
+        o.ch('\n');
+        o.i32(p->NSz);
+        for(int i = 0; i < p->NSz; ++i) o.f32(p->Sz[i]);
+        o.ch('\n');
+        o.i32(p->NRz);
+        for(int i = 0; i < p->NRz; ++i) o.f32(p->Rz[i]);
+        o.ch('\n');
+        o.i32(p->NRr);
+        for(int i = 0; i < p->NRr; ++i) real_item(p->Rr[i]);
+        o.ch('\n');
+        o.i32(p->Ntheta);
+        for(int i = 0; i < p->Ntheta; ++i) real_item(p->theta[i]);
+        o.ch('\n');
+        for(int isz = 0; isz < p->NSz; ++isz)
+            for(int isx = 0; isx < p->NSx; ++isx)
+                for(int isy = 0; isy < p->NSy; ++isy) {
+                    o.i32(maxn[((size_t)isz * p->NSx + isx) * p->NSy + isy]);
+                    o.ch('\n');
+                    for(int it = 0; it < p->Ntheta; ++it)
+                        for(int iz = 0; iz < p->NRz; ++iz)
+                            for(int ir = 0; ir < p->NRr; ++ir) {
+                                size_t b = rcv_index3(p, isz, isx, isy, it, iz, ir);
+                                o.i32(narr[b]);
+                                o.ch('\n');
+                                for(int k = 0; k < narr[b]; ++k) {
+                                    const bhc_arrival &a = arr[b * max_narr + k];
+                                    o.f32(a.a);
+                                    real_item((double)(RadDeg * (R)a.Phase));
+                                    o.f32(a.delay[0]);
+                                    o.f32(a.delay[1]);
+                                    o.f32(a.SrcDeclAngle);
+                                    o.f32(a.SrcAzimAngle);
+                                    o.f32(a.RcvrDeclAngle);
+                                    o.f32(a.RcvrAzimAngle);
+                                    o.i32(a.NTopBnc);
+                                    o.i32(a.NBotBnc);
+                                    o.ch('\n');
+                                }
+                            }
+                }
+        return o.close() ? 0 : -1;
+    }
+    UnfWriter o;
+    if(!o.open(path)) return -1;
+    o.rec();
+    o.bytes("'3D'", 4);
+    o.rec();
+    o.v((float)p->freq0);
+    o.rec();
+    o.v((int32_t)p->NSx);
+    for(int i = 0; i < p->NSx; ++i) o.v((R)p->Sx[i]);
+    o.rec();
+    o.v((int32_t)p->NSy);
+    for(int i = 0; i < p->NSy; ++i) o.v((R)p->Sy[i]);
+    o.rec();
+    o.v((int32_t)p->NSz);
+    o.bytes(p->Sz, sizeof(float) * p->NSz);
+    o.rec();
+    o.v((int32_t)p->NRz);
+    o.bytes(p->Rz, sizeof(float) * p->NRz);
+    o.rec();
+    o.v((int32_t)p->NRr);
+    for(int i = 0; i < p->NRr; ++i) o.v((R)p->Rr[i]);
+    o.rec();
+    o.v((int32_t)p->Ntheta);
+    for(int i = 0; i < p->Ntheta; ++i) o.v((R)p->theta[i]);
+    for(int isz = 0; isz < p->NSz; ++isz)
+        for(int isx = 0; isx < p->NSx; ++isx)
+            for(int isy = 0; isy < p->NSy; ++isy) {
+                o.rec();
+                o.v((int32_t)maxn[((size_t)isz * p->NSx + isx) * p->NSy + isy]);
+                for(int it = 0; it < p->Ntheta; ++it)
+                    for(int iz = 0; iz < p->NRz; ++iz)
+                        for(int ir = 0; ir < p->NRr; ++ir) {
+                            size_t b = rcv_index3(p, isz, isx, isy, it, iz, ir);
+                            o.rec();
+                            o.v((int32_t)narr[b]);
+                            for(int k = 0; k < narr[b]; ++k) {
+                                const bhc_arrival &a = arr[b * max_narr + k];
+                                o.rec();
+                                o.v(a.a);
+                                o.v((float)(RadDeg * (R)a.Phase));
+                                o.v(a.delay[0]);
+                                o.v(a.delay[1]);
+                                o.v(a.SrcDeclAngle);
+                                o.v(a.SrcAzimAngle);
+                                o.v(a.RcvrDeclAngle);
+                                o.v(a.RcvrAzimAngle);
+                                o.v((float)a.NTopBnc);
+                                o.v((float)a.NBotBnc);
+                            }
+                        }
+            }
+    return o.close() ? 0 : -1;
+}
+
+extern "C" int bhc_write_arr3d(const char *path, const bhc_problem3d *prob, const bhc_arrival *arr,
+                               const int32_t *narr, int32_t max_narr, const int32_t *maxn, int binary)
+{
+    if(!path || !prob || !arr || !narr || !maxn || max_narr < 1) return -1;
+    return prob->precision == 64 ? write_arr3<double>(path, prob, arr, narr, max_narr, maxn, binary != 0)
+                                 : write_arr3<float>(path, prob, arr, narr, max_narr, maxn, binary != 0);
+}
+
 // The list-directed writer as a stream, for companion files in the same layout.
 struct bhc_ldfile {
     LDWriter w;

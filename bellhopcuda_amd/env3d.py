@@ -6,6 +6,7 @@ the `bhc_env3d` C struct of include/bhc_hip.h. Angle and receiver vectors are
 given already expanded (see `env.subtab`).
 
 Workload builders:
+  analysis3d_env() -- the reference's isovelocity 3D arrivals test case.
   c3_env()       -- 3D Gaussian-beam TL over a 40 x 40 km shelf with a canyon,
                     hexahedral Munk-perturbed sound speed, 181 x 1001 beams,
                     36 bearings x 101 depths x 1001 ranges to 20 km.
@@ -164,6 +165,24 @@ def free_env3d(nalpha=91, nbeta=37, ntheta=4, nrz=21, nrr=51, precision=64, run_
         Rr_km=subtab(0.0, r_max_km, nrr), theta_deg=bearings(ntheta),
         alpha_deg=subtab(-89.0, 89.0, nalpha), beta_deg=bearings(nbeta) - 180.0,
         run_type=run_type, deltas=100.0, box_x_km=r_max_km + 1, box_y_km=r_max_km + 1, box_z=5001.0)
+
+
+def analysis3d_env(precision=64, run_type="AG   3"):
+    """The reference's 3D arrivals test case (test/individual/analysis_3d/
+    analysis_3d.env): isovelocity 1000 m/s water 0-1500 m, vacuum surface,
+    near-matched halfspace bottom; 4000 Hz source at (0, 0, 100 m); receivers
+    at 100 m depth, 0.1-1.5 km on the 34 deg bearing; 3 declinations on
+    [-1, 1] deg x bearings 34 and 35 deg; geometric hat beams, deltas 1.7646 m,
+    box 1.65 km x 1.65 km x 1650 m."""
+    z = np.array([0.0, 100.0, 500.0, 1500.0])
+    return Env3D(
+        precision=precision, freq0=4000.0, ssp_type="C", atten_unit="F ", ssp_z=z, ssp_c=np.full(4, 1000.0),
+        top=Halfspace("V", depth=0.0),
+        bot=Halfspace("A", depth=1500.0, alphaR=1000.0, betaR=0.0, rho=0.9, alphaI=0.0, zTemp=1000.0),
+        Sz=np.array([100.0], np.float32), Rz=np.array([100.0], np.float32),
+        Rr_km=subtab(0.1, 1.5, 15), theta_deg=np.array([34.0]),
+        alpha_deg=subtab(-1.0, 1.0, 3), beta_deg=np.array([34.0, 35.0]),
+        run_type=run_type, deltas=1.7646, box_x_km=1.65, box_y_km=1.65, box_z=1650.0)
 
 
 def munk_env3d(nalpha=201, nbeta=5, beta=(-2.0, 2.0), ntheta=1, theta0=0.0, nrz=51, nrr=201, precision=64,

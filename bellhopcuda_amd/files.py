@@ -1,5 +1,5 @@
-"""Output files of the 2D path in the reference's layouts: writers (through
-libbhc_hip.so) and readers.
+"""Output files in the reference's layouts: writers (through libbhc_hip.so)
+and readers, 2D and (for arrivals) 3D.
 
 * .shd  -- Fortran direct-access TL file (src/mode/tl.cpp:38-101, 211-248;
            read back as ReadOutTL does, :263-363);
@@ -28,6 +28,30 @@ def postprocess_arrivals(problem, arr, narr, max_narr):
     if rc != 0:
         raise BhcError("postprocess_arrivals failed")
     return narr_c, maxn
+
+
+def postprocess_arrivals3d(problem, arr, narr, max_narr):
+    """PostProcessArrivals for a 3D problem in place (counts clamped, amplitudes
+    unscaled); returns (narr, maximum count per source [NSz * NSx * NSy])."""
+    v = problem.view
+    maxn = np.zeros(v.NSz * v.NSx * v.NSy, np.int32)
+    narr_c = np.ascontiguousarray(narr, np.int32)
+    rc = lib().bhc_postprocess_arrivals3d(C.byref(v), C.cast(arr, C.POINTER(_abi.Arrival)),
+                                          narr_c.ctypes.data_as(_abi.c_int32_p), max_narr,
+                                          maxn.ctypes.data_as(_abi.c_int32_p))
+    if rc != 0:
+        raise BhcError("postprocess_arrivals3d failed")
+    return narr_c, maxn
+
+
+def write_arr3d(path, problem, arr, narr, max_narr, maxn, binary=False):
+    narr_c = np.ascontiguousarray(narr, np.int32)
+    maxn_c = np.ascontiguousarray(maxn, np.int32)
+    rc = lib().bhc_write_arr3d(path.encode(), C.byref(problem.view), C.cast(arr, C.POINTER(_abi.Arrival)),
+                               narr_c.ctypes.data_as(_abi.c_int32_p), max_narr,
+                               maxn_c.ctypes.data_as(_abi.c_int32_p), 1 if binary else 0)
+    if rc != 0:
+        raise BhcError("writing %s failed" % path)
 
 
 def write_arr(path, problem, arr, narr, max_narr, maxn, binary=False):
@@ -155,6 +179,86 @@ def read_arr(path, binary=False):
             per.append(a)
         out.append(per)
     return {"freq0": freq0, "Sz": sz, "Rz": rz, "Rr": rr, "maxn": maxn, "arrivals": out}
+
+
+ARR3_DTYPE = np.dtype([("a", "f8"), ("phase_deg", "f8"), ("delay_re", "f8"), ("delay_im", "f8"),
+                       ("src_decl", "f8"), ("src_azim", "f8"), ("rcvr_decl", "f8"), ("rcvr_azim", "f8"),
+                       ("ntop", "i4"), ("nbot", "i4")])
+
+
+def read_arr3d(path, binary=False):
+    """3D arrivals file -> dict(freq0, Sx, Sy, Sz, Rz, Rr, theta, maxn,
+    arrivals), arrivals a list over sources [isz][isx][isy] (flattened) of lists
+    over receivers [itheta][iz][ir] (flattened) of ARR3_DTYPE record arrays."""
+    if not binary:
+        toks = open(path).read().split("\n")
+        pos = 0
+
+        def nxt():
+            nonlocal pos
+            v = toks[pos].split()
+            pos += 1
+            return v
+
+        assert nxt()[0] == "'3D'"
+        freq0 = float(nxt()[0])
+        vecs = []
+        for _ in range(6):
+            v = nxt()
+            vecs.append(np.array(v[1:int(v[0]) + 1], np.float64))
+        sx, sy, sz, rz, rr, th = vecs
+        nrcv = len(th) * len(rz) * len(rr)
+        out, maxn = [], []
+        for _ in range(len(sx) * len(sy) * len(sz)):
+            maxn.append(int(nxt()[0]))
+            per = []
+            for _ in range(nrcv):
+                n = int(nxt()[0])
+                a = np.zeros(n, ARR3_DTYPE)
+                for k in range(n):
+                    t = nxt()
+                    a[k] = tuple(float(x) for x in t[:8]) + (int(t[8]), int(t[9]))
+                per.append(a)
+            out.append(per)
+        return {"freq0": freq0, "Sx": sx, "Sy": sy, "Sz": sz, "Rz": rz, "Rr": rr, "theta": th, "maxn": maxn,
+                "arrivals": out}
+    raw = open(path, "rb").read()
+    recs, pos = [], 0
+    while pos < len(raw):
+        n = struct.unpack_from("<i", raw, pos)[0]
+        recs.append(raw[pos + 4:pos + 4 + n])
+        assert struct.unpack_from("<i", raw, pos + 4 + n)[0] == n
+        pos += n + 8
+    assert recs[0] == b"'3D'"
+    freq0 = struct.unpack("<f", recs[1])[0]
+
+    def vec(rec, float_only=False):
+        n = struct.unpack_from("<i", rec, 0)[0]
+        w = 4 if float_only else (len(rec) - 4) // max(n, 1)
+        return np.frombuffer(rec[4:4 + w * n], "<f8" if w == 8 else "<f4").astype(np.float64)
+
+    sx, sy = vec(recs[2]), vec(recs[3])
+    sz, rz = vec(recs[4], True), vec(recs[5], True)
+    rr, th = vec(recs[6]), vec(recs[7])
+    k = 8
+    nrcv = len(th) * len(rz) * len(rr)
+    out, maxn = [], []
+    for _ in range(len(sx) * len(sy) * len(sz)):
+        maxn.append(struct.unpack("<i", recs[k])[0])
+        k += 1
+        per = []
+        for _ in range(nrcv):
+            n = struct.unpack("<i", recs[k])[0]
+            k += 1
+            a = np.zeros(n, ARR3_DTYPE)
+            for j in range(n):
+                f = struct.unpack("<10f", recs[k])
+                k += 1
+                a[j] = tuple(f[:8]) + (int(f[8]), int(f[9]))
+            per.append(a)
+        out.append(per)
+    return {"freq0": freq0, "Sx": sx, "Sy": sy, "Sz": sz, "Rz": rz, "Rr": rr, "theta": th, "maxn": maxn,
+            "arrivals": out}
 
 
 def read_ray(path):

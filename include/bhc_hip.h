@@ -457,6 +457,7 @@ typedef struct bhc_problem3d {
     double deltas, box_x, box_y, box_z;
     int32_t NSBPPts, pad3;
     const double *sbp;
+    const double *theta;        /* [Ntheta] receiver bearings, deg (Position::theta) */
 } bhc_problem3d;
 
 typedef struct bhc_problem3d_owner bhc_problem3d_owner;
@@ -468,7 +469,10 @@ const bhc_problem3d *bhc_problem3d_view(const bhc_problem3d_owner *p);
 void bhc_problem3d_free(bhc_problem3d_owner *p);
 const char *bhc_host3d_last_error(void);
 
-/* 3D field run, RunFieldModesImpl<CFG, true, true> (src/mode/fieldimpl.hpp:27-28):
+/* 3D field run, RunFieldModesImpl<CFG, true, true> (src/mode/fieldimpl.hpp:27-28),
+ * for TL (C / S / I), arrivals (A / a: out->arr and out->narr over the receivers
+ * in field order, AddArr src/arrivals.hpp:45-130 with azimuths) and eigen hits
+ * (E: RecordEigenHit src/eigenrays.hpp:24-41 with itheta, isx, isy, ibeta):
  * out->field is [NSz][NSx][NSy][Ntheta][NRz][NRr] complex floats, caller-zeroed
  * (GetFieldAddr, src/common.hpp:393-405); per-ray statistics as in 2D, with
  * ray_xend holding [rays][3] end points. Rays are numbered like GetJobIndices
@@ -492,6 +496,17 @@ int bhc_write_ray3d(const char *path, const bhc_problem3d *prob, const char *tit
 /* ScalePressure for 3D geometric beams (src/influence.hpp:1556-1627): incoherent
  * and semicoherent fields become sqrt(intensity); coherent fields are unchanged. */
 int bhc_scale_pressure3d(const bhc_problem3d *prob, float *field);
+/* PostProcessArrivals for 3D (src/mode/arr.cpp:24-73 with R3D): counts clamped
+ * to max_narr, amplitudes unscaled, maximum count per source (isz, isx, isy)
+ * written to maxn[NSz * NSx * NSy]. */
+int bhc_postprocess_arrivals3d(const bhc_problem3d *prob, bhc_arrival *arr, int32_t *narr, int32_t max_narr,
+                               int32_t *maxn);
+/* 3D .arr file (WriteOutArrivals<true>, src/mode/arr.cpp:88-238): sources x / y
+ * / z, receivers z / r / bearings, then per source and per (bearing, depth,
+ * range) the arrivals with source and receiver azimuths; ASCII (binary = 0,
+ * run type 'A') or Fortran unformatted ('a'). */
+int bhc_write_arr3d(const char *path, const bhc_problem3d *prob, const bhc_arrival *arr, const int32_t *narr,
+                    int32_t max_narr, const int32_t *maxn, int binary);
 
 /* ---- post-processing and file layouts (host) ----------------------------------- */
 

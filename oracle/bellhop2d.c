@@ -1294,15 +1294,11 @@ static int IsAtCaustic(const InflRay *ir, real q)
     return (q <= RL(0.0) && ir->qOld > RL(0.0)) || (q >= RL(0.0) && ir->qOld < RL(0.0));
 }
 
-static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, const RayInitInfo *ri,
-                   real RcvrDeclAngle, int NumTopBnc, int NumBotBnc, Outs *O)
-{ /* arrivals.hpp:45-130 */
-    const Prob *P = O->P;
-    size_t base = ((size_t)ri->isz * P->NRz_per_range + id) * P->NRr + ir;
-    bhc_arrival *baseArr = &O->arr[base * O->max_narr];
-    _Atomic int32_t *baseNArr = &O->narr[base];
-    float RcvrAzimAngle = NAN;
-    if(O->allow_merging) {
+static void AddArrAt(bhc_arrival *baseArr, _Atomic int32_t *baseNArr, int max_narr, int allow_merging, real Amp,
+                     real omega, real Phase, cpx delay, real SrcDeclAngle, real SrcAzimAngle, real RcvrDeclAngle,
+                     real RcvrAzimAngle, int NumTopBnc, int NumBotBnc)
+{ /* arrivals.hpp:45-130 on one receiver's slots */
+    if(allow_merging) {
         int Nt = atomic_load(baseNArr);
         const float PhaseTol = FL(0.05);
         int second = Nt >= 1
@@ -1310,10 +1306,10 @@ static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, 
             && RABS(baseArr[Nt - 1].Phase - Phase) < PhaseTol;
         if(!second) {
             int iArr;
-            if(Nt >= O->max_narr) {
+            if(Nt >= max_narr) {
                 iArr = -1;
                 real weakest = Amp;
-                for(Nt = 0; Nt < O->max_narr; ++Nt) {
+                for(Nt = 0; Nt < max_narr; ++Nt) {
                     if(baseArr[Nt].a < weakest) {
                         weakest = baseArr[Nt].a;
                         iArr = Nt;
@@ -1328,10 +1324,10 @@ static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, 
             baseArr[iArr].Phase = (float)Phase;
             baseArr[iArr].delay[0] = (float)creal(delay);
             baseArr[iArr].delay[1] = (float)cimag(delay);
-            baseArr[iArr].SrcDeclAngle = (float)ri->SrcDeclAngle;
-            baseArr[iArr].SrcAzimAngle = NAN;
+            baseArr[iArr].SrcDeclAngle = (float)SrcDeclAngle;
+            baseArr[iArr].SrcAzimAngle = (float)SrcAzimAngle;
             baseArr[iArr].RcvrDeclAngle = (float)RcvrDeclAngle;
-            baseArr[iArr].RcvrAzimAngle = RcvrAzimAngle;
+            baseArr[iArr].RcvrAzimAngle = (float)RcvrAzimAngle;
             baseArr[iArr].NTopBnc = NumTopBnc;
             baseArr[iArr].NBotBnc = NumBotBnc;
         } else {
@@ -1342,25 +1338,34 @@ static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, 
             baseArr[Nt - 1].delay[0] = w1 * baseArr[Nt - 1].delay[0] + w2 * dr;
             baseArr[Nt - 1].delay[1] = w1 * baseArr[Nt - 1].delay[1] + w2 * di;
             baseArr[Nt - 1].a = AmpTot;
-            baseArr[Nt - 1].SrcDeclAngle = w1 * baseArr[Nt - 1].SrcDeclAngle + w2 * (float)ri->SrcDeclAngle;
-            baseArr[Nt - 1].SrcAzimAngle = w1 * baseArr[Nt - 1].SrcAzimAngle + w2 * (float)NAN;
+            baseArr[Nt - 1].SrcDeclAngle = w1 * baseArr[Nt - 1].SrcDeclAngle + w2 * (float)SrcDeclAngle;
+            baseArr[Nt - 1].SrcAzimAngle = w1 * baseArr[Nt - 1].SrcAzimAngle + w2 * (float)SrcAzimAngle;
             baseArr[Nt - 1].RcvrDeclAngle = w1 * baseArr[Nt - 1].RcvrDeclAngle + w2 * (float)RcvrDeclAngle;
-            baseArr[Nt - 1].RcvrAzimAngle = w1 * baseArr[Nt - 1].RcvrAzimAngle + w2 * RcvrAzimAngle;
+            baseArr[Nt - 1].RcvrAzimAngle = w1 * baseArr[Nt - 1].RcvrAzimAngle + w2 * (float)RcvrAzimAngle;
         }
     } else {
         int Nt = atomic_fetch_add(baseNArr, 1);
-        if(Nt >= O->max_narr) return;
+        if(Nt >= max_narr) return;
         baseArr[Nt].a = (float)Amp;
         baseArr[Nt].Phase = (float)Phase;
         baseArr[Nt].delay[0] = (float)creal(delay);
         baseArr[Nt].delay[1] = (float)cimag(delay);
-        baseArr[Nt].SrcDeclAngle = (float)ri->SrcDeclAngle;
-        baseArr[Nt].SrcAzimAngle = NAN;
+        baseArr[Nt].SrcDeclAngle = (float)SrcDeclAngle;
+        baseArr[Nt].SrcAzimAngle = (float)SrcAzimAngle;
         baseArr[Nt].RcvrDeclAngle = (float)RcvrDeclAngle;
-        baseArr[Nt].RcvrAzimAngle = RcvrAzimAngle;
+        baseArr[Nt].RcvrAzimAngle = (float)RcvrAzimAngle;
         baseArr[Nt].NTopBnc = NumTopBnc;
         baseArr[Nt].NBotBnc = NumBotBnc;
     }
+}
+
+static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, const RayInitInfo *ri,
+                   real RcvrDeclAngle, int NumTopBnc, int NumBotBnc, Outs *O)
+{ /* arrivals.hpp:45-130, 2D: no azimuths */
+    const Prob *P = O->P;
+    size_t base = ((size_t)ri->isz * P->NRz_per_range + id) * P->NRr + ir;
+    AddArrAt(&O->arr[base * O->max_narr], &O->narr[base], O->max_narr, O->allow_merging, Amp, omega, Phase, delay,
+             ri->SrcDeclAngle, (real)NAN, RcvrDeclAngle, (real)NAN, NumTopBnc, NumBotBnc);
 }
 
 static void RecordEigenHit(int ir, int iz, int is, const RayInitInfo *ri, Outs *O)

@@ -936,12 +936,34 @@ typedef struct {
 typedef struct {
     const Prob3 *Q;
     float *field;
+    bhc_arrival *arr;
+    _Atomic int32_t *narr;
+    int max_narr, allow_merging;
+    bhc_eigen_hit *hits;
+    _Atomic int32_t *neigen;
+    int eigen_memsize;
     int isGaussian;
     _Atomic long long contrib;
 } Outs3;
 
-static void InfluenceGeoCore3(real s, real n1, real n2, m2 dq, cpx dtau, int itheta, int ir, int iz,
-                              const RayPt3 *p0, const RayPt3 *p1, const Infl3 *inf, Outs3 *O)
+static size_t field_addr3(const Prob3 *Q, const RayInit3Info *ri, int itheta, int iz, int ir)
+{ /* GetFieldAddr, common.hpp:393-405 */
+    return ((((((size_t)ri->isz * Q->NSx + ri->isx) * Q->NSy + ri->isy) * Q->Ntheta + itheta)
+             * Q->P2.NRz_per_range + iz) * Q->P2.NRr) + ir;
+}
+
+static void RecordEigenHit3(int itheta, int ir, int iz, int is, const RayInit3Info *ri, Outs3 *O)
+{ /* eigenrays.hpp:24-41 */
+    int mi = atomic_fetch_add(O->neigen, 1);
+    if(mi >= O->eigen_memsize) return;
+    bhc_eigen_hit *h = &O->hits[mi];
+    h->is = is; h->iz = iz; h->ir = ir; h->itheta = itheta;
+    h->isx = ri->isx; h->isy = ri->isy; h->isz = ri->isz; h->ialpha = ri->ialpha; h->ibeta = ri->ibeta;
+}
+
+static void InfluenceGeoCore3(real s, real n1, real n2, m2 dq, cpx dtau, int itheta, int ir, int iz, int is,
+                              real RcvrDeclAngle, real RcvrAzimAngle, const RayPt3 *p0, const RayPt3 *p1,
+                              const Infl3 *inf, Outs3 *O)
 { /* influence.hpp:897-1024 (3D Cartesian) + ApplyContribution :432-473 */
     const Prob *P = &O->Q->P2;
     m2 qI = m2add(p0->q, m2scl(s, dq));
@@ -967,8 +989,20 @@ static void InfluenceGeoCore3(real s, real n1, real n2, m2 dq, cpx dtau, int ith
     if((qFinal <= RL(0.0) && inf->qOld > RL(0.0)) || (qFinal >= RL(0.0) && inf->qOld < RL(0.0)))
         phaseInt = phaseInt + REAL_PI / FL(2.0);
     atomic_fetch_add_explicit(&O->contrib, 1, memory_order_relaxed);
+    const char rt = P->RunType[0];
+    if(rt == 'E') {
+        RecordEigenHit3(itheta, ir, iz, is, &inf->init, O);
+        return;
+    }
+    if(rt == 'A' || rt == 'a') {
+        size_t b = field_addr3(O->Q, &inf->init, itheta, iz, ir);
+        AddArrAt(&O->arr[b * O->max_narr], &O->narr[b], O->max_narr, O->allow_merging, cnst * w, inf->omega,
+                 phaseInt, delay, inf->init.SrcDeclAngle, inf->init.SrcAzimAngle, RcvrDeclAngle, RcvrAzimAngle,
+                 p1->NumTopBnc, p1->NumBotBnc);
+        return;
+    }
     float fr, fi;
-    if(P->RunType[0] == 'C') {
+    if(rt == 'C') {
         cpx e = CEXP(-I * (inf->omega * delay - phaseInt));
         cpx d = (cnst * w) * e;
         fr = (float)creal(d);
@@ -980,9 +1014,7 @@ static void InfluenceGeoCore3(real s, real n1, real n2, m2 dq, cpx dtau, int ith
         fr = (float)v;
         fi = 0.0f;
     }
-    const Prob3 *Q = O->Q;
-    size_t base = ((((((size_t)inf->init.isz * Q->NSx + inf->init.isx) * Q->NSy + inf->init.isy) * Q->Ntheta + itheta)
-                    * P->NRz_per_range + iz) * P->NRr) + ir;
+    size_t base = field_addr3(O->Q, &inf->init, itheta, iz, ir);
     atomic_add_float(&O->field[2 * base], fr);
     atomic_add_float(&O->field[2 * base + 1], fi);
 }
@@ -1003,6 +1035,9 @@ static void Step_InfluenceGeoCart3(const RayPt3 *p0, const RayPt3 *p1, Infl3 *in
     rayt = v3divs(rayt, rlen);
     v3 rayn1, rayn2;
     RayNormalImpl(rayt, p1->phi, 1, RL(1.0), &rayn1, &rayn2);
+    /* ReceiverAngles, influence.hpp:123-136 (3D) */
+    real RcvrAzimAngle = RadDeg * RATAN2(rayt.y, rayt.x);
+    real RcvrDeclAngle = RadDeg * RATAN2(rayt.z, RSQRT(rayt.x * rayt.x + rayt.y * rayt.y));
     m2 dq = m2sub(p1->q, p0->q);
     cpx dtau = p1->tau - p0->tau;
     real phaseq = m2det(p0->q);
@@ -1039,7 +1074,8 @@ static void Step_InfluenceGeoCart3(const RayPt3 *p0, const RayPt3 *p1, Infl3 *in
                     real s = v3dot(xrr, rayt) / rlen;
                     real n1 = RABS(v3dot(xrr, rayn1));
                     real n2 = RABS(v3dot(xrr, rayn2));
-                    InfluenceGeoCore3(s, n1, n2, dq, dtau, itheta, inf->ir, iz, p0, p1, inf, O);
+                    InfluenceGeoCore3(s, n1, n2, dq, dtau, itheta, inf->ir, iz, is, RcvrDeclAngle, RcvrAzimAngle, p0,
+                                      p1, inf, O);
                 }
             }
         }
@@ -1167,7 +1203,7 @@ static void *field_worker3(void *arg)
 int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthreads, int job_begin, int job_end)
 {
     char rt0 = env->RunType[0], rt1 = env->RunType[1];
-    if(rt0 != 'C' && rt0 != 'S' && rt0 != 'I') return -1;
+    if(rt0 != 'C' && rt0 != 'S' && rt0 != 'I' && rt0 != 'A' && rt0 != 'a' && rt0 != 'E') return -1;
     if(rt1 != 'G' && rt1 != '^' && rt1 != ' ' && rt1 != 'B') return -1;
     Prob3 Q;
     if(prob3_from_env(env, &Q)) return -1;
@@ -1175,6 +1211,15 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
     memset(&O, 0, sizeof(O));
     O.Q = &Q;
     O.field = out->field;
+    O.arr = out->arr;
+    O.narr = (_Atomic int32_t *)out->narr;
+    O.max_narr = out->max_narr;
+    O.allow_merging = out->allow_merging;
+    O.hits = out->hits;
+    O.neigen = (_Atomic int32_t *)out->neigen;
+    O.eigen_memsize = out->eigen_memsize;
+    if((rt0 == 'A' || rt0 == 'a') && (!O.arr || !O.narr || O.max_narr < 1)) { prob3_free(&Q); return -1; }
+    if(rt0 == 'E' && (!O.hits || !O.neigen)) { prob3_free(&Q); return -1; }
     O.isGaussian = (rt1 == 'B');
     atomic_init(&O.contrib, 0);
     ErrState err;
@@ -1196,6 +1241,7 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
     J.job_end = job_end;
     J.stats = stats;
     nthreads = resolve_threads(nthreads);
+    if((rt0 == 'A' || rt0 == 'a') && O.allow_merging) nthreads = 1; /* src/mode/arr.hpp:79 */
     pthread_t *th = xcalloc(nthreads, sizeof(pthread_t));
     for(int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, field_worker3, &J);
     for(int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);

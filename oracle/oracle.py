@@ -154,22 +154,41 @@ def _lib3():
     return L
 
 
-def field_run3d(env, nthreads=0, job_begin=0, job_end=-1):
+def field_run3d(env, nthreads=0, job_begin=0, job_end=-1, max_narr=0, allow_merging=1, eigen_memsize=0):
     """Run the 3D oracle on an Env3D. Returns a FieldResult with .field
     (complex64, shape env.field_shape, pre-ScalePressure) and per-ray stats
-    (.nsteps, .ntop, .nbot, .xend [rays, 3])."""
+    (.nsteps, .ntop, .nbot, .xend [rays, 3]); arrivals runs ('A', 'a') fill
+    .arr / .narr ([receivers][max_narr], receivers in field order), eigen
+    runs ('E') .hits / .neigen."""
     L = _lib3()
     ec, keep = env.to_c()
     nj = env.nrays
     if job_end < 0 or job_end > nj:
         job_end = nj
     n = job_end - job_begin
-    fld = np.zeros(int(np.prod(env.field_shape)) * 2, np.float32)
+    nrcv = int(np.prod(env.field_shape))
+    fld = np.zeros(nrcv * 2, np.float32)
+    r = FieldResult()
+    out = _abi.FieldOut()
+    rt = env.run_type[0]
+    if rt in "Aa":
+        r.max_narr = max_narr
+        r.arr = (_abi.Arrival * (nrcv * max_narr))()
+        r.narr = np.zeros(nrcv, np.int32)
+        out.arr = C.cast(r.arr, C.POINTER(_abi.Arrival))
+        out.narr = r.narr.ctypes.data_as(_abi.c_int32_p)
+        out.max_narr = max_narr
+        out.allow_merging = allow_merging
+    if rt == "E":
+        r.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
+        r.neigen = np.zeros(1, np.int32)
+        out.hits = C.cast(r.hits, C.POINTER(_abi.EigenHit))
+        out.neigen = r.neigen.ctypes.data_as(_abi.c_int32_p)
+        out.eigen_memsize = eigen_memsize
     ns = np.zeros(max(n, 1), np.int32)
     nt = np.zeros(max(n, 1), np.int32)
     nb = np.zeros(max(n, 1), np.int32)
     xe = np.zeros((max(n, 1), 3), np.float64)
-    out = _abi.FieldOut()
     out.field = fld.ctypes.data_as(_abi.c_float_p)
     out.ray_nsteps = ns.ctypes.data_as(_abi.c_int32_p)
     out.ray_ntop = nt.ctypes.data_as(_abi.c_int32_p)
@@ -179,7 +198,6 @@ def field_run3d(env, nthreads=0, job_begin=0, job_end=-1):
     rc = f(C.byref(ec), C.byref(out), nthreads, job_begin, job_end)
     if rc != 0:
         raise RuntimeError("oracle_field_run3d failed (unsupported 3D environment)")
-    r = FieldResult()
     r.field = fld.view(np.complex64).reshape(env.field_shape)
     r.nsteps, r.ntop, r.nbot, r.xend = ns[:n], nt[:n], nb[:n], xe[:n]
     r.total_steps, r.total_contrib = out.total_steps, out.total_contrib

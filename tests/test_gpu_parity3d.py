@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import bellhopcuda_amd as bh
-from bellhopcuda_amd.env3d import free_env3d, munk_env3d, c3_env
+from bellhopcuda_amd.env3d import free_env3d, munk_env3d, c3_env, analysis3d_env
 import oracle
 from parity import compare_field, tl_db
 
@@ -105,3 +105,62 @@ def test_ray_mode3d_parity(name, gpu_ctx):
         for f in ("p", "q"):
             a, b = ao[k, :n][f], ag[k, :n][f]
             assert np.allclose(a, b, rtol=tol, atol=tol * max(1.0, float(np.max(np.abs(a))))), (k, f)
+
+
+ARR_CASES = {
+    # (env builder, max_narr, allow_merging, oracle threads)
+    "analysis3d_hat_f64": (lambda: analysis3d_env(), 10, 1, 1),
+    "analysis3d_hat_f32": (lambda: analysis3d_env(precision=32), 10, 1, 1),
+    "munk_gauss_merging": (lambda: munk_env3d(nalpha=61, nbeta=5, beta=(-20.0, 20.0), ntheta=2, nrz=11, nrr=21,
+                                              run_type="AB   3"), 20, 1, 1),
+    "munk_hat_first_maxnarr": (lambda: munk_env3d(nalpha=61, nbeta=5, beta=(-20.0, 20.0), ntheta=2, nrz=11,
+                                                  nrr=21, run_type="aG   3"), 6, 0, 1),
+    "shelf_canyon_gauss_merging": (lambda: c3_env(nalpha=31, nbeta=12, ntheta=4, nrz=11, nrr=21, nxy=41,
+                                                  run_type="AB   3"), 30, 1, 1),
+}
+
+
+@pytest.mark.parametrize("name", list(ARR_CASES))
+def test_arrivals3d_parity(name, gpu_ctx):
+    """3D arrivals: candidates merged per receiver in (ray, step) order replay
+    the single-threaded AddArr, so counts, amplitudes, phases, delays and
+    bounces equal the oracle's (FP64; angles within 1e-6 relative, device
+    atan2); FP32 runs match counts and bounces and the values to 1e-4."""
+    print("\n[gpu-case-3d-arr]", name, flush=True)
+    mk, max_narr, merging, nth = ARR_CASES[name]
+    env = mk()
+    o = oracle.field_run3d(env, nthreads=nth, max_narr=max_narr, allow_merging=merging)
+    g = bh.field_run3d(env, ctx=gpu_ctx, max_narr=max_narr, allow_merging=merging)
+    assert g.err == o.err and g.total_contrib == o.total_contrib
+    gn = np.minimum(g.narr, max_narr)
+    on = np.minimum(o.narr, max_narr)
+    assert np.array_equal(gn, on)
+    assert on.sum() > 0
+    A = np.ctypeslib.as_array(o.arr).reshape(-1, max_narr)
+    B = np.ctypeslib.as_array(g.arr).reshape(-1, max_narr)
+    fp64 = env.precision == 64
+    for r in np.nonzero(on)[0]:
+        a, b = A[r, :on[r]], B[r, :on[r]]
+        assert np.array_equal(a["NTopBnc"], b["NTopBnc"]) and np.array_equal(a["NBotBnc"], b["NBotBnc"]), r
+        for f in ("a", "Phase", "delay"):
+            if fp64:
+                assert np.array_equal(a[f], b[f]), (r, f)
+            else:
+                assert np.allclose(a[f], b[f], rtol=1e-4, atol=1e-9), (r, f)
+        for f in ("SrcDeclAngle", "SrcAzimAngle", "RcvrDeclAngle", "RcvrAzimAngle"):
+            assert np.allclose(a[f], b[f], rtol=1e-6 if fp64 else 1e-4, atol=1e-5), (r, f)
+
+
+def test_eigen_hits3d_parity(gpu_ctx):
+    """3D eigen hits: the same set of (step, bearing, range, depth, ray) hits as
+    the oracle, in any order (both record them with an atomic counter)."""
+    print("\n[gpu-case-3d-eig] munk_eigen", flush=True)
+    env = munk_env3d(nalpha=61, nbeta=5, beta=(-20.0, 20.0), ntheta=2, nrz=11, nrr=21, run_type="EG   3")
+    o = oracle.field_run3d(env, eigen_memsize=200000)
+    g = bh.field_run3d(env, ctx=gpu_ctx, eigen_memsize=200000)
+    no, ng = int(o.neigen[0]), int(g.neigen[0])
+    assert no == ng > 0
+    key = ("is_", "itheta", "ir", "iz", "isx", "isy", "isz", "ialpha", "ibeta")
+    ho = np.sort(np.ctypeslib.as_array(o.hits)[:no], order=list(key))
+    hg = np.sort(np.ctypeslib.as_array(g.hits)[:ng], order=list(key))
+    assert np.array_equal(ho, hg)

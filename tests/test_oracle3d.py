@@ -13,12 +13,20 @@ What pins the 3D oracle (oracle/bellhop3d.c):
   * a 3D bathymetry grid that is flat reproduces the flat default boundary;
   * a hexahedral grid with no lateral variation reproduces the same field as
     the equivalent c-linear depth profile.
+  * arrivals: the reference's own 3D arrivals fixture
+    (test/individual/analysis_3d/example_analysis_3d.arr.txt, copied to
+    tests/golden/) is reproduced end to end -- oracle AddArr with azimuths,
+    PostProcessArrivals 3D, the 3D .arr writer -- to within one float ulp of
+    every amplitude and delay (the fixture's receiver ranges are single
+    precision, e.g. 900.000061 m) and exactly in counts, phases, angles and
+    bounces; eigen hits bracket the receivers they name.
 The product's 3D core (core3d.h, run serially by tests/harness/core_cpu.cpp
 with the scatter kernel's row / depth-window enumeration) is compared with
 the oracle bit for bit on trajectories and contribution counts. Its
 preprocessing (host3d.cpp) is compared value for value.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -26,7 +34,8 @@ import pytest
 import bellhopcuda_amd as bh
 from bellhopcuda_amd import _abi
 from bellhopcuda_amd.env import munk_env, munk_c
-from bellhopcuda_amd.env3d import free_env3d, munk_env3d, c3_env, Grid3D
+from bellhopcuda_amd import files
+from bellhopcuda_amd.env3d import free_env3d, munk_env3d, c3_env, analysis3d_env, Grid3D
 import oracle
 from parity import tl_db
 
@@ -192,7 +201,7 @@ def test_preprocess3d_matches_oracle(precision):
 
 
 def test_preprocess3d_rejects_unsupported():
-    for rt in ("Cg   3", "Cb   3", "AB   3", "EB   3"):
+    for rt in ("Cg   3", "Cb   3", "VB   3", "vB   3"):
         with pytest.raises(bh.BhcError):
             bh.Problem3D(free_env3d(run_type=rt))
     e = free_env3d()
@@ -204,7 +213,8 @@ def test_preprocess3d_rejects_unsupported():
 def test_abi_exports_3d_symbols():
     L = bh.lib()
     for name in ("bhc_preprocess3d", "bhc_problem3d_view", "bhc_problem3d_free", "bhc_host3d_last_error",
-                 "bhc_field_run3d", "bhc_scale_pressure3d", "bhc_opcount3d", "bhc_ray_run3d", "bhc_write_ray3d"):
+                 "bhc_field_run3d", "bhc_scale_pressure3d", "bhc_opcount3d", "bhc_ray_run3d", "bhc_write_ray3d",
+                 "bhc_postprocess_arrivals3d", "bhc_write_arr3d"):
         assert hasattr(L, name)
 
 
@@ -277,3 +287,94 @@ def test_core_ray_mode3d_matches_oracle():
             m = min(no[k], M)
             for f in ("x", "t", "p", "q", "phi", "tau", "Amp"):
                 assert np.array_equal(ao[k, :m][f], ag[k, :m][f]), (k, f)
+
+
+GOLDEN_ARR3 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "example_analysis_3d.arr.txt")
+
+
+def _ulps32(a, b):
+    a, b = np.float32(a), np.float32(b)
+    return abs(float(a) - float(b)) / float(np.spacing(np.float32(max(abs(a), abs(b)))))
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_oracle_arrivals3d_reproduce_reference_fixture(tmp_path, binary):
+    env = analysis3d_env()
+    o = oracle.field_run3d(env, max_narr=10, allow_merging=1)
+    prob = bh.Problem3D(env)
+    narr, maxn = files.postprocess_arrivals3d(prob, o.arr, o.narr, 10)
+    path = str(tmp_path / ("a3d.barr" if binary else "a3d.arr"))
+    files.write_arr3d(path, prob, o.arr, narr, 10, maxn, binary=binary)
+    got = files.read_arr3d(path, binary=binary)
+    ref = files.read_arr3d(GOLDEN_ARR3)
+    assert got["maxn"] == ref["maxn"] == [1]
+    assert float(np.float32(got["freq0"])) == ref["freq0"]
+    for k in ("Sx", "Sy", "Sz", "Rz", "theta"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.allclose(got["Rr"], ref["Rr"], rtol=1e-7)
+    assert len(got["arrivals"][0]) == len(ref["arrivals"][0]) == 15
+    for g, r in zip(got["arrivals"][0], ref["arrivals"][0]):
+        assert len(g) == len(r) == 1
+        for f in ("phase_deg", "delay_im", "src_decl", "src_azim", "rcvr_decl", "rcvr_azim", "ntop", "nbot"):
+            assert g[f][0] == r[f][0], f
+        assert _ulps32(g["a"][0], r["a"][0]) <= 2.0
+        assert _ulps32(g["delay_re"][0], r["delay_re"][0]) <= 2.0
+
+
+def test_oracle_arrivals3d_fp32_close_to_fixture():
+    """Single precision: the same arrivals within the FP32 build's accumulated
+    step error (up to about 2e-4 relative over the 1129 steps of each ray)."""
+    env = analysis3d_env(precision=32)
+    o = oracle.field_run3d(env, max_narr=10, allow_merging=1)
+    ref = files.read_arr3d(GOLDEN_ARR3)["arrivals"][0]
+    A = np.ctypeslib.as_array(o.arr).reshape(-1, 10)
+    assert list(o.narr) == [1] * 15
+    for ir in range(15):
+        a, r = A[ir, 0], ref[ir][0]
+        assert abs(a["a"] - r["a"]) <= 5e-4 * r["a"]
+        assert abs(a["delay"][0] - r["delay_re"]) <= 5e-4 * r["delay_re"]
+        assert abs(a["SrcAzimAngle"] - r["src_azim"]) < 1e-4 and a["NBotBnc"] == 0 and a["NTopBnc"] == 0
+        assert abs(a["RcvrAzimAngle"] - r["rcvr_azim"]) < 1e-3
+
+
+def test_oracle_eigen_hits3d_bracket_receivers():
+    """Eigen hits on the isovelocity case: the central ray (declination 0,
+    bearing 34 deg = the receivers' bearing) hits every receiver, from the
+    step whose segment brackets the receiver's range."""
+    env = analysis3d_env(run_type="EG   3")
+    o = oracle.field_run3d(env, eigen_memsize=1000)
+    n = int(o.neigen[0])
+    hits = np.ctypeslib.as_array(o.hits)[:n]
+    central = hits[(hits["ialpha"] == 1) & (hits["ibeta"] == 0)]
+    assert sorted(central["ir"].tolist()) == list(range(15))
+    assert np.all(hits["itheta"] == 0) and np.all(hits["iz"] == 0)
+    x = env.Rr_km * 1000.0
+    ds = 1.7646
+    for h in central:
+        # the hit step's segment [is * ds, (is + 1) * ds] brackets the receiver range
+        assert h["is_"] * ds <= x[h["ir"]] + 1e-6 and x[h["ir"]] <= (h["is_"] + 1) * ds + 1e-6
+
+
+def test_arr3d_writer_roundtrip(tmp_path):
+    """3D .arr ASCII and binary layouts read back to what was written."""
+    env = munk_env3d(nalpha=31, nbeta=4, beta=(-20.0, 20.0), ntheta=2, nrz=5, nrr=11, run_type="AB   3")
+    o = oracle.field_run3d(env, max_narr=8, allow_merging=1)
+    prob = bh.Problem3D(env)
+    narr, maxn = files.postprocess_arrivals3d(prob, o.arr, o.narr, 8)
+    A = np.ctypeslib.as_array(o.arr).reshape(-1, 8)
+    assert narr.sum() > 0
+    for binary in (False, True):
+        path = str(tmp_path / ("m.barr" if binary else "m.arr"))
+        files.write_arr3d(path, prob, o.arr, narr, 8, maxn, binary=binary)
+        got = files.read_arr3d(path, binary=binary)
+        assert got["maxn"] == list(maxn)
+        assert np.allclose(got["theta"], env.theta_deg)
+        for b, arrs in enumerate(got["arrivals"][0]):
+            assert len(arrs) == narr[b]
+            for k in range(narr[b]):
+                a = A[b, k]
+                assert np.float32(arrs["a"][k]) == a["a"]
+                assert np.float32(arrs["delay_re"][k]) == a["delay"][0]
+                assert np.float32(arrs["src_azim"][k]) == a["SrcAzimAngle"]
+                assert np.float32(arrs["rcvr_azim"][k]) == a["RcvrAzimAngle"]
+                assert arrs["nbot"][k] == a["NBotBnc"]
