@@ -178,6 +178,12 @@ class Context:
         _check(lib().bhc_last_kernel_ms(self._h, C.byref(v)))
         return v.value
 
+    def last_phase(self):
+        """(trace ms, scatter ms, passes) of the last single-problem launch."""
+        t, sc, p = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().bhc_last_phase_ms(self._h, C.byref(t), C.byref(sc), C.byref(p)))
+        return t.value, sc.value, p.value
+
 
 class FieldResult:
     pass

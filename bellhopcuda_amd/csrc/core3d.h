@@ -903,8 +903,8 @@ template<typename real> struct Seg3D {
     real F0, Fd;                // Frobenius norms of the scaled q at the segment start and of its change
     int32_t ir0, ir1, dir;      // receiver ranges visited by the walk
     int32_t nitems;             // > 0 when the walk brackets at least one range
-    int32_t isz, isx, isy, job; // source indices and ray (job) number
-    int32_t is, ntop, nbot, pad; // step index and the end point's bounce counts (arrivals, eigen hits)
+    int32_t isrc;               // source (isz * NSx + isx) * NSy + isy
+    int32_t job, is, ntop, nbot; // ray (job) number, step index, end point's bounce counts (arrivals, eigen hits)
 };
 
 template<typename real>
@@ -983,9 +983,7 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     }
     d.xs_x = I.xs.x;
     d.xs_y = I.xs.y;
-    d.isz = isz;
-    d.isx = isx;
-    d.isy = isy;
+    d.isrc = (isz * P.NSx + isx) * P.NSy + isy;
     d.is = is;
     d.ntop = p1.NumTopBnc;
     d.nbot = p1.NumBotBnc;
@@ -1130,7 +1128,7 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
     real phaseInt = d.phase0;
     if((qF <= RL(0.0) && d.qOld > RL(0.0)) || (qF >= RL(0.0) && d.qOld < RL(0.0)))
         phaseInt = phaseInt + RC<real>::PI / FL(2.0);
-    size_t idx = (((((size_t)d.isz * P.NSx + d.isx) * P.NSy + d.isy) * P.Ntheta + itheta) * P.NRz + iz) * P.NRr + ir;
+    size_t idx = (((size_t)d.isrc * P.Ntheta + itheta) * P.NRz + iz) * P.NRr + ir;
     if constexpr(RT == RT_ARR) {
         sink.arr3(idx, cnst * w, phaseInt, delay, d, P);
         sink.count();

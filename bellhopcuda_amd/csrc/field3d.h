@@ -590,6 +590,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         for(int k = np - 1; k >= 0; --k) todo.push_back({(int)((long long)k * njobs / np), (int)((long long)(k + 1) * njobs / np)});
     }
     c->passes = 0;
+    c->ph_trace_ms = c->ph_scatter_ms = 0.0;
     bool first = true;
     unsigned long long segs_total = 0, steps_done = 0;
     while(!todo.empty()) {
@@ -604,13 +605,16 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         int waves = (nj + WAVE - 1) / WAVE;
         int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
         if(first) HTRY(hipEventRecord(c->ev0, s));
+        HTRY(hipEventRecord(c->pa, s));
         hipLaunchKernelGGL((k_trace3d<real>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
         HTRY(hipGetLastError());
+        HTRY(hipEventRecord(c->pb, s));
         uint32_t of = 0;
         unsigned long long steps_now = 0;
         HTRY(hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HTRY(hipMemcpyAsync(&steps_now, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HTRY(hipStreamSynchronize(s));
+        TRY(phase_add(c, &c->ph_trace_ms));
         if(of) {
             if(nj <= 1) {
                 fail("segment buffer too small for a single 3D ray (raise BHC_SEG_BUFFER_MB)");
@@ -632,6 +636,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             segs_total += ns;
         }
         if(first) HTRY(hipEventRecord(c->evm, s));
+        HTRY(hipEventRecord(c->pa, s));
         int sblocks = c->num_cus * 8;
         size_t lds = P.stage_tables ? ((size_t)P.NRr + 2 * (size_t)P.Ntheta) * sizeof(real) + (size_t)P.NRz * sizeof(float) : 0;
         if(rt == RT_ARR) {
@@ -648,6 +653,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             else hipLaunchKernelGGL((k_scatter3d<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
         }
         HTRY(hipGetLastError());
+        HTRY(hipEventRecord(c->pb, s));
+        TRY(phase_add(c, &c->ph_scatter_ms));
         ++c->passes;
         first = false;
     }

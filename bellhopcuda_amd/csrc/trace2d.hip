@@ -818,7 +818,9 @@ __global__ void __launch_bounds__(256) k_arr_merge(const ArrCand *cand, const in
 struct bhc_ctx {
     int device;
     hipStream_t stream;
-    hipEvent_t ev0, ev1, evm;   // start, end, trace/scatter boundary of the last pass
+    hipEvent_t ev0, ev1, evm;   // start, end, trace/scatter boundary of the first pass
+    hipEvent_t pa, pb;          // per-pass phase brackets
+    double ph_trace_ms, ph_scatter_ms; // trace / scatter kernel time summed over the last run's passes
     int num_cus;
     int lds_max;
     int tpb, bpc;
@@ -908,6 +910,8 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     HIPCHECK(hipEventCreate(&c->ev0));
     HIPCHECK(hipEventCreate(&c->ev1));
     HIPCHECK(hipEventCreate(&c->evm));
+    HIPCHECK(hipEventCreate(&c->pa));
+    HIPCHECK(hipEventCreate(&c->pb));
     HIPCHECK(hipMalloc((void **)&c->d_seg_count, sizeof(uint32_t)));
     HIPCHECK(hipMalloc((void **)&c->d_overflow, sizeof(uint32_t)));
     *out = c;
@@ -922,6 +926,8 @@ extern "C" void bhc_ctx_destroy(bhc_ctx *ctx)
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipEventDestroy(ctx->evm);
+    (void)hipEventDestroy(ctx->pa);
+    (void)hipEventDestroy(ctx->pb);
     if(ctx->segs) (void)hipFree(ctx->segs);
     if(ctx->d_bin_idx) (void)hipFree(ctx->d_bin_idx);
     (void)hipFree(ctx->d_seg_count);
@@ -947,13 +953,21 @@ extern "C" int bhc_ctx_synchronize(bhc_ctx *ctx)
 
 extern "C" int bhc_last_phase_ms(bhc_ctx *ctx, double *trace_ms, double *scatter_ms, int *passes)
 {
-    float a = 0.f, b = 0.f;
     HIPCHECK(hipEventSynchronize(ctx->ev1));
-    HIPCHECK(hipEventElapsedTime(&a, ctx->ev0, ctx->evm));
-    HIPCHECK(hipEventElapsedTime(&b, ctx->evm, ctx->ev1));
-    *trace_ms = a;
-    *scatter_ms = b;
+    *trace_ms = ctx->ph_trace_ms;
+    *scatter_ms = ctx->ph_scatter_ms;
     *passes = ctx->passes;
+    return 0;
+}
+
+// Per-pass phase timing: a trace (or scatter) launch is bracketed by pa / pb on
+// the context's stream and its duration added once the pass has finished.
+static int phase_add(bhc_ctx *c, double *acc)
+{
+    float ms = 0.f;
+    HIPCHECK(hipEventSynchronize(c->pb));
+    HIPCHECK(hipEventElapsedTime(&ms, c->pa, c->pb));
+    *acc += ms;
     return 0;
 }
 
@@ -1198,7 +1212,9 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
     O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg<real>), 0xFFFFFFF0u);
     std::vector<std::pair<int, int>> todo{{0, dp->njobs}};
     c->passes = 0;
+    c->ph_trace_ms = c->ph_scatter_ms = 0.0;
     bool first = true;
+    unsigned long long steps_done = 0;
     while(!todo.empty()) {
         auto r = todo.back();
         todo.pop_back();
@@ -1213,6 +1229,7 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
         // kernel is register-bound at one wavefront per SIMD)
         int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
         if(first) HIPCHECK(hipEventRecord(c->ev0, s));
+        HIPCHECK(hipEventRecord(c->pa, s));
         switch(P.ssp_type) {
         case 'C': hipLaunchKernelGGL((k_trace2d<real, 'C'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
         case 'S': hipLaunchKernelGGL((k_trace2d<real, 'S'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
@@ -1222,20 +1239,28 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
         default: hipLaunchKernelGGL((k_trace2d<real, 'A'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
         }
         HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(c->pb, s));
         uint32_t of = 0;
+        unsigned long long steps_now = 0;
         HIPCHECK(hipMemcpyAsync(&of, c->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(&steps_now, dp->d_counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
+        if(phase_add(c, &c->ph_trace_ms)) return -1;
         if(of) {
             if(nj <= 1) return fail("segment buffer too small for a single ray (raise BHC_SEG_BUFFER_MB)");
             int mid = r.first + nj / 2;
             todo.push_back({mid, r.second});
             todo.push_back({r.first, mid});
             // the trace pass wrote only per-ray statistics and the step counter:
-            // the retried halves rewrite the statistics; undo the counter
-            HIPCHECK(hipMemsetAsync(dp->d_counters, 0, sizeof(unsigned long long), s));
+            // the retried halves rewrite the statistics; the counter goes back
+            // to the completed passes' total
+            HIPCHECK(hipMemcpyAsync(dp->d_counters, &steps_done, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipStreamSynchronize(s));
             continue;
         }
+        steps_done = steps_now;
         if(first) HIPCHECK(hipEventRecord(c->evm, s));
+        HIPCHECK(hipEventRecord(c->pa, s));
         int mode = dp->scatter_mode;
         if(mode == SCATTER_AUTO) mode = (RT == RT_TL && c->use_tiles && dp->nbins > 0) ? SCATTER_TILED : SCATTER_WAVE;
         if(mode == SCATTER_TILED && !(RT == RT_TL && dp->nbins > 0)) mode = SCATTER_WAVE;
@@ -1279,6 +1304,8 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
                 hipLaunchKernelGGL((k_scatter2d<real, RT, false>), dim3(sblocks), dim3(256), 0, s, P, O);
         }
         HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(c->pb, s));
+        if(phase_add(c, &c->ph_scatter_ms)) return -1;
         ++c->passes;
         first = false;
     }

@@ -81,6 +81,31 @@ def opcounts(L, _abi, probs, nsample):
     return [v / len(probs) for v in acc]
 
 
+def kernel_roofline(precision, kernels, F_path, path_ms):
+    """Roofline object for the dominant kernel (most time) of a launch.
+
+    kernels: dicts with name, ms (summed over its launches), launches and flop
+    (algorithmic FP ops of those launches: f_I R + f_S S for the trace kernel,
+    which also forms each step's influence prelude, f_C C for the scatter).
+    The whole path (trace + scatter) is reported beside it."""
+    peak = FP64_PEAK_TFLOPS if precision == 64 else FP32_PEAK_TFLOPS
+    out = []
+    for k in kernels:
+        ms = max(k["ms"], 1e-9)
+        t = k["flop"] / (ms * 1e-3) / 1e12
+        out.append(dict(k, ms_per_launch=ms / max(k["launches"], 1), flop_per_launch=k["flop"] / max(k["launches"], 1),
+                        achieved=t, frac=t / peak))
+    dom = max(out, key=lambda k: k["ms"])
+    path = F_path / (path_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "pipe": "FP%d vector ALU (the tracer has no matrix work)" % precision,
+            "kernel": dom["name"], "achieved": dom["achieved"], "peak": peak, "unit": "TFLOP/s",
+            "frac": dom["frac"], "traffic": None, "launches": dom["launches"],
+            "ms_per_launch": dom["ms_per_launch"], "flop_per_launch": dom["flop_per_launch"],
+            "kernels": out,
+            "path": {"kernels": "trace + scatter", "ms": path_ms, "flop": F_path, "achieved": path,
+                     "frac": path / peak}}
+
+
 def run_single(a):
     import bellhopcuda_amd as bh
     from bellhopcuda_amd import _abi
@@ -111,8 +136,14 @@ def run_single(a):
     R, S, Cn = prob.nrays, out.total_steps, out.total_contrib
     F = f_I * R + f_S * S + f_C * Cn
     k_ms = sum(kms) / len(kms)
-    peak = FP64_PEAK_TFLOPS if a.precision == 64 else FP32_PEAK_TFLOPS
-    achieved = F / (k_ms * 1e-3) / 1e12
+    nl = len(tms) * npass.value
+    rl = kernel_roofline(a.precision, [
+        {"name": "k_trace2d", "ms": sum(tms), "launches": nl,
+         "flop": (f_I * R + f_S * S) * len(tms)},
+        {"name": "scatter (k_scatter_tile with k_bin_count / k_bin_scan / k_bin_fill, or k_scatter2d / "
+                 "k_scatter2d_lanes, as autotuned)", "ms": sum(sms), "launches": nl, "flop": f_C * Cn * len(tms)},
+    ], F, k_ms)
+    rl.update({"f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S, "contributions": Cn})
     line = {
         "metric": "TL fields/s + rays/s on 1 MI355X",
         "value": fields_s, "unit": "TL fields/s", "rays_per_s": fields_s * R,
@@ -121,11 +152,7 @@ def run_single(a):
         "dtype": "f64" if a.precision == 64 else "f32", "data": "synthetic (seeded environment)",
         "config": {"workload": desc, "rays": R, "receivers": int(env.field_shape[1] * env.field_shape[2]),
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel_ms": k_ms, "trace_ms": sum(tms) / len(tms),
-                     "scatter_ms": sum(sms) / len(sms), "flop_per_field": F, "f_I": f_I, "f_S": f_S, "f_C": f_C,
-                     "rays": R, "steps": S, "contributions": Cn},
+        "roofline": rl,
         "errors": {"err": out.err, "warn": out.warn},
     }
     if not a.no_cpu:
@@ -169,8 +196,15 @@ def run_c3(a):
     k_ms = sum(kms) / len(kms)
     S, Cn = out.total_steps, out.total_contrib
     F = f_I * R + f_S * S + f_C * Cn
-    peak = FP64_PEAK_TFLOPS if a.precision == 64 else FP32_PEAK_TFLOPS
-    achieved = F / (k_ms * 1e-3) / 1e12
+    nl = len(tms) * npass.value
+    rl = kernel_roofline(a.precision, [
+        {"name": "k_trace3d<%s>" % ("double" if a.precision == 64 else "float"), "ms": sum(tms), "launches": nl,
+         "flop": (f_I * R + f_S * S) * len(tms)},
+        {"name": "k_scatter3d<%s, true>" % ("double" if a.precision == 64 else "float"), "ms": sum(sms),
+         "launches": nl, "flop": f_C * Cn * len(tms)},
+    ], F, k_ms)
+    rl.update({"f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S, "contributions": Cn,
+               "passes_per_field": npass.value})
     fields_s = 1e3 / k_ms
     line = {
         "metric": "TL fields/s + rays/s on 1 MI355X",
@@ -185,11 +219,7 @@ def run_c3(a):
         "timing": "value and ms_per_step from HIP events around the field kernels (inputs resident in HBM); "
                   "wall_ms_per_step adds the per-call upload, allocation and download",
         "wall_ms_per_step": 1e3 * sum(walls) / len(walls),
-        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel_ms": k_ms, "trace_ms_first_pass": sum(tms) / len(tms),
-                     "rest_ms": sum(sms) / len(sms), "flop_per_field": F, "f_I": f_I, "f_S": f_S, "f_C": f_C,
-                     "rays": R, "steps": S, "contributions": Cn},
+        "roofline": rl,
         "errors": {"err": out.err, "warn": out.warn},
     }
     if not a.no_cpu:
@@ -228,6 +258,10 @@ def run_sweep(a):
     warm.release()
     sw = bh.Sweep(probs, ctx)
     tuned = sw.autotune()
+    # kernel-by-kernel passes, so that HIP events time the trace and the scatter
+    # kernels of every pass separately (the graph-replayed passes are the same
+    # kernels; each pass ends in a host synchronisation either way)
+    sw.set_graph(False)
     R = sum(p.nrays for p in probs)
     for _ in range(max(0, a.warmup - 1)):
         sw.launch()
@@ -268,8 +302,16 @@ def run_sweep(a):
     S, Cn = st["steps"], st["contrib"]
     F = f_I * R + f_S * S + f_C * Cn
     k_ms = (sum(tr_ms) + sum(sc_ms)) / len(tr_ms)
-    peak = FP64_PEAK_TFLOPS if a.precision == 64 else FP32_PEAK_TFLOPS
-    achieved = F / (k_ms * 1e-3) / 1e12
+    real_t = "double" if a.precision == 64 else "float"
+    nl = a.steps * st["passes"]
+    rl = kernel_roofline(a.precision, [
+        {"name": "k_trace_batch<%s>" % real_t, "ms": sum(tr_ms), "launches": nl,
+         "flop": (f_I * R + f_S * S) * a.steps},
+        {"name": ("k_scatter_lanes<%s, true>" if tuned["scatter_lanes"] else "k_scatter_batch<%s, true>") % real_t,
+         "ms": sum(sc_ms), "launches": nl, "flop": f_C * Cn * a.steps},
+    ], F, k_ms)
+    rl.update({"passes": st["passes"], "f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S,
+               "contributions": Cn})
     line = {
         "metric": "TL fields/s + rays/s on 1 MI355X, batched/surrogate vs plain GPU",
         "value": fields_s, "unit": "TL fields/s", "rays_per_s": fields_s * R / len(probs),
@@ -287,11 +329,7 @@ def run_sweep(a):
                                         len(sample), len(probs), modes.count(1), modes.count(2), modes.count(3))},
         "batched_over_plain": fields_s / plain_fields_s,
         "autotuned": tuned,
-        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (the tracer has no matrix work)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel_ms": k_ms, "trace_ms": sum(tr_ms) / len(tr_ms),
-                     "scatter_ms": sum(sc_ms) / len(sc_ms), "passes": st["passes"], "flop_per_step": F,
-                     "f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S, "contributions": Cn},
+        "roofline": rl,
         "errors": {"problems_with_errors_in_sample": nerr},
         "host_preprocess_s": t_pre,
     }

@@ -30,3 +30,20 @@ def compare_field(o, g, scaled_o, scaled_g, fp64=True, min_exact=0.999, path_rto
     if fp64:
         ok = ok and rep["xend_max_rel"] <= path_rtol
     return ok, rep
+
+
+def small_segment_context(mb):
+    """A device context whose segment buffer holds only `mb` MiB, so that a
+    run is split into several trace / scatter passes, some of them retraced
+    after a segment-buffer overflow."""
+    import os
+    import bellhopcuda_amd as bh
+    old = os.environ.get("BHC_SEG_BUFFER_MB")
+    os.environ["BHC_SEG_BUFFER_MB"] = str(mb)
+    try:
+        return bh.Context(0)
+    finally:
+        if old is None:
+            del os.environ["BHC_SEG_BUFFER_MB"]
+        else:
+            os.environ["BHC_SEG_BUFFER_MB"] = old

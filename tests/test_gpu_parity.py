@@ -175,3 +175,25 @@ def test_scatter_mappings_agree(name, gpu_ctx):
     m = t0 < 120
     for f in fields[1:]:
         assert np.max(np.abs(tl_db(f) - t0)[m]) < 0.01
+
+
+def test_split_passes_keep_counts_and_field(gpu_ctx):
+    """A 64 MiB segment buffer splits a 2001-beam Munk field (~1 M steps) into
+    many passes, with overflows after completed passes: step and contribution
+    counts and the field equal the one-pass run's."""
+    from parity import small_segment_context
+    _announce("split_passes_2d")
+    env = munk_env(nbeams=2001, nrz=51, nrr=401)
+    small = small_segment_context(64)
+    try:
+        a = bh.field_run(bh.Problem(env), ctx=small, stats=True)
+        passes = small.last_phase()[2]
+    finally:
+        small.close()
+    b = bh.field_run(bh.Problem(env), ctx=gpu_ctx, stats=True)
+    print("passes", passes, a.total_steps, b.total_steps, flush=True)
+    assert passes >= 3
+    assert a.total_steps == b.total_steps == int(b.nsteps.sum())
+    assert a.total_contrib == b.total_contrib
+    assert np.array_equal(a.nsteps, b.nsteps)
+    assert np.array_equal(a.field, b.field) or np.allclose(a.field, b.field, rtol=1e-5, atol=1e-12)

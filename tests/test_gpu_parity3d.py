@@ -164,3 +164,25 @@ def test_eigen_hits3d_parity(gpu_ctx):
     ho = np.sort(np.ctypeslib.as_array(o.hits)[:no], order=list(key))
     hg = np.sort(np.ctypeslib.as_array(g.hits)[:ng], order=list(key))
     assert np.array_equal(ho, hg)
+
+
+def test_split_passes3d_keep_counts_and_field(gpu_ctx):
+    """A 16 MiB segment buffer splits a 1057-ray 3D Munk field (~118 k steps)
+    into passes, with overflows after completed passes: step and contribution
+    counts and the field equal the oracle's."""
+    from parity import small_segment_context
+    print("\n[gpu-case-3d] split_passes3d", flush=True)
+    env = munk_env3d(nalpha=151, nbeta=7, run_type="CG   3")
+    small = small_segment_context(16)
+    try:
+        g = bh.field_run3d(env, ctx=small, stats=True)
+    finally:
+        small.close()
+    o = oracle.field_run3d(env)
+    assert g.total_steps == o.total_steps == int(g.nsteps.sum())
+    assert g.total_contrib == o.total_contrib
+    assert np.array_equal(g.nsteps, o.nsteps)
+    so, sg = oracle.scale_pressure3d(env, o.field), bh.Problem3D(env).scale_pressure(g.field)
+    tlo, tlg = tl_db(so), tl_db(sg)
+    m = tlo < 120.0
+    assert np.max(np.abs(tlo - tlg)[m]) <= 0.1
