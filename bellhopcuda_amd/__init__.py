@@ -69,6 +69,7 @@ def lib():
         "bhc_sweep_release": (None, [C.c_void_p]),
         "bhc_sweep_autotune": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "bhc_sweep_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+        "bhc_sweep_set_scatter": (C.c_int, [C.c_void_p, C.c_int]),
         "bhc_last_phase_ms": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int)]),
         "bhc_opcount2d": (C.c_int, [P(_abi.Problem2D), C.c_int, P(C.c_double)]),
         "bhc_subtab": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
@@ -330,6 +331,11 @@ class Sweep:
 
     def set_graph(self, on):
         _check(lib().bhc_sweep_set_graph(self._h, 1 if on else 0))
+
+    def set_scatter(self, mapping):
+        """0: one wavefront per segment, 1: one lane per segment, 2: 64 segments'
+        receivers flattened over a wavefront."""
+        _check(lib().bhc_sweep_set_scatter(self._h, mapping))
 
     def stats(self):
         t, s, p = C.c_double(), C.c_double(), C.c_int()

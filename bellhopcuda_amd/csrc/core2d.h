@@ -190,6 +190,7 @@ template<typename real> struct Prob2D {
     HSd<real> top, bot;
     int32_t isGaussian, isRayCen;
     int32_t rz_sorted;     // receiver depths strictly increasing (enables depth windows)
+    double rz_rdz;         // (NRz - 1) / (Rz[NRz-1] - Rz[0]) when sorted, else 0 (depth-window index guess)
     float *field;          // this problem's complex field (TL runs)
     int32_t njobs;         // rays of this problem
     int32_t pad;
@@ -890,27 +891,46 @@ __host__ __device__ inline bool IsAtCaustic(real qOld, real q)
 }
 
 
+// First index i with Rz[i] >= z (GE) or Rz[i] > z (!GE) of strictly increasing
+// depths: an index guessed from the mean spacing (rdz = (n - 1) / (Rz[n-1] -
+// Rz[0]), 0 for none) and verified against its neighbours, moved by at most
+// four places; a binary search when the guess does not settle. Either way the
+// result is the exact bound.
+template<bool GE, typename real, typename ZT>
+__host__ __device__ inline int rz_bound(const ZT *Rz, int n, double rdz, real z)
+{
+    if(rdz > 0.0) {
+        const double g = ((double)z - (double)(real)Rz[0]) * rdz;
+        int i = !(g > 0.0) ? 0 : (g >= (double)n ? n : (int)g + 1);
+        for(int t = 0; t < 4; ++t) {
+            if(i > 0 && (GE ? (real)Rz[i - 1] >= z : (real)Rz[i - 1] > z)) {
+                --i;
+                continue;
+            }
+            if(i < n && (GE ? (real)Rz[i] < z : (real)Rz[i] <= z)) {
+                ++i;
+                continue;
+            }
+            return i;
+        }
+    }
+    int a = 0, b = n;
+    while(a < b) {
+        int m = (a + b) >> 1;
+        if(GE ? (real)Rz[m] < z : (real)Rz[m] <= z) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
 // Receiver-depth index window [i0, i1] (inclusive, clamped, possibly empty with
 // i1 < i0) that contains every depth Rz[i] with zlo <= Rz[i] <= zhi, widened by
 // one index on each side; requires strictly increasing Rz.
 template<typename real, typename ZT>
-__host__ __device__ inline void depth_window(const ZT *Rz, int n, real zlo, real zhi, int &i0, int &i1)
+__host__ __device__ inline void depth_window(const ZT *Rz, int n, double rdz, real zlo, real zhi, int &i0, int &i1)
 {
-    int lo = 0, hi = n; // first index with Rz >= zlo
-    while(lo < hi) {
-        int m = (lo + hi) >> 1;
-        if((real)Rz[m] < zlo) lo = m + 1;
-        else hi = m;
-    }
-    i0 = imax(lo - 1, 0);
-    lo = 0;
-    hi = n; // first index with Rz > zhi
-    while(lo < hi) {
-        int m = (lo + hi) >> 1;
-        if((real)Rz[m] <= zhi) lo = m + 1;
-        else hi = m;
-    }
-    i1 = imin(lo, n - 1);
+    i0 = imax(rz_bound<true>(Rz, n, rdz, zlo) - 1, 0);
+    i1 = imin(rz_bound<false>(Rz, n, rdz, zhi), n - 1);
 }
 
 template<typename real> struct SegDesc {
@@ -1012,7 +1032,7 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     d.nz = P.NRz_per_range;
     if(P.rz_sorted && P.run4 != 'I' && d.zmin > -RC<real>::MAXV) {
         int i0, i1;
-        depth_window(RzT, P.NRz_per_range, d.zmin, d.zmax, i0, i1);
+        depth_window(RzT, P.NRz_per_range, P.rz_rdz, d.zmin, d.zmax, i0, i1);
         d.iz0 = i0;
         d.nz = imax(0, i1 - i0 + 1);
     }
@@ -1073,7 +1093,7 @@ __host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<rea
         double slack = 1e-4 * W + 1e-6 * (std::fabs(ya) + std::fabs(yb)) + 1e-3;
         double zlo = (ya < yb ? ya : yb) - W - slack, zhi = (ya < yb ? yb : ya) + W + slack;
         int i0, i1;
-        depth_window(RzT, P.NRz_per_range, (real)zlo, (real)zhi, i0, i1);
+        depth_window(RzT, P.NRz_per_range, P.rz_rdz, (real)zlo, (real)zhi, i0, i1);
         d.iz0 = i0;
         d.nz = imax(0, i1 - i0 + 1);
     }
@@ -1228,7 +1248,7 @@ __host__ __device__ inline void cart_column(const SegDesc<real> &d, int j, const
     const double dz = rhs / coef * (1.0 + 1e-4) + 1e-4 * (std::fabs(zl) + std::fabs(rr) + std::fabs(za)) + 1e-2;
     if(!(dz == dz) || !(zl == zl) || !(dz < 1e30)) return;
     int a, b;
-    depth_window(P.Rz, P.NRz_per_range, zl - dz, zl + dz, a, b);
+    depth_window(P.Rz, P.NRz_per_range, P.rz_rdz, zl - dz, zl + dz, a, b);
     i0 = imax(i0, a);
     i1 = imin(i1, b);
 }

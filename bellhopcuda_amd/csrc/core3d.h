@@ -106,6 +106,7 @@ template<typename real> struct Prob3D {
     real rz_absmax;        // largest |receiver depth|
     HSd<real> top, bot;
     int32_t isGaussian, rz_sorted;
+    double rz_rdz;         // depth-window index guess, as in the 2D problem
     float *field;
     int32_t njobs, stage_tables; // stage_tables: the scatter copies Rr and the bearing table to LDS
 };

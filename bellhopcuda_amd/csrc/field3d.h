@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
                 real wlo, whi;
                 if(row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {
                     if(P.rz_sorted && wlo == wlo && whi == whi) {
-                        depth_window(Rz, P.NRz, wlo, whi, i0, i1);
+                        depth_window(Rz, P.NRz, P.rz_rdz, wlo, whi, i0, i1);
                     } else {
                         i0 = 0;
                         i1 = P.NRz - 1;
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3
                 if(!row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax)) continue;
                 if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
                 int i0 = 0, i1 = P.NRz - 1;
-                if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(Rz, P.NRz, wlo, whi, i0, i1);
+                if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(Rz, P.NRz, P.rz_rdz, wlo, whi, i0, i1);
                 for(int iz = i0; iz <= i1; ++iz)
                     item3<real, Sink3, FAST, RT>(d, ir, it, iz, xr, yr, zmin, zmax, P, Rz, sink);
             }

@@ -235,7 +235,7 @@ extern "C" int bhc_opcount3d(const bhc_problem3d *prob, int nsample, double *out
                 bool ok = row3(d, ir, it, P, P.base + P.o_Rr, P.base + P.o_trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi);
                 opc::n = save;
                 if(!ok) continue;
-                if(P.rz_sorted) depth_window(P.Rz, P.NRz, wlo.v, whi.v, i0, i1);
+                if(P.rz_sorted) depth_window(P.Rz, P.NRz, P.rz_rdz, wlo.v, whi.v, i0, i1);
                 for(int iz = i0; iz <= i1; ++iz) {
                     save = opc::n;
                     item3<CR, CountSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, P.Rz, sink);

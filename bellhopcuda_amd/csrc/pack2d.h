@@ -115,6 +115,8 @@ template<typename real> int pack(const bhc_problem2d *p, std::vector<real> &blob
     P.rz_sorted = 1;
     for(int i = 1; i < p->NRz; ++i)
         if(!(p->Rz[i] > p->Rz[i - 1])) P.rz_sorted = 0;
+    P.rz_rdz = (P.rz_sorted && p->NRz > 1) ? (double)(p->NRz - 1) / ((double)p->Rz[p->NRz - 1] - (double)p->Rz[0]) : 0.0;
+    if(!(P.rz_rdz > 0.0 && P.rz_rdz < 1e300)) P.rz_rdz = 0.0;
     char t0 = p->BeamType[0];
     P.isGaussian = (t0 == 'B' || t0 == 'b');
     P.isRayCen = (t0 == 'g' || t0 == 'b');

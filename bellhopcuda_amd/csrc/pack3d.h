@@ -122,6 +122,8 @@ int pack3(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &P, std:
     P.rz_sorted = 1;
     for(int i = 1; i < p->NRz; ++i)
         if(!(p->Rz[i] > p->Rz[i - 1])) P.rz_sorted = 0;
+    P.rz_rdz = (P.rz_sorted && p->NRz > 1) ? (double)(p->NRz - 1) / ((double)p->Rz[p->NRz - 1] - (double)p->Rz[0]) : 0.0;
+    if(!(P.rz_rdz > 0.0 && P.rz_rdz < 1e300)) P.rz_rdz = 0.0;
     P.isGaussian = p->RunType[1] == 'B';
     return 0;
 }

@@ -1818,6 +1818,84 @@ __global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut
     if((threadIdx.x & (WAVE - 1)) == 0 && nc) atomicAdd(O.counters + 1, nc);
 }
 
+// Scatter of a batched pass with the items of 64 segments flattened over the
+// wavefront: the wavefront copies 64 consecutive descriptors to LDS with
+// coalesced loads; the segments' range columns are spread over the lanes (a
+// prefix sum of the column counts), each column's depth band is computed, and
+// the receivers of all bands are spread over the lanes again, so every lane
+// evaluates the same number of receivers whatever the segments' sizes.
+// Ray-centred segments are walked by their own lane.
+template<typename real, bool FAST>
+__global__ void __launch_bounds__(128) k_scatter_flat(BatchDev<real> Bd, DevOut O)
+{
+    constexpr int NW = 2;                                  // wavefronts per workgroup
+    constexpr int W16 = (int)(sizeof(Seg<real>) / 16);     // 16-byte words per descriptor
+    __shared__ uint4 stage[NW][WAVE * W16];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    const uint4 *src = reinterpret_cast<const uint4 *>(O.segs);
+    const Seg<real> *S = reinterpret_cast<const Seg<real> *>(stage[w]);
+    unsigned long long nc = 0;
+    const uint32_t gw = blockIdx.x * NW + (uint32_t)w, nw = gridDim.x * NW;
+    for(uint32_t base = gw * WAVE; base < n; base += nw * WAVE) {
+        const int cnt = (int)min((uint32_t)WAVE, n - base);
+        const uint4 *g = src + (size_t)base * W16;
+        for(int k = lane; k < cnt * W16; k += WAVE) stage[w][k] = g[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int ncol = 0;
+        if(lane < cnt && S[lane].d.nitems > 0) {
+            const SegDesc<real> &d = S[lane].d;
+            const Prob2D<real> &P = Bd.probs[d.prob];
+            if(P.isRayCen) {
+                DevSink<real> sink{&O, &P, 0};
+                seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
+                nc += sink.nc;
+            } else {
+                ncol = (d.ir1 - d.ir0) * d.dir + 1;
+            }
+        }
+        int cexcl;
+        const int ctot = wave_excl_scan(ncol, lane, cexcl);
+        for(int c0 = 0; c0 < ctot; c0 += WAVE) {
+            const int ct = c0 + lane;
+            const int seg = wave_owner(cexcl, ct);
+            const int sexcl = __shfl(cexcl, seg);
+            int ir = 0, i0 = 0, i1 = -1;
+            if(ct < ctot) {
+                const SegDesc<real> &d = S[seg].d;
+                const Prob2D<real> &P = Bd.probs[d.prob];
+                cart_column(d, ct - sexcl, P, P.base, ir, i0, i1);
+            }
+            const int icnt = i1 >= i0 ? i1 - i0 + 1 : 0;
+            int iexcl;
+            const int itot = wave_excl_scan(icnt, lane, iexcl);
+            for(int t0 = 0; t0 < itot; t0 += WAVE) {
+                const int t = t0 + lane;
+                const int own = wave_owner(iexcl, t);
+                const int oir = __shfl(ir, own), oi0 = __shfl(i0, own), oex = __shfl(iexcl, own);
+                const int oseg = __shfl(seg, own);
+                if(t < itot) {
+                    const SegDesc<real> &d = S[oseg].d;
+                    const Prob2D<real> &P = Bd.probs[d.prob];
+                    DevSink<real> sink{&O, &P, 0};
+                    item_cart_at<real, RT_TL, DevSink<real>, FAST>(d, oir, oi0 + (t - oex), P, P.base, P.Rz, sink);
+                    nc += sink.nc;
+                }
+            }
+        }
+        // the stage is rewritten by the next group
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
 // Per-pass counters reset by a kernel (the passes are replayed as HIP graphs,
 // which take no memset nodes here).
 __global__ void k_pass_reset(uint32_t *seg_count, uint32_t *overflow, int32_t *resume, int32_t *nsave_out,
@@ -1856,7 +1934,7 @@ struct bhc_sweep {
     int lds_bytes, lds_bytes_max;
     int passes;
     int scatter_blocks;
-    int scatter_lanes;          // 1: one lane per segment, 0: one wavefront per segment
+    int scatter_lanes;          // 0: one wavefront per segment, 1: one lane per segment, 2: 64 segments' receivers flattened over a wavefront
     double trace_ms, scatter_ms;
     // HIP graphs of one pass (reset, trace, scatter), one per parity of the
     // saved-ray buffers; rebuilt when the launch configuration changes
@@ -2009,7 +2087,12 @@ template<typename real> void enqueue_pass(bhc_sweep *sw, BatchDev<real> Bd, cons
                        sw->d_nsave + (1 - cur), sw->d_more);
     hipLaunchKernelGGL(k_trace_batch<real>, dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
     if(e_mid) (void)hipEventRecord(e_mid, s);
-    if(sw->scatter_lanes) {
+    if(sw->scatter_lanes == 2) {
+        if(c->exact_phasor)
+            hipLaunchKernelGGL((k_scatter_flat<real, false>), dim3(2 * sw->scatter_blocks), dim3(128), 0, s, Bd, O);
+        else
+            hipLaunchKernelGGL((k_scatter_flat<real, true>), dim3(2 * sw->scatter_blocks), dim3(128), 0, s, Bd, O);
+    } else if(sw->scatter_lanes) {
         if(c->exact_phasor)
             hipLaunchKernelGGL((k_scatter_lanes<real, false>), dim3(sw->scatter_blocks), dim3(256), 0, s, Bd, O);
         else
@@ -2139,7 +2222,7 @@ template<typename real> int sweep_tune(bhc_sweep *sw)
     double best = 1e300;
     const int grids[3] = {cus * 4, cus * 8, cus * 16};
     const int ldss[2] = {sw->lds_bytes_max, 0};
-    for(int mode = 0; mode < 2; ++mode)
+    for(int mode = 0; mode < 3; ++mode)
         for(int li = 0; li < (sw->lds_bytes_max > 0 ? 2 : 1); ++li)
             for(int gi = 0; gi < 3; ++gi) {
                 sw->scatter_lanes = mode;
@@ -2213,6 +2296,13 @@ extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_bloc
     if(lds_bytes) *lds_bytes = sw->lds_bytes;
     if(scatter_lanes) *scatter_lanes = sw->scatter_lanes;
     return rc;
+}
+
+extern "C" int bhc_sweep_set_scatter(bhc_sweep *sw, int mapping)
+{
+    if(!sw || mapping < 0 || mapping > 2) return fail("bhc_sweep_set_scatter: bad arguments");
+    sw->scatter_lanes = mapping;
+    return 0;
 }
 
 extern "C" int bhc_sweep_set_graph(bhc_sweep *sw, int use_graph)

@@ -357,12 +357,17 @@ int bhc_sweep_download(bhc_ctx *ctx, bhc_sweep *sw, int iprob, bhc_field_out *ou
 int bhc_sweep_stats(bhc_sweep *sw, double *trace_ms, double *scatter_ms, int *passes, long long *steps,
                     long long *contrib);
 /* Launch-configuration search on the sweep's first 64 problems: scatter grid,
- * scatter mapping (one lane or one wavefront per segment) and LDS staging of
+ * scatter mapping (one wavefront per segment, one lane per segment, or 64
+ * segments' receivers flattened over a wavefront) and LDS staging of
  * the problems' tables; the fastest is kept for later launches and reported. */
 int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *lds_bytes, int *scatter_lanes);
 /* 1 (default): each pass (counter reset, trace, scatter) is replayed from a
  * captured HIP graph; 0: enqueued kernel by kernel with per-kernel timing. */
 int bhc_sweep_set_graph(bhc_sweep *sw, int use_graph);
+/* Scatter mapping of the sweep's passes: 0 one wavefront per segment, 1 one
+ * lane per segment, 2 the receivers of 64 segments flattened over a wavefront
+ * (bhc_sweep_autotune picks among them). */
+int bhc_sweep_set_scatter(bhc_sweep *sw, int mapping);
 void bhc_sweep_release(bhc_sweep *sw);
 
 /* ---- 3D (BELLHOP3D geometry: x, y, depth) ------------------------------------------

@@ -209,7 +209,7 @@ template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
                 real wlo, whi;
                 if(!row3(d, ir, it, P, P.base + P.o_Rr, P.base + P.o_trcv, xr, yr, zmin, zmax)) continue;
                 if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
-                if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(P.Rz, P.NRz, wlo, whi, i0, i1);
+                if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(P.Rz, P.NRz, P.rz_rdz, wlo, whi, i0, i1);
                 else {
                     i0 = 0;
                     i1 = P.NRz - 1;
@@ -319,4 +319,11 @@ template<typename real> int ray3(const bhc_problem3d *prob, bhc_ray_pt3d *pts, i
 extern "C" int core_ray_run3d(const bhc_problem3d *prob, bhc_ray_pt3d *pts, int32_t *nsteps, int M)
 {
     return prob->precision == 64 ? ray3<double>(prob, pts, nsteps, M) : ray3<float>(prob, pts, nsteps, M);
+}
+
+// The receiver-depth window of the scatter kernels (guessed index + exact
+// verification), for its test against a plain search.
+extern "C" void core_depth_window(const float *Rz, int n, double rdz, double zlo, double zhi, int *i0, int *i1)
+{
+    depth_window(Rz, n, rdz, zlo, zhi, *i0, *i1);
 }

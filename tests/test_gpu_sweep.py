@@ -141,3 +141,26 @@ def test_sweep_graph_replays_match_eager(gpu_ctx):
             assert _tl_close(p.scale_pressure(g.field), p.scale_pressure(ref[i].field), 0.01), (rep, i)
         bh.field_run(probs[rep], ctx=gpu_ctx)   # eager work between replays
     sw.release()
+
+
+@pytest.mark.parametrize("mapping", [0, 1, 2])
+def test_sweep_scatter_mappings(mapping, gpu_ctx):
+    """Every scatter mapping of the batched passes (a wavefront per segment, a
+    lane per segment, 64 segments' receivers flattened over a wavefront) gives
+    the single-problem runs' contribution counts and TL; the mix holds
+    Cartesian hat / Gaussian and ray-centred Gaussian problems."""
+    _announce("sweep_scatter_mapping_%d" % mapping)
+    envs = _mixed(64)
+    probs = [bh.Problem(e) for e in envs]
+    sw = bh.Sweep(probs, gpu_ctx, stats=True)
+    sw.set_scatter(mapping)
+    sw.launch()
+    st = sw.stats()
+    total = 0
+    for i, p in enumerate(probs):
+        g = sw.download(i)
+        s = bh.field_run(p, ctx=gpu_ctx)
+        total += s.total_contrib
+        assert _tl_close(p.scale_pressure(g.field), p.scale_pressure(s.field), 0.01), i
+    assert st["contrib"] == total
+    sw.release()

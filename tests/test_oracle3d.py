@@ -378,3 +378,32 @@ def test_arr3d_writer_roundtrip(tmp_path):
                 assert np.float32(arrs["src_azim"][k]) == a["SrcAzimAngle"]
                 assert np.float32(arrs["rcvr_azim"][k]) == a["RcvrAzimAngle"]
                 assert arrs["nbot"][k] == a["NBotBnc"]
+
+
+def test_depth_window_guess_equals_search():
+    """The scatter's depth window (index guessed from the mean spacing, then
+    verified) equals the plain search on uniform, perturbed and irregular
+    receiver grids, for bounds inside, outside, on and between the depths."""
+    H = harness()
+    H.core_depth_window.restype = None
+    H.core_depth_window.argtypes = [_abi.c_float_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    rng = np.random.default_rng(5)
+    grids = [np.linspace(0.0, 5000.0, 501).astype(np.float32),
+             np.linspace(10.0, 20.0, 2).astype(np.float32),
+             np.sort(rng.uniform(0, 4000, 300)).astype(np.float32),
+             (np.linspace(0.0, 1000.0, 101) + rng.normal(0, 3.0, 101)).astype(np.float32),
+             np.geomspace(1.0, 5000.0, 64).astype(np.float32)]
+    for rz in grids:
+        rz = np.unique(rz)
+        n = len(rz)
+        rdz = (n - 1) / (float(rz[-1]) - float(rz[0]))
+        pts = np.concatenate([rz.astype(np.float64), rz + 1e-3, rz - 1e-3, rng.uniform(-100, 5100, 200),
+                              [-1e30, 1e30, np.nan]])
+        for zlo in pts[::3]:
+            for zhi in (zlo, zlo + 7.5, zlo + 300.0):
+                a, b = C.c_int(), C.c_int()
+                H.core_depth_window(rz.ctypes.data_as(_abi.c_float_p), n, rdz, zlo, zhi, C.byref(a), C.byref(b))
+                lo = int(np.searchsorted(rz.astype(np.float64), zlo, "left")) if zlo == zlo else 0
+                hi = int(np.searchsorted(rz.astype(np.float64), zhi, "right")) if zhi == zhi else 0
+                assert (a.value, b.value) == (max(lo - 1, 0), min(hi, n - 1)), (n, zlo, zhi)
