@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _abi
 from .env import Env2D, munk_env, c2_env, sweep_envs, subtab  # noqa: F401
-from .env3d import Env3D, c3_env, free_env3d, munk_env3d  # noqa: F401
+from .env3d import Env3D, c3_env, free_env3d, munk_env3d, analysis3d_env  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbhc_hip.so")
