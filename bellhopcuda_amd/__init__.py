@@ -94,6 +94,8 @@ def lib():
         "bhc_scale_pressure3d": (C.c_int, [P(_abi.Problem3D), _abi.c_float_p]),
         "bhc_opcount3d": (C.c_int, [P(_abi.Problem3D), C.c_int, P(C.c_double)]),
         "bhc_ray_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.RayOut3D)]),
+        "bhc_ray_run3d_list": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.EigenHit), C.c_int32,
+                                         P(_abi.RayOut3D)]),
         "bhc_write_ray3d": (C.c_int, [C.c_char_p, P(_abi.Problem3D), C.c_char_p, P(_abi.RayPt3D), _abi.c_int32_p,
                                       C.c_int32, C.c_int32]),
         "bhc_postprocess_arrivals3d": (C.c_int, [P(_abi.Problem3D), P(_abi.Arrival), _abi.c_int32_p, C.c_int32,
@@ -472,6 +474,28 @@ def ray_run3d(env_or_prob, ctx=None, max_points=4000):
         out.nsteps = ns.ctypes.data_as(_abi.c_int32_p)
         out.max_points_per_ray = max_points
         _check(lib().bhc_ray_run3d(ctx._h, C.byref(prob.view), C.byref(out)))
+        return prob, pts, ns
+    finally:
+        if own:
+            ctx.close()
+
+
+def ray_run3d_list(env_or_prob, hits, nhits, ctx=None, max_points=4000):
+    """3D eigenray re-trace on the GPU (bhc_ray_run3d_list): one ray per hit,
+    stopped after the hit's step. Returns (problem, points, nsteps)."""
+    prob = env_or_prob if isinstance(env_or_prob, Problem3D) else Problem3D(env_or_prob)
+    own = ctx is None
+    if own:
+        ctx = Context(0)
+    try:
+        pts = (_abi.RayPt3D * (nhits * max_points))()
+        ns = np.zeros(nhits, np.int32)
+        out = _abi.RayOut3D()
+        out.points = C.cast(pts, C.POINTER(_abi.RayPt3D))
+        out.nsteps = ns.ctypes.data_as(_abi.c_int32_p)
+        out.max_points_per_ray = max_points
+        _check(lib().bhc_ray_run3d_list(ctx._h, C.byref(prob.view), C.cast(hits, C.POINTER(_abi.EigenHit)), nhits,
+                                        C.byref(out)))
         return prob, pts, ns
     finally:
         if own:

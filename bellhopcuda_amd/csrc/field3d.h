@@ -362,16 +362,27 @@ __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3
 }
 
 // Ray run type in 3D: MainRayMode (src/trace.hpp:482-520), one ray per lane,
-// every point stored.
+// every point stored. With a hit list (eigenray re-trace,
+// src/mode/eigen.cpp:24-48) ray k starts from hit k and stops after its step.
 template<typename real>
 __global__ void __launch_bounds__(64) k_ray3d(Prob3D<real> P, bhc_ray_pt3d *pts, int32_t *nsteps, int M, int njobs,
-                                            uint32_t *errw)
+                                            uint32_t *errw, const bhc_eigen_hit *hits)
 {
     const int job = blockIdx.x * blockDim.x + threadIdx.x;
     if(job >= njobs) return;
     Err e = {0, 0, 0, 0};
-    int ia, ib, isx, isy, isz;
-    JobIndices3(job, P, ia, ib, isy, isx, isz);
+    int ia, ib, isx, isy, isz, lim = -1;
+    if(hits) {
+        const bhc_eigen_hit h = hits[job];
+        ia = h.ialpha;
+        ib = h.ibeta;
+        isx = h.isx;
+        isy = h.isy;
+        isz = h.isz;
+        lim = h.is;
+    } else {
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
+    }
     bhc_ray_pt3d *out = pts + (size_t)job * M;
     Ray3<real> p0, p1, p2;
     Trace3<real> S;
@@ -388,6 +399,10 @@ __global__ void __launch_bounds__(64) k_ray3d(Prob3D<real> P, bhc_ray_pt3d *pts,
             bool two = RayUpdate3(p0, p1, p2, S, P, e);
             store_pt3(out[is + 1], p1);
             if(two) store_pt3(out[is + 2], p2);
+            if(lim >= 0 && is >= lim) {
+                N = is + 2;
+                break;
+            }
             is += two ? 2 : 1;
             p0 = two ? p2 : p1;
             if(RayTerminate3(p0, is, S, P, e, M, N)) break;
@@ -401,14 +416,17 @@ __global__ void __launch_bounds__(64) k_ray3d(Prob3D<real> P, bhc_ray_pt3d *pts,
     if(e.nwarn) atomicAdd(errw + 3, e.nwarn);
 }
 
-template<typename real> int ray3d(bhc_ctx *c, const bhc_problem3d *p, bhc_ray_out3d *out)
+template<typename real>
+int ray3d(bhc_ctx *c, const bhc_problem3d *p, bhc_ray_out3d *out, const bhc_eigen_hit *hits = nullptr,
+          int nhits = 0)
 {
     std::vector<real> blob;
     Prob3D<real> P;
     std::string perr;
     if(pack3<real>(p, blob, P, perr)) return fail("%s", perr.c_str());
-    long long nj64 = (long long)(p->iSingle_alpha >= 1 ? 1 : p->Nalpha) * (p->iSingle_beta >= 1 ? 1 : p->Nbeta)
-        * p->NSx * p->NSy * p->NSz;
+    long long nj64 = hits ? (long long)nhits
+                          : (long long)(p->iSingle_alpha >= 1 ? 1 : p->Nalpha) * (p->iSingle_beta >= 1 ? 1 : p->Nbeta)
+                                * p->NSx * p->NSy * p->NSz;
     const int M = out->max_points_per_ray;
     if(M < 4) return fail("bhc_ray_run3d: max_points_per_ray must be at least 4");
     if(nj64 * (long long)M > (1ll << 31)) return fail("bhc_ray_run3d: too many ray points for one call");
@@ -419,17 +437,23 @@ template<typename real> int ray3d(bhc_ctx *c, const bhc_problem3d *p, bhc_ray_ou
     bhc_ray_pt3d *d_pts = nullptr;
     int32_t *d_ns = nullptr;
     uint32_t *d_errw = nullptr;
+    bhc_eigen_hit *d_hits = nullptr;
     int rc = 0;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(s);
-        void *ptrs[] = {d_blob, d_Rz, d_Sz, d_pts, d_ns, d_errw};
+        void *ptrs[] = {d_blob, d_Rz, d_Sz, d_pts, d_ns, d_errw, d_hits};
         for(void *q : ptrs)
             if(q) (void)hipFree(q);
     };
     if(dalloc(&d_blob, blob.size()) || dalloc(&d_Rz, (size_t)p->NRz) || dalloc(&d_Sz, (size_t)p->NSz)
-       || dalloc(&d_pts, (size_t)njobs * M) || dalloc(&d_ns, (size_t)njobs) || dalloc(&d_errw, 4)) {
+       || dalloc(&d_pts, (size_t)njobs * M) || dalloc(&d_ns, (size_t)njobs) || dalloc(&d_errw, 4)
+       || (hits && dalloc(&d_hits, (size_t)njobs))) {
         cleanup();
         return -1;
+    }
+    if(hits && hipMemcpyAsync(d_hits, hits, (size_t)njobs * sizeof(bhc_eigen_hit), hipMemcpyHostToDevice, s) != hipSuccess) {
+        cleanup();
+        return fail("bhc_ray_run3d_list: upload failed");
     }
     if(hipMemcpyAsync(d_blob, blob.data(), blob.size() * sizeof(real), hipMemcpyHostToDevice, s) != hipSuccess
        || hipMemcpyAsync(d_Rz, p->Rz, p->NRz * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess
@@ -444,7 +468,7 @@ template<typename real> int ray3d(bhc_ctx *c, const bhc_problem3d *p, bhc_ray_ou
     P.Sz = d_Sz;
     P.njobs = njobs;
     hipLaunchKernelGGL((k_ray3d<real>), dim3((njobs + WAVE - 1) / WAVE), dim3(WAVE), 0, s, P, d_pts, d_ns, M, njobs,
-                       d_errw);
+                       d_errw, d_hits);
     if(hipGetLastError() != hipSuccess) rc = fail("bhc_ray_run3d: launch failed");
     uint32_t ew[4] = {0, 0, 0, 0};
     if(!rc && (hipMemcpyAsync(out->points, d_pts, (size_t)njobs * M * sizeof(bhc_ray_pt3d), hipMemcpyDeviceToHost, s)
@@ -748,4 +772,21 @@ extern "C" int bhc_ray_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_ou
     if(!(prob->deltas > 0)) return fail("bhc_ray_run3d: deltas must be positive");
     if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_ray_run3d: hipSetDevice failed");
     return prob->precision == 64 ? ray3d<double>(ctx, prob, out) : ray3d<float>(ctx, prob, out);
+}
+
+extern "C" int bhc_ray_run3d_list(bhc_ctx *ctx, const bhc_problem3d *prob, const bhc_eigen_hit *hits, int32_t nhits,
+                                  bhc_ray_out3d *out)
+{
+    if(!ctx || !prob || !hits || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d_list: null argument");
+    if(nhits < 1) return fail("bhc_ray_run3d_list: no hits");
+    for(int k = 0; k < nhits; ++k) {
+        const bhc_eigen_hit &h = hits[k];
+        if(h.ialpha < 0 || h.ialpha >= prob->Nalpha || h.ibeta < 0 || h.ibeta >= prob->Nbeta || h.isx < 0
+           || h.isx >= prob->NSx || h.isy < 0 || h.isy >= prob->NSy || h.isz < 0 || h.isz >= prob->NSz || h.is < 0)
+            return fail("bhc_ray_run3d_list: hit %d names a ray outside the problem", k);
+    }
+    if(!(prob->deltas > 0)) return fail("bhc_ray_run3d_list: deltas must be positive");
+    if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_ray_run3d_list: hipSetDevice failed");
+    return prob->precision == 64 ? ray3d<double>(ctx, prob, out, hits, nhits)
+                                 : ray3d<float>(ctx, prob, out, hits, nhits);
 }

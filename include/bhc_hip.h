@@ -494,6 +494,11 @@ typedef struct bhc_ray_out3d {
     uint32_t err, warn, err_count, warn_count;
 } bhc_ray_out3d;
 int bhc_ray_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_out3d *out);
+/* 3D eigenray re-trace (EigenModePostWorker, src/mode/eigen.cpp:24-48): ray k
+ * starts from hits[k]'s source and launch angles and stops after step
+ * hits[k].is (MainRayMode's Nsteps limit, src/trace.hpp:510-513). */
+int bhc_ray_run3d_list(bhc_ctx *ctx, const bhc_problem3d *prob, const bhc_eigen_hit *hits, int32_t nhits,
+                       bhc_ray_out3d *out);
 /* .ray file of a 3D ray run (OpenRAYFile + WriteRay, src/mode/ray.hpp:168-244):
  * nrays rays in job order. */
 int bhc_write_ray3d(const char *path, const bhc_problem3d *prob, const char *title, const bhc_ray_pt3d *pts,

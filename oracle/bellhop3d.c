@@ -1362,3 +1362,37 @@ int SFX(oracle_ray_run3d)(const bhc_env3d *env, bhc_ray_pt3d *points, int32_t *n
     prob3_free(&Q);
     return 0;
 }
+
+/* Exported: the eigenray re-trace in 3D (EigenModePostWorker,
+ * src/mode/eigen.cpp:24-48): ray k starts from hit k's source and angles and
+ * stops after the hit's step (Nsteps = hit.is on entry). */
+int SFX(oracle_ray_run3d_list)(const bhc_env3d *env, const bhc_eigen_hit *hits, int32_t nhits, bhc_ray_pt3d *points,
+                               int32_t *nsteps, int max_points)
+{
+    Prob3 Q;
+    if(prob3_from_env(env, &Q)) return -1;
+    ErrState err;
+    atomic_init(&err.error, 0u);
+    atomic_init(&err.warning, 0u);
+    atomic_init(&err.errCount, 0u);
+    atomic_init(&err.warnCount, 0u);
+    RayPt3 *buf = xcalloc((size_t)max_points + 3, sizeof(RayPt3));
+    for(int k = 0; k < nhits; ++k) {
+        RayInit3Info ri;
+        memset(&ri, 0, sizeof(ri));
+        ri.isx = hits[k].isx;
+        ri.isy = hits[k].isy;
+        ri.isz = hits[k].isz;
+        ri.ialpha = hits[k].ialpha;
+        ri.ibeta = hits[k].ibeta;
+        int N = hits[k].is;
+        memset(buf, 0, sizeof(RayPt3) * ((size_t)max_points + 3));
+        MainRayMode3(&ri, buf, &N, max_points, &Q, &err);
+        nsteps[k] = N;
+        int n = N < max_points ? N : max_points;
+        for(int i = 0; i < n; ++i) store_pt3(&points[(size_t)k * max_points + i], &buf[i]);
+    }
+    free(buf);
+    prob3_free(&Q);
+    return 0;
+}

@@ -239,3 +239,20 @@ def ray_run3d(env, max_points=4000):
     if f(C.byref(ec), pts, ns.ctypes.data_as(A.c_int32_p), max_points) != 0:
         raise RuntimeError("oracle_ray_run3d failed")
     return pts, ns
+
+
+def ray_run3d_list(env, hits, nhits, max_points=4000):
+    """3D eigenray re-trace of the oracle: ray k from hit k, stopped after its
+    step. Returns (points [nhits, max_points], nsteps [nhits])."""
+    L = _lib3()
+    from bellhopcuda_amd import _abi as A
+    f = getattr(L, "oracle_ray_run3d_list_" + _sfx(env))
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(A.Env3D_C), C.POINTER(A.EigenHit), C.c_int32, C.POINTER(A.RayPt3D), A.c_int32_p, C.c_int]
+    ec, keep = env.to_c()
+    pts = (A.RayPt3D * (nhits * max_points))()
+    ns = np.zeros(nhits, np.int32)
+    if f(C.byref(ec), C.cast(hits, C.POINTER(A.EigenHit)), nhits, pts, ns.ctypes.data_as(A.c_int32_p),
+         max_points) != 0:
+        raise RuntimeError("oracle_ray_run3d_list failed")
+    return pts, ns

@@ -186,3 +186,26 @@ def test_split_passes3d_keep_counts_and_field(gpu_ctx):
     tlo, tlg = tl_db(so), tl_db(sg)
     m = tlo < 120.0
     assert np.max(np.abs(tlo - tlg)[m]) <= 0.1
+
+
+def test_eigen_retrace3d_parity(gpu_ctx):
+    """3D eigenray re-trace on the GPU (bhc_ray_run3d_list) equals the oracle's
+    re-trace of the same hits: step counts, positions, tangents and bounces
+    exact, p and q to 1e-6."""
+    print("\n[gpu-case-3d-eig] munk_eigen_retrace", flush=True)
+    env = munk_env3d(nalpha=61, nbeta=5, beta=(-20.0, 20.0), ntheta=2, nrz=11, nrr=21, run_type="EG   3")
+    o = oracle.field_run3d(env, eigen_memsize=200000)
+    n = min(int(o.neigen[0]), 400)
+    M = 3000
+    po, no = oracle.ray_run3d_list(env, o.hits, n, max_points=M)
+    _, pg, ng = bh.ray_run3d_list(env, o.hits, n, ctx=gpu_ctx, max_points=M)
+    assert np.array_equal(no, ng)
+    ao = np.ctypeslib.as_array(po).reshape(n, M)
+    ag = np.ctypeslib.as_array(pg).reshape(n, M)
+    for k in range(n):
+        m = min(no[k], M)
+        for f in ("x", "t", "NumTopBnc", "NumBotBnc"):
+            assert np.array_equal(ao[k, :m][f], ag[k, :m][f]), (k, f)
+        for f in ("p", "q"):
+            a, b = ao[k, :m][f], ag[k, :m][f]
+            assert np.allclose(a, b, rtol=1e-6, atol=1e-6 * max(1.0, float(np.max(np.abs(a))))), (k, f)

@@ -214,7 +214,7 @@ def test_abi_exports_3d_symbols():
     L = bh.lib()
     for name in ("bhc_preprocess3d", "bhc_problem3d_view", "bhc_problem3d_free", "bhc_host3d_last_error",
                  "bhc_field_run3d", "bhc_scale_pressure3d", "bhc_opcount3d", "bhc_ray_run3d", "bhc_write_ray3d",
-                 "bhc_postprocess_arrivals3d", "bhc_write_arr3d"):
+                 "bhc_postprocess_arrivals3d", "bhc_write_arr3d", "bhc_ray_run3d_list"):
         assert hasattr(L, name)
 
 
@@ -407,3 +407,31 @@ def test_depth_window_guess_equals_search():
                 lo = int(np.searchsorted(rz.astype(np.float64), zlo, "left")) if zlo == zlo else 0
                 hi = int(np.searchsorted(rz.astype(np.float64), zhi, "right")) if zhi == zhi else 0
                 assert (a.value, b.value) == (max(lo - 1, 0), min(hi, n - 1)), (n, zlo, zhi)
+
+
+def test_oracle_eigen_retrace3d():
+    """Eigenray re-trace in 3D: each hit's ray is the ray run type's ray cut
+    after the hit's step (a prefix of it, point for point), and its last
+    segment brackets the receiver's range."""
+    env = analysis3d_env(run_type="EG   3")
+    o = oracle.field_run3d(env, eigen_memsize=1000)
+    n = int(o.neigen[0])
+    M = 2000
+    pts, ns = oracle.ray_run3d_list(env, o.hits, n, max_points=M)
+    full_env = analysis3d_env(run_type="R    3")
+    fp, fn = oracle.ray_run3d(full_env, max_points=M)
+    hits = np.ctypeslib.as_array(o.hits)[:n]
+    A = np.ctypeslib.as_array(pts).reshape(n, M)
+    F = np.ctypeslib.as_array(fp).reshape(len(fn), M)
+    rr = env.Rr_km * 1000.0
+    nb = len(env.beta_deg)
+    for k, h in enumerate(hits):
+        assert ns[k] == h["is_"] + 2
+        job = h["ibeta"] * len(env.alpha_deg) + h["ialpha"]
+        m = ns[k]
+        for f in ("x", "t", "p", "q", "tau"):
+            assert np.array_equal(A[k, :m][f], F[job, :m][f]), (k, f)
+        ra = np.hypot(A[k, h["is_"]]["x"][0], A[k, h["is_"]]["x"][1])
+        rb = np.hypot(A[k, h["is_"] + 1]["x"][0], A[k, h["is_"] + 1]["x"][1])
+        assert min(ra, rb) <= rr[h["ir"]] <= max(ra, rb) + 1e-9
+    assert nb == 2
