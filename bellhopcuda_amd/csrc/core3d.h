@@ -106,6 +106,8 @@ template<typename real> struct Prob3D {
     real rz_absmax;        // largest |receiver depth|
     HSd<real> top, bot;
     int32_t isGaussian, rz_sorted;
+    int32_t isRayCen, pad_rc;  // ray-centred beams ('g', 'b')
+    real Delta_r;          // receiver range spacing (RToIR)
     double rz_rdz;         // depth-window index guess, as in the 2D problem
     float *field;
     int32_t njobs, stage_tables; // stage_tables: the scatter copies Rr and the bearing table to LDS
@@ -724,6 +726,7 @@ template<typename real> struct Infl3 {
     real Ratio1, rcp_q0, rcp_qhat0, omega, BeamWindow;
     real phase, qOld;
     v3<real> xs;
+    v3<real> rn1, rn2;     // ray-centred beams: the previous point's ray normals (c-scaled)
     int32_t ir;
 };
 
@@ -803,6 +806,8 @@ __host__ __device__ inline bool RayInit3(int ia, int ib, int isx, int isy, int i
     I.qOld = mdet(p0.q);
     I.xs = p0.x;
     I.ir = BinarySearchGT(G + P.o_Rr, P.NRr, (real)RL(0.0));
+    I.rn1 = I.rn2 = V3((real)RL(0.0), (real)RL(0.0), (real)RL(0.0));
+    if(P.isRayCen) RayNormal3(p0.t, p0.phi, false, p0.c, I.rn1, I.rn2); // RayNormalRayCen
     return true;
 }
 
@@ -1089,6 +1094,10 @@ __host__ __device__ inline bool bearing3(const Seg3D<real> &d, real ct, real st,
     return fmin / a <= dmax;
 }
 
+template<typename real, typename D, typename Sink, bool FAST, int RT>
+__host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int itheta, int ir, int iz,
+                                      const v3<real> &rayt, const Prob3D<real> &P, Sink &sink);
+
 // RT selects ApplyContribution's branch (influence.hpp:432-473): TL field
 // term, AddArr candidate (arrivals) or RecordEigenHit (eigen hits).
 template<typename real, typename Sink, bool FAST = false, int RT = RT_TL>
@@ -1101,6 +1110,16 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
     real s = dot3(xrr, d.ta) / d.rlen;
     real n1 = std::fabs(dot3(xrr, d.e1));
     real n2 = std::fabs(dot3(xrr, d.e2));
+    core3<real, Seg3D<real>, Sink, FAST, RT>(d, s, n1, n2, itheta, ir, iz, d.ta, P, sink);
+}
+
+// InfluenceGeoCore (influence.hpp:897-1024, 3D) + ApplyContribution (:432-473)
+// for a receiver at beam coordinates (s, n1, n2) of segment d (Cartesian
+// Seg3D or ray-centred Seg3R); rayt: the direction ReceiverAngles uses.
+template<typename real, typename D, typename Sink, bool FAST, int RT>
+__host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int itheta, int ir, int iz,
+                                      const v3<real> &rayt, const Prob3D<real> &P, Sink &sink)
+{
     sink.mark();
     m2<real> qI = madd(d.q0, mscl(s, d.dq));
     qI.a00 *= d.rcp_q0;
@@ -1108,15 +1127,18 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
     qI.a01 *= d.rcp_qhat0;
     qI.a11 *= d.rcp_qhat0;
     real qF = mdet(qI);
-    real l1 = std::sqrt(qI.a00 * qI.a00 + qI.a10 * qI.a10);
-    real l2 = std::sqrt(qI.a01 * qI.a01 + qI.a11 * qI.a11);
-    if(std::fabs(l1) < RC<real>::EPS || std::fabs(l2) < RC<real>::EPS) return;
+    if(!P.isRayCen) { // the row-length test is Cartesian only
+        real l1 = std::sqrt(qI.a00 * qI.a00 + qI.a10 * qI.a10);
+        real l2 = std::sqrt(qI.a01 * qI.a01 + qI.a11 * qI.a11);
+        if(std::fabs(l1) < RC<real>::EPS || std::fabs(l2) < RC<real>::EPS) return;
+    }
     if(std::fabs(qF) < RC<real>::EPS) return;
     real n1p = std::fabs((-qI.a01 * n2 + qI.a11 * n1) / qF);
     real n2p = std::fabs((qI.a00 * n2 - qI.a10 * n1) / qF);
     real bcd = P.isGaussian ? n1p + n2p : rmax(n1p, n2p);
     real BW = P.isGaussian ? RL(4.0) : RL(1.0);
-    if(bcd > BW * RL(1.0)) return;
+    // the window test is not applied to ray-centred Gaussian beams in 3D
+    if(bcd > BW * RL(1.0) && !(P.isRayCen && P.isGaussian)) return;
     cpx<real> delay = d.tau0 + rmul(s, d.dtau);
     real cnst = d.A0 / std::sqrt(std::fabs(qF));
     real w;
@@ -1131,11 +1153,15 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
         phaseInt = phaseInt + RC<real>::PI / FL(2.0);
     size_t idx = (((size_t)d.isrc * P.Ntheta + itheta) * P.NRz + iz) * P.NRr + ir;
     if constexpr(RT == RT_ARR) {
-        sink.arr3(idx, cnst * w, phaseInt, delay, d, P);
+        // ReceiverAngles, influence.hpp:123-136 (3D)
+        const real RadDeg = RL(180.0) / RC<real>::PI;
+        real azim = RadDeg * std::atan2(rayt.y, rayt.x);
+        real decl = RadDeg * std::atan2(rayt.z, std::sqrt(rayt.x * rayt.x + rayt.y * rayt.y));
+        sink.arr3(idx, cnst * w, phaseInt, delay, decl, azim, d.job, d.is, d.ntop, d.nbot, P);
         sink.count();
         return;
     } else if constexpr(RT == RT_EIG) {
-        sink.eig3(itheta, ir, iz, d, P);
+        sink.eig3(itheta, ir, iz, d.job, d.is, P);
         sink.count();
         return;
     }
@@ -1167,6 +1193,134 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
     }
     sink.add3(idx, fr, fi);
     sink.count();
+}
+
+// ---------------------------------------------------------------------------
+// Ray-centred beams in 3D: Step_InfluenceGeoRayCen (influence.hpp:1030-1194)
+// ---------------------------------------------------------------------------
+
+// Step descriptor of a ray-centred segment: the two points with their ray
+// normals (c-scaled, RayNormalRayCen) and c t (e1 x e2), and the beam state.
+template<typename real> struct Seg3R {
+    v3<real> xa, xb;             // points A (previous) and B
+    v3<real> n1a, n2a, n1b, n2b; // ray normals at A and B
+    v3<real> ca, cb;             // c t at A and B
+    m2<real> q0, dq;
+    cpx<real> tau0, dtau;
+    real A0, phase0, qOld, rcp_q0, rcp_qhat0, xs_x, xs_y;
+    real rmax_m, rmax_n;         // Gaussian: BeamWindow x the larger |q| (scaled) at A and B, per normal
+    int32_t nitems;              // > 0: rows (depth, bearing) to visit
+    int32_t isrc, job, is, ntop, nbot;
+};
+
+template<typename real>
+__host__ __device__ inline void prelude3rc(Seg3R<real> &d, const Ray3<real> &p0, const Ray3<real> &p1,
+                                           Infl3<real> &I, int is, const Prob3D<real> &P, int isz, int isx, int isy)
+{
+    d.nitems = 0;
+    real phaseq = mdet(p0.q);
+    if((phaseq <= RL(0.0) && I.qOld > RL(0.0)) || (phaseq >= RL(0.0) && I.qOld < RL(0.0)))
+        I.phase += RC<real>::PI / FL(2.0);
+    I.qOld = phaseq;
+    v3<real> rn1, rn2;
+    RayNormal3(p1.t, p1.phi, false, p1.c, rn1, rn2);
+    // IsDuplicatePoint: hat beams compare with a constant 1e-4 m threshold
+    v3<real> dx = p1.x - p0.x;
+    real thresh = P.isGaussian ? RL(1.0e3) * spacing(p1.x.x) : (real)FL(1.0e-4);
+    bool dup = len3(dx) <= thresh;
+    if(!dup) {
+        d.xa = p0.x;
+        d.xb = p1.x;
+        d.n1a = I.rn1;
+        d.n2a = I.rn2;
+        d.n1b = rn1;
+        d.n2b = rn2;
+        d.ca = V3(p0.c * p0.t.x, p0.c * p0.t.y, p0.c * p0.t.z);
+        d.cb = V3(p1.c * p1.t.x, p1.c * p1.t.y, p1.c * p1.t.z);
+        d.q0 = p0.q;
+        d.dq = msub(p1.q, p0.q);
+        d.tau0 = p0.tau;
+        d.dtau = p1.tau - p0.tau;
+        d.A0 = I.Ratio1 * p1.c * p1.Amp;
+        d.phase0 = p0.Phase + I.phase;
+        d.qOld = I.qOld;
+        d.rcp_q0 = I.rcp_q0;
+        d.rcp_qhat0 = I.rcp_qhat0;
+        d.xs_x = I.xs.x;
+        d.xs_y = I.xs.y;
+        const real BW = P.isGaussian ? RL(4.0) : RL(1.0);
+        d.rmax_m = BW * rmax(std::fabs(p0.q.a11 * I.rcp_qhat0), std::fabs(p1.q.a11 * I.rcp_qhat0));
+        d.rmax_n = BW * rmax(std::fabs(p0.q.a00 * I.rcp_q0), std::fabs(p1.q.a00 * I.rcp_q0));
+        d.isrc = (isz * P.NSx + isx) * P.NSy + isy;
+        d.is = is;
+        d.ntop = p1.NumTopBnc;
+        d.nbot = p1.NumBotBnc;
+        d.nitems = P.NRz * P.Ntheta;
+    }
+    I.rn1 = rn1;
+    I.rn2 = rn2;
+}
+
+// RToIR, influence.hpp:82-89 (uniform receiver ranges)
+template<typename real> __host__ __device__ inline int rtoir3(real r, const real *Rr, const Prob3D<real> &P)
+{
+    real temp = (r - Rr[0]) / P.Delta_r;
+    if(std::fabs(temp - std::round(temp)) < RL(1e-6)) temp = std::round(temp);
+    return imax(imin(trunc_int_x86(temp), P.NRr - 1), 0);
+}
+
+// Compute_RayCenCross + Compute_M_N_R_IR (influence.hpp:154-189) at one end
+// of the segment for receiver depth zR and bearing (ct, st); false when the
+// ray normal is parallel to the receiver line.
+template<typename real>
+__host__ __device__ inline bool mnr3(const v3<real> &x, const v3<real> &n1, const v3<real> &n2, const v3<real> &e12,
+                                     real xsx, real xsy, real zR, real ct, real st, const real *Rr,
+                                     const Prob3D<real> &P, real &m, real &n, real &r, int &ir)
+{
+    v3<real> xt = x - V3(xsx, xsy, zR);
+    v3<real> c1 = cross3(xt, n1), c2 = cross3(xt, n2);
+    real delta = -(ct * e12.x + st * e12.y);
+    if(std::fabs(delta) < FL(1e-3)) return false;
+    m = (ct * c1.x + st * c1.y) / delta;
+    n = -(ct * c2.x + st * c2.y) / delta;
+    r = -dot3(xt, e12) / delta;
+    ir = rtoir3(r, Rr, P);
+    return true;
+}
+
+// One (depth, bearing) row of a ray-centred segment: the bracketed receiver
+// ranges irA + notii ... irB + notii - step and the interpolation ends.
+// Returns the number of receivers (0: the row is skipped).
+template<typename real>
+__host__ __device__ inline int row3rc(const Seg3R<real> &d, int iz, int itheta, const Prob3D<real> &P, const real *Rr,
+                                      const real *trcv, const float *Rz, real &mA, real &nA, real &rA, real &mB,
+                                      real &nB, real &rB, int &first, int &step)
+{
+    const real zR = (real)Rz[iz];
+    const real ct = trcv[2 * itheta], st = trcv[2 * itheta + 1];
+    int irA, irB;
+    if(!mnr3(d.xa, d.n1a, d.n2a, d.ca, d.xs_x, d.xs_y, zR, ct, st, Rr, P, mA, nA, rA, irA)) return 0;
+    if(!mnr3(d.xb, d.n1b, d.n2b, d.cb, d.xs_x, d.xs_y, zR, ct, st, Rr, P, mB, nB, rB, irB)) return 0;
+    if(P.isGaussian) {
+        if(d.rmax_m <= rmin(std::fabs(mA), std::fabs(mB)) && mA * mB >= RL(0.0)) return 0;
+        if(d.rmax_n <= rmin(std::fabs(nA), std::fabs(nB)) && nA * nB >= RL(0.0)) return 0;
+    }
+    if(irA == irB) return 0;
+    const int notii = (irB <= irA) ? 0 : 1;
+    step = (notii << 1) - 1;
+    first = irA + notii;
+    return (irB - irA) * step;
+}
+
+// Receiver k of a ray-centred row.
+template<typename real, typename Sink, bool FAST, int RT>
+__host__ __device__ inline void item3rc(const Seg3R<real> &d, int iz, int itheta, int ir, real mA, real nA, real rA,
+                                        real mB, real nB, real rB, const Prob3D<real> &P, const real *Rr, Sink &sink)
+{
+    real sw = (Rr[ir] - rA) / (rB - rA);
+    real n1 = std::fabs(nA + sw * (nB - nA));
+    real n2 = std::fabs(mA + sw * (mB - mA));
+    core3<real, Seg3R<real>, Sink, FAST, RT>(d, sw, n1, n2, itheta, ir, iz, d.xb - d.xa, P, sink);
 }
 
 } // namespace bhc3d

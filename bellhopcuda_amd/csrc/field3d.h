@@ -17,6 +17,11 @@ using namespace bhc3d;
 template<typename real> struct alignas(16) Seg3 {
     Seg3D<real> d;
 };
+template<typename real> struct alignas(16) Seg3RW {
+    Seg3R<real> d;
+};
+// Stored descriptor type of a 3D segment: Cartesian or ray-centred.
+template<typename real, bool RCEN> using SegSlot3 = std::conditional_t<RCEN, Seg3RW<real>, Seg3<real>>;
 
 struct DevOut3 {
     int32_t *stats;             // [njobs][3] steps, top / bottom bounces (or null)
@@ -52,37 +57,38 @@ struct Sink3 {
     // merge kernel replays the single-threaded rule per receiver. Angles as
     // RayInit (trace.hpp:99) and ReceiverAngles (influence.hpp:123-136, 3D).
     template<typename real>
-    __device__ void arr3(size_t idx, real amp, real phase, cpx<real> delay, const Seg3D<real> &d, const Prob3D<real> &P)
+    __device__ void arr3(size_t idx, real amp, real phase, cpx<real> delay, real decl, real azim, int job, int is,
+                         int ntop, int nbot, const Prob3D<real> &P)
     {
         int slot = atomicAdd(O->ncand + idx, 1);
         if(slot >= O->cand_cap) return;
         int ia, ib, isy, isx, isz;
-        JobIndices3(d.job, P, ia, ib, isy, isx, isz);
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
         const real RadDeg = RL(180.0) / RC<real>::PI;
         ArrCand c;
-        c.key = ((long long)d.job << 17) | (long long)d.is;
+        c.key = ((long long)job << 17) | (long long)is;
         c.a = (double)amp;
         c.phase = (double)phase;
         c.dre = (double)delay.re;
         c.dim = (double)delay.im;
         c.src = (float)(RadDeg * P.base[P.o_alpha + ia]);
         c.srcaz = (float)(RadDeg * P.base[P.o_beta + ib]);
-        c.rcv = (float)(RadDeg * std::atan2(d.ta.z, std::sqrt(d.ta.x * d.ta.x + d.ta.y * d.ta.y)));
-        c.rcvaz = (float)(RadDeg * std::atan2(d.ta.y, d.ta.x));
-        c.ntop = d.ntop;
-        c.nbot = d.nbot;
+        c.rcv = (float)decl;
+        c.rcvaz = (float)azim;
+        c.ntop = ntop;
+        c.nbot = nbot;
         O->cand[idx * (size_t)O->cand_cap + slot] = c;
     }
     // RecordEigenHit, eigenrays.hpp:24-41
     template<typename real>
-    __device__ void eig3(int itheta, int ir, int iz, const Seg3D<real> &d, const Prob3D<real> &P)
+    __device__ void eig3(int itheta, int ir, int iz, int job, int is, const Prob3D<real> &P)
     {
         int mi = atomicAdd(O->neigen, 1);
         if(mi >= O->eigen_memsize) return;
         int ia, ib, isy, isx, isz;
-        JobIndices3(d.job, P, ia, ib, isy, isx, isz);
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
         bhc_eigen_hit h;
-        h.is = d.is;
+        h.is = is;
         h.itheta = itheta;
         h.ir = ir;
         h.iz = iz;
@@ -95,24 +101,24 @@ struct Sink3 {
     }
 };
 
-template<typename real> __device__ inline void seal3(const SegBlock &B, int lane, const DevOut3 &O)
+template<typename real, bool RCEN = false> __device__ inline void seal3(const SegBlock &B, int lane, const DevOut3 &O)
 {
-    Seg3<real> *segs = reinterpret_cast<Seg3<real> *>(O.segs);
+    SegSlot3<real, RCEN> *segs = reinterpret_cast<SegSlot3<real, RCEN> *>(O.segs);
     for(uint32_t k = B.used + (uint32_t)lane; k < B.cap; k += WAVE) {
         uint32_t slot = B.base + k;
         if(slot < O.seg_cap) segs[slot].d.nitems = 0;
     }
 }
 
-template<typename real>
-__device__ inline void append3(const Seg3D<real> &d, int lane, const DevOut3 &O, SegBlock &B)
+template<typename real, typename Desc, bool RCEN = false>
+__device__ inline void append3(const Desc &d, int lane, const DevOut3 &O, SegBlock &B)
 {
     bool has = d.nitems > 0;
     unsigned long long m = __ballot(has);
     if(!m) return;
     uint32_t cnt = (uint32_t)__popcll(m);
     if(B.used + cnt > B.cap) {
-        seal3<real>(B, lane, O);
+        seal3<real, RCEN>(B, lane, O);
         uint32_t base = 0;
         if(lane == 0) base = atomicAdd(O.seg_count, SEG_BLOCK);
         B.base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
@@ -121,14 +127,14 @@ __device__ inline void append3(const Seg3D<real> &d, int lane, const DevOut3 &O,
     }
     if(has) {
         uint32_t slot = B.base + B.used + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if(slot < O.seg_cap) reinterpret_cast<Seg3<real> *>(O.segs)[slot].d = d;
+        if(slot < O.seg_cap) reinterpret_cast<SegSlot3<real, RCEN> *>(O.segs)[slot].d = d;
         else atomicOr(O.overflow, 1u);
     }
     B.used += cnt;
 }
 
 // Trace kernel: the ray loop of MainFieldModes (src/trace.hpp:525-589), 3D.
-template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<real> P, DevOut3 O)
+template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<real> P, DevOut3 O)
 {
     const int lane = threadIdx.x & (WAVE - 1);
     const int njobs = O.job_end - O.job_begin;
@@ -182,24 +188,26 @@ template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<r
             }
         }
         Ray3<real> p1, p2;
-        Seg3D<real> d;
+        std::conditional_t<RCEN, Seg3R<real>, Seg3D<real>> d;
         d.nitems = 0;
         bool two = false;
         if(active) {
             two = RayUpdate3(p0, p1, p2, S, P, e);
-            prelude3(d, p0, p1, I, is, P, isz, isx, isy);
+            if constexpr(RCEN) prelude3rc(d, p0, p1, I, is, P, isz, isx, isy);
+            else prelude3(d, p0, p1, I, is, P, isz, isx, isy);
             d.job = job;
             ++is;
         }
-        append3(d, lane, O, blk);
+        append3<real, decltype(d), RCEN>(d, lane, O, blk);
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
-                prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                if constexpr(RCEN) prelude3rc(d, p1, p2, I, is, P, isz, isx, isy);
+                else prelude3(d, p1, p2, I, is, P, isz, isx, isy);
                 d.job = job;
                 ++is;
             }
-            append3(d, lane, O, blk);
+            append3<real, decltype(d), RCEN>(d, lane, O, blk);
         }
         if(active) {
             p0 = two ? p2 : p1;
@@ -217,7 +225,7 @@ template<typename real> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<r
             }
         }
     }
-    seal3<real>(blk, lane, O);
+    seal3<real, RCEN>(blk, lane, O);
     if(e.err) atomicOr(O.errw + 0, e.err);
     if(e.warn) atomicOr(O.errw + 1, e.warn);
     if(e.nerr) atomicAdd(O.errw + 2, e.nerr);
@@ -307,6 +315,55 @@ __global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
                 const real ozmin = __shfl(zmin, own), ozmax = __shfl(zmax, own);
                 if(t < total)
                     item3<real, Sink3, FAST, RT>(d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, Rz, sink);
+            }
+        }
+    }
+    unsigned long long nc = sink.nc;
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
+// Scatter of ray-centred 3D segments (Step_InfluenceGeoRayCen,
+// influence.hpp:1030-1194): one wavefront per segment; the (depth, bearing)
+// rows are spread over the lanes, each lane finds its row's bracketed
+// receiver ranges from the ray normals at both ends, and the receivers of all
+// rows are flattened over the lanes by a prefix sum.
+template<typename real, int RT>
+__global__ void __launch_bounds__(256) k_scatter3rc(Prob3D<real> P, DevOut3 O)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
+    const int nwaves = gridDim.x * (blockDim.x / WAVE);
+    const real *Rr = P.base + P.o_Rr, *trcv = P.base + P.o_trcv;
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    Sink3 sink{P.field, 0, &O};
+    const Seg3RW<real> *segs = reinterpret_cast<const Seg3RW<real> *>(O.segs);
+    const int nrow = P.NRz * P.Ntheta;
+    for(uint32_t si = (uint32_t)wave; si < n; si += (uint32_t)nwaves) {
+        const Seg3R<real> d = segs[si].d;
+        if(d.nitems <= 0) continue;
+        for(int g = 0; g < nrow; g += WAVE) {
+            const int row = g + lane;
+            int iz = 0, it = 0, first = 0, step = 1, cnt = 0;
+            real mA = 0, nA = 0, rA = 0, mB = 0, nB = 0, rB = 0;
+            if(row < nrow) {
+                iz = row / P.Ntheta;
+                it = row - iz * P.Ntheta;
+                cnt = row3rc(d, iz, it, P, Rr, trcv, P.Rz, mA, nA, rA, mB, nB, rB, first, step);
+            }
+            int excl;
+            const int total = wave_excl_scan(cnt, lane, excl);
+            for(int t0 = 0; t0 < total; t0 += WAVE) {
+                const int t = t0 + lane;
+                const int own = wave_owner(excl, t);
+                const int oiz = __shfl(iz, own), oit = __shfl(it, own), ofirst = __shfl(first, own);
+                const int ostep = __shfl(step, own), oex = __shfl(excl, own);
+                const real omA = __shfl(mA, own), onA = __shfl(nA, own), orA = __shfl(rA, own);
+                const real omB = __shfl(mB, own), onB = __shfl(nB, own), orB = __shfl(rB, own);
+                if(t < total)
+                    item3rc<real, Sink3, false, RT>(d, oiz, oit, ofirst + (t - oex) * ostep, omA, onA, orA, omB, onB,
+                                                    orB, P, Rr, sink);
             }
         }
     }
@@ -593,7 +650,9 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     O.segs = c->segs;
     O.seg_count = c->d_seg_count;
     O.overflow = c->d_overflow;
-    O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg3<real>), 0xFFFFFFF0u);
+    const bool rcen = P.isRayCen != 0;
+    O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / (rcen ? sizeof(Seg3RW<real>) : sizeof(Seg3<real>)),
+                                           0xFFFFFFF0u);
     O.cand = d_cand;
     O.ncand = d_ncand;
     O.cand_cap = cand_cap;
@@ -630,7 +689,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
         if(first) HTRY(hipEventRecord(c->ev0, s));
         HTRY(hipEventRecord(c->pa, s));
-        hipLaunchKernelGGL((k_trace3d<real>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
+        if(rcen) hipLaunchKernelGGL((k_trace3d<real, true>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
+        else hipLaunchKernelGGL((k_trace3d<real, false>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
         HTRY(hipGetLastError());
         HTRY(hipEventRecord(c->pb, s));
         uint32_t of = 0;
@@ -663,7 +723,11 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         HTRY(hipEventRecord(c->pa, s));
         int sblocks = c->num_cus * 8;
         size_t lds = P.stage_tables ? ((size_t)P.NRr + 2 * (size_t)P.Ntheta) * sizeof(real) + (size_t)P.NRz * sizeof(float) : 0;
-        if(rt == RT_ARR) {
+        if(rcen) {
+            if(rt == RT_ARR) hipLaunchKernelGGL((k_scatter3rc<real, RT_ARR>), dim3(sblocks), dim3(256), 0, s, P, O);
+            else if(rt == RT_EIG) hipLaunchKernelGGL((k_scatter3rc<real, RT_EIG>), dim3(sblocks), dim3(256), 0, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3rc<real, RT_TL>), dim3(sblocks), dim3(256), 0, s, P, O);
+        } else if(rt == RT_ARR) {
             if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
             else hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
         } else if(rt == RT_EIG) {
@@ -753,8 +817,10 @@ extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_fiel
         return fail("bhc_field_run3d: arrivals run needs arr, narr and max_narr >= 1");
     if(r0 == 'E' && (!out->hits || !out->neigen || out->eigen_memsize < 1))
         return fail("bhc_field_run3d: eigenray run needs hits, neigen and eigen_memsize >= 1");
-    if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B')
-        return fail("bhc_field_run3d: beam type '%c' is not a Cartesian geometric beam", r1);
+    if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B' && r1 != 'g' && r1 != 'b')
+        return fail("bhc_field_run3d: beam type '%c' is not a geometric beam", r1);
+    if((r1 == 'g' || r1 == 'b') && prob->NRr > 1 && !(prob->Delta_r > 0))
+        return fail("bhc_field_run3d: ray-centred beams need equally spaced receiver ranges");
     if(prob->NSx < 1 || prob->NSy < 1 || prob->NSz < 1 || prob->NRz < 1 || prob->NRr < 1 || prob->Ntheta < 1
        || prob->Nalpha < 1 || prob->Nbeta < 1)
         return fail("bhc_field_run3d: empty source / receiver / angle set");

@@ -256,14 +256,15 @@ extern "C" int bhc_preprocess3d(const bhc_env3d *e, bhc_problem3d_owner **out)
        || e->Nbeta < 1)
         return fail3("sources, receivers and launch angles must be non-empty");
     // TL, arrivals ('A' ASCII / 'a' binary file), eigenray and ray run types with
-    // Cartesian geometric beams (src/module/runtype.hpp maps '^' and ' ' to
-    // 'G'). Ray-centred 3D beams ('g', 'b') and the combined arrivals +
-    // eigenray run types ('V', 'v') are not on this 3D path.
+    // geometric beams, Cartesian (src/module/runtype.hpp maps '^' and ' ' to
+    // 'G') or ray-centred ('g', 'b'). The combined arrivals + eigenray run
+    // types ('V', 'v') are not on this 3D path.
     const char r0 = e->RunType[0];
     if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'R' && r0 != 'A' && r0 != 'a' && r0 != 'E')
         return fail3("3D run type '%c' is not supported (C / S / I TL, A / a arrivals, E eigenrays, R rays)", r0);
-    if(e->RunType[1] != 'G' && e->RunType[1] != '^' && e->RunType[1] != ' ' && e->RunType[1] != 'B')
-        return fail3("3D beam type '%c' is not supported (Cartesian geometric beams G / B)", e->RunType[1]);
+    const char r1 = e->RunType[1];
+    if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B' && r1 != 'g' && r1 != 'b')
+        return fail3("3D beam type '%c' is not supported (geometric beams: G / B Cartesian, g / b ray-centred)", r1);
     if(e->ssp_type != 'C' && e->ssp_type != 'N' && e->ssp_type != 'S' && e->ssp_type != 'P' && e->ssp_type != 'H')
         return fail3("3D SSP type '%c' is not supported", e->ssp_type);
     bhc_env2d e2;

@@ -124,7 +124,10 @@ int pack3(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &P, std:
         if(!(p->Rz[i] > p->Rz[i - 1])) P.rz_sorted = 0;
     P.rz_rdz = (P.rz_sorted && p->NRz > 1) ? (double)(p->NRz - 1) / ((double)p->Rz[p->NRz - 1] - (double)p->Rz[0]) : 0.0;
     if(!(P.rz_rdz > 0.0 && P.rz_rdz < 1e300)) P.rz_rdz = 0.0;
-    P.isGaussian = p->RunType[1] == 'B';
+    P.isGaussian = p->RunType[1] == 'B' || p->RunType[1] == 'b';
+    P.isRayCen = p->RunType[1] == 'g' || p->RunType[1] == 'b';
+    P.pad_rc = 0;
+    P.Delta_r = (real)p->Delta_r;
     return 0;
 }
 

@@ -475,6 +475,8 @@ void bhc_problem3d_free(bhc_problem3d_owner *p);
 const char *bhc_host3d_last_error(void);
 
 /* 3D field run, RunFieldModesImpl<CFG, true, true> (src/mode/fieldimpl.hpp:27-28),
+ * with Cartesian ('G', 'B') or ray-centred ('g', 'b', Step_InfluenceGeoRayCen
+ * src/influence.hpp:1030-1194; equally spaced receiver ranges) geometric beams,
  * for TL (C / S / I), arrivals (A / a: out->arr and out->narr over the receivers
  * in field order, AddArr src/arrivals.hpp:45-130 with azimuths) and eigen hits
  * (E: RecordEigenHit src/eigenrays.hpp:24-41 with itheta, isx, isy, ibeta):

@@ -930,6 +930,7 @@ typedef struct {
     RayInit3Info init;
     real freq0, omega, BeamWindow, Ratio1, rcp_q0, rcp_qhat0, phase, qOld;
     v3 xs, x;
+    v3 rayn1, rayn2; /* ray-centred beams: the previous point's ray normals */
     int ir;
 } Infl3;
 
@@ -942,7 +943,7 @@ typedef struct {
     bhc_eigen_hit *hits;
     _Atomic int32_t *neigen;
     int eigen_memsize;
-    int isGaussian;
+    int isGaussian, isRayCen;
     _Atomic long long contrib;
 } Outs3;
 
@@ -972,14 +973,17 @@ static void InfluenceGeoCore3(real s, real n1, real n2, m2 dq, cpx dtau, int ith
     qI.m[0][1] *= inf->rcp_qhat0;
     qI.m[1][1] *= inf->rcp_qhat0;
     real qFinal = m2det(qI);
-    real l1 = RSQRT(qI.m[0][0] * qI.m[0][0] + qI.m[1][0] * qI.m[1][0]);
-    real l2 = RSQRT(qI.m[0][1] * qI.m[0][1] + qI.m[1][1] * qI.m[1][1]);
-    if(RABS(l1) < REAL_EPSILON || RABS(l2) < REAL_EPSILON) return;
+    if(!O->isRayCen) { /* the row-length test is Cartesian only */
+        real l1 = RSQRT(qI.m[0][0] * qI.m[0][0] + qI.m[1][0] * qI.m[1][0]);
+        real l2 = RSQRT(qI.m[0][1] * qI.m[0][1] + qI.m[1][1] * qI.m[1][1]);
+        if(RABS(l1) < REAL_EPSILON || RABS(l2) < REAL_EPSILON) return;
+    }
     if(RABS(qFinal) < REAL_EPSILON) return;
     real n1p = RABS((-qI.m[0][1] * n2 + qI.m[1][1] * n1) / qFinal);
     real n2p = RABS((qI.m[0][0] * n2 - qI.m[1][0] * n1) / qFinal);
     real bcd = O->isGaussian ? n1p + n2p : rmax(n1p, n2p);
-    if(bcd > inf->BeamWindow * RL(1.0)) return;
+    /* the window test is not applied to ray-centred Gaussian beams in 3D */
+    if(bcd > inf->BeamWindow * RL(1.0) && !(O->isRayCen && O->isGaussian)) return;
     cpx delay = p0->tau + s * dtau;
     real cnst = inf->Ratio1 * p1->c * p1->Amp / RSQRT(RABS(qFinal));
     real w;
@@ -1094,8 +1098,85 @@ static void Step_InfluenceGeoCart3(const RayPt3 *p0, const RayPt3 *p1, Infl3 *in
     inf->x = p1->x;
 }
 
-static void Init_Influence3(Infl3 *inf, const RayPt3 *p0, const RayInit3Info *ri, const Prob3 *Q, int isGaussian)
-{ /* influence.hpp:525-644 (geometric Cartesian, 3D) */
+static void RayCenCross3(v3 x, v3 rn1, v3 rn2, real zR, const Infl3 *inf, v3 *xt, v3 *xtxe1, v3 *xtxe2)
+{ /* Compute_RayCenCross, influence.hpp:154-163 */
+    *xt = v3sub(x, V3(inf->xs.x, inf->xs.y, zR));
+    *xtxe1 = v3cross(*xt, rn1);
+    *xtxe2 = v3cross(*xt, rn2);
+}
+
+static int M_N_R_IR3(real *m, real *n, real *r, int *ir, v3 e1xe2, v3 xtxe1, v3 xtxe2, v3 xt, v2 tr, const Prob *P)
+{ /* Compute_M_N_R_IR, influence.hpp:178-189; 1: the ray normal is parallel to the receiver line */
+    real delta = -vdot(tr, V2(e1xe2.x, e1xe2.y));
+    if(RABS(delta) < FL(1e-3)) return 1;
+    *m = vdot(tr, V2(xtxe1.x, xtxe1.y)) / delta;
+    *n = -vdot(tr, V2(xtxe2.x, xtxe2.y)) / delta;
+    *r = -v3dot(xt, e1xe2) / delta;
+    *ir = RToIR(*r, P);
+    return 0;
+}
+
+static void Step_InfluenceGeoRayCen3(const RayPt3 *p0, const RayPt3 *p1, Infl3 *inf, int is, Outs3 *O)
+{ /* influence.hpp:1030-1194 (3D) */
+    const Prob3 *Q = O->Q;
+    const Prob *P = &Q->P2;
+    real phaseq = m2det(p0->q);
+    if((phaseq <= RL(0.0) && inf->qOld > RL(0.0)) || (phaseq >= RL(0.0) && inf->qOld < RL(0.0)))
+        inf->phase += REAL_PI / FL(2.0);
+    inf->qOld = phaseq;
+    m2 dq = m2sub(p1->q, p0->q);
+    cpx dtau = p1->tau - p0->tau;
+    v3 e1xe2A = v3scl(p0->c, p0->t), e1xe2B = v3scl(p1->c, p1->t);
+    v3 rayn1, rayn2;
+    RayNormalImpl(p1->t, p1->phi, 0, p1->c, &rayn1, &rayn2);
+    /* IsDuplicatePoint: hat beams compare with a constant 1e-4 m threshold */
+    real thresh = O->isGaussian ? RL(1.0e3) * spacing(p1->x.x) : FL(1.0e-4);
+    if(v3len(v3sub(p1->x, p0->x)) <= thresh) {
+        inf->x = p1->x;
+        inf->rayn1 = rayn1;
+        inf->rayn2 = rayn2;
+        return;
+    }
+    v3 d = v3sub(p1->x, p0->x);
+    real RcvrAzimAngle = RadDeg * RATAN2(d.y, d.x);
+    real RcvrDeclAngle = RadDeg * RATAN2(d.z, RSQRT(d.x * d.x + d.y * d.y));
+    for(int iz = 0; iz < P->NRz_per_range; ++iz) {
+        real zR = (real)P->Rz[iz];
+        v3 xtA, xtxe1A, xtxe2A, xtB, xtxe1B, xtxe2B;
+        RayCenCross3(p0->x, inf->rayn1, inf->rayn2, zR, inf, &xtA, &xtxe1A, &xtxe2A);
+        RayCenCross3(p1->x, rayn1, rayn2, zR, inf, &xtB, &xtxe1B, &xtxe2B);
+        for(int itheta = 0; itheta < Q->Ntheta; ++itheta) {
+            real mA, nA, rA, mB, nB, rB;
+            int irA, irB;
+            if(M_N_R_IR3(&mA, &nA, &rA, &irA, e1xe2A, xtxe1A, xtxe2A, xtA, Q->t_rcvr[itheta], P)) continue;
+            if(M_N_R_IR3(&mB, &nB, &rB, &irB, e1xe2B, xtxe1B, xtxe2B, xtB, Q->t_rcvr[itheta], P)) continue;
+            if(O->isGaussian) {
+                real MaxRadius_m = inf->BeamWindow
+                    * rmax(RABS(p0->q.m[1][1] * inf->rcp_qhat0), RABS(p1->q.m[1][1] * inf->rcp_qhat0));
+                real MaxRadius_n = inf->BeamWindow
+                    * rmax(RABS(p0->q.m[0][0] * inf->rcp_q0), RABS(p1->q.m[0][0] * inf->rcp_q0));
+                if(MaxRadius_m <= rmin(RABS(mA), RABS(mB)) && mA * mB >= RL(0.0)) continue;
+                if(MaxRadius_n <= rmin(RABS(nA), RABS(nB)) && nA * nB >= RL(0.0)) continue;
+            }
+            if(irA == irB) continue;
+            int notii = (irB <= irA) ? 0 : 1;
+            for(int ir = irA + notii; ir != irB + notii; ir += (notii << 1) - 1) {
+                real sw = (P->Rr[ir] - rA) / (rB - rA);
+                real n1 = RABS(nA + sw * (nB - nA));
+                real n2 = RABS(mA + sw * (mB - mA));
+                InfluenceGeoCore3(sw, n1, n2, dq, dtau, itheta, ir, iz, is, RcvrDeclAngle, RcvrAzimAngle, p0, p1,
+                                  inf, O);
+            }
+        }
+    }
+    inf->x = p1->x;
+    inf->rayn1 = rayn1;
+    inf->rayn2 = rayn2;
+}
+
+static void Init_Influence3(Infl3 *inf, const RayPt3 *p0, const RayInit3Info *ri, const Prob3 *Q, int isGaussian,
+                            int isRayCen)
+{ /* influence.hpp:525-644 (geometric, 3D) */
     const Prob *P = &Q->P2;
     inf->init = *ri;
     inf->freq0 = P->freq0;
@@ -1111,6 +1192,8 @@ static void Init_Influence3(Infl3 *inf, const RayPt3 *p0, const RayInit3Info *ri
     inf->xs = p0->x;
     inf->x = p0->x;
     inf->ir = BinarySearchGT(P->Rr, P->NRr, RL(0.0));
+    inf->rayn1 = inf->rayn2 = V3(RL(0.0), RL(0.0), RL(0.0));
+    if(isRayCen) RayNormalImpl(p0->t, p0->phi, 0, p0->c, &inf->rayn1, &inf->rayn2); /* RayNormalRayCen */
 }
 
 typedef struct {
@@ -1127,15 +1210,17 @@ static void MainFieldModes3(RayInit3Info *ri, Outs3 *O, ErrState *err, RayStat3 
     memset(&p2, 0, sizeof(p2));
     if(st) { memset(st, 0, sizeof(*st)); }
     if(!RayInit3(ri, &p0, &S, Q, err)) return;
-    Init_Influence3(&inf, &p0, ri, Q, O->isGaussian);
+    Init_Influence3(&inf, &p0, ri, Q, O->isGaussian, O->isRayCen);
     int is = 0, Nsteps = 0;
     for(;;) {
         if(HasErrored(err)) break;
         int two = RayUpdate3(&p0, &p1, &p2, &S, Q, err);
-        Step_InfluenceGeoCart3(&p0, &p1, &inf, is, O);
+        if(O->isRayCen) Step_InfluenceGeoRayCen3(&p0, &p1, &inf, is, O);
+        else Step_InfluenceGeoCart3(&p0, &p1, &inf, is, O);
         ++is;
         if(two) {
-            Step_InfluenceGeoCart3(&p1, &p2, &inf, is, O);
+            if(O->isRayCen) Step_InfluenceGeoRayCen3(&p1, &p2, &inf, is, O);
+            else Step_InfluenceGeoCart3(&p1, &p2, &inf, is, O);
             p0 = p2;
             ++is;
         } else {
@@ -1204,7 +1289,7 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
 {
     char rt0 = env->RunType[0], rt1 = env->RunType[1];
     if(rt0 != 'C' && rt0 != 'S' && rt0 != 'I' && rt0 != 'A' && rt0 != 'a' && rt0 != 'E') return -1;
-    if(rt1 != 'G' && rt1 != '^' && rt1 != ' ' && rt1 != 'B') return -1;
+    if(rt1 != 'G' && rt1 != '^' && rt1 != ' ' && rt1 != 'B' && rt1 != 'g' && rt1 != 'b') return -1;
     Prob3 Q;
     if(prob3_from_env(env, &Q)) return -1;
     Outs3 O;
@@ -1220,7 +1305,8 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
     O.eigen_memsize = out->eigen_memsize;
     if((rt0 == 'A' || rt0 == 'a') && (!O.arr || !O.narr || O.max_narr < 1)) { prob3_free(&Q); return -1; }
     if(rt0 == 'E' && (!O.hits || !O.neigen)) { prob3_free(&Q); return -1; }
-    O.isGaussian = (rt1 == 'B');
+    O.isGaussian = (rt1 == 'B' || rt1 == 'b');
+    O.isRayCen = (rt1 == 'g' || rt1 == 'b');
     atomic_init(&O.contrib, 0);
     ErrState err;
     atomic_init(&err.error, 0u);

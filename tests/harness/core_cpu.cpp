@@ -217,6 +217,21 @@ template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
                 for(int iz = i0; iz <= i1; ++iz) item3<real, HostSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, P.Rz, sink);
             }
     };
+    // ray-centred: every (depth, bearing) row's bracketed ranges, in the
+    // reference's order
+    auto scatter_rc = [&](const Seg3R<real> &d) {
+        if(d.nitems <= 0) return;
+        const real *Rr = P.base + P.o_Rr, *tr = P.base + P.o_trcv;
+        for(int iz = 0; iz < P.NRz; ++iz)
+            for(int it = 0; it < P.Ntheta; ++it) {
+                real mA, nA, rA, mB, nB, rB;
+                int first, step;
+                int cnt = row3rc(d, iz, it, P, Rr, tr, P.Rz, mA, nA, rA, mB, nB, rB, first, step);
+                for(int k = 0; k < cnt; ++k)
+                    item3rc<real, HostSink3, false, RT_TL>(d, iz, it, first + k * step, mA, nA, rA, mB, nB, rB, P, Rr,
+                                                           sink);
+            }
+    };
     for(int job = 0; job < njobs; ++job) {
         int ia, ib, isx, isy, isz;
         JobIndices3(job, P, ia, ib, isy, isx, isz);
@@ -234,14 +249,26 @@ template<typename real> int run3(const bhc_problem3d *prob, bhc_field_out *out)
         for(;;) {
             if(e.err) break;
             bool two = RayUpdate3(p0, p1, p2, S, P, e);
-            Seg3D<real> d;
-            prelude3(d, p0, p1, I, is, P, isz, isx, isy);
-            scatter(d);
-            ++is;
-            if(two) {
-                prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+            if(P.isRayCen) {
+                Seg3R<real> d;
+                prelude3rc(d, p0, p1, I, is, P, isz, isx, isy);
+                scatter_rc(d);
+                ++is;
+                if(two) {
+                    prelude3rc(d, p1, p2, I, is, P, isz, isx, isy);
+                    scatter_rc(d);
+                    ++is;
+                }
+            } else {
+                Seg3D<real> d;
+                prelude3(d, p0, p1, I, is, P, isz, isx, isy);
                 scatter(d);
                 ++is;
+                if(two) {
+                    prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                    scatter(d);
+                    ++is;
+                }
             }
             p0 = two ? p2 : p1;
             if(RayTerminate3(p0, is, S, P, e)) break;

@@ -32,6 +32,12 @@ CASES = {
                                                   run_type="CG   3"),
     "shelf_canyon_hexahedral_f32": lambda: c3_env(nalpha=61, nbeta=36, ntheta=8, nrz=31, nrr=81, nxy=51,
                                                   precision=32),
+    "free_raycen_gauss_incoherent": lambda: free_env3d(run_type="Ib   3"),
+    "munk_raycen_hat": lambda: munk_env3d(nalpha=151, nbeta=7, run_type="Cg   3"),
+    "shelf_canyon_raycen_gauss": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=51,
+                                                run_type="Cb   3"),
+    "shelf_canyon_raycen_hat_f32": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=51,
+                                                  run_type="Cg   3", precision=32),
 }
 
 
@@ -68,7 +74,7 @@ def test_field_parity3d(name, gpu_ctx):
 
 
 def test_run3d_rejects_unsupported(gpu_ctx):
-    env = free_env3d(run_type="Cb   3")
+    env = free_env3d(run_type="CR   3")
     with pytest.raises(bh.BhcError):
         bh.Problem3D(env)
 
@@ -117,6 +123,8 @@ ARR_CASES = {
                                                   nrr=21, run_type="aG   3"), 6, 0, 1),
     "shelf_canyon_gauss_merging": (lambda: c3_env(nalpha=31, nbeta=12, ntheta=4, nrz=11, nrr=21, nxy=41,
                                                   run_type="AB   3"), 30, 1, 1),
+    "shelf_canyon_raycen_hat": (lambda: c3_env(nalpha=31, nbeta=12, ntheta=4, nrz=11, nrr=21, nxy=41,
+                                               run_type="Ag   3"), 30, 1, 1),
 }
 
 

@@ -157,6 +157,13 @@ CORE_CASES = {
     "shelf_canyon_hexahedral": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=41),
     "shelf_canyon_hexahedral_f32": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=41,
                                                   precision=32),
+    "free_raycen_gauss_incoherent": lambda: free_env3d(run_type="Ib   3", nalpha=45, nbeta=19, nrz=11, nrr=26),
+    "free_raycen_hat": lambda: free_env3d(run_type="Cg   3", nalpha=45, nbeta=19, nrz=11, nrr=26),
+    "munk_raycen_hat": lambda: munk_env3d(nalpha=101, nbeta=5, run_type="Cg   3"),
+    "shelf_canyon_raycen_gauss": lambda: c3_env(nalpha=31, nbeta=16, ntheta=4, nrz=11, nrr=41, nxy=41,
+                                                run_type="Cb   3"),
+    "shelf_canyon_raycen_hat_f32": lambda: c3_env(nalpha=31, nbeta=16, ntheta=4, nrz=11, nrr=41, nxy=41,
+                                                  run_type="Cg   3", precision=32),
 }
 
 
@@ -201,7 +208,7 @@ def test_preprocess3d_matches_oracle(precision):
 
 
 def test_preprocess3d_rejects_unsupported():
-    for rt in ("Cg   3", "Cb   3", "VB   3", "vB   3"):
+    for rt in ("CR   3", "CC   3", "VB   3", "vB   3"):
         with pytest.raises(bh.BhcError):
             bh.Problem3D(free_env3d(run_type=rt))
     e = free_env3d()
@@ -435,3 +442,29 @@ def test_oracle_eigen_retrace3d():
         rb = np.hypot(A[k, h["is_"] + 1]["x"][0], A[k, h["is_"] + 1]["x"][1])
         assert min(ra, rb) <= rr[h["ir"]] <= max(ra, rb) + 1e-9
     assert nb == 2
+
+
+@pytest.mark.parametrize("run_type,tol_med,tol_p95", [("Cg   3", 0.05, 0.2), ("Ib   3", 0.1, 0.25)])
+def test_free_space_spherical_spreading_raycen(run_type, tol_med, tol_p95):
+    """Ray-centred 3D beams (Step_InfluenceGeoRayCen, R3D) in free space:
+    TL = 20 log10 R on every bearing, to the Cartesian beams' bars. This fixes
+    the ray-normal receiver-line intercepts (m, n, r), the range brackets and
+    the interpolation of n1, n2 between the segment's ends."""
+    env = free_env3d(nalpha=181, nbeta=72, run_type=run_type)
+    o = oracle.field_run3d(env)
+    f = oracle.scale_pressure3d(env, o.field)[0, 0, 0]
+    d = _free_tl_error(env, f)
+    assert o.err == 0
+    assert abs(np.median(d)) < tol_med, np.median(d)
+    assert np.percentile(np.abs(d), 95) < tol_p95, np.percentile(np.abs(d), 95)
+
+
+def test_raycen_hat_equals_cartesian_hat_munk():
+    """On a depth-only Munk environment ray-centred and Cartesian hat beams
+    give the same TL (they differ only in how receivers are bracketed)."""
+    a = oracle.field_run3d(munk_env3d(nalpha=151, nbeta=7, run_type="Cg   3"))
+    b = oracle.field_run3d(munk_env3d(nalpha=151, nbeta=7, run_type="CG   3"))
+    ta, tb = tl_db(a.field)[0, 0, 0, 0], tl_db(b.field)[0, 0, 0, 0]
+    m = tb < 100.0
+    assert np.median(np.abs(ta - tb)[m]) < 0.02
+    assert np.percentile(np.abs(ta - tb)[m], 90) < 0.2
