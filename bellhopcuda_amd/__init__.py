@@ -426,7 +426,7 @@ def field_run3d(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1,
         out.field = res.field.ctypes.data_as(_abi.c_float_p)
         rt = prob.run_type[0]
         nrcv = int(np.prod(prob.field_shape))
-        if rt in "Aa":
+        if rt in "AaVv":
             res.max_narr = max_narr
             res.arr = (_abi.Arrival * (nrcv * max(max_narr, 1)))()
             res.narr = np.zeros(nrcv, np.int32)
@@ -434,7 +434,7 @@ def field_run3d(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1,
             out.narr = res.narr.ctypes.data_as(_abi.c_int32_p)
             out.max_narr = max_narr
             out.allow_merging = allow_merging
-        if rt == "E":
+        if rt in "EVv":
             res.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
             res.neigen = np.zeros(1, np.int32)
             out.hits = C.cast(res.hits, C.POINTER(_abi.EigenHit))

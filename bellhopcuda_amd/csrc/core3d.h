@@ -1158,6 +1158,7 @@ __host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int 
         real azim = RadDeg * std::atan2(rayt.y, rayt.x);
         real decl = RadDeg * std::atan2(rayt.z, std::sqrt(rayt.x * rayt.x + rayt.y * rayt.y));
         sink.arr3(idx, cnst * w, phaseInt, delay, decl, azim, d.job, d.is, d.ntop, d.nbot, P);
+        if(P.run0 == 'V' || P.run0 == 'v') sink.eig3(itheta, ir, iz, d.job, d.is, P); // IsAlsoEigenraysRun
         sink.count();
         return;
     } else if constexpr(RT == RT_EIG) {

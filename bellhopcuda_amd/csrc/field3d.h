@@ -570,7 +570,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     bhc_arrival *d_arr = nullptr;
     bhc_eigen_hit *d_hits = nullptr;
     const char r0 = p->RunType[0];
-    const int rt = (r0 == 'A' || r0 == 'a') ? RT_ARR : (r0 == 'E') ? RT_EIG : RT_TL;
+    const int rt = (r0 == 'A' || r0 == 'a' || r0 == 'V' || r0 == 'v') ? RT_ARR : (r0 == 'E') ? RT_EIG : RT_TL;
+    const bool hits_too = (r0 == 'E' || r0 == 'V' || r0 == 'v');
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(s);
         void *ptrs[] = {d_blob, d_Rz, d_Sz, d_field, d_stats, d_work, d_xend, d_cnt, d_errw,
@@ -627,7 +628,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         TRY(dalloc(&d_arr, nrcv * (size_t)out->max_narr));
         TRY(dalloc(&d_narr, nrcv));
         HTRY(hipMemsetAsync(d_ncand, 0, nrcv * sizeof(int32_t), s));
-    } else if(rt == RT_EIG) {
+    }
+    if(hits_too) {
         TRY(dalloc(&d_hits, (size_t)out->eigen_memsize));
         TRY(dalloc(&d_neigen, 1));
         HTRY(hipMemsetAsync(d_neigen, 0, sizeof(int32_t), s));
@@ -658,7 +660,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     O.cand_cap = cand_cap;
     O.hits = d_hits;
     O.neigen = d_neigen;
-    O.eigen_memsize = rt == RT_EIG ? out->eigen_memsize : 0;
+    O.eigen_memsize = hits_too ? out->eigen_memsize : 0;
     // split the rays into passes that fit the segment buffer, from the
     // segments per ray of earlier runs on this context (a pass that still
     // overflows is halved and retraced)
@@ -765,7 +767,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         HTRY(hipMemcpyAsync(out->narr, d_narr, nrcv * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     }
     int32_t ne = 0;
-    if(rt == RT_EIG) {
+    if(hits_too) {
         HTRY(hipMemcpyAsync(&ne, d_neigen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         HTRY(hipStreamSynchronize(s));
         *out->neigen = ne;
@@ -811,11 +813,11 @@ extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_fiel
     if(!ctx || !prob || !out) return fail("bhc_field_run3d: null argument");
     if(prob->precision != 64 && prob->precision != 32) return fail("bhc_field_run3d: precision must be 32 or 64");
     char r0 = prob->RunType[0], r1 = prob->RunType[1];
-    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'A' && r0 != 'a' && r0 != 'E')
+    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'A' && r0 != 'a' && r0 != 'E' && r0 != 'V' && r0 != 'v')
         return fail("bhc_field_run3d: run type '%c' is not a TL, arrivals or eigenray run type", r0);
-    if((r0 == 'A' || r0 == 'a') && (!out->arr || !out->narr || out->max_narr < 1))
+    if((r0 == 'A' || r0 == 'a' || r0 == 'V' || r0 == 'v') && (!out->arr || !out->narr || out->max_narr < 1))
         return fail("bhc_field_run3d: arrivals run needs arr, narr and max_narr >= 1");
-    if(r0 == 'E' && (!out->hits || !out->neigen || out->eigen_memsize < 1))
+    if((r0 == 'E' || r0 == 'V' || r0 == 'v') && (!out->hits || !out->neigen || out->eigen_memsize < 1))
         return fail("bhc_field_run3d: eigenray run needs hits, neigen and eigen_memsize >= 1");
     if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B' && r1 != 'g' && r1 != 'b')
         return fail("bhc_field_run3d: beam type '%c' is not a geometric beam", r1);

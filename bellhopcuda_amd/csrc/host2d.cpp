@@ -637,9 +637,10 @@ extern "C" int bhc_preprocess2d(const bhc_env2d *env, bhc_problem_owner **out)
     if(env->precision != 64 && env->precision != 32) return hfail("precision must be 32 or 64");
     if(env->NSz < 1 || env->NRz < 1 || env->NRr < 1 || env->Nalpha < 1)
         return hfail("sources, receivers and launch angles must be non-empty");
-    // Run types of the reference (src/module/runtype.hpp): 'C'oherent,
-    // 'S'emicoherent, 'I'ncoherent TL, 'R'ay, 'E'igenray, 'A'/'a' arrivals.
-    if(!std::strchr("CSIREAa", env->RunType[0]) || env->RunType[0] == '\0')
+    // Run types of the reference (src/module/runtype.hpp:55-68): 'C'oherent,
+    // 'S'emicoherent, 'I'ncoherent TL, 'R'ay, 'E'igenray, 'A'/'a' arrivals,
+    // 'V'/'v' arrivals and eigenrays together.
+    if(!std::strchr("CSIREAaVv", env->RunType[0]) || env->RunType[0] == '\0')
         return hfail("run type '%c' is not supported", env->RunType[0]);
     // Influence types on this path are the geometric ones (hat / Gaussian,
     // Cartesian / ray-centred); Cerveny ('R', 'C') and simple Gaussian ('S')

@@ -255,13 +255,14 @@ extern "C" int bhc_preprocess3d(const bhc_env3d *e, bhc_problem3d_owner **out)
     if(e->NSx < 1 || e->NSy < 1 || e->NSz < 1 || e->NRz < 1 || e->NRr < 1 || e->Ntheta < 1 || e->Nalpha < 1
        || e->Nbeta < 1)
         return fail3("sources, receivers and launch angles must be non-empty");
-    // TL, arrivals ('A' ASCII / 'a' binary file), eigenray and ray run types with
-    // geometric beams, Cartesian (src/module/runtype.hpp maps '^' and ' ' to
-    // 'G') or ray-centred ('g', 'b'). The combined arrivals + eigenray run
-    // types ('V', 'v') are not on this 3D path.
+    // TL, arrivals ('A' ASCII / 'a' binary file), eigenray, arrivals + eigenray
+    // ('V', 'v') and ray run types with geometric beams, Cartesian
+    // (src/module/runtype.hpp maps '^' and ' ' to 'G') or ray-centred ('g', 'b').
     const char r0 = e->RunType[0];
-    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'R' && r0 != 'A' && r0 != 'a' && r0 != 'E')
-        return fail3("3D run type '%c' is not supported (C / S / I TL, A / a arrivals, E eigenrays, R rays)", r0);
+    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'R' && r0 != 'A' && r0 != 'a' && r0 != 'E' && r0 != 'V'
+       && r0 != 'v')
+        return fail3("3D run type '%c' is not supported (C / S / I TL, A / a arrivals, E eigenrays, V / v both, "
+                     "R rays)", r0);
     const char r1 = e->RunType[1];
     if(r1 != 'G' && r1 != '^' && r1 != ' ' && r1 != 'B' && r1 != 'g' && r1 != 'b')
         return fail3("3D beam type '%c' is not supported (geometric beams: G / B Cartesian, g / b ray-centred)", r1);

@@ -998,11 +998,12 @@ static void InfluenceGeoCore3(real s, real n1, real n2, m2 dq, cpx dtau, int ith
         RecordEigenHit3(itheta, ir, iz, is, &inf->init, O);
         return;
     }
-    if(rt == 'A' || rt == 'a') {
+    if(rt == 'A' || rt == 'a' || rt == 'V' || rt == 'v') {
         size_t b = field_addr3(O->Q, &inf->init, itheta, iz, ir);
         AddArrAt(&O->arr[b * O->max_narr], &O->narr[b], O->max_narr, O->allow_merging, cnst * w, inf->omega,
                  phaseInt, delay, inf->init.SrcDeclAngle, inf->init.SrcAzimAngle, RcvrDeclAngle, RcvrAzimAngle,
                  p1->NumTopBnc, p1->NumBotBnc);
+        if(rt == 'V' || rt == 'v') RecordEigenHit3(itheta, ir, iz, is, &inf->init, O); /* IsAlsoEigenraysRun */
         return;
     }
     float fr, fi;
@@ -1288,7 +1289,8 @@ static void *field_worker3(void *arg)
 int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthreads, int job_begin, int job_end)
 {
     char rt0 = env->RunType[0], rt1 = env->RunType[1];
-    if(rt0 != 'C' && rt0 != 'S' && rt0 != 'I' && rt0 != 'A' && rt0 != 'a' && rt0 != 'E') return -1;
+    if(rt0 != 'C' && rt0 != 'S' && rt0 != 'I' && rt0 != 'A' && rt0 != 'a' && rt0 != 'E' && rt0 != 'V' && rt0 != 'v')
+        return -1;
     if(rt1 != 'G' && rt1 != '^' && rt1 != ' ' && rt1 != 'B' && rt1 != 'g' && rt1 != 'b') return -1;
     Prob3 Q;
     if(prob3_from_env(env, &Q)) return -1;
@@ -1303,8 +1305,10 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
     O.hits = out->hits;
     O.neigen = (_Atomic int32_t *)out->neigen;
     O.eigen_memsize = out->eigen_memsize;
-    if((rt0 == 'A' || rt0 == 'a') && (!O.arr || !O.narr || O.max_narr < 1)) { prob3_free(&Q); return -1; }
-    if(rt0 == 'E' && (!O.hits || !O.neigen)) { prob3_free(&Q); return -1; }
+    int arr_run = rt0 == 'A' || rt0 == 'a' || rt0 == 'V' || rt0 == 'v';
+    int eig_run = rt0 == 'E' || rt0 == 'V' || rt0 == 'v';
+    if(arr_run && (!O.arr || !O.narr || O.max_narr < 1)) { prob3_free(&Q); return -1; }
+    if(eig_run && (!O.hits || !O.neigen)) { prob3_free(&Q); return -1; }
     O.isGaussian = (rt1 == 'B' || rt1 == 'b');
     O.isRayCen = (rt1 == 'g' || rt1 == 'b');
     atomic_init(&O.contrib, 0);
@@ -1327,7 +1331,7 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
     J.job_end = job_end;
     J.stats = stats;
     nthreads = resolve_threads(nthreads);
-    if((rt0 == 'A' || rt0 == 'a') && O.allow_merging) nthreads = 1; /* src/mode/arr.hpp:79 */
+    if(arr_run && O.allow_merging) nthreads = 1; /* src/mode/arr.hpp:79 */
     pthread_t *th = xcalloc(nthreads, sizeof(pthread_t));
     for(int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, field_worker3, &J);
     for(int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);

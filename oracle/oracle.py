@@ -171,7 +171,7 @@ def field_run3d(env, nthreads=0, job_begin=0, job_end=-1, max_narr=0, allow_merg
     r = FieldResult()
     out = _abi.FieldOut()
     rt = env.run_type[0]
-    if rt in "Aa":
+    if rt in "AaVv":
         r.max_narr = max_narr
         r.arr = (_abi.Arrival * (nrcv * max_narr))()
         r.narr = np.zeros(nrcv, np.int32)
@@ -179,7 +179,7 @@ def field_run3d(env, nthreads=0, job_begin=0, job_end=-1, max_narr=0, allow_merg
         out.narr = r.narr.ctypes.data_as(_abi.c_int32_p)
         out.max_narr = max_narr
         out.allow_merging = allow_merging
-    if rt == "E":
+    if rt in "EVv":
         r.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
         r.neigen = np.zeros(1, np.int32)
         out.hits = C.cast(r.hits, C.POINTER(_abi.EigenHit))

@@ -150,6 +150,29 @@ def test_eigen_hits_parity(gpu_ctx):
     assert key(np.ctypeslib.as_array(o.hits)[:no]) == key(np.ctypeslib.as_array(g.hits)[:ng])
 
 
+def test_arrivals_and_eigen_parity(gpu_ctx):
+    """Run type 'V' (arrivals and eigenrays together, IsAlsoEigenraysRun):
+    the arrivals equal the single-threaded oracle's, the eigen hits its set."""
+    _announce("munk_arrivals_and_eigen")
+    env = munk_env(nbeams=201, nrz=5, nrr=11, run_type="VG")
+    max_narr = 50
+    o = oracle.field_run(env, nthreads=1, max_narr=max_narr, allow_merging=1, eigen_memsize=200000)
+    g = bh.field_run(env, ctx=gpu_ctx, max_narr=max_narr, allow_merging=1, eigen_memsize=200000)
+    ao, no = _arrivals(o, env.field_shape, max_narr)
+    ag, ng = _arrivals(g, env.field_shape, max_narr)
+    assert np.array_equal(no, ng) and no.sum() > 0
+    for idx in zip(*np.nonzero(no)):
+        n = no[idx]
+        x, y = ao[idx][:n], ag[idx][:n]
+        assert np.array_equal(x["NTopBnc"], y["NTopBnc"]) and np.array_equal(x["NBotBnc"], y["NBotBnc"])
+        assert np.allclose(x["delay"], y["delay"], rtol=1e-6, atol=1e-9)
+        assert np.allclose(x["a"], y["a"], rtol=1e-5, atol=1e-12)
+    ne_o, ne_g = int(o.neigen[0]), int(g.neigen[0])
+    assert ne_o == ne_g > 0
+    key = lambda h: sorted(zip(h["ialpha"], h["is_"], h["ir"], h["iz"], h["isz"]))  # noqa: E731
+    assert key(np.ctypeslib.as_array(o.hits)[:ne_o]) == key(np.ctypeslib.as_array(g.hits)[:ne_g])
+
+
 @pytest.mark.parametrize("name", ["munk_hat_cart_f64", "munk_gauss_raycen", "seamount_spline_raycen"])
 def test_scatter_mappings_agree(name, gpu_ctx):
     """The three scatter mappings (range-binned LDS tiles, one wavefront per

@@ -217,3 +217,24 @@ def test_eigen_retrace3d_parity(gpu_ctx):
         for f in ("p", "q"):
             a, b = ao[k, :m][f], ag[k, :m][f]
             assert np.allclose(a, b, rtol=1e-6, atol=1e-6 * max(1.0, float(np.max(np.abs(a))))), (k, f)
+
+
+def test_arrivals_and_eigen3d_parity(gpu_ctx):
+    """Run type 'V' in 3D: arrivals and eigen hits from the same run
+    (IsAlsoEigenraysRun), each equal to the oracle's."""
+    print("\n[gpu-case-3d-arr] munk_arrivals_and_eigen", flush=True)
+    env = munk_env3d(nalpha=61, nbeta=5, beta=(-20.0, 20.0), ntheta=2, nrz=11, nrr=21, run_type="VG   3")
+    o = oracle.field_run3d(env, max_narr=20, allow_merging=1, eigen_memsize=200000)
+    g = bh.field_run3d(env, ctx=gpu_ctx, max_narr=20, allow_merging=1, eigen_memsize=200000)
+    assert np.array_equal(np.minimum(o.narr, 20), np.minimum(g.narr, 20)) and o.narr.sum() > 0
+    A = np.ctypeslib.as_array(o.arr).reshape(-1, 20)
+    B = np.ctypeslib.as_array(g.arr).reshape(-1, 20)
+    for r in np.nonzero(o.narr)[0]:
+        n = min(o.narr[r], 20)
+        for f in ("a", "Phase", "delay", "NTopBnc", "NBotBnc"):
+            assert np.array_equal(A[r, :n][f], B[r, :n][f]), (r, f)
+    no, ng = int(o.neigen[0]), int(g.neigen[0])
+    assert no == ng > 0
+    key = ["is_", "itheta", "ir", "iz", "isx", "isy", "isz", "ialpha", "ibeta"]
+    assert np.array_equal(np.sort(np.ctypeslib.as_array(o.hits)[:no], order=key),
+                          np.sort(np.ctypeslib.as_array(g.hits)[:ng], order=key))

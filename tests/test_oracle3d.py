@@ -208,7 +208,7 @@ def test_preprocess3d_matches_oracle(precision):
 
 
 def test_preprocess3d_rejects_unsupported():
-    for rt in ("CR   3", "CC   3", "VB   3", "vB   3"):
+    for rt in ("CR   3", "CC   3", "XB   3", "CS   3"):
         with pytest.raises(bh.BhcError):
             bh.Problem3D(free_env3d(run_type=rt))
     e = free_env3d()
