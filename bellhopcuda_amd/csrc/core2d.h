@@ -1192,6 +1192,20 @@ template<typename real> __host__ __device__ inline int RToIR(real r, const Prob2
 }
 
 // Cartesian receiver (ir, iz) of a segment: influence.hpp:1298-1386 (2D).
+// Receiver (ir, iz) at range rr = Rr[ir], already inside the segment's [lo, hi).
+template<typename real, int RT, typename Sink, bool FAST = false>
+__host__ __device__ inline void item_cart_rr(const SegDesc<real> &d, int ir, real rr, int iz, const Prob2D<real> &P,
+                                             const float *Rz, Sink &sink)
+{
+    int tiz = (P.run4 == 'I') ? ir : iz;
+    real zr = (real)Rz[tiz];
+    if(zr < d.zmin || zr > d.zmax) return;
+    v2<real> xrr = V(rr, zr) - d.xa;
+    real s = dot(xrr, d.ta) / d.rlen;
+    real n1 = std::fabs(dot(xrr, d.tb));
+    contrib<real, RT, Sink, FAST>(d, s, n1, ir, iz, P, sink);
+}
+
 template<typename real, int RT, typename Sink, bool FAST = false>
 __host__ __device__ inline void item_cart_at(const SegDesc<real> &d, int ir, int iz, const Prob2D<real> &P,
                                              const real *G, const float *Rz, Sink &sink)
@@ -1372,7 +1386,10 @@ __host__ __device__ inline void seg_items(const SegDesc<real> &d, const Prob2D<r
     for(int j = 0; j < nr; ++j) {
         int ir, i0, i1;
         cart_column(d, j, P, G, ir, i0, i1);
-        for(int iz = i0; iz <= i1; ++iz) item_cart_at<real, RT, Sink, FAST>(d, ir, iz, P, G, Rz, sink);
+        // the range test depends on the column only: once per column
+        const real rr = G[P.o_Rr + ir];
+        if(!(rr >= d.lo && rr < d.hi)) continue;
+        for(int iz = i0; iz <= i1; ++iz) item_cart_rr<real, RT, Sink, FAST>(d, ir, rr, iz, P, Rz, sink);
     }
 }
 

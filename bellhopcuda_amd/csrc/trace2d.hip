@@ -1586,7 +1586,9 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bd, c
                                                      unsigned long long &all_steps)
 {
     const int lane = threadIdx.x;
-    const Prob2D<real> &P = Bd.probs[pid];
+    // a copy: through a reference the problem's fields would be reloaded after
+    // every descriptor store (they might alias)
+    const Prob2D<real> P = Bd.probs[pid];
     const real *B = stage_problem(P, Bd.lds_cap, lds);
     const real *G = P.base;
     const int nr = P.njobs;
@@ -1809,7 +1811,9 @@ __global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
         const SegDesc<real> d = segs[s].d;
         if(d.nitems <= 0) continue;
-        const Prob2D<real> &P = Bd.probs[d.prob];
+        // a copy: read through a reference, the problem's fields would be
+        // reloaded after every field atomic (they might alias)
+        const Prob2D<real> P = Bd.probs[d.prob];
         DevSink<real> sink{&O, &P, 0};
         seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
         nc += sink.nc;
