@@ -237,26 +237,32 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
 // Scatter kernel: one wavefront per segment. Rows are the (range, bearing)
 // pairs the segment's receiver walk visits (influence.hpp:1298-1386, 3D);
 // each row's depth window is flattened over the lanes.
-template<typename real, bool FAST, int RT = RT_TL>
+// STAGED: the receiver ranges, bearing table and depths are copied to LDS (the
+// launch checks they fit) and read only from there -- compile-time, so the
+// reads are LDS reads rather than generic ones.
+template<typename real, bool FAST, int RT = RT_TL, bool STAGED = true>
 __global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
 {
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
     const int nwaves = gridDim.x * (blockDim.x / WAVE);
-    // receiver ranges and bearing table staged in LDS when they fit
     extern __shared__ __align__(16) unsigned char lds3[];
-    const real *Rr = P.base + P.o_Rr, *trcv = P.base + P.o_trcv;
-    const float *Rz = P.Rz;
-    if(P.stage_tables) {
+    const real *Rr, *trcv;
+    const float *Rz;
+    if constexpr(STAGED) {
         real *L = reinterpret_cast<real *>(lds3);
         float *Lz = reinterpret_cast<float *>(L + P.NRr + 2 * P.Ntheta);
-        for(int i = threadIdx.x; i < P.NRr; i += blockDim.x) L[i] = Rr[i];
-        for(int i = threadIdx.x; i < 2 * P.Ntheta; i += blockDim.x) L[P.NRr + i] = trcv[i];
-        for(int i = threadIdx.x; i < P.NRz; i += blockDim.x) Lz[i] = Rz[i];
+        for(int i = threadIdx.x; i < P.NRr; i += blockDim.x) L[i] = P.base[P.o_Rr + i];
+        for(int i = threadIdx.x; i < 2 * P.Ntheta; i += blockDim.x) L[P.NRr + i] = P.base[P.o_trcv + i];
+        for(int i = threadIdx.x; i < P.NRz; i += blockDim.x) Lz[i] = P.Rz[i];
         __syncthreads();
         Rr = L;
         trcv = L + P.NRr;
         Rz = Lz;
+    } else {
+        Rr = P.base + P.o_Rr;
+        trcv = P.base + P.o_trcv;
+        Rz = P.Rz;
     }
     __shared__ int cand_all[4][WAVE];   // candidate bearings of each wavefront's segment
     int *cand = cand_all[(threadIdx.x / WAVE) & 3];
@@ -731,16 +737,21 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             else hipLaunchKernelGGL((k_scatter3rc<real, RT_TL>), dim3(sblocks), dim3(256), 0, s, P, O);
         } else if(rt == RT_ARR) {
             if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
-            else hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else if(P.stage_tables) hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR, false>), dim3(sblocks), dim3(256), 0, s, P, O);
         } else if(rt == RT_EIG) {
             if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
-            else hipLaunchKernelGGL((k_scatter3d<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else if(P.stage_tables) hipLaunchKernelGGL((k_scatter3d<real, false, RT_EIG, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, false, RT_EIG, false>), dim3(sblocks), dim3(256), 0, s, P, O);
         } else if(c->scatter3_lanes) {
             if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d_lanes<real, false>), dim3(sblocks), dim3(256), lds, s, P, O);
             else hipLaunchKernelGGL((k_scatter3d_lanes<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+        } else if(P.stage_tables) {
+            if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d<real, false, RT_TL, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, true, RT_TL, true>), dim3(sblocks), dim3(256), lds, s, P, O);
         } else {
-            if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d<real, false>), dim3(sblocks), dim3(256), lds, s, P, O);
-            else hipLaunchKernelGGL((k_scatter3d<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
+            if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d<real, false, RT_TL, false>), dim3(sblocks), dim3(256), 0, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d<real, true, RT_TL, false>), dim3(sblocks), dim3(256), 0, s, P, O);
         }
         HTRY(hipGetLastError());
         HTRY(hipEventRecord(c->pb, s));

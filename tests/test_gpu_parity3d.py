@@ -32,6 +32,8 @@ CASES = {
                                                   run_type="CG   3"),
     "shelf_canyon_hexahedral_f32": lambda: c3_env(nalpha=61, nbeta=36, ntheta=8, nrz=31, nrr=81, nxy=51,
                                                   precision=32),
+    # 8000 receiver ranges: the tables exceed the LDS budget, the scatter reads them from HBM
+    "free_gauss_unstaged_tables": lambda: free_env3d(nalpha=45, nbeta=19, ntheta=2, nrz=5, nrr=8000),
     "free_raycen_gauss_incoherent": lambda: free_env3d(run_type="Ib   3"),
     "munk_raycen_hat": lambda: munk_env3d(nalpha=151, nbeta=7, run_type="Cg   3"),
     "shelf_canyon_raycen_gauss": lambda: c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=51,
