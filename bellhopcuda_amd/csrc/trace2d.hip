@@ -2225,9 +2225,13 @@ template<typename real> int sweep_tune(bhc_sweep *sw)
     int best_blocks = sw->scatter_blocks, best_lds = sw->lds_bytes, best_lanes = sw->scatter_lanes;
     double best = 1e300;
     const int grids[3] = {cus * 4, cus * 8, cus * 16};
-    const int ldss[2] = {sw->lds_bytes_max, 0};
+    // staging budgets: the full one (one wavefront per SIMD), half of it (two
+    // wavefronts per SIMD, problems with larger tables unstaged) and none
+    const int half = (sw->lds_bytes_max / 2) / (int)(sw->precision == 64 ? sizeof(double) : sizeof(float))
+        * (int)(sw->precision == 64 ? sizeof(double) : sizeof(float));
+    const int ldss[3] = {sw->lds_bytes_max, half, 0};
     for(int mode = 0; mode < 3; ++mode)
-        for(int li = 0; li < (sw->lds_bytes_max > 0 ? 2 : 1); ++li)
+        for(int li = 0; li < (sw->lds_bytes_max > 0 ? 3 : 1); ++li)
             for(int gi = 0; gi < 3; ++gi) {
                 sw->scatter_lanes = mode;
                 sw->scatter_blocks = grids[gi];
