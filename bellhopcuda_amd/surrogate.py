@@ -251,6 +251,20 @@ def predict_fields(mlp, thetas, envs, ref):
     return np.clip(pred, TL_MIN, TL_MAX)
 
 
+def screen_sparse(mlp, thetas, envs, ref, spread_db=1.0, stride=(5, 8)):
+    """The screen's decision from a receiver subsample (every stride[0]-th
+    depth and stride[1]-th range): per environment, the RMS of the members'
+    spread there, and whether it exceeds `spread_db` (fall back to the
+    tracer). Costs 1 / (stride[0] * stride[1]) of a full prediction."""
+    rz, rr = np.asarray(envs[0].Rz)[::stride[0]], np.asarray(envs[0].Rr_km)[::stride[1]]
+    x = np.concatenate([inputs(th, rz, rr) for th in thetas])
+    pred = mlp.predict(x).reshape(mlp.members, len(thetas), len(rz), len(rr))
+    pred = np.clip(pred * TL_SCALE + ref[::stride[0], ::stride[1]][None, None], TL_MIN, TL_MAX)
+    spread = pred.max(0) - pred.min(0)
+    rms_spread = np.sqrt(np.mean(spread ** 2, axis=(1, 2)))
+    return rms_spread > spread_db, rms_spread
+
+
 def screen(pred, spread_db=1.0):
     """Per environment: ensemble mean and whether it falls back to the tracer
     (RMS over receivers of the members' spread above `spread_db`)."""

@@ -64,3 +64,25 @@ def test_mlp_training_reduces_loss():
     pred = m.predict(X[:2000])
     rms = np.sqrt(np.mean((pred.mean(0) - Y[:2000]) ** 2))
     assert rms < 0.25 * np.std(Y)
+
+
+def test_sparse_screen_matches_full_screen_at_unit_stride():
+    """The subsampled screen (ensemble spread on every stride-th receiver) is
+    the full screen when the stride is 1, and with a stride its spread
+    estimate stays close to the full one."""
+    _announce("surrogate_sparse_screen")
+    from bellhopcuda_amd import surrogate as sg
+    m = sg.Mlp(members=4, max_batch=16384, seed=5)
+    m.set_fourier(sg.fourier_matrix(3))
+    P, _ = m.params()
+    rng = np.random.default_rng(4)
+    m.set_params((P + rng.normal(scale=0.02, size=P.shape)).astype(np.float32))  # members differ
+    th = sg.draw_params(6, 99)
+    envs = [sg.environment(t, nbeams=11, nrz=11, nrr=41) for t in th]
+    ref = np.full((11, 41), 80.0)
+    pred = sg.predict_fields(m, th, envs, ref)
+    _, fb_full, rms_full = sg.screen(pred, 1.0)
+    fb_1, rms_1 = sg.screen_sparse(m, th, envs, ref, 1.0, stride=(1, 1))
+    assert np.array_equal(fb_full, fb_1) and np.allclose(rms_full, rms_1, rtol=1e-5)
+    _, rms_s = sg.screen_sparse(m, th, envs, ref, 1.0, stride=(2, 4))
+    assert np.allclose(rms_s, rms_full, rtol=0.5)
