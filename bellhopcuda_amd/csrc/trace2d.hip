@@ -1581,10 +1581,14 @@ __device__ inline void flush_problem_err(Err &e, uint32_t *perr)
 // wavefront. The SSP type is a template parameter, as in the single-problem
 // kernel, so that each type runs specialised code.
 template<typename real, char ST>
-__device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bd, const DevOut &O, int pid,
+__device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref, const DevOut &Oref, int pid,
                                                      const RaySave<real> *resume, SegBlock &blk, unsigned char *lds,
                                                      unsigned long long &all_steps)
 {
+    // copies, for the same reason as the problem's below: through the
+    // references their fields would be reloaded after every descriptor store
+    const DevOut O = Oref;
+    const BatchDev<real> Bd = Bdref;
     const int lane = threadIdx.x;
     // a copy: through a reference the problem's fields would be reloaded after
     // every descriptor store (they might alias)
