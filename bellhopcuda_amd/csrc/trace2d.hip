@@ -2226,32 +2226,42 @@ template<typename real> int sweep_tune(bhc_sweep *sw)
 {
     int nsub = std::min(sw->nprob, 64);
     int cus = sw->ctx->num_cus;
-    int best_blocks = sw->scatter_blocks, best_lds = sw->lds_bytes, best_lanes = sw->scatter_lanes;
-    double best = 1e300;
     const int grids[3] = {cus * 4, cus * 8, cus * 16};
     // staging budgets: the full one (one wavefront per SIMD), half of it (two
     // wavefronts per SIMD, problems with larger tables unstaged) and none
-    const int half = (sw->lds_bytes_max / 2) / (int)(sw->precision == 64 ? sizeof(double) : sizeof(float))
-        * (int)(sw->precision == 64 ? sizeof(double) : sizeof(float));
+    const int rsz = (int)(sw->precision == 64 ? sizeof(double) : sizeof(float));
+    const int half = (sw->lds_bytes_max / 2) / rsz * rsz;
     const int ldss[3] = {sw->lds_bytes_max, half, 0};
+    auto timed = [&](int mode, int blocks, int lds, double &t) -> int {
+        sw->scatter_lanes = mode;
+        sw->scatter_blocks = blocks;
+        sw->lds_bytes = lds;
+        for(int rep = 0; rep < 2; ++rep) {
+            if(sweep_core<real>(sw, nsub, false)) return -1;
+            if(rep == 1) t = sw->trace_ms + sw->scatter_ms;
+        }
+        return 0;
+    };
+    // the staging budget acts on the trace and the mapping / grid on the
+    // scatter: choose the scatter's at the full budget, then the budget
+    int best_blocks = grids[0], best_lanes = 1, best_lds = ldss[0];
+    double best = 1e300, t = 0.0;
     for(int mode = 0; mode < 3; ++mode)
-        for(int li = 0; li < (sw->lds_bytes_max > 0 ? 3 : 1); ++li)
-            for(int gi = 0; gi < 3; ++gi) {
-                sw->scatter_lanes = mode;
-                sw->scatter_blocks = grids[gi];
-                sw->lds_bytes = ldss[li];
-                double t = 0.0;
-                for(int rep = 0; rep < 2; ++rep) {
-                    if(sweep_core<real>(sw, nsub, false)) return -1;
-                    if(rep == 1) t = sw->trace_ms + sw->scatter_ms;
-                }
-                if(t < best) {
-                    best = t;
-                    best_blocks = grids[gi];
-                    best_lds = ldss[li];
-                    best_lanes = mode;
-                }
+        for(int gi = 0; gi < 3; ++gi) {
+            if(timed(mode, grids[gi], ldss[0], t)) return -1;
+            if(t < best) {
+                best = t;
+                best_blocks = grids[gi];
+                best_lanes = mode;
             }
+        }
+    for(int li = 1; li < (sw->lds_bytes_max > 0 ? 3 : 1); ++li) {
+        if(timed(best_lanes, best_blocks, ldss[li], t)) return -1;
+        if(t < best) {
+            best = t;
+            best_lds = ldss[li];
+        }
+    }
     sw->scatter_blocks = best_blocks;
     sw->lds_bytes = best_lds;
     sw->scatter_lanes = best_lanes;
