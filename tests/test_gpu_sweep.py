@@ -143,14 +143,15 @@ def test_sweep_graph_replays_match_eager(gpu_ctx):
     sw.release()
 
 
+@pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("mapping", [0, 1, 2])
-def test_sweep_scatter_mappings(mapping, gpu_ctx):
+def test_sweep_scatter_mappings(mapping, precision, gpu_ctx):
     """Every scatter mapping of the batched passes (a wavefront per segment, a
     lane per segment, 64 segments' receivers flattened over a wavefront) gives
     the single-problem runs' contribution counts and TL; the mix holds
     Cartesian hat / Gaussian and ray-centred Gaussian problems."""
-    _announce("sweep_scatter_mapping_%d" % mapping)
-    envs = _mixed(64)
+    _announce("sweep_scatter_mapping_%d_f%d" % (mapping, precision))
+    envs = _mixed(precision)
     probs = [bh.Problem(e) for e in envs]
     sw = bh.Sweep(probs, gpu_ctx, stats=True)
     sw.set_scatter(mapping)
