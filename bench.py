@@ -1,4 +1,4 @@
-"""Throughput benchmark of the 2D ray/beam-tracing hot path on one MI355X.
+"""Throughput benchmark of the ray/beam-tracing hot path on one MI355X.
 
 Workloads (one "step" = one pass of the hot path over one batch of input):
 
@@ -14,13 +14,18 @@ Workloads (one "step" = one pass of the hot path over one batch of input):
 * munk: one Munk-profile TL field, 2001 hat beams, 201 x 1001 receivers.
 * c3: one full-3D Gaussian-beam TL field -- hexahedral SSP, 201 x 201
   bathymetry grid, 181 x 1001 beams, 36 bearings x 101 depths x 1001 ranges.
+* surrogate: the surrogate-screened sweep -- an MLP ensemble trained on
+  tracer fields of a 14-parameter family answers held-out environments where
+  its members agree, the batched tracer where they do not.
 
 Inputs are resident in HBM before the timed region. Prints one JSON line:
-fields/s (value), rays/s, the FP64 roofline of the field kernels (algorithmic
-operations F = f_I R + f_S S + f_C C, f_* from the op-counting instantiation of
-the same core, R, S, C counted by the kernels, over the kernels' time from HIP
-events) and the CPU baseline (the oracle's thread pool on a bounded sample of
-rays of the same workload).
+fields/s (value), rays/s, the roofline of the dominant field kernel with the
+other kernel and the whole path beside it (algorithmic FP64 operations
+F = f_I R + f_S S + f_C C, f_* from the op-counting instantiation of the same
+core, R, S, C counted by the kernels; the trace kernel does f_I R + f_S S, the
+scatter f_C C; kernel times from HIP events around every launch) and the CPU
+baseline (the oracle's thread pool on a bounded sample of rays of the same
+workload).
 """
 import argparse
 import ctypes as C
