@@ -2,12 +2,22 @@
  * ORACLE -- test infrastructure only. Not part of the product.
  *
  * Plain-C restatement of the reference's full-3D ray/beam tracing path
- * (bellhopcxx 3D, CFG = geometric Cartesian beams, O3D = R3D = true): 3D ray
- * steps with 2x2 p / q matrices and torsion, triangulated boundary grids,
- * depth-only and hexahedral sound speed, Cartesian hat / Gaussian influence
- * with receiver bearings, TL. This file includes the 2D oracle so that the
- * depth-only SSP evaluation, the crci / spline / PCHIP preprocessing, the
- * halfspace setup and the reflection-table lookup are the 2D oracle's own.
+ * (bellhopcxx 3D, CFG = geometric beams, O3D = R3D = true): 3D ray steps with
+ * 2x2 p / q matrices and torsion, triangulated boundary grids, depth-only and
+ * hexahedral sound speed, Cartesian and ray-centred hat / Gaussian influence
+ * with receiver bearings; TL, arrivals (AddArr with azimuths), eigen hits,
+ * the ray run type and the eigenray re-trace. This file includes the 2D
+ * oracle so that the depth-only SSP evaluation, the crci / spline / PCHIP
+ * preprocessing, the halfspace setup, the reflection-table lookup and the
+ * AddArr merge rule are the 2D oracle's own.
+ *
+ * What pins it (tests/test_oracle3d.py): free-space spherical spreading
+ * (Cartesian and ray-centred beams), the 2D oracle's field on a depth-only
+ * environment, flat and laterally uniform grids, straight rays in ray mode,
+ * and the reference's own 3D arrivals fixture
+ * (test/individual/analysis_3d/example_analysis_3d.arr.txt), reproduced end
+ * to end. Parity unpinned: the torsion update and the p rotation at
+ * reflections (second order in those checks).
  *
  * The reference computes vectors and 2x2 matrices with glm (an un-vendored
  * submodule, absent here). Its published semantics are restated below:
