@@ -1790,7 +1790,7 @@ __global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut
     for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
         const SegDesc<real> d = segs[s].d;
         if(d.nitems <= 0) continue;
-        const Prob2D<real> &P = Bd.probs[__builtin_amdgcn_readfirstlane(d.prob)];
+        const Prob2D<real> P = Bd.probs[__builtin_amdgcn_readfirstlane(d.prob)];
         DevSink<real> sink{&O, &P, 0};
         scatter_seg<real, RT_TL, FAST>(d, P, sink, lane);
         nc += sink.nc;
@@ -1888,7 +1888,7 @@ __global__ void __launch_bounds__(128) k_scatter_flat(BatchDev<real> Bd, DevOut 
                 const int oseg = __shfl(seg, own);
                 if(t < itot) {
                     const SegDesc<real> &d = S[oseg].d;
-                    const Prob2D<real> &P = Bd.probs[d.prob];
+                    const Prob2D<real> P = Bd.probs[d.prob];
                     DevSink<real> sink{&O, &P, 0};
                     item_cart_at<real, RT_TL, DevSink<real>, FAST>(d, oir, oi0 + (t - oex), P, P.base, P.Rz, sink);
                     nc += sink.nc;
