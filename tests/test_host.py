@@ -299,3 +299,22 @@ def test_core_fast_phasor_tl(name, harness):
     m = tlo < 120
     assert m.sum() > 100
     assert np.max(np.abs(tlo - tlg)[m]) < 0.01
+
+
+def test_entry_points_reject_bad_arguments_before_any_device_call():
+    """Argument checks of the sweep-mapping setter and the 3D re-trace fail
+    with a message and touch no device (run here without one)."""
+    L = bh.lib()
+    assert L.bhc_sweep_set_scatter(None, 1) != 0
+    assert L.bhc_sweep_set_scatter(None, 7) != 0
+    assert b"bhc_sweep_set_scatter" in L.bhc_last_error()
+    assert L.bhc_ray_run3d_list(None, None, None, 0, None) != 0
+    assert b"bhc_ray_run3d_list" in L.bhc_last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
