@@ -48,6 +48,7 @@ def lib():
         "bhc_ctx_destroy": (None, [C.c_void_p]),
         "bhc_ctx_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "bhc_ctx_synchronize": (C.c_int, [C.c_void_p]),
+        "bhc_copy_calibrate": (C.c_int, [C.c_void_p, C.c_longlong, P(C.c_double)]),
         "bhc_last_kernel_ms": (C.c_int, [C.c_void_p, P(C.c_double)]),
         "bhc_field_run2d": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.FieldOut)]),
         "bhc_ray_run2d": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.RayOut)]),
@@ -63,6 +64,7 @@ def lib():
         "bhc_write_shd2d": (C.c_int, [C.c_char_p, P(_abi.Problem2D), C.c_char_p, _abi.c_float_p]),
         "bhc_sweep_upload2d": (C.c_int, [C.c_void_p, P(P(_abi.Problem2D)), C.c_int, C.c_int, P(C.c_void_p)]),
         "bhc_sweep_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "bhc_sweep_launch_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
         "bhc_sweep_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, P(_abi.FieldOut)]),
         "bhc_sweep_stats": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), P(C.c_longlong),
                                       P(C.c_longlong)]),
@@ -240,6 +242,7 @@ def field_run(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1, e
         if own:
             ctx.close()
     res.err, res.warn = out.err, out.warn
+    res.err_count, res.warn_count = out.err_count, out.warn_count
     res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
     res.problem = prob
     return res
@@ -323,8 +326,14 @@ class Sweep:
         self._h = h
         self.with_stats = stats
 
-    def launch(self):
-        _check(lib().bhc_sweep_launch(self.ctx._h, self._h))
+    def launch(self, p0=None, p1=None):
+        """Trace every problem, or problems [p0, p1) only."""
+        if p0 is None and p1 is None:
+            _check(lib().bhc_sweep_launch(self.ctx._h, self._h))
+        else:
+            p0 = 0 if p0 is None else p0
+            p1 = len(self.problems) if p1 is None else p1
+            _check(lib().bhc_sweep_launch_range(self.ctx._h, self._h, p0, p1))
 
     def autotune(self):
         b, l, m = C.c_int(), C.c_int(), C.c_int()

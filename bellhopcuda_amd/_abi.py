@@ -71,6 +71,7 @@ class Env2D_C(C.Structure):
         ("epsMultiplier", C.c_double), ("rLoop", C.c_double),
         ("iBeamWindow", C.c_int32), ("Nimage", C.c_int32),
         ("NSBPPts", C.c_int32), ("pad4", C.c_int32), ("sbp", c_double_p),
+        ("NBioLayers", C.c_int32), ("pad5", C.c_int32), ("bio", c_double_p),
     ]
 
 
@@ -162,6 +163,7 @@ class Env3D_C(C.Structure):
         ("RunType", C.c_char * 7), ("BeamType", C.c_char * 3), ("pad4", C.c_char * 6),
         ("deltas", C.c_double), ("box_x_km", C.c_double), ("box_y_km", C.c_double), ("box_z", C.c_double),
         ("NSBPPts", C.c_int32), ("pad5", C.c_int32), ("sbp", c_double_p),
+        ("NBioLayers", C.c_int32), ("pad6", C.c_int32), ("bio", c_double_p),
     ]
 
 

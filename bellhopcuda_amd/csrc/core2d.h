@@ -174,6 +174,7 @@ template<typename real> struct Prob2D {
     char run1;             // RunType[1]
     char run3, run4;       // RunType[3] ('X' line source), RunType[4] ('I' irregular)
     char type2;            // Beam->Type[2] curvature option
+    char type3;            // Beam->Type[3]: 'S' beam shift on halfspace reflection
     char ati_curv, bty_curv; // 'C' curvilinear
     char ati_long, bty_long; // 'L' long format (per-node geoacoustics)
     int32_t npts, Nr;
@@ -609,6 +610,44 @@ __host__ __device__ inline void Step(const RayPt<real> &ray0, RayPt<real> &ray2,
 // ---------------------------------------------------------------------------
 // Reflection (reference src/reflect.hpp, 2D)
 // ---------------------------------------------------------------------------
+
+// Beam displacement and width change on a halfspace reflection (Tindle's
+// displacement as in src/reflect.hpp:383-442, 2D): the reflected point moves
+// along the boundary by Re(delta), the delay grows by Re(delta) cos(theta)/c
+// and q by |d delta / d theta|-scaled p. Complex operations in the
+// reference's order (real / complex as a full complex division).
+template<typename real>
+__host__ __device__ inline void BeamShift(const RayPt<real> &oldp, RayPt<real> &newp, const HSd<real> &hs, v2<real> tB,
+                                          const SSPOut<real> &o, real omega)
+{
+    cpx<real> ch = realdiv(oldp.c, C(hs.cP.re, -hs.cP.im));
+    real co = oldp.t.x * oldp.c;
+    real si = oldp.t.y * oldp.c;
+    real ck = omega / oldp.c;
+    cpx<real> chsq = cmul(ch, ch);
+    cpx<real> a = rmul(FL(2.0) * hs.rho, C(FL(1.0) - chsq.re, -chsq.im));
+    cpx<real> b = C(co * co - chsq.re, -chsq.im);
+    cpx<real> d = C((hs.rho * hs.rho) * (si * si) + b.re, b.im);
+    cpx<real> sb = csqrt_(b);
+    real cco = co * co;
+    real ssi = si * si;
+    cpx<real> cksbd = cmul(rmul(ck, sb), d);
+    cpx<real> delta = C((real)0, (real)0);
+    if(std::fabs(si) >= RC<real>::EPS) delta = cdiv(rdivc(C(a.re * co, a.im * co), si), cksbd);
+    real pdelta = delta.re / (oldp.c / co);
+    cpx<real> t1 = realdiv(cco, cmul(cksbd, b));
+    cpx<real> t2 = realdiv(RL(1.0) + (cco / ssi), cksbd);
+    cpx<real> t3 = realdiv(FL(2.0) * (co * co) * ((hs.rho * hs.rho) - RL(1.0)), cmul(cksbd, d));
+    cpx<real> ddelta = cmul(a, (t1 - t2) - t3);
+    real rddelta = -ddelta.re;
+    real sddelta = rddelta / std::fabs(rddelta);
+    real theta_bot = std::atan(tB.y / tB.x);
+    newp.x.x = newp.x.x + delta.re * std::cos(theta_bot);
+    newp.x.y = newp.x.y + delta.re * std::sin(theta_bot);
+    newp.tau = C(newp.tau.re + pdelta, newp.tau.im);
+    newp.q = newp.q + sc(((sddelta * rddelta) * si) * o.c.re, oldp.p);
+}
+
 template<typename real, char ST = 0>
 __host__ __device__ inline void Reflect(const RayPt<real> &oldp, RayPt<real> &newp, const HSd<real> &hs, bool isTop,
                                         v2<real> tB, v2<real> nB, real kappa, const Prob2D<real> &P, const real *B,
@@ -706,6 +745,7 @@ __host__ __device__ inline void Reflect(const RayPt<real> &oldp, RayPt<real> &ne
         } else {
             newp.Amp = aR * oldp.Amp;
             newp.Phase = oldp.Phase + std::atan2(Refl.im, Refl.re);
+            if(P.type3 == 'S') BeamShift(oldp, newp, hs, tB, o, omega);
         }
     } else {
         e.error(4);

@@ -695,6 +695,10 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         int nj = r.second - r.first;
         int waves = (nj + WAVE - 1) / WAVE;
         int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
+        // error / warning words as the completed passes left them, restored
+        // if this trace pass overflows and its rays are traced again
+        uint32_t ew_start[4] = {0u, 0u, 0u, 0u};
+        HTRY(hipMemcpyAsync(ew_start, d_errw, sizeof(ew_start), hipMemcpyDeviceToHost, s));
         if(first) HTRY(hipEventRecord(c->ev0, s));
         HTRY(hipEventRecord(c->pa, s));
         if(rcen) hipLaunchKernelGGL((k_trace3d<real, true>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
@@ -716,8 +720,10 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             int mid = r.first + nj / 2;
             todo.push_back({mid, r.second});
             todo.push_back({r.first, mid});
-            // drop the abandoned pass's steps, keep the completed passes'
+            // drop the abandoned pass's steps and warnings, keep the completed
+            // passes' (the retried rays report theirs again)
             HTRY(hipMemcpyAsync(d_cnt, &steps_done, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+            HTRY(hipMemcpyAsync(d_errw, ew_start, sizeof(ew_start), hipMemcpyHostToDevice, s));
             HTRY(hipStreamSynchronize(s));
             continue;
         }

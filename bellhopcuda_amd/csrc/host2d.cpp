@@ -60,8 +60,9 @@ template<typename real> struct Pre {
     const bhc_env2d *env;
     real freq0;
 
-    // crci, src/module/atten.cpp:99-194 (Thorp 'T' and Francois-Garrison 'F' volume terms)
-    cx<real> crci(real c, real alpha, const char *au) const
+    // crci, src/module/atten.cpp:99-194 (Thorp 'T', Francois-Garrison 'F' and
+    // biological-layer 'B' volume terms); z selects the biological layers
+    cx<real> crci(real z, real c, real alpha, const char *au) const
     {
         real freq = freq0;
         real omega = FLH(2.0) * Lit<real>::PI * freq;
@@ -69,6 +70,12 @@ template<typename real> struct Pre {
         switch(au[0]) {
         case 'N': alphaT = alpha; break;
         case 'M': alphaT = alpha / RLH(8.6858896); break;
+        case 'm':
+            // power law in f / freq0: the run frequency is freq0, where both
+            // branches of the reference (below and above fT) give the factor 1
+            alphaT = alpha / RLH(8.6858896);
+            alphaT *= std::pow(freq / freq0, FLH(1.0));
+            break;
         case 'F': alphaT = alpha * freq / RLH(8685.8896); break;
         case 'W':
             if(std::abs(c) >= Lit<real>::EPS) alphaT = alpha * freq / (RLH(8.6858896) * c);
@@ -91,6 +98,19 @@ template<typename real> struct Pre {
             real fg = franc_garr(freq / FLH(1000.0));
             fg /= FLH(8685.8896);
             alphaT += fg;
+        } else if(au[1] == 'B') {
+            for(int ib = 0; ib < env->NBioLayers; ++ib) {
+                const double *L = env->bio + 5 * ib;
+                real z1 = (real)L[0], z2 = (real)L[1], f0 = (real)L[2], q = (real)L[3], a0 = (real)L[4];
+                if(z >= z1 && z <= z2) {
+                    // the reference's 1.0 literals are double: the sum is
+                    // formed in double precision in both builds (dB/km)
+                    double u = 1.0 - (double)((f0 * f0) / (freq * freq));
+                    real a = (real)((double)a0 / (u * u + 1.0 / (double)(q * q)));
+                    a /= FLH(8685.8896);
+                    alphaT += a;
+                }
+            }
         }
         alphaT *= (c * c) / omega;
         return cx<real>(c, alphaT);
@@ -306,8 +326,8 @@ int bdry(const Pre<real> &pre, const bhc_bdry_in &in, const bhc_halfspace_in &hs
             if(type[1] == 'L') {
                 if(!in.alphaR || !in.alphaI || !in.betaR || !in.betaI || !in.rho)
                     return hfail("long-format boundary needs per-point geoacoustics");
-                bd[i].cP = pre.crci((real)in.alphaR[i - 1], (real)in.alphaI[i - 1], "W ");
-                bd[i].cS = pre.crci((real)in.betaR[i - 1], (real)in.betaI[i - 1], "W ");
+                bd[i].cP = pre.crci(RLH(1.0e20), (real)in.alphaR[i - 1], (real)in.alphaI[i - 1], "W ");
+                bd[i].cS = pre.crci(RLH(1.0e20), (real)in.betaR[i - 1], (real)in.betaI[i - 1], "W ");
                 bd[i].rho = (real)in.rho[i - 1];
             }
             if(i > 1 && !(bd[i].x[0] > bd[i - 1].x[0])) return hfail("boundary ranges must increase");
@@ -393,8 +413,8 @@ template<typename real> int halfspace(const Pre<real> &pre, const bhc_halfspace_
     cx<real> cP(0), cS(0);
     const char *au = pre.env->atten_unit;
     if(in.bc == 'A') {
-        cP = pre.crci((real)in.alphaR, (real)in.alphaI, au);
-        cS = pre.crci((real)in.betaR, (real)in.betaI, au);
+        cP = pre.crci((real)in.zTemp, (real)in.alphaR, (real)in.alphaI, au);
+        cS = pre.crci((real)in.zTemp, (real)in.betaR, (real)in.betaI, au);
     } else if(in.bc == 'G') {
         real Mz = (real)in.Mz, vr, a2f;
         if(Mz >= FLH(-1.0) && Mz < FLH(1.0)) {
@@ -415,7 +435,7 @@ template<typename real> int halfspace(const Pre<real> &pre, const bhc_halfspace_
         else a2f = FLH(0.0601);
         real aR = vr * FLH(1500.0);
         real aI = a2f * (vr / FLH(1000.0)) * FLH(1500.0) * std::log(FLH(10.0)) / (FLH(40.0) * Lit<real>::PI);
-        cP = pre.crci(aR, aI, "L ");
+        cP = pre.crci((real)in.zTemp, aR, aI, "L ");
         cS = FLH(0.0);
     } else if(in.bc != 'V' && in.bc != 'R' && in.bc != 'F') {
         return hfail("boundary condition '%c' not supported", in.bc);
@@ -451,7 +471,7 @@ template<typename real> int preprocess(const bhc_env2d *env, bhc_problem_owner *
     for(int i = 1; i < N; ++i)
         if(!(z[i] > z[i - 1])) return hfail("SSP depths must increase");
     for(int iz = 0; iz < N; ++iz) {
-        c[iz] = pre.crci((real)env->ssp_alphaR[iz], (real)env->ssp_alphaI[iz], env->atten_unit);
+        c[iz] = pre.crci(z[iz], (real)env->ssp_alphaR[iz], (real)env->ssp_alphaI[iz], env->atten_unit);
         if(iz > 0) cz[iz - 1] = (c[iz] - c[iz - 1]) / (z[iz] - z[iz - 1]);
     }
     if(N > 0) cz[N - 1] = cx<real>(NAN, NAN);

@@ -307,6 +307,8 @@ extern "C" int bhc_preprocess3d(const bhc_env3d *e, bhc_problem3d_owner **out)
     e2.box_z = e->box_z;
     e2.NSBPPts = e->NSBPPts;
     e2.sbp = e->sbp;
+    e2.NBioLayers = e->NBioLayers;
+    e2.bio = e->bio;
     bhc_problem3d_owner *o = new bhc_problem3d_owner();
     if(bhc_preprocess2d(&e2, &o->p2)) {
         fail3("%s", bhc_host_last_error());

@@ -76,6 +76,7 @@ inline CR exp(CR a) { ++opc::n; return CR(std::exp(a.v)); }
 inline CR sin(CR a) { ++opc::n; return CR(std::sin(a.v)); }
 inline CR cos(CR a) { ++opc::n; return CR(std::cos(a.v)); }
 inline CR atan2(CR a, CR b) { ++opc::n; return CR(std::atan2(a.v, b.v)); }
+inline CR atan(CR a) { ++opc::n; return CR(std::atan(a.v)); }
 inline CR hypot(CR a, CR b) { ++opc::n; return CR(std::hypot(a.v, b.v)); }
 inline CR fabs(CR a) { return CR(std::fabs(a.v)); }
 inline CR nextafter(CR a, CR b) { return CR(std::nextafter(a.v, b.v)); }

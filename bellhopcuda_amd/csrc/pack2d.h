@@ -86,6 +86,7 @@ template<typename real> int pack(const bhc_problem2d *p, std::vector<real> &blob
     P.run3 = p->RunType[3];
     P.run4 = p->RunType[4];
     P.type2 = p->BeamType[2];
+    P.type3 = p->BeamType[3];
     P.ati_curv = p->ati_type[0];
     P.bty_curv = p->bty_type[0];
     P.ati_long = p->ati_type[1];

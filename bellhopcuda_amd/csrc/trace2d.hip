@@ -951,6 +951,45 @@ extern "C" int bhc_ctx_synchronize(bhc_ctx *ctx)
     return 0;
 }
 
+namespace {
+// Plain streaming copy (16 B per lane per iteration): the known-size kernel
+// against which profiler byte counters (FETCH_SIZE, WRITE_SIZE) are scaled.
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for(size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+} // namespace
+
+extern "C" int bhc_copy_calibrate(bhc_ctx *ctx, long long bytes, double *ms)
+{
+    if(!ctx || bytes < 16 || (bytes & 15)) return fail("bhc_copy_calibrate: bytes must be a positive multiple of 16");
+    HIPCHECK(hipSetDevice(ctx->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHECK(hipMalloc(&a, (size_t)bytes));
+    if(hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail("bhc_copy_calibrate: allocation failed");
+    }
+    int rc = 0;
+    const size_t n = (size_t)bytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cus * 16);
+    if(hipMemsetAsync(a, 1, (size_t)bytes, ctx->stream) != hipSuccess) rc = fail("bhc_copy_calibrate: memset failed");
+    if(!rc) {
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, ctx->stream, (const uint4 *)a, (uint4 *)b, n);
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        float f = 0.f;
+        if(hipGetLastError() != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess
+           || hipEventElapsedTime(&f, ctx->ev0, ctx->ev1) != hipSuccess)
+            rc = fail("bhc_copy_calibrate: copy kernel failed");
+        if(ms) *ms = f;
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 extern "C" int bhc_last_phase_ms(bhc_ctx *ctx, double *trace_ms, double *scatter_ms, int *passes)
 {
     HIPCHECK(hipEventSynchronize(ctx->ev1));
@@ -1228,6 +1267,11 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
         // one wavefront per workgroup spreads the rays over all CUs (the trace
         // kernel is register-bound at one wavefront per SIMD)
         int tblocks = std::max(1, std::min(waves, c->num_cus * 4 * c->bpc));
+        // error / warning words as the completed passes left them (stream
+        // order: after the previous pass's scatter), restored if this trace
+        // pass overflows and its rays are traced again
+        uint32_t ew_start[4] = {0u, 0u, 0u, 0u};
+        HIPCHECK(hipMemcpyAsync(ew_start, dp->d_errw, sizeof(ew_start), hipMemcpyDeviceToHost, s));
         if(first) HIPCHECK(hipEventRecord(c->ev0, s));
         HIPCHECK(hipEventRecord(c->pa, s));
         switch(P.ssp_type) {
@@ -1251,10 +1295,12 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
             int mid = r.first + nj / 2;
             todo.push_back({mid, r.second});
             todo.push_back({r.first, mid});
-            // the trace pass wrote only per-ray statistics and the step counter:
-            // the retried halves rewrite the statistics; the counter goes back
-            // to the completed passes' total
+            // the trace pass wrote only per-ray statistics, the step counter and
+            // the error / warning words: the retried halves rewrite the
+            // statistics; the counter and the words go back to the completed
+            // passes' values, so that the retried rays' warnings count once
             HIPCHECK(hipMemcpyAsync(dp->d_counters, &steps_done, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(dp->d_errw, ew_start, sizeof(ew_start), hipMemcpyHostToDevice, s));
             HIPCHECK(hipStreamSynchronize(s));
             continue;
         }
@@ -1948,7 +1994,7 @@ struct bhc_sweep {
     // saved-ray buffers; rebuilt when the launch configuration changes
     bool use_graph;
     hipGraphExec_t graph[2];
-    int graph_key[4];
+    int graph_key[5];
     bool tuned;
 };
 
@@ -2050,17 +2096,19 @@ int sweep_upload(bhc_sweep *sw, const bhc_problem2d *const *probs)
     return 0;
 }
 
-template<typename real> BatchDev<real> batch_dev(bhc_sweep *sw, int nprob)
+// Device view of problems [p0, p1): problem indices inside the launch (work
+// counters, saved rays, segment descriptors) are relative to p0.
+template<typename real> BatchDev<real> batch_dev(bhc_sweep *sw, int p0, int p1)
 {
     BatchDev<real> Bd;
     memset(&Bd, 0, sizeof(Bd));
-    Bd.probs = (const Prob2D<real> *)sw->d_probs;
-    Bd.nprob = nprob;
+    Bd.probs = (const Prob2D<real> *)sw->d_probs + p0;
+    Bd.nprob = p1 - p0;
     Bd.lds_cap = sw->lds_bytes / (int)sizeof(real);
-    Bd.ray_next = sw->d_ray_next;
+    Bd.ray_next = sw->d_ray_next + p0;
     Bd.pcursor = sw->d_pcursor;
-    Bd.perr = sw->d_perr;
-    Bd.psteps = sw->d_psteps;
+    Bd.perr = sw->d_perr + 4 * (size_t)p0;
+    Bd.psteps = sw->d_psteps + p0;
     Bd.resume_next = sw->d_resume;
     Bd.more = sw->d_more;
     return Bd;
@@ -2113,35 +2161,38 @@ template<typename real> void enqueue_pass(bhc_sweep *sw, BatchDev<real> Bd, cons
     }
 }
 
-// Trace + scatter the first `nprob` problems: zero their fields and counters,
-// then passes until no ray is left. Graph mode replays one captured graph per
-// pass; eager mode times the trace and scatter kernels of every pass.
-template<typename real> int sweep_core(bhc_sweep *sw, int nprob, bool graph)
+// Trace + scatter problems [p0, p1) in one persistent launch (a sequence of
+// passes): zero their fields and counters, then passes until no ray is left.
+// Graph mode replays one captured graph per pass; eager mode times the trace
+// and scatter kernels of every pass.
+template<typename real> int sweep_core(bhc_sweep *sw, int p0, int p1, bool graph)
 {
     bhc_ctx *c = sw->ctx;
     hipStream_t s = c->stream;
     if(ensure_segs(c)) return -1;
-    BatchDev<real> Bd = batch_dev<real>(sw, nprob);
+    BatchDev<real> Bd = batch_dev<real>(sw, p0, p1);
     DevOut O = batch_out<real>(sw);
     // every wavefront may add 2 segments per lane per step between two polls
     // (32 steps apart) plus one partly used slot block
     size_t margin = (size_t)sw->max_waves * (2 * 32 * WAVE + 2 * SEG_BLOCK);
     if(margin * 2 >= O.seg_cap) return fail("segment buffer too small for a batched sweep (raise BHC_SEG_BUFFER_MB)");
     Bd.suspend_at = (uint32_t)(O.seg_cap - margin);
-    size_t nf = nprob == sw->nprob ? sw->field_floats : sw->field_off[nprob];
-    HIPCHECK(hipMemsetAsync(sw->d_fields, 0, nf * sizeof(float), s));
-    HIPCHECK(hipMemsetAsync(sw->d_ray_next, 0, sw->nprob * sizeof(int32_t), s));
+    const int np = p1 - p0;
+    const size_t f0 = sw->field_off[p0], f1 = p1 == sw->nprob ? sw->field_floats : sw->field_off[p1];
+    HIPCHECK(hipMemsetAsync(sw->d_fields + f0, 0, (f1 - f0) * sizeof(float), s));
+    HIPCHECK(hipMemsetAsync(sw->d_ray_next + p0, 0, np * sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(sw->d_pcursor, 0, sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(sw->d_nsave, 0, 2 * sizeof(int32_t), s));
-    HIPCHECK(hipMemsetAsync(sw->d_perr, 0, 4 * sw->nprob * sizeof(uint32_t), s));
-    HIPCHECK(hipMemsetAsync(sw->d_psteps, 0, sw->nprob * sizeof(unsigned long long), s));
+    HIPCHECK(hipMemsetAsync(sw->d_perr + 4 * (size_t)p0, 0, 4 * np * sizeof(uint32_t), s));
+    HIPCHECK(hipMemsetAsync(sw->d_psteps + p0, 0, np * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(sw->d_counters, 0, 2 * sizeof(unsigned long long), s));
     if(sw->with_stats) {
-        HIPCHECK(hipMemsetAsync(sw->d_stats, 0, 3 * sw->total_rays * sizeof(int32_t), s));
-        HIPCHECK(hipMemsetAsync(sw->d_xend, 0, 2 * sw->total_rays * sizeof(double), s));
+        const size_t r0 = sw->stats_off[p0], r1 = p1 == sw->nprob ? sw->total_rays : sw->stats_off[p1];
+        HIPCHECK(hipMemsetAsync(sw->d_stats + 3 * r0, 0, 3 * (r1 - r0) * sizeof(int32_t), s));
+        HIPCHECK(hipMemsetAsync(sw->d_xend + 2 * r0, 0, 2 * (r1 - r0) * sizeof(double), s));
     }
     if(graph) {
-        int key[4] = {sw->scatter_blocks, sw->lds_bytes, nprob, sw->scatter_lanes};
+        int key[5] = {sw->scatter_blocks, sw->lds_bytes, p0, p1, sw->scatter_lanes};
         if(!sw->graph[0] || memcmp(key, sw->graph_key, sizeof(key)) != 0) {
             for(int k = 0; k < 2; ++k) {
                 if(sw->graph[k]) (void)hipGraphExecDestroy(sw->graph[k]);
@@ -2237,7 +2288,7 @@ template<typename real> int sweep_tune(bhc_sweep *sw)
         sw->scatter_blocks = blocks;
         sw->lds_bytes = lds;
         for(int rep = 0; rep < 2; ++rep) {
-            if(sweep_core<real>(sw, nsub, false)) return -1;
+            if(sweep_core<real>(sw, 0, nsub, false)) return -1;
             if(rep == 1) t = sw->trace_ms + sw->scatter_ms;
         }
         return 0;
@@ -2304,8 +2355,18 @@ extern "C" int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw)
 {
     if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_launch: bad arguments");
     HIPCHECK(hipSetDevice(ctx->device));
-    return sw->precision == 64 ? sweep_core<double>(sw, sw->nprob, sw->use_graph)
-                               : sweep_core<float>(sw, sw->nprob, sw->use_graph);
+    return sw->precision == 64 ? sweep_core<double>(sw, 0, sw->nprob, sw->use_graph)
+                               : sweep_core<float>(sw, 0, sw->nprob, sw->use_graph);
+}
+
+extern "C" int bhc_sweep_launch_range(bhc_ctx *ctx, bhc_sweep *sw, int p0, int p1)
+{
+    if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_launch_range: bad arguments");
+    if(p0 < 0 || p1 > sw->nprob || p0 >= p1)
+        return fail("bhc_sweep_launch_range: range [%d, %d) outside the sweep's %d problems", p0, p1, sw->nprob);
+    HIPCHECK(hipSetDevice(ctx->device));
+    return sw->precision == 64 ? sweep_core<double>(sw, p0, p1, sw->use_graph)
+                               : sweep_core<float>(sw, p0, p1, sw->use_graph);
 }
 
 extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *lds_bytes,

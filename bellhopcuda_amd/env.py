@@ -112,6 +112,7 @@ class Env2D:
     rLoop: float = 1.0
     iBeamWindow: int = 4
     sbp: Optional[np.ndarray] = None          # [n, 2] deg / dB
+    bio: Optional[np.ndarray] = None          # [n, 5] z1, z2 (m), f0 (Hz), Q, a0 (dB/km); atten_unit[1] == 'B'
 
     # ------------------------------------------------------------------
     @property
@@ -189,6 +190,9 @@ class Env2D:
         if self.sbp is not None:
             e.NSBPPts = len(self.sbp)
             e.sbp = dp(np.asarray(self.sbp, np.float64).reshape(-1))
+        if self.bio is not None:
+            e.NBioLayers = len(self.bio)
+            e.bio = dp(np.asarray(self.bio, np.float64).reshape(-1))
         return e, keep
 
 

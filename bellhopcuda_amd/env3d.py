@@ -70,6 +70,7 @@ class Env3D:
     box_y_km: float = 101.0
     box_z: float = 5500.0
     sbp: Optional[np.ndarray] = None
+    bio: Optional[np.ndarray] = None          # [n, 5] z1, z2 (m), f0 (Hz), Q, a0 (dB/km); atten_unit[1] == 'B'
 
     @property
     def nrays(self):
@@ -142,6 +143,9 @@ class Env3D:
         if self.sbp is not None:
             e.NSBPPts = len(self.sbp)
             e.sbp = dp(np.asarray(self.sbp, np.float64).reshape(-1))
+        if self.bio is not None:
+            e.NBioLayers = len(self.bio)
+            e.bio = dp(np.asarray(self.bio, np.float64).reshape(-1))
         return e, keep
 
 

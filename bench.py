@@ -5,10 +5,13 @@ Workloads (one "step" = one pass of the hot path over one batch of input):
 * sweep (default): the heterogeneous batched sweep -- 4096 independent 2D TL
   problems drawn from one seed (10-200 SSP knots, c-linear or PCHIP; flat or
   1-500-segment bathymetry, depth 100-5000 m; 25-1000 Hz; 500-20000 hat
-  beams; 51-501 x 201-5001 receivers), FP64. A step traces every problem in
-  one persistent launch (bhc_sweep_launch). The plain per-environment GPU
-  launch (one resident problem per launch) is timed on a sample of the same
-  problems and reported beside it.
+  beams; 51-501 x 201-5001 receivers), FP64. All problems are resident in
+  HBM; a step traces a slice of 512 of them in one persistent launch
+  (bhc_sweep_launch_range), and steps cycle through the slices (8 steps = the
+  whole sweep), so that any --steps / --warmup fits a few minutes. The plain
+  per-environment GPU launch (one resident problem per launch) is timed on a
+  sample of the same problems and reported beside it. Progress lines go to
+  stderr.
 * seamount: one range-dependent TL field -- 40-knot spline SSP, 500-segment
   bathymetry, 20001 ray-centred Gaussian beams, 501 x 5001 receivers, FP64.
 * munk: one Munk-profile TL field, 2001 hat beams, 201 x 1001 receivers.
@@ -102,7 +105,7 @@ def kernel_roofline(precision, kernels, F_path, path_ms):
                         achieved=t, frac=t / peak))
     dom = max(out, key=lambda k: k["ms"])
     path = F_path / (path_ms * 1e-3) / 1e12
-    return {"bound": "mfma", "pipe": "FP%d vector ALU (the tracer has no matrix work)" % precision,
+    return {"bound": "valu", "pipe": "FP%d vector ALU (the tracer has no matrix work)" % precision,
             "kernel": dom["name"], "achieved": dom["achieved"], "peak": peak, "unit": "TFLOP/s",
             "frac": dom["frac"], "traffic": None, "launches": dom["launches"],
             "ms_per_launch": dom["ms_per_launch"], "flop_per_launch": dom["flop_per_launch"],
@@ -248,6 +251,42 @@ def run_c3(a):
     return line
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result line is the only stdout)."""
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
+
+
+def cpu_fields_per_s(envs, target_s, cores):
+    """CPU baseline of a sweep: the oracle (plain-C restatement, thread pool of
+    `cores` host threads pulling rays from one atomic counter) on each sampled
+    problem's rays in chunks spread over its fan. Each problem's full time is
+    extrapolated from its own sample (time x rays / sampled rays), and
+    fields/s = problems / sum of their estimated times, so that problems of
+    different sizes are weighted by their own cost."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    per_env = target_s / len(envs)
+    est, done, spent = 0.0, 0, 0.0
+    for env in envs:
+        n = env.nrays
+        k, t, per = 0, 0.0, 8
+        while t < per_env and k < n:
+            p = min(per, n - k)
+            nch = max(1, min(8, p))
+            q = max(1, p // nch)
+            t0 = time.time()
+            for c in range(nch):
+                b = max(0, min(n - q, int((c + 0.5) * n / nch) - q // 2))
+                oracle.field_run(env, nthreads=cores, job_begin=b, job_end=b + q, stats=False)
+            t += time.time() - t0
+            k += nch * q
+            per *= 2
+        est += t * n / k
+        done += k
+        spent += t
+    return len(envs) / est, done, spent
+
+
 def run_sweep(a):
     import bellhopcuda_amd as bh
     from bellhopcuda_amd import _abi
@@ -256,36 +295,56 @@ def run_sweep(a):
     t0 = time.perf_counter()
     probs = [bh.Problem(e) for e in envs]
     t_pre = time.perf_counter() - t0
+    log("preprocessed %d problems in %.1f s" % (len(probs), t_pre))
     ctx = bh.Context(0)
+    if a.copy_calibration_mb > 0:
+        cms = C.c_double()
+        if L.bhc_copy_calibrate(ctx._h, a.copy_calibration_mb << 20, C.byref(cms)) != 0:
+            raise SystemExit("copy calibration failed: " + L.bhc_last_error().decode())
+        log("copy calibration: %d MiB in %.3f ms" % (a.copy_calibration_mb, cms.value))
     nwarm = min(len(probs), 64)
     warm = bh.Sweep(probs[:nwarm], ctx)
     warm.launch()
     warm.release()
     sw = bh.Sweep(probs, ctx)
+    log("uploaded; autotuning on the first 64 problems")
     tuned = sw.autotune()
+    log("autotuned: %s" % json.dumps(tuned))
     # kernel-by-kernel passes, so that HIP events time the trace and the scatter
     # kernels of every pass separately (the graph-replayed passes are the same
     # kernels; each pass ends in a host synchronisation either way)
     sw.set_graph(False)
-    R = sum(p.nrays for p in probs)
-    for _ in range(max(0, a.warmup - 1)):
-        sw.launch()
+    nchunk = max(1, min(a.sweep_chunk, len(probs)))
+    chunks = [(i, min(i + nchunk, len(probs))) for i in range(0, len(probs), nchunk)]
+    rays = [p.nrays for p in probs]
+    for w in range(a.warmup):
+        c0, c1 = chunks[w % len(chunks)]
+        sw.launch(c0, c1)
     L.bhc_ctx_synchronize(ctx._h)
-    kern, tr_ms, sc_ms = [], [], []
+    log("warm-up done (%d launches of %d problems)" % (a.warmup, nchunk))
+    tr_ms, sc_ms, npasses, nprob_done, R, S, Cn = [], [], 0, 0, 0, 0, 0
+    errs = []
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        sw.launch()
+    for k in range(a.steps):
+        c0, c1 = chunks[k % len(chunks)]
+        sw.launch(c0, c1)
         st = sw.stats()
         tr_ms.append(st["trace_ms"])
         sc_ms.append(st["scatter_ms"])
-        kern.append(ctx.last_kernel_ms())
+        npasses += st["passes"]
+        S += st["steps"]
+        Cn += st["contrib"]
+        R += sum(rays[c0:c1])
+        nprob_done += c1 - c0
+        if k < len(chunks):
+            errs += [sw.download(i) for i in range(c0, c1, max(1, (c1 - c0) // 8))]
+        log("step %d: problems [%d, %d) in %.0f ms (%d passes)" % (k, c0, c1, st["trace_ms"] + st["scatter_ms"],
+                                                                   st["passes"]))
     L.bhc_ctx_synchronize(ctx._h)
     wall = time.perf_counter() - t0
-    st = sw.stats()
-    errs = [sw.download(i) for i in range(0, len(probs), max(1, len(probs) // 64))]
     nerr = sum(1 for r in errs if r.err)
     sw.release()
-    fields_s = a.steps * len(probs) / wall
+    fields_s = nprob_done / wall
 
     # plain per-environment GPU launches on a sample of the same problems
     step = max(1, len(probs) // a.plain_sample)
@@ -302,32 +361,34 @@ def run_sweep(a):
         t_plain += time.perf_counter() - t1
         r.release()
     plain_fields_s = len(sample) / t_plain
+    log("plain per-environment launches: %.2f fields/s on %d problems" % (plain_fields_s, len(sample)))
 
     f_I, f_S, f_C = opcounts(L, _abi, probs[::max(1, len(probs) // 16)][:16], 16)
-    S, Cn = st["steps"], st["contrib"]
     F = f_I * R + f_S * S + f_C * Cn
     k_ms = (sum(tr_ms) + sum(sc_ms)) / len(tr_ms)
     real_t = "double" if a.precision == 64 else "float"
-    nl = a.steps * st["passes"]
     rl = kernel_roofline(a.precision, [
-        {"name": "k_trace_batch<%s>" % real_t, "ms": sum(tr_ms), "launches": nl,
-         "flop": (f_I * R + f_S * S) * a.steps},
-        {"name": ("k_scatter_lanes<%s, true>" if tuned["scatter_lanes"] else "k_scatter_batch<%s, true>") % real_t,
-         "ms": sum(sc_ms), "launches": nl, "flop": f_C * Cn * a.steps},
-    ], F, k_ms)
-    rl.update({"passes": st["passes"], "f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S,
+        {"name": "k_trace_batch<%s>" % real_t, "ms": sum(tr_ms), "launches": npasses, "flop": f_I * R + f_S * S},
+        {"name": {0: "k_scatter_batch<%s, true>", 1: "k_scatter_lanes<%s, true>", 2: "k_scatter_flat<%s, true>"}[
+            tuned["scatter_lanes"]] % real_t, "ms": sum(sc_ms), "launches": npasses, "flop": f_C * Cn},
+    ], F, k_ms * a.steps)
+    rl.update({"passes": npasses, "f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S,
                "contributions": Cn})
     line = {
         "metric": "TL fields/s + rays/s on 1 MI355X, batched/surrogate vs plain GPU",
-        "value": fields_s, "unit": "TL fields/s", "rays_per_s": fields_s * R / len(probs),
+        "value": fields_s, "unit": "TL fields/s", "rays_per_s": R / wall,
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.precision == 64 else "f32", "data": "synthetic (seeded sweep, seed 20260004)",
         "config": {"workload": "batched sweep of %d independent 2D TL problems (10-200 knots c-linear/PCHIP, "
                                "flat or 1-500-segment bathymetry, 25-1000 Hz, 500-20000 hat beams, "
-                               "51-501 x 201-5001 receivers)%s" % (
-                                   len(probs), "" if a.sweep_scale == 1.0 else ", beams/grids scaled %.3g" % a.sweep_scale),
-                   "problems": len(probs), "rays": R, "parallelism": "single GPU"},
+                               "51-501 x 201-5001 receivers)%s, %s; all problems resident in HBM, one step = one "
+                               "persistent launch over %d of them (steps cycle through the sweep, %d steps = the "
+                               "whole sweep)" % (
+                                   len(probs), "" if a.sweep_scale == 1.0 else ", beams/grids scaled %.3g" % a.sweep_scale,
+                                   "f64" if a.precision == 64 else "f32", nchunk, len(chunks)),
+                   "problems": len(probs), "problems_per_step": nchunk, "problems_timed": nprob_done,
+                   "rays_timed": R, "parallelism": "single GPU"},
         "plain_per_env": {"value": plain_fields_s, "unit": "TL fields/s",
                           "sample": "%d of the %d problems, one resident launch each with its autotuned scatter "
                                     "mapping (tiles / wavefront / lane: %d / %d / %d)" % (
@@ -335,19 +396,18 @@ def run_sweep(a):
         "batched_over_plain": fields_s / plain_fields_s,
         "autotuned": tuned,
         "roofline": rl,
-        "errors": {"problems_with_errors_in_sample": nerr},
+        "errors": {"problems_with_errors_in_sample": nerr, "sampled": len(errs)},
         "host_preprocess_s": t_pre,
     }
     if not a.no_cpu:
         cores = os.cpu_count() or 1
-        cs = envs[::max(1, len(envs) // 8)][:8]
-        rs, done, t = cpu_rays_per_s(cs, a.cpu_seconds, cores)
-        mean_rays = R / len(probs)
-        line["cpu_baseline"] = {"value": rs / mean_rays, "unit": "TL fields/s", "rays_per_s": rs, "cores": cores,
-                                "kind": "port",
-                                "sample": "%d rays from 8 of the sweep's problems (chunks spread over each fan, "
-                                          "full receiver grids), %.1f s; fields/s = rays/s / mean rays per problem"
-                                          % (done, t)}
+        cs = envs[::max(1, len(envs) // 16)][:16]
+        cf, done, t = cpu_fields_per_s(cs, a.cpu_seconds, cores)
+        line["cpu_baseline"] = {"value": cf, "unit": "TL fields/s", "cores": cores, "kind": "port",
+                                "sample": "%d rays of 16 of the sweep's problems (chunks spread over each fan, full "
+                                          "receiver grids), %.1f s; each problem's time extrapolated from its own "
+                                          "rays, fields/s = problems / summed estimated times" % (done, t)}
+        log("cpu baseline: %.4f fields/s" % cf)
     return line
 
 
@@ -453,7 +513,7 @@ def run_surrogate(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep", choices=["sweep", "seamount", "munk", "surrogate", "c3"])
     ap.add_argument("--sur-train", type=int, default=20000)
@@ -465,24 +525,19 @@ def main():
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--sweep-n", type=int, default=4096)
     ap.add_argument("--sweep-scale", type=float, default=1.0)
+    ap.add_argument("--sweep-chunk", type=int, default=512,
+                    help="problems per step (one persistent launch each); steps cycle through the sweep")
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="first run one device-to-device copy of this many MiB (the reference kernel for "
-                         "scaling profiler byte counters)")
+                    help="first run one launch of the library's plain copy kernel over this many MiB (the "
+                         "known-size kernel for scaling profiler byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the tracer has no multi-GPU path (problems and ray ranges would be "
               "sharded per GPU with no exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    if a.copy_calibration_mb > 0:
-        import torch
-        src = torch.ones(a.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     if a.workload == "sweep":
         line = run_sweep(a)
     elif a.workload == "surrogate":

@@ -178,6 +178,10 @@ typedef struct bhc_env2d {
     /* source beam pattern (SBPInfo), angle (deg) / level (dB) pairs; 0 = omni */
     int32_t NSBPPts, pad4;
     const double *sbp;      /* [2*NSBPPts] */
+    /* biological volume attenuation (atten_unit[1] == 'B', AttenInfo::bio):
+     * NBioLayers rows of z1, z2 (m), f0 (Hz), Q, a0 (dB/km) */
+    int32_t NBioLayers, pad5;
+    const double *bio;      /* [5*NBioLayers] */
 } bhc_env2d;
 
 /* ---- preprocessed 2D problem: what the launch layer consumes ----------------- */
@@ -325,6 +329,11 @@ int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode_out);
 /* Force a scatter mapping (0 = automatic choice, 1 tiles, 2 wavefront, 3 lane). */
 int bhc_problem_set_scatter(bhc_dev_problem *dp, int mode);
 int bhc_ctx_synchronize(bhc_ctx *ctx);
+/* Measurement aid with no counterpart in the reference: one launch of a plain
+ * 16-byte-per-lane copy kernel (k_copy16) over `bytes` (a multiple of 16) on
+ * the context's stream, *ms its duration. Profiler byte counters read for it
+ * give the factor that turns their units into bytes. */
+int bhc_copy_calibrate(bhc_ctx *ctx, long long bytes, double *ms);
 /* Duration (ms) of the tracer kernel of the last launch on this context,
  * from HIP events recorded on the context's stream around it. */
 int bhc_last_kernel_ms(bhc_ctx *ctx, double *ms);
@@ -348,6 +357,10 @@ int bhc_sweep_upload2d(bhc_ctx *ctx, const bhc_problem2d *const *probs, int npro
                        bhc_sweep **out);
 /* Zero every field and trace + scatter all problems (blocks until done). */
 int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw);
+/* As bhc_sweep_launch for problems [p0, p1) only: their fields and counters
+ * are zeroed and traced in one persistent launch; the other problems' fields
+ * are left as they are. Error if the range is empty or outside the sweep. */
+int bhc_sweep_launch_range(bhc_ctx *ctx, bhc_sweep *sw, int p0, int p1);
 /* Field, errors / warnings, step count and (with_stats) per-ray statistics of
  * problem iprob; out->field holds NSz * NRz_per_range * NRr complex floats. */
 int bhc_sweep_download(bhc_ctx *ctx, bhc_sweep *sw, int iprob, bhc_field_out *out);
@@ -423,6 +436,8 @@ typedef struct bhc_env3d {
     double box_x_km, box_y_km, box_z;
     int32_t NSBPPts, pad5;
     const double *sbp;
+    int32_t NBioLayers, pad6;   /* as in bhc_env2d */
+    const double *bio;
 } bhc_env3d;
 
 /* Boundary grid node: position and the unit normals of the two triangles of

@@ -48,6 +48,7 @@ typedef float complex cpx;
 #define RSQRT sqrtf
 #define RABS fabsf
 #define RCOS cosf
+#define RATAN atanf
 #define RSIN sinf
 #define RATAN2 atan2f
 #define REXP expf
@@ -56,6 +57,7 @@ typedef float complex cpx;
 #define RPOW powf
 #define RFMA fmaf
 #define CSQRT csqrtf
+#define CCONJ conjf
 #define CEXP cexpf
 #define CABS cabsf
 #else
@@ -70,6 +72,7 @@ typedef double complex cpx;
 #define RSQRT sqrt
 #define RABS fabs
 #define RCOS cos
+#define RATAN atan
 #define RSIN sin
 #define RATAN2 atan2
 #define REXP exp
@@ -78,6 +81,7 @@ typedef double complex cpx;
 #define RPOW pow
 #define RFMA fma
 #define CSQRT csqrt
+#define CCONJ conj
 #define CEXP cexp
 #define CABS cabs
 #endif
@@ -178,16 +182,21 @@ static real franc_garr(real f, const bhc_env2d *env)
         + a3 * p3 * SQ(f);
 }
 
-/* crci, src/module/atten.cpp:99-194 (bio layers not supported by this oracle) */
+/* crci, src/module/atten.cpp:99-194 */
 static cpx crci(const bhc_env2d *env, real freq0, real z, real c, real alpha, const char *AttenUnit)
 {
-    (void)z;
     real freq = freq0;
     real omega = FL(2.0) * REAL_PI * freq;
     real alphaT = FL(0.0);
     switch(AttenUnit[0]) {
     case 'N': alphaT = alpha; break;
     case 'M': alphaT = alpha / RL(8.6858896); break;
+    case 'm':
+        /* power law in f / freq0; the run frequency is freq0, where both of the
+         * reference's branches (below and above fT) give the factor 1 */
+        alphaT = alpha / RL(8.6858896);
+        alphaT *= RPOW(freq / freq0, FL(1.0));
+        break;
     case 'F': alphaT = alpha * freq / RL(8685.8896); break;
     case 'W':
         if(RABS(c) >= REAL_EPSILON) alphaT = alpha * freq / (RL(8.6858896) * c);
@@ -212,6 +221,17 @@ static cpx crci(const bhc_env2d *env, real freq0, real z, real c, real alpha, co
         fg /= FL(8685.8896);
         alphaT += fg;
     } break;
+    case 'B': /* biological layers (Diachok); 1.0 is a double literal in the reference */
+        for(int ib = 0; ib < env->NBioLayers; ++ib) {
+            real z1 = (real)env->bio[5 * ib], z2 = (real)env->bio[5 * ib + 1], f0 = (real)env->bio[5 * ib + 2];
+            real q = (real)env->bio[5 * ib + 3], a0 = (real)env->bio[5 * ib + 4];
+            if(z >= z1 && z <= z2) {
+                real a = a0 / (SQ(1.0 - SQ(f0) / SQ(freq)) + 1.0 / SQ(q)); /* dB/km */
+                a /= FL(8685.8896);
+                alphaT += a;
+            }
+        }
+        break;
     }
     alphaT *= SQ(c) / omega;
     return MKC(c, alphaT);
@@ -1096,6 +1116,33 @@ static void Reflect(const RayPt *oldp, RayPt *newp, const HS *hs, int isTop, v2 
         } else {
             newp->Amp = aR * oldp->Amp;
             newp->Phase = oldp->Phase + RATAN2(cimag(Refl), creal(Refl));
+            if(P->Type[3] == 'S') {
+                /* beam displacement and width change, reflect.hpp:383-442 (2D) */
+                cpx ch = oldp->c / CCONJ(hs->cP);
+                real co = oldp->t.x * oldp->c;
+                real si = oldp->t.y * oldp->c;
+                real ck = omega / oldp->c;
+                cpx a = FL(2.0) * hs->rho * (FL(1.0) - SQ(ch));
+                cpx b = SQ(co) - SQ(ch);
+                cpx d = SQ(hs->rho) * SQ(si) + b;
+                cpx sb = CSQRT(b);
+                real cco = SQ(co);
+                real ssi = SQ(si);
+                cpx delta;
+                if(RABS(si) >= REAL_EPSILON) delta = a * co / si / (ck * sb * d);
+                else delta = FL(0.0);
+                real pdelta = creal(delta) / (oldp->c / co);
+                cpx cksbd = ck * sb * d;
+                cpx ddelta = a * (cco / (cksbd * b) - (RL(1.0) + (cco / ssi)) / cksbd
+                                  - FL(2.0) * SQ(co) * (SQ(hs->rho) - RL(1.0)) / (cksbd * d));
+                real rddelta = -creal(ddelta);
+                real sddelta = rddelta / RABS(rddelta);
+                real theta_bot = RATAN(tBdry.y / tBdry.x);
+                newp->x.x = newp->x.x + creal(delta) * RCOS(theta_bot);
+                newp->x.y = newp->x.y + creal(delta) * RSIN(theta_bot);
+                newp->tau = newp->tau + pdelta;
+                newp->q = vadd(newp->q, vscl(sddelta * rddelta * si * creal(o.ccpx), oldp->p));
+            }
         }
     } else {
         RunError(err, BHC_ERR_BOUNDARY_CONDITION_TYPE);

@@ -188,6 +188,8 @@ static int prob3_from_env(const bhc_env3d *e, Prob3 *Q)
     e2.box_z = e->box_z;
     e2.NSBPPts = e->NSBPPts;
     e2.sbp = e->sbp;
+    e2.NBioLayers = e->NBioLayers;
+    e2.bio = e->bio;
     if(prob_from_env(&e2, &Q->P2)) return -1;
     Prob *P = &Q->P2;
     Q->ssp_type = e->ssp_type;

@@ -95,8 +95,32 @@ def attenuation_units(precision=64, unit="F "):
     field above the TL floor."""
     e = _base(precision=precision)
     e.atten_unit = unit
-    level = {"F": 0.002, "M": 2e-5, "Q": 5000.0, "L": 1e-6, "N": 2e-6, "W": 0.01}[unit[0]]
+    level = {"F": 0.002, "M": 2e-5, "m": 2e-5, "Q": 5000.0, "L": 1e-6, "N": 2e-6, "W": 0.01}[unit[0]]
     e.ssp_alphaI = np.full(len(e.ssp_z), level)
+    return e
+
+
+def biological_layers(precision=64):
+    """Biological volume attenuation ('B', Diachok's resonant layers) on top
+    of dB/wavelength attenuation: two layers, one with its resonance near the
+    run frequency."""
+    e = _base(precision=precision)
+    e.atten_unit = "WB"
+    e.ssp_alphaI = np.full(len(e.ssp_z), 0.01)
+    e.bio = np.array([[0.0, 350.0, 48.0, 5.0, 0.02], [900.0, 1600.0, 200.0, 20.0, 0.01]])
+    return e
+
+
+def beam_shift(precision=64):
+    """Beam displacement on reflection from a fluid halfspace (RunType[6] 'S'),
+    with hat beams at the Munk fan's angles."""
+    e = _base(precision=precision, run_type="CG    S")
+    return e
+
+
+def beam_shift_elastic(precision=64):
+    e = elastic_bottom(precision)
+    e.run_type = "CG    S"
     return e
 
 
@@ -117,4 +141,8 @@ FEATURES = {
     "atten_N": lambda precision=64: attenuation_units(precision, "N "),
     "atten_Q": lambda precision=64: attenuation_units(precision, "Q "),
     "atten_L_francois_garrison": lambda precision=64: attenuation_units(precision, "LF"),
+    "atten_m_power_law": lambda precision=64: attenuation_units(precision, "m "),
+    "atten_biological": biological_layers,
+    "beam_shift": beam_shift,
+    "beam_shift_elastic": beam_shift_elastic,
 }

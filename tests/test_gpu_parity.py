@@ -225,3 +225,27 @@ def test_split_passes_keep_counts_and_field(gpu_ctx):
     ta, tb = tl_db(p.scale_pressure(a.field)), tl_db(p.scale_pressure(b.field))
     m = tb < 120.0
     assert np.max(np.abs(ta - tb)[m]) < 0.01
+
+
+def test_split_passes_count_warnings_once(gpu_ctx):
+    """Rays retraced after a segment-buffer overflow report their warnings
+    once: a bottom reflection table that starts at 6 degrees warns
+    (OUTSIDE_REFLCOEF) at every more grazing bounce, and the warning count of
+    a run split over many passes equals the one-pass run's."""
+    from parity import small_segment_context
+    _announce("split_passes_warning_count")
+    env = munk_env(nbeams=2001, nrz=51, nrr=401)
+    env.bot = Halfspace("F", depth=5000.0)
+    env.brc = ReflTable(np.array([6.0, 20.0, 90.0]), np.array([0.9, 0.8, 0.5]), np.array([180.0, 175.0, 170.0]))
+    small = small_segment_context(64)
+    try:
+        a = bh.field_run(bh.Problem(env), ctx=small, stats=True)
+        passes = small.last_phase()[2]
+    finally:
+        small.close()
+    b = bh.field_run(bh.Problem(env), ctx=gpu_ctx, stats=True)
+    print("passes", passes, "warnings", a.warn_count, b.warn_count, flush=True)
+    assert passes >= 3
+    assert b.warn & (1 << 5) and b.warn_count > 0
+    assert (a.err, a.warn, a.err_count, a.warn_count) == (b.err, b.warn, b.err_count, b.warn_count)
+    assert np.array_equal(a.nsteps, b.nsteps)

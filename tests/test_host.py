@@ -23,6 +23,7 @@ import bellhopcuda_amd as bh
 from bellhopcuda_amd import _abi
 from bellhopcuda_amd.env import munk_env, c2_env, subtab, Boundary
 import oracle
+import cases
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "bhc_hip.h")
@@ -109,6 +110,8 @@ PRE_CASES = {
     "munk_C_f32": lambda: munk_env(nbeams=101, nrz=11, nrr=21, precision=32),
     "seamount": lambda: c2_env(nbeams=101, nrz=11, nrr=21),
     "seamount_f32": lambda: c2_env(nbeams=101, nrz=11, nrr=21, precision=32),
+    "biological_layers": lambda: cases.biological_layers(),
+    "biological_layers_f32": lambda: cases.biological_layers(32),
 }
 
 
@@ -122,6 +125,18 @@ def test_preprocessing_matches_oracle_bitwise(name):
         got = got.astype(np.float32).astype(np.float64)
     assert ref.shape == got.shape
     assert np.array_equal(ref, got, equal_nan=True), np.flatnonzero(~((ref == got) | (np.isnan(ref) & np.isnan(got))))[:10]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_power_law_attenuation_equals_db_per_m_at_freq0(precision):
+    """Attenuation unit 'm' (dB/m with a power law in f / freq0,
+    src/module/atten.cpp:120-128) preprocesses bit for bit as 'M' at the run
+    frequency freq0, in the SSP and in the halfspaces."""
+    a, b = cases.attenuation_units(precision, "m "), cases.attenuation_units(precision, "M ")
+    va, vb = _view_arrays(bh.Problem(a)), _view_arrays(bh.Problem(b))
+    assert np.array_equal(va, vb, equal_nan=True)
+    c = cases.attenuation_units(precision, "N ")
+    assert not np.array_equal(va, _view_arrays(bh.Problem(c)), equal_nan=True)
 
 
 def test_preprocess_rejects_unsupported_inputs():
@@ -167,7 +182,6 @@ CORE_CASES = {
 }
 
 
-import cases  # noqa: E402
 
 for _n, _f in cases.FEATURES.items():
     CORE_CASES[_n] = _f

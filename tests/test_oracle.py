@@ -437,13 +437,14 @@ def test_eigen_hits_bracket_receivers():
         assert normal_dist <= width * 1.01
 
 
-@pytest.mark.parametrize("unit,level", [("N ", 2e-5), ("M ", 1e-4), ("F ", 0.5), ("W ", 0.2), ("Q ", 300.0),
-                                        ("L ", 1e-5), ("WT", 0.0)])
+@pytest.mark.parametrize("unit,level", [("N ", 2e-5), ("M ", 1e-4), ("m ", 1e-4), ("F ", 0.5), ("W ", 0.2),
+                                        ("Q ", 300.0), ("L ", 1e-5), ("WT", 0.0)])
 def test_volume_attenuation_closed_form(unit, level):
     """crci (src/module/atten.cpp:99-194): in free space the attenuation adds
     8.686 alpha_T s dB along the path s, alpha_T in nepers/m from the unit's
     definition: N nepers/m, M dB/m, F dB/(m kHz), W dB/wavelength, Q quality
-    factor (omega / (2 c Q)), L loss parameter (alpha omega / c), and with a
+    factor (omega / (2 c Q)), L loss parameter (alpha omega / c), m dB/m with
+    a power law in f / freq0 (at the run frequency freq0 the factor is 1), and with a
     'T' second letter Thorp's formula in dB/km,
     3.3e-3 + 0.11 f^2/(1+f^2) + 44 f^2/(4100+f^2) + 3e-4 f^2 (f in kHz)."""
     D, zs, f, c = 20000.0, 10000.0, 200.0, 1500.0
@@ -458,7 +459,8 @@ def test_volume_attenuation_closed_form(unit, level):
         return -20 * np.log10(np.abs(oracle.scale_pressure(env, o.field)[0]))
 
     w = 2 * math.pi * f
-    alpha_np = {"N": level, "M": level / 8.6858896, "F": level * f / 8685.8896, "W": level * f / (8.6858896 * c),
+    alpha_np = {"N": level, "M": level / 8.6858896, "m": level / 8.6858896, "F": level * f / 8685.8896,
+                "W": level * f / (8.6858896 * c),
                 "Q": w / (2.0 * c * level) if level else 0.0, "L": level * w / c}[unit[0]]
     if unit[1] == "T":
         fk2 = (f / 1000.0) ** 2
@@ -468,3 +470,89 @@ def test_volume_attenuation_closed_form(unit, level):
     expect = 20.0 * np.log10(math.e) * alpha_np * R
     assert np.max(np.abs(excess - expect)) < 1e-3 + 1e-3 * np.max(expect)
     assert np.max(expect) > 0.02
+
+
+@pytest.mark.parametrize("layer,gated", [((-100.0, 25000.0), False), ((-100.0, 100.0), True)])
+def test_biological_attenuation_closed_form(layer, gated):
+    """Biological layers ('B', src/module/atten.cpp:169-180): a knot inside a
+    layer [z1, z2] gains a0 / ((1 - (f0/f)^2)^2 + 1/Q^2) dB/km. With both
+    knots of a 20 km deep isovelocity profile inside the layer the excess
+    attenuation is uniform; with only the top knot inside, the imaginary sound
+    speed falls linearly to zero at the bottom knot, so the excess is the
+    path integral of a(1 - z/D) (10 m steps there: inside a step the delay is
+    interpolated linearly, which takes the step's mean attenuation)."""
+    D, zs, f, c = 20000.0, 10000.0, 200.0, 1500.0
+    f0, Q, a0 = 180.0, 4.0, 0.3
+    rz = subtab(9500.0, 10500.0, 11, np.float32)
+    rr = subtab(0.5, 3.0, 11)
+
+    def tl(u, bio):
+        env = _field_env(D, zs, rz, rr, 2001, (-80.0, 80.0), "CG", freq=f)
+        env.atten_unit = u
+        env.ssp_alphaI = np.zeros(2)
+        env.bio = bio
+        if gated:
+            env.deltas = 10.0
+        o = oracle.field_run(env, nthreads=0, stats=False)
+        return -20 * np.log10(np.abs(oracle.scale_pressure(env, o.field)[0]))
+
+    a = a0 / ((1.0 - (f0 / f) ** 2) ** 2 + 1.0 / Q ** 2) / 8685.8896     # nepers/m
+    bio = np.array([[layer[0], layer[1], f0, Q, a0], [30000.0, 40000.0, f0, Q, 100.0]])  # second layer below
+    R = np.hypot(rr[None, :] * 1000.0, rz[:, None].astype(np.float64) - zs)
+    if gated:
+        zr = rz[:, None].astype(np.float64)
+        expect_np = a * R * (1.0 - 0.5 * (zs + zr) / D)      # mean of a (1 - z/D) along the straight path
+    else:
+        expect_np = a * R
+    excess = tl("NB", bio) - tl("N ", None)
+    expect = 20.0 * np.log10(math.e) * expect_np
+    assert np.max(np.abs(excess - expect)) < 1e-3 + 1e-3 * np.max(expect)
+    assert np.max(expect) > 0.02
+
+
+def test_beam_shift_is_goos_haenchen_shift():
+    """Beam shift on a fluid-halfspace reflection (RunType[6] 'S',
+    src/reflect.hpp:383-442): below the critical angle the reflected point
+    moves forward along the boundary by the lateral (Goos-Haenchen) shift
+    Delta = (d phi / d theta) / (k sin theta) of the Rayleigh phase phi, the
+    delay grows by Delta cos(theta) / c, and q by |d Delta / d theta| sin(theta)
+    c p; above it the shift is zero (the width term is 0/0 there, so the
+    reference's q becomes NaN for a lossless halfspace). Isovelocity water keeps
+    |c t| = 1, so the closed form applies at the ray's own angle."""
+    D, c1, c2, rho2, f = 1000.0, 1500.0, 1700.0, 1.5, 100.0
+    env = iso_env(D=D, c=c1, zs=500.0, alpha=(5.0, 80.0), nb=31, rmax_km=10.0, deltas=50.0, freq=f,
+                  bot_hs=Halfspace("A", depth=D, alphaR=c2, rho=rho2, zTemp=D))
+    env.run_type = "R     S"
+    w = 2 * math.pi * f
+
+    def phase(g):
+        kx, kz1 = w * math.cos(g) / c1, w * math.sin(g) / c1
+        kz2 = np.sqrt(complex((w / c2) ** 2 - kx ** 2))
+        if kz2.imag > 0:
+            kz2 = -kz2
+        return np.angle((rho2 * kz1 - kz2) / (rho2 * kz1 + kz2))
+
+    def shift(g, h=1e-6):
+        return (phase(g + h) - phase(g - h)) / (2 * h) / (w / c1 * math.sin(g))
+
+    crit = math.acos(c1 / c2)
+    below = above = 0
+    for a, r in zip(env.alpha_deg, _rays(env)):
+        before, after = _first_bottom_bounce(r)
+        g = math.radians(a)
+        dx = after["x"][0] - before["x"][0]
+        dtau = after["tau"][0] - before["tau"][0]
+        dq = after["q"][0] - before["q"][0]
+        assert after["x"][1] == before["x"][1]          # flat bottom: the shift is along it
+        if g < crit - math.radians(1.0):
+            below += 1
+            d = abs(shift(g))
+            assert dx == pytest.approx(d, rel=1e-7)
+            assert dtau == pytest.approx(d * math.cos(g) / c1, rel=1e-7)
+            dd = (abs(shift(g + 1e-5)) - abs(shift(g - 1e-5))) / 2e-5
+            assert dq == pytest.approx(abs(dd) * math.sin(g) * c1 * before["p"][0], rel=2e-4)
+        elif g > crit + math.radians(1.0):
+            above += 1
+            assert dx == 0.0 and dtau == 0.0
+            assert np.isnan(dq)
+    assert below >= 5 and above >= 5
