@@ -28,6 +28,10 @@ class BhcError(RuntimeError):
     pass
 
 
+# completion callback of the non-blocking runs: void (*)(void *user)
+COMPLETED_FN = C.CFUNCTYPE(None, C.c_void_p)
+
+
 def lib():
     """Load libbhc_hip.so (built by bellhopcuda_amd.build / __graft_entry__.build)."""
     global _LIB
@@ -65,6 +69,13 @@ def lib():
         "bhc_sweep_upload2d": (C.c_int, [C.c_void_p, P(P(_abi.Problem2D)), C.c_int, C.c_int, P(C.c_void_p)]),
         "bhc_sweep_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
         "bhc_sweep_launch_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+        "bhc_progress": (C.c_int, [C.c_void_p, P(C.c_int64), P(C.c_int64)]),
+        "bhc_field_run2d_async": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.FieldOut), COMPLETED_FN,
+                                            C.c_void_p]),
+        "bhc_field_run3d_async": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut), COMPLETED_FN,
+                                            C.c_void_p]),
+        "bhc_ctx_busy": (C.c_int, [C.c_void_p]),
+        "bhc_ctx_wait": (C.c_int, [C.c_void_p]),
         "bhc_sweep_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, P(_abi.FieldOut)]),
         "bhc_sweep_stats": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), P(C.c_longlong),
                                       P(C.c_longlong)]),
@@ -72,6 +83,7 @@ def lib():
         "bhc_sweep_autotune": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "bhc_sweep_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
         "bhc_sweep_set_scatter": (C.c_int, [C.c_void_p, C.c_int]),
+        "bhc_sweep_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "bhc_last_phase_ms": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int)]),
         "bhc_opcount2d": (C.c_int, [P(_abi.Problem2D), C.c_int, P(C.c_double)]),
         "bhc_subtab": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
@@ -178,6 +190,19 @@ class Context:
     def set_launch(self, blocks_per_cu=0):
         _check(lib().bhc_ctx_set_launch(self._h, 0, blocks_per_cu))
 
+    def progress(self):
+        """(rays finished, rays of the run) of the current or last run."""
+        d, t = C.c_int64(), C.c_int64()
+        _check(lib().bhc_progress(self._h, C.byref(d), C.byref(t)))
+        return d.value, t.value
+
+    def busy(self):
+        return bool(lib().bhc_ctx_busy(self._h))
+
+    def wait(self):
+        """Join a non-blocking run; raises if it failed."""
+        _check(lib().bhc_ctx_wait(self._h))
+
     def last_kernel_ms(self):
         v = C.c_double()
         _check(lib().bhc_last_kernel_ms(self._h, C.byref(v)))
@@ -246,6 +271,36 @@ def field_run(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1, e
     res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
     res.problem = prob
     return res
+
+
+class AsyncRun:
+    """A non-blocking field run (bhc_field_run2d_async / bhc_field_run3d_async):
+    returns at once; `on_done` (no arguments) is called on the context's
+    worker thread when the outputs are written. wait() joins it and returns
+    the FieldResult."""
+
+    def __init__(self, prob, ctx, on_done=None, stats=False, max_narr=0, allow_merging=1, eigen_memsize=0):
+        self.prob, self.ctx = prob, ctx
+        self.res = FieldResult()
+        mk = _make_out3d if isinstance(prob, Problem3D) else _make_out
+        self.out = mk(prob, self.res, stats, max_narr, allow_merging, eigen_memsize)
+        self.calls = 0
+
+        def done(_user):
+            self.calls += 1
+            if on_done is not None:
+                on_done()
+        self._cb = COMPLETED_FN(done)      # kept alive until the run has ended
+        f = lib().bhc_field_run3d_async if isinstance(prob, Problem3D) else lib().bhc_field_run2d_async
+        _check(f(ctx._h, C.byref(prob.view), C.byref(self.out), self._cb, None))
+
+    def wait(self):
+        self.ctx.wait()
+        r, out = self.res, self.out
+        r.err, r.warn, r.err_count, r.warn_count = out.err, out.warn, out.err_count, out.warn_count
+        r.total_steps, r.total_contrib = out.total_steps, out.total_contrib
+        r.problem = self.prob
+        return r
 
 
 class Resident:
@@ -348,6 +403,12 @@ class Sweep:
         receivers flattened over a wavefront."""
         _check(lib().bhc_sweep_set_scatter(self._h, mapping))
 
+    def set_launch(self, scatter_blocks, lds_bytes, mapping=None):
+        """Launch configuration without a search (as autotune() reports it)."""
+        _check(lib().bhc_sweep_set_launch(self._h, scatter_blocks, lds_bytes))
+        if mapping is not None:
+            self.set_scatter(mapping)
+
     def stats(self):
         t, s, p = C.c_double(), C.c_double(), C.c_int()
         st, co = C.c_longlong(), C.c_longlong()
@@ -416,6 +477,39 @@ class Problem3D:
         return f
 
 
+def _make_out3d(prob, res, stats, max_narr, allow_merging, eigen_memsize):
+    nrays = prob.nrays
+    res.field = np.zeros(prob.field_shape, np.complex64)
+    out = _abi.FieldOut()
+    out.field = res.field.ctypes.data_as(_abi.c_float_p)
+    rt = prob.run_type[0]
+    nrcv = int(np.prod(prob.field_shape))
+    if rt in "AaVv":
+        res.max_narr = max_narr
+        res.arr = (_abi.Arrival * (nrcv * max(max_narr, 1)))()
+        res.narr = np.zeros(nrcv, np.int32)
+        out.arr = C.cast(res.arr, C.POINTER(_abi.Arrival))
+        out.narr = res.narr.ctypes.data_as(_abi.c_int32_p)
+        out.max_narr = max_narr
+        out.allow_merging = allow_merging
+    if rt in "EVv":
+        res.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
+        res.neigen = np.zeros(1, np.int32)
+        out.hits = C.cast(res.hits, C.POINTER(_abi.EigenHit))
+        out.neigen = res.neigen.ctypes.data_as(_abi.c_int32_p)
+        out.eigen_memsize = eigen_memsize
+    if stats:
+        res.nsteps = np.zeros(nrays, np.int32)
+        res.ntop = np.zeros(nrays, np.int32)
+        res.nbot = np.zeros(nrays, np.int32)
+        res.xend = np.zeros((nrays, 3), np.float64)
+        out.ray_nsteps = res.nsteps.ctypes.data_as(_abi.c_int32_p)
+        out.ray_ntop = res.ntop.ctypes.data_as(_abi.c_int32_p)
+        out.ray_nbot = res.nbot.ctypes.data_as(_abi.c_int32_p)
+        out.ray_xend = res.xend.ctypes.data_as(_abi.c_double_p)
+    return out
+
+
 def field_run3d(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1, eigen_memsize=0):
     """3D field run on the GPU (RunFieldModesImpl<CFG, true, true>). Returns
     a FieldResult: .field (complex64, [NSz, NSx, NSy, Ntheta, NRz, NRr],
@@ -429,35 +523,7 @@ def field_run3d(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1,
         ctx = Context(0)
     try:
         res = FieldResult()
-        nrays = prob.nrays
-        res.field = np.zeros(prob.field_shape, np.complex64)
-        out = _abi.FieldOut()
-        out.field = res.field.ctypes.data_as(_abi.c_float_p)
-        rt = prob.run_type[0]
-        nrcv = int(np.prod(prob.field_shape))
-        if rt in "AaVv":
-            res.max_narr = max_narr
-            res.arr = (_abi.Arrival * (nrcv * max(max_narr, 1)))()
-            res.narr = np.zeros(nrcv, np.int32)
-            out.arr = C.cast(res.arr, C.POINTER(_abi.Arrival))
-            out.narr = res.narr.ctypes.data_as(_abi.c_int32_p)
-            out.max_narr = max_narr
-            out.allow_merging = allow_merging
-        if rt in "EVv":
-            res.hits = (_abi.EigenHit * max(eigen_memsize, 1))()
-            res.neigen = np.zeros(1, np.int32)
-            out.hits = C.cast(res.hits, C.POINTER(_abi.EigenHit))
-            out.neigen = res.neigen.ctypes.data_as(_abi.c_int32_p)
-            out.eigen_memsize = eigen_memsize
-        if stats:
-            res.nsteps = np.zeros(nrays, np.int32)
-            res.ntop = np.zeros(nrays, np.int32)
-            res.nbot = np.zeros(nrays, np.int32)
-            res.xend = np.zeros((nrays, 3), np.float64)
-            out.ray_nsteps = res.nsteps.ctypes.data_as(_abi.c_int32_p)
-            out.ray_ntop = res.ntop.ctypes.data_as(_abi.c_int32_p)
-            out.ray_nbot = res.nbot.ctypes.data_as(_abi.c_int32_p)
-            out.ray_xend = res.xend.ctypes.data_as(_abi.c_double_p)
+        out = _make_out3d(prob, res, stats, max_narr, allow_merging, eigen_memsize)
         _check(lib().bhc_field_run3d(ctx._h, C.byref(prob.view), C.byref(out)))
         res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
         res.err, res.warn = out.err, out.warn

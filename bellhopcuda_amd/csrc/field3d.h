@@ -40,6 +40,7 @@ struct DevOut3 {
     int32_t eigen_memsize;
     bhc_eigen_hit *hits;        // eigen hits
     int32_t *neigen;
+    Progress prog;              // finished-ray progress (as in the 2D trace)
 };
 
 struct Sink3 {
@@ -143,7 +144,7 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
     Trace3<real> S;
     Infl3<real> I;
     int job = 0, ia = 0, ib = 0, isx = 0, isy = 0, isz = 0, is = 0;
-    bool active = false, exhausted = false;
+    bool active = false, exhausted = false, init_fail = false;
     unsigned long long steps = 0;
     SegBlock blk{0u, 0u, 0u};
     for(int iter = 0;; ++iter) {
@@ -163,12 +164,15 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
                         JobIndices3(job, P, ia, ib, isy, isx, isz);
                         is = 0;
                         active = RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e);
+                        init_fail = !active;
                         if(!active && O.stats) {
                             O.stats[3 * job] = O.stats[3 * job + 1] = O.stats[3 * job + 2] = 0;
                             O.xend[3 * job] = O.xend[3 * job + 1] = O.xend[3 * job + 2] = 0.0;
                         }
                     }
                 }
+                note_done(O.prog, init_fail);
+                init_fail = false;
             }
         }
         if(!__any(active)) {
@@ -209,10 +213,12 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
             }
             append3<real, decltype(d), RCEN>(d, lane, O, blk);
         }
+        bool fin = false;
         if(active) {
             p0 = two ? p2 : p1;
             if(RayTerminate3(p0, is, S, P, e)) {
                 active = false;
+                fin = true;
                 steps += (unsigned long long)is;
                 if(O.stats) {
                     O.stats[3 * job] = is;
@@ -224,6 +230,7 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
                 }
             }
         }
+        note_done(O.prog, fin);
     }
     seal3<real, RCEN>(blk, lane, O);
     if(e.err) atomicOr(O.errw + 0, e.err);
@@ -667,6 +674,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
     O.hits = d_hits;
     O.neigen = d_neigen;
     O.eigen_memsize = hits_too ? out->eigen_memsize : 0;
+    O.prog = prog_of(c);
+    TRY(prog_begin(c, njobs));
     // split the rays into passes that fit the segment buffer, from the
     // segments per ray of earlier runs on this context (a pass that still
     // overflows is halved and retraced)
@@ -698,7 +707,9 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
         // error / warning words as the completed passes left them, restored
         // if this trace pass overflows and its rays are traced again
         uint32_t ew_start[4] = {0u, 0u, 0u, 0u};
+        unsigned long long prog_start = 0;
         HTRY(hipMemcpyAsync(ew_start, d_errw, sizeof(ew_start), hipMemcpyDeviceToHost, s));
+        HTRY(hipMemcpyAsync(&prog_start, c->d_prog, sizeof(prog_start), hipMemcpyDeviceToHost, s));
         if(first) HTRY(hipEventRecord(c->ev0, s));
         HTRY(hipEventRecord(c->pa, s));
         if(rcen) hipLaunchKernelGGL((k_trace3d<real, true>), dim3(tblocks), dim3(WAVE), 0, s, P, O);
@@ -724,6 +735,7 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             // passes' (the retried rays report theirs again)
             HTRY(hipMemcpyAsync(d_cnt, &steps_done, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
             HTRY(hipMemcpyAsync(d_errw, ew_start, sizeof(ew_start), hipMemcpyHostToDevice, s));
+            HTRY(hipMemcpyAsync(c->d_prog, &prog_start, sizeof(prog_start), hipMemcpyHostToDevice, s));
             HTRY(hipStreamSynchronize(s));
             continue;
         }
@@ -827,7 +839,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
 
 extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out)
 {
-    if(!ctx || !prob || !out) return fail("bhc_field_run3d: null argument");
+    if(check_idle(ctx, "bhc_field_run3d")) return -1;
+    if(!prob || !out) return fail("bhc_field_run3d: null argument");
     if(prob->precision != 64 && prob->precision != 32) return fail("bhc_field_run3d: precision must be 32 or 64");
     char r0 = prob->RunType[0], r1 = prob->RunType[1];
     if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'A' && r0 != 'a' && r0 != 'E' && r0 != 'V' && r0 != 'v')
@@ -846,23 +859,32 @@ extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_fiel
     if(prob->NSBPPts < 2) return fail("bhc_field_run3d: source beam pattern needs 2 points");
     if(!(prob->deltas > 0)) return fail("bhc_field_run3d: deltas must be positive");
     if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_field_run3d: hipSetDevice failed");
-    return prob->precision == 64 ? run3d<double>(ctx, prob, out) : run3d<float>(ctx, prob, out);
+    const int rc = prob->precision == 64 ? run3d<double>(ctx, prob, out) : run3d<float>(ctx, prob, out);
+    if(!rc) ctx->prog_final = 1;
+    return rc;
 }
 
 extern "C" int bhc_ray_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_out3d *out)
 {
-    if(!ctx || !prob || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d: null argument");
+    if(check_idle(ctx, "bhc_ray_run3d")) return -1;
+    if(!prob || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d: null argument");
     if(prob->NSx < 1 || prob->NSy < 1 || prob->NSz < 1 || prob->Nalpha < 1 || prob->Nbeta < 1)
         return fail("bhc_ray_run3d: empty source / angle set");
     if(!(prob->deltas > 0)) return fail("bhc_ray_run3d: deltas must be positive");
     if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_ray_run3d: hipSetDevice failed");
-    return prob->precision == 64 ? ray3d<double>(ctx, prob, out) : ray3d<float>(ctx, prob, out);
+    const long long n = (long long)(prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta)
+        * prob->NSx * prob->NSy * prob->NSz;
+    if(prog_begin(ctx, n)) return -1;
+    const int rc = prob->precision == 64 ? ray3d<double>(ctx, prob, out) : ray3d<float>(ctx, prob, out);
+    if(!rc) ctx->prog_final = 1;
+    return rc;
 }
 
 extern "C" int bhc_ray_run3d_list(bhc_ctx *ctx, const bhc_problem3d *prob, const bhc_eigen_hit *hits, int32_t nhits,
                                   bhc_ray_out3d *out)
 {
-    if(!ctx || !prob || !hits || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d_list: null argument");
+    if(check_idle(ctx, "bhc_ray_run3d_list")) return -1;
+    if(!prob || !hits || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d_list: null argument");
     if(nhits < 1) return fail("bhc_ray_run3d_list: no hits");
     for(int k = 0; k < nhits; ++k) {
         const bhc_eigen_hit &h = hits[k];
@@ -872,6 +894,81 @@ extern "C" int bhc_ray_run3d_list(bhc_ctx *ctx, const bhc_problem3d *prob, const
     }
     if(!(prob->deltas > 0)) return fail("bhc_ray_run3d_list: deltas must be positive");
     if(hipSetDevice(ctx->device) != hipSuccess) return fail("bhc_ray_run3d_list: hipSetDevice failed");
-    return prob->precision == 64 ? ray3d<double>(ctx, prob, out, hits, nhits)
-                                 : ray3d<float>(ctx, prob, out, hits, nhits);
+    if(prog_begin(ctx, nhits)) return -1;
+    const int rc = prob->precision == 64 ? ray3d<double>(ctx, prob, out, hits, nhits)
+                                         : ray3d<float>(ctx, prob, out, hits, nhits);
+    if(!rc) ctx->prog_final = 1;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Non-blocking runs (the reference's extsetup_blocking(params, false) mode,
+// src/mode/fieldimpl.cpp.in:46-52,66-71): the run is performed by a worker
+// thread of the context, which fires the completion callback once the outputs
+// have been written; bhc_progress reports its finished rays meanwhile.
+// ---------------------------------------------------------------------------
+namespace {
+template<typename F> int start_async(bhc_ctx *ctx, const char *who, long long total, F run, bhc_completed_fn cb,
+                                     void *user)
+{
+    if(!ctx) return fail("%s: null context", who);
+    if(tl_async_worker) return fail("%s: a run cannot be started from a completion callback", who);
+    int idle = 0;
+    if(!ctx->busy.compare_exchange_strong(idle, 1)) return fail("%s: a non-blocking run is in flight", who);
+    if(ctx->worker.joinable()) ctx->worker.join();   // the previous run has finished
+    ctx->prog_final = 0;
+    ctx->prog_seen = 0;
+    ctx->prog_total = total;
+    __atomic_store_n(ctx->h_prog, 0ull, __ATOMIC_RELAXED);
+    ctx->async_rc = 0;
+    ctx->async_msg.clear();
+    try {
+        ctx->worker = std::thread([ctx, run, cb, user]() {
+            tl_async_worker = true;
+            const int rc = run();
+            ctx->async_rc = rc;
+            if(rc) ctx->async_msg = g_last_error;
+            ctx->busy = 0;
+            if(cb) cb(user);
+        });
+    } catch(...) {
+        ctx->busy = 0;
+        return fail("%s: could not start the worker thread", who);
+    }
+    return 0;
+}
+} // namespace
+
+extern "C" int bhc_field_run2d_async(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_field_out *out,
+                                     bhc_completed_fn completed, void *user)
+{
+    if(!prob || !out) return fail("bhc_field_run2d_async: null argument");
+    if(check_problem(prob)) return -1;
+    const long long total = (long long)(prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * prob->NSz;
+    return start_async(ctx, "bhc_field_run2d_async", total, [ctx, prob, out]() { return bhc_field_run2d(ctx, prob, out); },
+                       completed, user);
+}
+
+extern "C" int bhc_field_run3d_async(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out,
+                                     bhc_completed_fn completed, void *user)
+{
+    if(!prob || !out) return fail("bhc_field_run3d_async: null argument");
+    const long long total = (long long)(prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha)
+        * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta) * prob->NSx * prob->NSy * prob->NSz;
+    return start_async(ctx, "bhc_field_run3d_async", total, [ctx, prob, out]() { return bhc_field_run3d(ctx, prob, out); },
+                       completed, user);
+}
+
+extern "C" int bhc_ctx_busy(bhc_ctx *ctx) { return ctx && ctx->busy.load() ? 1 : 0; }
+
+extern "C" int bhc_ctx_wait(bhc_ctx *ctx)
+{
+    if(!ctx) return fail("bhc_ctx_wait: null context");
+    if(tl_async_worker) return fail("bhc_ctx_wait: cannot wait from a completion callback");
+    if(ctx->worker.joinable()) ctx->worker.join();
+    if(ctx->async_rc) {
+        g_last_error = ctx->async_msg;
+        return ctx->async_rc;
+    }
+    return 0;
 }

@@ -31,6 +31,8 @@
 #include <vector>
 #include <algorithm>
 #include <cmath>
+#include <atomic>
+#include <thread>
 
 #include "../../include/bhc_hip.h"
 #include "core2d.h"
@@ -76,6 +78,30 @@ template<typename real> struct alignas(16) Seg {
     SegDesc<real> d;
 };
 
+// Finished-ray progress of a launch (completedRayCount / totalJobs,
+// src/mode/fieldimpl.cpp.in:43,62-63): wavefronts add their finished rays to
+// a device counter; each time the total crosses a multiple of PROG_QUANTUM it
+// is also stored (plain system-scope store, no host atomics) to the
+// context's pinned host word, which bhc_progress reads while kernels run.
+// Stores may land out of order; the reader keeps the largest value seen.
+constexpr unsigned long long PROG_QUANTUM = 256ull;
+struct Progress {
+    unsigned long long *dev;    // device counter (null: not counted)
+    unsigned long long *host;   // pinned host mirror
+};
+
+__device__ inline void note_done(const Progress &G, bool fin)
+{
+    const unsigned long long m = __ballot(fin);
+    if(!m || !G.dev) return;
+    if((int)(threadIdx.x & (WAVE - 1)) == __ffsll(m) - 1) {
+        const unsigned long long c = (unsigned long long)__popcll(m);
+        const unsigned long long old = atomicAdd(G.dev, c);
+        if((old + c) / PROG_QUANTUM != old / PROG_QUANTUM)
+            __hip_atomic_store(G.host, old + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 struct DevOut {
     ArrCand *cand;
     int32_t *ncand;
@@ -99,6 +125,7 @@ struct DevOut {
     uint32_t *seg_count;
     uint32_t seg_cap;
     uint32_t *overflow;
+    Progress prog;
 };
 
 template<typename real> struct DevSink {
@@ -300,7 +327,7 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
     Trace<real> S;
     Infl<real> I;
     int job = 0, isz = 0, ia = 0, is = 0;
-    bool active = false, exhausted = false;
+    bool active = false, exhausted = false, init_fail = false;
     unsigned long long steps = 0;
     SegBlock blk{0u, 0u, 0u};
     for(int iter = 0;; ++iter) {
@@ -320,6 +347,7 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
                         JobIndices(job, P, isz, ia);
                         is = 0;
                         active = RayInit<real, ST>(isz, ia, p0, S, I, P, B, G, e);
+                        init_fail = !active;
                         if(!active && O.stats) {
                             O.stats[3 * job] = 0;
                             O.stats[3 * job + 1] = 0;
@@ -329,6 +357,8 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
                         }
                     }
                 }
+                note_done(O.prog, init_fail);
+                init_fail = false;
             }
         }
         if(!__any(active)) {
@@ -366,11 +396,13 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
             }
             append_seg(d, lane, O, blk);
         }
+        bool fin = false;
         if(active) {
             p0 = two ? p2 : p1;
             int t = RayTerminate(p0, is, S, MAXN, P, e);
             if(t) {
                 active = false;
+                fin = true;
                 steps += (unsigned long long)is;
                 if(O.stats) {
                     O.stats[3 * job] = is;
@@ -381,6 +413,7 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
                 }
             }
         }
+        note_done(O.prog, fin);
     }
     seal_block<real>(blk, lane, O);
     flush_err(e, O);
@@ -836,7 +869,44 @@ struct bhc_ctx {
     size_t seg_mb;              // segment buffer size (MiB), allocated at first use
     double seg_per_ray3;        // 3D runs: largest segments-per-ray seen (sizes the first pass)
     int scatter3_lanes;         // 3D scatter mapping: 0 one wavefront per segment, 1 one lane per segment
+    // finished-ray progress of the current or last run (bhc_progress)
+    unsigned long long *h_prog; // pinned, device-visible host word written by the trace kernels
+    unsigned long long *d_prog; // device counter
+    std::atomic<long long> prog_total{0}, prog_seen{0};
+    std::atomic<int> prog_final{0};
+    // non-blocking runs: one worker thread at a time per context
+    std::thread worker;
+    std::atomic<int> busy{0};
+    int async_rc = 0;
+    std::string async_msg;
 };
+
+namespace {
+// set on a context's worker thread: the run it performs may use the context
+thread_local bool tl_async_worker = false;
+
+// Entry points that use a context refuse while a non-blocking run holds it.
+int check_idle(bhc_ctx *c, const char *who)
+{
+    if(!c) return fail("%s: null context", who);
+    if(c->busy.load() && !tl_async_worker) return fail("%s: a non-blocking run is in flight on this context", who);
+    return 0;
+}
+
+// Start of a run of `total` rays: the host word now, the device counter on the
+// stream ahead of the run's kernels.
+int prog_begin(bhc_ctx *c, long long total)
+{
+    c->prog_final = 0;
+    c->prog_seen = 0;
+    c->prog_total = total;
+    __atomic_store_n(c->h_prog, 0ull, __ATOMIC_RELAXED);
+    HIPCHECK(hipMemsetAsync(c->d_prog, 0, sizeof(unsigned long long), c->stream));
+    return 0;
+}
+
+Progress prog_of(bhc_ctx *c) { return Progress{c->d_prog, c->h_prog}; }
+} // namespace
 
 struct bhc_dev_problem {
     bhc_ctx *ctx;
@@ -914,6 +984,9 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     HIPCHECK(hipEventCreate(&c->pb));
     HIPCHECK(hipMalloc((void **)&c->d_seg_count, sizeof(uint32_t)));
     HIPCHECK(hipMalloc((void **)&c->d_overflow, sizeof(uint32_t)));
+    HIPCHECK(hipMalloc((void **)&c->d_prog, sizeof(unsigned long long)));
+    HIPCHECK(hipHostMalloc((void **)&c->h_prog, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    *c->h_prog = 0ull;
     *out = c;
     return 0;
 }
@@ -921,6 +994,10 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
 extern "C" void bhc_ctx_destroy(bhc_ctx *ctx)
 {
     if(!ctx) return;
+    if(ctx->worker.joinable()) {
+        if(ctx->worker.get_id() == std::this_thread::get_id()) ctx->worker.detach();
+        else ctx->worker.join();
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(ctx->ev0);
@@ -932,8 +1009,27 @@ extern "C" void bhc_ctx_destroy(bhc_ctx *ctx)
     if(ctx->d_bin_idx) (void)hipFree(ctx->d_bin_idx);
     (void)hipFree(ctx->d_seg_count);
     (void)hipFree(ctx->d_overflow);
+    (void)hipFree(ctx->d_prog);
+    (void)hipHostFree(ctx->h_prog);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+extern "C" int bhc_progress(bhc_ctx *ctx, int64_t *done, int64_t *total)
+{
+    if(!ctx || !done || !total) return fail("bhc_progress: null argument");
+    const long long t = ctx->prog_total.load();
+    long long d = t;
+    if(!ctx->prog_final.load()) {
+        d = std::min((long long)__atomic_load_n(ctx->h_prog, __ATOMIC_RELAXED), t);
+        long long seen = ctx->prog_seen.load();
+        while(d > seen && !ctx->prog_seen.compare_exchange_weak(seen, d)) {
+        }
+        d = std::max(d, seen);
+    }
+    *done = d;
+    *total = t;
+    return 0;
 }
 
 extern "C" int bhc_ctx_set_launch(bhc_ctx *ctx, int threads_per_block, int blocks_per_cu)
@@ -963,7 +1059,8 @@ __global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, u
 
 extern "C" int bhc_copy_calibrate(bhc_ctx *ctx, long long bytes, double *ms)
 {
-    if(!ctx || bytes < 16 || (bytes & 15)) return fail("bhc_copy_calibrate: bytes must be a positive multiple of 16");
+    if(check_idle(ctx, "bhc_copy_calibrate")) return -1;
+    if(bytes < 16 || (bytes & 15)) return fail("bhc_copy_calibrate: bytes must be a positive multiple of 16");
     HIPCHECK(hipSetDevice(ctx->device));
     void *a = nullptr, *b = nullptr;
     HIPCHECK(hipMalloc(&a, (size_t)bytes));
@@ -1224,6 +1321,7 @@ DevOut make_out(bhc_dev_problem *dp)
     O.work = dp->d_work;
     O.job_begin = 0;
     O.job_end = dp->njobs;
+    O.prog = prog_of(dp->ctx);
     return O;
 }
 
@@ -1271,7 +1369,9 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
         // order: after the previous pass's scatter), restored if this trace
         // pass overflows and its rays are traced again
         uint32_t ew_start[4] = {0u, 0u, 0u, 0u};
+        unsigned long long prog_start = 0;
         HIPCHECK(hipMemcpyAsync(ew_start, dp->d_errw, sizeof(ew_start), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(&prog_start, c->d_prog, sizeof(prog_start), hipMemcpyDeviceToHost, s));
         if(first) HIPCHECK(hipEventRecord(c->ev0, s));
         HIPCHECK(hipEventRecord(c->pa, s));
         switch(P.ssp_type) {
@@ -1301,6 +1401,7 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
             // passes' values, so that the retried rays' warnings count once
             HIPCHECK(hipMemcpyAsync(dp->d_counters, &steps_done, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
             HIPCHECK(hipMemcpyAsync(dp->d_errw, ew_start, sizeof(ew_start), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(c->d_prog, &prog_start, sizeof(prog_start), hipMemcpyHostToDevice, s));
             HIPCHECK(hipStreamSynchronize(s));
             continue;
         }
@@ -1364,9 +1465,11 @@ extern "C" int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode
 
 extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
 {
+    if(check_idle(ctx, "bhc_field_launch")) return -1;
     if(!dp || dp->ctx != ctx) return fail("bhc_field_launch: problem not uploaded on this context");
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
+    if(prog_begin(ctx, dp->njobs)) return -1;
     if(dp->d_field) HIPCHECK(hipMemsetAsync(dp->d_field, 0, 2 * dp->nrcv * sizeof(float), s));
     if(dp->d_ncand) HIPCHECK(hipMemsetAsync(dp->d_ncand, 0, dp->nrcv * sizeof(int32_t), s));
     if(dp->d_narr) HIPCHECK(hipMemsetAsync(dp->d_narr, 0, dp->nrcv * sizeof(int32_t), s));
@@ -1396,6 +1499,7 @@ extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
                            dp->d_errw);
         HIPCHECK(hipGetLastError());
     }
+    ctx->prog_final = 1;   // every trace pass has completed: all rays have finished
     return 0;
 }
 
@@ -1411,6 +1515,7 @@ extern "C" int bhc_problem_set_scatter(bhc_dev_problem *dp, int mode)
 // segment) is launched twice and the fastest kept for later launches.
 extern "C" int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode_out)
 {
+    if(check_idle(ctx, "bhc_problem_autotune")) return -1;
     if(!dp || dp->ctx != ctx) return fail("bhc_problem_autotune: problem not uploaded on this context");
     int modes[3] = {SCATTER_TILED, SCATTER_WAVE, SCATTER_LANES};
     int best_mode = SCATTER_AUTO;
@@ -1480,6 +1585,7 @@ extern "C" int bhc_field_download(bhc_ctx *ctx, bhc_dev_problem *dp, bhc_field_o
 
 extern "C" int bhc_field_run2d(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_field_out *out)
 {
+    if(check_idle(ctx, "bhc_field_run2d")) return -1;
     bhc_dev_problem *dp = nullptr;
     if(bhc_problem_upload2d(ctx, prob, out, &dp)) return -1;
     int rc = bhc_field_launch(ctx, dp);
@@ -1496,6 +1602,7 @@ extern "C" int bhc_ray_run2d(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_ray_ou
 extern "C" int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_ray_out *out, const int32_t *ialpha,
                                   const int32_t *isz, const int32_t *nsteps_in, int32_t nrays)
 {
+    if(check_idle(ctx, "bhc_ray_run2d")) return -1;
     if(check_problem(prob)) return -1;
     if(out->max_points_per_ray < 3) return fail("max_points_per_ray must be >= 3");
     HIPCHECK(hipSetDevice(ctx->device));
@@ -1506,6 +1613,10 @@ extern "C" int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_r
     bhc_dev_problem *dp = nullptr;
     if(bhc_problem_upload2d(ctx, &p2, &spec, &dp)) return -1;
     int njobs = nrays >= 0 ? nrays : dp->njobs;
+    if(prog_begin(ctx, njobs)) {
+        bhc_problem_release(dp);
+        return -1;
+    }
     DevOut O = make_out(dp);
     O.job_begin = 0;
     O.job_end = njobs;
@@ -1556,6 +1667,7 @@ extern "C" int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_r
     for(void *p : fr)
         if(p) (void)hipFree(p);
     bhc_problem_release(dp);
+    if(!rc) ctx->prog_final = 1;
     return rc;
 }
 
@@ -1642,7 +1754,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
     const real *B = stage_problem(P, Bd.lds_cap, lds);
     const real *G = P.base;
     const int nr = P.njobs;
-    bool pexh = false, active = false;
+    bool pexh = false, active = false, init_fail = false;
     int result = 0;
     Err e = {0, 0, 0, 0};
     RayPt<real> p0;
@@ -1678,6 +1790,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
                         JobIndices(job, P, isz, ia);
                         is = 0;
                         active = RayInit<real, ST>(isz, ia, p0, S, I, P, B, G, e);
+                        init_fail = !active;
                         if(!active && P.ray_stats) {
                             P.ray_stats[3 * job] = 0;
                             P.ray_stats[3 * job + 1] = 0;
@@ -1687,6 +1800,8 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
                         }
                     }
                 }
+                note_done(O.prog, init_fail);
+                init_fail = false;
             }
         }
         if(!__any(active)) break;
@@ -1738,10 +1853,12 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
             }
             append_seg(d, lane, O, blk);
         }
+        bool fin = false;
         if(active) {
             p0 = two ? p2 : p1;
             if(RayTerminate(p0, is, S, MAXN, P, e)) {
                 active = false;
+                fin = true;
                 steps += (unsigned long long)is;
                 if(P.ray_stats) {
                     P.ray_stats[3 * job] = is;
@@ -1752,6 +1869,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
                 }
             }
         }
+        note_done(O.prog, fin);
     }
     flush_problem_err(e, Bd.perr + 4 * pid);
     unsigned long long t = steps;
@@ -2124,6 +2242,7 @@ template<typename real> DevOut batch_out(bhc_sweep *sw)
     O.seg_count = c->d_seg_count;
     O.overflow = c->d_overflow;
     O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg<real>), 0xFFFFFFF0u);
+    O.prog = prog_of(c);
     return O;
 }
 
@@ -2178,6 +2297,11 @@ template<typename real> int sweep_core(bhc_sweep *sw, int p0, int p1, bool graph
     if(margin * 2 >= O.seg_cap) return fail("segment buffer too small for a batched sweep (raise BHC_SEG_BUFFER_MB)");
     Bd.suspend_at = (uint32_t)(O.seg_cap - margin);
     const int np = p1 - p0;
+    {
+        long long rays = 0;
+        for(int i = p0; i < p1; ++i) rays += sw->njobs[i];
+        if(prog_begin(c, rays)) return -1;
+    }
     const size_t f0 = sw->field_off[p0], f1 = p1 == sw->nprob ? sw->field_floats : sw->field_off[p1];
     HIPCHECK(hipMemsetAsync(sw->d_fields + f0, 0, (f1 - f0) * sizeof(float), s));
     HIPCHECK(hipMemsetAsync(sw->d_ray_next + p0, 0, np * sizeof(int32_t), s));
@@ -2267,6 +2391,7 @@ template<typename real> int sweep_core(bhc_sweep *sw, int p0, int p1, bool graph
     (void)hipEventDestroy(e2);
     HIPCHECK(hipEventRecord(c->ev1, s));
     c->passes = sw->passes;
+    if(!rc) c->prog_final = 1;
     return rc;
 }
 
@@ -2353,6 +2478,7 @@ extern "C" int bhc_sweep_upload2d(bhc_ctx *ctx, const bhc_problem2d *const *prob
 
 extern "C" int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw)
 {
+    if(check_idle(ctx, "bhc_sweep_launch")) return -1;
     if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_launch: bad arguments");
     HIPCHECK(hipSetDevice(ctx->device));
     return sw->precision == 64 ? sweep_core<double>(sw, 0, sw->nprob, sw->use_graph)
@@ -2361,6 +2487,7 @@ extern "C" int bhc_sweep_launch(bhc_ctx *ctx, bhc_sweep *sw)
 
 extern "C" int bhc_sweep_launch_range(bhc_ctx *ctx, bhc_sweep *sw, int p0, int p1)
 {
+    if(check_idle(ctx, "bhc_sweep_launch_range")) return -1;
     if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_launch_range: bad arguments");
     if(p0 < 0 || p1 > sw->nprob || p0 >= p1)
         return fail("bhc_sweep_launch_range: range [%d, %d) outside the sweep's %d problems", p0, p1, sw->nprob);
@@ -2372,6 +2499,7 @@ extern "C" int bhc_sweep_launch_range(bhc_ctx *ctx, bhc_sweep *sw, int p0, int p
 extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *lds_bytes,
                                   int *scatter_lanes)
 {
+    if(check_idle(ctx, "bhc_sweep_autotune")) return -1;
     if(!ctx || !sw || sw->ctx != ctx) return fail("bhc_sweep_autotune: bad arguments");
     HIPCHECK(hipSetDevice(ctx->device));
     int rc = sw->precision == 64 ? sweep_tune<double>(sw) : sweep_tune<float>(sw);
@@ -2379,6 +2507,19 @@ extern "C" int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_bloc
     if(lds_bytes) *lds_bytes = sw->lds_bytes;
     if(scatter_lanes) *scatter_lanes = sw->scatter_lanes;
     return rc;
+}
+
+extern "C" int bhc_sweep_set_launch(bhc_sweep *sw, int scatter_blocks, int lds_bytes)
+{
+    if(!sw) return fail("bhc_sweep_set_launch: null sweep");
+    const int rsz = sw->precision == 64 ? (int)sizeof(double) : (int)sizeof(float);
+    if(scatter_blocks < 1 || scatter_blocks > (1 << 20)) return fail("bhc_sweep_set_launch: scatter_blocks out of range");
+    if(lds_bytes < 0 || lds_bytes > sw->lds_bytes_max || lds_bytes % rsz)
+        return fail("bhc_sweep_set_launch: lds_bytes must be a multiple of %d in [0, %d]", rsz, sw->lds_bytes_max);
+    sw->scatter_blocks = scatter_blocks;
+    sw->lds_bytes = lds_bytes;
+    sw->tuned = true;
+    return 0;
 }
 
 extern "C" int bhc_sweep_set_scatter(bhc_sweep *sw, int mapping)

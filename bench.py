@@ -307,9 +307,15 @@ def run_sweep(a):
     warm.launch()
     warm.release()
     sw = bh.Sweep(probs, ctx)
-    log("uploaded; autotuning on the first 64 problems")
-    tuned = sw.autotune()
-    log("autotuned: %s" % json.dumps(tuned))
+    if a.sweep_config:
+        b, l, m = (int(v) for v in a.sweep_config.split(","))
+        sw.set_launch(b, l, m)
+        tuned = {"scatter_blocks": b, "lds_bytes": l, "scatter_lanes": m, "source": "--sweep-config"}
+        log("launch configuration given: %s" % json.dumps(tuned))
+    else:
+        log("uploaded; autotuning on the first 64 problems")
+        tuned = sw.autotune()
+        log("autotuned: %s" % json.dumps(tuned))
     # kernel-by-kernel passes, so that HIP events time the trace and the scatter
     # kernels of every pass separately (the graph-replayed passes are the same
     # kernels; each pass ends in a host synchronisation either way)
@@ -347,7 +353,7 @@ def run_sweep(a):
     fields_s = nprob_done / wall
 
     # plain per-environment GPU launches on a sample of the same problems
-    step = max(1, len(probs) // a.plain_sample)
+    step = max(1, len(probs) // max(a.plain_sample, 1))
     sample = probs[::step][:a.plain_sample]
     t_plain = 0.0
     modes = []
@@ -360,8 +366,9 @@ def run_sweep(a):
         L.bhc_ctx_synchronize(ctx._h)
         t_plain += time.perf_counter() - t1
         r.release()
-    plain_fields_s = len(sample) / t_plain
-    log("plain per-environment launches: %.2f fields/s on %d problems" % (plain_fields_s, len(sample)))
+    plain_fields_s = len(sample) / t_plain if sample else None
+    if sample:
+        log("plain per-environment launches: %.2f fields/s on %d problems" % (plain_fields_s, len(sample)))
 
     f_I, f_S, f_C = opcounts(L, _abi, probs[::max(1, len(probs) // 16)][:16], 16)
     F = f_I * R + f_S * S + f_C * Cn
@@ -393,7 +400,7 @@ def run_sweep(a):
                           "sample": "%d of the %d problems, one resident launch each with its autotuned scatter "
                                     "mapping (tiles / wavefront / lane: %d / %d / %d)" % (
                                         len(sample), len(probs), modes.count(1), modes.count(2), modes.count(3))},
-        "batched_over_plain": fields_s / plain_fields_s,
+        "batched_over_plain": fields_s / plain_fields_s if plain_fields_s else None,
         "autotuned": tuned,
         "roofline": rl,
         "errors": {"problems_with_errors_in_sample": nerr, "sampled": len(errs)},
@@ -527,7 +534,9 @@ def main():
     ap.add_argument("--sweep-scale", type=float, default=1.0)
     ap.add_argument("--sweep-chunk", type=int, default=512,
                     help="problems per step (one persistent launch each); steps cycle through the sweep")
-    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--plain-sample", type=int, default=64, help="problems timed as plain launches (0: none)")
+    ap.add_argument("--sweep-config", default="",
+                    help="scatter_blocks,lds_bytes,mapping: use this launch configuration instead of autotuning")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,

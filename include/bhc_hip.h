@@ -344,6 +344,33 @@ int bhc_last_phase_ms(bhc_ctx *ctx, double *trace_ms, double *scatter_ms, int *p
 /* Launch tuning (rays per workgroup, workgroups per CU); 0 = autotuned default. */
 int bhc_ctx_set_launch(bhc_ctx *ctx, int threads_per_block, int blocks_per_cu);
 
+/* ---- progress and non-blocking runs -------------------------------------------------
+ * Progress of the context's current or last run: rays finished so far and the
+ * run's ray count (bhcInternal::completedRayCount / totalJobs,
+ * src/common.hpp:438-440, read by get_percent_progress, include/bhc/bhc.hpp:345-347
+ * and src/api.cpp:780-790). May be called from any thread while a run is in
+ * flight; done never decreases during a run and equals total once it ends.
+ * Trace kernels report finished rays in steps of 256; ray-mode runs report at
+ * their end. */
+int bhc_progress(bhc_ctx *ctx, int64_t *done, int64_t *total);
+
+/* Non-blocking field runs (extsetup_blocking(params, false); the reference
+ * calls completedCallback when the last worker ends,
+ * src/mode/fieldimpl.cpp.in:46-52,66-71): the run is performed by a worker
+ * thread of the context and returns at once (0, or -1 if it could not start);
+ * `completed(user)` is called on the worker thread after the outputs have
+ * been written. `prob` and `out` must stay valid until then. While the run is
+ * in flight every other call that uses the context fails with an error
+ * (bhc_progress, bhc_ctx_busy and bhc_ctx_wait excepted); the callback must not
+ * start runs or wait on its own context. bhc_ctx_wait joins the worker and
+ * returns the run's status (its message by bhc_last_error). The 3D variant,
+ * bhc_field_run3d_async, is declared with the 3D entry points. */
+typedef void (*bhc_completed_fn)(void *user);
+int bhc_field_run2d_async(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_field_out *out, bhc_completed_fn completed,
+                          void *user);
+int bhc_ctx_busy(bhc_ctx *ctx);
+int bhc_ctx_wait(bhc_ctx *ctx);
+
 /* ---- batched sweeps ---------------------------------------------------------------
  * Many independent TL problems (run types C / S / I, one precision) traced in
  * one persistent launch: rays of all problems are taken from device-side work
@@ -381,6 +408,10 @@ int bhc_sweep_set_graph(bhc_sweep *sw, int use_graph);
  * lane per segment, 2 the receivers of 64 segments flattened over a wavefront
  * (bhc_sweep_autotune picks among them). */
 int bhc_sweep_set_scatter(bhc_sweep *sw, int mapping);
+/* Scatter grid (workgroups) and per-workgroup LDS staging budget of the trace
+ * (bytes, a multiple of the real size, at most the upload's budget) without a
+ * search: e.g. a configuration bhc_sweep_autotune chose before. */
+int bhc_sweep_set_launch(bhc_sweep *sw, int scatter_blocks, int lds_bytes);
 void bhc_sweep_release(bhc_sweep *sw);
 
 /* ---- 3D (BELLHOP3D geometry: x, y, depth) ------------------------------------------
@@ -500,6 +531,9 @@ const char *bhc_host3d_last_error(void);
  * ray_xend holding [rays][3] end points. Rays are numbered like GetJobIndices
  * (src/common.hpp:365-391): alpha fastest, then beta, Sy, Sx, Sz. */
 int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out);
+/* Non-blocking bhc_field_run3d, with the semantics of bhc_field_run2d_async. */
+int bhc_field_run3d_async(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out, bhc_completed_fn completed,
+                          void *user);
 /* 3D ray run type (RunRayMode, src/mode/ray.cpp:167-188, with MainRayMode
  * src/trace.hpp:482-520 in 3D): every ray of the fan in job order, up to
  * max_points_per_ray points each. */
