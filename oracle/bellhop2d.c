@@ -1406,11 +1406,44 @@ static void AddArrAt(bhc_arrival *baseArr, _Atomic int32_t *baseNArr, int max_na
     }
 }
 
+/* Checker hook: a log of the raw AddArr calls of a single-threaded run, in
+ * call order, 10 doubles each (receiver index, Amp, Phase, delay re / im,
+ * omega, source and receiver declination, top / bottom bounces), so that a
+ * test can replay the merge rules on the unrounded values. */
+static double *g_arrlog;
+static int64_t g_arrlog_cap, g_arrlog_n;
+
+int SFX(oracle_arrlog)(double *buf, int64_t cap)
+{
+    g_arrlog = buf;
+    g_arrlog_cap = cap;
+    g_arrlog_n = 0;
+    return 0;
+}
+
+int64_t SFX(oracle_arrlog_count)(void) { return g_arrlog_n; }
+
 static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, const RayInitInfo *ri,
                    real RcvrDeclAngle, int NumTopBnc, int NumBotBnc, Outs *O)
 { /* arrivals.hpp:45-130, 2D: no azimuths */
     const Prob *P = O->P;
     size_t base = ((size_t)ri->isz * P->NRz_per_range + id) * P->NRr + ir;
+    if(g_arrlog) {
+        if(g_arrlog_n < g_arrlog_cap) {
+            double *e = g_arrlog + 10 * g_arrlog_n;
+            e[0] = (double)base;
+            e[1] = (double)Amp;
+            e[2] = (double)Phase;
+            e[3] = (double)creal(delay);
+            e[4] = (double)cimag(delay);
+            e[5] = (double)omega;
+            e[6] = (double)ri->SrcDeclAngle;
+            e[7] = (double)RcvrDeclAngle;
+            e[8] = (double)NumTopBnc;
+            e[9] = (double)NumBotBnc;
+        }
+        ++g_arrlog_n;
+    }
     AddArrAt(&O->arr[base * O->max_narr], &O->narr[base], O->max_narr, O->allow_merging, Amp, omega, Phase, delay,
              ri->SrcDeclAngle, (real)NAN, RcvrDeclAngle, (real)NAN, NumTopBnc, NumBotBnc);
 }

@@ -40,6 +40,12 @@ def lib():
             f = getattr(L, "oracle_preprocess_dump2d_" + sfx)
             f.restype = C.c_int
             f.argtypes = [C.POINTER(_abi.Env2D_C), _abi.c_double_p, C.c_int64, C.POINTER(C.c_int64)]
+            f = getattr(L, "oracle_arrlog_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [_abi.c_double_p, C.c_int64]
+            f = getattr(L, "oracle_arrlog_count_" + sfx)
+            f.restype = C.c_int64
+            f.argtypes = []
         _LIB = L
     return _LIB
 
@@ -256,3 +262,21 @@ def ray_run3d_list(env, hits, nhits, max_points=4000):
          max_points) != 0:
         raise RuntimeError("oracle_ray_run3d_list failed")
     return pts, ns
+
+
+def arrivals_with_log(env, max_narr, allow_merging=1, cap=1 << 20):
+    """Single-threaded arrivals run that also returns the raw AddArr calls in
+    call order ([n, 10]: receiver, Amp, Phase, delay re, delay im, omega,
+    source and receiver declination, top and bottom bounces)."""
+    buf = np.zeros((cap, 10), np.float64)
+    L = lib()
+    sfx = _sfx(env)
+    getattr(L, "oracle_arrlog_" + sfx)(buf.ctypes.data_as(_abi.c_double_p), cap)
+    try:
+        res = field_run(env, nthreads=1, max_narr=max_narr, allow_merging=allow_merging)
+        n = getattr(L, "oracle_arrlog_count_" + sfx)()
+    finally:
+        getattr(L, "oracle_arrlog_" + sfx)(None, 0)
+    if n > cap:
+        raise RuntimeError("arrivals log overflow: %d calls" % n)
+    return res, buf[:n].copy()

@@ -17,12 +17,28 @@ What pins what (oracle function -> test):
       -> test_tx_invariant_range_independent
   EvaluateSSP c-linear / n2-linear / spline (not-a-knot) / Munk analytic
       -> test_ssp_values_along_rays
-  PCHIP: knot interpolation and monotonicity only (its derivative seeding
-  is the reference's own and no published routine reproduces it, so beyond
-  these properties it is parity unpinned)
-      -> test_pchip_properties
+  PCHIP: knot interpolation and monotonicity; exact reproduction of a
+  monotone quadratic (3-point end slopes, clamped-spline seeds, Hermite
+  coefficients); equality with SciPy's clamped cubic spline where the
+  Fritsch-Carlson bounds hold; the projection onto those bounds, zero slopes
+  at extrema and the end-slope limiter where they do not
+      -> test_pchip_properties, test_pchip_reproduces_monotone_quadratic,
+         test_pchip_equals_clamped_spline_where_monotone,
+         test_pchip_projection_where_the_spline_overshoots
+  Quad SSP: bilinear values (SciPy) and, on a sampled linear field, rays that
+  are circular arcs about the line c = 0 (fixes both gradient components)
+      -> test_quad_ssp_is_bilinear, test_quad_ssp_linear_field_rays_are_circular_arcs
   Reflect, acousto-elastic halfspace (fluid, cs = 0) -> Rayleigh coefficient
       -> test_halfspace_reflection_rayleigh
+  Reflect, elastic halfspace (cs > 0) -> Brekhovskikh's fluid-solid coefficient
+      -> test_elastic_halfspace_reflection_brekhovskikh
+  Reflect with beam shift ('S') -> Goos-Haenchen lateral shift of the Rayleigh
+  phase, its delay and width terms
+      -> test_beam_shift_is_goos_haenchen_shift
+  Curvilinear boundaries ('C'): curvature of a parabola sampled at equal
+  steps (exact for the chord scheme), node tangents / normals, the
+  interpolated reflecting normal
+      -> test_curvilinear_bottom_parabola
   Reflect, tabulated coefficient ('F') -> linear interpolation in grazing angle
       -> test_reflection_table_interpolation
   Influence geometric hat / Gaussian, Cartesian and ray-centred, coherent sum
@@ -32,11 +48,16 @@ What pins what (oracle function -> test):
   Incoherent / semicoherent sums -> test_incoherent_and_semicoherent_closed_form
   Arrivals (AddArr) delay and amplitude in free space
       -> test_free_space_arrivals
+  AddArr merging (pair test on the last slot, weakest-slot replacement,
+  drops, float averaging) -> a replay of the rules on the oracle's raw calls
+      -> test_arrivals_merge_replays_addarr_rules
   Eigen hits (RecordEigenHit) -> test_eigen_hits_bracket_receivers
-Parity unpinned (no closed form; held only by faithfulness to the reference
-source text): the PCHIP derivative seeding, curvilinear boundary interpolation
-('C' boundaries), the quad SSP, the attenuation-unit conversions other than
-dB/wavelength and the AddArr merge tie-breaks.
+  crci, every attenuation unit (N, M, m, F, W, Q, L), Thorp, biological layers
+      -> test_volume_attenuation_closed_form, test_biological_attenuation_closed_form
+Parity unpinned (held only by faithfulness to the reference source text):
+exact-amplitude ties in the AddArr weakest-slot search (they do not arise in
+the replayed runs), and Francois-Garrison volume attenuation beyond its use
+in the product-versus-oracle comparisons.
 """
 import math
 
@@ -45,7 +66,7 @@ import pytest
 from scipy.interpolate import CubicSpline
 
 import oracle
-from bellhopcuda_amd.env import Env2D, Halfspace, ReflTable, subtab, munk_c, munk_env
+from bellhopcuda_amd.env import Env2D, Halfspace, ReflTable, Boundary, subtab, munk_c, munk_env
 
 
 def iso_env(D=1000.0, c=1500.0, zs=300.0, alpha=(-30.0, 30.0), nb=61, top="V", bot="R", rmax_km=5.0,
@@ -236,6 +257,172 @@ def test_pchip_properties():
     assert seen > 100
 
 
+def _pchip_coefs(z, c, D=None):
+    """PCHIP polynomial coefficients c0..c3 per knot (c0 + c1 s + c2 s^2 +
+    c3 s^3, s = z - z_k) from the oracle's preprocessing."""
+    env = _ssp_env("P", np.asarray(z, float), np.asarray(c, float), D or float(z[-1]))
+    d = oracle.preprocess_dump(env)
+    n = len(z)
+    coef = d[4 * n:12 * n].reshape(4, n, 2)
+    assert np.all(coef[..., 1] == 0.0)          # no attenuation: real coefficients
+    return coef[..., 0]
+
+
+def test_pchip_reproduces_monotone_quadratic():
+    """src/curves.hpp:299-372: the end derivatives are 3-point differences,
+    exact for a quadratic; the interior seeds come from the cubic spline
+    clamped to them, which reproduces a quadratic exactly; the monotone
+    projection leaves them alone (they lie between the adjacent secants). So
+    on a monotone quadratic every knot holds the quadratic's own Taylor
+    coefficients: value, slope, half the curvature, and no cubic term."""
+    z = np.array([0.0, 150.0, 400.0, 700.0, 1200.0, 1500.0, 2300.0, 3000.0])
+    b, q = 0.016, 3.0e-6
+    c = 1490.0 + b * z + q * z ** 2
+    k = _pchip_coefs(z, c)
+    scale = np.array([1500.0, b, q, q / 3000.0])
+    for j in range(len(z) - 1):
+        exp = np.array([c[j], b + 2 * q * z[j], q, 0.0])
+        assert np.all(np.abs(k[:, j] - exp) <= 1e-9 * scale), (j, k[:, j], exp)
+
+
+def test_pchip_equals_clamped_spline_where_monotone():
+    """Smooth strictly monotone data: the spline's knot slopes already satisfy
+    the Fritsch-Carlson bounds 0 <= d <= 3 min(secants), so the PCHIP is the
+    cubic spline clamped to the 3-point end slopes (SciPy's CubicSpline with
+    first-derivative end conditions): slopes and the interval polynomials
+    agree."""
+    from scipy.interpolate import CubicSpline
+    z = np.array([0.0, 120.0, 300.0, 650.0, 900.0, 1400.0, 1800.0, 2500.0, 3000.0])
+    c = 1480.0 + 35.0 * np.tanh((z - 1300.0) / 900.0)
+    h1, h2 = z[1] - z[0], z[2] - z[1]
+    d1, d2 = (c[1] - c[0]) / h1, (c[2] - c[1]) / h2
+    d0 = ((2 * h1 + h2) * d1 - h1 * d2) / (h1 + h2)
+    h1, h2 = z[-2] - z[-3], z[-1] - z[-2]
+    d1, d2 = (c[-2] - c[-3]) / h1, (c[-1] - c[-2]) / h2
+    dn = (-h2 * d1 + (h1 + 2 * h2) * d2) / (h1 + h2)
+    sp = CubicSpline(z, c, bc_type=((1, d0), (1, dn)))
+    sec = np.diff(c) / np.diff(z)
+    slopes = sp(z, 1)
+    assert np.all(slopes[1:-1] >= 0) and np.all(slopes[1:-1] <= 3 * np.minimum(sec[:-1], sec[1:]))
+    k = _pchip_coefs(z, c)
+    for j in range(len(z) - 1):
+        exp = np.array([c[j], slopes[j], sp(z[j], 2) / 2.0, sp(z[j], 3) / 6.0])
+        # sp(z, 3) at a knot is the right-hand interval's third derivative
+        assert np.allclose(k[:, j], exp, rtol=1e-9, atol=1e-12), (j, k[:, j], exp)
+
+
+def test_pchip_projection_where_the_spline_overshoots():
+    """Where adjacent secants change sign the knot slope is 0 (an extremum of
+    the interpolant); where the spline's slope leaves [0, 3 min(secants)]
+    (Fritsch-Carlson) it is projected onto that interval; an end slope whose
+    sign differs from the end secant's is 0. Checked against SciPy's clamped
+    spline and the projection, knot by knot, on data with a sharp step."""
+    from scipy.interpolate import CubicSpline
+    z = np.array([0.0, 100.0, 200.0, 300.0, 400.0, 500.0, 600.0, 700.0, 800.0])
+    c = np.array([1500.0, 1499.0, 1499.5, 1520.0, 1521.0, 1521.2, 1515.0, 1516.0, 1530.0])
+    sec = np.diff(c) / np.diff(z)
+
+    def end(d1, d2, f):
+        if d1 * f <= 0:
+            return 0.0
+        if d1 * d2 <= 0 and abs(f) > abs(3 * d1):
+            return 3 * d1
+        return f
+    h1, h2 = z[1] - z[0], z[2] - z[1]
+    d0 = end(sec[0], sec[1], ((2 * h1 + h2) * sec[0] - h1 * sec[1]) / (h1 + h2))
+    h1, h2 = z[-2] - z[-3], z[-1] - z[-2]
+    dn = end(sec[-1], sec[-2], (-h2 * sec[-2] + (h1 + 2 * h2) * sec[-1]) / (h1 + h2))
+    sp = CubicSpline(z, c, bc_type=((1, d0), (1, dn)))
+    exp = np.zeros(len(z))
+    exp[0], exp[-1] = d0, dn
+    projected = extrema = 0
+    for j in range(1, len(z) - 1):
+        a, b = sec[j - 1], sec[j]
+        f = sp(z[j], 1)
+        if a * b > 0:
+            lo, hi = (0.0, 3 * min(a, b)) if a > 0 else (3 * max(a, b), 0.0)
+            exp[j] = min(max(f, lo), hi)
+            projected += exp[j] != f
+        else:
+            exp[j] = 0.0
+            extrema += 1
+    assert projected >= 1 and extrema >= 2
+    k = _pchip_coefs(z, c)
+    assert np.allclose(k[1], exp, rtol=1e-9, atol=1e-12), (k[1], exp)
+    # the Hermite cubic of each interval from the knot values and slopes
+    h = np.diff(z)
+    f1, f2, p1, p2 = c[:-1], c[1:], exp[:-1], exp[1:]
+    c2 = (3 * (f2 - f1) - h * (2 * p1 + p2)) / h ** 2
+    c3 = (h * (p1 + p2) - 2 * (f2 - f1)) / h ** 3
+    assert np.allclose(k[2, :-1], c2, rtol=1e-9, atol=1e-12)
+    assert np.allclose(k[3, :-1], c3, rtol=1e-9, atol=1e-15)
+
+
+def _quad_env(z, r_km, cmat, D, alpha=(-14.0, 14.0), nb=15, zs=900.0, deltas=97.0, rmax_km=30.0):
+    e = _ssp_env("Q", np.asarray(z, float), np.asarray(cmat, float)[:, 0].copy(), D)
+    e.ssp_r_km = np.asarray(r_km, float)
+    e.ssp_cmat = np.asarray(cmat, float)
+    e.Sz = np.array([zs], np.float32)
+    e.alpha_deg = subtab(alpha[0], alpha[1], nb)
+    e.deltas = deltas
+    e.Rr_km = np.array([0.0, rmax_km])
+    e.box_r_km = rmax_km
+    return e
+
+
+def test_quad_ssp_is_bilinear():
+    """Quad SSP (src/ssp.hpp:157-214): inside each (range, depth) cell the
+    sound speed is the bilinear interpolation of the four corner values."""
+    from scipy.interpolate import RegularGridInterpolator
+    D = 3000.0
+    z = np.array([0.0, 300.0, 800.0, 1500.0, 2200.0, 3000.0])
+    r_km = np.array([-1.0, 4.0, 9.0, 17.0, 31.0])
+    rng = np.random.default_rng(7)
+    cmat = 1490.0 + 0.012 * z[:, None] + rng.uniform(-8.0, 8.0, (len(z), len(r_km)))
+    env = _quad_env(z, r_km, cmat, D)
+    f = RegularGridInterpolator((z, r_km * 1000.0), cmat, bounds_error=False, fill_value=None)
+    n = 0
+    for r in _rays(env, max_points=20000):
+        ref = f(np.stack([r["x"][:, 1], r["x"][:, 0]], axis=1))
+        assert np.max(np.abs(r["c"] - ref) / ref) < 1e-12
+        n += len(r)
+    assert n > 500
+
+
+def test_quad_ssp_linear_field_rays_are_circular_arcs():
+    """A quad SSP sampling c = c0 + a r + b z (a linear field, exact on the
+    grid): rays are circles of radius 1 / (|g| xi), xi = t . m the ray
+    parameter along the unit vector m normal to the gradient g = (a, b), with
+    centres on the line c = 0. This fixes the gradient the quad evaluation
+    returns (both components and signs), not only its values."""
+    D = 3000.0
+    c0, a, b = 1500.0, 0.002, 0.03
+    z = np.linspace(0.0, D, 7)
+    r_km = np.linspace(-1.0, 21.0, 12)
+    cmat = c0 + a * r_km[None, :] * 1000.0 + b * z[:, None]
+    env = _quad_env(z, r_km, cmat, D, alpha=(-12.0, 12.0), nb=9, zs=1200.0, deltas=20.0, rmax_km=20.0)
+    g = np.array([a, b])
+    gn = np.linalg.norm(g)
+    m = np.array([g[1], -g[0]]) / gn
+    checked = 0
+    for r in _rays(env, max_points=20000):
+        nb = r["NumTopBnc"] + r["NumBotBnc"]
+        seg = r[nb == 0]
+        t, x = seg["t"], seg["x"]
+        xi = t[:, 0] * m[0] + t[:, 1] * m[1]
+        assert np.max(np.abs(xi - xi[0])) <= 1e-12 * abs(xi[0])
+        R = 1.0 / (gn * abs(xi[0]))
+        c_at = c0 + g @ x[0]
+        that = t[0] * c_at
+        kap = -(g - (g @ that) * that) / c_at          # curvature vector
+        centre = x[0] + kap / np.dot(kap, kap)
+        assert abs(c0 + g @ centre) < 1e-6 * c0         # on the line c = 0
+        dist = np.hypot(x[:, 0] - centre[0], x[:, 1] - centre[1])
+        assert np.max(np.abs(dist - R)) / R < 1e-7
+        checked += len(seg)
+    assert checked > 200
+
+
 # --------------------------------------------------------------------------
 # reflection
 # --------------------------------------------------------------------------
@@ -267,6 +454,88 @@ def test_halfspace_reflection_rayleigh():
         assert after["Amp"] / before["Amp"] == pytest.approx(abs(R), rel=1e-10)
         assert after["Phase"] - before["Phase"] == pytest.approx(np.angle(R), abs=1e-10)
     assert 0 < n_total < len(env.alpha_deg)   # both total and partial reflection covered
+
+
+def test_elastic_halfspace_reflection_brekhovskikh():
+    """Acousto-elastic halfspace with shear (src/reflect.hpp:316-350): for
+    angles where both the compressional and the shear wave propagate in the
+    bottom, the plane-wave coefficient of a fluid over an elastic solid
+    (Brekhovskikh) is
+        R = (Zp cos^2 2ts + Zs sin^2 2ts - Z1) / (Zp cos^2 2ts + Zs sin^2 2ts + Z1),
+    Z1 = rho1 c1 / cos t1, Zp = rho2 cp / cos tp, Zs = rho2 cs / cos ts, angles
+    from the normal and linked by Snell's law."""
+    D, c1, cp, cs, rho2, f = 1000.0, 1500.0, 1800.0, 600.0, 1.9, 100.0
+    env = iso_env(D=D, c=c1, zs=500.0, alpha=(60.0, 85.0), nb=11, rmax_km=10.0, deltas=50.0, freq=f,
+                  bot_hs=Halfspace("A", depth=D, alphaR=cp, betaR=cs, rho=rho2, zTemp=D))
+    for a, r in zip(env.alpha_deg, _rays(env)):
+        before, after = _first_bottom_bounce(r)
+        t1 = math.radians(90.0 - a)                      # from the normal
+        stp, sts = cp / c1 * math.sin(t1), cs / c1 * math.sin(t1)
+        assert stp < 1.0 and sts < 1.0
+        tp, ts = math.asin(stp), math.asin(sts)
+        Z1, Zp, Zs = c1 / math.cos(t1), rho2 * cp / math.cos(tp), rho2 * cs / math.cos(ts)
+        num = Zp * math.cos(2 * ts) ** 2 + Zs * math.sin(2 * ts) ** 2
+        R = (num - Z1) / (num + Z1)
+        assert after["Amp"] / before["Amp"] == pytest.approx(abs(R), rel=1e-9)
+        dphi = (after["Phase"] - before["Phase"]) % (2 * math.pi)
+        assert min(dphi, 2 * math.pi - dphi) == pytest.approx(0.0 if R > 0 else math.pi, abs=1e-9)
+
+
+def test_curvilinear_bottom_parabola():
+    """Curvilinear bathymetry ('C', src/module/boundary.hpp:600-890 and
+    src/trace.hpp:350-362) on a parabola z = z0 + q (r - rc)^2 sampled at equal
+    range steps. A parabola's chord slopes are exact, so the reference's
+    segment curvature (difference of chord slopes over the chord width, times
+    the chord's tangent x-component cubed) equals the graph curvature form
+    z'' cos^3 = 2 q t_x^3 exactly; node tangents are the mean of the adjacent
+    unit chord tangents and node normals their rotation; at a bounce the
+    reflecting normal is the node normals interpolated linearly in the
+    along-chord position (not renormalised), so the reflected tangent is
+    t - 2 (t.n) n with that n. All expectations are computed from the raw
+    points."""
+    D, c = 2500.0, 1500.0
+    rr = np.linspace(0.0, 30.0, 31)
+    q, rc = 400.0 / 15000.0 ** 2, 15000.0
+    zb = 1500.0 + q * (rr * 1000.0 - rc) ** 2
+    env = _ssp_env("C", np.array([0.0, D]), np.array([c, c]), D)
+    env.bty = Boundary("CS", r_km=rr, z=zb)
+    env.Sz = np.array([500.0], np.float32)
+    env.alpha_deg = subtab(5.0, 40.0, 15)
+    env.deltas = 50.0
+    env.Rr_km = np.array([0.0, 29.0])
+    env.box_r_km = 29.0
+    d = oracle.preprocess_dump(env)
+    N = 2
+    off = 17 * N + len(env.Rr_km) + len(env.alpha_deg)
+    nodes = d[off:off + 13 * (len(rr) + 2)].reshape(-1, 13)[1:-1]     # drop the two extension nodes
+    x = np.stack([rr * 1000.0, zb], axis=1)
+    assert np.array_equal(nodes[:, 0:2], x)
+    ch = np.diff(x, axis=0)
+    L = np.hypot(ch[:, 0], ch[:, 1])
+    t = ch / L[:, None]
+    # chords 0 .. n-3 have a parabola chord on both sides
+    kap = nodes[:-2, 11]
+    assert np.allclose(kap, 2 * q * t[:-1, 0] ** 3, rtol=1e-9, atol=0)
+    nodet = 0.5 * (t[:-1] + t[1:])
+    assert np.allclose(nodes[1:-1, 8:10], nodet, rtol=0, atol=1e-15)
+    noden = np.stack([-nodet[:, 1], nodet[:, 0]], axis=1)
+    assert np.allclose(nodes[1:-1, 6:8], noden, rtol=0, atol=1e-15)
+    bounces = 0
+    for r in _rays(env, max_points=20000):
+        k = np.nonzero(np.diff(r["NumBotBnc"]) == 1)[0]
+        for j in k:
+            a, b = r[j], r[j + 1]
+            xr = a["x"]
+            i = int(np.searchsorted(x[:, 0], xr[0]) - 1)
+            if i < 1 or i > len(rr) - 3:
+                continue                      # next to an end node: tangent (1, 0) by definition
+            sss = np.dot(xr - x[i], t[i]) / L[i]
+            n = (1 - sss) * noden[i - 1] + sss * noden[i]
+            tin = a["t"]
+            exp = tin - 2.0 * np.dot(tin, n) * n
+            assert np.allclose(b["t"], exp, rtol=1e-11, atol=1e-18), (b["t"], exp)
+            bounces += 1
+    assert bounces >= 10
 
 
 def test_reflection_table_interpolation():
@@ -556,3 +825,85 @@ def test_beam_shift_is_goos_haenchen_shift():
             assert dx == 0.0 and dtau == 0.0
             assert np.isnan(dq)
     assert below >= 5 and above >= 5
+
+
+def _replay_addarr(log, nrcv, max_narr):
+    """AddArr with merging (src/arrivals.hpp:31-130) replayed on the raw calls
+    in call order, with the reference's roundings: slots hold floats, the
+    pair test and the weakest-arrival search compare the unrounded call
+    values with the stored floats, a merge averages in float with the
+    amplitude weights. Only the last slot is tested for a pair; the weakest
+    slot is the first one strictly below the new amplitude (ties keep the
+    earlier slot); a new arrival weaker than every stored one is dropped."""
+    f32 = np.float32
+    A = np.zeros((nrcv, max_narr), f32)
+    Ph, Dr, Di, Sd, Rd = (np.zeros((nrcv, max_narr), f32) for _ in range(5))
+    Nt_, Nb_ = np.zeros((nrcv, max_narr), np.int32), np.zeros((nrcv, max_narr), np.int32)
+    narr = np.zeros(nrcv, np.int32)
+    tol = float(f32(0.05))
+    st = {"new": 0, "merged": 0, "replaced": 0, "dropped": 0, "ties": 0}
+    for row in log:
+        k = int(row[0])
+        amp, phase, dre, dim, om, sdec, rdec = (float(v) for v in row[1:8])
+        ntop, nbot = int(row[8]), int(row[9])
+        n = int(narr[k])
+        second = (n >= 1 and om * abs(complex(dre, dim) - complex(float(Dr[k, n - 1]), float(Di[k, n - 1]))) < tol
+                  and abs(float(Ph[k, n - 1]) - phase) < tol)
+        if not second:
+            if n >= max_narr:
+                i, weakest = -1, amp
+                for j in range(max_narr):
+                    if float(A[k, j]) < weakest:
+                        weakest, i = float(A[k, j]), j
+                    elif float(A[k, j]) == weakest:
+                        st["ties"] += 1          # the earlier slot (or the new arrival's drop) wins
+                if i < 0:
+                    st["dropped"] += 1
+                    continue
+                st["replaced"] += 1
+            else:
+                i = n
+                narr[k] = n + 1
+                st["new"] += 1
+            A[k, i], Ph[k, i], Dr[k, i], Di[k, i] = f32(amp), f32(phase), f32(dre), f32(dim)
+            Sd[k, i], Rd[k, i], Nt_[k, i], Nb_[k, i] = f32(sdec), f32(rdec), ntop, nbot
+        else:
+            st["merged"] += 1
+            i = n - 1
+            tot = f32(A[k, i] + f32(amp))
+            w1, w2 = f32(A[k, i] / tot), f32(f32(amp) / tot)
+            Dr[k, i] = f32(f32(w1 * Dr[k, i]) + f32(w2 * f32(dre)))
+            Di[k, i] = f32(f32(w1 * Di[k, i]) + f32(w2 * f32(dim)))
+            A[k, i] = tot
+            Sd[k, i] = f32(f32(w1 * Sd[k, i]) + f32(w2 * f32(sdec)))
+            Rd[k, i] = f32(f32(w1 * Rd[k, i]) + f32(w2 * f32(rdec)))
+    return narr, A, Ph, Dr, Di, Sd, Rd, Nt_, Nb_, st
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("max_narr", [2, 3, 6])
+def test_arrivals_merge_replays_addarr_rules(max_narr, precision):
+    """The oracle's arrivals merge equals a replay of the reference's AddArr
+    rules on its own raw calls, bit for bit, with few enough slots that pairs
+    merge, weaker arrivals are dropped and stored ones are replaced. (Exact
+    amplitude ties between distinct arrivals do not arise here: mirror rays of
+    a waveguide symmetric about the source share delay and phase, so the pair
+    rule merges them first.)"""
+    D = 1000.0
+    env = iso_env(D=D, zs=500.0, alpha=(-40.0, 40.0), nb=81, top="R", bot="R", rmax_km=6.0, deltas=97.0,
+                  rz=np.array([250.0, 500.0, 750.0], np.float32), rr_km=np.array([1.0, 2.5, 4.0, 5.5]),
+                  run_type="AG", precision=precision)
+    res, log = oracle.arrivals_with_log(env, max_narr)
+    nrcv = 12
+    narr, A, Ph, Dr, Di, Sd, Rd, Ntop, Nbot, st = _replay_addarr(log, nrcv, max_narr)
+    print(st)
+    assert st["merged"] > 0 and st["replaced"] > 0 and st["dropped"] > 0
+    arr = np.ctypeslib.as_array(res.arr).reshape(nrcv, max_narr)
+    assert np.array_equal(res.narr.reshape(-1), narr)
+    for k in range(nrcv):
+        n = narr[k]
+        got = arr[k, :n]
+        assert np.array_equal(got["a"], A[k, :n]) and np.array_equal(got["Phase"], Ph[k, :n])
+        assert np.array_equal(got["delay"][:, 0], Dr[k, :n]) and np.array_equal(got["delay"][:, 1], Di[k, :n])
+        assert np.array_equal(got["SrcDeclAngle"], Sd[k, :n]) and np.array_equal(got["RcvrDeclAngle"], Rd[k, :n])
+        assert np.array_equal(got["NTopBnc"], Ntop[k, :n]) and np.array_equal(got["NBotBnc"], Nbot[k, :n])
