@@ -60,6 +60,7 @@ def lib():
                                          _abi.c_int32_p, _abi.c_int32_p, C.c_int32]),
         "bhc_problem_upload2d": (C.c_int, [C.c_void_p, P(_abi.Problem2D), P(_abi.FieldOut), P(C.c_void_p)]),
         "bhc_field_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "bhc_field_launch_noalloc": (C.c_int, [C.c_void_p, C.c_void_p]),
         "bhc_field_download": (C.c_int, [C.c_void_p, C.c_void_p, P(_abi.FieldOut)]),
         "bhc_problem_release": (None, [C.c_void_p]),
         "bhc_problem_autotune": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int)]),
@@ -330,6 +331,7 @@ class Resident:
         _check(lib().bhc_field_download(self.ctx._h, self._h, C.byref(self.out)))
         r = self.res
         r.err, r.warn = self.out.err, self.out.warn
+        r.err_count, r.warn_count = self.out.err_count, self.out.warn_count
         r.total_steps, r.total_contrib = self.out.total_steps, self.out.total_contrib
         return r
 

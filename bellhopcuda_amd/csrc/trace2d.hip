@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <atomic>
+#include <mutex>
 #include <thread>
 
 #include "../../include/bhc_hip.h"
@@ -879,6 +880,8 @@ struct bhc_ctx {
     std::atomic<int> busy{0};
     int async_rc = 0;
     std::string async_msg;
+    // device address of the static scratch array on this context's device
+    void *scratch = nullptr;
 };
 
 namespace {
@@ -953,6 +956,35 @@ extern "C" int bhc_device_info(int device, char *name, int name_len, int *num_cu
     return 0;
 }
 
+// Segment buffer of bhc_field_launch_noalloc: a file-scope device array, so
+// that launch allocates nothing. Every GPU the library runs on holds its own
+// copy until the process ends. In host code the array's name is not a device
+// address: host code uses only the address hipGetSymbolAddress returns for
+// the current device (kept per device, and in each context).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES];
+
+// Address of `scratch` on `device` (which must be current), looked up once.
+int scratch_address(int device, void **ptr)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if(device < 0 || device >= MAX_DEVICES) return fail("scratch: device %d out of range", device);
+    if(!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !p) return fail("scratch: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return 0;
+}
+} // namespace
+
 extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
 {
     *out = nullptr;
@@ -987,6 +1019,7 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     HIPCHECK(hipMalloc((void **)&c->d_prog, sizeof(unsigned long long)));
     HIPCHECK(hipHostMalloc((void **)&c->h_prog, 64, hipHostMallocMapped | hipHostMallocCoherent));
     *c->h_prog = 0ull;
+    if(scratch_address(device, &c->scratch)) return -1;
     *out = c;
     return 0;
 }
@@ -1338,15 +1371,25 @@ int ensure_segs(bhc_ctx *c)
     return 0;
 }
 
-template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D<real> &P, DevOut O)
+// Trace / scatter passes over the problem's rays. segbuf == null: the
+// context's segment buffer (allocated at first use); otherwise the given
+// buffer of segbytes, with the scatter restricted to the mappings that need
+// no range bins (the bins are allocated with the context's buffer).
+template<typename real, int RT>
+int run_passes(bhc_dev_problem *dp, const Prob2D<real> &P, DevOut O, void *segbuf = nullptr, size_t segbytes = 0)
 {
     bhc_ctx *c = dp->ctx;
     hipStream_t s = c->stream;
-    if(ensure_segs(c)) return -1;
-    O.segs = c->segs;
+    const bool own_buffer = segbuf != nullptr;
+    if(!own_buffer) {
+        if(ensure_segs(c)) return -1;
+        segbuf = c->segs;
+        segbytes = c->seg_bytes;
+    }
+    O.segs = segbuf;
     O.seg_count = c->d_seg_count;
     O.overflow = c->d_overflow;
-    O.seg_cap = (uint32_t)std::min<size_t>(c->seg_bytes / sizeof(Seg<real>), 0xFFFFFFF0u);
+    O.seg_cap = (uint32_t)std::min<size_t>(segbytes / sizeof(Seg<real>), 0xFFFFFFF0u);
     std::vector<std::pair<int, int>> todo{{0, dp->njobs}};
     c->passes = 0;
     c->ph_trace_ms = c->ph_scatter_ms = 0.0;
@@ -1410,7 +1453,7 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
         HIPCHECK(hipEventRecord(c->pa, s));
         int mode = dp->scatter_mode;
         if(mode == SCATTER_AUTO) mode = (RT == RT_TL && c->use_tiles && dp->nbins > 0) ? SCATTER_TILED : SCATTER_WAVE;
-        if(mode == SCATTER_TILED && !(RT == RT_TL && dp->nbins > 0)) mode = SCATTER_WAVE;
+        if(mode == SCATTER_TILED && (!(RT == RT_TL && dp->nbins > 0) || own_buffer)) mode = SCATTER_WAVE;
         if(mode == SCATTER_LANES) {
             int sblocks = c->num_cus * 8;
             if(RT == RT_TL && !c->exact_phasor)
@@ -1463,10 +1506,9 @@ template<typename real, int RT> int run_passes(bhc_dev_problem *dp, const Prob2D
 
 extern "C" int bhc_problem_autotune(bhc_ctx *ctx, bhc_dev_problem *dp, int *mode_out);
 
-extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
+namespace {
+int field_launch(bhc_ctx *ctx, bhc_dev_problem *dp, void *segbuf, size_t segbytes)
 {
-    if(check_idle(ctx, "bhc_field_launch")) return -1;
-    if(!dp || dp->ctx != ctx) return fail("bhc_field_launch: problem not uploaded on this context");
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     if(prog_begin(ctx, dp->njobs)) return -1;
@@ -1481,13 +1523,13 @@ extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
     int rt = (r0 == 'C' || r0 == 'S' || r0 == 'I') ? RT_TL : (r0 == 'E') ? RT_EIG : RT_ARR;
     int rc;
     if(dp->precision == 64) {
-        rc = rt == RT_TL ? run_passes<double, RT_TL>(dp, dp->P64, O)
-            : rt == RT_ARR ? run_passes<double, RT_ARR>(dp, dp->P64, O)
-                           : run_passes<double, RT_EIG>(dp, dp->P64, O);
+        rc = rt == RT_TL ? run_passes<double, RT_TL>(dp, dp->P64, O, segbuf, segbytes)
+            : rt == RT_ARR ? run_passes<double, RT_ARR>(dp, dp->P64, O, segbuf, segbytes)
+                           : run_passes<double, RT_EIG>(dp, dp->P64, O, segbuf, segbytes);
     } else {
-        rc = rt == RT_TL ? run_passes<float, RT_TL>(dp, dp->P32, O)
-            : rt == RT_ARR ? run_passes<float, RT_ARR>(dp, dp->P32, O)
-                           : run_passes<float, RT_EIG>(dp, dp->P32, O);
+        rc = rt == RT_TL ? run_passes<float, RT_TL>(dp, dp->P32, O, segbuf, segbytes)
+            : rt == RT_ARR ? run_passes<float, RT_ARR>(dp, dp->P32, O, segbuf, segbytes)
+                           : run_passes<float, RT_EIG>(dp, dp->P32, O, segbuf, segbytes);
     }
     if(rc) return rc;
     if(rt == RT_ARR) {
@@ -1501,6 +1543,30 @@ extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
     }
     ctx->prog_final = 1;   // every trace pass has completed: all rays have finished
     return 0;
+}
+} // namespace
+
+extern "C" int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp)
+{
+    if(check_idle(ctx, "bhc_field_launch")) return -1;
+    if(!dp || dp->ctx != ctx) return fail("bhc_field_launch: problem not uploaded on this context");
+    return field_launch(ctx, dp, nullptr, 0);
+}
+
+extern "C" int bhc_field_launch_noalloc(bhc_ctx *ctx, bhc_dev_problem *dp)
+{
+    if(check_idle(ctx, "bhc_field_launch_noalloc")) return -1;
+    if(!dp || dp->ctx != ctx) return fail("bhc_field_launch_noalloc: problem not uploaded on this context");
+    if(dp->run0 != 'C' && dp->run0 != 'S' && dp->run0 != 'I')
+        return fail("bhc_field_launch_noalloc: TL run types (C, S, I) only, not '%c'", dp->run0);
+    if(!ctx->scratch) return fail("bhc_field_launch_noalloc: the scratch array is not mapped on device %d", ctx->device);
+    // a pass must hold one ray's segments (at most MAXN, plus a partly used
+    // slot block at either end); more rays are split over further passes
+    const size_t need = ((size_t)MAXN + 2 * (size_t)SEG_BLOCK) * seg_size(dp->precision);
+    if(need > NOALLOC_SCRATCH_BYTES)
+        return fail("bhc_field_launch_noalloc: one pass needs %zu bytes of scratch, the static scratch holds %zu",
+                    need, NOALLOC_SCRATCH_BYTES);
+    return field_launch(ctx, dp, ctx->scratch, NOALLOC_SCRATCH_BYTES);
 }
 
 extern "C" int bhc_problem_set_scatter(bhc_dev_problem *dp, int mode)

@@ -320,6 +320,17 @@ typedef struct bhc_dev_problem bhc_dev_problem;
 int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, const bhc_field_out *spec,
                          bhc_dev_problem **out);
 int bhc_field_launch(bhc_ctx *ctx, bhc_dev_problem *dp);   /* zero field + trace, async */
+/* bhc_field_launch without any allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory during a run (serving
+ * loops, code under HIP graph capture, harnesses that forbid hipMalloc). The
+ * trace -> scatter segment buffer is a static 64 MiB device array instead of
+ * the context's (allocated at its first use); rays are split over as many
+ * passes as that takes, and the scatter uses the wavefront / lane mappings
+ * (no range bins). TL run types (C, S, I) only; the problem must have been
+ * uploaded (bhc_problem_upload2d). Rejected with an error if one pass could
+ * not hold a ray's segments. Calls on one GPU share its array: calls that
+ * overlap on one GPU (from several contexts or threads) race. */
+int bhc_field_launch_noalloc(bhc_ctx *ctx, bhc_dev_problem *dp);
 int bhc_field_download(bhc_ctx *ctx, bhc_dev_problem *dp, bhc_field_out *out);
 void bhc_problem_release(bhc_dev_problem *dp);
 /* Scatter-mapping search for a resident problem (range-binned LDS tiles, one
