@@ -89,6 +89,21 @@ def opcounts(L, _abi, probs, nsample):
     return [v / len(probs) for v in acc]
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured by the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel
+    calibration), or None if it holds no figure for that kernel."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_sweep4096_f64.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch "):
+            name, value = line[len("traffic_bytes_per_launch "):].rsplit(" ", 1)
+            if name == kernel:
+                return float(value)
+    return None
+
+
 def kernel_roofline(precision, kernels, F_path, path_ms):
     """Roofline object for the dominant kernel (most time) of a launch.
 
@@ -103,11 +118,20 @@ def kernel_roofline(precision, kernels, F_path, path_ms):
         t = k["flop"] / (ms * 1e-3) / 1e12
         out.append(dict(k, ms_per_launch=ms / max(k["launches"], 1), flop_per_launch=k["flop"] / max(k["launches"], 1),
                         achieved=t, frac=t / peak))
+    for k in out:
+        t = pmc_traffic(k["name"])
+        if t is not None:
+            k["traffic_bytes_per_launch"] = t
+            k["hbm_tb_per_s"] = t / (k["ms_per_launch"] * 1e-3) / 1e12
     dom = max(out, key=lambda k: k["ms"])
     path = F_path / (path_ms * 1e-3) / 1e12
+    traffic = dom.get("traffic_bytes_per_launch")
     return {"bound": "valu", "pipe": "FP%d vector ALU (the tracer has no matrix work)" % precision,
             "kernel": dom["name"], "achieved": dom["achieved"], "peak": peak, "unit": "TFLOP/s",
-            "frac": dom["frac"], "traffic": None, "launches": dom["launches"],
+            "frac": dom["frac"], "traffic": traffic,
+            "traffic_source": "profiles/pmc_traffic_sweep4096_f64.md (rocprofv3 FETCH_SIZE / WRITE_SIZE x copy-kernel "
+                              "calibration)" if traffic is not None else None,
+            "launches": dom["launches"],
             "ms_per_launch": dom["ms_per_launch"], "flop_per_launch": dom["flop_per_launch"],
             "kernels": out,
             "path": {"kernels": "trace + scatter", "ms": path_ms, "flop": F_path, "achieved": path,
