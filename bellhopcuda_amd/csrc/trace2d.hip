@@ -1799,15 +1799,21 @@ __device__ inline void flush_problem_err(Err &e, uint32_t *perr)
     e = Err{0, 0, 0, 0};
 }
 
+// How a batched trace hands each step to the receivers: as a descriptor in
+// the segment buffer for a later scatter pass, or fused (each lane evaluates
+// its own step's receivers at once, with the fast or the reference-arithmetic
+// phasor), which needs no segment buffer and no second kernel.
+enum { TRACE_DESC = 0, TRACE_FUSED_FAST = 1, TRACE_FUSED_EXACT = 2 };
+
 // One wavefront on one problem until the problem has no unclaimed rays and
 // the wavefront's rays have ended (returns 0), or until the segment buffer
 // nears its end and the rays are saved (returns 1). `resume` restores a saved
 // wavefront. The SSP type is a template parameter, as in the single-problem
 // kernel, so that each type runs specialised code.
-template<typename real, char ST>
+template<typename real, char ST, int MODE>
 __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref, const DevOut &Oref, int pid,
                                                      const RaySave<real> *resume, SegBlock &blk, unsigned char *lds,
-                                                     unsigned long long &all_steps)
+                                                     unsigned long long &all_steps, unsigned long long &all_contrib)
 {
     // copies, for the same reason as the problem's below: through the
     // references their fields would be reloaded after every descriptor store
@@ -1827,7 +1833,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
     Trace<real> S;
     Infl<real> I;
     int job = 0, isz = 0, ia = 0, is = 0;
-    unsigned long long steps = 0;
+    unsigned long long steps = 0, contrib = 0;
     if(resume) {
         const RaySave<real> &sv = resume[lane];
         p0 = sv.p0;
@@ -1909,7 +1915,13 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
             d.prob = pid;
             ++is;
         }
-        append_seg(d, lane, O, blk);
+        if constexpr(MODE == TRACE_DESC) {
+            append_seg(d, lane, O, blk);
+        } else {
+            DevSink<real> sink{&O, &P, 0};
+            seg_items<real, RT_TL, DevSink<real>, MODE == TRACE_FUSED_FAST>(d, P, G, P.Rz, sink);
+            contrib += sink.nc;
+        }
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
@@ -1917,7 +1929,13 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
                 d.prob = pid;
                 ++is;
             }
-            append_seg(d, lane, O, blk);
+            if constexpr(MODE == TRACE_DESC) {
+                append_seg(d, lane, O, blk);
+            } else {
+                DevSink<real> sink{&O, &P, 0};
+                seg_items<real, RT_TL, DevSink<real>, MODE == TRACE_FUSED_FAST>(d, P, G, P.Rz, sink);
+                contrib += sink.nc;
+            }
         }
         bool fin = false;
         if(active) {
@@ -1942,30 +1960,32 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
     for(int off = WAVE / 2; off > 0; off >>= 1) t += __shfl_xor(t, off);
     if(lane == 0 && t) atomicAdd(Bd.psteps + pid, t);
     all_steps += steps;
+    all_contrib += contrib;
     return result;
 }
 
-template<typename real>
+template<typename real, int MODE>
 __device__ inline int run_problem_any(const BatchDev<real> &Bd, const DevOut &O, int pid, const RaySave<real> *resume,
-                                      SegBlock &blk, unsigned char *lds, unsigned long long &all_steps)
+                                      SegBlock &blk, unsigned char *lds, unsigned long long &all_steps,
+                                      unsigned long long &all_contrib)
 {
     switch(Bd.probs[pid].ssp_type) {
-    case 'C': return run_problem<real, 'C'>(Bd, O, pid, resume, blk, lds, all_steps);
-    case 'P': return run_problem<real, 'P'>(Bd, O, pid, resume, blk, lds, all_steps);
-    case 'S': return run_problem<real, 'S'>(Bd, O, pid, resume, blk, lds, all_steps);
-    case 'N': return run_problem<real, 'N'>(Bd, O, pid, resume, blk, lds, all_steps);
-    case 'Q': return run_problem<real, 'Q'>(Bd, O, pid, resume, blk, lds, all_steps);
-    default: return run_problem<real, 'A'>(Bd, O, pid, resume, blk, lds, all_steps);
+    case 'C': return run_problem<real, 'C', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
+    case 'P': return run_problem<real, 'P', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
+    case 'S': return run_problem<real, 'S', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
+    case 'N': return run_problem<real, 'N', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
+    case 'Q': return run_problem<real, 'Q', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
+    default: return run_problem<real, 'A', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
     }
 }
 
-template<typename real>
+template<typename real, int MODE>
 __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x;
     SegBlock blk{0u, 0u, 0u};
-    unsigned long long all_steps = 0;
+    unsigned long long all_steps = 0, all_contrib = 0;
     const int nsave_in = __builtin_amdgcn_readfirstlane(*Bd.nsave_in);
     for(;;) {
         // a saved wavefront of the previous pass first
@@ -1976,7 +1996,9 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
         }
         if(u >= 0 && u < nsave_in) {
             int pid = __builtin_amdgcn_readfirstlane(Bd.save_in_pid[u]);
-            if(run_problem_any(Bd, O, pid, Bd.save_in + (size_t)u * WAVE, blk, lds_raw, all_steps)) break;
+            if(run_problem_any<real, MODE>(Bd, O, pid, Bd.save_in + (size_t)u * WAVE, blk, lds_raw, all_steps,
+                                           all_contrib))
+                break;
             continue;
         }
         // no new problems once the segment buffer nears its end: the next
@@ -1998,11 +2020,15 @@ __global__ void __launch_bounds__(64) k_trace_batch(BatchDev<real> Bd, DevOut O)
         }
         p = __builtin_amdgcn_readfirstlane(__shfl(p, 0));
         if(p >= Bd.nprob) break;
-        if(run_problem_any<real>(Bd, O, p, nullptr, blk, lds_raw, all_steps)) break;
+        if(run_problem_any<real, MODE>(Bd, O, p, nullptr, blk, lds_raw, all_steps, all_contrib)) break;
     }
     seal_block<real>(blk, lane, O);
-    for(int off = WAVE / 2; off > 0; off >>= 1) all_steps += __shfl_xor(all_steps, off);
+    for(int off = WAVE / 2; off > 0; off >>= 1) {
+        all_steps += __shfl_xor(all_steps, off);
+        all_contrib += __shfl_xor(all_contrib, off);
+    }
     if(lane == 0) atomicAdd(O.counters + 0, all_steps);
+    if(lane == 0 && all_contrib) atomicAdd(O.counters + 1, all_contrib);
 }
 
 // Scatter of a batched pass: one wavefront per segment, the segment's problem
@@ -2326,7 +2352,18 @@ template<typename real> void enqueue_pass(bhc_sweep *sw, BatchDev<real> Bd, cons
     Bd.nsave_out = sw->d_nsave + (1 - cur);
     hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, c->d_seg_count, c->d_overflow, sw->d_resume,
                        sw->d_nsave + (1 - cur), sw->d_more);
-    hipLaunchKernelGGL(k_trace_batch<real>, dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
+    if(sw->scatter_lanes == 3) {
+        // fused: the trace evaluates every step's receivers itself
+        if(c->exact_phasor)
+            hipLaunchKernelGGL((k_trace_batch<real, TRACE_FUSED_EXACT>), dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes,
+                               s, Bd, O);
+        else
+            hipLaunchKernelGGL((k_trace_batch<real, TRACE_FUSED_FAST>), dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes,
+                               s, Bd, O);
+        if(e_mid) (void)hipEventRecord(e_mid, s);
+        return;
+    }
+    hipLaunchKernelGGL((k_trace_batch<real, TRACE_DESC>), dim3(sw->max_waves), dim3(WAVE), sw->lds_bytes, s, Bd, O);
     if(e_mid) (void)hipEventRecord(e_mid, s);
     if(sw->scatter_lanes == 2) {
         if(c->exact_phasor)
@@ -2488,8 +2525,8 @@ template<typename real> int sweep_tune(bhc_sweep *sw)
     // scatter: choose the scatter's at the full budget, then the budget
     int best_blocks = grids[0], best_lanes = 1, best_lds = ldss[0];
     double best = 1e300, t = 0.0;
-    for(int mode = 0; mode < 3; ++mode)
-        for(int gi = 0; gi < 3; ++gi) {
+    for(int mode = 0; mode < 4; ++mode)
+        for(int gi = 0; gi < (mode == 3 ? 1 : 3); ++gi) {   // the fused mode launches no scatter
             if(timed(mode, grids[gi], ldss[0], t)) return -1;
             if(t < best) {
                 best = t;
@@ -2590,7 +2627,7 @@ extern "C" int bhc_sweep_set_launch(bhc_sweep *sw, int scatter_blocks, int lds_b
 
 extern "C" int bhc_sweep_set_scatter(bhc_sweep *sw, int mapping)
 {
-    if(!sw || mapping < 0 || mapping > 2) return fail("bhc_sweep_set_scatter: bad arguments");
+    if(!sw || mapping < 0 || mapping > 3) return fail("bhc_sweep_set_scatter: bad arguments");
     sw->scatter_lanes = mapping;
     return 0;
 }

@@ -10,8 +10,9 @@ decides where its prediction is trusted and where the tracer runs instead.
 Environment family (14 parameters, each mapped from [0, 1] or N(0, 1)):
   source depth 20-500 m; frequency 50-500 Hz (log-uniform); 8 sound-speed
   EOFs -- sin(k pi z / D), k = 1..8, amplitude N(0, 1) x 5 m/s -- on the Munk
-  profile over D = 4000 m; a Gaussian seamount (height 0-2500 m, centre
-  10-40 km, width 2-10 km) and the bottom's compressional speed 1550-1800 m/s.
+  profile over D = 4000 m; and 4 seamount parameters: height 0-2000 m, centre
+  10-40 km, near-flank width 2-10 km and far-flank width 0.5-2 times it (an
+  asymmetric Gaussian); bottom an acousto-elastic halfspace of 1600 m/s.
   Receivers: 51 depths x 201 ranges to 50 km; 1001 Gaussian beams over
   +-30 degrees, incoherent TL (the smooth field a coordinate network can
   represent; coherent interference is left to the tracer).
@@ -139,8 +140,8 @@ class Mlp:
 # --------------------------------------------------------------------------
 
 def draw_params(n, seed):
-    """n parameter vectors: [zs, logf, h, rc, w, cp] in [0, 1] and 8 EOF
-    amplitudes N(0, 1) -> array [n, 14] ordered (zs, logf, eof1..8, h, rc, w, cp)."""
+    """n parameter vectors: [zs, logf, h, rc, w, asym] in [0, 1] and 8 EOF
+    amplitudes N(0, 1) -> array [n, 14] ordered (zs, logf, eof1..8, h, rc, w, asym)."""
     rng = np.random.default_rng(seed)
     th = np.zeros((n, 14))
     th[:, 0:2] = rng.random((n, 2))
@@ -156,12 +157,9 @@ def environment(theta, precision=64, nbeams=1001, nrz=51, nrr=201):
     c = munk_c(z * (5000.0 / DEPTH))
     for k in range(8):
         c = c + 5.0 * theta[2 + k] * np.sin((k + 1) * np.pi * z / DEPTH)
-    h = 2500.0 * theta[10]
-    rc = 10.0 + 30.0 * theta[11]
-    w = 2.0 + 8.0 * theta[12]
-    cp = 1550.0 + 250.0 * theta[13]
     rb = np.linspace(0.0, R_MAX_KM + 1.0, 201)
-    db = DEPTH - h * np.exp(-0.5 * ((rb - rc) / w) ** 2)
+    db = seabed_depth(theta, rb)
+    cp = 1600.0
     return Env2D(
         title="surrogate family", precision=precision, freq0=float(freq), ssp_type="C", atten_unit="W ",
         ssp_z=z, ssp_c=c, top=Halfspace("V", depth=0.0),
@@ -172,6 +170,18 @@ def environment(theta, precision=64, nbeams=1001, nrz=51, nrr=201):
         box_r_km=R_MAX_KM + 1.0, box_z=DEPTH + 100.0)
 
 
+def seabed_depth(theta, r_km):
+    """Bathymetry (m) of environment `theta` at ranges r_km: an asymmetric
+    Gaussian seamount on the flat 4000 m bottom."""
+    h = 2000.0 * theta[10]
+    rc = 10.0 + 30.0 * theta[11]
+    w1 = 2.0 + 8.0 * theta[12]
+    w2 = w1 * (0.5 + 1.5 * theta[13])
+    r = np.asarray(r_km, np.float64)
+    w = np.where(r < rc, w1, w2)
+    return DEPTH - h * np.exp(-0.5 * ((r - rc) / w) ** 2)
+
+
 # Frequency scale of each input in the Fourier projection: range and depth
 # carry the field's structure, the environment parameters modulate it
 # smoothly (low frequencies there keep the network from keying on individual
@@ -179,9 +189,21 @@ def environment(theta, precision=64, nbeams=1001, nrz=51, nrr=201):
 INPUT_SCALES = np.array([2.0, 2.0, 0.5, 0.5] + [0.25] * 8 + [0.5] * 4)
 
 
-def fourier_matrix(seed):
+def fourier_matrix(seed, kind="random"):
+    """Projection B [32][16] of the features sin / cos(2 pi B x). "random":
+    Gaussian rows scaled per input (INPUT_SCALES); "structured": 10 range
+    and 8 depth rows at geometrically spaced frequencies (0.5-32 and 0.5-16
+    cycles over the normalised span), the other 14 rows random mixtures."""
     rng = np.random.default_rng(seed)
-    return (rng.normal(size=(32, 16)) * INPUT_SCALES[None, :]).astype(np.float32)
+    if kind == "random":
+        return (rng.normal(size=(32, 16)) * INPUT_SCALES[None, :]).astype(np.float32)
+    if kind != "structured":
+        raise ValueError(kind)
+    B = np.zeros((32, 16), np.float32)
+    B[np.arange(10), 0] = np.geomspace(0.5, 32.0, 10)
+    B[10 + np.arange(8), 1] = np.geomspace(0.5, 16.0, 8)
+    B[18:] = rng.normal(size=(14, 16)) * np.array([1.0, 1.0] + [0.5] * 14)[None, :]
+    return B
 
 
 def inputs(theta, rz, rr_km):

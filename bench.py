@@ -398,11 +398,16 @@ def run_sweep(a):
     F = f_I * R + f_S * S + f_C * Cn
     k_ms = (sum(tr_ms) + sum(sc_ms)) / len(tr_ms)
     real_t = "double" if a.precision == 64 else "float"
-    rl = kernel_roofline(a.precision, [
-        {"name": "k_trace_batch<%s>" % real_t, "ms": sum(tr_ms), "launches": npasses, "flop": f_I * R + f_S * S},
-        {"name": {0: "k_scatter_batch<%s, true>", 1: "k_scatter_lanes<%s, true>", 2: "k_scatter_flat<%s, true>"}[
-            tuned["scatter_lanes"]] % real_t, "ms": sum(sc_ms), "launches": npasses, "flop": f_C * Cn},
-    ], F, k_ms * a.steps)
+    if tuned["scatter_lanes"] == 3:
+        # fused: one kernel traces and evaluates the receivers (f_I R + f_S S + f_C C)
+        kern = [{"name": "k_trace_batch<%s, fused>" % real_t, "ms": sum(tr_ms) + sum(sc_ms), "launches": npasses,
+                 "flop": F}]
+    else:
+        kern = [
+            {"name": "k_trace_batch<%s>" % real_t, "ms": sum(tr_ms), "launches": npasses, "flop": f_I * R + f_S * S},
+            {"name": {0: "k_scatter_batch<%s, true>", 1: "k_scatter_lanes<%s, true>", 2: "k_scatter_flat<%s, true>"}[
+                tuned["scatter_lanes"]] % real_t, "ms": sum(sc_ms), "launches": npasses, "flop": f_C * Cn}]
+    rl = kernel_roofline(a.precision, kern, F, k_ms * a.steps)
     rl.update({"passes": npasses, "f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S,
                "contributions": Cn})
     line = {
@@ -458,13 +463,14 @@ def run_surrogate(a):
     ref = sg.reference_field(f_tr)
     X, Y = sg.dataset(th_tr, f_tr, e_tr, a.sur_points, 1, ref)
     mlp = sg.Mlp(members=4, max_batch=a.sur_batch, seed=11)
-    mlp.set_fourier(sg.fourier_matrix(12))
+    mlp.set_fourier(sg.fourier_matrix(12, a.sur_fourier))
     mlp.set_dataset(X, Y)
     t0 = time.perf_counter()
     done, train_ms = 0, 0.0
     while done < a.sur_steps:
         k = min(500, a.sur_steps - done)
-        lr = a.sur_lr * (0.1 if done >= 0.9 * a.sur_steps else 1.0)
+        # cosine decay from sur_lr, one value per chunk of steps
+        lr = a.sur_lr * (0.02 + 0.98 * 0.5 * (1.0 + np.cos(np.pi * (done + 0.5 * k) / a.sur_steps)))
         loss = mlp.train(k, a.sur_batch, lr, seed=done)
         train_ms += mlp.last_ms()
         done += k
@@ -553,6 +559,7 @@ def main():
     ap.add_argument("--sur-steps", type=int, default=20000)
     ap.add_argument("--sur-batch", type=int, default=16384)
     ap.add_argument("--sur-lr", type=float, default=1e-3)
+    ap.add_argument("--sur-fourier", default="random", choices=["random", "structured"])
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--sweep-n", type=int, default=4096)
     ap.add_argument("--sweep-scale", type=float, default=1.0)

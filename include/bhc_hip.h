@@ -416,8 +416,10 @@ int bhc_sweep_autotune(bhc_ctx *ctx, bhc_sweep *sw, int *scatter_blocks, int *ld
  * captured HIP graph; 0: enqueued kernel by kernel with per-kernel timing. */
 int bhc_sweep_set_graph(bhc_sweep *sw, int use_graph);
 /* Scatter mapping of the sweep's passes: 0 one wavefront per segment, 1 one
- * lane per segment, 2 the receivers of 64 segments flattened over a wavefront
- * (bhc_sweep_autotune picks among them). */
+ * lane per segment, 2 the receivers of 64 segments flattened over a wavefront,
+ * 3 fused: the trace kernel evaluates each step's receivers itself (no
+ * segment buffer, no scatter kernel, one pass) (bhc_sweep_autotune picks
+ * among them). */
 int bhc_sweep_set_scatter(bhc_sweep *sw, int mapping);
 /* Scatter grid (workgroups) and per-workgroup LDS staging budget of the trace
  * (bytes, a multiple of the real size, at most the upload's budget) without a

@@ -144,12 +144,13 @@ def test_sweep_graph_replays_match_eager(gpu_ctx):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("mapping", [0, 1, 2])
+@pytest.mark.parametrize("mapping", [0, 1, 2, 3])
 def test_sweep_scatter_mappings(mapping, precision, gpu_ctx):
     """Every scatter mapping of the batched passes (a wavefront per segment, a
-    lane per segment, 64 segments' receivers flattened over a wavefront) gives
-    the single-problem runs' contribution counts and TL; the mix holds
-    Cartesian hat / Gaussian and ray-centred Gaussian problems."""
+    lane per segment, 64 segments' receivers flattened over a wavefront, and
+    the fused trace that evaluates each step's receivers itself) gives the
+    single-problem runs' contribution counts and TL; the mix holds Cartesian
+    hat / Gaussian and ray-centred Gaussian problems."""
     _announce("sweep_scatter_mapping_%d_f%d" % (mapping, precision))
     envs = _mixed(precision)
     probs = [bh.Problem(e) for e in envs]
