@@ -73,9 +73,17 @@ class Env3D:
     bio: Optional[np.ndarray] = None          # [n, 5] z1, z2 (m), f0 (Hz), Q, a0 (dB/km); atten_unit[1] == 'B'
 
     @property
+    def nx2d(self):
+        """Nx2D run (RunType[5] == '2'): 2D rays in the vertical plane of each bearing."""
+        return (self.run_type + " " * 6)[5] == "2"
+
+    @property
     def nrays(self):
         na = 1 if self.iSingle_alpha > 0 else len(self.alpha_deg)
-        nb = 1 if self.iSingle_beta > 0 else len(self.beta_deg)
+        # an Nx2D run that is not a ray run traces on the receiver bearings
+        # (src/module/rayangles.hpp:163-178)
+        fan = self.theta_deg if (self.nx2d and self.run_type[0] != "R") else self.beta_deg
+        nb = 1 if self.iSingle_beta > 0 else len(fan)
         return na * nb * len(self.Sx_km) * len(self.Sy_km) * len(self.Sz)
 
     @property

@@ -67,6 +67,7 @@ extern "C" {
 #define BHC_WARN_TRIDIAG_H_GROWING 10
 #define BHC_WARN_CPCHIP_INVALIDXT 16
 #define BHC_WARN_CPCHIP_INVALIDCCOEF 17
+#define BHC_WARN_OCEANTORAYX_GAVEUP 18
 
 /* ---- output record layouts (byte-identical to the reference) ----------------- */
 

@@ -1157,6 +1157,11 @@ static void Reflect(const RayPt *oldp, RayPt *newp, const HS *hs, int isTop, v2 
 typedef struct {
     int isz, ialpha;
     real alpha, SrcDeclAngle;
+    /* Nx2D rays (nx = 1): isz above is the field's source slot
+     * ((isz3 * NSx + isx) * NSy + isy) * Ntheta + ibeta, GetFieldAddr
+     * (common.hpp:393-405) with itheta = ibeta (influence.hpp:439) */
+    int nx, isx, isy, isz3, ibeta;
+    real SrcAzimAngle;
 } RayInitInfo;
 
 typedef struct {
@@ -1444,8 +1449,10 @@ static void AddArr(int id, int ir, real Amp, real omega, real Phase, cpx delay, 
         }
         ++g_arrlog_n;
     }
+    /* Nx2D: RcvrAzimAngle = SrcAzimAngle (influence.hpp:122-135) */
+    real az = ri->nx ? ri->SrcAzimAngle : (real)NAN;
     AddArrAt(&O->arr[base * O->max_narr], &O->narr[base], O->max_narr, O->allow_merging, Amp, omega, Phase, delay,
-             ri->SrcDeclAngle, (real)NAN, RcvrDeclAngle, (real)NAN, NumTopBnc, NumBotBnc);
+             ri->SrcDeclAngle, az, RcvrDeclAngle, az, NumTopBnc, NumBotBnc);
 }
 
 static void RecordEigenHit(int ir, int iz, int is, const RayInitInfo *ri, Outs *O)
@@ -1453,8 +1460,9 @@ static void RecordEigenHit(int ir, int iz, int is, const RayInitInfo *ri, Outs *
     int mi = atomic_fetch_add(O->neigen, 1);
     if(mi >= O->eigen_memsize) return;
     bhc_eigen_hit *h = &O->hits[mi];
-    h->is = is; h->iz = iz; h->ir = ir; h->itheta = 0;
-    h->isx = 0; h->isy = 0; h->isz = ri->isz; h->ialpha = ri->ialpha; h->ibeta = 0;
+    h->is = is; h->iz = iz; h->ir = ir; h->itheta = ri->nx ? ri->ibeta : 0;
+    h->isx = ri->isx; h->isy = ri->isy; h->isz = ri->nx ? ri->isz3 : ri->isz; h->ialpha = ri->ialpha;
+    h->ibeta = ri->ibeta;
 }
 
 static void ApplyContribution(real cnst, real w, real omega, cpx delay, real phaseInt, real RcvrDeclAngle, int ir,
@@ -1747,6 +1755,7 @@ static int num_jobs(const Prob *P) /* common.hpp:347-359 (2D) */
 }
 static int job_indices(RayInitInfo *ri, int job, const Prob *P) /* common.hpp:365-391 (2D) */
 {
+    memset(ri, 0, sizeof(*ri));
     if(P->iSingle >= 1) {
         ri->ialpha = P->iSingle - 1;
     } else {
@@ -1857,6 +1866,7 @@ int SFX(oracle_ray_run2d)(const bhc_env2d *env, bhc_ray_out *out, const int32_t 
     RayPt *buf = xcalloc((size_t)M + 3, sizeof(RayPt));
     for(int k = 0; k < nrays; ++k) {
         RayInitInfo ri;
+        memset(&ri, 0, sizeof(ri));
         if(ialpha_list) {
             ri.ialpha = ialpha_list[k];
             ri.isz = isz_list ? isz_list[k] : 0;

@@ -243,11 +243,17 @@ static int prob3_from_env(const bhc_env3d *e, Prob3 *Q)
         real th = DegRad * (real)e->theta_deg[i];
         Q->t_rcvr[i] = V2(RCOS(th), RSIN(th));
     }
-    /* beta: rayangles.hpp:181-189 */
+    /* beta: rayangles.hpp:157-189; an Nx2D run that is not a ray run traces
+     * its beams on the receiver bearings (rayangles.hpp:163-178) */
+    const double *bdeg = e->beta_deg;
     Q->Nbeta = e->Nbeta;
+    if(e->RunType[5] == '2' && e->RunType[0] != 'R') {
+        bdeg = e->theta_deg;
+        Q->Nbeta = e->Ntheta;
+    }
     Q->iSingleB = e->iSingle_beta;
     Q->beta = xcalloc(Q->Nbeta, sizeof(real));
-    for(int i = 0; i < Q->Nbeta; ++i) Q->beta[i] = (real)e->beta_deg[i] * DegRad;
+    for(int i = 0; i < Q->Nbeta; ++i) Q->beta[i] = (real)bdeg[i] * DegRad;
     Q->Dbeta = FL(0.0);
     if(Q->Nbeta != 1) Q->Dbeta = (Q->beta[Q->Nbeta - 1] - Q->beta[0]) / (Q->Nbeta - 1);
     /* beam box: beaminfo.hpp:226-230 */
@@ -395,8 +401,9 @@ typedef struct {
     v3 xs;
 } Ctx3;
 
-static void DepthInterfaceCrossing3(int stepTo, real *hInt, v3 *x, v3 x0, v3 urayt, const Seg3 *iSeg0, const Prob3 *Q)
-{ /* step.hpp:41-80 */
+static void DepthInterfaceCrossing3(int stepTo, real *hInt, v3 *x, v3 x0, v3 urayt, const Seg3 *iSeg0, const Prob3 *Q,
+                                    int *snap)
+{ /* step.hpp:41-80; snap: the snapDim of OceanToRayX (null when not stepping) */
     if(!stepTo) *hInt = REAL_MAX;
     if(RABS(urayt.z) > REAL_EPSILON) {
         const real *z = Q->z3;
@@ -406,18 +413,21 @@ static void DepthInterfaceCrossing3(int stepTo, real *hInt, v3 *x, v3 x0, v3 ura
             if(stepTo) {
                 *x = v3add(x0, v3scl(*hInt, urayt));
                 x->z = z[iz];
+                if(snap) *snap = 2;
             }
         } else if(z[iz + 1] < x->z && iz + 1 < Q->nz3 - 1) {
             *hInt = (z[iz + 1] - x0.z) / urayt.z;
             if(stepTo) {
                 *x = v3add(x0, v3scl(*hInt, urayt));
                 x->z = z[iz + 1];
+                if(snap) *snap = 2;
             }
         }
     }
 }
 
-static void BeamBoxCrossing3(int stepTo, int dim, real *hBox, v3 *x, v3 x0, v3 urayt, const Prob3 *Q, v3 xs)
+static void BeamBoxCrossing3(int stepTo, int dim, real *hBox, v3 *x, v3 x0, v3 urayt, const Prob3 *Q, v3 xs,
+                             int *snap)
 { /* step.hpp:85-111, common_run.hpp:33-47 */
     if(!stepTo) *hBox = REAL_MAX;
     v3 ctr = V3(xs.x, xs.y, RL(0.0));
@@ -428,11 +438,12 @@ static void BeamBoxCrossing3(int stepTo, int dim, real *hBox, v3 *x, v3 x0, v3 u
         if(stepTo) {
             *x = v3add(x0, v3scl(*hBox, urayt));
             v3set(x, dim, v3get(ctr, dim) + RCOPYSIGN(box, d0));
+            if(snap) *snap = dim;
         }
     }
 }
 
-static void TopBotCrossing3(int stepTo, real *h, const BState3 *bd, v3 *x, v3 x0, v3 urayt, int *refl)
+static void TopBotCrossing3(int stepTo, real *h, const BState3 *bd, v3 *x, v3 x0, v3 urayt, int *refl, int *snap)
 { /* step.hpp:116-151 */
     if(!stepTo) *h = REAL_MAX;
     v3 d = v3sub(*x, bd->x), d0 = v3sub(x0, bd->x);
@@ -442,6 +453,7 @@ static void TopBotCrossing3(int stepTo, real *h, const BState3 *bd, v3 *x, v3 x0
         if(stepTo) {
             *x = v3add(x0, v3scl(*h, urayt));
             if(RABS(bd->n.x) < REAL_EPSILON && RABS(bd->n.y) < REAL_EPSILON) x->z = bd->x.z;
+            if(snap) *snap = 2;
         }
         *refl = 1;
     } else {
@@ -450,7 +462,7 @@ static void TopBotCrossing3(int stepTo, real *h, const BState3 *bd, v3 *x, v3 x0
 }
 
 static void TopBotSegCrossing3(int stepTo, int isY, real *hSeg, const BState3 *top, const BState3 *bot, int iSeg,
-                               v3 *x, v3 x0, v3 urayt, const Prob3 *Q, int *topRefl, int *botRefl)
+                               v3 *x, v3 x0, v3 urayt, const Prob3 *Q, int *topRefl, int *botRefl, int *snap)
 { /* step.hpp:157-207 */
     real lmin = rmax(isY ? top->ymin : top->xmin, isY ? bot->ymin : bot->xmin);
     real lmax = rmin(isY ? top->ymax : top->xmax, isY ? bot->ymax : bot->xmax);
@@ -467,6 +479,7 @@ static void TopBotSegCrossing3(int stepTo, int isY, real *hSeg, const BState3 *t
             if(stepTo) {
                 *x = v3add(x0, v3scl(*hSeg, urayt));
                 if(isY) x->y = lmin; else x->x = lmin;
+                if(snap) *snap = isY;
                 *topRefl = *botRefl = 0;
             }
         } else if(xw > lmax) {
@@ -474,6 +487,7 @@ static void TopBotSegCrossing3(int stepTo, int isY, real *hSeg, const BState3 *t
             if(stepTo) {
                 *x = v3add(x0, v3scl(*hSeg, urayt));
                 if(isY) x->y = lmax; else x->x = lmax;
+                if(snap) *snap = isY;
                 *topRefl = *botRefl = 0;
             }
         }
@@ -481,7 +495,7 @@ static void TopBotSegCrossing3(int stepTo, int isY, real *hSeg, const BState3 *t
 }
 
 static void TriDiagCrossing(int stepTo, real *h, BState3 *bd, v3 *x, v3 x0, v3 urayt, int *topRefl, int *botRefl,
-                            ErrState *err)
+                            ErrState *err, int *snap)
 { /* step.hpp:212-260 */
     if(!stepTo) *h = REAL_MAX;
     if(bd->onEdge) return;
@@ -504,6 +518,7 @@ static void TriDiagCrossing(int stepTo, real *h, BState3 *bd, v3 *x, v3 x0, v3 u
         *x = v3add(x0, v3scl(*h, urayt));
         bd->justSteppedTo = 1;
         bd->outgoingSide = newSide;
+        if(snap) *snap = -2;
         *topRefl = *botRefl = 0;
     }
 }
@@ -514,16 +529,16 @@ static void ReduceStep3(v3 x0, v3 urayt, const Seg3 *iSeg0, BState3 *top, BState
     real hInt, hBx, hBy, hBz, hTop, hBot, hxSeg, hySeg, hTD, hBD;
     int dummy;
     v3 x = v3add(x0, v3scl(*h, urayt));
-    DepthInterfaceCrossing3(0, &hInt, &x, x0, urayt, iSeg0, Q);
-    BeamBoxCrossing3(0, 0, &hBx, &x, x0, urayt, Q, xs);
-    BeamBoxCrossing3(0, 1, &hBy, &x, x0, urayt, Q, xs);
-    BeamBoxCrossing3(0, 2, &hBz, &x, x0, urayt, Q, xs);
-    TopBotCrossing3(0, &hTop, top, &x, x0, urayt, &dummy);
-    TopBotCrossing3(0, &hBot, bot, &x, x0, urayt, &dummy);
-    TopBotSegCrossing3(0, 0, &hxSeg, top, bot, iSeg0->x, &x, x0, urayt, Q, &dummy, &dummy);
-    TopBotSegCrossing3(0, 1, &hySeg, top, bot, iSeg0->y, &x, x0, urayt, Q, &dummy, &dummy);
-    TriDiagCrossing(0, &hTD, top, &x, x0, urayt, &dummy, &dummy, err);
-    TriDiagCrossing(0, &hBD, bot, &x, x0, urayt, &dummy, &dummy, err);
+    DepthInterfaceCrossing3(0, &hInt, &x, x0, urayt, iSeg0, Q, NULL);
+    BeamBoxCrossing3(0, 0, &hBx, &x, x0, urayt, Q, xs, NULL);
+    BeamBoxCrossing3(0, 1, &hBy, &x, x0, urayt, Q, xs, NULL);
+    BeamBoxCrossing3(0, 2, &hBz, &x, x0, urayt, Q, xs, NULL);
+    TopBotCrossing3(0, &hTop, top, &x, x0, urayt, &dummy, NULL);
+    TopBotCrossing3(0, &hBot, bot, &x, x0, urayt, &dummy, NULL);
+    TopBotSegCrossing3(0, 0, &hxSeg, top, bot, iSeg0->x, &x, x0, urayt, Q, &dummy, &dummy, NULL);
+    TopBotSegCrossing3(0, 1, &hySeg, top, bot, iSeg0->y, &x, x0, urayt, Q, &dummy, &dummy, NULL);
+    TriDiagCrossing(0, &hTD, top, &x, x0, urayt, &dummy, &dummy, err, NULL);
+    TriDiagCrossing(0, &hBD, bot, &x, x0, urayt, &dummy, &dummy, err, NULL);
     *h = rmin(rmin(rmin(*h, hInt), rmin(rmin(rmin(hBx, hBy), hBz), rmin(hTop, hBot))),
               rmin(rmin(hxSeg, hySeg), rmin(hTD, hBD)));
     if(*h < RL(-1e-4)) RunWarning(err, BHC_WARN_STEP_NEGATIVE_H);
@@ -536,28 +551,31 @@ static void ReduceStep3(v3 x0, v3 urayt, const Seg3 *iSeg0, BState3 *top, BState
 }
 
 static void StepToBdry3(v3 x0, v3 *x2, v3 urayt, real *h, int *topRefl, int *botRefl, const Seg3 *iSeg0,
-                        BState3 *top, BState3 *bot, const Prob3 *Q, v3 xs, ErrState *err)
-{ /* step.hpp:342-420 */
+                        BState3 *top, BState3 *bot, const Prob3 *Q, v3 xs, ErrState *err, int *snapDim)
+{ /* step.hpp:342-420; snapDim: OceanToRayX's snap choice (common_run.hpp:243-252) */
+    int snap = -1;
     *x2 = v3add(x0, v3scl(*h, urayt));
-    DepthInterfaceCrossing3(1, h, x2, x0, urayt, iSeg0, Q);
-    BeamBoxCrossing3(1, 0, h, x2, x0, urayt, Q, xs);
-    BeamBoxCrossing3(1, 1, h, x2, x0, urayt, Q, xs);
-    BeamBoxCrossing3(1, 2, h, x2, x0, urayt, Q, xs);
-    TopBotCrossing3(1, h, top, x2, x0, urayt, topRefl);
-    TopBotCrossing3(1, h, bot, x2, x0, urayt, botRefl);
+    DepthInterfaceCrossing3(1, h, x2, x0, urayt, iSeg0, Q, &snap);
+    BeamBoxCrossing3(1, 0, h, x2, x0, urayt, Q, xs, &snap);
+    BeamBoxCrossing3(1, 1, h, x2, x0, urayt, Q, xs, &snap);
+    BeamBoxCrossing3(1, 2, h, x2, x0, urayt, Q, xs, &snap);
+    TopBotCrossing3(1, h, top, x2, x0, urayt, topRefl, &snap);
+    TopBotCrossing3(1, h, bot, x2, x0, urayt, botRefl, &snap);
     if(*botRefl) *topRefl = 0;
-    TopBotSegCrossing3(1, 0, h, top, bot, iSeg0->x, x2, x0, urayt, Q, topRefl, botRefl);
-    TopBotSegCrossing3(1, 1, h, top, bot, iSeg0->y, x2, x0, urayt, Q, topRefl, botRefl);
-    TriDiagCrossing(1, h, top, x2, x0, urayt, topRefl, botRefl, err);
-    TriDiagCrossing(1, h, bot, x2, x0, urayt, topRefl, botRefl, err);
+    TopBotSegCrossing3(1, 0, h, top, bot, iSeg0->x, x2, x0, urayt, Q, topRefl, botRefl, &snap);
+    TopBotSegCrossing3(1, 1, h, top, bot, iSeg0->y, x2, x0, urayt, Q, topRefl, botRefl, &snap);
+    TriDiagCrossing(1, h, top, x2, x0, urayt, topRefl, botRefl, err, &snap);
+    TriDiagCrossing(1, h, bot, x2, x0, urayt, topRefl, botRefl, err, &snap);
     if(*h < INFINITESIMAL_STEP_SIZE * Q->P2.deltas) {
         *h = INFINITESIMAL_STEP_SIZE * Q->P2.deltas;
         *x2 = v3add(x0, v3scl(*h, urayt));
+        snap = -1;
         v3 d = v3sub(*x2, top->x);
         if(v3dot(top->n, d) > REAL_EPSILON) {
             *topRefl = 1;
             top->justSteppedTo = 0;
             bot->justSteppedTo = 0;
+            snap = 2;
         } else {
             *topRefl = 0;
         }
@@ -567,10 +585,12 @@ static void StepToBdry3(v3 x0, v3 *x2, v3 urayt, real *h, int *topRefl, int *bot
             *topRefl = 0;
             top->justSteppedTo = 0;
             bot->justSteppedTo = 0;
+            snap = 2;
         } else {
             *botRefl = 0;
         }
     }
+    if(snapDim) *snapDim = snap;
 }
 
 /* Ray state, rayPt<true> (include/bhc/structs.hpp:397-424). */
@@ -733,7 +753,7 @@ static void Step3(RayPt3 ray0, RayPt3 *ray2, TraceState3 *S, const Prob3 *Q, Err
     real w0 = RL(1.0) - w1;
     v3 urayt2 = v3add(v3scl(w0, urayt0), v3scl(w1, urayt1));
     v3 x2;
-    StepToBdry3(ray0.x, &x2, urayt2, &h, topRefl, botRefl, &iSeg0, &S->top, &S->bot, Q, S->xs, err);
+    StepToBdry3(ray0.x, &x2, urayt2, &h, topRefl, botRefl, &iSeg0, &S->top, &S->bot, Q, S->xs, err, NULL);
     ray2->x = x2;
     real hw0 = h * w0, hw1 = h * w1;
     ray2->t = v3sub(v3sub(ray0.t, v3divs(v3scl(hw0, o0.gradc), csq0)), v3divs(v3scl(hw1, o1.gradc), csq1));
@@ -1373,6 +1393,565 @@ int SFX(oracle_field_run3d)(const bhc_env3d *env, bhc_field_out *out, int nthrea
 /* Exported: the preprocessed 3D tables, for the preprocessing parity test.
  * Order: boundary nodes (bty then ati: x, n1, n2), hexahedral cz (if 'H'),
  * receiver bearing cos / sin, beta (rad), Sx, Sy, scalars deltas, box x / y / z, Dbeta. */
+/* ------------------------------------------------------------------------- */
+/* Nx2D: 2D rays in the vertical plane of each bearing of the 3D environment   */
+/* (bellhopcxx Nx2D, O3D = true, R3D = false)                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct { v3 xs; v2 tradial; } OriginNx; /* Origin<true, false>, include/bhc/structs.hpp:233-237 */
+
+static v3 RayToOceanX(v2 x, const OriginNx *o)
+{ /* common_run.hpp:221-231 */
+    return V3(o->xs.x + x.x * o->tradial.x, o->xs.y + x.x * o->tradial.y, x.y);
+}
+static v3 RayToOceanT(v2 t, const OriginNx *o)
+{ /* common_run.hpp:233-242 */
+    return V3(t.x * o->tradial.x, t.x * o->tradial.y, t.y);
+}
+static v2 OceanToRayT(v3 t, const OriginNx *o)
+{ /* common_run.hpp:308-317 */
+    return V2(vdot(V2(t.x, t.y), o->tradial), t.z);
+}
+static real RealBitsAddInt(real r, int i)
+{ /* common.hpp:316-341: step through the representation */
+#if OR_IS_FLOAT
+    int32_t k;
+    memcpy(&k, &r, 4);
+    k += i;
+#else
+    int64_t k;
+    memcpy(&k, &r, 8);
+    k += (int64_t)i;
+#endif
+    real x;
+    memcpy(&x, &k, sizeof(real));
+    return x;
+}
+static v2 OceanToRayX(v3 x, const OriginNx *o, v2 t, int snapDim, ErrState *err)
+{ /* common_run.hpp:253-306: range from the larger bearing component, then up
+   * to 4 one-ulp moves along the ray until the round trip lands on (or just
+   * past, in the ray's direction) the snapped ocean coordinate */
+    v2 ret;
+    ret.y = x.z;
+    if(RABS(o->tradial.x) >= RABS(o->tradial.y)) ret.x = (x.x - o->xs.x) / o->tradial.x;
+    else ret.x = (x.y - o->xs.y) / o->tradial.y;
+    if(snapDim < -2 || snapDim == -1 || snapDim >= 2) return ret;
+    for(int i = 0; i < 4; ++i) {
+        v3 xb = RayToOceanX(ret, o);
+        v2 want = vscl(t.x, o->tradial);
+        v2 errd = V2(xb.x - x.x, xb.y - x.y);
+        int okx = (want.x * errd.x) >= RL(0.0);
+        int oky = (want.y * errd.y) >= RL(0.0);
+        if((snapDim == 0 && okx) || (snapDim == 1 && oky) || (okx && oky)) return ret;
+        ret.x = RealBitsAddInt(ret.x, (t.x > RL(0.0)) ? 1 : -1);
+    }
+    RunWarning(err, BHC_WARN_OCEANTORAYX_GAVEUP);
+    return ret;
+}
+
+static void EvaluateSSPNx(v2 x, v2 t, SSPOut *o, const OriginNx *org, const Prob3 *Q, Seg3 *iSeg, ErrState *err)
+{ /* ssp.hpp:377-465 with O3D && !R3D: the 3D profile evaluated at the ocean
+   * point, gradient and second derivatives projected on the bearing */
+    SSPOut3 p;
+    EvaluateSSP3(RayToOceanX(x, org), RayToOceanT(t, org), &p, Q, iSeg, err);
+    o->gradc.x = vdot(org->tradial, V2(p.gradc.x, p.gradc.y));
+    o->gradc.y = p.gradc.z;
+    o->crz = org->tradial.x * p.cxz + org->tradial.y * p.cyz;
+    o->crr = p.cxx * SQ(org->tradial.x) + FL(2.0) * p.cxy * org->tradial.x * org->tradial.y
+        + p.cyy * SQ(org->tradial.y);
+    o->ccpx = p.ccpx;
+    o->rho = p.rho;
+    o->czz = p.czz;
+}
+
+typedef struct {
+    Seg3 iSeg;
+    BState3 top, bot;
+    v3 xs;
+    OriginNx org;
+    int iSmallStepCtr;
+} TraceStateNx;
+
+static void StepNx(RayPt ray0, RayPt *ray2, TraceStateNx *S, const Prob3 *Q, ErrState *err, int *topRefl,
+                   int *botRefl)
+{ /* step.hpp:654-811 with O3D && !R3D: 2D integration in ray coordinates,
+   * step limits from the 3D boundaries and SSP cells in ocean coordinates */
+    const OriginNx *org = &S->org;
+    RayPt ray1;
+    SSPOut o0, o1, o2;
+    EvaluateSSPNx(ray0.x, ray0.t, &o0, org, Q, &S->iSeg, err);
+    real c0 = creal(o0.ccpx);
+    real cnn0 = o0.crr * SQ(ray0.t.y) - FL(2.0) * o0.crz * ray0.t.x * ray0.t.y + o0.czz * SQ(ray0.t.x);
+    v2 pq0p = vscl(-cnn0, ray0.q), pq0q = vscl(c0, ray0.p);
+    Seg3 iSeg0 = S->iSeg;
+    real csq0 = SQ(c0);
+    v2 urayt0 = vscl(c0, ray0.t);
+    real h = Q->P2.deltas;
+    v3 x_o = RayToOceanX(ray0.x, org);
+    v3 t_o = RayToOceanT(urayt0, org);
+    ReduceStep3(x_o, t_o, &iSeg0, &S->top, &S->bot, Q, S->xs, err, &h, &S->iSmallStepCtr);
+    real halfh = FL(0.5) * h;
+    ray1.x = vadd(ray0.x, vscl(halfh, urayt0));
+    ray1.t = vsub(ray0.t, vdivs(vscl(halfh, o0.gradc), csq0));
+    ray1.p = vadd(ray0.p, vscl(halfh, pq0p));
+    ray1.q = vadd(ray0.q, vscl(halfh, pq0q));
+
+    EvaluateSSPNx(ray1.x, ray1.t, &o1, org, Q, &S->iSeg, err);
+    real c1 = creal(o1.ccpx);
+    real cnn1 = o1.crr * SQ(ray1.t.y) - FL(2.0) * o1.crz * ray1.t.x * ray1.t.y + o1.czz * SQ(ray1.t.x);
+    v2 pq1p = vscl(-cnn1, ray1.q), pq1q = vscl(c1, ray1.p);
+    real csq1 = SQ(c1);
+    v2 urayt1 = vscl(c1, ray1.t);
+    t_o = RayToOceanT(urayt1, org);
+    ReduceStep3(x_o, t_o, &iSeg0, &S->top, &S->bot, Q, S->xs, err, &h, &S->iSmallStepCtr);
+
+    real w1 = h / (RL(2.0) * halfh);
+    real w0 = RL(1.0) - w1;
+    v2 urayt2 = vadd(vscl(w0, urayt0), vscl(w1, urayt1));
+    t_o = RayToOceanT(urayt2, org);
+    v3 x2_o;
+    int snapDim = -1;
+    StepToBdry3(x_o, &x2_o, t_o, &h, topRefl, botRefl, &iSeg0, &S->top, &S->bot, Q, S->xs, err, &snapDim);
+    ray2->x = OceanToRayX(x2_o, org, urayt2, snapDim, err);
+    real hw0 = h * w0, hw1 = h * w1;
+    ray2->t = vsub(vsub(ray0.t, vdivs(vscl(hw0, o0.gradc), csq0)), vdivs(vscl(hw1, o1.gradc), csq1));
+    ray2->tau = ray0.tau + hw0 / o0.ccpx + hw1 / o1.ccpx;
+    ray2->p = vadd(ray0.p, vscl(hw0, pq0p));
+    ray2->q = vadd(ray0.q, vscl(hw0, pq0q));
+    ray2->p = vadd(ray2->p, vscl(hw1, pq1p));
+    ray2->q = vadd(ray2->q, vscl(hw1, pq1q));
+    ray2->Amp = ray0.Amp;
+    ray2->Phase = ray0.Phase;
+    ray2->NumTopBnc = ray0.NumTopBnc;
+    ray2->NumBotBnc = ray0.NumBotBnc;
+
+    EvaluateSSPNx(ray2->x, ray2->t, &o2, org, Q, &S->iSeg, err);
+    ray2->c = creal(o2.ccpx);
+    if(S->iSeg.z != iSeg0.z) { /* CurvatureCorrection, step.hpp:624-648 (depth crossings only) */
+        v2 gj = vsub(o2.gradc, o0.gradc);
+        v2 ray2n = V2(-ray2->t.y, ray2->t.x);
+        real cnjump = vdot(gj, ray2n);
+        real csjump = vdot(gj, ray2->t);
+        real rm = ray2->t.x / ray2->t.y;
+        real rn = rm * (FL(2.0) * cnjump - rm * csjump) / ray2->c;
+        ray2->p = vsub(ray2->p, vscl(rn, ray2->q));
+    }
+}
+
+static void ReflectNx(const RayPt *oldp, RayPt *newp, const HS *hs, int isTop, v3 nBdry, const OriginNx *org,
+                      const Prob3 *Q, Seg3 *iSeg, ErrState *err)
+{ /* reflect.hpp:94-452 with O3D && !R3D on a rectilinear (uncurved) boundary */
+    const Prob *P = &Q->P2;
+    v2 nB = OceanToRayT(nBdry, org);
+    nB = vscl(RL(1.0) / vlen(nB), nB);
+    real Th = vdot(oldp->t, nB);
+    v2 tB = vsub(oldp->t, vscl(Th, nB));
+    tB = vscl(RL(1.0) / vlen(tB), tB);
+    real Tg = vdot(oldp->t, tB);
+    /* OceanToRayCurvature (reflect.hpp:94-114) of kappa_xx = kappa_xy = kappa_yy = 0 */
+    const real kxx = RL(0.0), kxy = RL(0.0), kyy = RL(0.0);
+    real kappa = kxx * SQ(org->tradial.x) + FL(2.0) * kxy * org->tradial.x * org->tradial.y + kyy * SQ(org->tradial.y);
+    if(isTop) kappa = -kappa;
+    newp->NumTopBnc = oldp->NumTopBnc + isTop;
+    newp->NumBotBnc = oldp->NumBotBnc + (1 - isTop);
+    newp->x = oldp->x;
+    newp->t = vsub(oldp->t, vscl(FL(2.0) * Th, nB));
+    SSPOut o;
+    EvaluateSSPNx(oldp->x, newp->t, &o, org, Q, iSeg, err);
+    real c = creal(o.ccpx);
+    newp->c = c;
+    newp->tau = oldp->tau;
+    v2 rayt = vscl(c, oldp->t);
+    v2 rayn = V2(-rayt.y, rayt.x);
+    v2 rayt_t = vscl(c, newp->t);
+    v2 rayn_t = V2(-(-rayt_t.y), -(rayt_t.x));
+    real rn = FL(2.0) * kappa / SQ(c) / Th;
+    real cnjump = -vdot(o.gradc, vsub(rayn_t, rayn));
+    real csjump = -vdot(o.gradc, vsub(rayt_t, rayt));
+    if(isTop) {
+        cnjump = -cnjump;
+        rn = -rn;
+    }
+    real rm = Tg / Th;
+    rn += rm * (FL(2.0) * cnjump - rm * csjump) / SQ(c);
+    if(P->Type[2] == 'D') rn *= FL(2.0);
+    else if(P->Type[2] == 'Z') rn = FL(0.0);
+    newp->p = vadd(oldp->p, vscl(rn, oldp->q));
+    newp->q = oldp->q;
+
+    if(hs->bc == 'R') {
+        newp->Amp = oldp->Amp;
+        newp->Phase = oldp->Phase;
+    } else if(hs->bc == 'V') {
+        newp->Amp = oldp->Amp;
+        newp->Phase = oldp->Phase + REAL_PI;
+    } else if(hs->bc == 'F') {
+        ReflCoef RInt;
+        RInt.theta = RadDeg * RABS(RATAN2(Th, Tg));
+        if(RInt.theta > FL(90.0)) RInt.theta = FL(180.0) - RInt.theta;
+        if(isTop) InterpolateReflectionCoefficient(&RInt, P->trc, P->ntrc, err);
+        else InterpolateReflectionCoefficient(&RInt, P->brc, P->nbrc, err);
+        newp->Amp = oldp->Amp * RInt.r;
+        newp->Phase = oldp->Phase + RInt.phi;
+    } else if(hs->bc == 'A' || hs->bc == 'G') {
+        /* the O3D halfspace formula (reflect.hpp:279-299) */
+        real omega = FL(2.0) * REAL_PI * P->freq0;
+        cpx gk = omega * Tg;
+        const cpx J = I;
+        cpx g1sq = SQ(omega / c) - SQ(gk) - J * REAL_MINPOS;
+        cpx g2sq = SQ(omega / hs->cP) - SQ(gk) - J * REAL_MINPOS;
+        cpx g1 = CSQRT(-g1sq);
+        cpx g2 = CSQRT(-g2sq);
+        cpx Refl = (hs->rho * g1 - o.rho * g2) / (hs->rho * g1 + o.rho * g2);
+        real aR = CABS(Refl);
+        if(aR < FL(1.0e-5)) {
+            newp->Amp = FL(0.0);
+            newp->Phase = oldp->Phase;
+        } else {
+            newp->Amp = aR * oldp->Amp;
+            newp->Phase = oldp->Phase + RATAN2(cimag(Refl), creal(Refl));
+            if(P->Type[3] == 'S') {
+                /* beam displacement and width change, reflect.hpp:383-442 (!R3D) */
+                cpx ch = oldp->c / CCONJ(hs->cP);
+                real co = oldp->t.x * oldp->c;
+                real si = oldp->t.y * oldp->c;
+                real ck = omega / oldp->c;
+                cpx a = FL(2.0) * hs->rho * (FL(1.0) - SQ(ch));
+                cpx b = SQ(co) - SQ(ch);
+                cpx d = SQ(hs->rho) * SQ(si) + b;
+                cpx sb = CSQRT(b);
+                real cco = SQ(co);
+                real ssi = SQ(si);
+                cpx delta;
+                if(RABS(si) >= REAL_EPSILON) delta = a * co / si / (ck * sb * d);
+                else delta = FL(0.0);
+                real pdelta = creal(delta) / (oldp->c / co);
+                cpx cksbd = ck * sb * d;
+                cpx ddelta = a * (cco / (cksbd * b) - (RL(1.0) + (cco / ssi)) / cksbd
+                                  - FL(2.0) * SQ(co) * (SQ(hs->rho) - RL(1.0)) / (cksbd * d));
+                real rddelta = -creal(ddelta);
+                real sddelta = rddelta / RABS(rddelta);
+                real theta_bot = RATAN(tB.y / tB.x);
+                newp->x.x = newp->x.x + creal(delta) * RCOS(theta_bot);
+                newp->x.y = newp->x.y + creal(delta) * RSIN(theta_bot);
+                newp->tau = newp->tau + pdelta;
+                newp->q = vadd(newp->q, vscl(sddelta * rddelta * si * creal(o.ccpx), oldp->p));
+            }
+        }
+    } else {
+        RunError(err, BHC_ERR_BOUNDARY_CONDITION_TYPE);
+        newp->Amp = NAN;
+        newp->Phase = NAN;
+    }
+}
+
+static int RayInitNx(RayInit3Info *ri, RayPt *p0, TraceStateNx *S, const Prob3 *Q, ErrState *err)
+{ /* trace.hpp:51-67, 76-266 with O3D && !R3D */
+    const Prob *P = &Q->P2;
+    if(ri->isz < 0 || ri->isz >= P->NSz || ri->ialpha < 0 || ri->ialpha >= P->Nalpha || ri->isx < 0
+       || ri->isx >= Q->NSx || ri->isy < 0 || ri->isy >= Q->NSy || ri->ibeta < 0 || ri->ibeta >= Q->Nbeta) {
+        RunError(err, BHC_ERR_RAYINIT);
+        return 0;
+    }
+    ri->alpha = P->alpha[ri->ialpha];
+    ri->SrcDeclAngle = RadDeg * ri->alpha;
+    ri->beta = Q->beta[ri->ibeta];
+    ri->SrcAzimAngle = RadDeg * ri->beta;
+    S->iSeg.x = S->iSeg.y = S->iSeg.z = S->iSeg.r = 0;
+    S->xs = V3(Q->Sx[ri->isx], Q->Sy[ri->isy], (real)P->Sz[ri->isz]);
+    SSPOut3 o;
+    EvaluateSSP3(S->xs, V3(FL(0.0), FL(0.0), FL(1.0)), &o, Q, &S->iSeg, err);
+    S->org.xs = S->xs;
+    S->org.tradial = V2(RCOS(ri->beta), RSIN(ri->beta));
+    real c = creal(o.ccpx);
+    int ibp = BinarySearchLEQ(P->sbp, P->nsbp, 2, 0, ri->SrcDeclAngle);
+    ibp = imin(ibp, P->nsbp - 2);
+    real s = (ri->SrcDeclAngle - P->sbp[2 * ibp]) / (P->sbp[2 * (ibp + 1)] - P->sbp[2 * ibp]);
+    float Amp0 = (float)((FL(1.0) - s) * P->sbp[2 * ibp + 1] + s * P->sbp[2 * (ibp + 1) + 1]);
+    if(P->RunType[0] == 'S') {
+        float omega = (float)(FL(2.0) * REAL_PI * P->freq0);
+        Amp0 = (float)(Amp0 * (sqrtf(FL(2.0)) * RABS(RSIN(omega / c * S->xs.z * RSIN(ri->alpha)))));
+    }
+    v2 tinit2 = V2(RCOS(ri->alpha), RSIN(ri->alpha));
+    p0->x = V2(RL(0.0), S->xs.z);
+    p0->c = c;
+    p0->t = vdivs(tinit2, c);
+    p0->tau = 0;
+    p0->Amp = Amp0;
+    p0->Phase = FL(0.0);
+    p0->NumTopBnc = p0->NumBotBnc = 0;
+    p0->p = V2(FL(1.0), FL(0.0));
+    p0->q = V2(FL(0.0), FL(1.0));
+    memset(&S->top, 0, sizeof(S->top));
+    memset(&S->bot, 0, sizeof(S->bot));
+    v3 t_o = RayToOceanT(p0->t, &S->org);
+    GetBdrySeg3(S->xs, t_o, &S->top, &Q->ati, 1, 1, err);
+    GetBdrySeg3(S->xs, t_o, &S->bot, &Q->bty, 0, 1, err);
+    real dt, db;
+    Distances3(S->xs, &S->top, &S->bot, &dt, &db);
+    if(dt <= FL(0.0) || db <= FL(0.0)) {
+        RunWarning(err, BHC_WARN_SOURCE_OUTSIDE_BOUNDARIES);
+        return 0;
+    }
+    S->iSmallStepCtr = 0;
+    return 1;
+}
+
+static int RayUpdateNx(const RayPt *p0, RayPt *p1, RayPt *p2, TraceStateNx *S, const Prob3 *Q, ErrState *err)
+{ /* trace.hpp:274-381 with O3D && !R3D, rectilinear boundaries */
+    int topRefl = 0, botRefl = 0;
+    StepNx(*p0, p1, S, Q, err, &topRefl, &botRefl);
+    v3 x_o = RayToOceanX(p1->x, &S->org), t_o = RayToOceanT(p1->t, &S->org);
+    GetBdrySeg3(x_o, t_o, &S->top, &Q->ati, 1, 0, err);
+    GetBdrySeg3(x_o, t_o, &S->bot, &Q->bty, 0, 0, err);
+    if(topRefl || botRefl) {
+        const BState3 *b = topRefl ? &S->top : &S->bot;
+        const HS *hs = topRefl ? &Q->P2.top : &Q->P2.bot;
+        ReflectNx(p1, p2, hs, topRefl, b->n, &S->org, Q, &S->iSeg, err);
+        return 1;
+    }
+    return 0;
+}
+
+static int RayTerminateNx(const RayPt *pt, int *Nsteps, int is, const TraceStateNx *S, int MaxPointsPerRay,
+                          const Prob3 *Q, ErrState *err)
+{ /* trace.hpp:391-477 with O3D && !R3D: the 3D box and grid extents on the ocean point */
+    v3 x = RayToOceanX(pt->x, &S->org), t = RayToOceanT(pt->t, &S->org), xs = S->xs;
+    int leftbox = RABS(x.x - xs.x) >= Q->box_x || RABS(x.y - xs.y) >= Q->box_y || RABS(x.z - RL(0.0)) >= Q->box_z;
+    const Bdry3 *bt = &Q->bty, *at = &Q->ati;
+    real minx = rmax(bt->bd[0].x.x, at->bd[0].x.x);
+    real miny = rmax(bt->bd[0].x.y, at->bd[0].x.y);
+    real maxx = rmin(bt->bd[(bt->nx - 1) * bt->ny].x.x, at->bd[(at->nx - 1) * at->ny].x.x);
+    real maxy = rmin(bt->bd[bt->ny - 1].x.y, at->bd[at->ny - 1].x.y);
+    int e0 = x.x < minx || x.y < miny;
+    int eN = x.x > maxx || x.y > maxy;
+    e0 = e0 || (x.x == minx && t.x < RL(0.0)) || (x.y == miny && t.y < RL(0.0));
+    eN = eN || (x.x == maxx && t.x > RL(0.0)) || (x.y == maxy && t.y > RL(0.0));
+    int small = S->iSmallStepCtr > 50;
+    int lost = pt->Amp < FL(0.005);
+    if(leftbox || lost || e0 || eN || small) {
+        *Nsteps = is + 1;
+        return 1;
+    } else if(is >= MaxPointsPerRay - 3) {
+        RunWarning(err, BHC_WARN_ONERAY_OUTOFMEMORY);
+        *Nsteps = is;
+        return 1;
+    }
+    return 0;
+}
+
+typedef struct {
+    Outs *O;
+    const Prob3 *Q;
+    ErrState *err;
+    _Atomic int32_t next;
+    int32_t job_begin, job_end;
+    RayStat3 *stats;
+} FieldJobNx;
+
+static void MainFieldModesNx(RayInit3Info *ri3, Outs *O, const Prob3 *Q, ErrState *err, RayStat3 *st)
+{ /* trace.hpp:525-589 with O3D && !R3D: the 2D influence functions on the
+   * bearing's receiver plane, field slot itheta = ibeta */
+    const Prob *P = &Q->P2;
+    TraceStateNx S;
+    RayPt p0, p1, p2;
+    InflRay inf;
+    memset(&p2, 0, sizeof(p2));
+    if(st) { st->nsteps = 0; st->ntop = st->nbot = 0; st->xend[0] = st->xend[1] = st->xend[2] = 0; }
+    if(!RayInitNx(ri3, &p0, &S, Q, err)) return;
+    RayInitInfo ri;
+    memset(&ri, 0, sizeof(ri));
+    ri.isz = ((ri3->isz * Q->NSx + ri3->isx) * Q->NSy + ri3->isy) * Q->Ntheta + ri3->ibeta;
+    ri.ialpha = ri3->ialpha;
+    ri.alpha = ri3->alpha;
+    ri.SrcDeclAngle = ri3->SrcDeclAngle;
+    ri.nx = 1;
+    ri.isx = ri3->isx;
+    ri.isy = ri3->isy;
+    ri.isz3 = ri3->isz;
+    ri.ibeta = ri3->ibeta;
+    ri.SrcAzimAngle = ri3->SrcAzimAngle;
+    Init_Influence(&inf, &p0, &ri, P, O->isGaussian, O->isRayCen);
+    int is = 0, Nsteps = 0;
+    for(;;) {
+        if(HasErrored(err)) break;
+        int two = RayUpdateNx(&p0, &p1, &p2, &S, Q, err);
+        if(O->isRayCen) Step_InfluenceGeoRayCen(&p0, &p1, &inf, is, O);
+        else Step_InfluenceGeoCart(&p0, &p1, &inf, is, O);
+        ++is;
+        if(two) {
+            if(O->isRayCen) Step_InfluenceGeoRayCen(&p1, &p2, &inf, is, O);
+            else Step_InfluenceGeoCart(&p1, &p2, &inf, is, O);
+            p0 = p2;
+            ++is;
+        } else {
+            p0 = p1;
+        }
+        if(RayTerminateNx(&p0, &Nsteps, is, &S, MaxN, Q, err)) break;
+    }
+    if(st) {
+        v3 xe = RayToOceanX(p0.x, &S.org);
+        st->nsteps = is;
+        st->ntop = p0.NumTopBnc;
+        st->nbot = p0.NumBotBnc;
+        st->xend[0] = (double)xe.x;
+        st->xend[1] = (double)xe.y;
+        st->xend[2] = (double)xe.z;
+    }
+}
+
+static void *field_workerNx(void *arg)
+{
+    FieldJobNx *J = (FieldJobNx *)arg;
+    for(;;) {
+        int32_t job = atomic_fetch_add(&J->next, 1);
+        if(job >= J->job_end) break;
+        RayInit3Info ri;
+        if(!job_indices3(&ri, job, J->Q)) break;
+        MainFieldModesNx(&ri, J->O, J->Q, J->err, J->stats ? &J->stats[job - J->job_begin] : NULL);
+    }
+    return NULL;
+}
+
+/* Exported: Nx2D field run (TL / arrivals / eigen hits) on jobs [job_begin,
+ * job_end) (job_end < 0: all); env->RunType[5] must be '2'. out->field is
+ * [NSz][NSx][NSy][Ntheta][NRz][NRr] complex floats; ray_xend is [rays][3]
+ * (ocean coordinates of the last point). */
+int SFX(oracle_field_runnx2d)(const bhc_env3d *env, bhc_field_out *out, int nthreads, int job_begin, int job_end)
+{
+    char rt0 = env->RunType[0], rt1 = env->RunType[1];
+    if(env->RunType[5] != '2') return -1;
+    if(rt0 != 'C' && rt0 != 'S' && rt0 != 'I' && rt0 != 'A' && rt0 != 'a' && rt0 != 'E' && rt0 != 'V' && rt0 != 'v')
+        return -1;
+    if(rt1 != 'G' && rt1 != '^' && rt1 != ' ' && rt1 != 'B' && rt1 != 'g' && rt1 != 'b') return -1;
+    Prob3 Q;
+    if(prob3_from_env(env, &Q)) return -1;
+    Outs O;
+    memset(&O, 0, sizeof(O));
+    O.P = &Q.P2;
+    O.field = out->field;
+    O.arr = out->arr;
+    O.narr = (_Atomic int32_t *)out->narr;
+    O.max_narr = out->max_narr;
+    O.allow_merging = out->allow_merging;
+    O.hits = out->hits;
+    O.neigen = (_Atomic int32_t *)out->neigen;
+    O.eigen_memsize = out->eigen_memsize;
+    int arr_run = rt0 == 'A' || rt0 == 'a' || rt0 == 'V' || rt0 == 'v';
+    int eig_run = rt0 == 'E' || rt0 == 'V' || rt0 == 'v';
+    if(arr_run && (!O.arr || !O.narr || O.max_narr < 1)) { prob3_free(&Q); return -1; }
+    if(eig_run && (!O.hits || !O.neigen)) { prob3_free(&Q); return -1; }
+    O.isGaussian = (rt1 == 'B' || rt1 == 'b');
+    O.isRayCen = (rt1 == 'g' || rt1 == 'b');
+    atomic_init(&O.contrib, 0);
+    ErrState err;
+    atomic_init(&err.error, 0u);
+    atomic_init(&err.warning, 0u);
+    atomic_init(&err.errCount, 0u);
+    atomic_init(&err.warnCount, 0u);
+    int nj = num_jobs3(&Q);
+    if(job_end < 0 || job_end > nj) job_end = nj;
+    if(job_begin < 0) job_begin = 0;
+    int nrays = job_end - job_begin;
+    RayStat3 *stats = NULL;
+    if(out->ray_nsteps) stats = xcalloc(nrays > 0 ? nrays : 1, sizeof(RayStat3));
+    FieldJobNx J;
+    J.O = &O;
+    J.Q = &Q;
+    J.err = &err;
+    atomic_init(&J.next, job_begin);
+    J.job_begin = job_begin;
+    J.job_end = job_end;
+    J.stats = stats;
+    nthreads = resolve_threads(nthreads);
+    if(arr_run && O.allow_merging) nthreads = 1; /* src/mode/arr.hpp:79 */
+    pthread_t *th = xcalloc(nthreads, sizeof(pthread_t));
+    for(int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, field_workerNx, &J);
+    for(int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    if(stats) {
+        int64_t tot = 0;
+        for(int i = 0; i < nrays; ++i) {
+            out->ray_nsteps[i] = stats[i].nsteps;
+            if(out->ray_ntop) out->ray_ntop[i] = stats[i].ntop;
+            if(out->ray_nbot) out->ray_nbot[i] = stats[i].nbot;
+            if(out->ray_xend)
+                for(int k = 0; k < 3; ++k) out->ray_xend[3 * i + k] = stats[i].xend[k];
+            tot += stats[i].nsteps;
+        }
+        out->total_steps = tot;
+        free(stats);
+    }
+    out->total_contrib = atomic_load(&O.contrib);
+    out->err = atomic_load(&err.error);
+    out->warn = atomic_load(&err.warning);
+    out->err_count = atomic_load(&err.errCount);
+    out->warn_count = atomic_load(&err.warnCount);
+    prob3_free(&Q);
+    return 0;
+}
+
+static void MainRayModeNx(RayInit3Info *ri, RayPt *ray, int *Nsteps, int MaxPointsPerRay, const Prob3 *Q,
+                          ErrState *err, OriginNx *org)
+{ /* trace.hpp:482-520 with O3D && !R3D */
+    TraceStateNx S;
+    if(!RayInitNx(ri, &ray[0], &S, Q, err)) {
+        *Nsteps = 1;
+        org->xs = V3(Q->Sx[ri->isx], Q->Sy[ri->isy], (real)Q->P2.Sz[ri->isz]);
+        org->tradial = V2(RCOS(Q->beta[ri->ibeta]), RSIN(Q->beta[ri->ibeta]));
+        return;
+    }
+    *org = S.org;
+    int is = 0;
+    for(;;) {
+        if(HasErrored(err)) break;
+        int two = RayUpdateNx(&ray[is], &ray[is + 1], &ray[is + 2], &S, Q, err);
+        if(*Nsteps >= 0 && is >= *Nsteps) {
+            *Nsteps = is + 2;
+            break;
+        }
+        is += two ? 2 : 1;
+        if(RayTerminateNx(&ray[is], Nsteps, is, &S, MaxPointsPerRay, Q, err)) break;
+    }
+}
+
+/* Exported: the Nx2D ray run, every ray in job order (beams on the beta fan),
+ * points in ray coordinates (range along the bearing, depth) and per ray the
+ * origin (xs.x, xs.y, cos beta, sin beta) that maps them to the ocean. */
+int SFX(oracle_ray_runnx2d)(const bhc_env3d *env, bhc_ray_pt2d *points, int32_t *nsteps, double *origin,
+                            int max_points)
+{
+    if(env->RunType[5] != '2') return -1;
+    Prob3 Q;
+    if(prob3_from_env(env, &Q)) return -1;
+    ErrState err;
+    atomic_init(&err.error, 0u);
+    atomic_init(&err.warning, 0u);
+    atomic_init(&err.errCount, 0u);
+    atomic_init(&err.warnCount, 0u);
+    int nj = num_jobs3(&Q);
+    RayPt *buf = xcalloc((size_t)max_points + 3, sizeof(RayPt));
+    for(int k = 0; k < nj; ++k) {
+        RayInit3Info ri;
+        if(!job_indices3(&ri, k, &Q)) break;
+        int N = -1;
+        OriginNx org;
+        memset(buf, 0, sizeof(RayPt) * ((size_t)max_points + 3));
+        MainRayModeNx(&ri, buf, &N, max_points, &Q, &err, &org);
+        nsteps[k] = N;
+        if(origin) {
+            origin[4 * k] = (double)org.xs.x;
+            origin[4 * k + 1] = (double)org.xs.y;
+            origin[4 * k + 2] = (double)org.tradial.x;
+            origin[4 * k + 3] = (double)org.tradial.y;
+        }
+        int n = N < max_points ? N : max_points;
+        for(int i = 0; i < n; ++i) store_pt(&points[(size_t)k * max_points + i], &buf[i]);
+    }
+    free(buf);
+    prob3_free(&Q);
+    return 0;
+}
+
 int SFX(oracle_preprocess_dump3d)(const bhc_env3d *env, double *dst, int64_t cap, int64_t *used)
 {
     Prob3 Q;

@@ -153,6 +153,12 @@ def _lib3():
             f = getattr(L, "oracle_field_run3d_" + sfx)
             f.restype = C.c_int
             f.argtypes = [C.POINTER(A.Env3D_C), C.POINTER(A.FieldOut), C.c_int, C.c_int, C.c_int]
+            f = getattr(L, "oracle_field_runnx2d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(A.Env3D_C), C.POINTER(A.FieldOut), C.c_int, C.c_int, C.c_int]
+            f = getattr(L, "oracle_ray_runnx2d_" + sfx)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(A.Env3D_C), C.POINTER(A.RayPt2D), A.c_int32_p, A.c_double_p, C.c_int]
             f = getattr(L, "oracle_preprocess_dump3d_" + sfx)
             f.restype = C.c_int
             f.argtypes = [C.POINTER(A.Env3D_C), A.c_double_p, C.c_int64, C.POINTER(C.c_int64)]
@@ -200,10 +206,11 @@ def field_run3d(env, nthreads=0, job_begin=0, job_end=-1, max_narr=0, allow_merg
     out.ray_ntop = nt.ctypes.data_as(_abi.c_int32_p)
     out.ray_nbot = nb.ctypes.data_as(_abi.c_int32_p)
     out.ray_xend = xe.ctypes.data_as(_abi.c_double_p)
-    f = getattr(L, "oracle_field_run3d_" + _sfx(env))
+    name = "oracle_field_runnx2d_" if env.nx2d else "oracle_field_run3d_"
+    f = getattr(L, name + _sfx(env))
     rc = f(C.byref(ec), C.byref(out), nthreads, job_begin, job_end)
     if rc != 0:
-        raise RuntimeError("oracle_field_run3d failed (unsupported 3D environment)")
+        raise RuntimeError(name + " failed (unsupported environment)")
     r.field = fld.view(np.complex64).reshape(env.field_shape)
     r.nsteps, r.ntop, r.nbot, r.xend = ns[:n], nt[:n], nb[:n], xe[:n]
     r.total_steps, r.total_contrib = out.total_steps, out.total_contrib
@@ -213,10 +220,27 @@ def field_run3d(env, nthreads=0, job_begin=0, job_end=-1, max_narr=0, allow_merg
 
 def scale_pressure3d(env, field):
     """ScalePressure for 3D geometric beams (src/influence.hpp:1556-1627):
-    sqrt of the intensity for incoherent / semicoherent runs."""
+    sqrt of the intensity for incoherent / semicoherent runs. Nx2D runs
+    (R3D = false) then take the 2D cylindrical spreading -1 / sqrt(|r|) per
+    receiver range (0 at r = 0, -4 sqrt(pi) for a line source)."""
     f = np.array(field, np.complex64, copy=True)
     if env.run_type[0] != "C":
         f = np.sqrt(f.real.astype(np.float32)).astype(np.complex64)
+    if env.nx2d:
+        line = (env.run_type + " " * 6)[3] == "X"
+        if env.precision == 64:
+            r = np.asarray(env.Rr_km, np.float64) * 1000.0
+            fac = np.where(np.abs(r) < np.finfo(np.float64).eps, 0.0, -1.0 / np.sqrt(np.abs(r)))
+            if line:
+                fac[:] = np.float64(-4.0 * np.float32(np.sqrt(np.float32(3.14159265))) * -1.0)
+        else:
+            r = (np.asarray(env.Rr_km, np.float32) * np.float32(1000.0)).astype(np.float32)
+            with np.errstate(divide="ignore"):
+                fac = np.where(np.abs(r) < np.finfo(np.float32).eps, np.float32(0.0),
+                               np.float32(-1.0) / np.sqrt(np.abs(r))).astype(np.float32)
+            if line:
+                fac[:] = np.float32(-4.0) * np.sqrt(np.float32(3.14159265)) * np.float32(-1.0)
+        f = (f * fac.astype(np.float32)).astype(np.complex64)
     return f
 
 
@@ -280,3 +304,20 @@ def arrivals_with_log(env, max_narr, allow_merging=1, cap=1 << 20):
     if n > cap:
         raise RuntimeError("arrivals log overflow: %d calls" % n)
     return res, buf[:n].copy()
+
+
+def ray_runnx2d(env, max_points=4000):
+    """Nx2D ray run of the oracle: (points [rays, max_points] of bhc_ray_pt2d in
+    ray coordinates, nsteps [rays], origin [rays, 4] = xs.x, xs.y, cos beta,
+    sin beta)."""
+    L = _lib3()
+    from bellhopcuda_amd import _abi as A
+    f = getattr(L, "oracle_ray_runnx2d_" + _sfx(env))
+    ec, keep = env.to_c()
+    n = env.nrays
+    pts = (A.RayPt2D * (n * max_points))()
+    ns = np.zeros(n, np.int32)
+    org = np.zeros((n, 4), np.float64)
+    if f(C.byref(ec), pts, ns.ctypes.data_as(A.c_int32_p), org.ctypes.data_as(A.c_double_p), max_points) != 0:
+        raise RuntimeError("oracle_ray_runnx2d failed")
+    return pts, ns, org
