@@ -106,6 +106,10 @@ def lib():
         "bhc_problem3d_free": (None, [C.c_void_p]),
         "bhc_host3d_last_error": (C.c_char_p, []),
         "bhc_field_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut)]),
+        "bhc_field_runnx2d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut)]),
+        "bhc_field_runnx2d_async": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.FieldOut), COMPLETED_FN,
+                                              C.c_void_p]),
+        "bhc_ray_runnx2d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.RayOut), _abi.c_double_p]),
         "bhc_scale_pressure3d": (C.c_int, [P(_abi.Problem3D), _abi.c_float_p]),
         "bhc_opcount3d": (C.c_int, [P(_abi.Problem3D), C.c_int, P(C.c_double)]),
         "bhc_ray_run3d": (C.c_int, [C.c_void_p, P(_abi.Problem3D), P(_abi.RayOut3D)]),
@@ -292,7 +296,10 @@ class AsyncRun:
             if on_done is not None:
                 on_done()
         self._cb = COMPLETED_FN(done)      # kept alive until the run has ended
-        f = lib().bhc_field_run3d_async if isinstance(prob, Problem3D) else lib().bhc_field_run2d_async
+        if isinstance(prob, Problem3D):
+            f = lib().bhc_field_runnx2d_async if prob.nx2d else lib().bhc_field_run3d_async
+        else:
+            f = lib().bhc_field_run2d_async
         _check(f(ctx._h, C.byref(prob.view), C.byref(self.out), self._cb, None))
 
     def wait(self):
@@ -472,6 +479,11 @@ class Problem3D:
     def run_type(self):
         return bytes(self.view.RunType).decode("latin-1")
 
+    @property
+    def nx2d(self):
+        """Nx2D problem (RunType[5] '2'): 2D rays on the bearing planes."""
+        return self.run_type[5] == "2"
+
     def scale_pressure(self, field):
         f = np.ascontiguousarray(field, np.complex64).copy()
         if lib().bhc_scale_pressure3d(C.byref(self.view), f.ctypes.data_as(_abi.c_float_p)) != 0:
@@ -526,10 +538,38 @@ def field_run3d(env_or_prob, ctx=None, stats=False, max_narr=0, allow_merging=1,
     try:
         res = FieldResult()
         out = _make_out3d(prob, res, stats, max_narr, allow_merging, eigen_memsize)
-        _check(lib().bhc_field_run3d(ctx._h, C.byref(prob.view), C.byref(out)))
+        f = lib().bhc_field_runnx2d if prob.nx2d else lib().bhc_field_run3d
+        _check(f(ctx._h, C.byref(prob.view), C.byref(out)))
         res.total_steps, res.total_contrib = out.total_steps, out.total_contrib
         res.err, res.warn = out.err, out.warn
         return res
+    finally:
+        if own:
+            ctx.close()
+
+
+def ray_runnx2d(env_or_prob, ctx=None, max_points=4000):
+    """Nx2D ray run type on the GPU (bhc_ray_runnx2d). Returns (problem,
+    points [rays * max_points] of bhc_ray_pt2d in ray coordinates (range
+    along the bearing, depth), nsteps [rays], origin [rays, 4] = source x, y
+    and cos, sin of the bearing)."""
+    prob = env_or_prob if isinstance(env_or_prob, Problem3D) else Problem3D(env_or_prob)
+    if not prob.nx2d:
+        raise BhcError("ray_runnx2d: not an Nx2D problem (run type character 6 must be '2')")
+    own = ctx is None
+    if own:
+        ctx = Context(0)
+    try:
+        n = prob.nrays
+        pts = (_abi.RayPt2D * (n * max_points))()
+        ns = np.zeros(n, np.int32)
+        org = np.zeros((n, 4), np.float64)
+        out = _abi.RayOut()
+        out.points = C.cast(pts, C.POINTER(_abi.RayPt2D))
+        out.nsteps = ns.ctypes.data_as(_abi.c_int32_p)
+        out.max_points_per_ray = max_points
+        _check(lib().bhc_ray_runnx2d(ctx._h, C.byref(prob.view), C.byref(out), org.ctypes.data_as(_abi.c_double_p)))
+        return prob, pts, ns, org
     finally:
         if own:
             ctx.close()

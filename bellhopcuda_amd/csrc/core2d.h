@@ -197,6 +197,10 @@ template<typename real> struct Prob2D {
     int32_t pad;
     int32_t *ray_stats;    // batched runs: per-ray steps, top / bottom bounces (or null)
     double *ray_xend;      // batched runs: per-ray final (r, z) (or null)
+    // Nx2D runs (nx_ntheta > 0): the source slot isz of a segment is the field
+    // slot ((isz3 * NSx + isx) * NSy + isy) * Ntheta + itheta of its bearing
+    // plane; o_beta: the bearings (rad) for the arrivals' azimuths
+    int32_t nx_ntheta, nx_nsx, nx_nsy, o_beta;
 };
 
 // ---------------------------------------------------------------------------

@@ -274,8 +274,10 @@ template<typename real> __host__ __device__ inline void set3(v3<real> &a, int d,
 
 template<typename real>
 __host__ __device__ inline void DepthInterfaceCrossing3(bool stepTo, real &hInt, v3<real> &x, v3<real> x0,
-                                                        v3<real> u, int iz, const Prob3D<real> &P)
-{ // step.hpp:41-80 (3D: the top and bottom layers extrapolate)
+                                                        v3<real> u, int iz, const Prob3D<real> &P,
+                                                        int *snap = nullptr)
+{ // step.hpp:41-80 (3D: the top and bottom layers extrapolate); snap: the
+  // snapDim of OceanToRayX (Nx2D), set where the step is cut
     if(!stepTo) hInt = RC<real>::MAXV;
     if(std::fabs(u.z) > RC<real>::EPS) {
         const real *z = P.base + P.o_z3;
@@ -284,12 +286,14 @@ __host__ __device__ inline void DepthInterfaceCrossing3(bool stepTo, real &hInt,
             if(stepTo) {
                 x = x0 + sc3(hInt, u);
                 x.z = z[iz];
+                if(snap) *snap = 2;
             }
         } else if(z[iz + 1] < x.z && iz + 1 < P.nz3 - 1) {
             hInt = (z[iz + 1] - x0.z) / u.z;
             if(stepTo) {
                 x = x0 + sc3(hInt, u);
                 x.z = z[iz + 1];
+                if(snap) *snap = 2;
             }
         }
     }
@@ -297,7 +301,7 @@ __host__ __device__ inline void DepthInterfaceCrossing3(bool stepTo, real &hInt,
 
 template<typename real>
 __host__ __device__ inline void BeamBoxCrossing3(bool stepTo, int dim, real &hBox, v3<real> &x, v3<real> x0,
-                                                 v3<real> u, const Prob3D<real> &P, v3<real> xs)
+                                                 v3<real> u, const Prob3D<real> &P, v3<real> xs, int *snap = nullptr)
 { // step.hpp:85-111, common_run.hpp:33-47
     if(!stepTo) hBox = RC<real>::MAXV;
     v3<real> ctr = V3(xs.x, xs.y, (real)RL(0.0));
@@ -308,13 +312,14 @@ __host__ __device__ inline void BeamBoxCrossing3(bool stepTo, int dim, real &hBo
         if(stepTo) {
             x = x0 + sc3(hBox, u);
             set3(x, dim, cmp3(ctr, dim) + std::copysign(box, d0));
+            if(snap) *snap = dim;
         }
     }
 }
 
 template<typename real>
 __host__ __device__ inline void TopBotCrossing3(bool stepTo, real &h, const BState3<real> &bd, v3<real> &x,
-                                                v3<real> x0, v3<real> u, bool &refl)
+                                                v3<real> x0, v3<real> u, bool &refl, int *snap = nullptr)
 { // step.hpp:116-151
     if(!stepTo) h = RC<real>::MAXV;
     v3<real> d = x - bd.x, d0 = x0 - bd.x;
@@ -324,6 +329,7 @@ __host__ __device__ inline void TopBotCrossing3(bool stepTo, real &h, const BSta
         if(stepTo) {
             x = x0 + sc3(h, u);
             if(std::fabs(bd.n.x) < RC<real>::EPS && std::fabs(bd.n.y) < RC<real>::EPS) x.z = bd.x.z;
+            if(snap) *snap = 2;
         }
         refl = true;
     } else {
@@ -334,7 +340,8 @@ __host__ __device__ inline void TopBotCrossing3(bool stepTo, real &h, const BSta
 template<typename real>
 __host__ __device__ inline void TopBotSegCrossing3(bool stepTo, bool isY, real &hSeg, const Trace3<real> &S,
                                                    int iSeg, v3<real> &x, v3<real> x0, v3<real> u,
-                                                   const Prob3D<real> &P, bool &topRefl, bool &botRefl)
+                                                   const Prob3D<real> &P, bool &topRefl, bool &botRefl,
+                                                   int *snap = nullptr)
 { // step.hpp:157-207
     real lmin = rmax(isY ? S.top.ymin : S.top.xmin, isY ? S.bot.ymin : S.bot.xmin);
     real lmax = rmin(isY ? S.top.ymax : S.top.xmax, isY ? S.bot.ymax : S.bot.xmax);
@@ -352,6 +359,7 @@ __host__ __device__ inline void TopBotSegCrossing3(bool stepTo, bool isY, real &
                 x = x0 + sc3(hSeg, u);
                 if(isY) x.y = lmin;
                 else x.x = lmin;
+                if(snap) *snap = isY ? 1 : 0;
                 topRefl = botRefl = false;
             }
         } else if(xw > lmax) {
@@ -360,6 +368,7 @@ __host__ __device__ inline void TopBotSegCrossing3(bool stepTo, bool isY, real &
                 x = x0 + sc3(hSeg, u);
                 if(isY) x.y = lmax;
                 else x.x = lmax;
+                if(snap) *snap = isY ? 1 : 0;
                 topRefl = botRefl = false;
             }
         }
@@ -368,7 +377,8 @@ __host__ __device__ inline void TopBotSegCrossing3(bool stepTo, bool isY, real &
 
 template<typename real>
 __host__ __device__ inline void TriDiagCrossing(bool stepTo, real &h, BState3<real> &bd, v3<real> &x, v3<real> x0,
-                                                v3<real> u, bool &topRefl, bool &botRefl, Err &e)
+                                                v3<real> u, bool &topRefl, bool &botRefl, Err &e,
+                                                int *snap = nullptr)
 { // step.hpp:212-260
     if(!stepTo) h = RC<real>::MAXV;
     if(bd.onEdge) return;
@@ -391,6 +401,7 @@ __host__ __device__ inline void TriDiagCrossing(bool stepTo, real &h, BState3<re
         x = x0 + sc3(h, u);
         bd.justSteppedTo = true;
         bd.outgoingSide = newSide;
+        if(snap) *snap = -2;
         topRefl = botRefl = false;
     }
 }
@@ -425,27 +436,32 @@ __host__ __device__ inline void ReduceStep3(v3<real> x0, v3<real> u, const int i
 template<typename real>
 __host__ __device__ inline void StepToBdry3(v3<real> x0, v3<real> &x2, v3<real> u, real &h, bool &topRefl,
                                             bool &botRefl, const int iSeg0[3], Trace3<real> &S, const Prob3D<real> &P,
-                                            Err &e)
-{ // step.hpp:342-420
+                                            Err &e, int *snapDim = nullptr)
+{ // step.hpp:342-420; snapDim (Nx2D): which ocean coordinate the step end
+  // was put on exactly (common_run.hpp:243-252)
+    int snap = -1;
+    int *sp = snapDim ? &snap : nullptr;
     x2 = x0 + sc3(h, u);
-    DepthInterfaceCrossing3(true, h, x2, x0, u, iSeg0[2], P);
-    BeamBoxCrossing3(true, 0, h, x2, x0, u, P, S.xs);
-    BeamBoxCrossing3(true, 1, h, x2, x0, u, P, S.xs);
-    BeamBoxCrossing3(true, 2, h, x2, x0, u, P, S.xs);
-    TopBotCrossing3(true, h, S.top, x2, x0, u, topRefl);
-    TopBotCrossing3(true, h, S.bot, x2, x0, u, botRefl);
+    DepthInterfaceCrossing3(true, h, x2, x0, u, iSeg0[2], P, sp);
+    BeamBoxCrossing3(true, 0, h, x2, x0, u, P, S.xs, sp);
+    BeamBoxCrossing3(true, 1, h, x2, x0, u, P, S.xs, sp);
+    BeamBoxCrossing3(true, 2, h, x2, x0, u, P, S.xs, sp);
+    TopBotCrossing3(true, h, S.top, x2, x0, u, topRefl, sp);
+    TopBotCrossing3(true, h, S.bot, x2, x0, u, botRefl, sp);
     if(botRefl) topRefl = false;
-    TopBotSegCrossing3(true, false, h, S, iSeg0[0], x2, x0, u, P, topRefl, botRefl);
-    TopBotSegCrossing3(true, true, h, S, iSeg0[1], x2, x0, u, P, topRefl, botRefl);
-    TriDiagCrossing(true, h, S.top, x2, x0, u, topRefl, botRefl, e);
-    TriDiagCrossing(true, h, S.bot, x2, x0, u, topRefl, botRefl, e);
+    TopBotSegCrossing3(true, false, h, S, iSeg0[0], x2, x0, u, P, topRefl, botRefl, sp);
+    TopBotSegCrossing3(true, true, h, S, iSeg0[1], x2, x0, u, P, topRefl, botRefl, sp);
+    TriDiagCrossing(true, h, S.top, x2, x0, u, topRefl, botRefl, e, sp);
+    TriDiagCrossing(true, h, S.bot, x2, x0, u, topRefl, botRefl, e, sp);
     if(h < RC<real>::INF_STEP * P.deltas) {
         h = RC<real>::INF_STEP * P.deltas;
         x2 = x0 + sc3(h, u);
+        snap = -1;
         if(dot3(S.top.n, x2 - S.top.x) > RC<real>::EPS) {
             topRefl = true;
             S.top.justSteppedTo = false;
             S.bot.justSteppedTo = false;
+            snap = 2;
         } else {
             topRefl = false;
         }
@@ -454,10 +470,12 @@ __host__ __device__ inline void StepToBdry3(v3<real> x0, v3<real> &x2, v3<real> 
             topRefl = false;
             S.top.justSteppedTo = false;
             S.bot.justSteppedTo = false;
+            snap = 2;
         } else {
             botRefl = false;
         }
     }
+    if(snapDim) *snapDim = snap;
 }
 
 // RayNormal / RayNormal_unit, src/common_run.hpp:144-215
@@ -624,6 +642,48 @@ __host__ __device__ inline void Step3(const Ray3<real> &ray0, Ray3<real> &ray2, 
 // ---------------------------------------------------------------------------
 // Reflection (src/reflect.hpp:125-452, 3D, rectilinear boundaries)
 // ---------------------------------------------------------------------------
+// InterpolateReflectionCoefficient (reflect.hpp:39-92) at the incidence angle
+// |atan2(Th, Tg)| folded about 90 degrees; tab holds (theta, |R|, phase) rows.
+template<typename real>
+__host__ __device__ inline void ReflTableO3D(const real *tab, int n, real Th, real Tg, real &r, real &phi, Err &e)
+{
+    real theta = (RL(180.0) / RC<real>::PI) * std::fabs(std::atan2(Th, Tg));
+    if(theta > FL(90.0)) theta = FL(180.0) - theta;
+    int iL = 0, iR = n - 1;
+    if(theta < tab[0]) {
+        r = phi = FL(0.0);
+        e.warning(5);
+    } else if(theta > tab[3 * iR]) {
+        r = phi = FL(0.0);
+    } else {
+        while(iL != iR - 1) {
+            int iM = (iL + iR) / 2;
+            if(tab[3 * iM] > theta) iR = iM;
+            else iL = iM;
+        }
+        real a = (theta - tab[3 * iL]) / (tab[3 * iR] - tab[3 * iL]);
+        r = (FL(1.0) - a) * tab[3 * iL + 1] + a * tab[3 * iR + 1];
+        phi = (FL(1.0) - a) * tab[3 * iL + 2] + a * tab[3 * iR + 2];
+    }
+}
+
+// Fluid-halfspace reflection coefficient of the O3D branch (reflect.hpp:279-299),
+// used by 3D and Nx2D rays: Tg the tangential slowness, c and rho the water's.
+template<typename real>
+__host__ __device__ inline cpx<real> HalfspaceReflO3D(real omega, real Tg, real c, const HSd<real> &hs, real rho)
+{
+    cpx<real> gk = C(omega * Tg, (real)0);
+    cpx<real> gk2 = cmul(gk, gk);
+    real w1 = omega / c;
+    cpx<real> eps = C((real)0, (real)RC<real>::MINPOS); // J * REAL_MINPOS
+    cpx<real> g1sq = (C(w1 * w1, (real)0) - gk2) - eps;
+    cpx<real> wP = realdiv(omega, hs.cP);
+    cpx<real> g2sq = (cmul(wP, wP) - gk2) - eps;
+    cpx<real> g1 = csqrt_(-g1sq), g2 = csqrt_(-g2sq);
+    cpx<real> a = rmul(hs.rho, g1), b = rmul(rho, g2);
+    return cdiv(a - b, a + b);
+}
+
 template<typename real>
 __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &newp, const HSd<real> &hs, bool isTop,
                                          v3<real> nB, const Prob3D<real> &P, Trace3<real> &S, Err &e)
@@ -668,42 +728,13 @@ __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &new
     } else if(hs.bc == 'V') {
         newp.Amp = oldp.Amp;
         newp.Phase = oldp.Phase + RC<real>::PI;
-    } else if(hs.bc == 'F') { // InterpolateReflectionCoefficient reflect.hpp:39-92
-        const real *tab = P.base + (isTop ? P.o_trc : P.o_brc);
-        int n = isTop ? P.ntrc : P.nbrc;
-        real theta = (RL(180.0) / RC<real>::PI) * std::fabs(std::atan2(Th, Tg));
-        if(theta > FL(90.0)) theta = FL(180.0) - theta;
+    } else if(hs.bc == 'F') {
         real r, phi;
-        int iL = 0, iR = n - 1;
-        if(theta < tab[0]) {
-            r = phi = FL(0.0);
-            e.warning(5);
-        } else if(theta > tab[3 * iR]) {
-            r = phi = FL(0.0);
-        } else {
-            while(iL != iR - 1) {
-                int iM = (iL + iR) / 2;
-                if(tab[3 * iM] > theta) iR = iM;
-                else iL = iM;
-            }
-            real a = (theta - tab[3 * iL]) / (tab[3 * iR] - tab[3 * iL]);
-            r = (FL(1.0) - a) * tab[3 * iL + 1] + a * tab[3 * iR + 1];
-            phi = (FL(1.0) - a) * tab[3 * iL + 2] + a * tab[3 * iR + 2];
-        }
+        ReflTableO3D(P.base + (isTop ? P.o_trc : P.o_brc), isTop ? P.ntrc : P.nbrc, Th, Tg, r, phi, e);
         newp.Amp = oldp.Amp * r;
         newp.Phase = oldp.Phase + phi;
-    } else if(hs.bc == 'A' || hs.bc == 'G') { // fluid halfspace, O3D branch reflect.hpp:279-299
-        real omega = FL(2.0) * RC<real>::PI * P.freq0;
-        cpx<real> gk = C(omega * Tg, (real)0);
-        cpx<real> gk2 = cmul(gk, gk);
-        real w1 = omega / c;
-        cpx<real> eps = C((real)0, (real)RC<real>::MINPOS); // J * REAL_MINPOS
-        cpx<real> g1sq = (C(w1 * w1, (real)0) - gk2) - eps;
-        cpx<real> wP = realdiv(omega, hs.cP);
-        cpx<real> g2sq = (cmul(wP, wP) - gk2) - eps;
-        cpx<real> g1 = csqrt_(-g1sq), g2 = csqrt_(-g2sq);
-        cpx<real> a = rmul(hs.rho, g1), b = rmul(o.rho, g2);
-        cpx<real> Refl = cdiv(a - b, a + b);
+    } else if(hs.bc == 'A' || hs.bc == 'G') {
+        cpx<real> Refl = HalfspaceReflO3D(FL(2.0) * RC<real>::PI * P.freq0, Tg, c, hs, o.rho);
         real aR = cabs_(Refl);
         if(aR < FL(1.0e-5)) {
             newp.Amp = FL(0.0);

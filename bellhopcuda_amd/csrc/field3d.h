@@ -842,6 +842,7 @@ extern "C" int bhc_field_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_fiel
     if(check_idle(ctx, "bhc_field_run3d")) return -1;
     if(!prob || !out) return fail("bhc_field_run3d: null argument");
     if(prob->precision != 64 && prob->precision != 32) return fail("bhc_field_run3d: precision must be 32 or 64");
+    if(prob->RunType[5] == '2') return fail("bhc_field_run3d: an Nx2D problem (RunType[5] '2') runs with bhc_field_runnx2d");
     char r0 = prob->RunType[0], r1 = prob->RunType[1];
     if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'A' && r0 != 'a' && r0 != 'E' && r0 != 'V' && r0 != 'v')
         return fail("bhc_field_run3d: run type '%c' is not a TL, arrivals or eigenray run type", r0);
@@ -868,6 +869,7 @@ extern "C" int bhc_ray_run3d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_ou
 {
     if(check_idle(ctx, "bhc_ray_run3d")) return -1;
     if(!prob || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d: null argument");
+    if(prob->RunType[5] == '2') return fail("bhc_ray_run3d: an Nx2D problem (RunType[5] '2') runs with bhc_ray_runnx2d");
     if(prob->NSx < 1 || prob->NSy < 1 || prob->NSz < 1 || prob->Nalpha < 1 || prob->Nbeta < 1)
         return fail("bhc_ray_run3d: empty source / angle set");
     if(!(prob->deltas > 0)) return fail("bhc_ray_run3d: deltas must be positive");
@@ -886,6 +888,7 @@ extern "C" int bhc_ray_run3d_list(bhc_ctx *ctx, const bhc_problem3d *prob, const
     if(check_idle(ctx, "bhc_ray_run3d_list")) return -1;
     if(!prob || !hits || !out || !out->points || !out->nsteps) return fail("bhc_ray_run3d_list: null argument");
     if(nhits < 1) return fail("bhc_ray_run3d_list: no hits");
+    if(prob->RunType[5] == '2') return fail("bhc_ray_run3d_list: Nx2D problems are not supported");
     for(int k = 0; k < nhits; ++k) {
         const bhc_eigen_hit &h = hits[k];
         if(h.ialpha < 0 || h.ialpha >= prob->Nalpha || h.ibeta < 0 || h.ibeta >= prob->Nbeta || h.isx < 0
@@ -957,6 +960,17 @@ extern "C" int bhc_field_run3d_async(bhc_ctx *ctx, const bhc_problem3d *prob, bh
         * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta) * prob->NSx * prob->NSy * prob->NSz;
     return start_async(ctx, "bhc_field_run3d_async", total, [ctx, prob, out]() { return bhc_field_run3d(ctx, prob, out); },
                        completed, user);
+}
+
+extern "C" int bhc_field_runnx2d_async(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out,
+                                       bhc_completed_fn completed, void *user)
+{
+    if(!prob || !out) return fail("bhc_field_runnx2d_async: null argument");
+    if(prob->RunType[5] != '2') return fail("bhc_field_runnx2d_async: not an Nx2D problem (RunType[5] must be '2')");
+    const long long total = (long long)(prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha)
+        * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta) * prob->NSx * prob->NSy * prob->NSz;
+    return start_async(ctx, "bhc_field_runnx2d_async", total,
+                       [ctx, prob, out]() { return bhc_field_runnx2d(ctx, prob, out); }, completed, user);
 }
 
 extern "C" int bhc_ctx_busy(bhc_ctx *ctx) { return ctx && ctx->busy.load() ? 1 : 0; }

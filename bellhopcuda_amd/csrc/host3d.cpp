@@ -126,6 +126,7 @@ template<typename real> int preprocess3d(const bhc_env3d *e, bhc_problem3d_owner
     memset(&v, 0, sizeof(v));
     v.precision = e->precision;
     memcpy(v.RunType, p.RunType, 7);
+    v.RunType[5] = e->RunType[5] == '2' ? '2' : '3';   // Nx2D or full 3D
     memcpy(v.BeamType, p.BeamType, 4);
     v.ssp_type = e->ssp_type;
     v.freq0 = p.freq0;
@@ -217,16 +218,22 @@ template<typename real> int preprocess3d(const bhc_env3d *e, bhc_problem3d_owner
     v.alpha_cos = p.alpha_cos;
     v.alpha_sin = p.alpha_sin;
     v.Dalpha = p.Dalpha;
-    v.Nbeta = e->Nbeta;
+    // an Nx2D run that is not a ray run traces its beams on the receiver
+    // bearings (rayangles.hpp:163-178)
+    const bool on_bearings = e->RunType[5] == '2' && e->RunType[0] != 'R';
+    const int nb = on_bearings ? e->Ntheta : e->Nbeta;
+    const double *bdeg = on_bearings ? e->theta_deg : e->beta_deg;
+    v.Nbeta = nb;
     v.iSingle_beta = e->iSingle_beta;
-    std::vector<real> b(e->Nbeta);
-    for(int i = 0; i < e->Nbeta; ++i) b[i] = (real)e->beta_deg[i] * DegRad;
-    for(int i = 1; i < e->Nbeta; ++i)
+    if(v.iSingle_beta > nb) return fail3("iSingle_beta out of range");
+    std::vector<real> b(nb);
+    for(int i = 0; i < nb; ++i) b[i] = (real)bdeg[i] * DegRad;
+    for(int i = 1; i < nb; ++i)
         if(!(b[i] > b[i - 1])) return fail3("bearing launch angles must increase");
     real Db = (real)0.0f;
-    if(e->Nbeta != 1) Db = (b[e->Nbeta - 1] - b[0]) / (real)(e->Nbeta - 1);
+    if(nb != 1) Db = (b[nb - 1] - b[0]) / (real)(nb - 1);
     v.Dbeta = (double)Db;
-    for(int i = 0; i < e->Nbeta; ++i) {
+    for(int i = 0; i < nb; ++i) {
         real s, c;
         sincos_(b[i], &s, &c);
         o->beta.push_back((double)b[i]);
@@ -332,14 +339,39 @@ extern "C" const char *bhc_host3d_last_error(void) { return g_err3.c_str(); }
 
 // ScalePressure (src/influence.hpp:1556-1627) for 3D geometric beams: no
 // spreading factor; incoherent and semicoherent intensities become pressures.
+// Nx2D runs (R3D = false) then take the 2D cylindrical spreading per receiver
+// range on every bearing plane, as bhc_scale_pressure2d.
 extern "C" int bhc_scale_pressure3d(const bhc_problem3d *p, float *field)
 {
     if(!p || !field) return fail3("null argument");
-    if(p->RunType[0] == 'C') return 0;
     size_t n = (size_t)p->NSz * p->NSx * p->NSy * p->Ntheta * p->NRz * p->NRr;
-    for(size_t k = 0; k < n; ++k) {
-        field[2 * k] = std::sqrt(field[2 * k]);
-        field[2 * k + 1] = 0.0f;
+    if(p->RunType[0] != 'C')
+        for(size_t k = 0; k < n; ++k) {
+            field[2 * k] = std::sqrt(field[2 * k]);
+            field[2 * k + 1] = 0.0f;
+        }
+    if(p->RunType[5] != '2') return 0;
+    const bool line = p->RunType[3] == 'X';
+    const size_t rows = n / (size_t)p->NRr;
+    for(int ir = 0; ir < p->NRr; ++ir) {
+        double factor;
+        if(p->precision == 64) {
+            double cnst = -1.0;
+            if(line) factor = -4.0f * std::sqrt(3.14159265f) * cnst;
+            else if(std::abs(p->Rr[ir]) < 2.220446049250313e-16) factor = 0.0;
+            else factor = cnst / std::sqrt(std::abs(p->Rr[ir]));
+        } else {
+            float cnst = -1.0f, r = (float)p->Rr[ir];
+            if(line) factor = -4.0f * std::sqrt(3.14159265f) * cnst;
+            else if(std::abs(r) < 1.1920929e-07f) factor = 0.0f;
+            else factor = cnst / std::sqrt(std::abs(r));
+        }
+        const float f = (float)factor;
+        for(size_t row = 0; row < rows; ++row) {
+            size_t a = row * p->NRr + ir;
+            field[2 * a] *= f;
+            field[2 * a + 1] *= f;
+        }
     }
     return 0;
 }

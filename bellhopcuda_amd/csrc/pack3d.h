@@ -131,5 +131,62 @@ int pack3(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &P, std:
     return 0;
 }
 
+// Nx2D: the 3D environment's blob (pack3) plus the receiver depths as reals,
+// and the 2D view of it that the 2D influence and scatter read -- receivers,
+// beam type, the field's source slots (NSz * NSx * NSy * Ntheta bearing
+// planes) and the decomposition of a slot for arrivals and eigen hits.
+template<typename real>
+int pack_nx(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &N, Prob2D<real> &S, std::string &err)
+{
+    if(pack3<real>(p, blob, N, err)) return -1;
+    memset(&S, 0, sizeof(S));
+    S.o_rzs = (int32_t)blob.size();
+    for(int i = 0; i < p->NRz; ++i) blob.push_back((real)p->Rz[i]);
+    if(blob.size() >= (size_t)0x7FFFFFFF) {
+        err = "Nx2D problem too large for 32-bit offsets";
+        return -1;
+    }
+    S.staged_reals = 0;
+    S.ssp_type = 'C';
+    S.run0 = p->RunType[0];
+    S.run1 = p->RunType[1];
+    S.run3 = p->RunType[3];
+    S.run4 = p->RunType[4];
+    S.type2 = p->BeamType[2];
+    S.type3 = p->BeamType[3];
+    S.o_Rr = N.o_Rr;
+    S.o_trc = N.o_trc;
+    S.o_brc = N.o_brc;
+    S.ntrc = N.ntrc;
+    S.nbrc = N.nbrc;
+    S.o_sbp = N.o_sbp;
+    S.nsbp = N.nsbp;
+    S.o_alpha = N.o_alpha;
+    S.o_acos = N.o_acos;
+    S.o_asin = N.o_asin;
+    S.NSz = p->NSz * p->NSx * p->NSy * p->Ntheta;
+    S.NRz = p->NRz;
+    S.NRr = p->NRr;
+    S.NRz_per_range = p->NRz;
+    S.Nalpha = p->Nalpha;
+    S.iSingle = p->iSingle_alpha;
+    S.Delta_r = (real)p->Delta_r;
+    S.Dalpha = (real)p->Dalpha;
+    S.freq0 = (real)p->freq0;
+    S.deltas = (real)p->deltas;
+    S.box_r = (real)p->box_x;
+    S.box_z = (real)p->box_z;
+    S.top = N.top;
+    S.bot = N.bot;
+    S.rz_sorted = N.rz_sorted;
+    S.rz_rdz = N.rz_rdz;
+    S.isGaussian = N.isGaussian;
+    S.isRayCen = N.isRayCen;
+    S.nx_ntheta = p->Ntheta;
+    S.nx_nsx = p->NSx;
+    S.nx_nsy = p->NSy;
+    S.o_beta = N.o_beta;
+    return 0;
+}
 
 } // namespace bhc3d

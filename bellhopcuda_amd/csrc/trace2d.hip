@@ -38,6 +38,8 @@
 #include "../../include/bhc_hip.h"
 #include "core2d.h"
 #include "pack2d.h"
+#include "nx2d.h"
+#include "pack3d.h"
 
 using namespace bhc2d;
 
@@ -154,6 +156,11 @@ template<typename real> struct DevSink {
         c.rcv = (float)rcv;
         c.srcaz = NAN;
         c.rcvaz = NAN;
+        if(P->nx_ntheta) { // Nx2D: RcvrAzimAngle = SrcAzimAngle = the bearing (influence.hpp:122-135)
+            float az = (float)((RL(180.0) / RC<real>::PI) * P->base[P->o_beta + isz % P->nx_ntheta]);
+            c.srcaz = az;
+            c.rcvaz = az;
+        }
         c.ntop = ntop;
         c.nbot = nbot;
         O->cand[idx * (size_t)O->cand_cap + slot] = c;
@@ -172,6 +179,14 @@ template<typename real> struct DevSink {
         h.isz = isz;
         h.ialpha = ialpha;
         h.ibeta = 0;
+        if(P->nx_ntheta) { // Nx2D: itheta = ibeta (influence.hpp:439) and the source indices
+            int r = isz / P->nx_ntheta;
+            h.itheta = h.ibeta = isz - r * P->nx_ntheta;
+            h.isy = r % P->nx_nsy;
+            r /= P->nx_nsy;
+            h.isx = r % P->nx_nsx;
+            h.isz = r / P->nx_nsx;
+        }
         O->hits[mi] = h;
     }
     __device__ void count() { ++nc; }
@@ -742,6 +757,167 @@ __global__ void __launch_bounds__(256) k_ray2d(Prob2D<real> P, DevOut O)
     flush_err(e, O);
 }
 
+// Nx2D trace kernel: the ray loop of MainFieldModes (src/trace.hpp:525-589)
+// with O3D && !R3D -- rays on the bearing planes of a 3D environment (nx2d.h),
+// influence state and segment descriptors of the 2D beams, so that the 2D
+// scatter kernels apply the receivers of every bearing plane. One wavefront
+// per workgroup; the 3D environment stays in global memory (L2-resident).
+template<typename real>
+__global__ void __launch_bounds__(64) k_tracenx2d(Prob2D<real> P, bhc3d::Prob3D<real> N, DevOut O)
+{
+    using bhc3d::OrgNx;
+    using bhc3d::Trace3;
+    const real *G = P.base;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int njobs = O.job_end - O.job_begin;
+    Err e = {0, 0, 0, 0};
+    RayPt<real> p0;
+    Trace3<real> S;
+    OrgNx<real> org;
+    Infl<real> I;
+    int job = 0, isf = 0, ia = 0, is = 0;
+    bool active = false, exhausted = false, init_fail = false;
+    unsigned long long steps = 0;
+    SegBlock blk{0u, 0u, 0u};
+    for(int iter = 0;; ++iter) {
+        if(!exhausted) {
+            unsigned long long need = __ballot(!active);
+            if(need) {
+                int cnt = __popcll(need);
+                int leader = __ffsll((unsigned long long)need) - 1;
+                int base = 0;
+                if(lane == leader) base = atomicAdd(O.work, cnt);
+                base = __shfl(base, leader);
+                if(base + cnt >= njobs) exhausted = true;
+                if(!active) {
+                    int my = base + __popcll(need & ((1ull << lane) - 1ull));
+                    if(my < njobs) {
+                        job = O.job_begin + my;
+                        int ib, isx, isy, isz;
+                        isf = bhc3d::JobIndicesNx(job, N, ia, ib, isx, isy, isz);
+                        is = 0;
+                        active = bhc3d::RayInitNx(ia, ib, isx, isy, isz, p0, S, org, I, P, N, e);
+                        init_fail = !active;
+                        if(!active && O.stats) {
+                            O.stats[3 * job] = O.stats[3 * job + 1] = O.stats[3 * job + 2] = 0;
+                            O.xend[3 * job] = O.xend[3 * job + 1] = O.xend[3 * job + 2] = 0.0;
+                        }
+                    }
+                }
+                note_done(O.prog, init_fail);
+                init_fail = false;
+            }
+        }
+        if(!__any(active)) {
+            if(exhausted) break;
+            continue;
+        }
+        if((iter & 31) == 31) {
+            // HasErrored (src/util/errors.hpp:48-51) and a full segment buffer
+            // end the pass for the whole wavefront
+            if(e.err) atomicOr(O.errw + 0, e.err);
+            uint32_t ge = __hip_atomic_load(O.errw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t of = __hip_atomic_load(O.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if(__any(ge != 0u || e.err != 0u || of != 0u)) {
+                exhausted = true;
+                active = false;
+                continue;
+            }
+        }
+        RayPt<real> p1, p2;
+        SegDesc<real> d;
+        d.nitems = 0;
+        bool two = false;
+        if(active) {
+            two = bhc3d::RayUpdateNx(p0, p1, p2, S, org, N, P, e);
+            prelude(d, p0, p1, I, is, P, G, G + P.o_rzs, isf, ia);
+            ++is;
+        }
+        append_seg(d, lane, O, blk);
+        if(__any(two)) {
+            d.nitems = 0;
+            if(active && two) {
+                prelude(d, p1, p2, I, is, P, G, G + P.o_rzs, isf, ia);
+                ++is;
+            }
+            append_seg(d, lane, O, blk);
+        }
+        bool fin = false;
+        if(active) {
+            p0 = two ? p2 : p1;
+            if(bhc3d::RayTerminateNx(p0, is, S, org, N, e, MAXN)) {
+                active = false;
+                fin = true;
+                steps += (unsigned long long)is;
+                if(O.stats) {
+                    const bhc3d::v3<real> xe = bhc3d::RayToOceanX(p0.x, org);
+                    O.stats[3 * job] = is;
+                    O.stats[3 * job + 1] = p0.NumTopBnc;
+                    O.stats[3 * job + 2] = p0.NumBotBnc;
+                    O.xend[3 * job] = (double)xe.x;
+                    O.xend[3 * job + 1] = (double)xe.y;
+                    O.xend[3 * job + 2] = (double)xe.z;
+                }
+            }
+        }
+        note_done(O.prog, fin);
+    }
+    seal_block<real>(blk, lane, O);
+    flush_err(e, O);
+    for(int off = WAVE / 2; off > 0; off >>= 1) steps += __shfl_xor(steps, off);
+    if(lane == 0) atomicAdd(O.counters + 0, steps);
+}
+
+// Nx2D ray run type: MainRayMode (src/trace.hpp:482-520) with O3D && !R3D.
+// Points are in ray coordinates (range along the bearing, depth); org_out
+// gets each ray's origin (xs.x, xs.y, cos beta, sin beta).
+template<typename real>
+__global__ void __launch_bounds__(256) k_raynx2d(Prob2D<real> P, bhc3d::Prob3D<real> N, DevOut O, double *org_out)
+{
+    Err e = {0, 0, 0, 0};
+    const int M = O.max_points;
+    for(int job = blockIdx.x * blockDim.x + threadIdx.x; job < O.job_end; job += gridDim.x * blockDim.x) {
+        int ia, ib, isx, isy, isz;
+        bhc3d::JobIndicesNx(job, N, ia, ib, isx, isy, isz);
+        bhc_ray_pt2d *ray = O.points + (size_t)job * M;
+        RayPt<real> p0, p1, p2;
+        bhc3d::Trace3<real> S;
+        bhc3d::OrgNx<real> org;
+        Infl<real> I;
+        int Nsteps = -1;
+        bool ok = bhc3d::RayInitNx(ia, ib, isx, isy, isz, p0, S, org, I, P, N, e);
+        store_pt(ray, p0);
+        org_out[4 * job] = (double)org.xs.x;
+        org_out[4 * job + 1] = (double)org.xs.y;
+        org_out[4 * job + 2] = (double)org.tr.x;
+        org_out[4 * job + 3] = (double)org.tr.y;
+        if(!ok) {
+            O.nsteps_out[job] = 1;
+            continue;
+        }
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            bool two = bhc3d::RayUpdateNx(p0, p1, p2, S, org, N, P, e);
+            store_pt(ray + is + 1, p1);
+            if(two) store_pt(ray + is + 2, p2);
+            is += two ? 2 : 1;
+            p0 = two ? p2 : p1;
+            int t = bhc3d::RayTerminateNx(p0, is, S, org, N, e, M);
+            if(t == 1) {
+                Nsteps = is + 1;
+                break;
+            }
+            if(t == 2) {
+                Nsteps = is;
+                break;
+            }
+        }
+        O.nsteps_out[job] = Nsteps;
+    }
+    flush_err(e, O);
+}
+
 // Deterministic arrivals: one thread per receiver sorts its candidates by
 // (ray, step) -- the order of the reference's single-threaded run -- and
 // replays AddArr (src/arrivals.hpp:45-130) on them.
@@ -942,6 +1118,11 @@ struct bhc_dev_problem {
     int nbins, bin_w, tile_w, tile_lo_off, chunk, max_chunks;
     int scatter_mode;           // SCATTER_AUTO / _TILED / _WAVE / _LANES
     int32_t *d_bins;            // counts | offsets | cursor | chunk_start | nchunks
+    // Nx2D runs: the 3D environment the trace kernel reads (P64 / P32 then view
+    // the same blob as the receiver side), and 3 reals per ray end point
+    int nx2d, xend_dim;
+    bhc3d::Prob3D<double> N64;
+    bhc3d::Prob3D<float> N32;
 };
 
 extern "C" const char *bhc_last_error(void) { return g_last_error.c_str(); }
@@ -1209,6 +1390,7 @@ extern "C" int bhc_problem_upload2d(bhc_ctx *ctx, const bhc_problem2d *prob, con
     dp->njobs = (prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * prob->NSz;
     dp->nrcv = (size_t)prob->NSz * prob->NRz_per_range * prob->NRr;
     dp->freq0 = prob->freq0;
+    dp->xend_dim = 2;
     int rc = 0;
     size_t blob_bytes = 0, staged = 0;
     if(prob->precision == 64) {
@@ -1371,6 +1553,10 @@ int ensure_segs(bhc_ctx *c)
     return 0;
 }
 
+template<typename real> const bhc3d::Prob3D<real> &nx_of(const bhc_dev_problem *dp);
+template<> const bhc3d::Prob3D<double> &nx_of<double>(const bhc_dev_problem *dp) { return dp->N64; }
+template<> const bhc3d::Prob3D<float> &nx_of<float>(const bhc_dev_problem *dp) { return dp->N32; }
+
 // Trace / scatter passes over the problem's rays. segbuf == null: the
 // context's segment buffer (allocated at first use); otherwise the given
 // buffer of segbytes, with the scatter restricted to the mappings that need
@@ -1417,7 +1603,9 @@ int run_passes(bhc_dev_problem *dp, const Prob2D<real> &P, DevOut O, void *segbu
         HIPCHECK(hipMemcpyAsync(&prog_start, c->d_prog, sizeof(prog_start), hipMemcpyDeviceToHost, s));
         if(first) HIPCHECK(hipEventRecord(c->ev0, s));
         HIPCHECK(hipEventRecord(c->pa, s));
-        switch(P.ssp_type) {
+        if(dp->nx2d) {
+            hipLaunchKernelGGL((k_tracenx2d<real>), dim3(tblocks), dim3(WAVE), 0, s, P, nx_of<real>(dp), O);
+        } else switch(P.ssp_type) {
         case 'C': hipLaunchKernelGGL((k_trace2d<real, 'C'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
         case 'S': hipLaunchKernelGGL((k_trace2d<real, 'S'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
         case 'P': hipLaunchKernelGGL((k_trace2d<real, 'P'>), dim3(tblocks), dim3(WAVE), dp->lds_bytes, s, P, O); break;
@@ -1634,7 +1822,8 @@ extern "C" int bhc_field_download(bhc_ctx *ctx, bhc_dev_problem *dp, bhc_field_o
             if(out->ray_nbot) out->ray_nbot[j] = st[3 * j + 2];
         }
         if(out->ray_xend)
-            HIPCHECK(hipMemcpy(out->ray_xend, dp->d_xend, 2 * (size_t)dp->njobs * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(out->ray_xend, dp->d_xend, (size_t)dp->xend_dim * dp->njobs * sizeof(double),
+                               hipMemcpyDeviceToHost));
     }
     unsigned long long cnt[2];
     uint32_t ew[4];
@@ -1732,6 +1921,211 @@ extern "C" int bhc_ray_run2d_list(bhc_ctx *ctx, const bhc_problem2d *prob, bhc_r
     void *fr[] = {d_ia, d_isz, d_nin, d_nout, d_pts};
     for(void *p : fr)
         if(p) (void)hipFree(p);
+    bhc_problem_release(dp);
+    if(!rc) ctx->prog_final = 1;
+    return rc;
+}
+
+// ===========================================================================
+// Nx2D runs (bellhopcxx Nx2D, RunType[5] == '2'): 2D rays on the bearing
+// planes of a 3D environment. The receiver side -- field, arrivals, eigen
+// hits, range bins, per-ray statistics, the trace / scatter passes and the
+// arrivals merge -- is the 2D run's over NSz * NSx * NSy * Ntheta source
+// slots (one per bearing plane); only the trace kernel differs.
+// ===========================================================================
+namespace {
+int upload_nx2d(bhc_ctx *ctx, const bhc_problem3d *p, const bhc_field_out *spec, char run0, bhc_dev_problem **out)
+{
+    *out = nullptr;
+    if(!p) return fail("null problem");
+    if(p->RunType[5] != '2') return fail("not an Nx2D problem (RunType[5] must be '2')");
+    if(p->precision != 64 && p->precision != 32) return fail("precision must be 32 or 64");
+    if(p->NSx < 1 || p->NSy < 1 || p->NSz < 1 || p->NRz < 1 || p->NRr < 1 || p->Ntheta < 1 || p->Nalpha < 1
+       || p->Nbeta < 1)
+        return fail("Nx2D problem: sources, receivers and launch angles must be non-empty");
+    if(p->RunType[4] == 'I') return fail("Nx2D problem: irregular receiver grids are not supported");
+    if(run0 != 'R' && p->Nbeta != p->Ntheta)
+        return fail("Nx2D problem: the beams must lie on the receiver bearings (Nbeta == Ntheta)");
+    if(p->iSingle_beta < 0 || p->iSingle_beta > p->Nbeta) return fail("iSingle_beta out of range");
+    const long long nsrc = (long long)p->NSz * p->NSx * p->NSy * p->Ntheta;
+    const long long nj = (long long)(p->iSingle_alpha >= 1 ? 1 : p->Nalpha) * (p->iSingle_beta >= 1 ? 1 : p->Nbeta)
+        * p->NSx * p->NSy * p->NSz;
+    if(nsrc * p->Nalpha >= (1ll << 30) || nj >= (1ll << 30)) return fail("Nx2D problem: too many rays");
+    // the 2D stand-in that sizes the receiver side: nsrc source slots, the
+    // problem's receivers, declinations and beam pattern; its profile and
+    // boundaries are placeholders (the trace kernel reads the 3D ones)
+    static const double z2[2] = {0.0, 1.0}, c2[4] = {1500.0, 0.0, 1500.0, 0.0}, zero4[4] = {0, 0, 0, 0};
+    static const double rho2[2] = {1.0, 1.0};
+    bhc_bdry_node2d nodes[2];
+    memset(nodes, 0, sizeof(nodes));
+    nodes[0].x[0] = -1.0e10;
+    nodes[1].x[0] = 1.0e10;
+    nodes[0].t[0] = nodes[1].t[0] = 1.0;
+    nodes[0].n[1] = nodes[1].n[1] = 1.0;
+    std::vector<float> sz((size_t)nsrc, 0.0f);
+    for(int i = 0; i < p->NSz; ++i) sz[i] = p->Sz[i];
+    bhc_problem2d q;
+    memset(&q, 0, sizeof(q));
+    q.precision = p->precision;
+    memcpy(q.RunType, p->RunType, 7);
+    q.RunType[0] = run0;
+    q.RunType[5] = ' ';
+    memcpy(q.BeamType, p->BeamType, 4);
+    q.BeamType[0] = p->RunType[1];
+    q.freq0 = p->freq0;
+    q.ssp_type = 'C';
+    q.ssp_npts = 2;
+    q.ssp_z = z2;
+    q.ssp_c = c2;
+    q.ssp_cz = zero4;
+    q.ssp_rho = rho2;
+    q.top_hs = p->top_hs;
+    q.bot_hs = p->bot_hs;
+    q.ati_type[0] = q.bty_type[0] = 'L';
+    q.ati_type[1] = q.bty_type[1] = 'S';
+    q.ati_npts = q.bty_npts = 2;
+    q.ati = nodes;
+    q.bty = nodes;
+    q.trc_npts = p->trc_npts;
+    q.brc_npts = p->brc_npts;
+    q.trc = p->trc;
+    q.brc = p->brc;
+    q.NSz = (int32_t)nsrc;
+    q.NRz = p->NRz;
+    q.NRr = p->NRr;
+    q.NRz_per_range = p->NRz;
+    q.Sz = sz.data();
+    q.Rz = p->Rz;
+    q.Rr = p->Rr;
+    q.Delta_r = p->Delta_r;
+    q.Nalpha = p->Nalpha;
+    q.iSingle_alpha = p->iSingle_alpha;
+    q.alpha = p->alpha;
+    q.alpha_cos = p->alpha_cos;
+    q.alpha_sin = p->alpha_sin;
+    q.Dalpha = p->Dalpha;
+    q.deltas = p->deltas;
+    q.box_r = p->box_x;
+    q.box_z = p->box_z;
+    q.NSBPPts = p->NSBPPts;
+    q.sbp = p->sbp;
+    bhc_dev_problem *dp = nullptr;
+    if(bhc_problem_upload2d(ctx, &q, spec, &dp)) return -1;
+    dp->njobs = (int)nj;
+    dp->nx2d = 1;
+    dp->xend_dim = 3;
+    int rc = 0;
+    if(dp->d_xend) {
+        (void)hipFree(dp->d_xend);
+        dp->d_xend = nullptr;
+        rc = dalloc(&dp->d_xend, 3 * (size_t)nj);
+    }
+    // the Nx2D blob replaces the stand-in's
+    (void)hipFree(dp->d_blob);
+    dp->d_blob = nullptr;
+    std::string err;
+    if(!rc && p->precision == 64) {
+        std::vector<double> blob;
+        rc = bhc3d::pack_nx<double>(p, blob, dp->N64, dp->P64, err);
+        if(rc) fail("%s", err.c_str());
+        if(!rc) rc = dalloc((double **)&dp->d_blob, blob.size());
+        if(!rc && hipMemcpy(dp->d_blob, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail("blob upload failed");
+    } else if(!rc) {
+        std::vector<float> blob;
+        rc = bhc3d::pack_nx<float>(p, blob, dp->N32, dp->P32, err);
+        if(rc) fail("%s", err.c_str());
+        if(!rc) rc = dalloc((float **)&dp->d_blob, blob.size());
+        if(!rc && hipMemcpy(dp->d_blob, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail("blob upload failed");
+    }
+    if(rc) {
+        bhc_problem_release(dp);
+        return -1;
+    }
+    dp->lds_bytes = 0;
+    dp->P64.base = dp->N64.base = dp->N64.s.base = (const double *)dp->d_blob;
+    dp->P32.base = dp->N32.base = dp->N32.s.base = (const float *)dp->d_blob;
+    dp->P64.Rz = dp->P32.Rz = dp->N64.Rz = dp->N32.Rz = dp->d_Rz;
+    dp->P64.Sz = dp->P32.Sz = dp->N64.Sz = dp->N32.Sz = dp->d_Sz;
+    dp->P64.field = dp->P32.field = dp->N64.field = dp->N32.field = dp->d_field;
+    dp->P64.njobs = dp->P32.njobs = dp->N64.njobs = dp->N32.njobs = (int)nj;
+    *out = dp;
+    return 0;
+}
+} // namespace
+
+extern "C" int bhc_field_runnx2d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out)
+{
+    if(check_idle(ctx, "bhc_field_runnx2d")) return -1;
+    HIPCHECK(hipSetDevice(ctx->device));
+    const char r0 = prob ? prob->RunType[0] : 0;
+    if(r0 != 'C' && r0 != 'S' && r0 != 'I' && r0 != 'A' && r0 != 'a' && r0 != 'E' && r0 != 'V' && r0 != 'v')
+        return fail("bhc_field_runnx2d: run type '%c' is not a field run", r0);
+    if(!out) return fail("bhc_field_runnx2d: null output");
+    const char r1 = prob->RunType[1];
+    if((r1 == 'g' || r1 == 'b') && prob->NRr > 1 && !(prob->Delta_r > 0))
+        return fail("bhc_field_runnx2d: ray-centred beams need equally spaced receiver ranges");
+    bhc_dev_problem *dp = nullptr;
+    if(upload_nx2d(ctx, prob, out, r0, &dp)) return -1;
+    int rc = field_launch(ctx, dp, nullptr, 0);
+    if(!rc) rc = bhc_field_download(ctx, dp, out);
+    bhc_problem_release(dp);
+    return rc;
+}
+
+extern "C" int bhc_ray_runnx2d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_out *out, double *origin)
+{
+    if(check_idle(ctx, "bhc_ray_runnx2d")) return -1;
+    if(!out || !out->points || !out->nsteps || !origin) return fail("bhc_ray_runnx2d: null output");
+    if(out->max_points_per_ray < 3) return fail("max_points_per_ray must be >= 3");
+    HIPCHECK(hipSetDevice(ctx->device));
+    bhc_field_out spec;
+    memset(&spec, 0, sizeof(spec));
+    bhc_dev_problem *dp = nullptr;
+    if(upload_nx2d(ctx, prob, &spec, 'R', &dp)) return -1;
+    const int njobs = dp->njobs;
+    if(prog_begin(ctx, njobs)) {
+        bhc_problem_release(dp);
+        return -1;
+    }
+    DevOut O = make_out(dp);
+    O.job_begin = 0;
+    O.job_end = njobs;
+    O.max_points = out->max_points_per_ray;
+    int32_t *d_nout = nullptr;
+    double *d_org = nullptr;
+    bhc_ray_pt2d *d_pts = nullptr;
+    int rc = 0;
+    const size_t npts = (size_t)njobs * out->max_points_per_ray;
+    if(dalloc(&d_pts, npts) || dalloc(&d_nout, njobs) || dalloc(&d_org, 4 * (size_t)njobs)) rc = -1;
+    if(!rc) {
+        O.points = d_pts;
+        O.nsteps_out = d_nout;
+        hipMemsetAsync(dp->d_errw, 0, 4 * sizeof(uint32_t), ctx->stream);
+        hipMemsetAsync(d_pts, 0, npts * sizeof(bhc_ray_pt2d), ctx->stream);
+        int blocks = std::max(1, std::min((njobs + 255) / 256, ctx->num_cus * ctx->bpc));
+        if(dp->precision == 64)
+            hipLaunchKernelGGL(k_raynx2d<double>, dim3(blocks), dim3(256), 0, ctx->stream, dp->P64, dp->N64, O, d_org);
+        else
+            hipLaunchKernelGGL(k_raynx2d<float>, dim3(blocks), dim3(256), 0, ctx->stream, dp->P32, dp->N32, O, d_org);
+        if(hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = fail("Nx2D ray kernel failed");
+        } else {
+            hipMemcpy(out->points, d_pts, npts * sizeof(bhc_ray_pt2d), hipMemcpyDeviceToHost);
+            hipMemcpy(out->nsteps, d_nout, njobs * sizeof(int32_t), hipMemcpyDeviceToHost);
+            hipMemcpy(origin, d_org, 4 * (size_t)njobs * sizeof(double), hipMemcpyDeviceToHost);
+            uint32_t ew[4];
+            hipMemcpy(ew, dp->d_errw, sizeof(ew), hipMemcpyDeviceToHost);
+            out->err = ew[0];
+            out->warn = ew[1];
+            out->err_count = ew[2];
+            out->warn_count = ew[3];
+        }
+    }
+    void *fr[] = {d_nout, d_pts, d_org};
+    for(void *q : fr)
+        if(q) (void)hipFree(q);
     bhc_problem_release(dp);
     if(!rc) ctx->prog_final = 1;
     return rc;

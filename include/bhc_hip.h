@@ -569,8 +569,39 @@ int bhc_ray_run3d_list(bhc_ctx *ctx, const bhc_problem3d *prob, const bhc_eigen_
 int bhc_write_ray3d(const char *path, const bhc_problem3d *prob, const char *title, const bhc_ray_pt3d *pts,
                     const int32_t *nsteps, int32_t max_points, int32_t nrays);
 /* ScalePressure for 3D geometric beams (src/influence.hpp:1556-1627): incoherent
- * and semicoherent fields become sqrt(intensity); coherent fields are unchanged. */
+ * and semicoherent fields become sqrt(intensity); coherent fields are unchanged.
+ * An Nx2D problem (RunType[5] '2', R3D = false) then takes the 2D cylindrical
+ * spreading -1 / sqrt(|r|) of each receiver range on every bearing plane. */
 int bhc_scale_pressure3d(const bhc_problem3d *prob, float *field);
+
+/* ---- Nx2D: 2D rays on the bearing planes of a 3D environment --------------- */
+/* The reference's Nx2D mode (O3D = true, R3D = false; bellhopcxx -Nx2D,
+ * run<true, false> of include/bhc/bhc.hpp:308-316). The problem comes from
+ * bhc_preprocess3d of an environment with RunType[5] == '2'; for every run
+ * type but 'R' its launch bearings are then the receiver bearings
+ * (src/module/rayangles.hpp:163-178), so that ray ibeta lies on bearing plane
+ * itheta = ibeta (src/influence.hpp:439). Rays are 2D (range along the
+ * bearing, depth) and see the 3D boundaries and sound speed along their plane
+ * (Origin / RayToOceanX / OceanToRayX, src/common_run.hpp:221-317; Step
+ * src/step.hpp:716-765; SSP projection src/ssp.hpp:449-462; Reflect
+ * src/reflect.hpp:94-140). Geometric beams (G / B Cartesian, g / b
+ * ray-centred), rectilinear boundary grids.
+ *
+ * Field run (RunFieldModesImpl<CFG, true, false>, src/mode/fieldimpl.hpp:27-28):
+ * out->field is [NSz][NSx][NSy][Ntheta][NRz][NRr] complex floats, caller-zeroed
+ * (GetFieldAddr, src/common.hpp:393-405); arrivals carry SrcAzimAngle =
+ * RcvrAzimAngle = the bearing; eigen hits itheta = ibeta with isx, isy, isz;
+ * ray_xend holds [rays][3] ocean end points. Rays are numbered as in 3D. */
+int bhc_field_runnx2d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out);
+/* Non-blocking bhc_field_runnx2d, with the semantics of bhc_field_run2d_async. */
+int bhc_field_runnx2d_async(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_field_out *out, bhc_completed_fn completed,
+                            void *user);
+/* Nx2D ray run type (RunRayMode, src/mode/ray.cpp:167-188, MainRayMode
+ * src/trace.hpp:482-520 with O3D && !R3D): every ray of the beta fan in job
+ * order, points in ray coordinates (rayPt<false>); origin[4 * ray + 0..3] =
+ * source x, y (m) and cos, sin of the ray's bearing, which map a point to the
+ * ocean as (x + r cos, y + r sin, z) (RayToOceanX; the .ray writer's form). */
+int bhc_ray_runnx2d(bhc_ctx *ctx, const bhc_problem3d *prob, bhc_ray_out *out, double *origin);
 /* PostProcessArrivals for 3D (src/mode/arr.cpp:24-73 with R3D): counts clamped
  * to max_narr, amplitudes unscaled, maximum count per source (isz, isx, isy)
  * written to maxn[NSz * NSx * NSy]. */

@@ -354,3 +354,86 @@ extern "C" void core_depth_window(const float *Rz, int n, double rdz, double zlo
 {
     depth_window(Rz, n, rdz, zlo, zhi, *i0, *i1);
 }
+
+// ---------------------------------------------------------------------------
+// Nx2D: the product's Nx2D core (nx2d.h) ray by ray, with the 2D influence
+// descriptors and the 2D receiver loop of every bearing plane.
+// ---------------------------------------------------------------------------
+#include "../../bellhopcuda_amd/csrc/nx2d.h"
+
+namespace {
+template<typename real> int runnx(const bhc_problem3d *prob, bhc_field_out *out)
+{
+    using namespace bhc3d;
+    std::vector<real> blob;
+    Prob3D<real> N;
+    Prob2D<real> P;
+    std::string err;
+    if(pack_nx<real>(prob, blob, N, P, err)) {
+        fprintf(stderr, "%s\n", err.c_str());
+        return -1;
+    }
+    P.base = N.base = N.s.base = blob.data();
+    P.Rz = N.Rz = prob->Rz;
+    P.Sz = N.Sz = prob->Sz;
+    const real *G = blob.data();
+    HostSink sink{out->field};
+    Err e = {0, 0, 0, 0};
+    int njobs = (prob->iSingle_alpha >= 1 ? 1 : prob->Nalpha) * (prob->iSingle_beta >= 1 ? 1 : prob->Nbeta)
+        * prob->NSx * prob->NSy * prob->NSz;
+    long long steps = 0;
+    for(int job = 0; job < njobs; ++job) {
+        int ia, ib, isx, isy, isz;
+        const int isf = JobIndicesNx(job, N, ia, ib, isx, isy, isz);
+        RayPt<real> p0, p1, p2;
+        Trace3<real> S;
+        OrgNx<real> org;
+        Infl<real> I;
+        if(!RayInitNx(ia, ib, isx, isy, isz, p0, S, org, I, P, N, e)) {
+            if(out->ray_nsteps) {
+                out->ray_nsteps[job] = out->ray_ntop[job] = out->ray_nbot[job] = 0;
+                for(int k = 0; k < 3; ++k) out->ray_xend[3 * job + k] = 0;
+            }
+            continue;
+        }
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            bool two = RayUpdateNx(p0, p1, p2, S, org, N, P, e);
+            SegDesc<real> d;
+            prelude(d, p0, p1, I, is, P, G, G + P.o_rzs, isf, ia);
+            seg_items<real, RT_TL, HostSink, false>(d, P, G, prob->Rz, sink);
+            ++is;
+            if(two) {
+                prelude(d, p1, p2, I, is, P, G, G + P.o_rzs, isf, ia);
+                seg_items<real, RT_TL, HostSink, false>(d, P, G, prob->Rz, sink);
+                ++is;
+            }
+            p0 = two ? p2 : p1;
+            if(RayTerminateNx(p0, is, S, org, N, e, 100000)) break;
+        }
+        steps += is;
+        if(out->ray_nsteps) {
+            const v3<real> xe = RayToOceanX(p0.x, org);
+            out->ray_nsteps[job] = is;
+            out->ray_ntop[job] = p0.NumTopBnc;
+            out->ray_nbot[job] = p0.NumBotBnc;
+            out->ray_xend[3 * job] = (double)xe.x;
+            out->ray_xend[3 * job + 1] = (double)xe.y;
+            out->ray_xend[3 * job + 2] = (double)xe.z;
+        }
+    }
+    out->total_steps = steps;
+    out->total_contrib = sink.nc;
+    out->err = e.err;
+    out->warn = e.warn;
+    out->err_count = e.nerr;
+    out->warn_count = e.nwarn;
+    return 0;
+}
+} // namespace
+
+extern "C" int core_field_runnx2d(const bhc_problem3d *prob, bhc_field_out *out)
+{
+    return prob->precision == 64 ? runnx<double>(prob, out) : runnx<float>(prob, out);
+}

@@ -239,3 +239,139 @@ def test_nx2d_arrivals_and_eigen_carry_the_bearing():
     assert set(np.unique(hits["itheta"])) == {0, 1}
     assert np.array_equal(hits["itheta"], hits["ibeta"])
     assert np.all(hits["isx"] == 0) and np.all(hits["isy"] == 0) and np.all(hits["isz"] == 0)
+
+
+# ---------------------------------------------------------------------------
+# The product's Nx2D core (bellhopcuda_amd/csrc/nx2d.h with the 2D influence
+# descriptors and receiver loop), run serially on the CPU by
+# tests/harness/core_cpu.cpp, against the oracle: bit for bit on trajectories,
+# bounces, end points and contribution counts.
+# ---------------------------------------------------------------------------
+import bellhopcuda_amd as bh
+from bellhopcuda_amd.env3d import c3_env
+from test_oracle3d import harness
+
+
+class _Res:
+    pass
+
+
+def core_runnx2d(env):
+    p = bh.Problem3D(env)
+    assert p.nx2d
+    H = harness()
+    if not getattr(H, "_nx", False):
+        H.core_field_runnx2d.restype = C.c_int
+        H.core_field_runnx2d.argtypes = [C.POINTER(_abi.Problem3D), C.POINTER(_abi.FieldOut)]
+        H._nx = True
+    n = p.nrays
+    r = _Res()
+    r.field = np.zeros(p.field_shape, np.complex64)
+    r.nsteps, r.ntop, r.nbot = (np.zeros(n, np.int32) for _ in range(3))
+    r.xend = np.zeros((n, 3))
+    out = _abi.FieldOut()
+    out.field = r.field.ctypes.data_as(_abi.c_float_p)
+    out.ray_nsteps = r.nsteps.ctypes.data_as(_abi.c_int32_p)
+    out.ray_ntop = r.ntop.ctypes.data_as(_abi.c_int32_p)
+    out.ray_nbot = r.nbot.ctypes.data_as(_abi.c_int32_p)
+    out.ray_xend = r.xend.ctypes.data_as(_abi.c_double_p)
+    assert H.core_field_runnx2d(C.byref(p.view), C.byref(out)) == 0
+    r.total_contrib, r.total_steps, r.err, r.warn = out.total_contrib, out.total_steps, out.err, out.warn
+    r.problem = p
+    return r
+
+
+def _rotated_munk(precision=64, rt="CB   2"):
+    e = munk_env3d(nalpha=151, nbeta=1, ntheta=1, nrz=31, nrr=81, run_type=rt, precision=precision)
+    e.theta_deg = np.array([0.0, 37.0, 90.0, 143.0, 200.0, 315.0])
+    e.Sx_km, e.Sy_km = np.array([3.0]), np.array([-2.0])
+    e.box_x_km = e.box_y_km = 30.0
+    return e
+
+
+NX_CASES = {
+    "munk_hat_bearing0": lambda: munk_env3d(nalpha=201, nbeta=1, ntheta=1, nrz=31, nrr=81, run_type="CG   2"),
+    "munk_gauss_rotated": lambda: _rotated_munk(),
+    "munk_gauss_rotated_f32": lambda: _rotated_munk(precision=32),
+    "munk_raycen_hat_rotated": lambda: _rotated_munk(rt="Cg   2"),
+    "munk_semicoherent_n2linear": lambda: munk_env3d(nalpha=151, nbeta=1, ntheta=1, nrz=31, nrr=81,
+                                                     run_type="SB   2", ssp_type="N"),
+    "wedge_x": lambda: _wedge_pair(0)[1],
+    "wedge_y": lambda: _wedge_pair(1)[1],
+    "shelf_canyon_hexahedral": lambda: c3_env(nalpha=61, nbeta=6, ntheta=6, nrz=21, nrr=51, nxy=41,
+                                              run_type="CB   2"),
+    "shelf_canyon_hexahedral_f32": lambda: c3_env(nalpha=61, nbeta=6, ntheta=6, nrz=21, nrr=51, nxy=41,
+                                                  run_type="CB   2", precision=32),
+    "shelf_canyon_raycen_incoherent": lambda: c3_env(nalpha=41, nbeta=6, ntheta=6, nrz=21, nrr=51, nxy=41,
+                                                     run_type="Ib   2"),
+}
+
+
+@pytest.mark.parametrize("name", list(NX_CASES))
+def test_core_nx2d_matches_oracle(name):
+    env = NX_CASES[name]()
+    o = oracle.field_run3d(env)
+    g = core_runnx2d(env)
+    assert np.array_equal(g.nsteps, o.nsteps)
+    assert np.array_equal(g.ntop, o.ntop) and np.array_equal(g.nbot, o.nbot)
+    assert np.array_equal(g.xend, o.xend)
+    assert g.total_contrib == o.total_contrib
+    assert g.err == o.err and g.warn == o.warn
+    to, tg = tl_db(o.field), tl_db(g.field)
+    m = to < 120
+    assert m.sum() > 100
+    assert np.max(np.abs(to - tg)[m]) < 1e-3
+
+
+def test_shelf_canyon_nx2d_exercises_snapping_and_diagonals():
+    """The sloping, canyoned grid makes the Nx2D rays cross cell edges and
+    diagonals off the bearing axes, so the OceanToRayX snap runs for x, y and
+    diagonal step ends; the rays still end in their bearing planes."""
+    env = NX_CASES["shelf_canyon_hexahedral"]()
+    o = oracle.field_run3d(env)
+    assert o.err == 0
+    assert o.nbot.sum() > 0
+    xe = o.xend.reshape(6, 61, 3)
+    for k, th in enumerate(env.theta_deg):
+        b = np.deg2rad(th)
+        assert np.max(np.abs(xe[k, :, 0] * np.sin(b) - xe[k, :, 1] * np.cos(b))) < 1e-6 * 40000.0
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_preprocess_nx2d_beams_on_receiver_bearings(precision):
+    env = c3_env(nalpha=21, nbeta=12, ntheta=6, nrz=11, nrr=21, nxy=31, precision=precision, run_type="CB   2")
+    p = bh.Problem3D(env)
+    v = p.view
+    assert p.nx2d and v.Nbeta == 6
+    ref = oracle.preprocess_dump3d(env)
+    got = np.array([v.beta[i] for i in range(v.Nbeta)])
+    # the dump holds nodes, hexahedral cz, bearings, beta, Sx, Sy, ...: locate beta after the bearings
+    nb = (v.bty_nx * v.bty_ny + v.ati_nx * v.ati_ny) * 9 + v.hex_nx * v.hex_ny * (v.hex_nz - 1) + 2 * v.Ntheta
+    assert np.array_equal(ref[nb:nb + 6], got)
+    dr = np.float64(np.pi) / 180.0 if precision == 64 else np.float32(np.pi) / np.float32(180.0)
+    assert np.allclose(got, np.asarray(env.theta_deg) * dr, rtol=1e-6)
+    # a ray run keeps the beta fan
+    env.run_type = "R    2"
+    assert bh.Problem3D(env).view.Nbeta == 12
+
+
+def test_scale_pressure_nx2d_matches_oracle():
+    for rt in ("CB   2", "IB   2"):
+        env = munk_env3d(nalpha=51, nbeta=1, ntheta=1, nrz=11, nrr=21, run_type=rt)
+        env.theta_deg = np.array([0.0, 45.0])
+        p = bh.Problem3D(env)
+        rng = np.random.default_rng(3)
+        f = (rng.random(p.field_shape) + 1j * rng.random(p.field_shape)).astype(np.complex64)
+        a = p.scale_pressure(f)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            b = oracle.scale_pressure3d(env, f)
+        assert np.array_equal(a, b)
+
+
+def test_abi_exports_nx2d_symbols():
+    L = bh.lib()
+    for name in ("bhc_field_runnx2d", "bhc_field_runnx2d_async", "bhc_ray_runnx2d"):
+        assert hasattr(L, name)
+    with pytest.raises(bh.BhcError):
+        bh.ray_runnx2d(munk_env3d(nalpha=3, nbeta=1, run_type="R    3"))
