@@ -287,6 +287,31 @@ def screen_sparse(mlp, thetas, envs, ref, spread_db=1.0, stride=(5, 8)):
     return rms_spread > spread_db, rms_spread
 
 
+def screened_answer(mlp, thetas, envs, ref, ctx, spread_db=1.0, stride=(5, 8)):
+    """The screened sweep's answer for environments `thetas` / `envs`: the
+    ensemble's spread on a receiver subsample decides, per environment,
+    between the network's full field (ensemble mean) and the batched tracer.
+    Returns (fields [n, nrz, nrr] in dB, fallback flags, spread RMS per
+    environment, seconds predicting, seconds tracing the fallbacks)."""
+    t0 = time.perf_counter()
+    fallback, rms_spread = screen_sparse(mlp, thetas, envs, ref, spread_db, stride)
+    shape = (len(envs[0].Rz), len(envs[0].Rr_km))
+    final = np.zeros((len(thetas),) + shape)
+    keep = [i for i in range(len(thetas)) if not fallback[i]]
+    if keep:
+        final[keep] = predict_fields(mlp, thetas[keep], [envs[i] for i in keep], ref).mean(0)
+    t_pred = time.perf_counter() - t0
+    fb = [i for i in range(len(thetas)) if fallback[i]]
+    t_fb = 0.0
+    if fb:
+        t0 = time.perf_counter()
+        f_fb, _, _ = trace_fields(thetas[fb], ctx)
+        t_fb = time.perf_counter() - t0
+        for k, i in enumerate(fb):
+            final[i] = f_fb[k]
+    return final, fallback, rms_spread, t_pred, t_fb
+
+
 def screen(pred, spread_db=1.0):
     """Per environment: ensemble mean and whether it falls back to the tracer
     (RMS over receivers of the members' spread above `spread_db`)."""

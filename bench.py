@@ -494,21 +494,7 @@ def run_surrogate(a):
     plain_fields_s = ns / t_plain
     # screened answer: the ensemble's spread on a receiver subsample decides,
     # per environment, between the network's full field and the tracer
-    t0 = time.perf_counter()
-    fallback, rms_spread = sg.screen_sparse(mlp, th_ev, e_ev, ref, 1.0)
-    keep = [i for i in range(len(th_ev)) if not fallback[i]]
-    final = np.zeros_like(truth)
-    if keep:
-        final[keep] = sg.predict_fields(mlp, th_ev[keep], [e_ev[i] for i in keep], ref).mean(0)
-    t_pred = time.perf_counter() - t0
-    fb = [i for i in range(len(th_ev)) if fallback[i]]
-    t_fb = 0.0
-    if fb:
-        t0 = time.perf_counter()
-        f_fb, _, _ = sg.trace_fields(th_ev[fb], ctx)
-        t_fb = time.perf_counter() - t0
-        for k, i in enumerate(fb):
-            final[i] = f_fb[k]
+    final, fallback, rms_spread, t_pred, t_fb = sg.screened_answer(mlp, th_ev, e_ev, ref, ctx, 1.0)
     err = (final - truth).reshape(-1)
     # diagnostics outside the timed region: the network's full-field answer everywhere
     mean = sg.predict_fields(mlp, th_ev, e_ev, ref).mean(0)
