@@ -139,7 +139,8 @@ class RayOut(C.Structure):
 
 class Bdry3DIn(C.Structure):
     _fields_ = [("type", C.c_char * 2), ("pad", C.c_char * 6), ("nx", C.c_int32), ("ny", C.c_int32),
-                ("x_km", c_double_p), ("y_km", c_double_p), ("z", c_double_p)]
+                ("x_km", c_double_p), ("y_km", c_double_p), ("z", c_double_p),
+                ("province", c_int32_p), ("nprov", C.c_int32), ("pad2", C.c_int32), ("prov", c_double_p)]
 
 
 class Env3D_C(C.Structure):
@@ -168,7 +169,10 @@ class Env3D_C(C.Structure):
 
 
 class BdryNode3D(C.Structure):
-    _fields_ = [("x", C.c_double * 3), ("n1", C.c_double * 3), ("n2", C.c_double * 3)]
+    _fields_ = [("x", C.c_double * 3), ("n1", C.c_double * 3), ("n2", C.c_double * 3), ("Noden", C.c_double * 3),
+                ("z_xx", C.c_double), ("z_xy", C.c_double), ("z_yy", C.c_double),
+                ("kappa_xx", C.c_double), ("kappa_xy", C.c_double), ("kappa_yy", C.c_double),
+                ("province", C.c_int32), ("pad", C.c_int32)]
 
 
 class Problem3D(C.Structure):
@@ -184,6 +188,8 @@ class Problem3D(C.Structure):
         ("top_hs", HS2D), ("bot_hs", HS2D),
         ("ati_nx", C.c_int32), ("ati_ny", C.c_int32), ("bty_nx", C.c_int32), ("bty_ny", C.c_int32),
         ("ati", C.POINTER(BdryNode3D)), ("bty", C.POINTER(BdryNode3D)),
+        ("ati_type", C.c_char * 2), ("bty_type", C.c_char * 2), ("ati_nprov", C.c_int32), ("bty_nprov", C.c_int32),
+        ("ati_prov", c_double_p), ("bty_prov", c_double_p),
         ("trc_npts", C.c_int32), ("brc_npts", C.c_int32), ("trc", c_double_p), ("brc", c_double_p),
         ("NSx", C.c_int32), ("NSy", C.c_int32), ("NSz", C.c_int32), ("NRz", C.c_int32),
         ("NRr", C.c_int32), ("Ntheta", C.c_int32),

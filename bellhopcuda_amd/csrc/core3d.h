@@ -88,6 +88,9 @@ template<> struct TD3<float> {
 // Boundary grid node in the blob: position, then the unit normals of the two
 // triangles of the cell whose lower corner it is.
 enum : int { N3_X = 0, N3_N1 = 3, N3_N2 = 6, NODE3 = 9 };
+// Per-node extras of curvilinear ('C') and province ('L') grids: node normal,
+// the cell's z and kappa curvatures, the cell's province (0-based, as a real).
+enum : int { NX_N = 0, NX_ZXX = 3, NX_ZXY = 4, NX_ZYY = 5, NX_KXX = 6, NX_KXY = 7, NX_KYY = 8, NX_PROV = 9, NODEX = 10 };
 
 // Device view of a 3D problem. Offsets index `base` (reals).
 template<typename real> struct Prob3D {
@@ -98,6 +101,8 @@ template<typename real> struct Prob3D {
     int32_t nz3, o_z3, o_rho3;          // depth table and densities (after SegZToZ for 'H')
     int32_t hx, hy, o_hx, o_hy, o_hc, o_hcz;
     int32_t ati_nx, ati_ny, bty_nx, bty_ny, o_ati, o_bty;
+    char ati_t0, ati_t1, bty_t0, bty_t1;    // grid types ('R' / 'C', 'S' / 'L')
+    int32_t o_atix, o_btyx, o_atip, o_btyp; // per-node extras [NODEX], province tables [5] (cP, cS, rho)
     int32_t o_Rr, o_trc, o_brc, ntrc, nbrc, o_sbp, nsbp;
     int32_t o_alpha, o_acos, o_asin, o_beta, o_bcos, o_bsin, o_Sx, o_Sy, o_trcv;
     const float *Rz, *Sz;
@@ -127,6 +132,7 @@ template<typename real> struct BState3 {
     real xmin, xmax, ymin, ymax;
     v3<real> x, xmid, n;
     bool onEdge, side, justSteppedTo, outgoingSide;
+    int32_t prov;           // 'L' grids: the current cell's province (CopyHSInfo, boundary.hpp:134-140)
 };
 
 template<typename real> struct SSPOut3 {
@@ -208,8 +214,8 @@ template<typename real> __host__ __device__ inline v3<real> ld3(const real *p) {
 
 template<typename real>
 __host__ __device__ inline void GetBdrySeg3(v3<real> x, v3<real> t, BState3<real> &b, const real *bd, int nx, int ny,
-                                            bool isTop, bool isInit, Err &e)
-{
+                                            bool isTop, bool isInit, Err &e, const real *prov = nullptr)
+{ // prov: the grid's per-node extras when it has provinces, else null
 #define BX(i) bd[(size_t)(i) * NODE3 + 0]
 #define BY(i) bd[(size_t)(i) * NODE3 + 1]
     b.ix = imin(imax(b.ix, 0), nx - 2);
@@ -246,6 +252,7 @@ __host__ __device__ inline void GetBdrySeg3(v3<real> x, v3<real> t, BState3<real
     const real *n0 = bd + ((size_t)b.ix * ny + b.iy) * NODE3;
     const real *n11 = bd + ((size_t)(b.ix + 1) * ny + (b.iy + 1)) * NODE3;
     b.x = ld3(n0);
+    if(prov) b.prov = (int32_t)prov[((size_t)b.ix * ny + b.iy) * NODEX + NX_PROV];
     v3<real> s = b.x + ld3(n11);
     b.xmid = V3(s.x * RL(0.5), s.y * RL(0.5), s.z * RL(0.5));
     v2<real> tri_n = V(-(b.ymax - b.ymin), b.xmax - b.xmin);
@@ -642,6 +649,55 @@ __host__ __device__ inline void Step3(const Ray3<real> &ray0, Ray3<real> &ray2, 
 // ---------------------------------------------------------------------------
 // Reflection (src/reflect.hpp:125-452, 3D, rectilinear boundaries)
 // ---------------------------------------------------------------------------
+// The grids' extras as GetBdrySeg3 takes them (null unless the grid has provinces).
+template<typename real> __host__ __device__ inline const real *prov_of(const Prob3D<real> &P, bool isTop)
+{
+    return (isTop ? P.ati_t1 : P.bty_t1) == 'L' ? P.base + (isTop ? P.o_atix : P.o_btyx) : nullptr;
+}
+
+// The reflecting boundary's data at the ray's ocean point x (trace.hpp:309-360,
+// O3D): the halfspace, its cP, cS, rho replaced by the cell's province on 'L'
+// grids; on 'C' grids the node normals interpolated bilinearly over the cell
+// (not renormalised) and the cell's curvatures, else the triangle's normal and
+// zero curvature.
+template<typename real> struct ReflB3 {
+    HSd<real> hs;
+    v3<real> n;
+    real zxx, zxy, zyy, kxx, kxy, kyy;
+};
+template<typename real>
+__host__ __device__ inline void refl_bdry3(const Prob3D<real> &P, bool isTop, const BState3<real> &b, v3<real> x,
+                                           ReflB3<real> &r)
+{
+    r.hs = isTop ? P.top : P.bot;
+    if((isTop ? P.ati_t1 : P.bty_t1) == 'L') {
+        const real *pv = P.base + (isTop ? P.o_atip : P.o_btyp) + 5 * b.prov;
+        r.hs.cP = C(pv[0], pv[1]);
+        r.hs.cS = C(pv[2], pv[3]);
+        r.hs.rho = pv[4];
+    }
+    r.n = b.n;
+    r.zxx = r.zxy = r.zyy = r.kxx = r.kxy = r.kyy = FL(0.0);
+    if((isTop ? P.ati_t0 : P.bty_t0) == 'C') {
+        const int ny = isTop ? P.ati_ny : P.bty_ny;
+        const real *X = P.base + (isTop ? P.o_atix : P.o_btyx);
+        const size_t k00 = (size_t)b.ix * ny + b.iy, k01 = k00 + 1, k10 = k00 + ny, k11 = k10 + 1;
+        real s1 = (x.x - b.x.x) / (b.xmax - b.xmin);
+        real s2 = (x.y - b.x.y) / (b.ymax - b.ymin);
+        real m1 = FL(1.0) - s1, m2 = FL(1.0) - s2;
+        r.n = ((sc3(m2, sc3(m1, ld3(X + k00 * NODEX))) + sc3(m2, sc3(s1, ld3(X + k10 * NODEX))))
+               + sc3(s2, sc3(s1, ld3(X + k11 * NODEX))))
+            + sc3(s2, sc3(m1, ld3(X + k01 * NODEX)));
+        const real *c = X + k00 * NODEX;
+        r.zxx = c[NX_ZXX];
+        r.zxy = c[NX_ZXY];
+        r.zyy = c[NX_ZYY];
+        r.kxx = c[NX_KXX];
+        r.kxy = c[NX_KXY];
+        r.kyy = c[NX_KYY];
+    }
+}
+
 // InterpolateReflectionCoefficient (reflect.hpp:39-92) at the incidence angle
 // |atan2(Th, Tg)| folded about 90 degrees; tab holds (theta, |R|, phase) rows.
 template<typename real>
@@ -686,7 +742,8 @@ __host__ __device__ inline cpx<real> HalfspaceReflO3D(real omega, real Tg, real 
 
 template<typename real>
 __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &newp, const HSd<real> &hs, bool isTop,
-                                         v3<real> nB, const Prob3D<real> &P, Trace3<real> &S, Err &e)
+                                         v3<real> nB, real zxx, real zxy, real zyy, const Prob3D<real> &P,
+                                         Trace3<real> &S, Err &e)
 {
     real Th = dot3(oldp.t, nB);
     v3<real> tB = oldp.t - sc3(Th, nB);
@@ -709,9 +766,9 @@ __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &new
     v2<real> n_rot = V(rayn2.x, rayn2.y);
     n_rot = sc(RL(1.0) / std::sqrt(dot(n_rot, n_rot)), n_rot);
     m2<real> R = M2(t_rot.x, t_rot.y, n_rot.x, n_rot.y);
-    // a rectilinear grid carries no curvature: kappaMat = 0
-    const real k0 = RL(0.0) / FL(2.0);
-    m2<real> DMat = mmul(mtr(R), mmul(mscl((real)FL(2.0), M2(k0, k0, k0, k0)), R));
+    // kappaMat from the boundary's z curvatures (zero off curvilinear grids)
+    const m2<real> K = M2(zxx / FL(2.0), zxy / FL(2.0), zxy / FL(2.0), zyy / FL(2.0));
+    m2<real> DMat = mmul(mtr(R), mmul(mscl((real)FL(2.0), K), R));
     real cn1 = dot3(o.gradc, -rayn1_t - rayn1);
     real cn2 = dot3(o.gradc, -rayn2_t - rayn2);
     real cs = -dot3(o.gradc, rayt_t - rayt);
@@ -817,8 +874,8 @@ __host__ __device__ inline bool RayInit3(int ia, int ib, int isx, int isy, int i
     p0.phi = FL(0.0);
     S.top = BState3<real>{};
     S.bot = BState3<real>{};
-    GetBdrySeg3(S.xs, p0.t, S.top, G + P.o_ati, P.ati_nx, P.ati_ny, true, true, e);
-    GetBdrySeg3(S.xs, p0.t, S.bot, G + P.o_bty, P.bty_nx, P.bty_ny, false, true, e);
+    GetBdrySeg3(S.xs, p0.t, S.top, G + P.o_ati, P.ati_nx, P.ati_ny, true, true, e, prov_of(P, true));
+    GetBdrySeg3(S.xs, p0.t, S.bot, G + P.o_bty, P.bty_nx, P.bty_ny, false, true, e, prov_of(P, false));
     real dt = -dot3(S.top.n, S.xs - S.top.x), db = -dot3(S.bot.n, S.xs - S.bot.x);
     if(dt <= FL(0.0) || db <= FL(0.0)) {
         e.warning(4);
@@ -848,11 +905,12 @@ __host__ __device__ inline bool RayUpdate3(const Ray3<real> &p0, Ray3<real> &p1,
 { // trace.hpp:274-381 (3D, rectilinear boundaries)
     bool topRefl = false, botRefl = false;
     Step3(p0, p1, S, P, e, topRefl, botRefl);
-    GetBdrySeg3(p1.x, p1.t, S.top, P.base + P.o_ati, P.ati_nx, P.ati_ny, true, false, e);
-    GetBdrySeg3(p1.x, p1.t, S.bot, P.base + P.o_bty, P.bty_nx, P.bty_ny, false, false, e);
+    GetBdrySeg3(p1.x, p1.t, S.top, P.base + P.o_ati, P.ati_nx, P.ati_ny, true, false, e, prov_of(P, true));
+    GetBdrySeg3(p1.x, p1.t, S.bot, P.base + P.o_bty, P.bty_nx, P.bty_ny, false, false, e, prov_of(P, false));
     if(topRefl || botRefl) {
-        v3<real> n = topRefl ? S.top.n : S.bot.n;
-        Reflect3(p1, p2, topRefl ? P.top : P.bot, topRefl, n, P, S, e);
+        ReflB3<real> r;
+        refl_bdry3(P, topRefl, topRefl ? S.top : S.bot, p1.x, r);
+        Reflect3(p1, p2, r.hs, topRefl, r.n, r.zxx, r.zxy, r.zyy, P, S, e);
         return true;
     }
     return false;

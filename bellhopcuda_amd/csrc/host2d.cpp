@@ -721,3 +721,23 @@ extern "C" int bhc_scale_pressure2d(const bhc_problem2d *p, float *field)
     }
     return 0;
 }
+
+// Internal to the library (used by the 3D preprocessing, not part of the C
+// ABI): crci in dB/wavelength with no volume term, for geoacoustics given per
+// boundary cell (src/module/boundary.hpp:527-537), in the run's real type.
+void bhc_crci_w(int precision, double freq0, double c, double alpha, double out[2])
+{
+    bhc_env2d e;
+    memset(&e, 0, sizeof(e));
+    if(precision == 64) {
+        Pre<double> pre{&e, freq0};
+        cx<double> r = pre.crci(1.0e20, c, alpha, "W ");
+        out[0] = r.real();
+        out[1] = r.imag();
+    } else {
+        Pre<float> pre{&e, (float)freq0};
+        cx<float> r = pre.crci(1.0e20f, (float)c, (float)alpha, "W ");
+        out[0] = (double)r.real();
+        out[1] = (double)r.imag();
+    }
+}

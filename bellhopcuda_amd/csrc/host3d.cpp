@@ -24,10 +24,12 @@
 struct bhc_problem3d_owner {
     bhc_problem3d v;
     bhc_problem_owner *p2 = nullptr;
-    std::vector<double> z3, rho3, hx, hy, hc, hcz, Sx, Sy, trcv, theta, beta, bcos, bsin;
+    std::vector<double> z3, rho3, hx, hy, hc, hcz, Sx, Sy, trcv, theta, beta, bcos, bsin, atip, btyp;
     std::vector<bhc_bdry_node3d> ati, bty;
     ~bhc_problem3d_owner() { bhc_problem_free(p2); }
 };
+
+void bhc_crci_w(int precision, double freq0, double c, double alpha, double out[2]); // host2d.cpp
 
 namespace {
 
@@ -53,14 +55,71 @@ template<typename real> V3<real> cross(V3<real> a, V3<real> b)
 }
 template<typename real> real len(V3<real> a) { return std::sqrt((a.x * a.x + a.y * a.y) + a.z * a.z); }
 
+inline double atan2_(double y, double x) { return std::atan2(y, x); }
+inline float atan2_(float y, float x) { return ::atan2f(y, x); }
+
+// Node normals and, on curvilinear grids, the cells' curvatures:
+// ComputeBdryTangentNormal 3D (src/module/boundary.hpp:650-830). The node
+// angle phi_xx exists only at the cells' lower corners; a cell in the last
+// row or column reads its neighbour's, which the reference leaves unset: 0.
+template<typename real>
+void curvature(const std::vector<V3<real>> &x, int nx, int ny, bool curvilinear, std::vector<bhc_bdry_node3d> &out)
+{
+    std::vector<V3<real>> nu(x.size());
+    std::vector<real> phi(x.size(), (real)0.0f);
+    auto X = [&](int i, int j) -> const V3<real> & { return x[(size_t)i * ny + j]; };
+    for(int ix = 0; ix < nx; ++ix)
+        for(int iy = 0; iy < ny; ++iy) {
+            real mx, my;
+            if(ix == 0) mx = (X(ix + 1, iy).z - X(ix, iy).z) / (X(ix + 1, iy).x - X(ix, iy).x);
+            else if(ix == nx - 1) mx = (X(ix, iy).z - X(ix - 1, iy).z) / (X(ix, iy).x - X(ix - 1, iy).x);
+            else mx = (X(ix + 1, iy).z - X(ix - 1, iy).z) / (X(ix + 1, iy).x - X(ix - 1, iy).x);
+            if(iy == 0) my = (X(ix, iy + 1).z - X(ix, iy).z) / (X(ix, iy + 1).y - X(ix, iy).y);
+            else if(iy == ny - 1) my = (X(ix, iy).z - X(ix, iy - 1).z) / (X(ix, iy).y - X(ix, iy - 1).y);
+            else my = (X(ix, iy + 1).z - X(ix, iy - 1).z) / (X(ix, iy + 1).y - X(ix, iy - 1).y);
+            V3<real> n = {-mx, -my, (real)1.0f};
+            size_t k = (size_t)ix * ny + iy;
+            if(ix < nx - 1 && iy < ny - 1) phi[k] = atan2_(n.z, n.x);
+            nu[k] = n;
+            real l = len(n);
+            out[k].Noden[0] = (double)(n.x / l);
+            out[k].Noden[1] = (double)(n.y / l);
+            out[k].Noden[2] = (double)(n.z / l);
+        }
+    if(!curvilinear) return;
+    for(int ix = 0; ix < nx - 1; ++ix)
+        for(int iy = 0; iy < ny - 1; ++iy) {
+            size_t k = (size_t)ix * ny + iy, kx = (size_t)(ix + 1) * ny + iy, ky = k + 1;
+            real zxx = -(nu[kx].x - nu[k].x) / (x[kx].x - x[k].x);
+            V3<real> tv = sub(x[kx], x[k]);
+            real L = std::sqrt(tv.x * tv.x + tv.z * tv.z);
+            real kxx = (phi[kx] - phi[k]) / L;
+            real zxy = -(nu[ky].x - nu[k].x) / (x[ky].y - x[k].y);
+            tv = sub(x[ky], x[k]);
+            L = std::sqrt(tv.y * tv.y + tv.z * tv.z);
+            real kxy = (phi[ky] - phi[k]) / L;
+            real zyy = -(nu[ky].y - nu[k].y) / (x[ky].y - x[k].y);
+            real kyy = (phi[ky] - phi[k]) / L;
+            L = len(nu[k]);
+            out[k].z_xx = (double)(zxx / L);
+            out[k].z_xy = (double)(zxy / L);
+            out[k].z_yy = (double)(zyy / L);
+            out[k].kappa_xx = (double)kxx;
+            out[k].kappa_xy = (double)kxy;
+            out[k].kappa_yy = (double)kyy;
+        }
+}
+
 inline void sincos_(double a, double *s, double *c) { ::sincos(a, s, c); }
 inline void sincos_(float a, float *s, float *c) { ::sincosf(a, s, c); }
 
 // Triangulated grid (or the flat default) -> nodes with the two cell normals.
 template<typename real>
 int grid(const bhc_bdry3d_in &in, double depth, bool isTop, std::vector<bhc_bdry_node3d> &out, int32_t &nx,
-         int32_t &ny)
+         int32_t &ny, char type[2])
 {
+    type[0] = 'R';
+    type[1] = 'S';
     if(in.nx <= 0) { // Default, src/module/boundary.hpp:56-78
         const real big = (real)1.0e25;
         const real d = (real)depth, nt = isTop ? (real)-1.0 : (real)1.0;
@@ -72,12 +131,15 @@ int grid(const bhc_bdry3d_in &in, double depth, bool isTop, std::vector<bhc_bdry
             out[i].x[1] = (double)ys[i];
             out[i].x[2] = (double)d;
             out[i].n1[2] = out[i].n2[2] = (double)nt;
+            out[i].province = -1;
         }
         return 0;
     }
-    if(in.type[0] != 'R' || in.type[1] != 'S')
-        return fail3("3D boundary type '%c%c' is not supported (rectilinear 'RS' grids only)", in.type[0],
+    if((in.type[0] != 'R' && in.type[0] != 'C') || (in.type[1] != 'S' && in.type[1] != 'L'))
+        return fail3("3D boundary type '%c%c' is not supported ('R' or 'C', then 'S' or 'L')", in.type[0],
                      in.type[1]);
+    type[0] = in.type[0];
+    type[1] = in.type[1];
     if(in.nx < 2 || in.ny < 2) return fail3("3D boundary grid needs at least 2 x 2 nodes");
     nx = in.nx;
     ny = in.ny;
@@ -116,6 +178,16 @@ int grid(const bhc_bdry3d_in &in, double depth, bool isTop, std::vector<bhc_bdry
             b.n2[1] = (double)(n2.y / l2);
             b.n2[2] = (double)(n2.z / l2);
         }
+    curvature<real>(x, nx, ny, in.type[0] == 'C', out);
+    for(size_t i = 0; i < out.size(); ++i) out[i].province = -1;
+    if(in.type[1] == 'L') {
+        if(!in.province || !in.prov || in.nprov < 1) return fail3("province grid needs provinces");
+        for(size_t i = 0; i < out.size(); ++i) {
+            int pv = in.province[i];
+            if(pv < 1 || pv > in.nprov) return fail3("province number %d outside 1..%d", pv, in.nprov);
+            out[i].province = pv - 1;
+        }
+    }
     return 0;
 }
 
@@ -180,10 +252,28 @@ template<typename real> int preprocess3d(const bhc_env3d *e, bhc_problem3d_owner
     }
     v.top_hs = p.top_hs;
     v.bot_hs = p.bot_hs;
-    if(grid<real>(e->ati, p.top_hs.depth, true, o->ati, v.ati_nx, v.ati_ny)) return -1;
-    if(grid<real>(e->bty, p.bot_hs.depth, false, o->bty, v.bty_nx, v.bty_ny)) return -1;
+    if(grid<real>(e->ati, p.top_hs.depth, true, o->ati, v.ati_nx, v.ati_ny, v.ati_type)) return -1;
+    if(grid<real>(e->bty, p.bot_hs.depth, false, o->bty, v.bty_nx, v.bty_ny, v.bty_type)) return -1;
     v.ati = o->ati.data();
     v.bty = o->bty.data();
+    // provinces: cP, cS from dB/wavelength at z = 1e20 (boundary.hpp:527-537), rho
+    auto provs = [&](const bhc_bdry3d_in &in, const char *type, std::vector<double> &dst, int32_t &n,
+                     const double *&ptr) {
+        n = 0;
+        ptr = nullptr;
+        if(type[1] != 'L') return;
+        for(int i = 0; i < in.nprov; ++i) {
+            const double *r = in.prov + 5 * (size_t)i;
+            double cp[2], cs[2];
+            bhc_crci_w(e->precision, e->freq0, r[0], r[3], cp);
+            bhc_crci_w(e->precision, e->freq0, r[1], r[4], cs);
+            dst.insert(dst.end(), {cp[0], cp[1], cs[0], cs[1], (double)(real)r[2]});
+        }
+        n = in.nprov;
+        ptr = dst.data();
+    };
+    provs(e->ati, v.ati_type, o->atip, v.ati_nprov, v.ati_prov);
+    provs(e->bty, v.bty_type, o->btyp, v.bty_nprov, v.bty_prov);
     v.trc_npts = p.trc_npts;
     v.brc_npts = p.brc_npts;
     v.trc = p.trc;

@@ -154,15 +154,15 @@ __host__ __device__ inline void StepNx(const RayPt<real> &ray0, RayPt<real> &ray
     }
 }
 
-// Reflect with O3D && !R3D on a rectilinear (uncurved) boundary: the ocean
-// normal projected on the bearing plane and renormalised, the boundary tangent
-// rebuilt from the incident tangent (reflect.hpp:130-142), the 2D curvature
-// update with the projected curvature OceanToRayCurvature (reflect.hpp:94-114)
-// of zero curvatures, the O3D halfspace coefficient and the 2D beam shift.
+// Reflect with O3D && !R3D: the ocean normal projected on the bearing plane
+// and renormalised, the boundary tangent rebuilt from the incident tangent
+// (reflect.hpp:130-142), the 2D curvature update with the cell's curvature
+// projected on the bearing (OceanToRayCurvature, reflect.hpp:94-114), the O3D
+// halfspace coefficient and the 2D beam shift.
 template<typename real>
 __host__ __device__ inline void ReflectNx(const RayPt<real> &oldp, RayPt<real> &newp, const HSd<real> &hs, bool isTop,
-                                          v3<real> nBdry, const OrgNx<real> &org, const Prob3D<real> &N,
-                                          const Prob2D<real> &P, Trace3<real> &S, Err &e)
+                                          v3<real> nBdry, real kxx, real kxy, real kyy, const OrgNx<real> &org,
+                                          const Prob3D<real> &N, const Prob2D<real> &P, Trace3<real> &S, Err &e)
 {
     v2<real> nB = OceanToRayT(nBdry, org);
     nB = sc(RL(1.0) / std::sqrt(dot(nB, nB)), nB);
@@ -170,8 +170,7 @@ __host__ __device__ inline void ReflectNx(const RayPt<real> &oldp, RayPt<real> &
     v2<real> tB = oldp.t - sc(Th, nB);
     tB = sc(RL(1.0) / std::sqrt(dot(tB, tB)), tB);
     real Tg = dot(oldp.t, tB);
-    const real k0 = RL(0.0);
-    real kappa = k0 * (org.tr.x * org.tr.x) + FL(2.0) * k0 * org.tr.x * org.tr.y + k0 * (org.tr.y * org.tr.y);
+    real kappa = kxx * (org.tr.x * org.tr.x) + FL(2.0) * kxy * org.tr.x * org.tr.y + kyy * (org.tr.y * org.tr.y);
     if(isTop) kappa = -kappa;
     newp.NumTopBnc = oldp.NumTopBnc + (isTop ? 1 : 0);
     newp.NumBotBnc = oldp.NumBotBnc + (isTop ? 0 : 1);
@@ -284,8 +283,8 @@ __host__ __device__ inline bool RayInitNx(int ia, int ib, int isx, int isy, int 
     S.top = BState3<real>{};
     S.bot = BState3<real>{};
     const v3<real> t_o = RayToOceanT(p0.t, org);
-    GetBdrySeg3(S.xs, t_o, S.top, G + N.o_ati, N.ati_nx, N.ati_ny, true, true, e);
-    GetBdrySeg3(S.xs, t_o, S.bot, G + N.o_bty, N.bty_nx, N.bty_ny, false, true, e);
+    GetBdrySeg3(S.xs, t_o, S.top, G + N.o_ati, N.ati_nx, N.ati_ny, true, true, e, prov_of(N, true));
+    GetBdrySeg3(S.xs, t_o, S.bot, G + N.o_bty, N.bty_nx, N.bty_ny, false, true, e, prov_of(N, false));
     real dt = -dot3(S.top.n, S.xs - S.top.x), db = -dot3(S.bot.n, S.xs - S.bot.x);
     if(dt <= FL(0.0) || db <= FL(0.0)) {
         e.warning(4);
@@ -315,7 +314,7 @@ __host__ __device__ inline bool RayInitNx(int ia, int ib, int isx, int isy, int 
     return true;
 }
 
-// RayUpdate (trace.hpp:274-381, O3D && !R3D, rectilinear boundaries).
+// RayUpdate (trace.hpp:274-381, O3D && !R3D).
 template<typename real>
 __host__ __device__ inline bool RayUpdateNx(const RayPt<real> &p0, RayPt<real> &p1, RayPt<real> &p2, Trace3<real> &S,
                                             const OrgNx<real> &org, const Prob3D<real> &N, const Prob2D<real> &P,
@@ -324,10 +323,12 @@ __host__ __device__ inline bool RayUpdateNx(const RayPt<real> &p0, RayPt<real> &
     bool topRefl = false, botRefl = false;
     StepNx(p0, p1, S, org, N, e, topRefl, botRefl);
     const v3<real> x_o = RayToOceanX(p1.x, org), t_o = RayToOceanT(p1.t, org);
-    GetBdrySeg3(x_o, t_o, S.top, N.base + N.o_ati, N.ati_nx, N.ati_ny, true, false, e);
-    GetBdrySeg3(x_o, t_o, S.bot, N.base + N.o_bty, N.bty_nx, N.bty_ny, false, false, e);
+    GetBdrySeg3(x_o, t_o, S.top, N.base + N.o_ati, N.ati_nx, N.ati_ny, true, false, e, prov_of(N, true));
+    GetBdrySeg3(x_o, t_o, S.bot, N.base + N.o_bty, N.bty_nx, N.bty_ny, false, false, e, prov_of(N, false));
     if(topRefl || botRefl) {
-        ReflectNx(p1, p2, topRefl ? N.top : N.bot, topRefl, topRefl ? S.top.n : S.bot.n, org, N, P, S, e);
+        ReflB3<real> r;
+        refl_bdry3(N, topRefl, topRefl ? S.top : S.bot, x_o, r);
+        ReflectNx(p1, p2, r.hs, topRefl, r.n, r.kxx, r.kxy, r.kyy, org, N, P, S, e);
         return true;
     }
     return false;

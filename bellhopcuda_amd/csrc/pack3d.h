@@ -71,6 +71,32 @@ int pack3(const bhc_problem3d *p, std::vector<real> &blob, Prob3D<real> &P, std:
     P.ati_ny = p->ati_ny;
     P.o_bty = put_nodes(p->bty, p->bty_nx * p->bty_ny);
     P.o_ati = put_nodes(p->ati, p->ati_nx * p->ati_ny);
+    // curvilinear / province grids: per-node extras [NODEX] and province tables
+    auto put_extras = [&](const bhc_bdry_node3d *nd, int n) {
+        int32_t off = (int32_t)blob.size();
+        for(int i = 0; i < n; ++i) {
+            const real rec[NODEX] = {(real)nd[i].Noden[0], (real)nd[i].Noden[1], (real)nd[i].Noden[2],
+                                     (real)nd[i].z_xx,     (real)nd[i].z_xy,     (real)nd[i].z_yy,
+                                     (real)nd[i].kappa_xx, (real)nd[i].kappa_xy, (real)nd[i].kappa_yy,
+                                     (real)nd[i].province};
+            for(int k = 0; k < NODEX; ++k) blob.push_back(rec[k]);
+        }
+        return off;
+    };
+    P.ati_t0 = p->ati_type[0] ? p->ati_type[0] : 'R';
+    P.ati_t1 = p->ati_type[1] ? p->ati_type[1] : 'S';
+    P.bty_t0 = p->bty_type[0] ? p->bty_type[0] : 'R';
+    P.bty_t1 = p->bty_type[1] ? p->bty_type[1] : 'S';
+    if(P.ati_t0 == 'C' || P.ati_t1 == 'L') P.o_atix = put_extras(p->ati, p->ati_nx * p->ati_ny);
+    if(P.bty_t0 == 'C' || P.bty_t1 == 'L') P.o_btyx = put_extras(p->bty, p->bty_nx * p->bty_ny);
+    if(P.ati_t1 == 'L') {
+        if(p->ati_nprov < 1 || !p->ati_prov) return fail("3D problem: province grid without provinces");
+        P.o_atip = put(p->ati_prov, 5 * (size_t)p->ati_nprov);
+    }
+    if(P.bty_t1 == 'L') {
+        if(p->bty_nprov < 1 || !p->bty_prov) return fail("3D problem: province grid without provinces");
+        P.o_btyp = put(p->bty_prov, 5 * (size_t)p->bty_nprov);
+    }
     P.o_Rr = put(p->Rr, p->NRr);
     P.o_trc = put(p->trc, 3 * (size_t)p->trc_npts);
     P.o_brc = put(p->brc, 3 * (size_t)p->brc_npts);

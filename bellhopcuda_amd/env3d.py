@@ -26,10 +26,16 @@ from .env import Halfspace, ReflTable, munk_c, subtab
 
 @dataclass
 class Grid3D:
-    """Altimetry/bathymetry grid (x, y in km, z[ix, iy] in m); empty = flat default."""
+    """Altimetry/bathymetry grid (x, y in km, z[ix, iy] in m); empty = flat default.
+    type: 'R' (flat triangles) or 'C' (curvilinear), then 'S' or 'L'
+    (provinces: province[ix, iy] 1-based numbers into prov rows of alphaR,
+    betaR, rho, alphaI, betaI with attenuation in dB/wavelength)."""
     x_km: np.ndarray = field(default_factory=lambda: np.zeros(0))
     y_km: np.ndarray = field(default_factory=lambda: np.zeros(0))
     z: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+    type: str = "RS"
+    province: Optional[np.ndarray] = None
+    prov: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -130,10 +136,17 @@ class Env3D:
             dst.depth, dst.alphaR, dst.betaR, dst.rho = hs.depth, hs.alphaR, hs.betaR, hs.rho
             dst.alphaI, dst.betaI, dst.zTemp, dst.Mz = hs.alphaI, hs.betaI, hs.zTemp, hs.Mz
         for dst, g in ((e.ati, self.ati), (e.bty, self.bty)):
-            dst.type = b"RS"
+            dst.type = (g.type + "  ")[:2].encode()
             dst.nx, dst.ny = len(g.x_km), len(g.y_km)
             if dst.nx:
                 dst.x_km, dst.y_km, dst.z = dp(g.x_km), dp(g.y_km), dp(np.asarray(g.z, np.float64))
+            if g.province is not None:
+                pv = np.ascontiguousarray(g.province, dtype=np.int32).reshape(-1)
+                keep.append(pv)
+                dst.province = pv.ctypes.data_as(_abi.c_int32_p)
+                pr = np.asarray(g.prov, np.float64).reshape(-1, 5)
+                dst.nprov = len(pr)
+                dst.prov = dp(pr)
         for dst, t in ((e.trc, self.trc), (e.brc, self.brc)):
             dst.npts = len(t.theta)
             if dst.npts:

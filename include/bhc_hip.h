@@ -433,11 +433,13 @@ void bhc_sweep_release(bhc_sweep *sw);
  * torsion, triangulated bathymetry / altimetry grids, depth-only or hexahedral
  * sound speed, geometric hat / Gaussian beams in Cartesian coordinates, TL. */
 
-/* Altimetry / bathymetry grid (BdryInfoTopBot<true>, src/module/boundary.hpp:107-150):
+/* Altimetry / bathymetry grid (BdryInfoTopBot<true>, src/module/boundary.hpp:107-197):
  * nx x ny nodes, each rectangle split into two triangles along its diagonal.
  * nx == 0 selects the flat default boundary at the halfspace depth
- * (src/module/boundary.hpp:56-78). Only type "RS" (rectilinear, one set of
- * geoacoustics) is accepted. */
+ * (src/module/boundary.hpp:56-78). type[0]: 'R' flat triangles or 'C'
+ * curvilinear (node normals interpolated over the cell, cell curvatures at
+ * reflections); type[1]: 'S' one set of geoacoustics or 'L' provinces (each
+ * cell takes the cP, cS, rho of its lower corner's province, src/boundary.hpp:134-140). */
 typedef struct bhc_bdry3d_in {
     char type[2];
     char pad[6];
@@ -445,6 +447,9 @@ typedef struct bhc_bdry3d_in {
     const double *x_km;     /* [nx] */
     const double *y_km;     /* [ny] */
     const double *z;        /* [nx][ny] (m), index ix * ny + iy */
+    const int32_t *province; /* 'L': [nx][ny] province numbers (1-based), index ix * ny + iy */
+    int32_t nprov, pad2;
+    const double *prov;     /* 'L': [nprov][5] alphaR, betaR, rho, alphaI, betaI (attenuation in dB/wavelength) */
 } bhc_bdry3d_in;
 
 typedef struct bhc_env3d {
@@ -486,9 +491,15 @@ typedef struct bhc_env3d {
 } bhc_env3d;
 
 /* Boundary grid node: position and the unit normals of the two triangles of
- * the cell whose lower corner it is (BdryPtFull<true>::x, n1, n2). */
+ * the cell whose lower corner it is (BdryPtFull<true>::x, n1, n2); the unit
+ * node normal Noden; for curvilinear grids the cell's curvatures z_xx, z_xy,
+ * z_yy and kappa_xx, kappa_xy, kappa_yy (src/module/boundary.hpp:650-830);
+ * for province grids the cell's province (0-based; -1 otherwise). */
 typedef struct bhc_bdry_node3d {
     double x[3], n1[3], n2[3];
+    double Noden[3];
+    double z_xx, z_xy, z_yy, kappa_xx, kappa_xy, kappa_yy;
+    int32_t province, pad;
 } bhc_bdry_node3d;
 
 typedef struct bhc_problem3d {
@@ -507,6 +518,9 @@ typedef struct bhc_problem3d {
     bhc_hs2d top_hs, bot_hs;
     int32_t ati_nx, ati_ny, bty_nx, bty_ny;
     const bhc_bdry_node3d *ati, *bty;
+    char ati_type[2], bty_type[2];  /* [0] 'R' / 'C', [1] 'S' / 'L' */
+    int32_t ati_nprov, bty_nprov;
+    const double *ati_prov, *bty_prov; /* [nprov][5] cP (re, im), cS (re, im), rho of each province */
     int32_t trc_npts, brc_npts;
     const double *trc, *brc;
     int32_t NSx, NSy, NSz, NRz, NRr, Ntheta;

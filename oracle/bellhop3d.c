@@ -84,8 +84,14 @@ static inline real m2det(m2 a) { return a.m[0][0] * a.m[1][1] - a.m[1][0] * a.m[
 /* ------------------------------------------------------------------------- */
 /* Preprocessed 3D problem                                                     */
 /* ------------------------------------------------------------------------- */
-typedef struct { v3 x, n1, n2; } BNode3;
-typedef struct { int nx, ny; BNode3 *bd; } Bdry3;
+typedef struct {
+    v3 x, n1, n2;
+    v3 Noden, Nu;                           /* node normal, unit and as (-z_x, -z_y, 1) */
+    real phi_xx;                            /* node angle (cells' lower corners; 0 elsewhere) */
+    real z_xx, z_xy, z_yy, kappa_xx, kappa_xy, kappa_yy; /* 'C': the cell's curvatures */
+    int prov;                               /* 'L': the cell's province (0-based) */
+} BNode3;
+typedef struct { int nx, ny; char type[2]; BNode3 *bd; int nprov; HS *prov; } Bdry3;
 
 typedef struct {
     Prob P2;            /* depth-only SSP, halfspaces, reflection tables, deltas, sbp, alpha, Sz, Rz, Rr */
@@ -121,8 +127,58 @@ static void bdry3_normals(Bdry3 *B, int isTop)
         }
 }
 
-static int bdry3_from_env(const bhc_bdry3d_in *in, real depth, int isTop, Bdry3 *B)
+/* Node normals and, for curvilinear grids, the cells' curvatures:
+ * ComputeBdryTangentNormal 3D (src/module/boundary.hpp:650-830). The node
+ * angles phi_xx exist only at the cells' lower corners; the curvature of a
+ * cell in the last row or column reads its neighbour's, which the reference
+ * never sets (an uninitialised allocation): here it is 0. */
+static void bdry3_curvature(Bdry3 *B)
 {
+    int nx = B->nx, ny = B->ny;
+    BNode3 *b = B->bd;
+#define BD(i, j) b[(i) * ny + (j)]
+    for(int ix = 0; ix < nx; ++ix)
+        for(int iy = 0; iy < ny; ++iy) {
+            real mx, my;
+            if(ix == 0) mx = (BD(ix + 1, iy).x.z - BD(ix, iy).x.z) / (BD(ix + 1, iy).x.x - BD(ix, iy).x.x);
+            else if(ix == nx - 1) mx = (BD(ix, iy).x.z - BD(ix - 1, iy).x.z) / (BD(ix, iy).x.x - BD(ix - 1, iy).x.x);
+            else mx = (BD(ix + 1, iy).x.z - BD(ix - 1, iy).x.z) / (BD(ix + 1, iy).x.x - BD(ix - 1, iy).x.x);
+            if(iy == 0) my = (BD(ix, iy + 1).x.z - BD(ix, iy).x.z) / (BD(ix, iy + 1).x.y - BD(ix, iy).x.y);
+            else if(iy == ny - 1) my = (BD(ix, iy).x.z - BD(ix, iy - 1).x.z) / (BD(ix, iy).x.y - BD(ix, iy - 1).x.y);
+            else my = (BD(ix, iy + 1).x.z - BD(ix, iy - 1).x.z) / (BD(ix, iy + 1).x.y - BD(ix, iy - 1).x.y);
+            v3 n = V3(-mx, -my, RL(1.0));
+            BD(ix, iy).phi_xx = (ix < nx - 1 && iy < ny - 1) ? RATAN2(n.z, n.x) : RL(0.0);
+            BD(ix, iy).Nu = n;
+            BD(ix, iy).Noden = v3divs(n, v3len(n));
+        }
+    if(B->type[0] != 'C') return;
+    for(int ix = 0; ix < nx - 1; ++ix)
+        for(int iy = 0; iy < ny - 1; ++iy) {
+            BNode3 *c = &BD(ix, iy);
+            c->z_xx = -(BD(ix + 1, iy).Nu.x - c->Nu.x) / (BD(ix + 1, iy).x.x - c->x.x);
+            v3 tv = v3sub(BD(ix + 1, iy).x, c->x);
+            real Len = RSQRT(SQ(tv.x) + SQ(tv.z));
+            c->kappa_xx = (BD(ix + 1, iy).phi_xx - c->phi_xx) / Len;
+            c->z_xy = -(BD(ix, iy + 1).Nu.x - c->Nu.x) / (BD(ix, iy + 1).x.y - c->x.y);
+            tv = v3sub(BD(ix, iy + 1).x, c->x);
+            Len = RSQRT(SQ(tv.y) + SQ(tv.z));
+            c->kappa_xy = (BD(ix, iy + 1).phi_xx - c->phi_xx) / Len;
+            c->z_yy = -(BD(ix, iy + 1).Nu.y - c->Nu.y) / (BD(ix, iy + 1).x.y - c->x.y);
+            c->kappa_yy = (BD(ix, iy + 1).phi_xx - c->phi_xx) / Len;
+            Len = v3len(c->Nu);
+            c->z_xx /= Len;
+            c->z_xy /= Len;
+            c->z_yy /= Len;
+        }
+#undef BD
+}
+
+static int bdry3_from_env(const bhc_env2d *e2, const bhc_bdry3d_in *in, real depth, int isTop, Bdry3 *B)
+{
+    B->type[0] = 'R';
+    B->type[1] = 'S';
+    B->nprov = 0;
+    B->prov = NULL;
     if(in->nx <= 0) { /* Default, src/module/boundary.hpp:56-78 */
         const real big = RL(1.0e25);
         real negtop = isTop ? RL(-1.0) : RL(1.0);
@@ -135,7 +191,9 @@ static int bdry3_from_env(const bhc_bdry3d_in *in, real depth, int isTop, Bdry3 
         for(int i = 0; i < 4; ++i) B->bd[i].n1 = B->bd[i].n2 = V3(FL(0.0), FL(0.0), negtop);
         return 0;
     }
-    if(in->type[0] != 'R' || in->type[1] != 'S') return -1;
+    if((in->type[0] != 'R' && in->type[0] != 'C') || (in->type[1] != 'S' && in->type[1] != 'L')) return -1;
+    B->type[0] = in->type[0];
+    B->type[1] = in->type[1];
     B->nx = in->nx;
     B->ny = in->ny;
     B->bd = xcalloc((size_t)B->nx * B->ny, sizeof(BNode3));
@@ -146,6 +204,25 @@ static int bdry3_from_env(const bhc_bdry3d_in *in, real depth, int isTop, Bdry3 
             b->x = V3((real)in->x_km[ix] * RL(1000.0), (real)in->y_km[iy] * RL(1000.0), (real)in->z[ix * B->ny + iy]);
         }
     bdry3_normals(B, isTop);
+    bdry3_curvature(B);
+    if(B->type[1] == 'L') {
+        /* provinces (boundary.hpp:152-197): cell -> province, geoacoustics in
+         * dB/wavelength converted by crci at z = 1e20 (boundary.hpp:527-537) */
+        if(!in->province || !in->prov || in->nprov < 1) return -1;
+        B->nprov = in->nprov;
+        B->prov = xcalloc(B->nprov, sizeof(HS));
+        for(int i = 0; i < B->nprov; ++i) {
+            const double *r = in->prov + 5 * i;
+            B->prov[i].cP = crci(e2, e2->freq0, RL(1.0e20), (real)r[0], (real)r[3], "W ");
+            B->prov[i].cS = crci(e2, e2->freq0, RL(1.0e20), (real)r[1], (real)r[4], "W ");
+            B->prov[i].rho = (real)r[2];
+        }
+        for(int i = 0; i < B->nx * B->ny; ++i) {
+            int pv = in->province[i];
+            if(pv < 1 || pv > B->nprov) return -1;
+            B->bd[i].prov = pv - 1;
+        }
+    }
     return 0;
 }
 
@@ -154,7 +231,7 @@ static void prob3_free(Prob3 *Q)
     prob_free(&Q->P2);
     free(Q->segx); free(Q->segy); free(Q->cmat); free(Q->czmat);
     free(Q->z3); free(Q->rho3);
-    free(Q->ati.bd); free(Q->bty.bd);
+    free(Q->ati.bd); free(Q->bty.bd); free(Q->ati.prov); free(Q->bty.prov);
     free(Q->Sx); free(Q->Sy); free(Q->t_rcvr); free(Q->beta);
 }
 
@@ -225,7 +302,7 @@ static int prob3_from_env(const bhc_env3d *e, Prob3 *Q)
         memcpy(Q->z3, P->z, sizeof(real) * P->npts);
         memcpy(Q->rho3, P->rho, sizeof(real) * P->npts);
     }
-    if(bdry3_from_env(&e->ati, P->top.depth, 1, &Q->ati) || bdry3_from_env(&e->bty, P->bot.depth, 0, &Q->bty)) {
+    if(bdry3_from_env(&e2, &e->ati, P->top.depth, 1, &Q->ati) || bdry3_from_env(&e2, &e->bty, P->bot.depth, 0, &Q->bty)) {
         prob3_free(Q);
         return -1;
     }
@@ -338,6 +415,7 @@ typedef struct {
     real xmin, xmax, ymin, ymax;
     v3 x, xmid, n;
     int onEdge, side, justSteppedTo, outgoingSide;
+    int prov; /* 'L' grids: the province of the current cell (CopyHSInfo, boundary.hpp:134-140) */
 } BState3;
 
 static void GetBdrySeg3(v3 x, v3 t, BState3 *b, const Bdry3 *B, int isTop, int isInit, ErrState *err)
@@ -375,6 +453,7 @@ static void GetBdrySeg3(v3 x, v3 t, BState3 *b, const Bdry3 *B, int isTop, int i
     b->ymax = bd[b->iy + 1].x.y;
     b->x = bd[b->ix * ny + b->iy].x;
     b->xmid = v3scl(RL(0.5), v3add(b->x, bd[(b->ix + 1) * ny + (b->iy + 1)].x));
+    if(B->type[1] == 'L') b->prov = bd[b->ix * ny + b->iy].prov;
     /* (x + x11) * 0.5 is componentwise */
     v2 tri_n = V2(-(b->ymax - b->ymin), b->xmax - b->xmin);
     tri_n = vdivs(tri_n, vlen(tri_n));
@@ -773,8 +852,47 @@ static void Step3(RayPt3 ray0, RayPt3 *ray2, TraceState3 *S, const Prob3 *Q, Err
 /* ------------------------------------------------------------------------- */
 /* Reflect, src/reflect.hpp:125-452 (3D)                                       */
 /* ------------------------------------------------------------------------- */
-static void Reflect3(const RayPt3 *oldp, RayPt3 *newp, const HS *hs, int isTop, v3 nB, const Prob3 *Q, Seg3 *iSeg,
-                     ErrState *err)
+/* The reflecting boundary's data at the ray's ocean point x (trace.hpp:309-360,
+ * O3D): the halfspace, its geoacoustics replaced by the cell's province on 'L'
+ * grids; on 'C' grids the node normals interpolated bilinearly over the cell
+ * (not renormalised) and the cell's curvatures, else the triangle's normal and
+ * zero curvature. */
+typedef struct { HS hs; v3 n; real z_xx, z_xy, z_yy, kappa_xx, kappa_xy, kappa_yy; } ReflBdry3;
+
+static ReflBdry3 refl_bdry3(const Bdry3 *B, const BState3 *b, const HS *hs0, v3 x)
+{
+    ReflBdry3 r;
+    r.hs = *hs0;
+    if(B->type[1] == 'L') {
+        const HS *pv = &B->prov[b->prov];
+        r.hs.cP = pv->cP;
+        r.hs.cS = pv->cS;
+        r.hs.rho = pv->rho;
+    }
+    r.n = b->n;
+    r.z_xx = r.z_xy = r.z_yy = r.kappa_xx = r.kappa_xy = r.kappa_yy = FL(0.0);
+    if(B->type[0] == 'C') {
+        int ny = B->ny;
+        const BNode3 *bd00 = &B->bd[b->ix * ny + b->iy], *bd01 = &B->bd[b->ix * ny + b->iy + 1];
+        const BNode3 *bd10 = &B->bd[(b->ix + 1) * ny + b->iy], *bd11 = &B->bd[(b->ix + 1) * ny + b->iy + 1];
+        real s1 = (x.x - b->x.x) / (b->xmax - b->xmin);
+        real s2 = (x.y - b->x.y) / (b->ymax - b->ymin);
+        real m1 = FL(1.0) - s1, m2 = FL(1.0) - s2;
+        r.n = v3add(v3add(v3add(v3scl(m2, v3scl(m1, bd00->Noden)), v3scl(m2, v3scl(s1, bd10->Noden))),
+                          v3scl(s2, v3scl(s1, bd11->Noden))),
+                    v3scl(s2, v3scl(m1, bd01->Noden)));
+        r.z_xx = bd00->z_xx;
+        r.z_xy = bd00->z_xy;
+        r.z_yy = bd00->z_yy;
+        r.kappa_xx = bd00->kappa_xx;
+        r.kappa_xy = bd00->kappa_xy;
+        r.kappa_yy = bd00->kappa_yy;
+    }
+    return r;
+}
+
+static void Reflect3(const RayPt3 *oldp, RayPt3 *newp, const HS *hs, int isTop, v3 nB, real z_xx, real z_xy,
+                     real z_yy, const Prob3 *Q, Seg3 *iSeg, ErrState *err)
 {
     real Th = v3dot(oldp->t, nB);
     v3 tB = v3sub(oldp->t, v3scl(Th, nB));
@@ -797,8 +915,8 @@ static void Reflect3(const RayPt3 *oldp, RayPt3 *newp, const HS *hs, int isTop, 
     v2 n_rot = V2(rayn2.x, rayn2.y);
     n_rot = vscl(RL(1.0) / vlen(n_rot), n_rot);
     m2 R = M2(t_rot.x, t_rot.y, n_rot.x, n_rot.y);
-    /* rectilinear boundaries carry no curvature: kappaMat = 0 */
-    m2 kappa = M2(RL(0.0) / FL(2.0), RL(0.0) / FL(2.0), RL(0.0) / FL(2.0), RL(0.0) / FL(2.0));
+    /* kappaMat from the boundary's z curvatures (zero off curvilinear grids) */
+    m2 kappa = M2(z_xx / FL(2.0), z_xy / FL(2.0), z_xy / FL(2.0), z_yy / FL(2.0));
     m2 DMatTemp = m2mul(m2scl(FL(2.0), kappa), R);
     m2 DMat = m2mul(m2tr(R), DMatTemp);
     real cn1jump = v3dot(o.gradc, v3sub(v3neg(rayn1_t), rayn1));
@@ -920,9 +1038,9 @@ static int RayUpdate3(const RayPt3 *p0, RayPt3 *p1, RayPt3 *p2, TraceState3 *S, 
     GetBdrySeg3(p1->x, p1->t, &S->top, &Q->ati, 1, 0, err);
     GetBdrySeg3(p1->x, p1->t, &S->bot, &Q->bty, 0, 0, err);
     if(topRefl || botRefl) {
-        const BState3 *b = topRefl ? &S->top : &S->bot;
-        const HS *hs = topRefl ? &Q->P2.top : &Q->P2.bot;
-        Reflect3(p1, p2, hs, topRefl, b->n, Q, &S->iSeg, err);
+        ReflBdry3 r = refl_bdry3(topRefl ? &Q->ati : &Q->bty, topRefl ? &S->top : &S->bot,
+                                 topRefl ? &Q->P2.top : &Q->P2.bot, p1->x);
+        Reflect3(p1, p2, &r.hs, topRefl, r.n, r.z_xx, r.z_xy, r.z_yy, Q, &S->iSeg, err);
         return 1;
     }
     return 0;
@@ -1537,9 +1655,9 @@ static void StepNx(RayPt ray0, RayPt *ray2, TraceStateNx *S, const Prob3 *Q, Err
     }
 }
 
-static void ReflectNx(const RayPt *oldp, RayPt *newp, const HS *hs, int isTop, v3 nBdry, const OriginNx *org,
-                      const Prob3 *Q, Seg3 *iSeg, ErrState *err)
-{ /* reflect.hpp:94-452 with O3D && !R3D on a rectilinear (uncurved) boundary */
+static void ReflectNx(const RayPt *oldp, RayPt *newp, const HS *hs, int isTop, v3 nBdry, real kxx, real kxy,
+                      real kyy, const OriginNx *org, const Prob3 *Q, Seg3 *iSeg, ErrState *err)
+{ /* reflect.hpp:94-452 with O3D && !R3D */
     const Prob *P = &Q->P2;
     v2 nB = OceanToRayT(nBdry, org);
     nB = vscl(RL(1.0) / vlen(nB), nB);
@@ -1547,8 +1665,7 @@ static void ReflectNx(const RayPt *oldp, RayPt *newp, const HS *hs, int isTop, v
     v2 tB = vsub(oldp->t, vscl(Th, nB));
     tB = vscl(RL(1.0) / vlen(tB), tB);
     real Tg = vdot(oldp->t, tB);
-    /* OceanToRayCurvature (reflect.hpp:94-114) of kappa_xx = kappa_xy = kappa_yy = 0 */
-    const real kxx = RL(0.0), kxy = RL(0.0), kyy = RL(0.0);
+    /* OceanToRayCurvature (reflect.hpp:94-114) */
     real kappa = kxx * SQ(org->tradial.x) + FL(2.0) * kxy * org->tradial.x * org->tradial.y + kyy * SQ(org->tradial.y);
     if(isTop) kappa = -kappa;
     newp->NumTopBnc = oldp->NumTopBnc + isTop;
@@ -1704,9 +1821,9 @@ static int RayUpdateNx(const RayPt *p0, RayPt *p1, RayPt *p2, TraceStateNx *S, c
     GetBdrySeg3(x_o, t_o, &S->top, &Q->ati, 1, 0, err);
     GetBdrySeg3(x_o, t_o, &S->bot, &Q->bty, 0, 0, err);
     if(topRefl || botRefl) {
-        const BState3 *b = topRefl ? &S->top : &S->bot;
-        const HS *hs = topRefl ? &Q->P2.top : &Q->P2.bot;
-        ReflectNx(p1, p2, hs, topRefl, b->n, &S->org, Q, &S->iSeg, err);
+        ReflBdry3 r = refl_bdry3(topRefl ? &Q->ati : &Q->bty, topRefl ? &S->top : &S->bot,
+                                 topRefl ? &Q->P2.top : &Q->P2.bot, x_o);
+        ReflectNx(p1, p2, &r.hs, topRefl, r.n, r.kappa_xx, r.kappa_xy, r.kappa_yy, &S->org, Q, &S->iSeg, err);
         return 1;
     }
     return 0;
@@ -1966,6 +2083,23 @@ int SFX(oracle_preprocess_dump3d)(const bhc_env3d *env, double *dst, int64_t cap
             PUT(n->n1.x); PUT(n->n1.y); PUT(n->n1.z);
             PUT(n->n2.x); PUT(n->n2.y); PUT(n->n2.z);
         }
+    /* curvilinear / province extras of non-default grids */
+    for(int b = 0; b < 2; ++b) {
+        const Bdry3 *B = Bs[b];
+        if(B->type[0] != 'C' && B->type[1] != 'L') continue;
+        for(int i = 0; i < B->nx * B->ny; ++i) {
+            const BNode3 *n = &B->bd[i];
+            PUT(n->Noden.x); PUT(n->Noden.y); PUT(n->Noden.z);
+            PUT(n->z_xx); PUT(n->z_xy); PUT(n->z_yy);
+            PUT(n->kappa_xx); PUT(n->kappa_xy); PUT(n->kappa_yy);
+            PUT(B->type[1] == 'L' ? n->prov : -1);
+        }
+        for(int i = 0; i < B->nprov; ++i) {
+            PUT(creal(B->prov[i].cP)); PUT(cimag(B->prov[i].cP));
+            PUT(creal(B->prov[i].cS)); PUT(cimag(B->prov[i].cS));
+            PUT(B->prov[i].rho);
+        }
+    }
     if(Q.ssp_type == 'H')
         for(size_t i = 0; i < (size_t)Q.hx * Q.hy * (Q.hz - 1); ++i) PUT(Q.czmat[i]);
     for(int i = 0; i < Q.Ntheta; ++i) { PUT(Q.t_rcvr[i].x); PUT(Q.t_rcvr[i].y); }
