@@ -1,12 +1,16 @@
-// mlp.hip -- the TL surrogate: a coordinate MLP trained on tracer fields,
-// forward and backward on gfx950 matrix cores (BF16 inputs, FP32 accumulate).
+// mlp.hip -- the TL surrogate's network: an MLP ensemble trained on tracer
+// fields, forward and backward on gfx950 matrix cores (BF16 inputs, FP32
+// accumulate).
 //
 // Network (one ensemble member; E members are trained side by side):
-//   16 normalised inputs -> 64 Fourier features [sin, cos](2 pi B x), B fixed
-//   -> 6 hidden layers of 256 units, GELU (erf form) -> 1 output (TL in a
-//   normalised scale). Weights are kept in FP32 (master, Adam state) and as a
-//   BF16 copy that the matrix products read; hidden activations are stored in
-//   BF16, pre-activations in FP32.
+//   16 normalised inputs -> 64 features -> 6 hidden layers of 256 units, GELU
+//   (erf form) -> n_out outputs (linear). Features: front 0, [sin, cos](2 pi B x)
+//   of 32 projections B x; front 1, the 16 inputs themselves followed by
+//   [sin, cos] of the first 24 projections. n_out = 1 gives a coordinate
+//   network (TL at one receiver); n_out = K (a multiple of 32, <= 256) gives
+//   the K coefficients of a field expanded in a fixed basis. Weights are kept
+//   in FP32 (master, Adam state) and as a BF16 copy that the matrix products
+//   read; hidden activations are stored in BF16, pre-activations in FP32.
 //
 // Matrix products use v_mfma_f32_32x32x16_bf16: a workgroup of 4 wavefronts
 // computes a 64 x 64 tile of C = A * B, each wavefront one 32 x 32 block, with
@@ -47,6 +51,7 @@ int mfail(const char *fmt, ...)
     } while(0)
 
 constexpr int D_IN = 16, N_FREQ = 32, D_FF = 2 * N_FREQ, WIDTH = 256, N_HID = 6;
+constexpr int N_FREQ_RAW = (D_FF - D_IN) / 2; // projections used by front 1
 constexpr int TILE = 64, KSTEP = 32;
 
 typedef uint16_t bf16_t;
@@ -54,12 +59,12 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 // parameter layout of one member (floats): W0 [D_FF][WIDTH], b0 [WIDTH],
-// W1..W5 [WIDTH][WIDTH], b1..b5 [WIDTH], Wout [WIDTH], bout [1]
+// W1..W5 [WIDTH][WIDTH], b1..b5 [WIDTH], Wout [WIDTH][n_out], bout [n_out]
 constexpr size_t off_W(int l) { return l == 0 ? 0 : (size_t)(D_FF + 1) * WIDTH + (size_t)(l - 1) * (WIDTH + 1) * WIDTH; }
 constexpr size_t off_b(int l) { return off_W(l) + (size_t)(l == 0 ? D_FF : WIDTH) * WIDTH; }
 constexpr size_t OFF_WOUT = off_W(N_HID);
-constexpr size_t OFF_BOUT = OFF_WOUT + WIDTH;
-constexpr size_t N_PARAM = OFF_BOUT + 1;
+constexpr size_t off_bout(int n_out) { return OFF_WOUT + (size_t)WIDTH * n_out; }
+constexpr size_t n_param(int n_out) { return off_bout(n_out) + n_out; }
 
 __host__ __device__ inline bf16_t f2bf(float f)
 { // round to nearest even
@@ -85,7 +90,7 @@ __device__ inline float gelu_d(float z)
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-enum : int { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_ATOMIC = 2 };
+enum : int { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_ATOMIC = 2, EPI_BIAS = 3 };
 
 struct GemmArgs {
     const void *A;
@@ -94,13 +99,14 @@ struct GemmArgs {
     long long sam, sak, sbk, sbn, ldc;
     long long zA, zB, zC;     // member strides (elements)
     int M, N, K;
-    const float *bias;        // EPI_BIAS_GELU: bias[n] (member stride zbias)
+    const float *bias;        // EPI_BIAS_GELU, EPI_BIAS: bias[n] (member stride zbias)
     long long zbias;
     bf16_t *H;                // EPI_BIAS_GELU: GELU(C) as BF16, same layout as C
     int ksplit;               // EPI_ATOMIC: K is split over ksplit slices (grid z = members x ksplit)
 };
 
-// C = A * B (FP32), or with EPI_BIAS_GELU: C = A * B + bias and H = GELU(C).
+// C = A * B (FP32); EPI_BIAS: C = A * B + bias; EPI_BIAS_GELU: C = A * B + bias
+// and H = GELU(C); EPI_ATOMIC: C += A * B over K slices.
 // M, N arbitrary (guarded), K a multiple of 32.
 template<typename TA, typename TB, int EPI>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
@@ -159,74 +165,107 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
         int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         if(m >= g.M) continue;
         float v = acc[r];
-        if constexpr(EPI == EPI_BIAS_GELU) {
-            v += g.bias[z * g.zbias + n];
-            g.H[z * g.zC + m * g.ldc + n] = f2bf(gelu(v));
-        }
+        if constexpr(EPI == EPI_BIAS_GELU || EPI == EPI_BIAS) v += g.bias[z * g.zbias + n];
+        if constexpr(EPI == EPI_BIAS_GELU) g.H[z * g.zC + m * g.ldc + n] = f2bf(gelu(v));
         if constexpr(EPI == EPI_ATOMIC) unsafeAtomicAdd(C + m * g.ldc + n, v);
         else C[m * g.ldc + n] = v;
     }
 }
 
-// Fourier features of normalised inputs: F[b][2j] = sin(2 pi <Bf_j, x_b>),
-// F[b][2j+1] = cos(...), stored BF16 (shared by all members).
-__global__ void k_features(const float *x, int n, const float *Bf, bf16_t *F)
+// Features of normalised inputs x[b][16], stored BF16 (shared by all members).
+// Front 0: F[b][2j] = sin(2 pi <B_j, x_b>), F[b][2j+1] = cos(...), j < 32.
+// Front 1: F[b][d] = x[b][d] for d < 16, then sin / cos of projections j < 24
+// at F[b][16 + 2j], F[b][17 + 2j]. One thread per (b, j), j < 32; under
+// front 1 threads j >= 24 copy two inputs each.
+__global__ void k_features(const float *x, int n, const float *Bf, int front, bf16_t *F)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n * N_FREQ) return;
     int b = i / N_FREQ, j = i % N_FREQ;
+    if(front == 1 && j >= N_FREQ_RAW) {
+        int d = 2 * (j - N_FREQ_RAW);
+        F[b * D_FF + d] = f2bf(x[b * D_IN + d]);
+        F[b * D_FF + d + 1] = f2bf(x[b * D_IN + d + 1]);
+        return;
+    }
     float s = 0.f;
     for(int d = 0; d < D_IN; ++d) s += Bf[j * D_IN + d] * x[b * D_IN + d];
     float sn, cs;
     sincospif(2.0f * s, &sn, &cs);
-    F[b * D_FF + 2 * j] = f2bf(sn);
-    F[b * D_FF + 2 * j + 1] = f2bf(cs);
+    const int o = front == 1 ? D_IN : 0;
+    F[b * D_FF + o + 2 * j] = f2bf(sn);
+    F[b * D_FF + o + 2 * j + 1] = f2bf(cs);
 }
 
-// Output layer: out[e][b] = <H[e][b][:], Wout[e]> + bout[e] (BF16 operands, FP32 sum).
-__global__ void k_out_fwd(const bf16_t *H, const bf16_t *Wbf, const float *P, int n, long long zH, float *out)
+// Single-output layer: out[e][b] = <H[e][b][:], Wout[e]> + bout[e] (BF16
+// operands, FP32 sum). np = parameters per member.
+__global__ void k_out_fwd(const bf16_t *H, const bf16_t *Wbf, const float *P, int n, long long zH, long long zO,
+                          long long np, float *out)
 {
     int e = blockIdx.y, b = blockIdx.x * blockDim.x + threadIdx.x;
     if(b >= n) return;
     const bf16_t *h = H + e * zH + (long long)b * WIDTH;
-    const bf16_t *w = Wbf + e * N_PARAM + OFF_WOUT;
+    const bf16_t *w = Wbf + e * np + OFF_WOUT;
     float s = 0.f;
     for(int k = 0; k < WIDTH; ++k) s += bf2f(h[k]) * bf2f(w[k]);
-    out[(long long)e * n + b] = s + P[e * N_PARAM + OFF_BOUT];
+    out[(long long)e * zO + b] = s + P[e * np + off_bout(1)];
 }
 
-// Loss and output-layer backward for a batch: dOut = 2 (out - y) / n,
+// Loss and single-output backward for a batch: dOut = 2 (out - y) / n,
 // dH5[e][b][k] = dOut * Wout[k] (FP32), loss[e] += sum (out - y)^2 / n.
 __global__ void k_out_bwd(const float *out, const float *y, int n, const bf16_t *Wbf, float *dH, long long zH,
-                          float *dout, float *loss)
+                          long long zO, long long np, float *dout, float *loss)
 {
     int e = blockIdx.y, b = blockIdx.x;
     if(b >= n) return;
-    float d = out[(long long)e * n + b] - y[b];
+    float d = out[(long long)e * zO + b] - y[b];
     float g = 2.0f * d / (float)n;
     if(threadIdx.x == 0) {
-        dout[(long long)e * n + b] = g;
+        dout[(long long)e * zO + b] = g;
         atomicAdd(loss + e, d * d / (float)n);
     }
-    const bf16_t *w = Wbf + e * N_PARAM + OFF_WOUT;
+    const bf16_t *w = Wbf + e * np + OFF_WOUT;
     for(int k = threadIdx.x; k < WIDTH; k += blockDim.x) dH[e * zH + (long long)b * WIDTH + k] = g * bf2f(w[k]);
+}
+
+// Loss and output gradient of a multi-output batch (K outputs per point):
+// dOut = 2 (out - y) / (n K), loss[e] += sum (out - y)^2 / (n K). One block of
+// 256 threads per (chunk of 256 elements, member).
+__global__ void k_out_loss(const float *out, const float *y, int total, long long zO, float *dout, float *loss)
+{
+    __shared__ float red[256];
+    int e = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.f;
+    if(i < total) {
+        float d = out[e * zO + i] - y[i];
+        dout[e * zO + i] = 2.0f * d / (float)total;
+        d2 = d * d / (float)total;
+    }
+    red[threadIdx.x] = d2;
+    __syncthreads();
+    for(int h = 128; h > 0; h >>= 1) {
+        if((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) atomicAdd(loss + e, red[0]);
 }
 
 constexpr int RCHUNK = 128; // batch rows per block of the column reductions
 
-// dWout[e][k] += sum over a chunk of b of H5[e][b][k] dOut[e][b]; dbout likewise
-// (G's output-layer slots are zeroed first).
-__global__ void k_out_wgrad(const bf16_t *H, const float *dout, int n, long long zH, float *G)
+// Single output: dWout[e][k] += sum over a chunk of b of H5[e][b][k] dOut[e][b];
+// dbout likewise (G's output-layer slots are zeroed first).
+__global__ void k_out_wgrad(const bf16_t *H, const float *dout, int n, long long zH, long long zO, long long np,
+                            float *G)
 {
     int e = blockIdx.y, k = threadIdx.x;
     int b0 = blockIdx.x * RCHUNK, b1 = min(n, b0 + RCHUNK);
     float s = 0.f;
     if(k < WIDTH) {
-        for(int b = b0; b < b1; ++b) s += bf2f(H[e * zH + (long long)b * WIDTH + k]) * dout[(long long)e * n + b];
-        unsafeAtomicAdd(G + e * N_PARAM + OFF_WOUT + k, s);
+        for(int b = b0; b < b1; ++b) s += bf2f(H[e * zH + (long long)b * WIDTH + k]) * dout[(long long)e * zO + b];
+        unsafeAtomicAdd(G + e * np + OFF_WOUT + k, s);
     } else if(k == WIDTH) {
-        for(int b = b0; b < b1; ++b) s += dout[(long long)e * n + b];
-        unsafeAtomicAdd(G + e * N_PARAM + OFF_BOUT, s);
+        for(int b = b0; b < b1; ++b) s += dout[(long long)e * zO + b];
+        unsafeAtomicAdd(G + e * np + off_bout(1), s);
     }
 }
 
@@ -245,15 +284,16 @@ __global__ void k_gelu_bwd(float *dH, const float *Z, long long total)
     if(i < total) dH[i] *= gelu_d(Z[i]);
 }
 
-// db[e][c] += sum over a chunk of b of dZ[e][b][c] (one thread per column).
-__global__ void k_bias_grad(const float *dZ, int n, long long zZ, float *G, size_t off)
+// db[e][c] += sum over a chunk of b of dZ[e][b][c], c < cols (one thread per
+// column, rows of `cols` values).
+__global__ void k_bias_grad(const float *dZ, int n, int cols, long long zZ, long long np, float *G, size_t off)
 {
     int e = blockIdx.y, c = threadIdx.x;
     int b0 = blockIdx.x * RCHUNK, b1 = min(n, b0 + RCHUNK);
-    if(c >= WIDTH) return;
+    if(c >= cols) return;
     float s = 0.f;
-    for(int b = b0; b < b1; ++b) s += dZ[e * zZ + (long long)b * WIDTH + c];
-    unsafeAtomicAdd(G + e * N_PARAM + off + c, s);
+    for(int b = b0; b < b1; ++b) s += dZ[e * zZ + (long long)b * cols + c];
+    unsafeAtomicAdd(G + e * np + off + c, s);
 }
 
 // Adam (FP32 master weights) and the BF16 copy the products read.
@@ -278,15 +318,16 @@ __global__ void k_to_bf(const float *P, bf16_t *Pbf, long long total)
     if(i < total) Pbf[i] = f2bf(P[i]);
 }
 
-// Batch gather from the resident training set: idx[b] -> x, y.
-__global__ void k_gather(const float *X, const float *Y, const int32_t *idx, int n, float *x, float *y)
+// Batch gather from the resident training set: idx[b] -> x[b][16], y[b][n_out].
+__global__ void k_gather(const float *X, const float *Y, const int32_t *idx, int n, int n_out, float *x, float *y)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n * D_IN) return;
-    int b = i / D_IN, d = i % D_IN;
-    int s = idx[b];
-    x[i] = X[(long long)s * D_IN + d];
-    if(d == 0) y[b] = Y[s];
+    const int row = D_IN + n_out;
+    if(i >= n * row) return;
+    int b = i / row, d = i % row;
+    long long s = idx[b];
+    if(d < D_IN) x[b * D_IN + d] = X[s * D_IN + d];
+    else y[(long long)b * n_out + (d - D_IN)] = Y[s * n_out + (d - D_IN)];
 }
 
 } // namespace
@@ -295,15 +336,17 @@ struct bhc_mlp {
     int device;
     hipStream_t stream;
     int members, max_batch;
-    float *P, *Mo, *Vo, *G;    // [members][N_PARAM]
+    int n_out, front;          // outputs per point; feature front (0 Fourier, 1 inputs + Fourier)
+    long long np;              // parameters per member
+    float *P, *Mo, *Vo, *G;    // [members][np]
     bf16_t *Pbf;
     float *Bf;                 // [N_FREQ][D_IN]
-    float *x, *y, *out, *dout, *loss;
+    float *x, *y, *out, *dout, *loss; // y [max_batch][n_out], out and dout [members][max_batch][n_out]
     bf16_t *F;                 // [max_batch][D_FF]
     float *Z[N_HID];           // [members][max_batch][WIDTH]
     bf16_t *H[N_HID];
     float *dH, *dH2;           // [members][max_batch][WIDTH]
-    float *X, *Y;              // resident training set
+    float *X, *Y;              // resident training set: X [nset][16], Y [nset][n_out]
     int32_t *idx;
     long long nset;
     long long step;
@@ -346,50 +389,71 @@ void forward(bhc_mlp *m, int n)
 {
     hipStream_t s = m->stream;
     const int E = m->members;
-    const long long zH = (long long)m->max_batch * WIDTH;
-    hipLaunchKernelGGL(k_features, dim3((n * N_FREQ + 255) / 256), dim3(256), 0, s, m->x, n, m->Bf, m->F);
+    const long long np = m->np;
+    const long long zH = (long long)m->max_batch * WIDTH, zO = (long long)m->max_batch * m->n_out;
+    hipLaunchKernelGGL(k_features, dim3((n * N_FREQ + 255) / 256), dim3(256), 0, s, m->x, n, m->Bf, m->front, m->F);
     for(int l = 0; l < N_HID; ++l) {
         const int K = l == 0 ? D_FF : WIDTH;
         const void *A = l == 0 ? (const void *)m->F : (const void *)m->H[l - 1];
         const long long zA = l == 0 ? 0 : zH;
-        gemm<bf16_t, bf16_t, EPI_BIAS_GELU>(s, E, A, K, 1, zA, m->Pbf + off_W(l), WIDTH, 1, N_PARAM, m->Z[l], WIDTH,
-                                            zH, n, WIDTH, K, m->P + off_b(l), N_PARAM, m->H[l]);
+        gemm<bf16_t, bf16_t, EPI_BIAS_GELU>(s, E, A, K, 1, zA, m->Pbf + off_W(l), WIDTH, 1, np, m->Z[l], WIDTH, zH, n,
+                                            WIDTH, K, m->P + off_b(l), np, m->H[l]);
     }
-    hipLaunchKernelGGL(k_out_fwd, dim3((n + 255) / 256, E), dim3(256), 0, s, m->H[N_HID - 1], m->Pbf, m->P, n, zH,
-                       m->out);
+    if(m->n_out == 1) {
+        hipLaunchKernelGGL(k_out_fwd, dim3((n + 255) / 256, E), dim3(256), 0, s, m->H[N_HID - 1], m->Pbf, m->P, n, zH,
+                           zO, np, m->out);
+    } else { // out = H5 Wout + bout: M = n, N = n_out, K = WIDTH
+        gemm<bf16_t, bf16_t, EPI_BIAS>(s, E, m->H[N_HID - 1], WIDTH, 1, zH, m->Pbf + OFF_WOUT, m->n_out, 1, np, m->out,
+                                       m->n_out, zO, n, m->n_out, WIDTH, m->P + off_bout(m->n_out), np);
+    }
 }
 
 // backward of the batch after forward(): gradients into m->G, loss into m->loss
 void backward(bhc_mlp *m, int n)
 {
     hipStream_t s = m->stream;
-    const int E = m->members;
-    const long long zH = (long long)m->max_batch * WIDTH;
-    const long long np = (long long)E * N_PARAM;
+    const int E = m->members, K_out = m->n_out;
+    const long long np = m->np;
+    const long long zH = (long long)m->max_batch * WIDTH, zO = (long long)m->max_batch * K_out;
+    const long long npt = (long long)E * np;
     const int nchunk = (n + RCHUNK - 1) / RCHUNK;
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, m->G, np);
+    const int ksplit = std::max(1, std::min(64, n / 512));
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, s, m->G, npt);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, m->loss, (long long)E);
-    hipLaunchKernelGGL(k_out_bwd, dim3(n, E), dim3(256), 0, s, m->out, m->y, n, m->Pbf, m->dH, zH, m->dout,
-                       m->loss);
-    hipLaunchKernelGGL(k_out_wgrad, dim3(nchunk, E), dim3(WIDTH + 64), 0, s, m->H[N_HID - 1], m->dout, n, zH,
-                       m->G);
+    if(K_out == 1) {
+        hipLaunchKernelGGL(k_out_bwd, dim3(n, E), dim3(256), 0, s, m->out, m->y, n, m->Pbf, m->dH, zH, zO, np, m->dout,
+                           m->loss);
+        hipLaunchKernelGGL(k_out_wgrad, dim3(nchunk, E), dim3(WIDTH + 64), 0, s, m->H[N_HID - 1], m->dout, n, zH, zO,
+                           np, m->G);
+    } else {
+        const int total = n * K_out;
+        hipLaunchKernelGGL(k_out_loss, dim3((total + 255) / 256, E), dim3(256), 0, s, m->out, m->y, total, zO,
+                           m->dout, m->loss);
+        // dWout = H5^T dOut : M = WIDTH, N = n_out, K = n
+        gemm<bf16_t, float, EPI_ATOMIC>(s, E, m->H[N_HID - 1], 1, WIDTH, zH, m->dout, K_out, 1, zO, m->G + OFF_WOUT,
+                                        K_out, np, WIDTH, K_out, n, nullptr, 0, nullptr, ksplit);
+        hipLaunchKernelGGL(k_bias_grad, dim3(nchunk, E), dim3(WIDTH), 0, s, m->dout, n, K_out, zO, np, m->G,
+                           off_bout(K_out));
+        // dH5 = dOut Wout^T : M = n, N = WIDTH, K = n_out
+        gemm<float, bf16_t, EPI_STORE>(s, E, m->dout, K_out, 1, zO, m->Pbf + OFF_WOUT, 1, K_out, np, m->dH, WIDTH, zH,
+                                       n, WIDTH, K_out);
+    }
     float *dH = m->dH, *dHn = m->dH2;
     for(int l = N_HID - 1; l >= 0; --l) {
         const long long tot = (long long)E * zH;
         hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dH, m->Z[l], tot);
-        hipLaunchKernelGGL(k_bias_grad, dim3(nchunk, E), dim3(WIDTH), 0, s, dH, n, zH, m->G, off_b(l));
+        hipLaunchKernelGGL(k_bias_grad, dim3(nchunk, E), dim3(WIDTH), 0, s, dH, n, WIDTH, zH, np, m->G, off_b(l));
         const int K_in = l == 0 ? D_FF : WIDTH;
         // dW = Hprev^T dZ : M = K_in, N = WIDTH, K = n, split over K so that
         // the 16 (or 4) output tiles per member still fill the GPU
         const void *Hp = l == 0 ? (const void *)m->F : (const void *)m->H[l - 1];
         const long long zHp = l == 0 ? 0 : zH;
-        const int ksplit = std::max(1, std::min(64, n / 512));
-        gemm<bf16_t, float, EPI_ATOMIC>(s, E, Hp, 1, K_in, zHp, dH, WIDTH, 1, zH, m->G + off_W(l), WIDTH, N_PARAM,
-                                        K_in, WIDTH, n, nullptr, 0, nullptr, ksplit);
+        gemm<bf16_t, float, EPI_ATOMIC>(s, E, Hp, 1, K_in, zHp, dH, WIDTH, 1, zH, m->G + off_W(l), WIDTH, np, K_in,
+                                        WIDTH, n, nullptr, 0, nullptr, ksplit);
         if(l > 0) {
             // dHprev = dZ W^T : M = n, N = WIDTH (K_in), K = WIDTH
-            gemm<float, bf16_t, EPI_STORE>(s, E, dH, WIDTH, 1, zH, m->Pbf + off_W(l), 1, WIDTH, N_PARAM, dHn, WIDTH,
-                                           zH, n, K_in, WIDTH);
+            gemm<float, bf16_t, EPI_STORE>(s, E, dH, WIDTH, 1, zH, m->Pbf + off_W(l), 1, WIDTH, np, dHn, WIDTH, zH, n,
+                                           K_in, WIDTH);
             float *t = dH;
             dH = dHn;
             dHn = t;
@@ -401,7 +465,9 @@ void backward(bhc_mlp *m, int n)
 
 extern "C" const char *bhc_mlp_last_error(void) { return g_mlp_error.c_str(); }
 
-extern "C" int bhc_mlp_num_params(void) { return (int)N_PARAM; }
+extern "C" int bhc_mlp_num_params(void) { return (int)n_param(1); }
+
+extern "C" long long bhc_mlp_param_count(bhc_mlp *m) { return m ? m->np : 0; }
 
 extern "C" void bhc_mlp_destroy(bhc_mlp *m)
 {
@@ -422,31 +488,39 @@ extern "C" void bhc_mlp_destroy(bhc_mlp *m)
     delete m;
 }
 
-// Create an ensemble of `members` networks for batches of up to max_batch
-// points (a multiple of 32). Weights: He-uniform from `seed`, biases 0; the
-// Fourier matrix has N(0, sigma^2) entries from the same seed.
-extern "C" int bhc_mlp_create(int device, int members, int max_batch, unsigned long long seed, float sigma,
-                              bhc_mlp **out)
+// Create an ensemble of `members` networks with n_out outputs (1, or a
+// multiple of 32 up to 256) and feature front `front` (0 or 1), for batches of
+// up to max_batch points (a multiple of 32). Weights: He-uniform from `seed`,
+// biases and the output layer 0; the projection matrix has N(0, sigma^2)
+// entries from the same seed.
+extern "C" int bhc_mlp_create_ex(int device, int members, int max_batch, int n_out, int front,
+                                 unsigned long long seed, float sigma, bhc_mlp **out)
 {
     *out = nullptr;
     if(members < 1 || members > 16 || max_batch < 32 || max_batch % 32) return mfail("bhc_mlp_create: bad sizes");
+    if(!(n_out == 1 || (n_out % 32 == 0 && n_out <= WIDTH))) return mfail("bhc_mlp_create: n_out must be 1 or a multiple of 32 <= 256");
+    if(front != 0 && front != 1) return mfail("bhc_mlp_create: front must be 0 or 1");
     MCHECK(hipSetDevice(device));
     bhc_mlp *m = new bhc_mlp();
     memset(m, 0, sizeof(*m));
     m->device = device;
     m->members = members;
     m->max_batch = max_batch;
+    m->n_out = n_out;
+    m->front = front;
+    m->np = (long long)n_param(n_out);
     MCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     MCHECK(hipEventCreate(&m->e0));
     MCHECK(hipEventCreate(&m->e1));
-    size_t np = (size_t)members * N_PARAM;
+    size_t np = (size_t)members * m->np;
     size_t act = (size_t)members * max_batch * WIDTH;
+    size_t outs = (size_t)max_batch * n_out;
     bool bad = hipMalloc((void **)&m->P, np * 4) || hipMalloc((void **)&m->Mo, np * 4)
         || hipMalloc((void **)&m->Vo, np * 4) || hipMalloc((void **)&m->G, np * 4)
         || hipMalloc((void **)&m->Pbf, np * 2) || hipMalloc((void **)&m->Bf, N_FREQ * D_IN * 4)
-        || hipMalloc((void **)&m->x, (size_t)max_batch * D_IN * 4) || hipMalloc((void **)&m->y, (size_t)max_batch * 4)
-        || hipMalloc((void **)&m->out, (size_t)members * max_batch * 4)
-        || hipMalloc((void **)&m->dout, (size_t)members * max_batch * 4) || hipMalloc((void **)&m->loss, 16 * 4)
+        || hipMalloc((void **)&m->x, (size_t)max_batch * D_IN * 4) || hipMalloc((void **)&m->y, outs * 4)
+        || hipMalloc((void **)&m->out, (size_t)members * outs * 4)
+        || hipMalloc((void **)&m->dout, (size_t)members * outs * 4) || hipMalloc((void **)&m->loss, 16 * 4)
         || hipMalloc((void **)&m->F, (size_t)max_batch * D_FF * 2) || hipMalloc((void **)&m->dH, act * 4)
         || hipMalloc((void **)&m->dH2, act * 4) || hipMalloc((void **)&m->idx, (size_t)max_batch * 4);
     for(int l = 0; l < N_HID && !bad; ++l)
@@ -463,11 +537,11 @@ extern "C" int bhc_mlp_create(int device, int members, int max_batch, unsigned l
     // network trained on residuals starts from the reference it corrects
     for(int e = 0; e < members; ++e) {
         for(int l = 0; l < N_HID; ++l) {
-            int fan_in = l == 0 ? D_FF : WIDTH, fan_out = l == N_HID ? 1 : WIDTH;
+            int fan_in = l == 0 ? D_FF : WIDTH;
             double a = std::sqrt(6.0 / fan_in);
             std::uniform_real_distribution<double> ud(-a, a);
-            size_t o = (size_t)e * N_PARAM + (l == N_HID ? OFF_WOUT : off_W(l));
-            for(int i = 0; i < fan_in * fan_out; ++i) P[o + i] = (float)ud(rng);
+            size_t o = (size_t)e * m->np + off_W(l);
+            for(int i = 0; i < fan_in * WIDTH; ++i) P[o + i] = (float)ud(rng);
         }
     }
     MCHECK(hipMemcpy(m->P, P.data(), np * 4, hipMemcpyHostToDevice));
@@ -481,11 +555,18 @@ extern "C" int bhc_mlp_create(int device, int members, int max_batch, unsigned l
     return 0;
 }
 
+// A coordinate network (one output, Fourier front).
+extern "C" int bhc_mlp_create(int device, int members, int max_batch, unsigned long long seed, float sigma,
+                              bhc_mlp **out)
+{
+    return bhc_mlp_create_ex(device, members, max_batch, 1, 0, seed, sigma, out);
+}
+
 extern "C" int bhc_mlp_get_params(bhc_mlp *m, float *params, float *fourier)
 {
     MCHECK(hipSetDevice(m->device));
     MCHECK(hipStreamSynchronize(m->stream));
-    if(params) MCHECK(hipMemcpy(params, m->P, (size_t)m->members * N_PARAM * 4, hipMemcpyDeviceToHost));
+    if(params) MCHECK(hipMemcpy(params, m->P, (size_t)m->members * m->np * 4, hipMemcpyDeviceToHost));
     if(fourier) MCHECK(hipMemcpy(fourier, m->Bf, N_FREQ * D_IN * 4, hipMemcpyDeviceToHost));
     return 0;
 }
@@ -502,7 +583,7 @@ extern "C" int bhc_mlp_set_fourier(bhc_mlp *m, const float *B)
 extern "C" int bhc_mlp_set_params(bhc_mlp *m, const float *params)
 {
     MCHECK(hipSetDevice(m->device));
-    size_t np = (size_t)m->members * N_PARAM;
+    size_t np = (size_t)m->members * m->np;
     MCHECK(hipMemcpy(m->P, params, np * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, m->stream, m->P, m->Pbf,
                        (long long)np);
@@ -510,42 +591,45 @@ extern "C" int bhc_mlp_set_params(bhc_mlp *m, const float *params)
     return 0;
 }
 
-// Predictions of every member for n points x[n][16] (host); out[members][n].
+// Predictions of every member for n points x[n][16] (host); out[members][n][n_out].
 extern "C" int bhc_mlp_predict(bhc_mlp *m, const float *x, long long n, float *out)
 {
     MCHECK(hipSetDevice(m->device));
-    std::vector<float> tmp((size_t)m->members * m->max_batch);
+    const int K = m->n_out;
+    const long long zO = (long long)m->max_batch * K;
+    std::vector<float> tmp((size_t)m->members * zO);
     for(long long b0 = 0; b0 < n; b0 += m->max_batch) {
         int nb = (int)std::min<long long>(m->max_batch, n - b0);
         MCHECK(hipMemcpyAsync(m->x, x + b0 * D_IN, (size_t)nb * D_IN * 4, hipMemcpyHostToDevice, m->stream));
         forward(m, nb);
-        MCHECK(hipMemcpyAsync(tmp.data(), m->out, (size_t)m->members * nb * 4, hipMemcpyDeviceToHost, m->stream));
+        MCHECK(hipMemcpyAsync(tmp.data(), m->out, (size_t)m->members * zO * 4, hipMemcpyDeviceToHost, m->stream));
         MCHECK(hipStreamSynchronize(m->stream));
         for(int e = 0; e < m->members; ++e)
-            memcpy(out + (size_t)e * n + b0, tmp.data() + (size_t)e * nb, (size_t)nb * 4);
+            memcpy(out + ((size_t)e * n + b0) * K, tmp.data() + (size_t)e * zO, (size_t)nb * K * 4);
     }
     return 0;
 }
 
-// Gradients (FP32, [members][N_PARAM]) and per-member loss for one batch
-// (n <= max_batch points, host arrays), without updating the weights.
+// Gradients (FP32, [members][params]) and per-member loss for one batch
+// (n <= max_batch points, x [n][16], y [n][n_out], host arrays), without
+// updating the weights.
 extern "C" int bhc_mlp_gradients(bhc_mlp *m, const float *x, const float *y, int n, float *grads, float *loss)
 {
     if(n < 1 || n > m->max_batch || n % 32) return mfail("bhc_mlp_gradients: n must be a multiple of 32 <= max_batch");
     MCHECK(hipSetDevice(m->device));
     MCHECK(hipMemcpyAsync(m->x, x, (size_t)n * D_IN * 4, hipMemcpyHostToDevice, m->stream));
-    MCHECK(hipMemcpyAsync(m->y, y, (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
+    MCHECK(hipMemcpyAsync(m->y, y, (size_t)n * m->n_out * 4, hipMemcpyHostToDevice, m->stream));
     forward(m, n);
     backward(m, n);
     MCHECK(hipGetLastError());
     if(grads)
-        MCHECK(hipMemcpyAsync(grads, m->G, (size_t)m->members * N_PARAM * 4, hipMemcpyDeviceToHost, m->stream));
+        MCHECK(hipMemcpyAsync(grads, m->G, (size_t)m->members * m->np * 4, hipMemcpyDeviceToHost, m->stream));
     if(loss) MCHECK(hipMemcpyAsync(loss, m->loss, (size_t)m->members * 4, hipMemcpyDeviceToHost, m->stream));
     MCHECK(hipStreamSynchronize(m->stream));
     return 0;
 }
 
-// Resident training set: X[n][16] normalised inputs, Y[n] normalised targets.
+// Resident training set: X[n][16] normalised inputs, Y[n][n_out] normalised targets.
 extern "C" int bhc_mlp_set_dataset(bhc_mlp *m, const float *X, const float *Y, long long n)
 {
     MCHECK(hipSetDevice(m->device));
@@ -553,9 +637,9 @@ extern "C" int bhc_mlp_set_dataset(bhc_mlp *m, const float *X, const float *Y, l
     if(m->Y) (void)hipFree(m->Y);
     m->X = m->Y = nullptr;
     MCHECK(hipMalloc((void **)&m->X, (size_t)n * D_IN * 4));
-    MCHECK(hipMalloc((void **)&m->Y, (size_t)n * 4));
+    MCHECK(hipMalloc((void **)&m->Y, (size_t)n * m->n_out * 4));
     MCHECK(hipMemcpy(m->X, X, (size_t)n * D_IN * 4, hipMemcpyHostToDevice));
-    MCHECK(hipMemcpy(m->Y, Y, (size_t)n * 4, hipMemcpyHostToDevice));
+    MCHECK(hipMemcpy(m->Y, Y, (size_t)n * m->n_out * 4, hipMemcpyHostToDevice));
     m->nset = n;
     return 0;
 }
@@ -576,11 +660,11 @@ extern "C" int bhc_mlp_train(bhc_mlp *m, int steps, int batch, float lr, unsigne
     MCHECK(hipMalloc((void **)&d_idx, idx.size() * 4));
     MCHECK(hipMemcpy(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const long long np = (long long)m->members * N_PARAM;
+    const long long np = (long long)m->members * m->np;
     MCHECK(hipEventRecord(m->e0, m->stream));
     for(int s = 0; s < steps; ++s) {
-        hipLaunchKernelGGL(k_gather, dim3((batch * D_IN + 255) / 256), dim3(256), 0, m->stream, m->X, m->Y,
-                           d_idx + (size_t)s * batch, batch, m->x, m->y);
+        hipLaunchKernelGGL(k_gather, dim3((batch * (D_IN + m->n_out) + 255) / 256), dim3(256), 0, m->stream, m->X,
+                           m->Y, d_idx + (size_t)s * batch, batch, m->n_out, m->x, m->y);
         forward(m, batch);
         backward(m, batch);
         ++m->step;
@@ -602,7 +686,8 @@ extern "C" int bhc_mlp_train(bhc_mlp *m, int steps, int batch, float lr, unsigne
 extern "C" double bhc_mlp_last_ms(bhc_mlp *m) { return m ? m->last_ms : 0.0; }
 
 // Matrix-core FLOPs of one training step on `batch` points (forward +
-// backward products of the hidden layers, all members).
+// backward products of the hidden layers and, with several outputs, of the
+// output layer; all members).
 extern "C" double bhc_mlp_step_flops(bhc_mlp *m, int batch)
 {
     double f = 0.0;
@@ -610,5 +695,6 @@ extern "C" double bhc_mlp_step_flops(bhc_mlp *m, int batch)
         double k = l == 0 ? D_FF : WIDTH;
         f += 2.0 * batch * k * WIDTH * (l == 0 ? 2.0 : 3.0); // forward, dW, (dH for l > 0)
     }
+    if(m->n_out > 1) f += 2.0 * batch * WIDTH * m->n_out * 3.0;
     return f * m->members;
 }

@@ -1,21 +1,31 @@
 """Learned TL surrogate with tracer fallback.
 
-An ensemble of coordinate MLPs (libbhc_hip.so, bhc_mlp_*: 16 inputs -> 64
-Fourier features -> 6 x 256 GELU -> TL, BF16 matrix-core forward/backward,
-Adam) predicts the TL (dB) of a receiver from its position and the
-environment's parameters. It is trained on fields the batched tracer computes
-for a family of environments; on new environments the ensemble's spread
-decides where its prediction is trusted and where the tracer runs instead.
+A field of an environment family is written as the mean training field plus
+K orthonormal modes (the leading principal components of the training
+fields, receiver by receiver) with coefficients that an ensemble of MLPs
+predicts from the environment's parameters (libbhc_hip.so, bhc_mlp_*: 16
+normalised parameters -> the inputs and 24 Fourier pairs -> 6 x 256 GELU -> K
+coefficients, BF16 matrix-core forward/backward, Adam). TL is learned capped
+at 120 dB, the level above which receivers count as shadowed. On new
+environments the ensemble's spread decides, environment by environment,
+whether its field is the answer or the batched tracer runs instead; errors
+against the tracer are counted at water-column receivers whose tracer TL is
+below 120 dB.
 
-Environment family (14 parameters, each mapped from [0, 1] or N(0, 1)):
+A coordinate network (TL of one receiver from its position and the
+parameters, one output) uses the same kernels with n_out = 1.
+
+Environment family (14 parameters):
   source depth 20-500 m; frequency 50-500 Hz (log-uniform); 8 sound-speed
   EOFs -- sin(k pi z / D), k = 1..8, amplitude N(0, 1) x 5 m/s -- on the Munk
   profile over D = 4000 m; and 4 seamount parameters: height 0-2000 m, centre
   10-40 km, near-flank width 2-10 km and far-flank width 0.5-2 times it (an
   asymmetric Gaussian); bottom an acousto-elastic halfspace of 1600 m/s.
   Receivers: 51 depths x 201 ranges to 50 km; 1001 Gaussian beams over
-  +-30 degrees, incoherent TL (the smooth field a coordinate network can
-  represent; coherent interference is left to the tracer).
+  +-30 degrees, incoherent TL. Family "global" draws every parameter over its
+  full range; family "local" is an uncertainty sweep around one nominal
+  environment (each [0, 1] parameter within +-0.05 of its nominal value, EOF
+  amplitudes N(0, 0.2^2)).
 """
 import ctypes as C
 import math
@@ -27,9 +37,14 @@ from . import _abi, lib, BhcError, Problem, Sweep, Resident
 from .env import Env2D, Halfspace, Boundary, subtab, munk_c
 
 D_IN = 16
-TL_MIN, TL_MAX, TL_SCALE = 40.0, 140.0, 25.0
+TL_MIN, TL_MAX, TL_CAP = 40.0, 140.0, 120.0
 DEPTH = 4000.0
 R_MAX_KM = 50.0
+N_PARAMS = 14
+# family "local": nominal parameters and half-widths (uniform parameters) or
+# standard deviations (EOF amplitudes, N(0, 1) units)
+LOCAL_CENTRE = np.array([0.3, 0.5] + [0.0] * 8 + [0.5] * 4)
+LOCAL_SPREAD = np.array([0.05, 0.05] + [0.2] * 8 + [0.05] * 4)
 
 
 def _mlp_lib():
@@ -39,9 +54,12 @@ def _mlp_lib():
         fp = P(C.c_float)
         for name, res, args in [
             ("bhc_mlp_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_float, P(C.c_void_p)]),
+            ("bhc_mlp_create_ex", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_float,
+                                            P(C.c_void_p)]),
             ("bhc_mlp_destroy", None, [C.c_void_p]),
             ("bhc_mlp_last_error", C.c_char_p, []),
             ("bhc_mlp_num_params", C.c_int, []),
+            ("bhc_mlp_param_count", C.c_longlong, [C.c_void_p]),
             ("bhc_mlp_get_params", C.c_int, [C.c_void_p, fp, fp]),
             ("bhc_mlp_set_params", C.c_int, [C.c_void_p, fp]),
             ("bhc_mlp_set_fourier", C.c_int, [C.c_void_p, fp]),
@@ -64,15 +82,18 @@ def _fp(a):
 
 
 class Mlp:
-    """Ensemble of `members` networks (bhc_mlp_*)."""
+    """Ensemble of `members` networks (bhc_mlp_*) with n_out outputs and
+    feature front `front` (0: Fourier features only; 1: the inputs and 24
+    Fourier pairs)."""
 
-    def __init__(self, members=4, max_batch=16384, seed=0, sigma=1.0, device=0):
+    def __init__(self, members=4, max_batch=16384, seed=0, sigma=1.0, device=0, n_out=1, front=0):
         L = _mlp_lib()
         h = C.c_void_p()
-        if L.bhc_mlp_create(device, members, max_batch, seed, sigma, C.byref(h)) != 0:
+        if L.bhc_mlp_create_ex(device, members, max_batch, n_out, front, seed, sigma, C.byref(h)) != 0:
             raise BhcError(L.bhc_mlp_last_error().decode())
         self._h, self.members, self.max_batch = h, members, max_batch
-        self.nparam = L.bhc_mlp_num_params()
+        self.n_out, self.front = n_out, front
+        self.nparam = int(L.bhc_mlp_param_count(h))
 
     def _check(self, rc):
         if rc != 0:
@@ -91,25 +112,32 @@ class Mlp:
 
     def set_params(self, P):
         P = np.ascontiguousarray(P, np.float32)
+        assert P.shape == (self.members, self.nparam)
         self._check(_mlp_lib().bhc_mlp_set_params(self._h, _fp(P)))
 
     def predict(self, x):
+        """[members][n] for one output, [members][n][n_out] otherwise."""
         x = np.ascontiguousarray(x, np.float32)
-        out = np.zeros((self.members, x.shape[0]), np.float32)
+        assert x.ndim == 2 and x.shape[1] == D_IN
+        out = np.zeros((self.members, x.shape[0], self.n_out), np.float32)
         self._check(_mlp_lib().bhc_mlp_predict(self._h, _fp(x), x.shape[0], _fp(out)))
-        return out
+        return out[..., 0] if self.n_out == 1 else out
 
-    def gradients(self, x, y):
+    def _targets(self, x, y):
         x = np.ascontiguousarray(x, np.float32)
         y = np.ascontiguousarray(y, np.float32)
+        assert x.ndim == 2 and x.shape[1] == D_IN and y.size == x.shape[0] * self.n_out
+        return x, y
+
+    def gradients(self, x, y):
+        x, y = self._targets(x, y)
         g = np.zeros((self.members, self.nparam), np.float32)
         loss = np.zeros(self.members, np.float32)
         self._check(_mlp_lib().bhc_mlp_gradients(self._h, _fp(x), _fp(y), x.shape[0], _fp(g), _fp(loss)))
         return g, loss
 
     def set_dataset(self, X, Y):
-        self._X = np.ascontiguousarray(X, np.float32)
-        self._Y = np.ascontiguousarray(Y, np.float32)
+        self._X, self._Y = self._targets(X, Y)
         self._check(_mlp_lib().bhc_mlp_set_dataset(self._h, _fp(self._X), _fp(self._Y), self._X.shape[0]))
 
     def train(self, steps, batch, lr=1e-3, seed=0):
@@ -139,14 +167,24 @@ class Mlp:
 # environment family
 # --------------------------------------------------------------------------
 
-def draw_params(n, seed):
-    """n parameter vectors: [zs, logf, h, rc, w, asym] in [0, 1] and 8 EOF
-    amplitudes N(0, 1) -> array [n, 14] ordered (zs, logf, eof1..8, h, rc, w, asym)."""
+def draw_params(n, seed, family="global"):
+    """n parameter vectors [n, 14] ordered (zs, logf, eof1..8, h, rc, w, asym):
+    zs, logf and the seamount's four in [0, 1], the EOF amplitudes in N(0, 1)
+    units (family "global"), or within the local sweep around the nominal
+    environment (family "local")."""
     rng = np.random.default_rng(seed)
-    th = np.zeros((n, 14))
-    th[:, 0:2] = rng.random((n, 2))
-    th[:, 2:10] = rng.normal(size=(n, 8))
-    th[:, 10:14] = rng.random((n, 4))
+    th = np.zeros((n, N_PARAMS))
+    if family == "global":
+        th[:, 0:2] = rng.random((n, 2))
+        th[:, 2:10] = rng.normal(size=(n, 8))
+        th[:, 10:14] = rng.random((n, 4))
+    elif family == "local":
+        th[:] = LOCAL_CENTRE
+        th[:, 0:2] += LOCAL_SPREAD[0:2] * (2.0 * rng.random((n, 2)) - 1.0)
+        th[:, 2:10] += LOCAL_SPREAD[2:10] * rng.normal(size=(n, 8))
+        th[:, 10:14] += LOCAL_SPREAD[10:14] * (2.0 * rng.random((n, 4)) - 1.0)
+    else:
+        raise ValueError("family must be 'global' or 'local'")
     return th
 
 
@@ -182,42 +220,37 @@ def seabed_depth(theta, r_km):
     return DEPTH - h * np.exp(-0.5 * ((r - rc) / w) ** 2)
 
 
-# Frequency scale of each input in the Fourier projection: range and depth
-# carry the field's structure, the environment parameters modulate it
-# smoothly (low frequencies there keep the network from keying on individual
-# training environments).
-INPUT_SCALES = np.array([2.0, 2.0, 0.5, 0.5] + [0.25] * 8 + [0.5] * 4)
-
-
-def fourier_matrix(seed, kind="random"):
-    """Projection B [32][16] of the features sin / cos(2 pi B x). "random":
-    Gaussian rows scaled per input (INPUT_SCALES); "structured": 10 range
-    and 8 depth rows at geometrically spaced frequencies (0.5-32 and 0.5-16
-    cycles over the normalised span), the other 14 rows random mixtures."""
-    rng = np.random.default_rng(seed)
-    if kind == "random":
-        return (rng.normal(size=(32, 16)) * INPUT_SCALES[None, :]).astype(np.float32)
-    if kind != "structured":
-        raise ValueError(kind)
-    B = np.zeros((32, 16), np.float32)
-    B[np.arange(10), 0] = np.geomspace(0.5, 32.0, 10)
-    B[10 + np.arange(8), 1] = np.geomspace(0.5, 16.0, 8)
-    B[18:] = rng.normal(size=(14, 16)) * np.array([1.0, 1.0] + [0.5] * 14)[None, :]
-    return B
-
-
-def inputs(theta, rz, rr_km):
-    """Network inputs [n_rcv, 16] of one environment's receiver grid."""
-    Z, R = np.meshgrid(np.asarray(rz, np.float64), np.asarray(rr_km, np.float64), indexing="ij")
-    n = Z.size
-    x = np.zeros((n, D_IN), np.float32)
-    x[:, 0] = (R.reshape(-1) / R_MAX_KM)
-    x[:, 1] = (Z.reshape(-1) / DEPTH)
-    x[:, 2] = theta[1]
-    x[:, 3] = theta[0]
-    x[:, 4:12] = np.asarray(theta[2:10]) / 3.0
-    x[:, 12:16] = np.asarray(theta[10:14])
+def param_inputs(thetas, family="global"):
+    """Network inputs [n, 16]: the 14 parameters scaled to O(1) for the family
+    (uniform parameters to [-1, 1], EOF amplitudes in standard deviations),
+    then two zeros."""
+    th = np.atleast_2d(np.asarray(thetas, np.float64))
+    if family == "local":
+        t = (th - LOCAL_CENTRE) / LOCAL_SPREAD
+    elif family == "global":
+        t = th.copy()
+        t[:, 0:2] = 2.0 * t[:, 0:2] - 1.0
+        t[:, 10:14] = 2.0 * t[:, 10:14] - 1.0
+    else:
+        raise ValueError("family must be 'global' or 'local'")
+    x = np.zeros((th.shape[0], D_IN), np.float32)
+    x[:, :N_PARAMS] = t
     return x
+
+
+def water_mask(theta, rz, rr_km):
+    """Receivers [nrz, nrr] in the water column (at or above the seabed)."""
+    return np.asarray(rz, np.float64)[:, None] <= seabed_depth(theta, rr_km)[None, :]
+
+
+def fourier_matrix(seed, sigma=0.05):
+    """Projection B [32][16] of the Fourier features: N(0, sigma^2) entries on
+    the 14 parameter inputs (small: the parameters act smoothly), zero on the
+    two padding inputs."""
+    rng = np.random.default_rng(seed)
+    B = np.zeros((32, 16), np.float32)
+    B[:, :N_PARAMS] = rng.normal(scale=sigma, size=(32, N_PARAMS))
+    return B
 
 
 def tl_db(problem, field):
@@ -246,76 +279,129 @@ def trace_fields(thetas, ctx, chunk=256):
     return out, envs, t_sweep
 
 
-def reference_field(fields):
-    """Mean TL over the training fields, receiver by receiver: the network
-    learns each environment's deviation from it."""
-    return np.mean(np.stack(fields), axis=0)
+# --------------------------------------------------------------------------
+# modal basis and the surrogate
+# --------------------------------------------------------------------------
+
+class ModalBasis:
+    """Mean field and K orthonormal modes of a set of training fields (TL in
+    dB capped at TL_CAP), from a randomised range finder (two power
+    iterations) and the SVD of the projected fields."""
+
+    def __init__(self, fields, k, seed=0, oversample=32, power=2):
+        T = np.minimum(np.stack(fields), TL_CAP)
+        self.shape = T.shape[1:]
+        X = T.reshape(T.shape[0], -1).astype(np.float32)
+        self.ref = X.mean(0)
+        X -= self.ref
+        rng = np.random.default_rng(seed)
+        l = min(k + oversample, X.shape[0], X.shape[1])
+        Q = X @ rng.standard_normal((X.shape[1], l)).astype(np.float32)
+        for _ in range(power):
+            Q, _ = np.linalg.qr(Q)
+            Q = X @ (X.T @ Q)
+        Q, _ = np.linalg.qr(Q)
+        _, sv, Vt = np.linalg.svd((Q.T @ X).astype(np.float64), full_matrices=False)
+        self.k = k
+        self.modes = np.ascontiguousarray(Vt[:k], np.float32)        # [k][n_rcv]
+        self.singular = sv[:k]
+        c = X @ self.modes.T
+        self.scale = float(np.std(c[:, 0])) or 1.0                   # coefficient unit of the network
+        self.train_coeffs = (c / self.scale).astype(np.float32)
+
+    def coeffs(self, fields):
+        X = np.minimum(np.stack(fields), TL_CAP).reshape(len(fields), -1).astype(np.float32) - self.ref
+        return (X @ self.modes.T) / self.scale
+
+    def fields(self, coeffs):
+        """Fields [..., nrz, nrr] (dB, clipped to [TL_MIN, TL_CAP]) of network coefficients [..., k]."""
+        c = np.asarray(coeffs, np.float32)
+        f = (c.reshape(-1, self.k) * self.scale) @ self.modes + self.ref
+        return np.clip(f, TL_MIN, TL_CAP).reshape(c.shape[:-1] + self.shape)
 
 
-def dataset(thetas, fields, envs, points_per_field, seed, ref):
-    rng = np.random.default_rng(seed)
-    X, Y = [], []
-    for th, tl, e in zip(thetas, fields, envs):
-        x = inputs(th, e.Rz, e.Rr_km)
-        y = ((tl - ref).reshape(-1) / TL_SCALE).astype(np.float32)
-        k = rng.choice(x.shape[0], size=min(points_per_field, x.shape[0]), replace=False)
-        X.append(x[k])
-        Y.append(y[k])
-    return np.concatenate(X), np.concatenate(Y)
+class Surrogate:
+    """The modal surrogate: a ModalBasis of the training fields and an
+    ensemble of `members` networks mapping parameters to its coefficients."""
+
+    def __init__(self, thetas, fields, family="global", k=64, members=4, seed=11, sigma=0.05, max_batch=1024):
+        self.family = family
+        self.basis = ModalBasis(fields, k, seed=seed)
+        self.mlp = Mlp(members=members, max_batch=max_batch, seed=seed, sigma=sigma, n_out=k, front=1)
+        self.mlp.set_fourier(fourier_matrix(seed, sigma))
+        self.X = param_inputs(thetas, family)
+        self.mlp.set_dataset(self.X, self.basis.train_coeffs)
+        self.steps, self.train_ms = 0, 0.0
+
+    def train(self, steps, batch=512, lr=1e-3, chunk=500, floor=0.02):
+        """Adam with a cosine learning-rate decay from lr to floor * lr, one
+        value per chunk of steps. Returns the last per-member loss."""
+        done, loss = 0, None
+        while done < steps:
+            n = min(chunk, steps - done)
+            f = floor + (1.0 - floor) * 0.5 * (1.0 + math.cos(math.pi * (done + 0.5 * n) / steps))
+            loss = self.mlp.train(n, batch, lr * f, seed=self.steps + done)
+            self.train_ms += self.mlp.last_ms()
+            done += n
+        self.steps += steps
+        return loss
+
+    def predict(self, thetas):
+        """Every member's fields [members, n, nrz, nrr] (dB, clipped)."""
+        c = self.mlp.predict(param_inputs(thetas, self.family))
+        return self.basis.fields(c)
+
+    def close(self):
+        self.mlp.close()
 
 
-def predict_fields(mlp, thetas, envs, ref):
-    """Ensemble predictions (dB) [members, n_env, nrz, nrr]."""
-    xs = [inputs(th, e.Rz, e.Rr_km) for th, e in zip(thetas, envs)]
-    shape = (len(envs[0].Rz), len(envs[0].Rr_km))
-    pred = mlp.predict(np.concatenate(xs)).reshape(mlp.members, len(thetas), *shape)
-    pred = pred * TL_SCALE + ref[None, None]
-    return np.clip(pred, TL_MIN, TL_MAX)
+def screen(pred, masks, spread_db=1.0):
+    """Per environment: ensemble mean and whether it falls back to the tracer
+    -- the RMS over water-column receivers (masks [n, nrz, nrr]) of the
+    ensemble spread (the members' sample standard deviation about their mean)
+    above `spread_db`."""
+    mean = pred.mean(0)
+    spread = pred.std(0, ddof=1)
+    m = np.asarray(masks, bool)
+    rms_spread = np.sqrt(np.sum(np.where(m, spread, 0.0) ** 2, axis=(1, 2)) / np.maximum(m.sum(axis=(1, 2)), 1))
+    return mean, rms_spread > spread_db, rms_spread
 
 
-def screen_sparse(mlp, thetas, envs, ref, spread_db=1.0, stride=(5, 8)):
-    """The screen's decision from a receiver subsample (every stride[0]-th
-    depth and stride[1]-th range): per environment, the RMS of the members'
-    spread there, and whether it exceeds `spread_db` (fall back to the
-    tracer). Costs 1 / (stride[0] * stride[1]) of a full prediction."""
-    rz, rr = np.asarray(envs[0].Rz)[::stride[0]], np.asarray(envs[0].Rr_km)[::stride[1]]
-    x = np.concatenate([inputs(th, rz, rr) for th in thetas])
-    pred = mlp.predict(x).reshape(mlp.members, len(thetas), len(rz), len(rr))
-    pred = np.clip(pred * TL_SCALE + ref[::stride[0], ::stride[1]][None, None], TL_MIN, TL_MAX)
-    spread = pred.max(0) - pred.min(0)
-    rms_spread = np.sqrt(np.mean(spread ** 2, axis=(1, 2)))
-    return rms_spread > spread_db, rms_spread
-
-
-def screened_answer(mlp, thetas, envs, ref, ctx, spread_db=1.0, stride=(5, 8)):
-    """The screened sweep's answer for environments `thetas` / `envs`: the
-    ensemble's spread on a receiver subsample decides, per environment,
-    between the network's full field (ensemble mean) and the batched tracer.
-    Returns (fields [n, nrz, nrr] in dB, fallback flags, spread RMS per
-    environment, seconds predicting, seconds tracing the fallbacks)."""
+def screened_answer(sur, thetas, envs, ctx, spread_db=1.0):
+    """The screened sweep's answer for environments `thetas` / `envs`: per
+    environment, the ensemble mean where the members agree (RMS of the
+    ensemble spread at or below `spread_db`) and the batched tracer's field
+    elsewhere. Returns
+    (fields [n, nrz, nrr] in dB, fallback flags, spread RMS per environment,
+    seconds predicting and screening, seconds tracing the fallbacks)."""
     t0 = time.perf_counter()
-    fallback, rms_spread = screen_sparse(mlp, thetas, envs, ref, spread_db, stride)
-    shape = (len(envs[0].Rz), len(envs[0].Rr_km))
-    final = np.zeros((len(thetas),) + shape)
-    keep = [i for i in range(len(thetas)) if not fallback[i]]
-    if keep:
-        final[keep] = predict_fields(mlp, thetas[keep], [envs[i] for i in keep], ref).mean(0)
+    pred = sur.predict(thetas)
+    masks = np.stack([water_mask(th, e.Rz, e.Rr_km) for th, e in zip(thetas, envs)])
+    final, fallback, rms_spread = screen(pred, masks, spread_db)
+    final = final.astype(np.float64)
     t_pred = time.perf_counter() - t0
     fb = [i for i in range(len(thetas)) if fallback[i]]
     t_fb = 0.0
     if fb:
         t0 = time.perf_counter()
-        f_fb, _, _ = trace_fields(thetas[fb], ctx)
+        f_fb, _, _ = trace_fields(np.asarray(thetas)[fb], ctx)
         t_fb = time.perf_counter() - t0
         for k, i in enumerate(fb):
             final[i] = f_fb[k]
     return final, fallback, rms_spread, t_pred, t_fb
 
 
-def screen(pred, spread_db=1.0):
-    """Per environment: ensemble mean and whether it falls back to the tracer
-    (RMS over receivers of the members' spread above `spread_db`)."""
-    mean = pred.mean(0)
-    spread = pred.max(0) - pred.min(0)
-    rms_spread = np.sqrt(np.mean(spread ** 2, axis=(1, 2)))
-    return mean, rms_spread > spread_db, rms_spread
+def field_errors(answer, truth, thetas, rz, rr_km):
+    """Errors (dB) of fields [n, nrz, nrr] against the tracer's, at
+    water-column receivers whose tracer TL is below TL_CAP: (RMS, p95 of |e|,
+    per-environment RMS)."""
+    errs, per = [], []
+    for a, t, th in zip(answer, truth, thetas):
+        m = water_mask(th, rz, rr_km) & (t < TL_CAP)
+        e = (np.minimum(a, TL_CAP) - t)[m]
+        errs.append(e)
+        per.append(float(np.sqrt(np.mean(e ** 2))) if e.size else 0.0)
+    e = np.concatenate(errs) if errs else np.zeros(0)
+    if e.size == 0:
+        return 0.0, 0.0, np.asarray(per)
+    return float(np.sqrt(np.mean(e ** 2))), float(np.percentile(np.abs(e), 95)), np.asarray(per)

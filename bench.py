@@ -17,9 +17,11 @@ Workloads (one "step" = one pass of the hot path over one batch of input):
 * munk: one Munk-profile TL field, 2001 hat beams, 201 x 1001 receivers.
 * c3: one full-3D Gaussian-beam TL field -- hexahedral SSP, 201 x 201
   bathymetry grid, 181 x 1001 beams, 36 bearings x 101 depths x 1001 ranges.
-* surrogate: the surrogate-screened sweep -- an MLP ensemble trained on
-  tracer fields of a 14-parameter family answers held-out environments where
-  its members agree, the batched tracer where they do not.
+* surrogate: the surrogate-screened sweep -- an MLP ensemble trained on the
+  modal coefficients of tracer fields of a 14-parameter family answers
+  held-out environments where its members agree, the batched tracer where
+  they do not (--sur-family global: every parameter over its full range;
+  local: an uncertainty sweep around one nominal environment).
 
 Inputs are resident in HBM before the timed region. Prints one JSON line:
 fields/s (value), rays/s, the roofline of the dominant field kernel with the
@@ -448,37 +450,34 @@ def run_sweep(a):
 
 
 def run_surrogate(a):
-    """Surrogate-screened sweep: train the ensemble on tracer fields of the
-    environment family, then answer held-out environments with the network
-    where the ensemble agrees and with the batched tracer where it does not."""
+    """Surrogate-screened sweep: fit the modal basis to tracer fields of the
+    environment family and train the ensemble on their coefficients, then
+    answer held-out environments with the ensemble's field where its members
+    agree and with the batched tracer where they do not."""
     import numpy as np
     import bellhopcuda_amd as bh
     from bellhopcuda_amd import surrogate as sg
     ctx = bh.Context(0)
-    th_tr = sg.draw_params(a.sur_train, 20260005)
-    th_ev = sg.draw_params(a.sur_eval, 20260006)
+    fam = a.sur_family
+    th_tr = sg.draw_params(a.sur_train, 20260005, fam)
+    th_ev = sg.draw_params(a.sur_eval, 20260006, fam)
     t0 = time.perf_counter()
     f_tr, e_tr, t_tr = sg.trace_fields(th_tr, ctx)
     t_train_fields = time.perf_counter() - t0
-    ref = sg.reference_field(f_tr)
-    X, Y = sg.dataset(th_tr, f_tr, e_tr, a.sur_points, 1, ref)
-    mlp = sg.Mlp(members=4, max_batch=a.sur_batch, seed=11)
-    mlp.set_fourier(sg.fourier_matrix(12, a.sur_fourier))
-    mlp.set_dataset(X, Y)
+    log("traced %d training fields in %.1f s" % (len(f_tr), t_train_fields))
     t0 = time.perf_counter()
-    done, train_ms = 0, 0.0
-    while done < a.sur_steps:
-        k = min(500, a.sur_steps - done)
-        # cosine decay from sur_lr, one value per chunk of steps
-        lr = a.sur_lr * (0.02 + 0.98 * 0.5 * (1.0 + np.cos(np.pi * (done + 0.5 * k) / a.sur_steps)))
-        loss = mlp.train(k, a.sur_batch, lr, seed=done)
-        train_ms += mlp.last_ms()
-        done += k
+    sur = sg.Surrogate(th_tr, f_tr, fam, k=a.sur_modes, members=4, seed=11, max_batch=a.sur_batch)
+    t_basis = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = sur.train(a.sur_steps, a.sur_batch, a.sur_lr)
     t_train = time.perf_counter() - t0
-    step_flops = mlp.step_flops(a.sur_batch)
+    train_ms = sur.train_ms
+    step_flops = sur.mlp.step_flops(a.sur_batch)
+    log("basis %.1f s, training %.1f s (%d steps)" % (t_basis, t_train, a.sur_steps))
     # ground truth of the held-out environments (batched tracer)
     f_ev, e_ev, t_ev = sg.trace_fields(th_ev, ctx)
     truth = np.stack(f_ev)
+    rz, rr = e_ev[0].Rz, e_ev[0].Rr_km
     # plain per-environment tracer on a sample
     t_plain, ns = 0.0, 0
     for e in e_ev[::max(1, len(e_ev) // 32)][:32]:
@@ -492,44 +491,57 @@ def run_surrogate(a):
         ns += 1
         r.release()
     plain_fields_s = ns / t_plain
-    # screened answer: the ensemble's spread on a receiver subsample decides,
-    # per environment, between the network's full field and the tracer
-    final, fallback, rms_spread, t_pred, t_fb = sg.screened_answer(mlp, th_ev, e_ev, ref, ctx, 1.0)
-    err = (final - truth).reshape(-1)
-    # diagnostics outside the timed region: the network's full-field answer everywhere
-    mean = sg.predict_fields(mlp, th_ev, e_ev, ref).mean(0)
-    err_net = (mean - truth).reshape(-1)
+    # the screened answer (timed): prediction and screen, then the tracer for
+    # the environments the screen sends back
+    final, fallback, rms_spread, t_pred, t_fb = sg.screened_answer(sur, th_ev, e_ev, ctx, a.sur_spread)
+    rms, p95, per = sg.field_errors(final, truth, th_ev, rz, rr)
+    acc = ~fallback
+    rms_acc, p95_acc, _ = (sg.field_errors(final[acc], truth[acc], th_ev[acc], rz, rr) if acc.any()
+                           else (0.0, 0.0, None))
+    # diagnostics outside the timed region: the network's field everywhere,
+    # in sample, and the mean training field as the method without a network
+    net = sur.predict(th_ev).mean(0)
+    net_rms, net_p95, net_per = sg.field_errors(net, truth, th_ev, rz, rr)
+    base = np.broadcast_to(np.clip(sur.basis.ref.reshape(truth.shape[1:]), sg.TL_MIN, sg.TL_CAP), truth.shape)
+    base_rms, base_p95, base_per = sg.field_errors(base, truth, th_ev, rz, rr)
     k_in = list(range(0, len(th_tr), max(1, len(th_tr) // 100)))[:100]
-    pred_in = sg.predict_fields(mlp, th_tr[k_in], [e_tr[i] for i in k_in], ref).mean(0)
-    err_in = (pred_in - np.stack([f_tr[i] for i in k_in])).reshape(-1)
-    err_base = (ref[None] - truth).reshape(-1)
+    pin = sur.predict(th_tr[k_in]).mean(0)
+    in_rms, _, _ = sg.field_errors(pin, np.stack([f_tr[i] for i in k_in]), th_tr[k_in], rz, rr)
+    spread_err = float(np.corrcoef(rms_spread, net_per)[0, 1]) if len(net_per) > 2 else None
     screened_fields_s = len(th_ev) / (t_pred + t_fb)
+    achieved = step_flops * a.sur_steps / (train_ms * 1e-3) / 1e12
     peak = 2500.0
+    sur.close()
     return {
         "metric": "TL fields/s on 1 MI355X, surrogate-screened sweep vs plain GPU",
         "value": screened_fields_s, "unit": "TL fields/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * (t_pred + t_fb), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (network), f64 (tracer)",
-        "data": "synthetic (seeded environment family, seeds 20260005 / 20260006)",
-        "config": {"workload": "surrogate-screened sweep: %d training fields, %d held-out environments, "
-                               "51 x 201 receivers, 1001 'B' beams, incoherent TL; 4 x MLP(64 Fourier features, 6 x 256 GELU)"
-                               % (len(th_tr), len(th_ev)), "parallelism": "single GPU"},
+        "data": "synthetic (seeded environment family '%s', seeds 20260005 / 20260006)" % fam,
+        "config": {"workload": "surrogate-screened sweep, family '%s': %d training fields, %d held-out "
+                               "environments, 51 x 201 receivers, 1001 'B' beams, incoherent TL; %d modes, "
+                               "4 x MLP(16 inputs + 24 Fourier pairs, 6 x 256 GELU, %d outputs)"
+                               % (fam, len(th_tr), len(th_ev), a.sur_modes, a.sur_modes),
+                   "parallelism": "single GPU"},
         "plain_per_env": {"value": plain_fields_s, "unit": "TL fields/s"},
         "screened_over_plain": screened_fields_s / plain_fields_s,
         "fallback_fraction": float(np.mean(fallback)),
-        "error_db": {"rms": float(np.sqrt(np.mean(err ** 2))), "p95": float(np.percentile(np.abs(err), 95)),
-                     "network_only_rms": float(np.sqrt(np.mean(err_net ** 2))),
-                     "network_in_sample_rms": float(np.sqrt(np.mean(err_in ** 2))),
+        "spread_threshold_db": a.sur_spread,
+        "error_db": {"note": "water-column receivers with tracer TL below %g dB" % sg.TL_CAP,
+                     "rms": rms, "p95": p95, "accepted_rms": rms_acc, "accepted_p95": p95_acc,
+                     "network_only_rms": net_rms, "network_only_p95": net_p95,
+                     "network_env_frac_rms_below_1p5": float(np.mean(net_per <= 1.5)),
+                     "network_in_sample_rms": in_rms,
+                     "mean_field_baseline_rms": base_rms, "mean_field_baseline_p95": base_p95,
                      "median_rms_spread": float(np.median(rms_spread)),
-                     "mean_field_baseline_rms": float(np.sqrt(np.mean(err_base ** 2)))},
-        "training": {"fields": len(th_tr), "points": int(X.shape[0]), "steps": a.sur_steps,
-                     "batch": a.sur_batch, "final_loss": [float(v) for v in loss], "seconds": t_train,
+                     "corr_spread_vs_network_error": spread_err},
+        "training": {"fields": len(th_tr), "modes": a.sur_modes, "steps": a.sur_steps, "batch": a.sur_batch,
+                     "final_loss": [float(v) for v in loss], "seconds": t_train, "basis_seconds": t_basis,
                      "tracer_seconds_for_training_fields": t_train_fields},
         "roofline": {"bound": "mfma", "kernel": "BF16 MLP training step (forward + backward products)",
-                     "achieved": step_flops * a.sur_steps / (train_ms * 1e-3) / 1e12, "peak": peak,
-                     "unit": "TFLOP/s", "frac": step_flops * a.sur_steps / (train_ms * 1e-3) / 1e12 / peak,
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": None, "step_ms": train_ms / a.sur_steps, "flop_per_step": step_flops},
-        "timing_s": {"predict": t_pred, "fallback_trace": t_fb, "eval_truth_trace": t_ev},
+        "timing_s": {"predict_and_screen": t_pred, "fallback_trace": t_fb, "eval_truth_trace": t_ev},
     }
 
 
@@ -541,11 +553,12 @@ def main():
     ap.add_argument("--workload", default="sweep", choices=["sweep", "seamount", "munk", "surrogate", "c3"])
     ap.add_argument("--sur-train", type=int, default=20000)
     ap.add_argument("--sur-eval", type=int, default=1000)
-    ap.add_argument("--sur-points", type=int, default=1000)
-    ap.add_argument("--sur-steps", type=int, default=20000)
-    ap.add_argument("--sur-batch", type=int, default=16384)
+    ap.add_argument("--sur-steps", type=int, default=5000)
+    ap.add_argument("--sur-batch", type=int, default=2048)
     ap.add_argument("--sur-lr", type=float, default=1e-3)
-    ap.add_argument("--sur-fourier", default="random", choices=["random", "structured"])
+    ap.add_argument("--sur-modes", type=int, default=128)
+    ap.add_argument("--sur-spread", type=float, default=1.0)
+    ap.add_argument("--sur-family", default="global", choices=["global", "local"])
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--sweep-n", type=int, default=4096)
     ap.add_argument("--sweep-scale", type=float, default=1.0)

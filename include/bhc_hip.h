@@ -661,20 +661,28 @@ void bhc_ld_char(bhc_ldfile *f, char c);
 int bhc_ld_close(bhc_ldfile *f);
 
 /* ---- TL surrogate (no counterpart in the reference) ------------------------------
- * An ensemble of coordinate MLPs (16 normalised inputs -> 64 Fourier features
- * -> 6 x 256 GELU -> TL) trained on tracer fields, forward and backward on the
- * matrix cores in BF16 with FP32 accumulation, Adam on FP32 master weights. */
+ * An ensemble of MLPs (16 normalised inputs -> 64 features -> 6 x 256 GELU ->
+ * n_out outputs) trained on tracer fields, forward and backward on the matrix
+ * cores in BF16 with FP32 accumulation, Adam on FP32 master weights. Feature
+ * front 0: sin / cos of 32 projections of the inputs; front 1: the 16 inputs
+ * and sin / cos of the first 24 projections. n_out = 1 (a coordinate network:
+ * TL at one receiver) or a multiple of 32 up to 256 (the coefficients of a
+ * field in a fixed basis). */
 typedef struct bhc_mlp bhc_mlp;
-int bhc_mlp_create(int device, int members, int max_batch, unsigned long long seed, float sigma, bhc_mlp **out);
+int bhc_mlp_create_ex(int device, int members, int max_batch, int n_out, int front, unsigned long long seed,
+                      float sigma, bhc_mlp **out);
+int bhc_mlp_create(int device, int members, int max_batch, unsigned long long seed, float sigma,
+                   bhc_mlp **out);                              /* n_out 1, front 0 */
 void bhc_mlp_destroy(bhc_mlp *m);
 const char *bhc_mlp_last_error(void);
-int bhc_mlp_num_params(void);                                   /* per member */
+int bhc_mlp_num_params(void);                                   /* per member, n_out 1 */
+long long bhc_mlp_param_count(bhc_mlp *m);                      /* per member */
 int bhc_mlp_get_params(bhc_mlp *m, float *params, float *fourier); /* [members][P], [32][16] */
 int bhc_mlp_set_params(bhc_mlp *m, const float *params);
 int bhc_mlp_set_fourier(bhc_mlp *m, const float *B);            /* [32][16] */
-int bhc_mlp_predict(bhc_mlp *m, const float *x, long long n, float *out); /* x [n][16] -> out [members][n] */
-int bhc_mlp_gradients(bhc_mlp *m, const float *x, const float *y, int n, float *grads, float *loss);
-int bhc_mlp_set_dataset(bhc_mlp *m, const float *X, const float *Y, long long n);
+int bhc_mlp_predict(bhc_mlp *m, const float *x, long long n, float *out); /* x [n][16] -> out [members][n][n_out] */
+int bhc_mlp_gradients(bhc_mlp *m, const float *x, const float *y, int n, float *grads, float *loss); /* y [n][n_out] */
+int bhc_mlp_set_dataset(bhc_mlp *m, const float *X, const float *Y, long long n);      /* Y [n][n_out] */
 int bhc_mlp_train(bhc_mlp *m, int steps, int batch, float lr, unsigned long long seed, float *loss_out);
 double bhc_mlp_last_ms(bhc_mlp *m);
 double bhc_mlp_step_flops(bhc_mlp *m, int batch);
