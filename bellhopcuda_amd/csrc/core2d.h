@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 
 namespace bhc2d {
@@ -151,6 +152,9 @@ template<typename real> __host__ __device__ inline real dot(v2<real> a, v2<real>
 // ---------------------------------------------------------------------------
 enum : int { RT_TL = 0, RT_ARR = 1, RT_EIG = 2, RT_RAY = 3 };
 
+// RunType[0] of a TL run: coherent, semi-coherent or incoherent
+__host__ __device__ inline bool tl_run(char run0) { return run0 == 'C' || run0 == 'S' || run0 == 'I'; }
+
 // Boundary node record in the blob (BdryPtFull<false> subset). Straight-segment
 // boundaries with one set of geoacoustics use the first NODE_SHORT reals of a
 // node; curvilinear ('C') or per-node geoacoustics ('L') use all NODE_FULL.
@@ -251,8 +255,15 @@ template<typename real> struct Infl {
     real alpha, SrcDeclAngle, omega, BeamWindow, Ratio1, rcp_q0;
     real phase, qOld;
     v2<real> rayn1, x;
+    v2<real> rbr;               // Cartesian: receiver ranges Rr[ir], Rr[ir + 1] (the walk's bracket)
     int32_t lastValid, ir;
 };
+
+// The walk's bracket (Rr[ir], Rr[ir + 1]) after ir is set; Rr[ir] twice at the last range.
+template<typename real> __host__ __device__ inline v2<real> range_bracket(const real *Rr, int n, int ir)
+{
+    return V(Rr[ir], ir < n - 1 ? Rr[ir + 1] : Rr[ir]);
+}
 
 // ---------------------------------------------------------------------------
 // SSP evaluation (reference src/ssp.hpp)
@@ -863,6 +874,7 @@ __host__ __device__ inline bool RayInit(int isz, int ialpha, RayPt<real> &p0, Tr
         I.lastValid = 0;
         I.ir = BinarySearchGT(G + P.o_Rr, P.NRr, p0.x.x);
         if(p0.t.x < RL(0.0) && I.ir > 0) --I.ir;
+        I.rbr = range_bracket(G + P.o_Rr, P.NRr, I.ir);
     }
     return true;
 }
@@ -978,23 +990,40 @@ __host__ __device__ inline void depth_window(const ZT *Rz, int n, double rdz, re
 }
 
 template<typename real> struct SegDesc {
-    // geometry (Cartesian: x_ray, rayt, rayn1, rlen, zmin, zmax, lo, hi, ir range;
-    //           ray-centred: A = previous normal point / normal, B = this one)
-    v2<real> xa, ta, xb, tb;
+    // core -- every field a Cartesian TL scatter reads, and all that is stored
+    // for a Cartesian TL segment. Geometry: Cartesian, the step's start and
+    // unit tangent (its normal is (-t_z, t_r)); ray-centred, A = the previous
+    // normal point / normal.
+    v2<real> xa, ta;
     real rlen, zmin, zmax, lo, hi;
     // beam quantities of InfluenceGeoCore
-    real q0x, dqx, phasePt, phase, qOld, rcp_q0, RcvrDecl, SrcDecl;
+    real q0x, dqx, phasePt, phase, qOld, rcp_q0;
     real A0;                    // Ratio1 * sqrt(c1) * Amp1: numerator of the contribution amplitude
     real sigfloor;              // Gaussian beams: lower bound of the beam width (AdjustSigma)
-    real wmax;                  // upper bound of BeamWindow * sigma over the segment
     cpx<real> tau0, dtau;
     int32_t ir0, ir1, dir;      // Cartesian: visited receiver ranges ir0..ir1 (step dir)
     int32_t iz0, nz;            // depth window: receiver depths iz0 .. iz0+nz-1
     int32_t nitems;             // work items (0 = nothing to scatter)
-    int32_t lastValid;          // ray-centred
-    int32_t ntop, nbot, isz, ialpha, is;
+    int32_t isz;
     int32_t prob;               // problem index (batched runs)
+    // rest -- ray-centred geometry (B = this step's normal point / normal),
+    // arrivals and eigen hits
+    v2<real> xb, tb;
+    real RcvrDecl, SrcDecl;
+    real wmax;                  // ray-centred: upper bound of BeamWindow * sigma over the segment
+    int32_t lastValid;          // ray-centred
+    int32_t ntop, nbot, ialpha, is;
 };
+
+// Bytes of a descriptor's core (its leading part up to the rest).
+template<typename real> constexpr size_t seg_core_bytes() { return offsetof(SegDesc<real>, xb); }
+
+// Whether a segment needs the descriptor's rest: ray-centred geometry, or a
+// run that records arrivals or eigen hits.
+template<typename real> __host__ __device__ inline bool seg_full(const Prob2D<real> &P)
+{
+    return P.isRayCen || !tl_run(P.run0);
+}
 
 template<typename real>
 __host__ __device__ inline void desc_common(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
@@ -1051,22 +1080,31 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     }
     d.lo = rmin(rA, rB);
     d.hi = rmax(rA, rB);
-    // walk: the set of visited receiver ranges is contiguous
+    // walk: the set of visited receiver ranges is contiguous. The bracket
+    // (Rr[ir], Rr[ir + 1]) travels with the ray, so a step that crosses no
+    // receiver range reads no range from memory; the comparisons are the walk's.
     int ir = I.ir, first = I.ir;
+    real rlo = I.rbr.x, rhi = I.rbr.y;
     for(;;) {
         int irTT;
-        if(rB > Rr[ir]) {
+        if(rB > rlo) {
             if(ir >= P.NRr - 1) break;
             irTT = ir + 1;
-            if(Rr[irTT] >= rB) break;
+            if(rhi >= rB) break;
+            rlo = rhi;
+            if(irTT < P.NRr - 1) rhi = Rr[irTT + 1];
         } else {
             if(ir <= 0) break;
             irTT = ir - 1;
-            if(Rr[irTT] <= rB) break;
+            const real r = Rr[irTT];
+            if(r <= rB) break;
+            rhi = rlo;
+            rlo = r;
         }
         ir = irTT;
     }
     I.ir = ir;
+    I.rbr = V(rlo, rhi);
     I.x = p1.x;
     d.ir0 = first;
     d.ir1 = ir;
@@ -1083,9 +1121,9 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     d.nitems = nr * d.nz;
     d.xa = p0.x;
     d.ta = rayt;
-    d.tb = V(-rayt.y, rayt.x);
     d.rlen = rlen;
-    d.RcvrDecl = (RL(180.0) / RC<real>::PI) * std::atan2(rayt.y, rayt.x);
+    // the receiver angle enters arrivals only (TL runs skip the atan2)
+    d.RcvrDecl = tl_run(P.run0) ? RL(0.0) : (RL(180.0) / RC<real>::PI) * std::atan2(rayt.y, rayt.x);
     desc_common(d, p0, p1, I, P, is, isz, ialpha);
 }
 
@@ -1112,7 +1150,7 @@ __host__ __device__ inline void prelude_raycen(SegDesc<real> &d, const RayPt<rea
     d.ta = I.rayn1;
     d.xb = p1.x;
     d.tb = rayn1;
-    d.RcvrDecl = (RL(180.0) / RC<real>::PI) * std::atan2(p1.t.y, p1.t.x);
+    d.RcvrDecl = tl_run(P.run0) ? RL(0.0) : (RL(180.0) / RC<real>::PI) * std::atan2(p1.t.y, p1.t.x);
     desc_common(d, p0, p1, I, P, is, isz, ialpha);
     d.nitems = P.NRz_per_range;
     {
@@ -1246,7 +1284,7 @@ __host__ __device__ inline void item_cart_rr(const SegDesc<real> &d, int ir, rea
     if(zr < d.zmin || zr > d.zmax) return;
     v2<real> xrr = V(rr, zr) - d.xa;
     real s = dot(xrr, d.ta) / d.rlen;
-    real n1 = std::fabs(dot(xrr, d.tb));
+    real n1 = std::fabs(dot(xrr, V(-d.ta.y, d.ta.x))); // the unit normal (rayn1)
     contrib<real, RT, Sink, FAST>(d, s, n1, ir, iz, P, sink);
 }
 
@@ -1261,7 +1299,7 @@ __host__ __device__ inline void item_cart_at(const SegDesc<real> &d, int ir, int
     if(zr < d.zmin || zr > d.zmax) return;
     v2<real> xrr = V(rr, zr) - d.xa;
     real s = dot(xrr, d.ta) / d.rlen;
-    real n1 = std::fabs(dot(xrr, d.tb));
+    real n1 = std::fabs(dot(xrr, V(-d.ta.y, d.ta.x))); // the unit normal (rayn1)
     contrib<real, RT, Sink, FAST>(d, s, n1, ir, iz, P, sink);
 }
 

@@ -310,6 +310,7 @@ __host__ __device__ inline bool RayInitNx(int ia, int ib, int isx, int isy, int 
         I.lastValid = 0;
         I.ir = BinarySearchGT(P.base + P.o_Rr, P.NRr, p0.x.x);
         if(p0.t.x < RL(0.0) && I.ir > 0) --I.ir;
+        I.rbr = range_bracket(P.base + P.o_Rr, P.NRr, I.ir);
     }
     return true;
 }

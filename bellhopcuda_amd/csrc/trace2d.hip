@@ -81,6 +81,20 @@ template<typename real> struct alignas(16) Seg {
     SegDesc<real> d;
 };
 
+// A stored descriptor's core; its rest (ray-centred geometry, arrival and
+// eigen fields) only where seg_full: Cartesian TL segments store the core
+// alone and their scatter reads the core alone.
+template<typename real> __device__ inline void load_seg_core(const Seg<real> *src, SegDesc<real> &d)
+{
+    __builtin_memcpy(&d, &src->d, seg_core_bytes<real>());
+}
+template<typename real> __device__ inline void load_seg_rest(const Seg<real> *src, SegDesc<real> &d)
+{
+    constexpr size_t c = seg_core_bytes<real>();
+    __builtin_memcpy(reinterpret_cast<char *>(&d) + c, reinterpret_cast<const char *>(&src->d) + c,
+                     sizeof(SegDesc<real>) - c);
+}
+
 // Finished-ray progress of a launch (completedRayCount / totalJobs,
 // src/mode/fieldimpl.cpp.in:43,62-63): wavefronts add their finished rays to
 // a device counter; each time the total crosses a multiple of PROG_QUANTUM it
@@ -229,11 +243,12 @@ template<typename real> __device__ inline void seal_block(const SegBlock &B, int
     }
 }
 
-// Append this lane's descriptor (if it has work items) to the segment buffer.
-// Called by all lanes of the wavefront. A slot beyond the buffer sets the
-// overflow flag: the host then discards the pass and splits the ray range.
+// Append this lane's descriptor (if it has work items) to the segment buffer:
+// all of it, or its core when `full` is false (seg_full). Called by all lanes
+// of the wavefront. A slot beyond the buffer sets the overflow flag: the host
+// then discards the pass and splits the ray range.
 template<typename real>
-__device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut &O, SegBlock &B)
+__device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut &O, SegBlock &B, bool full)
 {
     bool has = d.nitems > 0;
     unsigned long long m = __ballot(has);
@@ -250,7 +265,9 @@ __device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut
     if(has) {
         uint32_t slot = B.base + B.used + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if(slot < O.seg_cap) {
-            reinterpret_cast<Seg<real> *>(O.segs)[slot].d = d;
+            SegDesc<real> &dst = reinterpret_cast<Seg<real> *>(O.segs)[slot].d;
+            if(full) dst = d;
+            else __builtin_memcpy(&dst, &d, seg_core_bytes<real>());
         } else {
             atomicOr(O.overflow, 1u);
         }
@@ -403,14 +420,14 @@ __global__ void __launch_bounds__(256) k_trace2d(Prob2D<real> P, DevOut O)
             prelude(d, p0, p1, I, is, P, G, B + P.o_rzs, isz, ia);
             ++is;
         }
-        append_seg(d, lane, O, blk);
+        append_seg(d, lane, O, blk, seg_full(P));
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
                 prelude(d, p1, p2, I, is, P, G, B + P.o_rzs, isz, ia);
                 ++is;
             }
-            append_seg(d, lane, O, blk);
+            append_seg(d, lane, O, blk, seg_full(P));
         }
         bool fin = false;
         if(active) {
@@ -452,8 +469,11 @@ __global__ void __launch_bounds__(256) k_scatter2d(Prob2D<real> P, DevOut O)
     DevSink<real> sink{&O, &P, 0};
     const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
-        const SegDesc<real> d = segs[s].d;
-        if(d.nitems > 0) scatter_seg<real, RT, FAST>(d, P, sink, lane);
+        SegDesc<real> d;
+        load_seg_core(segs + s, d);
+        if(d.nitems <= 0) continue;
+        if(seg_full(P)) load_seg_rest(segs + s, d);
+        scatter_seg<real, RT, FAST>(d, P, sink, lane);
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
     if(lane == 0 && sink.nc) atomicAdd(O.counters + 1, sink.nc);
@@ -481,7 +501,10 @@ __global__ void __launch_bounds__(256) k_scatter2d_lanes(Prob2D<real> P, DevOut 
     const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
-        const SegDesc<real> d = segs[s].d;
+        SegDesc<real> d;
+        load_seg_core(segs + s, d);
+        if(d.nitems <= 0) continue;
+        if(seg_full(P)) load_seg_rest(segs + s, d);
         seg_items<real, RT, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
@@ -652,7 +675,9 @@ __global__ void __launch_bounds__(TILE_THREADS, 4) k_scatter_tile(Prob2D<real> P
         TileSink<real> sink{tile, col0, ncol, nz, P.NRr, 0};
         for(int k = k0 + wave; k < k1; k += nw) {
             uint32_t si = __builtin_amdgcn_readfirstlane(C.idx[k]);
-            const SegDesc<real> d = segs[si].d;
+            SegDesc<real> d;
+            load_seg_core(segs + si, d);
+            if(seg_full(P)) load_seg_rest(segs + si, d);
             if(d.isz != 0) {
                 DevSink<real> gs{&O, &P, 0};
                 scatter_seg<real, RT_TL, FAST>(d, P, gs, lane);
@@ -833,14 +858,14 @@ __global__ void __launch_bounds__(64) k_tracenx2d(Prob2D<real> P, bhc3d::Prob3D<
             prelude(d, p0, p1, I, is, P, G, G + P.o_rzs, isf, ia);
             ++is;
         }
-        append_seg(d, lane, O, blk);
+        append_seg(d, lane, O, blk, seg_full(P));
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
                 prelude(d, p1, p2, I, is, P, G, G + P.o_rzs, isf, ia);
                 ++is;
             }
-            append_seg(d, lane, O, blk);
+            append_seg(d, lane, O, blk, seg_full(P));
         }
         bool fin = false;
         if(active) {
@@ -2310,7 +2335,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
             ++is;
         }
         if constexpr(MODE == TRACE_DESC) {
-            append_seg(d, lane, O, blk);
+            append_seg(d, lane, O, blk, seg_full(P));
         } else {
             DevSink<real> sink{&O, &P, 0};
             seg_items<real, RT_TL, DevSink<real>, MODE == TRACE_FUSED_FAST>(d, P, G, P.Rz, sink);
@@ -2324,7 +2349,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
                 ++is;
             }
             if constexpr(MODE == TRACE_DESC) {
-                append_seg(d, lane, O, blk);
+                append_seg(d, lane, O, blk, seg_full(P));
             } else {
                 DevSink<real> sink{&O, &P, 0};
                 seg_items<real, RT_TL, DevSink<real>, MODE == TRACE_FUSED_FAST>(d, P, G, P.Rz, sink);
@@ -2438,9 +2463,11 @@ __global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut
     const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     unsigned long long nc = 0;
     for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
-        const SegDesc<real> d = segs[s].d;
+        SegDesc<real> d;
+        load_seg_core(segs + s, d);
         if(d.nitems <= 0) continue;
         const Prob2D<real> P = Bd.probs[__builtin_amdgcn_readfirstlane(d.prob)];
+        if(seg_full(P)) load_seg_rest(segs + s, d);
         DevSink<real> sink{&O, &P, 0};
         scatter_seg<real, RT_TL, FAST>(d, P, sink, lane);
         nc += sink.nc;
@@ -2463,11 +2490,13 @@ __global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut
     unsigned long long nc = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
-        const SegDesc<real> d = segs[s].d;
+        SegDesc<real> d;
+        load_seg_core(segs + s, d);
         if(d.nitems <= 0) continue;
         // a copy: read through a reference, the problem's fields would be
         // reloaded after every field atomic (they might alias)
         const Prob2D<real> P = Bd.probs[d.prob];
+        if(seg_full(P)) load_seg_rest(segs + s, d);
         DevSink<real> sink{&O, &P, 0};
         seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
         nc += sink.nc;
