@@ -485,13 +485,37 @@ __host__ __device__ inline void StepToBdry3(v3<real> x0, v3<real> &x2, v3<real> 
     if(snapDim) *snapDim = snap;
 }
 
+// The torsion's trigonometry (RayNormal, Reflect). A CPU build of the core
+// that defines BHC_TORSION_ULP_TEST moves every single-precision result by
+// one ulp (up or down, chosen from the argument's bits) to bound, on the
+// host, what one-ulp differences between libm implementations do to FP32
+// fields; the product never defines it.
+template<typename real> __host__ __device__ inline real torsion_ulp(real v, real arg)
+{
+#ifdef BHC_TORSION_ULP_TEST
+    if constexpr(sizeof(real) == 4) {
+        uint32_t b;
+        __builtin_memcpy(&b, &arg, 4);
+        b ^= b >> 13;
+        b *= 0x5bd1e995u;
+        b ^= b >> 15;
+        return std::nextafter(v, (b & 1u) ? (real)1.0e30 : (real)-1.0e30);
+    }
+#endif
+    (void)arg;
+    return v;
+}
+template<typename real> __host__ __device__ inline real torsion_cos(real a) { return torsion_ulp(std::cos(a), a); }
+template<typename real> __host__ __device__ inline real torsion_sin(real a) { return torsion_ulp(std::sin(a), a); }
+template<typename real> __host__ __device__ inline real torsion_acos(real a) { return torsion_ulp(std::acos(a), a); }
+
 // RayNormal / RayNormal_unit, src/common_run.hpp:144-215
 template<typename real>
 __host__ __device__ inline void RayNormal3(v3<real> t, real phi, bool ignorephi0, real c, v3<real> &e1, v3<real> &e2)
 {
     real rl = std::sqrt(t.x * t.x + t.y * t.y);
     if(std::fabs(phi) >= RC<real>::EPS || ignorephi0) {
-        real cosphi = std::cos(phi), sinphi = std::sin(phi);
+        real cosphi = torsion_cos(phi), sinphi = torsion_sin(phi);
         e1.x = (c * t.x * t.z * cosphi + t.y * sinphi) / rl;
         e1.y = (c * t.y * t.z * cosphi - t.x * sinphi) / rl;
         e1.z = -c * rl * cosphi;
@@ -571,7 +595,7 @@ __host__ __device__ inline void CurvCorr3D(Ray3<real> &ray, m2<real> D, real Tg,
     }
     if(reflect) {
         real d = dot3(rayn1, e1);
-        ray.phi += FL(2.0) * std::acos(rmax(rmin(d, RL(1.0)), RL(-1.0)));
+        ray.phi += FL(2.0) * torsion_acos(rmax(rmin(d, RL(1.0)), RL(-1.0)));
     }
 }
 
