@@ -1050,7 +1050,11 @@ __host__ __device__ inline void desc_common(SegDesc<real> &d, const RayPt<real> 
 }
 
 // Cartesian prelude: influence.hpp:1201-1297 and the receiver walk :1388-1402.
-template<typename real>
+// WINDOW = false leaves the segment's depth window to the scatter (every
+// depth in the descriptor; the scatter's per-column band and the zmin / zmax
+// test of each receiver select the same contributions): a batched trace,
+// latency-bound at one wavefront per SIMD, hands that search to the scatter.
+template<typename real, bool WINDOW = true>
 __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1,
                                              Infl<real> &I, int is, const Prob2D<real> &P, const real *G,
                                              const real *RzT, int isz, int ialpha)
@@ -1112,7 +1116,7 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     int nr = (ir >= first ? ir - first : first - ir) + 1;
     d.iz0 = 0;
     d.nz = P.NRz_per_range;
-    if(P.rz_sorted && P.run4 != 'I' && d.zmin > -RC<real>::MAXV) {
+    if(WINDOW && P.rz_sorted && P.run4 != 'I' && d.zmin > -RC<real>::MAXV) {
         int i0, i1;
         depth_window(RzT, P.NRz_per_range, P.rz_rdz, d.zmin, d.zmax, i0, i1);
         d.iz0 = i0;
@@ -1436,13 +1440,13 @@ __host__ __device__ inline void item_raycen(const SegDesc<real> &d, int iz, cons
 
 // RzT: the receiver depths as reals (the staged copy when the problem's
 // tables sit in LDS).
-template<typename real>
+template<typename real, bool WINDOW = true>
 __host__ __device__ inline void prelude(SegDesc<real> &d, const RayPt<real> &p0, const RayPt<real> &p1, Infl<real> &I,
                                         int is, const Prob2D<real> &P, const real *G, const real *RzT, int isz,
                                         int ialpha)
 {
     if(P.isRayCen) prelude_raycen(d, p0, p1, I, is, P, RzT, isz, ialpha);
-    else prelude_cart(d, p0, p1, I, is, P, G, RzT, isz, ialpha);
+    else prelude_cart<real, WINDOW>(d, p0, p1, I, is, P, G, RzT, isz, ialpha);
 }
 
 template<typename real, int RT, typename Sink, bool FAST = false>

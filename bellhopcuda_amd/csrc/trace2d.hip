@@ -2330,7 +2330,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
         bool two = false;
         if(active) {
             two = RayUpdate<real, ST>(p0, p1, p2, S, P, B, G, e);
-            prelude(d, p0, p1, I, is, P, G, B + P.o_rzs, isz, ia);
+            prelude<real, MODE != TRACE_DESC>(d, p0, p1, I, is, P, G, B + P.o_rzs, isz, ia);
             d.prob = pid;
             ++is;
         }
@@ -2344,7 +2344,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
         if(__any(two)) {
             d.nitems = 0;
             if(active && two) {
-                prelude(d, p1, p2, I, is, P, G, B + P.o_rzs, isz, ia);
+                prelude<real, MODE != TRACE_DESC>(d, p1, p2, I, is, P, G, B + P.o_rzs, isz, ia);
                 d.prob = pid;
                 ++is;
             }
