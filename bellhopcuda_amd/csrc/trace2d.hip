@@ -2480,9 +2480,12 @@ __global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut
 // descriptors at once (one memory round trip for all of them) and every lane
 // walks its own segment's receivers. Batched sweeps of hat beams produce a
 // segment per step with only a few receivers each, where a wavefront per
-// segment would spend its time waiting for the descriptor.
+// segment would spend its time waiting for the descriptor. Four wavefronts
+// per SIMD (128 VGPRs, a few spills in FP64): the lanes wait on descriptor,
+// receiver and problem loads, and more wavefronts hide more of it (3% faster
+// than three).
 template<typename real, bool FAST>
-__global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut O)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_scatter_lanes(BatchDev<real> Bd, DevOut O)
 {
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
@@ -2492,14 +2495,31 @@ __global__ void __launch_bounds__(256) k_scatter_lanes(BatchDev<real> Bd, DevOut
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
         SegDesc<real> d;
         load_seg_core(segs + s, d);
-        if(d.nitems <= 0) continue;
-        // a copy: read through a reference, the problem's fields would be
-        // reloaded after every field atomic (they might alias)
-        const Prob2D<real> P = Bd.probs[d.prob];
-        if(seg_full(P)) load_seg_rest(segs + s, d);
-        DevSink<real> sink{&O, &P, 0};
-        seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
-        nc += sink.nc;
+        const bool act = d.nitems > 0;
+        const unsigned long long am = __ballot(act);
+        if(!am) continue;
+        // The segments of a wavefront mostly belong to one problem (slots are
+        // reserved SEG_BLOCK at a time by one tracing wavefront): then its
+        // descriptor is read once, with wavefront-uniform (scalar) loads,
+        // instead of by every lane.
+        const int pu = __builtin_amdgcn_readfirstlane(__shfl(d.prob, __ffsll(am) - 1));
+        if(!__any(act && d.prob != pu)) {
+            if(!act) continue;
+            // a copy: read through a reference, the problem's fields would be
+            // reloaded after every field atomic (they might alias)
+            const Prob2D<real> P = Bd.probs[pu];
+            if(seg_full(P)) load_seg_rest(segs + s, d);
+            DevSink<real> sink{&O, &P, 0};
+            seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
+            nc += sink.nc;
+        } else {
+            if(!act) continue;
+            const Prob2D<real> P = Bd.probs[d.prob];
+            if(seg_full(P)) load_seg_rest(segs + s, d);
+            DevSink<real> sink{&O, &P, 0};
+            seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
+            nc += sink.nc;
+        }
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
     if((threadIdx.x & (WAVE - 1)) == 0 && nc) atomicAdd(O.counters + 1, nc);
