@@ -839,6 +839,7 @@ template<typename real> struct Infl3 {
     real phase, qOld;
     v3<real> xs;
     v3<real> rn1, rn2;     // ray-centred beams: the previous point's ray normals (c-scaled)
+    v2<real> rbr;          // Cartesian: receiver ranges Rr[ir], Rr[ir + 1] (the walk's bracket)
     int32_t ir;
 };
 
@@ -918,6 +919,7 @@ __host__ __device__ inline bool RayInit3(int ia, int ib, int isx, int isy, int i
     I.qOld = mdet(p0.q);
     I.xs = p0.x;
     I.ir = BinarySearchGT(G + P.o_Rr, P.NRr, (real)RL(0.0));
+    I.rbr = range_bracket(G + P.o_Rr, P.NRr, I.ir);
     I.rn1 = I.rn2 = V3((real)RL(0.0), (real)RL(0.0), (real)RL(0.0));
     if(P.isRayCen) RayNormal3(p0.t, p0.phi, false, p0.c, I.rn1, I.rn2); // RayNormalRayCen
     return true;
@@ -1036,7 +1038,10 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     real rA = std::sqrt(dax * dax + day * day);
     real rB = std::sqrt(dbx * dbx + dby * dby);
     if(std::fabs(rA - rB) <= RL(1.0e3) * spacing(rA)) return; // IsDuplicatePoint
-    if(is == 0) I.ir = (rB > rA) ? 0 : P.NRr - 1;
+    if(is == 0) {
+        I.ir = (rB > rA) ? 0 : P.NRr - 1;
+        I.rbr = range_bracket(Rr, P.NRr, I.ir);
+    }
     v3<real> rayt = p1.x - p0.x;
     real rlen = len3(rayt);
     if(rlen <= RL(1.0e3) * spacing(p1.x.x)) return;
@@ -1055,22 +1060,31 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     b1 = p1.q.a11 * I.rcp_qhat0;
     real l2 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
     d.Ldiag = std::sqrt(l1 * l1 + l2 * l2);
-    // receiver walk: the visited ranges are contiguous
+    // receiver walk: the visited ranges are contiguous; the bracket
+    // (Rr[ir], Rr[ir + 1]) travels with the ray, so a step that crosses no
+    // receiver range reads none (the comparisons are the walk's)
     int ir = I.ir, first = I.ir;
+    real rlo = I.rbr.x, rhi = I.rbr.y;
     for(;;) {
         int irTT;
-        if(rB > Rr[ir]) {
+        if(rB > rlo) {
             if(ir >= P.NRr - 1) break;
             irTT = ir + 1;
-            if(Rr[irTT] >= rB) break;
+            if(rhi >= rB) break;
+            rlo = rhi;
+            if(irTT < P.NRr - 1) rhi = Rr[irTT + 1];
         } else {
             if(ir <= 0) break;
             irTT = ir - 1;
-            if(Rr[irTT] <= rB) break;
+            const real r = Rr[irTT];
+            if(r <= rB) break;
+            rhi = rlo;
+            rlo = r;
         }
         ir = irTT;
     }
     I.ir = ir;
+    I.rbr = V(rlo, rhi);
     d.ir0 = first;
     d.ir1 = ir;
     d.dir = (ir >= first) ? 1 : -1;

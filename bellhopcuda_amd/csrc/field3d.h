@@ -246,9 +246,12 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
 // each row's depth window is flattened over the lanes.
 // STAGED: the receiver ranges, bearing table and depths are copied to LDS (the
 // launch checks they fit) and read only from there -- compile-time, so the
-// reads are LDS reads rather than generic ones.
+// reads are LDS reads rather than generic ones. Three wavefronts per SIMD
+// (at most 168 VGPRs): the per-segment chain of loads and FP64 row bounds is
+// latency-bound, and the third wavefront hides more of it (14 % faster than
+// two on the shelf-canyon field; four spill and are slower).
 template<typename real, bool FAST, int RT = RT_TL, bool STAGED = true>
-__global__ void __launch_bounds__(256) k_scatter3d(Prob3D<real> P, DevOut3 O)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_scatter3d(Prob3D<real> P, DevOut3 O)
 {
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE));
