@@ -230,6 +230,15 @@ __device__ inline void flush_err(const Err &e, const DevOut &O)
 // empty (nitems = 0) and skipped by the scatter phase.
 constexpr uint32_t SEG_BLOCK = 256;
 
+// The same pointer, marked as global memory: after inlining, the loads and
+// stores through it become global_* instructions instead of flat_* ones,
+// which the compiler must also count against LDS traffic (every wait for a
+// flat load then drains outstanding LDS operations and vice versa).
+template<typename T> __device__ inline T *as_global(T *p)
+{
+    return (T *)((__attribute__((address_space(1))) T *)p);
+}
+
 struct SegBlock {
     uint32_t base, used, cap;
 };
@@ -265,9 +274,22 @@ __device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut
     if(has) {
         uint32_t slot = B.base + B.used + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if(slot < O.seg_cap) {
-            SegDesc<real> &dst = reinterpret_cast<Seg<real> *>(O.segs)[slot].d;
-            if(full) dst = d;
-            else __builtin_memcpy(&dst, &d, seg_core_bytes<real>());
+            // 16-byte stores through a global-memory pointer (global_store,
+            // counted apart from LDS reads; a generic store would make every
+            // later LDS wait wait for it too)
+            typedef __attribute__((address_space(1))) unsigned long long gu64;
+            constexpr int NW = (int)(sizeof(Seg<real>) / 8);
+            constexpr int NC = (int)((seg_core_bytes<real>() + 15) / 16) * 2;
+            union {
+                SegDesc<real> d;
+                unsigned long long w[NW];
+            } u;
+            u.d = d;
+            gu64 *dst = (gu64 *)(reinterpret_cast<Seg<real> *>(O.segs) + slot);
+            const int nw = full ? NW : NC;
+#pragma unroll
+            for(int k = 0; k < NW; ++k)
+                if(k < nw) dst[k] = u.w[k];
         } else {
             atomicOr(O.overflow, 1u);
         }
@@ -2228,22 +2250,45 @@ enum { TRACE_DESC = 0, TRACE_FUSED_FAST = 1, TRACE_FUSED_EXACT = 2 };
 // the wavefront's rays have ended (returns 0), or until the segment buffer
 // nears its end and the rays are saved (returns 1). `resume` restores a saved
 // wavefront. The SSP type is a template parameter, as in the single-problem
-// kernel, so that each type runs specialised code.
-template<typename real, char ST, int MODE>
+// kernel, so that each type runs specialised code. STAGE: where the problem's
+// staged tables (SSP, boundaries, receiver depths) are read from -- 1, a copy
+// in LDS that the compiler knows to be LDS (ds_read, which does not wait for
+// the descriptor stores in flight); 2, global memory known to be global; 0,
+// either, through a generic pointer (the fused mappings).
+template<typename real, char ST, int MODE, int STAGE>
 __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref, const DevOut &Oref, int pid,
-                                                     const RaySave<real> *resume, SegBlock &blk, unsigned char *lds,
+                                                     const RaySave<real> *resume, SegBlock &blk_io, unsigned char *lds,
                                                      unsigned long long &all_steps, unsigned long long &all_contrib)
 {
     // copies, for the same reason as the problem's below: through the
-    // references their fields would be reloaded after every descriptor store
+    // references their fields would be reloaded after every descriptor store;
+    // the segment block in registers (through its reference, every append
+    // would read it back from memory and wait for the stores in flight)
     const DevOut O = Oref;
+    SegBlock blk = blk_io;
     const BatchDev<real> Bd = Bdref;
     const int lane = threadIdx.x;
     // a copy: through a reference the problem's fields would be reloaded after
     // every descriptor store (they might alias)
-    const Prob2D<real> P = Bd.probs[pid];
-    const real *B = stage_problem(P, Bd.lds_cap, lds);
-    const real *G = P.base;
+    Prob2D<real> P = Bd.probs[pid];
+    const real *G = STAGE == 0 ? P.base : as_global(P.base);
+    const real *B;
+    if constexpr(STAGE == 1) {
+        extern __shared__ __align__(16) unsigned char lds_dyn[];
+        real *L = reinterpret_cast<real *>(lds_dyn);
+        __syncthreads();
+        for(int i = threadIdx.x; i < P.staged_reals; i += blockDim.x) L[i] = G[i];
+        __syncthreads();
+        B = L;
+    } else if constexpr(STAGE == 2) {
+        B = G;
+    } else {
+        B = stage_problem(P, Bd.lds_cap, lds);
+    }
+    if constexpr(STAGE != 0) { // (null stays null)
+        P.ray_stats = as_global(P.ray_stats);
+        P.ray_xend = as_global(P.ray_xend);
+    }
     const int nr = P.njobs;
     bool pexh = false, active = false, init_fail = false;
     int result = 0;
@@ -2380,6 +2425,7 @@ __device__ __attribute__((noinline)) int run_problem(const BatchDev<real> &Bdref
     if(lane == 0 && t) atomicAdd(Bd.psteps + pid, t);
     all_steps += steps;
     all_contrib += contrib;
+    blk_io = blk;
     return result;
 }
 
@@ -2388,14 +2434,24 @@ __device__ inline int run_problem_any(const BatchDev<real> &Bd, const DevOut &O,
                                       SegBlock &blk, unsigned char *lds, unsigned long long &all_steps,
                                       unsigned long long &all_contrib)
 {
-    switch(Bd.probs[pid].ssp_type) {
-    case 'C': return run_problem<real, 'C', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
-    case 'P': return run_problem<real, 'P', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
-    case 'S': return run_problem<real, 'S', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
-    case 'N': return run_problem<real, 'N', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
-    case 'Q': return run_problem<real, 'Q', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
-    default: return run_problem<real, 'A', MODE>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);
+    const Prob2D<real> &Pp = Bd.probs[pid];
+    const bool lds_fits = Pp.staged_reals > 0 && Pp.staged_reals <= Bd.lds_cap;
+#define BHC_RUN(ST)                                                                                                   \
+    if constexpr(MODE == TRACE_DESC) {                                                                                \
+        if(lds_fits) return run_problem<real, ST, MODE, 1>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);    \
+        return run_problem<real, ST, MODE, 2>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);                 \
+    } else {                                                                                                          \
+        return run_problem<real, ST, MODE, 0>(Bd, O, pid, resume, blk, lds, all_steps, all_contrib);                 \
     }
+    switch(Pp.ssp_type) {
+    case 'C': BHC_RUN('C')
+    case 'P': BHC_RUN('P')
+    case 'S': BHC_RUN('S')
+    case 'N': BHC_RUN('N')
+    case 'Q': BHC_RUN('Q')
+    default: BHC_RUN('A')
+    }
+#undef BHC_RUN
 }
 
 template<typename real, int MODE>
