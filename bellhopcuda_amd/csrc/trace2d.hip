@@ -81,19 +81,16 @@ template<typename real> struct alignas(16) Seg {
     SegDesc<real> d;
 };
 
-// A stored descriptor's core; its rest (ray-centred geometry, arrival and
-// eigen fields) only where seg_full: Cartesian TL segments store the core
-// alone and their scatter reads the core alone.
-template<typename real> __device__ inline void load_seg_core(const Seg<real> *src, SegDesc<real> &d)
-{
-    __builtin_memcpy(&d, &src->d, seg_core_bytes<real>());
-}
-template<typename real> __device__ inline void load_seg_rest(const Seg<real> *src, SegDesc<real> &d)
-{
-    constexpr size_t c = seg_core_bytes<real>();
-    __builtin_memcpy(reinterpret_cast<char *>(&d) + c, reinterpret_cast<const char *>(&src->d) + c,
-                     sizeof(SegDesc<real>) - c);
-}
+// Segment buffer layout: slot s's descriptor core (its first SEG_CS bytes,
+// every field a Cartesian TL scatter reads) at segs + s SEG_CS, packed so
+// that the scatter fetches no other bytes; its rest (ray-centred geometry,
+// arrival and eigen fields) at segs + seg_cap SEG_CS + s SEG_RS, stored only
+// where seg_full. A slot takes sizeof(Seg<real>) bytes in all.
+template<typename real> constexpr size_t SEG_CS = (seg_core_bytes<real>() + 15) / 16 * 16;
+template<typename real> constexpr size_t SEG_RS = sizeof(Seg<real>) - SEG_CS<real>;
+static_assert(SEG_RS<double> >= sizeof(SegDesc<double>) - SEG_CS<double>, "descriptor rest does not fit");
+static_assert(SEG_RS<float> >= sizeof(SegDesc<float>) - SEG_CS<float>, "descriptor rest does not fit");
+
 
 // Finished-ray progress of a launch (completedRayCount / totalJobs,
 // src/mode/fieldimpl.cpp.in:43,62-63): wavefronts add their finished rays to
@@ -138,12 +135,34 @@ struct DevOut {
     uint32_t *errw;             // err, warn, nerr, nwarn
     int32_t *work;              // ray work counter (relative to job_begin)
     int32_t job_begin, job_end; // ray range of this trace pass
-    void *segs;                 // segment buffer (Seg<real>[seg_cap])
+    void *segs;                 // segment buffer: seg_cap descriptor cores, then seg_cap rests (seg_core / seg_rest)
     uint32_t *seg_count;
     uint32_t seg_cap;
     uint32_t *overflow;
     Progress prog;
 };
+
+template<typename real> __device__ inline char *seg_core(const DevOut &O, size_t s)
+{
+    return reinterpret_cast<char *>(O.segs) + s * SEG_CS<real>;
+}
+template<typename real> __device__ inline char *seg_rest(const DevOut &O, size_t s)
+{
+    return reinterpret_cast<char *>(O.segs) + (size_t)O.seg_cap * SEG_CS<real> + s * SEG_RS<real>;
+}
+template<typename real> __device__ inline int seg_nitems(const DevOut &O, size_t s)
+{
+    return reinterpret_cast<const SegDesc<real> *>(seg_core<real>(O, s))->nitems;
+}
+template<typename real> __device__ inline void load_seg_core(const DevOut &O, size_t s, SegDesc<real> &d)
+{
+    __builtin_memcpy(&d, seg_core<real>(O, s), SEG_CS<real>);
+}
+template<typename real> __device__ inline void load_seg_rest(const DevOut &O, size_t s, SegDesc<real> &d)
+{
+    __builtin_memcpy(reinterpret_cast<char *>(&d) + SEG_CS<real>, seg_rest<real>(O, s),
+                     sizeof(SegDesc<real>) - SEG_CS<real>);
+}
 
 template<typename real> struct DevSink {
     const DevOut *O;
@@ -245,10 +264,9 @@ struct SegBlock {
 
 template<typename real> __device__ inline void seal_block(const SegBlock &B, int lane, const DevOut &O)
 {
-    Seg<real> *segs = reinterpret_cast<Seg<real> *>(O.segs);
     for(uint32_t k = B.used + (uint32_t)lane; k < B.cap; k += WAVE) {
         uint32_t slot = B.base + k;
-        if(slot < O.seg_cap) segs[slot].d.nitems = 0;
+        if(slot < O.seg_cap) reinterpret_cast<SegDesc<real> *>(seg_core<real>(O, slot))->nitems = 0;
     }
 }
 
@@ -278,18 +296,21 @@ __device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut
             // counted apart from LDS reads; a generic store would make every
             // later LDS wait wait for it too)
             typedef __attribute__((address_space(1))) unsigned long long gu64;
-            constexpr int NW = (int)(sizeof(Seg<real>) / 8);
-            constexpr int NC = (int)((seg_core_bytes<real>() + 15) / 16) * 2;
+            constexpr int NC = (int)(SEG_CS<real> / 8);
+            constexpr int NR = (int)((sizeof(SegDesc<real>) - SEG_CS<real> + 7) / 8);
             union {
                 SegDesc<real> d;
-                unsigned long long w[NW];
+                unsigned long long w[NC + NR];
             } u;
             u.d = d;
-            gu64 *dst = (gu64 *)(reinterpret_cast<Seg<real> *>(O.segs) + slot);
-            const int nw = full ? NW : NC;
+            gu64 *dc = (gu64 *)seg_core<real>(O, slot);
 #pragma unroll
-            for(int k = 0; k < NW; ++k)
-                if(k < nw) dst[k] = u.w[k];
+            for(int k = 0; k < NC; ++k) dc[k] = u.w[k];
+            if(full) {
+                gu64 *dr = (gu64 *)seg_rest<real>(O, slot);
+#pragma unroll
+                for(int k = 0; k < NR; ++k) dr[k] = u.w[NC + k];
+            }
         } else {
             atomicOr(O.overflow, 1u);
         }
@@ -489,12 +510,11 @@ __global__ void __launch_bounds__(256) k_scatter2d(Prob2D<real> P, DevOut O)
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
     DevSink<real> sink{&O, &P, 0};
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
         SegDesc<real> d;
-        load_seg_core(segs + s, d);
+        load_seg_core(O, s, d);
         if(d.nitems <= 0) continue;
-        if(seg_full(P)) load_seg_rest(segs + s, d);
+        if(seg_full(P)) load_seg_rest(O, s, d);
         scatter_seg<real, RT, FAST>(d, P, sink, lane);
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
@@ -520,13 +540,12 @@ __global__ void __launch_bounds__(256) k_scatter2d_lanes(Prob2D<real> P, DevOut 
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
     DevSink<real> sink{&O, &P, 0};
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
         SegDesc<real> d;
-        load_seg_core(segs + s, d);
+        load_seg_core(O, s, d);
         if(d.nitems <= 0) continue;
-        if(seg_full(P)) load_seg_rest(segs + s, d);
+        if(seg_full(P)) load_seg_rest(O, s, d);
         seg_items<real, RT, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
     }
     for(int off = WAVE / 2; off > 0; off >>= 1) sink.nc += __shfl_xor(sink.nc, off);
@@ -558,6 +577,15 @@ template<typename real> __device__ inline int seg_bin(const SegDesc<real> &d, co
     return imin(col / C.bin_w, C.nbins - 1);
 }
 
+// The bin of stored segment s (its rest read for ray-centred segments only).
+template<typename real> __device__ inline int seg_bin_at(const DevOut &O, size_t s, const Prob2D<real> &P, const BinCfg &C)
+{
+    SegDesc<real> d;
+    load_seg_core(O, s, d);
+    if(P.isRayCen) load_seg_rest(O, s, d);
+    return seg_bin(d, P, C);
+}
+
 // Bin histogram: each workgroup owns a contiguous slice of the segments and
 // counts it in LDS first (one global atomic per bin per workgroup).
 template<typename real> __global__ void __launch_bounds__(256) k_bin_count(Prob2D<real> P, DevOut O, BinCfg C)
@@ -570,9 +598,8 @@ template<typename real> __global__ void __launch_bounds__(256) k_bin_count(Prob2
     uint32_t s0 = blockIdx.x * per, s1 = min(n, s0 + per);
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x) h[b] = 0;
     __syncthreads();
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x)
-        if(segs[s].d.nitems > 0) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+        if(seg_nitems<real>(O, s) > 0) atomicAdd(h + seg_bin_at(O, s, P, C), 1);
     __syncthreads();
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x)
         if(h[b]) atomicAdd(C.counts + b, h[b]);
@@ -629,16 +656,15 @@ template<typename real> __global__ void __launch_bounds__(256) k_bin_fill(Prob2D
     uint32_t s0 = blockIdx.x * per, s1 = min(n, s0 + per);
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x) h[b] = 0;
     __syncthreads();
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x)
-        if(segs[s].d.nitems > 0) atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+        if(seg_nitems<real>(O, s) > 0) atomicAdd(h + seg_bin_at(O, s, P, C), 1);
     __syncthreads();
     for(int b = threadIdx.x; b < C.nbins; b += blockDim.x)
         if(h[b]) h[b] = atomicAdd(C.cursor + b, h[b]);
     __syncthreads();
     for(uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-        if(segs[s].d.nitems <= 0) continue;
-        int pos = atomicAdd(h + seg_bin(segs[s].d, P, C), 1);
+        if(seg_nitems<real>(O, s) <= 0) continue;
+        int pos = atomicAdd(h + seg_bin_at(O, s, P, C), 1);
         C.idx[pos] = s;
     }
 }
@@ -682,7 +708,6 @@ __global__ void __launch_bounds__(TILE_THREADS, 4) k_scatter_tile(Prob2D<real> P
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int nw = blockDim.x / WAVE;
     const int nz = P.NRz_per_range;
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     unsigned long long total = 0;
     for(int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         int b = C.chunk_start[ch];
@@ -698,8 +723,8 @@ __global__ void __launch_bounds__(TILE_THREADS, 4) k_scatter_tile(Prob2D<real> P
         for(int k = k0 + wave; k < k1; k += nw) {
             uint32_t si = __builtin_amdgcn_readfirstlane(C.idx[k]);
             SegDesc<real> d;
-            load_seg_core(segs + si, d);
-            if(seg_full(P)) load_seg_rest(segs + si, d);
+            load_seg_core(O, si, d);
+            if(seg_full(P)) load_seg_rest(O, si, d);
             if(d.isz != 0) {
                 DevSink<real> gs{&O, &P, 0};
                 scatter_seg<real, RT_TL, FAST>(d, P, gs, lane);
@@ -2516,14 +2541,13 @@ __global__ void __launch_bounds__(256) k_scatter_batch(BatchDev<real> Bd, DevOut
     const int nwaves = gridDim.x * (blockDim.x / WAVE);
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     unsigned long long nc = 0;
     for(uint32_t s = (uint32_t)wave; s < n; s += (uint32_t)nwaves) {
         SegDesc<real> d;
-        load_seg_core(segs + s, d);
+        load_seg_core(O, s, d);
         if(d.nitems <= 0) continue;
         const Prob2D<real> P = Bd.probs[__builtin_amdgcn_readfirstlane(d.prob)];
-        if(seg_full(P)) load_seg_rest(segs + s, d);
+        if(seg_full(P)) load_seg_rest(O, s, d);
         DevSink<real> sink{&O, &P, 0};
         scatter_seg<real, RT_TL, FAST>(d, P, sink, lane);
         nc += sink.nc;
@@ -2545,12 +2569,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
 {
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
-    const Seg<real> *segs = reinterpret_cast<const Seg<real> *>(O.segs);
     unsigned long long nc = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
         SegDesc<real> d;
-        load_seg_core(segs + s, d);
+        load_seg_core(O, s, d);
         const bool act = d.nitems > 0;
         const unsigned long long am = __ballot(act);
         if(!am) continue;
@@ -2564,14 +2587,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
             // a copy: read through a reference, the problem's fields would be
             // reloaded after every field atomic (they might alias)
             const Prob2D<real> P = Bd.probs[pu];
-            if(seg_full(P)) load_seg_rest(segs + s, d);
+            if(seg_full(P)) load_seg_rest(O, s, d);
             DevSink<real> sink{&O, &P, 0};
             seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
             nc += sink.nc;
         } else {
             if(!act) continue;
             const Prob2D<real> P = Bd.probs[d.prob];
-            if(seg_full(P)) load_seg_rest(segs + s, d);
+            if(seg_full(P)) load_seg_rest(O, s, d);
             DevSink<real> sink{&O, &P, 0};
             seg_items<real, RT_TL, DevSink<real>, FAST>(d, P, P.base, P.Rz, sink);
             nc += sink.nc;
@@ -2598,14 +2621,21 @@ __global__ void __launch_bounds__(128) k_scatter_flat(BatchDev<real> Bd, DevOut 
     const int w = threadIdx.x / WAVE;
     uint32_t n = *O.seg_count;
     if(n > O.seg_cap) n = O.seg_cap;
-    const uint4 *src = reinterpret_cast<const uint4 *>(O.segs);
     const Seg<real> *S = reinterpret_cast<const Seg<real> *>(stage[w]);
     unsigned long long nc = 0;
     const uint32_t gw = blockIdx.x * NW + (uint32_t)w, nw = gridDim.x * NW;
     for(uint32_t base = gw * WAVE; base < n; base += nw * WAVE) {
         const int cnt = (int)min((uint32_t)WAVE, n - base);
-        const uint4 *g = src + (size_t)base * W16;
-        for(int k = lane; k < cnt * W16; k += WAVE) stage[w][k] = g[k];
+        for(int k = lane; k < cnt * W16; k += WAVE) {
+            // word wd of descriptor i: from its core, or from its rest
+            const int i = k / W16, wd = k - i * W16;
+            const size_t off = (size_t)wd * 16;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if(off < SEG_CS<real>) v = *reinterpret_cast<const uint4 *>(seg_core<real>(O, base + i) + off);
+            else if(off + 16 <= SEG_CS<real> + SEG_RS<real>)
+                v = *reinterpret_cast<const uint4 *>(seg_rest<real>(O, base + i) + (off - SEG_CS<real>));
+            stage[w][k] = v;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
