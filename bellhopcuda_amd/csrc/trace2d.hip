@@ -81,15 +81,20 @@ template<typename real> struct alignas(16) Seg {
     SegDesc<real> d;
 };
 
-// Segment buffer layout: slot s's descriptor core (its first SEG_CS bytes,
-// every field a Cartesian TL scatter reads) at segs + s SEG_CS, packed so
-// that the scatter fetches no other bytes; its rest (ray-centred geometry,
-// arrival and eigen fields) at segs + seg_cap SEG_CS + s SEG_RS, stored only
-// where seg_full. A slot takes sizeof(Seg<real>) bytes in all.
-template<typename real> constexpr size_t SEG_CS = (seg_core_bytes<real>() + 15) / 16 * 16;
-template<typename real> constexpr size_t SEG_RS = sizeof(Seg<real>) - SEG_CS<real>;
-static_assert(SEG_RS<double> >= sizeof(SegDesc<double>) - SEG_CS<double>, "descriptor rest does not fit");
-static_assert(SEG_RS<float> >= sizeof(SegDesc<float>) - SEG_CS<float>, "descriptor rest does not fit");
+// Segment buffer layout, structure of arrays: 16-byte word k of slot s's
+// descriptor at segs + (k seg_cap + s) 16, so that a wavefront's 64 lanes
+// store and load consecutive words (coalesced, one 16-byte access per lane
+// and word). Words below SEG_NC hold the core (every field a Cartesian TL
+// scatter reads); the rest (ray-centred geometry, arrival and eigen fields)
+// is stored only where seg_full. A slot takes sizeof(Seg<real>) bytes of the
+// buffer in all.
+template<typename real> constexpr int SEG_NC = (int)((seg_core_bytes<real>() + 15) / 16);
+template<typename real> constexpr int SEG_NW = (int)((sizeof(SegDesc<real>) + 15) / 16);
+static_assert(SEG_NW<double> * 16 <= sizeof(Seg<double>) && SEG_NW<float> * 16 <= sizeof(Seg<float>), "slot size");
+template<typename real> union SegWords {
+    SegDesc<real> d;
+    unsigned long long w[2 * SEG_NW<real>];
+};
 
 
 // Finished-ray progress of a launch (completedRayCount / totalJobs,
@@ -142,26 +147,44 @@ struct DevOut {
     Progress prog;
 };
 
-template<typename real> __device__ inline char *seg_core(const DevOut &O, size_t s)
+// First 8 bytes of 16-byte word k of slot s.
+__device__ inline unsigned long long *seg_word(const DevOut &O, int k, size_t s)
 {
-    return reinterpret_cast<char *>(O.segs) + s * SEG_CS<real>;
-}
-template<typename real> __device__ inline char *seg_rest(const DevOut &O, size_t s)
-{
-    return reinterpret_cast<char *>(O.segs) + (size_t)O.seg_cap * SEG_CS<real> + s * SEG_RS<real>;
+    return reinterpret_cast<unsigned long long *>(O.segs) + 2 * ((size_t)k * O.seg_cap + s);
 }
 template<typename real> __device__ inline int seg_nitems(const DevOut &O, size_t s)
 {
-    return reinterpret_cast<const SegDesc<real> *>(seg_core<real>(O, s))->nitems;
+    constexpr size_t off = offsetof(SegDesc<real>, nitems);
+    return reinterpret_cast<const int32_t *>(seg_word(O, (int)(off / 16), s))[(off % 16) / 4];
+}
+template<typename real> __device__ inline void clear_seg(const DevOut &O, size_t s)
+{
+    constexpr size_t off = offsetof(SegDesc<real>, nitems);
+    reinterpret_cast<int32_t *>(seg_word(O, (int)(off / 16), s))[(off % 16) / 4] = 0;
+}
+template<typename real> __device__ inline void load_seg_words(const DevOut &O, size_t s, int k0, int k1, SegWords<real> &u)
+{
+#pragma unroll
+    for(int k = 0; k < SEG_NW<real>; ++k)
+        if(k >= k0 && k < k1) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(seg_word(O, k, s));
+            u.w[2 * k] = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
+            u.w[2 * k + 1] = (unsigned long long)v.z | ((unsigned long long)v.w << 32);
+        }
 }
 template<typename real> __device__ inline void load_seg_core(const DevOut &O, size_t s, SegDesc<real> &d)
 {
-    __builtin_memcpy(&d, seg_core<real>(O, s), SEG_CS<real>);
+    SegWords<real> u;
+    load_seg_words(O, s, 0, SEG_NC<real>, u);
+    __builtin_memcpy(&d, &u, seg_core_bytes<real>());
 }
 template<typename real> __device__ inline void load_seg_rest(const DevOut &O, size_t s, SegDesc<real> &d)
 {
-    __builtin_memcpy(reinterpret_cast<char *>(&d) + SEG_CS<real>, seg_rest<real>(O, s),
-                     sizeof(SegDesc<real>) - SEG_CS<real>);
+    SegWords<real> u;
+    constexpr size_t c = seg_core_bytes<real>();
+    load_seg_words(O, s, (int)(c / 16), SEG_NW<real>, u);
+    __builtin_memcpy(reinterpret_cast<char *>(&d) + c, reinterpret_cast<const char *>(&u) + c,
+                     sizeof(SegDesc<real>) - c);
 }
 
 template<typename real> struct DevSink {
@@ -266,7 +289,7 @@ template<typename real> __device__ inline void seal_block(const SegBlock &B, int
 {
     for(uint32_t k = B.used + (uint32_t)lane; k < B.cap; k += WAVE) {
         uint32_t slot = B.base + k;
-        if(slot < O.seg_cap) reinterpret_cast<SegDesc<real> *>(seg_core<real>(O, slot))->nitems = 0;
+        if(slot < O.seg_cap) clear_seg<real>(O, slot);
     }
 }
 
@@ -296,21 +319,17 @@ __device__ inline void append_seg(const SegDesc<real> &d, int lane, const DevOut
             // counted apart from LDS reads; a generic store would make every
             // later LDS wait wait for it too)
             typedef __attribute__((address_space(1))) unsigned long long gu64;
-            constexpr int NC = (int)(SEG_CS<real> / 8);
-            constexpr int NR = (int)((sizeof(SegDesc<real>) - SEG_CS<real> + 7) / 8);
-            union {
-                SegDesc<real> d;
-                unsigned long long w[NC + NR];
-            } u;
+            SegWords<real> u;
             u.d = d;
-            gu64 *dc = (gu64 *)seg_core<real>(O, slot);
+            gu64 *dst = (gu64 *)O.segs + 2 * (size_t)slot;
+            const size_t cap2 = 2 * (size_t)O.seg_cap;
+            const int nw = full ? SEG_NW<real> : SEG_NC<real>;
 #pragma unroll
-            for(int k = 0; k < NC; ++k) dc[k] = u.w[k];
-            if(full) {
-                gu64 *dr = (gu64 *)seg_rest<real>(O, slot);
-#pragma unroll
-                for(int k = 0; k < NR; ++k) dr[k] = u.w[NC + k];
-            }
+            for(int k = 0; k < SEG_NW<real>; ++k)
+                if(k < nw) {
+                    dst[k * cap2] = u.w[2 * k];
+                    dst[k * cap2 + 1] = u.w[2 * k + 1];
+                }
         } else {
             atomicOr(O.overflow, 1u);
         }
@@ -2626,15 +2645,11 @@ __global__ void __launch_bounds__(128) k_scatter_flat(BatchDev<real> Bd, DevOut 
     const uint32_t gw = blockIdx.x * NW + (uint32_t)w, nw = gridDim.x * NW;
     for(uint32_t base = gw * WAVE; base < n; base += nw * WAVE) {
         const int cnt = (int)min((uint32_t)WAVE, n - base);
-        for(int k = lane; k < cnt * W16; k += WAVE) {
-            // word wd of descriptor i: from its core, or from its rest
-            const int i = k / W16, wd = k - i * W16;
-            const size_t off = (size_t)wd * 16;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if(off < SEG_CS<real>) v = *reinterpret_cast<const uint4 *>(seg_core<real>(O, base + i) + off);
-            else if(off + 16 <= SEG_CS<real> + SEG_RS<real>)
-                v = *reinterpret_cast<const uint4 *>(seg_rest<real>(O, base + i) + (off - SEG_CS<real>));
-            stage[w][k] = v;
+        // word k of the descriptors base .. base + cnt - 1 (consecutive lanes,
+        // consecutive slots) into the stage's records
+        for(int t = lane; t < cnt * SEG_NW<real>; t += WAVE) {
+            const int k = t / cnt, i = t - k * cnt;
+            stage[w][(size_t)i * W16 + k] = *reinterpret_cast<const uint4 *>(seg_word(O, k, base + i));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
