@@ -1089,6 +1089,7 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     // receiver range reads no range from memory; the comparisons are the walk's.
     int ir = I.ir, first = I.ir;
     real rlo = I.rbr.x, rhi = I.rbr.y;
+    const real rfirst = rlo;    // Rr[first]
     for(;;) {
         int irTT;
         if(rB > rlo) {
@@ -1110,6 +1111,10 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     I.ir = ir;
     I.rbr = V(rlo, rhi);
     I.x = p1.x;
+    // A step that crosses no receiver range has the one column Rr[first]:
+    // outside [lo, hi) the scatter's column test accepts none of its
+    // receivers, so the segment is not handed to the scatter at all.
+    if(ir == first && !(rfirst >= d.lo && rfirst < d.hi)) return;
     d.ir0 = first;
     d.ir1 = ir;
     d.dir = (ir >= first) ? 1 : -1;

@@ -80,10 +80,11 @@ def test_sweep_against_oracle(gpu_ctx):
 
 
 def test_sweep_suspend_and_resume():
-    """A 3 GiB segment buffer holds ~11.5 M FP64 segments; the suspension
+    """A 3 GiB segment buffer holds ~10.9 M FP64 segments; the suspension
     margin (one poll interval of every wavefront) leaves room for ~6.8 M, so
-    12 Munk fields (~11.6 M steps, every step reaching a receiver range) need
-    at least two passes."""
+    24 Munk fields with 1001 receiver ranges (~23 M steps, most of them
+    crossing a receiver range and so handed to the scatter) need at least two
+    passes."""
     _announce("sweep_suspend")
     old = os.environ.get("BHC_SEG_BUFFER_MB")
     os.environ["BHC_SEG_BUFFER_MB"] = "3000"
@@ -95,7 +96,7 @@ def test_sweep_suspend_and_resume():
         else:
             os.environ["BHC_SEG_BUFFER_MB"] = old
     big = bh.Context(0)
-    envs = [munk_env(nbeams=2001 - 10 * k, nrz=51, nrr=401) for k in range(12)]
+    envs = [munk_env(nbeams=2001 - 10 * k, nrz=51, nrr=1001) for k in range(24)]
     probs = [bh.Problem(e) for e in envs]
     a = bh.Sweep(probs, small, stats=True)
     a.launch()
