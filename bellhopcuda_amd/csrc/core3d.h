@@ -1065,6 +1065,7 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     // receiver range reads none (the comparisons are the walk's)
     int ir = I.ir, first = I.ir;
     real rlo = I.rbr.x, rhi = I.rbr.y;
+    const real rfirst = rlo;    // Rr[first]
     for(;;) {
         int irTT;
         if(rB > rlo) {
@@ -1085,11 +1086,14 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     }
     I.ir = ir;
     I.rbr = V(rlo, rhi);
+    d.lo = rmin(rA, rB);
+    d.hi = rmax(rA, rB);
+    // a step that crosses no receiver range has the one range Rr[first]:
+    // outside [lo, hi) the scatter's row test accepts none of its receivers
+    if(ir == first && !(rfirst >= d.lo && rfirst < d.hi)) return;
     d.ir0 = first;
     d.ir1 = ir;
     d.dir = (ir >= first) ? 1 : -1;
-    d.lo = rmin(rA, rB);
-    d.hi = rmax(rA, rB);
     d.xa = p0.x;
     d.ta = rayt;
     d.e1 = e1;
