@@ -704,7 +704,7 @@ __host__ __device__ inline void Reflect(const RayPt<real> &oldp, RayPt<real> &ne
         newp.Amp = oldp.Amp;
         newp.Phase = oldp.Phase + RC<real>::PI;
     } else if(hs.bc == 'F') { // InterpolateReflectionCoefficient reflect.hpp:39-92
-        const real *tab = G + (isTop ? P.o_trc : P.o_brc);
+        const real *tab = B + (isTop ? P.o_trc : P.o_brc); // staged with the boundaries (LDS when it fits)
         int n = isTop ? P.ntrc : P.nbrc;
         real theta = (RL(180.0) / RC<real>::PI) * std::fabs(std::atan2(Th, Tg));
         if(theta > FL(90.0)) theta = FL(180.0) - theta;

@@ -135,8 +135,13 @@ __device__ inline void append3(const Desc &d, int lane, const DevOut3 &O, SegBlo
 }
 
 // Trace kernel: the ray loop of MainFieldModes (src/trace.hpp:525-589), 3D.
+// Reflection tables staged in LDS up to this many reals (top and bottom together).
+constexpr int REFL_LDS = 768;
+
 template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<real> P, DevOut3 O)
 {
+    __shared__ real refl_lds[REFL_LDS];
+    stage_refl_tables(P, refl_lds, REFL_LDS);
     const int lane = threadIdx.x & (WAVE - 1);
     const int njobs = O.job_end - O.job_begin;
     Err e = {0, 0, 0, 0};

@@ -207,7 +207,7 @@ __host__ __device__ inline void ReflectNx(const RayPt<real> &oldp, RayPt<real> &
         newp.Phase = oldp.Phase + RC<real>::PI;
     } else if(hs.bc == 'F') {
         real r, phi;
-        ReflTableO3D(N.base + (isTop ? N.o_trc : N.o_brc), isTop ? N.ntrc : N.nbrc, Th, Tg, r, phi, e);
+        ReflTableO3D(refl_tab(N, isTop), isTop ? N.ntrc : N.nbrc, Th, Tg, r, phi, e);
         newp.Amp = oldp.Amp * r;
         newp.Phase = oldp.Phase + phi;
     } else if(hs.bc == 'A' || hs.bc == 'G') {

@@ -11,8 +11,8 @@
 
 namespace bhc2d {
 
-// Blob layout: staged prefix [z, rho, c, cz, n2, n2z, coef, segr, cmat, czmat, ati, bty, Rz]
-// then [Rr, trc, brc, sbp, alpha, cos alpha, sin alpha].
+// Blob layout: staged prefix [z, rho, c, cz, n2, n2z, coef, segr, cmat, czmat, ati, bty, trc, brc, Rz]
+// then [Rr, sbp, alpha, cos alpha, sin alpha].
 template<typename real> int pack(const bhc_problem2d *p, std::vector<real> &blob, Prob2D<real> &P, std::string &err)
 {
     auto fail = [&](const char *m, char c = 0) {
@@ -67,13 +67,14 @@ template<typename real> int pack(const bhc_problem2d *p, std::vector<real> &blob
     P.o_bty = put_bdry(p->bty, p->bty_npts);
     P.ati_n = p->ati_npts;
     P.bty_n = p->bty_npts;
+    // reflection-coefficient tables (read at every 'F' reflection)
+    P.o_trc = put(p->trc, 3 * (size_t)p->trc_npts);
+    P.o_brc = put(p->brc, 3 * (size_t)p->brc_npts);
     // receiver depths (exact as reals) for the per-step depth windows
     P.o_rzs = (int32_t)blob.size();
     for(int i = 0; i < p->NRz; ++i) blob.push_back((real)p->Rz[i]);
     P.staged_reals = (int32_t)blob.size();
     P.o_Rr = put(p->Rr, p->NRr);
-    P.o_trc = put(p->trc, 3 * (size_t)p->trc_npts);
-    P.o_brc = put(p->brc, 3 * (size_t)p->brc_npts);
     P.ntrc = p->trc_npts;
     P.nbrc = p->brc_npts;
     P.o_sbp = put(p->sbp, 2 * (size_t)p->NSBPPts);

@@ -27,7 +27,18 @@ def _announce(tag, name):
     sys.stdout.flush()
 
 
+def _refl_table(env, n):
+    """Bottom reflection coefficients from a table ('F') of n angles (staged
+    in LDS by the trace kernel when they fit)."""
+    from bellhopcuda_amd.env import ReflTable
+    th = np.linspace(0.0, 90.0, n)
+    env.brc = ReflTable(th, 1.0 - 0.6 * th / 90.0, 180.0 - 0.5 * th)
+    return env
+
+
 CASES = {
+    "munk_gauss_refl_table": lambda: _refl_table(munk_env3d(nalpha=301, nbeta=1, ntheta=1, nrz=51, nrr=101,
+                                                            run_type="CB   2", bot="F"), 12),
     "munk_hat_bearing0": lambda: munk_env3d(nalpha=401, nbeta=1, ntheta=1, nrz=51, nrr=201, run_type="CG   2"),
     "munk_gauss_rotated": lambda: _rotated_munk(),
     "munk_gauss_rotated_f32": lambda: _rotated_munk(precision=32),

@@ -19,7 +19,19 @@ from parity import compare_field, tl_db
 
 pytestmark = pytest.mark.gpu
 
+def _refl_table(env, n):
+    """Bottom reflection coefficients read from a table ('F') of n angles:
+    12 angles are staged in LDS by the trace kernel, 300 (900 reals) are read
+    from HBM."""
+    from bellhopcuda_amd.env import ReflTable
+    th = np.linspace(0.0, 90.0, n)
+    env.brc = ReflTable(th, 1.0 - 0.6 * th / 90.0, 180.0 - 0.5 * th)
+    return env
+
+
 CASES = {
+    "munk_gauss_refl_table": lambda: _refl_table(munk_env3d(nalpha=151, nbeta=7, bot="F"), 12),
+    "munk_gauss_refl_table_unstaged": lambda: _refl_table(munk_env3d(nalpha=151, nbeta=7, bot="F"), 300),
     "free_gauss_f64": lambda: free_env3d(),
     "free_hat_f64": lambda: free_env3d(run_type="CG   3"),
     "free_gauss_incoherent_f32": lambda: free_env3d(run_type="IB   3", precision=32),
