@@ -104,7 +104,6 @@ template<typename real> struct Prob3D {
     char ati_t0, ati_t1, bty_t0, bty_t1;    // grid types ('R' / 'C', 'S' / 'L')
     int32_t o_atix, o_btyx, o_atip, o_btyp; // per-node extras [NODEX], province tables [5] (cP, cS, rho)
     int32_t o_Rr, o_trc, o_brc, ntrc, nbrc, o_sbp, nsbp;
-    const real *tabs;      // when set: a staged copy of the top then the bottom reflection table (refl_tab)
     int32_t o_alpha, o_acos, o_asin, o_beta, o_bcos, o_bsin, o_Sx, o_Sy, o_trcv;
     const float *Rz, *Sz;
     int32_t NSx, NSy, NSz, NRz, NRr, Ntheta, Nalpha, iSingleA, Nbeta, iSingleB;
@@ -725,25 +724,27 @@ __host__ __device__ inline void refl_bdry3(const Prob3D<real> &P, bool isTop, co
 
 // InterpolateReflectionCoefficient (reflect.hpp:39-92) at the incidence angle
 // |atan2(Th, Tg)| folded about 90 degrees; tab holds (theta, |R|, phase) rows.
-// The reflection-coefficient table of a boundary: the staged copy when a
-// kernel made one (stage_refl_tables), else the problem's blob.
-template<typename real> __host__ __device__ inline const real *refl_tab(const Prob3D<real> &P, bool isTop)
+// The reflection-coefficient table of a boundary: in `tabs`, a staged copy
+// of the top then the bottom table (stage_refl_tables), when a kernel made
+// one, else in the problem's blob.
+template<typename real>
+__host__ __device__ inline const real *refl_tab(const Prob3D<real> &P, bool isTop, const real *tabs)
 {
-    if(P.tabs) return P.tabs + (isTop ? 0 : 3 * P.ntrc);
+    if(tabs) return tabs + (isTop ? 0 : 3 * P.ntrc);
     return P.base + (isTop ? P.o_trc : P.o_brc);
 }
 
 // Copy the top and bottom reflection tables into `lds` (cap reals) when they
-// fit, for Reflect to read them from LDS. Called by every thread of the
-// workgroup (it synchronises it).
-template<typename real> __device__ inline void stage_refl_tables(Prob3D<real> &P, real *lds, int cap)
+// fit, for Reflect to read them from LDS; returns the copy, or null. Called
+// by every thread of the workgroup (it synchronises it).
+template<typename real> __device__ inline const real *stage_refl_tables(const Prob3D<real> &P, real *lds, int cap)
 {
     const int nt = 3 * P.ntrc, nb = 3 * P.nbrc;
-    if(nt + nb <= 0 || nt + nb > cap) return;
+    if(nt + nb <= 0 || nt + nb > cap) return nullptr;
     for(int i = threadIdx.x; i < nt; i += blockDim.x) lds[i] = P.base[P.o_trc + i];
     for(int i = threadIdx.x; i < nb; i += blockDim.x) lds[nt + i] = P.base[P.o_brc + i];
     __syncthreads();
-    P.tabs = lds;
+    return lds;
 }
 
 template<typename real>
@@ -789,7 +790,7 @@ __host__ __device__ inline cpx<real> HalfspaceReflO3D(real omega, real Tg, real 
 template<typename real>
 __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &newp, const HSd<real> &hs, bool isTop,
                                          v3<real> nB, real zxx, real zxy, real zyy, const Prob3D<real> &P,
-                                         Trace3<real> &S, Err &e)
+                                         Trace3<real> &S, Err &e, const real *tabs = nullptr)
 {
     real Th = dot3(oldp.t, nB);
     v3<real> tB = oldp.t - sc3(Th, nB);
@@ -833,7 +834,7 @@ __host__ __device__ inline void Reflect3(const Ray3<real> &oldp, Ray3<real> &new
         newp.Phase = oldp.Phase + RC<real>::PI;
     } else if(hs.bc == 'F') {
         real r, phi;
-        ReflTableO3D(refl_tab(P, isTop), isTop ? P.ntrc : P.nbrc, Th, Tg, r, phi, e);
+        ReflTableO3D(refl_tab(P, isTop, tabs), isTop ? P.ntrc : P.nbrc, Th, Tg, r, phi, e);
         newp.Amp = oldp.Amp * r;
         newp.Phase = oldp.Phase + phi;
     } else if(hs.bc == 'A' || hs.bc == 'G') {
@@ -949,7 +950,7 @@ __host__ __device__ inline bool RayInit3(int ia, int ib, int isx, int isy, int i
 
 template<typename real>
 __host__ __device__ inline bool RayUpdate3(const Ray3<real> &p0, Ray3<real> &p1, Ray3<real> &p2, Trace3<real> &S,
-                                           const Prob3D<real> &P, Err &e)
+                                           const Prob3D<real> &P, Err &e, const real *tabs = nullptr)
 { // trace.hpp:274-381 (3D, rectilinear boundaries)
     bool topRefl = false, botRefl = false;
     Step3(p0, p1, S, P, e, topRefl, botRefl);
@@ -958,7 +959,7 @@ __host__ __device__ inline bool RayUpdate3(const Ray3<real> &p0, Ray3<real> &p1,
     if(topRefl || botRefl) {
         ReflB3<real> r;
         refl_bdry3(P, topRefl, topRefl ? S.top : S.bot, p1.x, r);
-        Reflect3(p1, p2, r.hs, topRefl, r.n, r.zxx, r.zxy, r.zyy, P, S, e);
+        Reflect3(p1, p2, r.hs, topRefl, r.n, r.zxx, r.zxy, r.zyy, P, S, e, tabs);
         return true;
     }
     return false;

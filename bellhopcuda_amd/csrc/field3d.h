@@ -141,7 +141,7 @@ constexpr int REFL_LDS = 768;
 template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64) k_trace3d(Prob3D<real> P, DevOut3 O)
 {
     __shared__ real refl_lds[REFL_LDS];
-    stage_refl_tables(P, refl_lds, REFL_LDS);
+    const real *tabs = stage_refl_tables(P, refl_lds, REFL_LDS);
     const int lane = threadIdx.x & (WAVE - 1);
     const int njobs = O.job_end - O.job_begin;
     Err e = {0, 0, 0, 0};
@@ -201,7 +201,7 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
         d.nitems = 0;
         bool two = false;
         if(active) {
-            two = RayUpdate3(p0, p1, p2, S, P, e);
+            two = RayUpdate3(p0, p1, p2, S, P, e, tabs);
             if constexpr(RCEN) prelude3rc(d, p0, p1, I, is, P, isz, isx, isy);
             else prelude3(d, p0, p1, I, is, P, isz, isx, isy);
             d.job = job;

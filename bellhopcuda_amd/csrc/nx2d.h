@@ -162,7 +162,8 @@ __host__ __device__ inline void StepNx(const RayPt<real> &ray0, RayPt<real> &ray
 template<typename real>
 __host__ __device__ inline void ReflectNx(const RayPt<real> &oldp, RayPt<real> &newp, const HSd<real> &hs, bool isTop,
                                           v3<real> nBdry, real kxx, real kxy, real kyy, const OrgNx<real> &org,
-                                          const Prob3D<real> &N, const Prob2D<real> &P, Trace3<real> &S, Err &e)
+                                          const Prob3D<real> &N, const Prob2D<real> &P, Trace3<real> &S, Err &e,
+                                          const real *tabs = nullptr)
 {
     v2<real> nB = OceanToRayT(nBdry, org);
     nB = sc(RL(1.0) / std::sqrt(dot(nB, nB)), nB);
@@ -207,7 +208,7 @@ __host__ __device__ inline void ReflectNx(const RayPt<real> &oldp, RayPt<real> &
         newp.Phase = oldp.Phase + RC<real>::PI;
     } else if(hs.bc == 'F') {
         real r, phi;
-        ReflTableO3D(refl_tab(N, isTop), isTop ? N.ntrc : N.nbrc, Th, Tg, r, phi, e);
+        ReflTableO3D(refl_tab(N, isTop, tabs), isTop ? N.ntrc : N.nbrc, Th, Tg, r, phi, e);
         newp.Amp = oldp.Amp * r;
         newp.Phase = oldp.Phase + phi;
     } else if(hs.bc == 'A' || hs.bc == 'G') {
@@ -319,7 +320,7 @@ __host__ __device__ inline bool RayInitNx(int ia, int ib, int isx, int isy, int 
 template<typename real>
 __host__ __device__ inline bool RayUpdateNx(const RayPt<real> &p0, RayPt<real> &p1, RayPt<real> &p2, Trace3<real> &S,
                                             const OrgNx<real> &org, const Prob3D<real> &N, const Prob2D<real> &P,
-                                            Err &e)
+                                            Err &e, const real *tabs = nullptr)
 {
     bool topRefl = false, botRefl = false;
     StepNx(p0, p1, S, org, N, e, topRefl, botRefl);
@@ -329,7 +330,7 @@ __host__ __device__ inline bool RayUpdateNx(const RayPt<real> &p0, RayPt<real> &
     if(topRefl || botRefl) {
         ReflB3<real> r;
         refl_bdry3(N, topRefl, topRefl ? S.top : S.bot, x_o, r);
-        ReflectNx(p1, p2, r.hs, topRefl, r.n, r.kxx, r.kxy, r.kyy, org, N, P, S, e);
+        ReflectNx(p1, p2, r.hs, topRefl, r.n, r.kxx, r.kxy, r.kyy, org, N, P, S, e, tabs);
         return true;
     }
     return false;

@@ -859,7 +859,7 @@ __global__ void __launch_bounds__(64) k_tracenx2d(Prob2D<real> P, bhc3d::Prob3D<
     using bhc3d::OrgNx;
     using bhc3d::Trace3;
     __shared__ real refl_lds[768];  // the reflection tables in LDS when they fit
-    bhc3d::stage_refl_tables(N, refl_lds, 768);
+    const real *tabs = bhc3d::stage_refl_tables(N, refl_lds, 768);
     const real *G = P.base;
     const int lane = threadIdx.x & (WAVE - 1);
     const int njobs = O.job_end - O.job_begin;
@@ -922,7 +922,7 @@ __global__ void __launch_bounds__(64) k_tracenx2d(Prob2D<real> P, bhc3d::Prob3D<
         d.nitems = 0;
         bool two = false;
         if(active) {
-            two = bhc3d::RayUpdateNx(p0, p1, p2, S, org, N, P, e);
+            two = bhc3d::RayUpdateNx(p0, p1, p2, S, org, N, P, e, tabs);
             prelude(d, p0, p1, I, is, P, G, G + P.o_rzs, isf, ia);
             ++is;
         }
