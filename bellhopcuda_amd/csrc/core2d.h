@@ -1065,23 +1065,9 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     v2<real> rayt = p1.x - p0.x;
     real rlen = std::sqrt(dot(rayt, rayt));
     if(rlen < RL(1.0e3) * spacing(p1.x.x)) return;
-    rayt = dv(rayt, rlen);
     real phaseq = p0.q.x;
     if(IsAtCaustic(I.qOld, phaseq)) I.phase += RC<real>::PI / FL(2.0);
     I.qOld = phaseq;
-    real sigma = rmax(std::fabs(p0.q.x), std::fabs(p1.q.x)) * I.rcp_q0 / std::fabs(rayt.x);
-    if(P.isGaussian) {
-        real lambda = p0.c / P.freq0;
-        sigma = rmax(sigma, rmin(FL(0.2) * P.freq0 * p1.tau.re, RC<real>::PI * lambda));
-    }
-    real RadiusMax = I.BeamWindow * sigma;
-    if(std::fabs(rayt.x) > FL(0.50001)) {
-        d.zmin = rmin(p0.x.y, p1.x.y) - RadiusMax;
-        d.zmax = rmax(p0.x.y, p1.x.y) + RadiusMax;
-    } else {
-        d.zmin = -RC<real>::MAXV;
-        d.zmax = RC<real>::MAXV;
-    }
     d.lo = rmin(rA, rB);
     d.hi = rmax(rA, rB);
     // walk: the set of visited receiver ranges is contiguous. The bracket
@@ -1115,6 +1101,21 @@ __host__ __device__ inline void prelude_cart(SegDesc<real> &d, const RayPt<real>
     // outside [lo, hi) the scatter's column test accepts none of its
     // receivers, so the segment is not handed to the scatter at all.
     if(ir == first && !(rfirst >= d.lo && rfirst < d.hi)) return;
+    // the beam's reach: only for steps handed to the scatter
+    rayt = dv(rayt, rlen);
+    real sigma = rmax(std::fabs(p0.q.x), std::fabs(p1.q.x)) * I.rcp_q0 / std::fabs(rayt.x);
+    if(P.isGaussian) {
+        real lambda = p0.c / P.freq0;
+        sigma = rmax(sigma, rmin(FL(0.2) * P.freq0 * p1.tau.re, RC<real>::PI * lambda));
+    }
+    real RadiusMax = I.BeamWindow * sigma;
+    if(std::fabs(rayt.x) > FL(0.50001)) {
+        d.zmin = rmin(p0.x.y, p1.x.y) - RadiusMax;
+        d.zmax = rmax(p0.x.y, p1.x.y) + RadiusMax;
+    } else {
+        d.zmin = -RC<real>::MAXV;
+        d.zmax = RC<real>::MAXV;
+    }
     d.ir0 = first;
     d.ir1 = ir;
     d.dir = (ir >= first) ? 1 : -1;
