@@ -324,7 +324,7 @@ class Surrogate:
     """The modal surrogate: a ModalBasis of the training fields and an
     ensemble of `members` networks mapping parameters to its coefficients."""
 
-    def __init__(self, thetas, fields, family="global", k=64, members=4, seed=11, sigma=0.05, max_batch=1024):
+    def __init__(self, thetas, fields, family="global", k=128, members=4, seed=11, sigma=0.05, max_batch=2048):
         self.family = family
         self.basis = ModalBasis(fields, k, seed=seed)
         self.mlp = Mlp(members=members, max_batch=max_batch, seed=seed, sigma=sigma, n_out=k, front=1)
@@ -333,7 +333,7 @@ class Surrogate:
         self.mlp.set_dataset(self.X, self.basis.train_coeffs)
         self.steps, self.train_ms = 0, 0.0
 
-    def train(self, steps, batch=512, lr=1e-3, chunk=500, floor=0.02):
+    def train(self, steps, batch=2048, lr=1e-3, chunk=500, floor=0.02):
         """Adam with a cosine learning-rate decay from lr to floor * lr, one
         value per chunk of steps. Returns the last per-member loss."""
         done, loss = 0, None
