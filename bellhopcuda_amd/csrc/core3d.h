@@ -1068,21 +1068,10 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     v3<real> rayt = p1.x - p0.x;
     real rlen = len3(rayt);
     if(rlen <= RL(1.0e3) * spacing(p1.x.x)) return;
-    rayt = dv3(rayt, rlen);
-    v3<real> e1, e2;
-    RayNormal3(rayt, p1.phi, true, (real)RL(1.0), e1, e2);
     real phaseq = mdet(p0.q);
     if((phaseq <= RL(0.0) && I.qOld > RL(0.0)) || (phaseq >= RL(0.0) && I.qOld < RL(0.0)))
         I.phase += RC<real>::PI / FL(2.0);
     I.qOld = phaseq;
-    real a0 = p0.q.a00 * I.rcp_q0, b0 = p0.q.a10 * I.rcp_q0, a1 = p1.q.a00 * I.rcp_q0, b1 = p1.q.a10 * I.rcp_q0;
-    real l1 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
-    a0 = p0.q.a01 * I.rcp_qhat0;
-    b0 = p0.q.a11 * I.rcp_qhat0;
-    a1 = p1.q.a01 * I.rcp_qhat0;
-    b1 = p1.q.a11 * I.rcp_qhat0;
-    real l2 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
-    d.Ldiag = std::sqrt(l1 * l1 + l2 * l2);
     // receiver walk: the visited ranges are contiguous; the bracket
     // (Rr[ir], Rr[ir + 1]) travels with the ray, so a step that crosses no
     // receiver range reads none (the comparisons are the walk's)
@@ -1114,6 +1103,18 @@ __host__ __device__ inline void prelude3(Seg3D<real> &d, const Ray3<real> &p0, c
     // a step that crosses no receiver range has the one range Rr[first]:
     // outside [lo, hi) the scatter's row test accepts none of its receivers
     if(ir == first && !(rfirst >= d.lo && rfirst < d.hi)) return;
+    // the beam's frame and reach: only for steps handed to the scatter
+    rayt = dv3(rayt, rlen);
+    v3<real> e1, e2;
+    RayNormal3(rayt, p1.phi, true, (real)RL(1.0), e1, e2);
+    real a0 = p0.q.a00 * I.rcp_q0, b0 = p0.q.a10 * I.rcp_q0, a1 = p1.q.a00 * I.rcp_q0, b1 = p1.q.a10 * I.rcp_q0;
+    real l1 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
+    a0 = p0.q.a01 * I.rcp_qhat0;
+    b0 = p0.q.a11 * I.rcp_qhat0;
+    a1 = p1.q.a01 * I.rcp_qhat0;
+    b1 = p1.q.a11 * I.rcp_qhat0;
+    real l2 = rmax(std::sqrt(a0 * a0 + b0 * b0), std::sqrt(a1 * a1 + b1 * b1));
+    d.Ldiag = std::sqrt(l1 * l1 + l2 * l2);
     d.ir0 = first;
     d.ir1 = ir;
     d.dir = (ir >= first) ? 1 : -1;
