@@ -1190,8 +1190,8 @@ __host__ __device__ inline bool row3(const Seg3D<real> &d, int ir, int itheta, c
 // the reference. Returns false for an empty row; (wlo, whi) is the band, or
 // the full window [zmin, zmax] when the bound does not apply.
 template<typename real>
-__host__ __device__ inline bool band3(const Seg3D<real> &d, real xr, real yr, real zmin, real zmax,
-                                      const Prob3D<real> &P, real &wlo, real &whi)
+__host__ __device__ inline bool band3_circle(const Seg3D<real> &d, real xr, real yr, real zmin, real zmax,
+                                             const Prob3D<real> &P, real &wlo, real &whi)
 {
     wlo = zmin;
     whi = zmax;
@@ -1218,6 +1218,210 @@ __host__ __device__ inline bool band3(const Seg3D<real> &d, real xr, real yr, re
     wlo = rmax(zmin, (real)(zc - dz));
     whi = rmin(zmax, (real)(zc + dz));
     return whi >= wlo;
+}
+
+// Narrows a row's depth band by each beam-plane coordinate separately. With
+// w = (n1, n2) the receiver's offsets along e1, e2 and A the row-scaled q at
+// its s (rows (a00, a01), (a10, a11) as the beam-window test uses them), the
+// test's beam coordinates are m = A^-1 w, so w1 = a00 m1 + a01 m2 and
+// w2 = a10 m1 + a11 m2: Gaussian beams (|m1| + |m2| <= 4) need |w1| <= 4
+// max(|a00|, |a01|), hat beams (max |mi| <= 1) |w1| <= |a00| + |a01|, and
+// likewise for w2. |a_ij| is bounded over the band by |A0_ij| + smax |dA_ij|.
+// Along the column w_i is affine in z, so each bound is an interval of z.
+// For a flat beam (widths 50:1 on the shelf-canyon field) the coordinate
+// along the narrow width limits the band where the circle of radius
+// K |A|_F does not. The bounds are widened for the rounding of the test's own
+// arithmetic (which grows with A's condition number); ill-conditioned or
+// degenerate segments keep the circle band.
+// a z <= b applied to the interval [L, H] (a == 0 and b < 0 empties it).
+__host__ __device__ inline void clip_le(double a, double b, double &L, double &H)
+{
+    if(a > 0.0) H = rmin(H, b / a);
+    else if(a < 0.0) L = rmax(L, b / a);
+    else if(b < 0.0) H = -1e300;
+}
+
+// z in [zl, zh] with |c + g z| <= alpha + beta |s(z)|, s(z) = p + sig z: the
+// hull of the solutions with s >= 0 and with s <= 0 (an empty result sets
+// lo > hi).
+__host__ __device__ inline void abs_affine_hull(double c, double g, double alpha, double beta, double p, double sig,
+                                                double zl, double zh, double &lo, double &hi)
+{
+    double L1 = zl, H1 = zh, L2 = zl, H2 = zh;
+    // s >= 0: |s| = p + sig z
+    clip_le(g - beta * sig, alpha + beta * p - c, L1, H1);
+    clip_le(-g - beta * sig, alpha + beta * p + c, L1, H1);
+    clip_le(-sig, p, L1, H1);
+    // s <= 0: |s| = -p - sig z
+    clip_le(g + beta * sig, alpha - beta * p - c, L2, H2);
+    clip_le(-g + beta * sig, alpha - beta * p + c, L2, H2);
+    clip_le(sig, -p, L2, H2);
+    const bool e1 = !(L1 <= H1), e2 = !(L2 <= H2);
+    if(e1 && e2) {
+        lo = 1.0;
+        hi = -1.0;
+    } else if(e1) {
+        lo = L2;
+        hi = H2;
+    } else if(e2) {
+        lo = L1;
+        hi = H1;
+    } else {
+        lo = rmin(L1, L2);
+        hi = rmax(H1, H2);
+    }
+}
+
+// Roots of c2 s^2 + c1 s + c0 = 0 appended to r (at most two).
+__host__ __device__ inline void quad_roots(double c2, double c1, double c0, double *r, int &n)
+{
+    if(std::fabs(c2) <= 1e-300 * (std::fabs(c1) + std::fabs(c0)) || c2 == 0.0) {
+        if(c1 != 0.0) r[n++] = -c0 / c1;
+        return;
+    }
+    const double disc = c1 * c1 - 4.0 * c2 * c0;
+    if(!(disc >= 0.0)) return;
+    const double sq = std::sqrt(disc);
+    const double qq = -0.5 * (c1 + (c1 >= 0.0 ? sq : -sq));
+    r[n++] = qq / c2;
+    if(qq != 0.0) r[n++] = c0 / qq;
+}
+
+// Narrows a row's depth band by the beam-plane coordinates separately. With
+// w = (n1, n2) the receiver's offsets along e1, e2 and A the row-scaled q at
+// its s (rows (a00, a01), (a10, a11) as the beam-window test uses them), the
+// test's beam coordinates are m = A^-1 w, so w1 = a00 m1 + a01 m2 and
+// w2 = a10 m1 + a11 m2: Gaussian beams (|m1| + |m2| <= 4) need |w1| <= 4
+// max(|a00|, |a01|), hat beams (max |mi| <= 1) |w1| <= |a00| + |a01|, and
+// likewise for w2 and for the column's own direction in the (e1, e2) plane.
+// |a_ij(s)| <= |A0_ij| + |s| |dA_ij|, and along the column w_i and s are
+// affine in z, so each bound is a condition |c + g z| <= alpha + beta |s(z)|
+// solved exactly for z. For a flat beam (widths 50:1 on the shelf-canyon
+// field) the coordinate along the narrow width limits the band where the
+// circle of radius K |A|_F does not.
+// The test's own arithmetic rounds with an error that grows with A's
+// condition number (|a00 a11| + |a01 a10|) / |det A|, so the column is cut
+// where |det A(s)| = tau N (N bounding the numerator over the band): pieces
+// with |det A| >= tau N get the bounds widened by 256 eps / tau, the others
+// (around the zeros of det A(s)) are kept whole. The result is the hull of
+// the pieces' bands.
+template<typename real>
+__host__ __device__ inline bool band3_axes(const Seg3D<real> &d, real xr, real yr, const Prob3D<real> &P, real &wlo,
+                                           real &whi)
+{
+    const double zl = (double)wlo, zh = (double)whi;
+    if(!(zl == zl) || !(zh == zh) || !(std::fabs(zl) < 1e30) || !(std::fabs(zh) < 1e30)) return true;
+    const double L = (double)d.rlen, za = (double)d.xa.z;
+    const double hx = (double)xr - (double)d.xa.x, hy = (double)yr - (double)d.xa.y;
+    const double sig = (double)d.ta.z / L;
+    const double p = (hx * (double)d.ta.x + hy * (double)d.ta.y) / L - za * sig;   // s(z) = p + sig z
+    const double smax = rmax(std::fabs(p + sig * zl), std::fabs(p + sig * zh));
+    const double rq = (double)d.rcp_q0, rh = (double)d.rcp_qhat0;
+    const double sa00 = (double)d.q0.a00 * rq, sa01 = (double)d.q0.a01 * rh;
+    const double sa10 = (double)d.q0.a10 * rq, sa11 = (double)d.q0.a11 * rh;
+    const double sb00 = (double)d.dq.a00 * rq, sb01 = (double)d.dq.a01 * rh;
+    const double sb10 = (double)d.dq.a10 * rq, sb11 = (double)d.dq.a11 * rh;
+    const double a00 = std::fabs(sa00), a01 = std::fabs(sa01), a10 = std::fabs(sa10), a11 = std::fabs(sa11);
+    const double b00 = std::fabs(sb00), b01 = std::fabs(sb01), b10 = std::fabs(sb10), b11 = std::fabs(sb11);
+    // det A(s) = det0 + lin s + ddet s^2
+    const double det0 = sa00 * sa11 - sa10 * sa01;
+    const double lin = sa00 * sb11 + sb00 * sa11 - sa10 * sb01 - sb10 * sa01;
+    const double ddet = sb00 * sb11 - sb10 * sb01;
+    const double Nmax = (a00 + smax * b00) * (a11 + smax * b11) + (a01 + smax * b01) * (a10 + smax * b10);
+    if(!(Nmax > 0.0) || !(Nmax < 1e300)) return true;
+    const bool gauss = P.isGaussian;
+    const double u = (double)RC<real>::EPS;
+    const double tau = sizeof(real) == 8 ? 1e-9 : 1e-3;
+    const double T = tau * Nmax;
+    const double rel = rmax(1e-4, 256.0 * u / tau);
+    const double K = gauss ? 4.0 : 1.4142135623730951;
+    const double mag = std::fabs(hx) + std::fabs(hy) + rmax(std::fabs(zl - za), std::fabs(zh - za));
+    // alpha + beta |s| bounds the accepted |w| along a direction whose row
+    // weights are (x, y): x |a_0j| + y |a_1j| for columns j = 0, 1
+    const double slack0 = rel * K * (double)d.F0 + 1e-4 * mag + 1e-3, slack1 = rel * K * (double)d.Fd;
+    auto coef = [&](double x, double y, double &alpha, double &beta) {
+        const double c0 = x * a00 + y * a10, c1 = x * a01 + y * a11;
+        const double e0 = x * b00 + y * b10, e1 = x * b01 + y * b11;
+        alpha = (gauss ? 4.0 * rmax(c0, c1) : c0 + c1) * (1.0 + rel) + slack0;
+        beta = (gauss ? 4.0 * rmax(e0, e1) : e0 + e1) * (1.0 + rel) + slack1;
+    };
+    const double g1 = (double)d.e1.z, g2 = (double)d.e2.z;
+    const double c1 = hx * (double)d.e1.x + hy * (double)d.e1.y - za * g1;   // w1 = c1 + z g1
+    const double c2 = hx * (double)d.e2.x + hy * (double)d.e2.y - za * g2;   // w2 = c2 + z g2
+    const double gn = std::sqrt(g1 * g1 + g2 * g2);
+    double cc[3], gg[3], al[3], be[3];
+    cc[0] = c1;
+    gg[0] = g1;
+    coef(1.0, 0.0, al[0], be[0]);
+    cc[1] = c2;
+    gg[1] = g2;
+    coef(0.0, 1.0, al[1], be[1]);
+    int ncon = 2;
+    if(gn > 1e-9) {
+        // the column's direction n = (g1, g2) / |g|: |n . w| <= n' . (|w1|, |w2|), n' = |n|
+        cc[2] = (g1 * c1 + g2 * c2) / gn;
+        gg[2] = gn;
+        coef(std::fabs(g1) / gn, std::fabs(g2) / gn, al[2], be[2]);
+        ncon = 3;
+    }
+    for(int i = 0; i < ncon; ++i)
+        if(!(al[i] < 1e30) || !(be[i] < 1e30)) return true;
+    // pieces of [zl, zh] between the crossings of |det A(s)| = T
+    double zb[6];
+    int nb = 0;
+    zb[nb++] = zl;
+    if(sig != 0.0) {
+        double r[4];
+        int nr = 0;
+        quad_roots(ddet, lin, det0 - T, r, nr);
+        quad_roots(ddet, lin, det0 + T, r, nr);
+        for(int k = 0; k < nr; ++k) {
+            const double z = (r[k] - p) / sig;
+            if(z > zl && z < zh) zb[nb++] = z;
+        }
+    }
+    zb[nb++] = zh;
+    for(int k = 1; k < nb; ++k)
+        for(int j = k; j > 0 && zb[j - 1] > zb[j]; --j) {
+            const double t = zb[j];
+            zb[j] = zb[j - 1];
+            zb[j - 1] = t;
+        }
+    double lo = 1e300, hi = -1e300;
+    for(int k = 0; k + 1 < nb; ++k) {
+        const double zA = zb[k], zB = zb[k + 1];
+        const double sm = p + sig * (0.5 * (zA + zB));
+        const double fm = det0 + lin * sm + ddet * sm * sm;
+        double l = zA, h = zB;
+        if(std::fabs(fm) >= T) {
+            for(int i = 0; i < ncon && l <= h; ++i) {
+                double l2, h2;
+                abs_affine_hull(cc[i], gg[i], al[i], be[i], p, sig, l, h, l2, h2);
+                l = l2;
+                h = h2;
+            }
+        }
+        if(l <= h) {
+            lo = rmin(lo, l);
+            hi = rmax(hi, h);
+        }
+    }
+    if(!(lo <= hi)) return false;
+    // round outwards to real
+    real rlo = (real)lo, rhi = (real)hi;
+    if((double)rlo > lo) rlo = (real)(lo - std::fabs(lo) * 4.0 * u - 1e-30);
+    if((double)rhi < hi) rhi = (real)(hi + std::fabs(hi) * 4.0 * u + 1e-30);
+    wlo = rmax(wlo, rlo);
+    whi = rmin(whi, rhi);
+    return whi >= wlo;
+}
+
+template<typename real>
+__host__ __device__ inline bool band3(const Seg3D<real> &d, real xr, real yr, real zmin, real zmax,
+                                      const Prob3D<real> &P, real &wlo, real &whi)
+{
+    if(!band3_circle(d, xr, yr, zmin, zmax, P, wlo, whi)) return false;
+    return band3_axes(d, xr, yr, P, wlo, whi);
 }
 
 // Bearing prefilter of a segment: can any receiver on bearing (ct, st) with a

@@ -344,6 +344,189 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
 }
 
+// Index of the k-th set bit of m (k < popcount(m)), by halving.
+__device__ inline int nth_set_bit(unsigned long long m, int k)
+{
+    int pos = 0;
+    for(int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if(k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+// Makes one wavefront's LDS writes visible to its other lanes (and orders its
+// earlier LDS reads before later writes).
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Segments per wavefront group in k_scatter3d_flat.
+constexpr int SCAT3_G = 16;
+
+// Scatter with SCAT3_G consecutive segments per wavefront: their descriptors
+// are copied to LDS by coalesced loads; every (segment, bearing) pair goes
+// through the bearing prefilter, one pair per lane; the surviving (segment,
+// range, bearing) rows of all the group's segments are spread over the lanes
+// (a prefix sum of the row counts), each row's tests and depth window are
+// computed, and the receivers of all rows are spread over the lanes again.
+// The tests and contributions are those of k_scatter3d; the lanes stay busy
+// when each segment has only a few rows. The receiver ranges, bearing table
+// and depths are staged in LDS (the launch checks they fit).
+template<typename real, bool FAST, int RT = RT_TL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_scatter3d_flat(Prob3D<real> P, DevOut3 O)
+{
+    constexpr int G = SCAT3_G;
+    constexpr int W16 = (int)(sizeof(Seg3<real>) / 16);   // 16-byte words per descriptor
+    __shared__ uint4 stage_all[4][G * W16];
+    __shared__ unsigned long long bmask_all[4][G];
+    __shared__ int2 rowq_all[4][2 * WAVE];
+    extern __shared__ __align__(16) unsigned char lds3f[];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wib = threadIdx.x / WAVE;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / WAVE) + wib));
+    const uint32_t nwaves = gridDim.x * (blockDim.x / WAVE);
+    real *L = reinterpret_cast<real *>(lds3f);
+    float *Lz = reinterpret_cast<float *>(L + P.NRr + 2 * P.Ntheta);
+    for(int i = threadIdx.x; i < P.NRr; i += blockDim.x) L[i] = P.base[P.o_Rr + i];
+    for(int i = threadIdx.x; i < 2 * P.Ntheta; i += blockDim.x) L[P.NRr + i] = P.base[P.o_trcv + i];
+    for(int i = threadIdx.x; i < P.NRz; i += blockDim.x) Lz[i] = P.Rz[i];
+    __syncthreads();
+    const real *Rr = L, *trcv = L + P.NRr;
+    const float *Rz = Lz;
+    uint4 *stage = stage_all[wib];
+    unsigned long long *bmask = bmask_all[wib];
+    int2 *rowq = rowq_all[wib];
+    const Seg3<real> *S = reinterpret_cast<const Seg3<real> *>(stage);
+    const uint4 *gw = reinterpret_cast<const uint4 *>(O.segs);
+    const bool pre = P.Ntheta <= WAVE;
+    uint32_t n = *O.seg_count;
+    if(n > O.seg_cap) n = O.seg_cap;
+    Sink3 sink{P.field, 0, &O};
+    for(uint32_t base = wave * G; base < n; base += nwaves * G) {
+        const int ng = (int)min((uint32_t)G, n - base);
+        for(int t = lane; t < ng * W16; t += WAVE) stage[t] = gw[(size_t)base * W16 + t];
+        if(lane < G) bmask[lane] = 0ull;
+        wave_lds_sync();
+        if(pre) {
+            // bearing prefilter, one (segment, bearing) pair per lane
+            const int npair = ng * P.Ntheta;
+            for(int p0 = 0; p0 < npair; p0 += WAVE) {
+                const int p = p0 + lane;
+                if(p < npair) {
+                    const int g = p / P.Ntheta, t = p - g * P.Ntheta;
+                    const Seg3D<real> &d = S[g].d;
+                    if(d.nitems > 0) {
+                        const real r0 = Rr[d.ir0], r1 = Rr[d.ir1];
+                        const real zs = P.rz_absmax + std::fabs(d.xa.z);
+                        if(bearing3(d, trcv[2 * t], trcv[2 * t + 1], rmin(r0, r1), rmax(r0, r1), zs, P))
+                            atomicOr(bmask + g, 1ull << t);
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+        int nrow = 0, nb = 0;
+        unsigned long long bm = 0ull;
+        if(lane < ng && S[lane].d.nitems > 0) {
+            const Seg3D<real> &d = S[lane].d;
+            bm = bmask[lane];
+            nb = pre ? __popcll(bm) : P.Ntheta;
+            nrow = ((d.ir1 - d.ir0) * d.dir + 1) * nb;
+        }
+        // rows that pass the row test are queued in LDS as (ir, g Ntheta + it)
+        // and taken 64 at a time, so the band and depth-window work runs on
+        // full wavefronts
+        auto do_rows = [&](int k) {
+            int g = 0, ir = 0, it = 0, i0 = 0, i1 = -1;
+            real xr = 0, yr = 0, zmin = 0, zmax = 0;
+            if(lane < k) {
+                const int2 e = rowq[lane];
+                ir = e.x;
+                g = e.y / P.Ntheta;
+                it = e.y - g * P.Ntheta;
+                const Seg3D<real> &d = S[g].d;
+                real wlo, whi;
+                if(row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {
+                    if(P.rz_sorted && wlo == wlo && whi == whi) {
+                        depth_window(Rz, P.NRz, P.rz_rdz, wlo, whi, i0, i1);
+                    } else {
+                        i0 = 0;
+                        i1 = P.NRz - 1;
+                    }
+                }
+            }
+            const int cnt = i1 >= i0 ? i1 - i0 + 1 : 0;
+            int excl;
+            const int total = wave_excl_scan(cnt, lane, excl);
+            for(int t0 = 0; t0 < total; t0 += WAVE) {
+                const int t = t0 + lane;
+                const int own = wave_owner(excl, t);
+                const int og = __shfl(g, own), oir = __shfl(ir, own), oit = __shfl(it, own);
+                const int oi0 = __shfl(i0, own), oexcl = __shfl(excl, own);
+                const real oxr = __shfl(xr, own), oyr = __shfl(yr, own);
+                const real ozmin = __shfl(zmin, own), ozmax = __shfl(zmax, own);
+                if(t < total)
+                    item3<real, Sink3, FAST, RT>(S[og].d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, Rz,
+                                                 sink);
+            }
+        };
+        int rexcl;
+        const int nrows = wave_excl_scan(nrow, lane, rexcl);
+        int qn = 0;
+        for(int r0 = 0; r0 < nrows; r0 += WAVE) {
+            const int rt = r0 + lane;
+            const int g = wave_owner(rexcl, rt);
+            const int loc = rt - __shfl(rexcl, g);
+            const int gnb = __shfl(nb, g);
+            const unsigned long long gbm = __shfl(bm, g);
+            bool pass = false;
+            int ir = 0, it = 0;
+            if(rt < nrows) {
+                const Seg3D<real> &d = S[g].d;
+                const int j = loc / gnb, k = loc - j * gnb;
+                it = pre ? nth_set_bit(gbm, k) : k;
+                ir = d.ir0 + j * d.dir;
+                real xr, yr, zmin, zmax;
+                pass = row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax);
+            }
+            const unsigned long long pm = __ballot(pass);
+            if(pass) rowq[qn + __popcll(pm & ((1ull << lane) - 1ull))] = make_int2(ir, g * P.Ntheta + it);
+            qn += __popcll(pm);
+            if(qn >= WAVE) {
+                wave_lds_sync();
+                do_rows(WAVE);
+                int2 v = make_int2(0, 0);
+                const bool mv = lane < qn - WAVE;
+                if(mv) v = rowq[WAVE + lane];
+                wave_lds_sync();
+                if(mv) rowq[lane] = v;
+                qn -= WAVE;
+                wave_lds_sync();
+            }
+        }
+        if(qn > 0) {
+            wave_lds_sync();
+            do_rows(qn);
+        }
+        // the stage is rewritten by the next group
+        wave_lds_sync();
+    }
+    unsigned long long nc = sink.nc;
+    for(int off = WAVE / 2; off > 0; off >>= 1) nc += __shfl_xor(nc, off);
+    if(lane == 0 && nc) atomicAdd(O.counters + 1, nc);
+}
+
 // Scatter of ray-centred 3D segments (Step_InfluenceGeoRayCen,
 // influence.hpp:1030-1194): one wavefront per segment; the (depth, bearing)
 // rows are spread over the lanes, each lane finds its row's bracketed
@@ -761,15 +944,23 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             if(rt == RT_ARR) hipLaunchKernelGGL((k_scatter3rc<real, RT_ARR>), dim3(sblocks), dim3(256), 0, s, P, O);
             else if(rt == RT_EIG) hipLaunchKernelGGL((k_scatter3rc<real, RT_EIG>), dim3(sblocks), dim3(256), 0, s, P, O);
             else hipLaunchKernelGGL((k_scatter3rc<real, RT_TL>), dim3(sblocks), dim3(256), 0, s, P, O);
+        } else if(c->scatter3_map == 2 && P.stage_tables &&
+                  P.Ntheta < (1 << 26) &&
+                  lds + 4 * ((size_t)SCAT3_G * (sizeof(Seg3<real>) + sizeof(unsigned long long)) + 2 * WAVE * sizeof(int2))
+                      <= (size_t)c->lds_max) {
+            if(rt == RT_ARR) hipLaunchKernelGGL((k_scatter3d_flat<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else if(rt == RT_EIG) hipLaunchKernelGGL((k_scatter3d_flat<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d_flat<real, false, RT_TL>), dim3(sblocks), dim3(256), lds, s, P, O);
+            else hipLaunchKernelGGL((k_scatter3d_flat<real, true, RT_TL>), dim3(sblocks), dim3(256), lds, s, P, O);
         } else if(rt == RT_ARR) {
-            if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
+            if(c->scatter3_map == 1) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
             else if(P.stage_tables) hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR, true>), dim3(sblocks), dim3(256), lds, s, P, O);
             else hipLaunchKernelGGL((k_scatter3d<real, false, RT_ARR, false>), dim3(sblocks), dim3(256), 0, s, P, O);
         } else if(rt == RT_EIG) {
-            if(c->scatter3_lanes) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
+            if(c->scatter3_map == 1) hipLaunchKernelGGL((k_scatter3d_lanes<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
             else if(P.stage_tables) hipLaunchKernelGGL((k_scatter3d<real, false, RT_EIG, true>), dim3(sblocks), dim3(256), lds, s, P, O);
             else hipLaunchKernelGGL((k_scatter3d<real, false, RT_EIG, false>), dim3(sblocks), dim3(256), 0, s, P, O);
-        } else if(c->scatter3_lanes) {
+        } else if(c->scatter3_map == 1) {
             if(c->exact_phasor) hipLaunchKernelGGL((k_scatter3d_lanes<real, false>), dim3(sblocks), dim3(256), lds, s, P, O);
             else hipLaunchKernelGGL((k_scatter3d_lanes<real, true>), dim3(sblocks), dim3(256), lds, s, P, O);
         } else if(P.stage_tables) {

@@ -1138,7 +1138,8 @@ struct bhc_ctx {
     int exact_phasor;           // 1: TL phasor in the reference's full arithmetic
     size_t seg_mb;              // segment buffer size (MiB), allocated at first use
     double seg_per_ray3;        // 3D runs: largest segments-per-ray seen (sizes the first pass)
-    int scatter3_lanes;         // 3D scatter mapping: 0 one wavefront per segment, 1 one lane per segment
+    int scatter3_map;           // 3D scatter mapping: 0 one wavefront per segment, 1 one lane per segment,
+                                // 2 (default) segment groups flattened per wavefront
     // finished-ray progress of the current or last run (bhc_progress)
     unsigned long long *h_prog; // pinned, device-visible host word written by the trace kernels
     unsigned long long *d_prog; // device counter
@@ -1280,7 +1281,8 @@ extern "C" int bhc_ctx_create(int device, bhc_ctx **out)
     c->exact_phasor = 0;
     if(const char *e = getenv("BHC_EXACT_PHASOR")) c->exact_phasor = atoi(e) != 0;
     c->seg_mb = 16384;
-    if(const char *e = getenv("BHC_SCATTER3_LANES")) c->scatter3_lanes = atoi(e) != 0;
+    c->scatter3_map = 2;
+    if(const char *e = getenv("BHC_SCATTER3_MAP")) c->scatter3_map = std::min(std::max(atoi(e), 0), 2);
     if(const char *e = getenv("BHC_SEG_BUFFER_MB")) c->seg_mb = (size_t)std::max(16L, atol(e));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&c->ev0));

@@ -32,18 +32,25 @@ def compare_field(o, g, scaled_o, scaled_g, fp64=True, min_exact=0.999, path_rto
     return ok, rep
 
 
+def env_context(**settings):
+    """A device context created with the given BHC_* environment settings
+    (read once, at context creation), e.g. env_context(BHC_SCATTER3_MAP=0)."""
+    import os
+    import bellhopcuda_amd as bh
+    old = {k: os.environ.get(k) for k in settings}
+    os.environ.update({k: str(v) for k, v in settings.items()})
+    try:
+        return bh.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
 def small_segment_context(mb):
     """A device context whose segment buffer holds only `mb` MiB, so that a
     run is split into several trace / scatter passes, some of them retraced
     after a segment-buffer overflow."""
-    import os
-    import bellhopcuda_amd as bh
-    old = os.environ.get("BHC_SEG_BUFFER_MB")
-    os.environ["BHC_SEG_BUFFER_MB"] = str(mb)
-    try:
-        return bh.Context(0)
-    finally:
-        if old is None:
-            del os.environ["BHC_SEG_BUFFER_MB"]
-        else:
-            os.environ["BHC_SEG_BUFFER_MB"] = old
+    return env_context(BHC_SEG_BUFFER_MB=mb)

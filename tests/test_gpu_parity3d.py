@@ -252,3 +252,41 @@ def test_arrivals_and_eigen3d_parity(gpu_ctx):
     key = ["is_", "itheta", "ir", "iz", "isx", "isy", "isz", "ialpha", "ibeta"]
     assert np.array_equal(np.sort(np.ctypeslib.as_array(o.hits)[:no], order=key),
                           np.sort(np.ctypeslib.as_array(g.hits)[:ng], order=key))
+
+
+@pytest.mark.parametrize("run_type", ["CB   3", "AB   3"])
+def test_scatter3_mappings_agree(run_type, gpu_ctx):
+    """The three mappings of the Cartesian 3D scatter (one wavefront per
+    segment, one lane per segment, segment groups flattened per wavefront)
+    make the same contributions as the oracle: equal counts, TL within 1e-3 dB
+    of each other (FP32 atomic order) and, for arrivals, equal merged lists."""
+    from parity import env_context
+    print("\n[gpu-case-3d] scatter3 mappings", run_type, flush=True)
+    env = c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=51, run_type=run_type)
+    arr = run_type[0] == "A"
+    o = oracle.field_run3d(env, max_narr=20) if arr else oracle.field_run3d(env)
+    res = {}
+    for m in (0, 1, 2):
+        ctx = env_context(BHC_SCATTER3_MAP=m, BHC_SEG_BUFFER_MB=512)
+        try:
+            res[m] = bh.field_run3d(env, ctx=ctx, max_narr=20) if arr else bh.field_run3d(env, ctx=ctx)
+        finally:
+            ctx.close()
+        assert res[m].err == o.err and res[m].total_contrib == o.total_contrib, (m, res[m].total_contrib)
+    if arr:
+        for m in (0, 1):
+            assert np.array_equal(res[m].narr, res[2].narr)
+            A = np.ctypeslib.as_array(res[m].arr).reshape(-1, 20)
+            B = np.ctypeslib.as_array(res[2].arr).reshape(-1, 20)
+            n = np.minimum(res[2].narr, 20)
+            assert n.sum() > 0
+            for r in np.nonzero(n)[0]:
+                for f in ("a", "Phase", "delay", "NTopBnc", "NBotBnc"):
+                    assert np.array_equal(A[r, :n[r]][f], B[r, :n[r]][f]), (m, r, f)
+    else:
+        prob = bh.Problem3D(env)
+        tl = {m: tl_db(prob.scale_pressure(r.field)) for m, r in res.items()}
+        mask = tl[2] < 120.0
+        assert mask.sum() > 1000
+        for m in (0, 1):
+            assert np.max(np.abs(tl[m] - tl[2])[mask]) <= 1e-3, m
