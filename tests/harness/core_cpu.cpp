@@ -348,6 +348,98 @@ extern "C" int core_ray_run3d(const bhc_problem3d *prob, bhc_ray_pt3d *pts, int3
     return prob->precision == 64 ? ray3<double>(prob, pts, nsteps, M) : ray3<float>(prob, pts, nsteps, M);
 }
 
+// Brute-force check of the 3D Cartesian scatter's pruning: every step of
+// every ray (one in `stride`) is tested against every (range, bearing, depth)
+// receiver the reference's row test admits, with no prefilter or band; each
+// receiver the beam-window test accepts must have passed the bearing
+// prefilter and lie inside the row's band. out: [0] accepted receivers,
+// [1] accepted but pruned by the bearing prefilter, [2] accepted but outside
+// the band, [3] receivers in the rows' full depth windows, [4] receivers in
+// the bands.
+namespace {
+struct HitSink3 {
+    bool hit = false;
+    void add3(size_t, float, float) {}
+    void count() { hit = true; }
+    void mark() {}
+};
+
+template<typename real> int band_check3(const bhc_problem3d *prob, int stride, long long *out)
+{
+    using namespace bhc3d;
+    std::vector<real> blob;
+    Prob3D<real> P;
+    std::string err;
+    if(pack3<real>(prob, blob, P, err)) return -1;
+    P.base = blob.data();
+    P.s.base = blob.data();
+    P.Rz = prob->Rz;
+    P.Sz = prob->Sz;
+    Err e = {0, 0, 0, 0};
+    const int njobs = prob->Nalpha * prob->Nbeta * prob->NSx * prob->NSy * prob->NSz;
+    const real *Rr = P.base + P.o_Rr, *tr = P.base + P.o_trcv;
+    auto check = [&](const Seg3D<real> &d) {
+        if(d.nitems <= 0) return;
+        const int nr = (d.ir1 - d.ir0) * d.dir + 1;
+        real rlo = rmin(Rr[d.ir0], Rr[d.ir1]), rhi = rmax(Rr[d.ir0], Rr[d.ir1]);
+        real zs = P.rz_absmax + std::fabs(d.xa.z);
+        for(int it = 0; it < P.Ntheta; ++it) {
+            const bool bpass = bearing3(d, tr[2 * it], tr[2 * it + 1], rlo, rhi, zs, P);
+            for(int j = 0; j < nr; ++j) {
+                const int ir = d.ir0 + j * d.dir;
+                real xr, yr, zmin, zmax, wlo = 0, whi = -1;
+                if(!row3(d, ir, it, P, Rr, tr, xr, yr, zmin, zmax)) continue;
+                const bool in_band = band3(d, xr, yr, zmin, zmax, P, wlo, whi);
+                for(int iz = 0; iz < P.NRz; ++iz) {
+                    const real zr = (real)P.Rz[iz];
+                    if(zr < zmin || zr > zmax) continue;
+                    ++out[3];
+                    const bool inside = in_band && zr >= wlo && zr <= whi;
+                    if(inside) ++out[4];
+                    HitSink3 s;
+                    item3<real, HitSink3, false>(d, ir, it, iz, xr, yr, zmin, zmax, P, P.Rz, s);
+                    if(!s.hit) continue;
+                    ++out[0];
+                    if(!bpass) ++out[1];
+                    if(!inside) ++out[2];
+                }
+            }
+        }
+    };
+    for(int job = 0; job < njobs; job += stride) {
+        int ia, ib, isx, isy, isz;
+        JobIndices3(job, P, ia, ib, isy, isx, isz);
+        Ray3<real> p0, p1, p2;
+        Trace3<real> S;
+        Infl3<real> I;
+        if(!RayInit3(ia, ib, isx, isy, isz, p0, S, I, P, e)) continue;
+        int is = 0;
+        for(;;) {
+            if(e.err) break;
+            const bool two = RayUpdate3(p0, p1, p2, S, P, e);
+            Seg3D<real> d;
+            prelude3(d, p0, p1, I, is, P, isz, isx, isy);
+            check(d);
+            ++is;
+            if(two) {
+                prelude3(d, p1, p2, I, is, P, isz, isx, isy);
+                check(d);
+                ++is;
+            }
+            p0 = two ? p2 : p1;
+            if(RayTerminate3(p0, is, S, P, e)) break;
+        }
+    }
+    return 0;
+}
+} // namespace
+
+extern "C" int core_band_check3d(const bhc_problem3d *prob, int stride, long long *out)
+{
+    for(int k = 0; k < 5; ++k) out[k] = 0;
+    return prob->precision == 64 ? band_check3<double>(prob, stride, out) : band_check3<float>(prob, stride, out);
+}
+
 // The receiver-depth window of the scatter kernels (guessed index + exact
 // verification), for its test against a plain search.
 extern "C" void core_depth_window(const float *Rz, int n, double rdz, double zlo, double zhi, int *i0, int *i1)
