@@ -1233,29 +1233,67 @@ __host__ __device__ inline bool band3_circle(const Seg3D<real> &d, real xr, real
 // K |A|_F does not. The bounds are widened for the rounding of the test's own
 // arithmetic (which grows with A's condition number); ill-conditioned or
 // degenerate segments keep the circle band.
-// a z <= b applied to the interval [L, H] (a == 0 and b < 0 empties it).
-__host__ __device__ inline void clip_le(double a, double b, double &L, double &H)
+// a z <= b applied to the interval [L, H], with ra = 1 / a (a == 0 and b < 0
+// empties it; a NaN bound leaves it unchanged).
+__host__ __device__ inline void clip_le(double a, double ra, double b, double &L, double &H)
 {
-    if(a > 0.0) H = rmin(H, b / a);
-    else if(a < 0.0) L = rmax(L, b / a);
+    if(a > 0.0) H = rmin(H, b * ra);
+    else if(a < 0.0) L = rmax(L, b * ra);
     else if(b < 0.0) H = -1e300;
 }
 
-// z in [zl, zh] with |c + g z| <= alpha + beta |s(z)|, s(z) = p + sig z: the
-// hull of the solutions with s >= 0 and with s <= 0 (an empty result sets
-// lo > hi).
-__host__ __device__ inline void abs_affine_hull(double c, double g, double alpha, double beta, double p, double sig,
-                                                double zl, double zh, double &lo, double &hi)
+// Bounds |c + g z| <= alpha + beta |s(z)|, s(z) = p + sig z, of one beam-plane
+// direction, with the reciprocals of its two slopes g -/+ beta sig.
+struct AbsAffine {
+    double c, g, alpha, beta, rA, rB;
+};
+
+// [L, H] cut to the z where bound f holds, all z of the interval on the side
+// sgn of s = 0 (sgn = 1: s >= 0, -1: s <= 0).
+__host__ __device__ inline void clip_side(const AbsAffine &f, double p, double sig, int sgn, double &L, double &H)
 {
-    double L1 = zl, H1 = zh, L2 = zl, H2 = zh;
-    // s >= 0: |s| = p + sig z
-    clip_le(g - beta * sig, alpha + beta * p - c, L1, H1);
-    clip_le(-g - beta * sig, alpha + beta * p + c, L1, H1);
-    clip_le(-sig, p, L1, H1);
-    // s <= 0: |s| = -p - sig z
-    clip_le(g + beta * sig, alpha - beta * p - c, L2, H2);
-    clip_le(-g + beta * sig, alpha - beta * p + c, L2, H2);
-    clip_le(sig, -p, L2, H2);
+    const double base = f.alpha + sgn * f.beta * p;
+    // s >= 0: (g - beta sig) z <= base - c and (-g - beta sig) z <= base + c
+    // s <= 0: -(-g - beta sig) z <= base - c and -(g - beta sig) z <= base + c
+    const double a1 = f.g - f.beta * sig, a2 = -f.g - f.beta * sig;
+    if(sgn > 0) {
+        clip_le(a1, f.rA, base - f.c, L, H);
+        clip_le(a2, f.rB, base + f.c, L, H);
+    } else {
+        clip_le(-a2, -f.rB, base - f.c, L, H);
+        clip_le(-a1, -f.rA, base + f.c, L, H);
+    }
+}
+
+// The hull of [zA, zB]'s solutions of bounds f0, f1 and (use2) f2, the
+// sides s >= 0 and s <= 0 solved separately (an empty result sets lo > hi).
+__host__ __device__ inline void abs_affine_hull(const AbsAffine &f0, const AbsAffine &f1, const AbsAffine &f2,
+                                                bool use2, double p, double sig, double zA, double zB, double &lo,
+                                                double &hi)
+{
+    double L1 = zA, H1 = zB, L2 = zA, H2 = zB;
+    // the side of s = 0 in z (s = p + sig z)
+    if(sig > 0.0) {
+        const double z0 = -p / sig;
+        L1 = rmax(L1, z0);
+        H2 = rmin(H2, z0);
+    } else if(sig < 0.0) {
+        const double z0 = -p / sig;
+        H1 = rmin(H1, z0);
+        L2 = rmax(L2, z0);
+    } else if(p >= 0.0) {
+        L2 = 1.0;
+        H2 = -1.0;
+    } else {
+        L1 = 1.0;
+        H1 = -1.0;
+    }
+    clip_side(f0, p, sig, 1, L1, H1);
+    clip_side(f1, p, sig, 1, L1, H1);
+    if(use2) clip_side(f2, p, sig, 1, L1, H1);
+    clip_side(f0, p, sig, -1, L2, H2);
+    clip_side(f1, p, sig, -1, L2, H2);
+    if(use2) clip_side(f2, p, sig, -1, L2, H2);
     const bool e1 = !(L1 <= H1), e2 = !(L2 <= H2);
     if(e1 && e2) {
         lo = 1.0;
@@ -1272,19 +1310,27 @@ __host__ __device__ inline void abs_affine_hull(double c, double g, double alpha
     }
 }
 
-// Roots of c2 s^2 + c1 s + c0 = 0 appended to r (at most two).
-__host__ __device__ inline void quad_roots(double c2, double c1, double c0, double *r, int &n)
+// Roots of c2 s^2 + c1 s + c0 = 0 in r0, r1 (NaN where there is none).
+__host__ __device__ inline void quad_roots(double c2, double c1, double c0, double &r0, double &r1)
 {
+    r0 = r1 = __builtin_nan("");
     if(std::fabs(c2) <= 1e-300 * (std::fabs(c1) + std::fabs(c0)) || c2 == 0.0) {
-        if(c1 != 0.0) r[n++] = -c0 / c1;
+        if(c1 != 0.0) r0 = -c0 / c1;
         return;
     }
     const double disc = c1 * c1 - 4.0 * c2 * c0;
     if(!(disc >= 0.0)) return;
     const double sq = std::sqrt(disc);
     const double qq = -0.5 * (c1 + (c1 >= 0.0 ? sq : -sq));
-    r[n++] = qq / c2;
-    if(qq != 0.0) r[n++] = c0 / qq;
+    r0 = qq / c2;
+    if(qq != 0.0) r1 = c0 / qq;
+}
+
+__host__ __device__ inline void sort2(double &a, double &b)
+{
+    const double lo = rmin(a, b), hi = rmax(a, b);
+    a = lo;
+    b = hi;
 }
 
 // Narrows a row's depth band by the beam-plane coordinates separately. With
@@ -1349,61 +1395,59 @@ __host__ __device__ inline bool band3_axes(const Seg3D<real> &d, real xr, real y
     const double c1 = hx * (double)d.e1.x + hy * (double)d.e1.y - za * g1;   // w1 = c1 + z g1
     const double c2 = hx * (double)d.e2.x + hy * (double)d.e2.y - za * g2;   // w2 = c2 + z g2
     const double gn = std::sqrt(g1 * g1 + g2 * g2);
-    double cc[3], gg[3], al[3], be[3];
-    cc[0] = c1;
-    gg[0] = g1;
-    coef(1.0, 0.0, al[0], be[0]);
-    cc[1] = c2;
-    gg[1] = g2;
-    coef(0.0, 1.0, al[1], be[1]);
-    int ncon = 2;
-    if(gn > 1e-9) {
-        // the column's direction n = (g1, g2) / |g|: |n . w| <= n' . (|w1|, |w2|), n' = |n|
-        cc[2] = (g1 * c1 + g2 * c2) / gn;
-        gg[2] = gn;
-        coef(std::fabs(g1) / gn, std::fabs(g2) / gn, al[2], be[2]);
-        ncon = 3;
-    }
-    for(int i = 0; i < ncon; ++i)
-        if(!(al[i] < 1e30) || !(be[i] < 1e30)) return true;
-    // pieces of [zl, zh] between the crossings of |det A(s)| = T
-    double zb[6];
-    int nb = 0;
-    zb[nb++] = zl;
-    if(sig != 0.0) {
-        double r[4];
-        int nr = 0;
-        quad_roots(ddet, lin, det0 - T, r, nr);
-        quad_roots(ddet, lin, det0 + T, r, nr);
-        for(int k = 0; k < nr; ++k) {
-            const double z = (r[k] - p) / sig;
-            if(z > zl && z < zh) zb[nb++] = z;
-        }
-    }
-    zb[nb++] = zh;
-    for(int k = 1; k < nb; ++k)
-        for(int j = k; j > 0 && zb[j - 1] > zb[j]; --j) {
-            const double t = zb[j];
-            zb[j] = zb[j - 1];
-            zb[j - 1] = t;
-        }
+    AbsAffine A0, A1, A2;
+    A0.c = c1;
+    A0.g = g1;
+    coef(1.0, 0.0, A0.alpha, A0.beta);
+    A1.c = c2;
+    A1.g = g2;
+    coef(0.0, 1.0, A1.alpha, A1.beta);
+    // the column's direction n = (g1, g2) / |g|: |n . w| <= n' . (|w1|, |w2|), n' = |n|
+    const bool use2 = gn > 1e-9;
+    A2.c = use2 ? (g1 * c1 + g2 * c2) / gn : 0.0;
+    A2.g = gn;
+    coef(use2 ? std::fabs(g1) / gn : 0.0, use2 ? std::fabs(g2) / gn : 0.0, A2.alpha, A2.beta);
+    if(!(A0.alpha < 1e30) || !(A0.beta < 1e30) || !(A1.alpha < 1e30) || !(A1.beta < 1e30) || !(A2.alpha < 1e30) ||
+       !(A2.beta < 1e30))
+        return true;
+    A0.rA = 1.0 / (A0.g - A0.beta * sig);
+    A0.rB = 1.0 / (-A0.g - A0.beta * sig);
+    A1.rA = 1.0 / (A1.g - A1.beta * sig);
+    A1.rB = 1.0 / (-A1.g - A1.beta * sig);
+    A2.rA = 1.0 / (A2.g - A2.beta * sig);
+    A2.rB = 1.0 / (-A2.g - A2.beta * sig);
     double lo = 1e300, hi = -1e300;
-    for(int k = 0; k + 1 < nb; ++k) {
-        const double zA = zb[k], zB = zb[k + 1];
-        const double sm = p + sig * (0.5 * (zA + zB));
-        const double fm = det0 + lin * sm + ddet * sm * sm;
-        double l = zA, h = zB;
-        if(std::fabs(fm) >= T) {
-            for(int i = 0; i < ncon && l <= h; ++i) {
-                double l2, h2;
-                abs_affine_hull(cc[i], gg[i], al[i], be[i], p, sig, l, h, l2, h2);
-                l = l2;
-                h = h2;
+    if(std::fabs(det0) - smax * std::fabs(lin) - smax * smax * std::fabs(ddet) >= T) {
+        // |det A(s)| >= T over the whole band: one piece
+        abs_affine_hull(A0, A1, A2, use2, p, sig, zl, zh, lo, hi);
+    } else {
+        // pieces of [zl, zh] between the crossings of |det A(s)| = T (the
+        // crossings outside the band, or missing, collapse onto zh)
+        double r0, r1, r2, r3;
+        quad_roots(ddet, lin, det0 - T, r0, r1);
+        quad_roots(ddet, lin, det0 + T, r2, r3);
+        auto toz = [&](double r) {
+            const double z = (r - p) / sig;
+            return (sig != 0.0 && z > zl && z < zh) ? z : zh;
+        };
+        double z1 = toz(r0), z2 = toz(r1), z3 = toz(r2), z4 = toz(r3);
+        sort2(z1, z2);
+        sort2(z3, z4);
+        sort2(z1, z3);
+        sort2(z2, z4);
+        sort2(z2, z3);
+        const double zb[6] = {zl, z1, z2, z3, z4, zh};
+#pragma unroll
+        for(int k = 0; k < 5; ++k) {
+            const double zA = zb[k], zB = zb[k + 1];
+            const double sm = p + sig * (0.5 * (zA + zB));
+            const double fm = det0 + lin * sm + ddet * sm * sm;
+            double l = zA, h = zB;
+            if(std::fabs(fm) >= T) abs_affine_hull(A0, A1, A2, use2, p, sig, zA, zB, l, h);
+            if(l <= h) {
+                lo = rmin(lo, l);
+                hi = rmax(hi, h);
             }
-        }
-        if(l <= h) {
-            lo = rmin(lo, l);
-            hi = rmax(hi, h);
         }
     }
     if(!(lo <= hi)) return false;
@@ -1486,14 +1530,27 @@ __host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int 
     qI.a01 *= d.rcp_qhat0;
     qI.a11 *= d.rcp_qhat0;
     real qF = mdet(qI);
+    // numerators of the beam coordinates n1' = |num1 / qF|, n2' = |num2 / qF|
+    const real num1 = -qI.a01 * n2 + qI.a11 * n1;
+    const real num2 = qI.a00 * n2 - qI.a10 * n1;
+    if(!(P.isRayCen && P.isGaussian)) {
+        // receivers the beam-window test below rejects, found before its
+        // square roots and divisions: its bcd is at least (|num1| + |num2|)
+        // / |qF| (their max for hat beams) less two roundings, so a margin of
+        // 1e-6 makes this reject only what the test rejects (as do the
+        // row-length and |qF| tests it skips)
+        const double N1 = std::fabs((double)num1), N2 = std::fabs((double)num2), Q = std::fabs((double)qF);
+        const double lhs = P.isGaussian ? N1 + N2 : rmax(N1, N2);
+        if(lhs > (P.isGaussian ? 4.0 : 1.0) * Q * (1.0 + 1e-6)) return;
+    }
     if(!P.isRayCen) { // the row-length test is Cartesian only
         real l1 = std::sqrt(qI.a00 * qI.a00 + qI.a10 * qI.a10);
         real l2 = std::sqrt(qI.a01 * qI.a01 + qI.a11 * qI.a11);
         if(std::fabs(l1) < RC<real>::EPS || std::fabs(l2) < RC<real>::EPS) return;
     }
     if(std::fabs(qF) < RC<real>::EPS) return;
-    real n1p = std::fabs((-qI.a01 * n2 + qI.a11 * n1) / qF);
-    real n2p = std::fabs((qI.a00 * n2 - qI.a10 * n1) / qF);
+    real n1p = std::fabs(num1 / qF);
+    real n2p = std::fabs(num2 / qF);
     real bcd = P.isGaussian ? n1p + n2p : rmax(n1p, n2p);
     real BW = P.isGaussian ? RL(4.0) : RL(1.0);
     // the window test is not applied to ray-centred Gaussian beams in 3D

@@ -199,6 +199,18 @@ def run_single(a):
     return line
 
 
+def scatter3_name():
+    """The Cartesian 3D scatter kernel the library launches for a TL run whose
+    receiver tables fit LDS: BHC_SCATTER3_MAP 0 one wavefront per segment, 1
+    one lane per segment, 2 (default) segment groups per wavefront."""
+    m = os.environ.get("BHC_SCATTER3_MAP")
+    try:
+        k = 2 if m is None else min(max(int(m.strip()), 0), 2)
+    except ValueError:
+        k = 0   # the library reads the value with atoi
+    return ("k_scatter3d", "k_scatter3d_lanes", "k_scatter3d_flat")[k]
+
+
 def run_c3(a):
     """3D Gaussian-beam TL on the shelf-canyon environment: hexahedral SSP,
     201 x 201 bathymetry grid, 181 bearings x 1001 declinations, 36 bearings x
@@ -234,7 +246,7 @@ def run_c3(a):
     rl = kernel_roofline(a.precision, [
         {"name": "k_trace3d<%s>" % ("double" if a.precision == 64 else "float"), "ms": sum(tms), "launches": nl,
          "flop": (f_I * R + f_S * S) * len(tms)},
-        {"name": "k_scatter3d<%s, true>" % ("double" if a.precision == 64 else "float"), "ms": sum(sms),
+        {"name": "%s<%s, true>" % (scatter3_name(), "double" if a.precision == 64 else "float"), "ms": sum(sms),
          "launches": nl, "flop": f_C * Cn * len(tms)},
     ], F, k_ms)
     rl.update({"f_I": f_I, "f_S": f_S, "f_C": f_C, "rays": R, "steps": S, "contributions": Cn,
