@@ -1516,20 +1516,40 @@ __host__ __device__ inline void item3(const Seg3D<real> &d, int ir, int itheta, 
     core3<real, Seg3D<real>, Sink, FAST, RT>(d, s, n1, n2, itheta, ir, iz, d.ta, P, sink);
 }
 
-// InfluenceGeoCore (influence.hpp:897-1024, 3D) + ApplyContribution (:432-473)
-// for a receiver at beam coordinates (s, n1, n2) of segment d (Cartesian
-// Seg3D or ray-centred Seg3R); rayt: the direction ReceiverAngles uses.
-template<typename real, typename D, typename Sink, bool FAST, int RT>
-__host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int itheta, int ir, int iz,
-                                      const v3<real> &rayt, const Prob3D<real> &P, Sink &sink)
+template<typename real, typename D>
+__host__ __device__ inline bool window3(const D &d, real s, real n1, real n2, const Prob3D<real> &P, m2<real> &qI,
+                                        real &qF, real &n1p, real &n2p);
+
+// Whether item3 contributes for this receiver: its depth test and the beam
+// tests, with item3's arithmetic.
+template<typename real>
+__host__ __device__ inline bool item3_accepts(const Seg3D<real> &d, real xr, real yr, int iz, real zmin, real zmax,
+                                              const Prob3D<real> &P, const float *Rz)
 {
-    sink.mark();
-    m2<real> qI = madd(d.q0, mscl(s, d.dq));
+    real zr = (real)Rz[iz];
+    if(zr < zmin || zr > zmax) return false;
+    v3<real> xrr = V3(xr, yr, zr) - d.xa;
+    real s = dot3(xrr, d.ta) / d.rlen;
+    real n1 = std::fabs(dot3(xrr, d.e1));
+    real n2 = std::fabs(dot3(xrr, d.e2));
+    m2<real> qI;
+    real qF, n1p, n2p;
+    return window3(d, s, n1, n2, P, qI, qF, n1p, n2p);
+}
+
+// The tests of InfluenceGeoCore (influence.hpp:897-1024, 3D) for a receiver
+// at beam coordinates (s, n1, n2) of segment d: true when it contributes, with
+// the row-scaled q at s, its determinant and the beam coordinates n1', n2'.
+template<typename real, typename D>
+__host__ __device__ inline bool window3(const D &d, real s, real n1, real n2, const Prob3D<real> &P, m2<real> &qI,
+                                        real &qF, real &n1p, real &n2p)
+{
+    qI = madd(d.q0, mscl(s, d.dq));
     qI.a00 *= d.rcp_q0;
     qI.a10 *= d.rcp_q0;
     qI.a01 *= d.rcp_qhat0;
     qI.a11 *= d.rcp_qhat0;
-    real qF = mdet(qI);
+    qF = mdet(qI);
     // numerators of the beam coordinates n1' = |num1 / qF|, n2' = |num2 / qF|
     const real num1 = -qI.a01 * n2 + qI.a11 * n1;
     const real num2 = qI.a00 * n2 - qI.a10 * n1;
@@ -1541,20 +1561,33 @@ __host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int 
         // row-length and |qF| tests it skips)
         const double N1 = std::fabs((double)num1), N2 = std::fabs((double)num2), Q = std::fabs((double)qF);
         const double lhs = P.isGaussian ? N1 + N2 : rmax(N1, N2);
-        if(lhs > (P.isGaussian ? 4.0 : 1.0) * Q * (1.0 + 1e-6)) return;
+        if(lhs > (P.isGaussian ? 4.0 : 1.0) * Q * (1.0 + 1e-6)) return false;
     }
     if(!P.isRayCen) { // the row-length test is Cartesian only
         real l1 = std::sqrt(qI.a00 * qI.a00 + qI.a10 * qI.a10);
         real l2 = std::sqrt(qI.a01 * qI.a01 + qI.a11 * qI.a11);
-        if(std::fabs(l1) < RC<real>::EPS || std::fabs(l2) < RC<real>::EPS) return;
+        if(std::fabs(l1) < RC<real>::EPS || std::fabs(l2) < RC<real>::EPS) return false;
     }
-    if(std::fabs(qF) < RC<real>::EPS) return;
-    real n1p = std::fabs(num1 / qF);
-    real n2p = std::fabs(num2 / qF);
+    if(std::fabs(qF) < RC<real>::EPS) return false;
+    n1p = std::fabs(num1 / qF);
+    n2p = std::fabs(num2 / qF);
     real bcd = P.isGaussian ? n1p + n2p : rmax(n1p, n2p);
     real BW = P.isGaussian ? RL(4.0) : RL(1.0);
     // the window test is not applied to ray-centred Gaussian beams in 3D
-    if(bcd > BW * RL(1.0) && !(P.isRayCen && P.isGaussian)) return;
+    return !(bcd > BW * RL(1.0) && !(P.isRayCen && P.isGaussian));
+}
+
+// InfluenceGeoCore (influence.hpp:897-1024, 3D) + ApplyContribution (:432-473)
+// for a receiver at beam coordinates (s, n1, n2) of segment d (Cartesian
+// Seg3D or ray-centred Seg3R); rayt: the direction ReceiverAngles uses.
+template<typename real, typename D, typename Sink, bool FAST, int RT>
+__host__ __device__ inline void core3(const D &d, real s, real n1, real n2, int itheta, int ir, int iz,
+                                      const v3<real> &rayt, const Prob3D<real> &P, Sink &sink)
+{
+    sink.mark();
+    m2<real> qI;
+    real qF, n1p, n2p;
+    if(!window3(d, s, n1, n2, P, qI, qF, n1p, n2p)) return;
     cpx<real> delay = d.tau0 + rmul(s, d.dtau);
     real cnst = d.A0 / std::sqrt(std::fabs(qF));
     real w;

@@ -394,6 +394,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     __shared__ uint4 stage_all[4][G * W16];
     __shared__ unsigned long long bmask_all[4][G];
     __shared__ int2 rowq_all[4][2 * WAVE];
+    __shared__ int2 accq_all[4][2 * WAVE];
     extern __shared__ __align__(16) unsigned char lds3f[];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wib = threadIdx.x / WAVE;
@@ -410,6 +411,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     uint4 *stage = stage_all[wib];
     unsigned long long *bmask = bmask_all[wib];
     int2 *rowq = rowq_all[wib];
+    int2 *accq = accq_all[wib];
     const Seg3<real> *S = reinterpret_cast<const Seg3<real> *>(stage);
     const uint4 *gw = reinterpret_cast<const uint4 *>(O.segs);
     const bool pre = P.Ntheta <= WAVE;
@@ -447,6 +449,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
             nb = pre ? __popcll(bm) : P.Ntheta;
             nrow = ((d.ir1 - d.ir0) * d.dir + 1) * nb;
         }
+        // receivers that pass the beam tests are queued in LDS as
+        // (iz NRr + ir, g Ntheta + it) and their contributions made 64 at a
+        // time (item3 again, with its row recomputed), so the contribution
+        // work runs on full wavefronts
+        int qa = 0;
+        auto do_acc = [&](int k) {
+            if(lane < k) {
+                const int2 e = accq[lane];
+                const int iz = e.x / P.NRr, ir = e.x - iz * P.NRr;
+                const int g = e.y / P.Ntheta, it = e.y - g * P.Ntheta;
+                const Seg3D<real> &d = S[g].d;
+                real xr, yr, zmin, zmax;
+                row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax);
+                item3<real, Sink3, FAST, RT>(d, ir, it, iz, xr, yr, zmin, zmax, P, Rz, sink);
+            }
+        };
         // rows that pass the row test are queued in LDS as (ir, g Ntheta + it)
         // and taken 64 at a time, so the band and depth-window work runs on
         // full wavefronts
@@ -479,9 +497,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
                 const int oi0 = __shfl(i0, own), oexcl = __shfl(excl, own);
                 const real oxr = __shfl(xr, own), oyr = __shfl(yr, own);
                 const real ozmin = __shfl(zmin, own), ozmax = __shfl(zmax, own);
-                if(t < total)
-                    item3<real, Sink3, FAST, RT>(S[og].d, oir, oit, oi0 + (t - oexcl), oxr, oyr, ozmin, ozmax, P, Rz,
-                                                 sink);
+                const int oiz = oi0 + (t - oexcl);
+                const bool acc = t < total && item3_accepts(S[og].d, oxr, oyr, oiz, ozmin, ozmax, P, Rz);
+                const unsigned long long am = __ballot(acc);
+                if(acc) accq[qa + __popcll(am & ((1ull << lane) - 1ull))] = make_int2(oiz * P.NRr + oir, og * P.Ntheta + oit);
+                qa += __popcll(am);
+                if(qa >= WAVE) {
+                    wave_lds_sync();
+                    do_acc(WAVE);
+                    int2 v = make_int2(0, 0);
+                    const bool mv = lane < qa - WAVE;
+                    if(mv) v = accq[WAVE + lane];
+                    wave_lds_sync();
+                    if(mv) accq[lane] = v;
+                    qa -= WAVE;
+                    wave_lds_sync();
+                }
             }
         };
         int rexcl;
@@ -521,6 +552,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
         if(qn > 0) {
             wave_lds_sync();
             do_rows(qn);
+        }
+        if(qa > 0) {
+            wave_lds_sync();
+            do_acc(qa);
         }
         // the stage is rewritten by the next group
         wave_lds_sync();
@@ -948,8 +983,8 @@ template<typename real> int run3d(bhc_ctx *c, const bhc_problem3d *p, bhc_field_
             else if(rt == RT_EIG) hipLaunchKernelGGL((k_scatter3rc<real, RT_EIG>), dim3(sblocks), dim3(256), 0, s, P, O);
             else hipLaunchKernelGGL((k_scatter3rc<real, RT_TL>), dim3(sblocks), dim3(256), 0, s, P, O);
         } else if(c->scatter3_map == 2 && P.stage_tables &&
-                  P.Ntheta < (1 << 26) &&
-                  lds + 4 * ((size_t)SCAT3_G * (sizeof(Seg3<real>) + sizeof(unsigned long long)) + 2 * WAVE * sizeof(int2))
+                  P.Ntheta < (1 << 26) && (long long)P.NRz * P.NRr < (1ll << 31) &&
+                  lds + 4 * ((size_t)SCAT3_G * (sizeof(Seg3<real>) + sizeof(unsigned long long)) + 4 * WAVE * sizeof(int2))
                       <= (size_t)c->lds_max) {
             if(rt == RT_ARR) hipLaunchKernelGGL((k_scatter3d_flat<real, false, RT_ARR>), dim3(sblocks), dim3(256), lds, s, P, O);
             else if(rt == RT_EIG) hipLaunchKernelGGL((k_scatter3d_flat<real, false, RT_EIG>), dim3(sblocks), dim3(256), lds, s, P, O);
