@@ -254,15 +254,17 @@ def test_arrivals_and_eigen3d_parity(gpu_ctx):
                           np.sort(np.ctypeslib.as_array(g.hits)[:ng], order=key))
 
 
-@pytest.mark.parametrize("run_type", ["CB   3", "AB   3"])
-def test_scatter3_mappings_agree(run_type, gpu_ctx):
+@pytest.mark.parametrize("run_type,precision", [("CB   3", 64), ("AB   3", 64), ("CB   3", 32)])
+def test_scatter3_mappings_agree(run_type, precision, gpu_ctx):
     """The three mappings of the Cartesian 3D scatter (one wavefront per
     segment, one lane per segment, segment groups flattened per wavefront)
-    make the same contributions as the oracle: equal counts, TL within 1e-3 dB
-    of each other (FP32 atomic order) and, for arrivals, equal merged lists."""
+    make the same contributions: equal counts (FP64: the oracle's), TL within
+    1e-3 dB of each other (FP32 atomic order) and, for arrivals, equal merged
+    lists. In FP32 the three run the same float arithmetic on the same
+    segments, so they agree with each other exactly in count."""
     from parity import env_context
-    print("\n[gpu-case-3d] scatter3 mappings", run_type, flush=True)
-    env = c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=51, run_type=run_type)
+    print("\n[gpu-case-3d] scatter3 mappings", run_type, precision, flush=True)
+    env = c3_env(nalpha=41, nbeta=24, ntheta=6, nrz=21, nrr=51, nxy=51, run_type=run_type, precision=precision)
     arr = run_type[0] == "A"
     o = oracle.field_run3d(env, max_narr=20) if arr else oracle.field_run3d(env)
     res = {}
@@ -272,7 +274,12 @@ def test_scatter3_mappings_agree(run_type, gpu_ctx):
             res[m] = bh.field_run3d(env, ctx=ctx, max_narr=20) if arr else bh.field_run3d(env, ctx=ctx)
         finally:
             ctx.close()
-        assert res[m].err == o.err and res[m].total_contrib == o.total_contrib, (m, res[m].total_contrib)
+        assert res[m].err == o.err, m
+        if precision == 64:
+            assert res[m].total_contrib == o.total_contrib, (m, res[m].total_contrib)
+        else:
+            assert res[m].total_contrib == res[0].total_contrib, (m, res[m].total_contrib)
+            assert abs(res[m].total_contrib - o.total_contrib) <= 0.005 * o.total_contrib
     if arr:
         for m in (0, 1):
             assert np.array_equal(res[m].narr, res[2].narr)
