@@ -251,7 +251,9 @@ template<typename real, bool RCEN = false> __global__ void __launch_bounds__(64)
 // each row's depth window is flattened over the lanes.
 // STAGED: the receiver ranges, bearing table and depths are copied to LDS (the
 // launch checks they fit) and read only from there -- compile-time, so the
-// reads are LDS reads rather than generic ones. Three wavefronts per SIMD
+// reads are LDS reads rather than generic ones. Rows take the circle band
+// only: with a wavefront per segment the per-axis bounds cost more than the
+// receivers they remove (260 against 139 ms per shelf-canyon pass). Three wavefronts per SIMD
 // (at most 168 VGPRs): the per-segment chain of loads and FP64 row bounds is
 // latency-bound, and the third wavefront hides more of it (14 % faster than
 // two on the shelf-canyon field; four spill and are slower).
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
                 it = pre ? cand[k] : k;
                 ir = d.ir0 + j * d.dir;
                 real wlo, whi;
-                if(row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax) && band3(d, xr, yr, zmin, zmax, P, wlo, whi)) {
+                if(row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax) && band3_circle(d, xr, yr, zmin, zmax, P, wlo, whi)) {
                     if(P.rz_sorted && wlo == wlo && whi == whi) {
                         depth_window(Rz, P.NRz, P.rz_rdz, wlo, whi, i0, i1);
                     } else {
@@ -615,7 +617,8 @@ __global__ void __launch_bounds__(256) k_scatter3rc(Prob3D<real> P, DevOut3 O)
 }
 
 // Scatter with one lane per segment: each lane walks its own segment's rows
-// and depth windows serially (no cross-lane flattening).
+// and depth windows serially (no cross-lane flattening), with the circle band
+// (the per-axis bounds: 199 against 156 ms per shelf-canyon pass).
 template<typename real, bool FAST, int RT = RT_TL>
 __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3 O)
 {
@@ -647,7 +650,7 @@ __global__ void __launch_bounds__(256) k_scatter3d_lanes(Prob3D<real> P, DevOut3
             for(int it = 0; it < P.Ntheta; ++it) {
                 real xr, yr, zmin, zmax, wlo, whi;
                 if(!row3(d, ir, it, P, Rr, trcv, xr, yr, zmin, zmax)) continue;
-                if(!band3(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
+                if(!band3_circle(d, xr, yr, zmin, zmax, P, wlo, whi)) continue;
                 int i0 = 0, i1 = P.NRz - 1;
                 if(P.rz_sorted && wlo == wlo && whi == whi) depth_window(Rz, P.NRz, P.rz_rdz, wlo, whi, i0, i1);
                 for(int iz = i0; iz <= i1; ++iz)
